@@ -1,0 +1,14 @@
+"""crabsaxs_amd: MI355X-native crabSAXS profile path (HIP kernels behind a C ABI).
+
+The public names mirror the reference crate's API (src/lib.rs:16-50).
+"""
+from .api import (  # noqa: F401
+    Atom, ComputeOptions, Context, CurveMetadata, DeviceError, Element, FitDidNotConverge,
+    HydrationGrid, HydrationModel, HydrogenMode, InvalidInput, MultipoleOptions, NoDevice,
+    OutOfMemory, Precision, ProfileCalculator, SaxsError, ScatteringCurve, ScatteringMethod,
+    SolventParams, Structure, build_adaptive_hydration_grid, build_hydration_grid,
+    combine_solvent_basis, compute_curve, debye_intensity, debye_intensity_with_hydrogens,
+    debye_pair_tables, debye_solvent_basis, default_context, device_count,
+    excluded_volume_scale, form_factor_for_atom, generate_domain_chain, generate_globule,
+    generate_shell_sites, solvent_corrected_intensity, water_form_factor,
+)

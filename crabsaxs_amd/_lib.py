@@ -1,0 +1,119 @@
+"""ctypes binding of libcrabsaxs_hip.so (the C ABI declared in include/crabsaxs.h).
+
+The library is loaded from this package directory only.  There is no
+fallback: if it has not been built, importing the compute API raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+LIB_PATH = Path(__file__).resolve().parent / "libcrabsaxs_hip.so"
+
+c_double_p = C.POINTER(C.c_double)
+c_size_t_p = C.POINTER(C.c_size_t)
+
+
+class SolventParamsC(C.Structure):
+    _fields_ = [
+        ("c1", C.c_double),
+        ("c2", C.c_double),
+        ("shell_thickness_angstrom", C.c_double),
+        ("bulk_density_e_per_a3", C.c_double),
+        ("hydration_model", C.c_int32),
+        ("hydration_grid_spacing_angstrom", C.c_double),
+    ]
+
+
+class ComputeOptionsC(C.Structure):
+    _fields_ = [
+        ("method", C.c_int32),
+        ("implicit_hydrogen", C.c_int32),
+        ("has_solvent", C.c_int32),
+        ("solvent", SolventParamsC),
+        ("l_max", C.c_int32),
+        ("precision", C.c_int32),
+    ]
+
+
+# name -> (restype, argtypes); every symbol declared in include/crabsaxs.h.
+SIGNATURES = {
+    "csx_last_error": (C.c_char_p, []),
+    "csx_solvent_default": (None, [C.POINTER(SolventParamsC)]),
+    "csx_compute_options_default": (None, [C.POINTER(ComputeOptionsC)]),
+    "csx_device_count": (C.c_int, []),
+    "csx_ctx_create": (C.c_int, [C.c_int, C.POINTER(C.c_void_p)]),
+    "csx_ctx_destroy": (None, [C.c_void_p]),
+    "csx_ctx_set_timing": (C.c_int, [C.c_void_p, C.c_int]),
+    "csx_ctx_kernel_time": (C.c_int, [C.c_void_p, C.c_char_p, C.POINTER(C.c_int64),
+                                      c_double_p]),
+    "csx_ctx_reset_timing": (C.c_int, [C.c_void_p]),
+    "csx_ctx_set_tuning": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int]),
+    "csx_element_from_symbol": (C.c_int, [C.c_char_p]),
+    "csx_united_atom_group": (C.c_int, [C.c_char_p, C.c_char_p]),
+    "csx_form_factor": (C.c_double, [C.c_int, C.c_int, C.c_double, C.c_int, C.c_double]),
+    "csx_excluded_volume": (C.c_double, [C.c_int, C.c_int, C.c_int]),
+    "csx_water_form_factor": (C.c_double, [C.c_double]),
+    "csx_excluded_volume_scale": (C.c_double, [C.c_double, C.c_double]),
+    "csx_structure_create": (C.c_int, [C.c_void_p, C.c_size_t, c_double_p,
+                                       C.POINTER(C.c_uint8), C.POINTER(C.c_int8), c_double_p,
+                                       C.POINTER(C.c_void_p)]),
+    "csx_structure_destroy": (None, [C.c_void_p]),
+    "csx_structure_atom_count": (C.c_size_t, [C.c_void_p]),
+    "csx_build_adaptive_hydration_grid": (C.c_int, [c_double_p, C.c_size_t, C.c_double,
+                                                    C.c_double, C.c_double, c_double_p,
+                                                    C.c_size_t, c_size_t_p, c_double_p]),
+    "csx_build_hydration_grid": (C.c_int, [c_double_p, C.POINTER(C.c_uint8), C.c_size_t,
+                                           C.c_double, C.c_double, C.c_double, c_double_p,
+                                           C.c_size_t, c_size_t_p, c_double_p]),
+    "csx_debye_intensity": (C.c_int, [C.c_void_p, c_double_p, C.c_size_t, C.c_int, C.c_int,
+                                      c_double_p]),
+    "csx_debye_solvent_basis": (C.c_int, [C.c_void_p, c_double_p, C.c_size_t,
+                                          C.POINTER(SolventParamsC), C.c_int, C.c_int,
+                                          c_double_p]),
+    "csx_solvent_corrected_intensity": (C.c_int, [C.c_void_p, c_double_p, C.c_size_t,
+                                                  C.POINTER(SolventParamsC), C.c_int, C.c_int,
+                                                  c_double_p]),
+    "csx_debye_pair_tables": (C.c_int, [C.c_void_p, c_double_p, C.c_size_t,
+                                        C.POINTER(SolventParamsC), c_double_p, c_double_p,
+                                        c_size_t_p]),
+    "csx_combine_solvent_basis": (C.c_int, [c_double_p, c_double_p, C.c_size_t, C.c_double,
+                                            C.c_double, c_double_p]),
+    "csx_compute_curve": (C.c_int, [C.c_void_p, c_double_p, C.c_size_t,
+                                    C.POINTER(ComputeOptionsC), c_double_p,
+                                    C.POINTER(C.c_int)]),
+    "csx_generate_globule": (C.c_int, [C.c_size_t, C.c_double, C.c_double, C.c_uint64,
+                                       c_double_p, C.POINTER(C.c_uint8)]),
+    "csx_generate_domain_chain": (C.c_int, [C.c_size_t, C.c_int, C.c_uint64, c_double_p,
+                                            C.POINTER(C.c_uint8)]),
+    "csx_generate_shell_sites": (C.c_int, [c_double_p, C.c_size_t, C.c_double, C.c_double,
+                                           C.c_double, c_double_p, C.c_size_t, c_size_t_p]),
+}
+
+_lib = None
+
+
+def load() -> C.CDLL:
+    """Load the HIP library (raises if it was not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise ImportError(f"{LIB_PATH} is missing: run `python -m crabsaxs_amd.build` first")
+    lib = C.CDLL(str(LIB_PATH))
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def dptr(a: np.ndarray):
+    return a.ctypes.data_as(c_double_p)
+
+
+def f64(a) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(a, dtype=np.float64))

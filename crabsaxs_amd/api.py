@@ -1,0 +1,480 @@
+"""Host-side mirror of crabSAXS's profile API over the C ABI.
+
+Names, argument meaning and error behaviour follow the reference's public
+Rust API (src/scattering.rs, src/solvent.rs, src/spherical.rs, src/fit.rs,
+src/ensemble.rs, src/analysis.rs); all arithmetic of the profile path runs in
+libcrabsaxs_hip.so on the GPU.  Errors raise SaxsError subclasses mirroring
+`SaxsError` (src/error.rs:4-25).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import enum
+import threading
+from dataclasses import dataclass, field, replace
+from typing import Optional, Sequence
+
+import numpy as np
+
+from . import _lib
+from ._lib import ComputeOptionsC, SolventParamsC, dptr, f64
+
+
+# ------------------------------------------------------------------ errors --
+
+class SaxsError(Exception):
+    """Base error, `SaxsError` (src/error.rs:4-25)."""
+
+
+class InvalidInput(SaxsError):
+    pass
+
+
+class FitDidNotConverge(SaxsError):
+    pass
+
+
+class NoDevice(SaxsError):
+    pass
+
+
+class DeviceError(SaxsError):
+    pass
+
+
+class OutOfMemory(SaxsError):
+    pass
+
+
+_STATUS = {1: InvalidInput, 2: FitDidNotConverge, 3: NoDevice, 4: DeviceError, 5: OutOfMemory}
+
+
+def _check(status: int) -> None:
+    if status != 0:
+        msg = _lib.load().csx_last_error().decode()
+        raise _STATUS.get(status, SaxsError)(msg)
+
+
+# ------------------------------------------------------------------- enums --
+
+class Element(enum.IntEnum):
+    """`Element` (src/structure.rs:90-106); codes match include/crabsaxs_tables.h."""
+    H = 0
+    C = 1
+    N = 2
+    O = 3
+    S = 4
+    P = 5
+    Na = 6
+    Mg = 7
+    Cl = 8
+    Ca = 9
+    Fe = 10
+    Zn = 11
+    K = 12
+    I = 13
+    Other = 14
+
+    @classmethod
+    def from_symbol(cls, symbol: str) -> "Element":
+        return cls(_lib.load().csx_element_from_symbol(symbol.encode()))
+
+
+class HydrogenMode(enum.Enum):
+    Implicit = "Implicit"
+    Explicit = "Explicit"
+
+
+class ScatteringMethod(enum.IntEnum):
+    Auto = 0
+    Debye = 1
+    Multipole = 2
+
+
+class HydrationModel(enum.IntEnum):
+    Analytic = 0
+    Grid = 1
+    AdaptiveGrid = 2
+
+
+class Precision(enum.IntEnum):
+    """Pair-sum arithmetic (no reference counterpart: the reference is FP64)."""
+    FP64_PLAIN = 0
+    FP32_COMPENSATED = 1
+
+
+# --------------------------------------------------------------- structure --
+
+@dataclass
+class Atom:
+    """`Atom` (src/structure.rs:151-160)."""
+    pos: Sequence[float]
+    element: Element = Element.C
+    occupancy: float = 1.0
+    is_hetatm: bool = False
+    atom_name: str = "C"
+    residue_name: str = "UNK"
+    chain_id: str = "A"
+    residue_id: int = 1
+
+
+class Structure:
+    """`Structure` (src/structure.rs:163-165), held as SoA arrays.
+
+    Built from `Atom`s (typed by residue/atom names exactly like
+    formfactor.rs:357-417) or directly from arrays.  The device copy is made
+    on first use and reused; treat a Structure as immutable once computed on.
+    """
+
+    def __init__(self, atoms: Sequence[Atom] = ()):
+        atoms = list(atoms)
+        lib = _lib.load()
+        self.xyz = f64([a.pos for a in atoms]).reshape(-1, 3)
+        self.element = np.array([int(a.element) for a in atoms], dtype=np.uint8)
+        self.group = np.array(
+            [lib.csx_united_atom_group(a.residue_name.encode(), a.atom_name.encode())
+             for a in atoms], dtype=np.int8)
+        self.occupancy = f64([a.occupancy for a in atoms])
+        self._handle = None
+        self._ctx = None
+
+    @classmethod
+    def from_arrays(cls, xyz, element, occupancy=None, group=None) -> "Structure":
+        s = cls.__new__(cls)
+        s.xyz = f64(xyz).reshape(-1, 3)
+        n = s.xyz.shape[0]
+        s.element = np.ascontiguousarray(np.asarray(element, dtype=np.uint8).reshape(n))
+        s.occupancy = f64(np.ones(n) if occupancy is None else occupancy).reshape(n)
+        s.group = np.ascontiguousarray(
+            np.full(n, -1, dtype=np.int8) if group is None else np.asarray(group, dtype=np.int8))
+        s._handle = None
+        s._ctx = None
+        return s
+
+    def __len__(self) -> int:
+        return int(self.xyz.shape[0])
+
+    @property
+    def atoms_count(self) -> int:
+        return len(self)
+
+    def handle(self, ctx: "Context" = None):
+        ctx = ctx or default_context()
+        if self._handle is None or self._ctx is not ctx:
+            h = C.c_void_p()
+            lib = _lib.load()
+            _check(lib.csx_structure_create(
+                ctx.ptr, len(self), dptr(self.xyz),
+                self.element.ctypes.data_as(C.POINTER(C.c_uint8)),
+                self.group.ctypes.data_as(C.POINTER(C.c_int8)), dptr(self.occupancy),
+                C.byref(h)))
+            self._free()
+            self._handle, self._ctx = h, ctx
+        return self._handle
+
+    def _free(self):
+        if getattr(self, "_handle", None) is not None:
+            _lib.load().csx_structure_destroy(self._handle)
+            self._handle = None
+
+    def __del__(self):
+        try:
+            self._free()
+        except Exception:
+            pass
+
+
+# ----------------------------------------------------------------- context --
+
+class Context:
+    """One HIP device; owns streams and kernel timing statistics."""
+
+    def __init__(self, device: int = 0):
+        self.ptr = C.c_void_p()
+        _check(_lib.load().csx_ctx_create(device, C.byref(self.ptr)))
+        self.device = device
+
+    def set_timing(self, enabled: bool) -> None:
+        _check(_lib.load().csx_ctx_set_timing(self.ptr, int(enabled)))
+
+    def reset_timing(self) -> None:
+        _check(_lib.load().csx_ctx_reset_timing(self.ptr))
+
+    def kernel_time(self, name: str):
+        n, ms = C.c_int64(), C.c_double()
+        _check(_lib.load().csx_ctx_kernel_time(self.ptr, name.encode(), C.byref(n), C.byref(ms)))
+        return int(n.value), float(ms.value)
+
+    def set_tuning(self, q_tile: int = 16, sin_mode: int = 0, j_chunk: int = 2048) -> None:
+        _check(_lib.load().csx_ctx_set_tuning(self.ptr, q_tile, sin_mode, j_chunk))
+
+    def close(self):
+        if self.ptr:
+            _lib.load().csx_ctx_destroy(self.ptr)
+            self.ptr = C.c_void_p()
+
+
+_default_ctx = None
+_ctx_lock = threading.Lock()
+
+
+def default_context() -> Context:
+    global _default_ctx
+    with _ctx_lock:
+        if _default_ctx is None:
+            _default_ctx = Context(0)
+        return _default_ctx
+
+
+def device_count() -> int:
+    return int(_lib.load().csx_device_count())
+
+
+# ---------------------------------------------------------------- options --
+
+@dataclass
+class SolventParams:
+    """`SolventParams` (src/solvent.rs:27-50)."""
+    c1: float = 1.0
+    c2: float = 0.0
+    shell_thickness_angstrom: float = 3.0
+    bulk_density_e_per_a3: float = 0.334
+    hydration_model: HydrationModel = HydrationModel.Analytic
+    hydration_grid_spacing_angstrom: float = 4.0
+
+    def to_c(self) -> SolventParamsC:
+        return SolventParamsC(self.c1, self.c2, self.shell_thickness_angstrom,
+                              self.bulk_density_e_per_a3, int(self.hydration_model),
+                              self.hydration_grid_spacing_angstrom)
+
+
+@dataclass
+class MultipoleOptions:
+    l_max: Optional[int] = None
+
+
+@dataclass
+class ComputeOptions:
+    """`ComputeOptions` (src/scattering.rs:28-47) plus the pair-sum precision."""
+    method: ScatteringMethod = ScatteringMethod.Auto
+    hydrogen_mode: HydrogenMode = HydrogenMode.Implicit
+    include_hetatm: bool = False
+    multipole: MultipoleOptions = field(default_factory=MultipoleOptions)
+    solvent: Optional[SolventParams] = None
+    precision: Precision = Precision.FP32_COMPENSATED
+
+    def to_c(self) -> ComputeOptionsC:
+        o = ComputeOptionsC()
+        o.method = int(self.method)
+        o.implicit_hydrogen = int(self.hydrogen_mode == HydrogenMode.Implicit)
+        o.has_solvent = int(self.solvent is not None)
+        o.solvent = (self.solvent or SolventParams()).to_c()
+        o.l_max = 0 if self.multipole.l_max is None else int(self.multipole.l_max)
+        o.precision = int(self.precision)
+        return o
+
+
+@dataclass
+class CurveMetadata:
+    method: str
+    atom_count: int
+
+
+@dataclass
+class ScatteringCurve:
+    q: np.ndarray
+    intensity: np.ndarray
+    metadata: CurveMetadata
+
+
+# --------------------------------------------------------------- functions --
+
+def _out(n: int) -> np.ndarray:
+    return np.zeros(n, dtype=np.float64)
+
+
+def compute_curve(structure: Structure, q_values, options: ComputeOptions = None,
+                  ctx: Context = None) -> ScatteringCurve:
+    """`compute_curve` (src/scattering.rs:112-212)."""
+    options = options or ComputeOptions()
+    q = f64(q_values).reshape(-1)
+    out = _out(q.size)
+    used = C.c_int()
+    opts = options.to_c()
+    _check(_lib.load().csx_compute_curve(structure.handle(ctx), dptr(q), q.size, C.byref(opts),
+                                         dptr(out), C.byref(used)))
+    return ScatteringCurve(q.copy(), out, CurveMetadata(ScatteringMethod(used.value).name,
+                                                        len(structure)))
+
+
+def debye_intensity_with_hydrogens(structure: Structure, q_values, implicit_hydrogen: bool,
+                                   precision: Precision = Precision.FP32_COMPENSATED,
+                                   ctx: Context = None) -> np.ndarray:
+    """`debye_intensity_with_hydrogens` (src/scattering.rs:86-110)."""
+    q = f64(q_values).reshape(-1)
+    out = _out(q.size)
+    _check(_lib.load().csx_debye_intensity(structure.handle(ctx), dptr(q), q.size,
+                                           int(implicit_hydrogen), int(precision), dptr(out)))
+    return out
+
+
+def debye_intensity(structure: Structure, q_values,
+                    precision: Precision = Precision.FP32_COMPENSATED,
+                    ctx: Context = None) -> np.ndarray:
+    """`debye_intensity` (src/scattering.rs:82-84): implicit hydrogens."""
+    return debye_intensity_with_hydrogens(structure, q_values, True, precision, ctx)
+
+
+def debye_solvent_basis(structure: Structure, q_values, params: SolventParams,
+                        implicit_hydrogen: bool = True,
+                        precision: Precision = Precision.FP32_COMPENSATED,
+                        ctx: Context = None) -> np.ndarray:
+    """`debye_solvent_basis` (src/solvent.rs:504-584): array [6, nq]."""
+    q = f64(q_values).reshape(-1)
+    out = _out(6 * q.size)
+    p = params.to_c()
+    _check(_lib.load().csx_debye_solvent_basis(structure.handle(ctx), dptr(q), q.size,
+                                               C.byref(p), int(implicit_hydrogen),
+                                               int(precision), dptr(out)))
+    return out.reshape(6, q.size)
+
+
+def solvent_corrected_intensity(structure: Structure, q_values, params: SolventParams,
+                                implicit_hydrogen: bool = True,
+                                precision: Precision = Precision.FP32_COMPENSATED,
+                                ctx: Context = None) -> np.ndarray:
+    """`solvent_corrected_intensity` (src/solvent.rs:490-500)."""
+    q = f64(q_values).reshape(-1)
+    out = _out(q.size)
+    p = params.to_c()
+    _check(_lib.load().csx_solvent_corrected_intensity(structure.handle(ctx), dptr(q), q.size,
+                                                       C.byref(p), int(implicit_hydrogen),
+                                                       int(precision), dptr(out)))
+    return out
+
+
+def debye_pair_tables(structure: Structure, q_values, params: SolventParams = None,
+                      ctx: Context = None):
+    """Per-type pair tables: (S [nbins, nq], self [T])."""
+    q = f64(q_values).reshape(-1)
+    lib = _lib.load()
+    h = structure.handle(ctx)
+    pc = params.to_c() if params is not None else None
+    pp = C.byref(pc) if pc is not None else None
+    nt = C.c_size_t()
+    _check(lib.csx_debye_pair_tables(h, dptr(q), q.size, pp, None, None, C.byref(nt)))
+    t = nt.value
+    nb = t * (t + 1) // 2
+    out = _out(nb * q.size)
+    self_terms = _out(t)
+    _check(lib.csx_debye_pair_tables(h, dptr(q), q.size, pp, dptr(out), dptr(self_terms),
+                                     C.byref(nt)))
+    return out.reshape(nb, q.size), self_terms
+
+
+def combine_solvent_basis(basis, q_values, c1: float, c2: float) -> np.ndarray:
+    """`combine_solvent_basis` (src/spherical.rs:567-584)."""
+    q = f64(q_values).reshape(-1)
+    b = f64(basis).reshape(6, q.size)
+    out = _out(q.size)
+    _check(_lib.load().csx_combine_solvent_basis(dptr(b), dptr(q), q.size, c1, c2, dptr(out)))
+    return out
+
+
+def excluded_volume_scale(q: float, radius_scale: float) -> float:
+    """`excluded_volume_scale` (src/solvent.rs:448-452)."""
+    return float(_lib.load().csx_excluded_volume_scale(q, radius_scale))
+
+
+def water_form_factor(q: float) -> float:
+    """`water_form_factor` (src/solvent.rs:227-231)."""
+    return float(_lib.load().csx_water_form_factor(q))
+
+
+def form_factor_for_atom(atom: Atom, implicit_hydrogen: bool, q: float) -> float:
+    """`form_factor_for_atom` (src/formfactor.rs:425-439)."""
+    lib = _lib.load()
+    group = lib.csx_united_atom_group(atom.residue_name.encode(), atom.atom_name.encode())
+    return float(lib.csx_form_factor(int(atom.element), group, atom.occupancy,
+                                     int(implicit_hydrogen), q))
+
+
+@dataclass
+class HydrationGrid:
+    sites: np.ndarray
+    voxel_volume_a3: float
+
+
+def _grid(call, *args) -> HydrationGrid:
+    count, vol = C.c_size_t(), C.c_double()
+    _check(call(*args, None, 0, C.byref(count), C.byref(vol)))
+    sites = np.zeros((count.value, 3))
+    _check(call(*args, dptr(sites), count.value, C.byref(count), C.byref(vol)))
+    return HydrationGrid(sites, float(vol.value))
+
+
+def build_adaptive_hydration_grid(structure: Structure, spacing_angstrom: float = 4.0,
+                                  molecular_clearance_angstrom: float = 3.0,
+                                  padding_angstrom: float = 12.0) -> HydrationGrid:
+    """`build_adaptive_hydration_grid` (src/solvent.rs:121-223)."""
+    lib = _lib.load()
+    return _grid(lib.csx_build_adaptive_hydration_grid, dptr(structure.xyz), len(structure),
+                 spacing_angstrom, molecular_clearance_angstrom, padding_angstrom)
+
+
+def build_hydration_grid(structure: Structure, spacing_angstrom: float = 1.0,
+                         probe_radius_angstrom: float = 1.4,
+                         shell_thickness_angstrom: float = 3.0) -> HydrationGrid:
+    """`build_hydration_grid` (src/solvent.rs:259-401)."""
+    lib = _lib.load()
+    return _grid(lib.csx_build_hydration_grid, dptr(structure.xyz),
+                 structure.element.ctypes.data_as(C.POINTER(C.c_uint8)), len(structure),
+                 spacing_angstrom, probe_radius_angstrom, shell_thickness_angstrom)
+
+
+# -------------------------------------------------------- synthetic inputs --
+
+def generate_globule(n: int, radius: float, elongation: float = 1.0,
+                     seed: int = 0) -> Structure:
+    """Seeded synthetic globular protein (see include/crabsaxs.h)."""
+    xyz = np.zeros((n, 3))
+    el = np.zeros(n, dtype=np.uint8)
+    _check(_lib.load().csx_generate_globule(n, radius, elongation, seed, dptr(xyz),
+                                            el.ctypes.data_as(C.POINTER(C.c_uint8))))
+    return Structure.from_arrays(xyz, el)
+
+
+def generate_domain_chain(n: int, domains: int, seed: int = 0) -> Structure:
+    xyz = np.zeros((n, 3))
+    el = np.zeros(n, dtype=np.uint8)
+    _check(_lib.load().csx_generate_domain_chain(n, domains, seed, dptr(xyz),
+                                                 el.ctypes.data_as(C.POINTER(C.c_uint8))))
+    return Structure.from_arrays(xyz, el)
+
+
+def generate_shell_sites(structure: Structure, spacing: float, inner: float,
+                         outer: float) -> np.ndarray:
+    lib = _lib.load()
+    count = C.c_size_t()
+    _check(lib.csx_generate_shell_sites(dptr(structure.xyz), len(structure), spacing, inner,
+                                        outer, None, 0, C.byref(count)))
+    sites = np.zeros((count.value, 3))
+    _check(lib.csx_generate_shell_sites(dptr(structure.xyz), len(structure), spacing, inner,
+                                        outer, dptr(sites), count.value, C.byref(count)))
+    return sites
+
+
+# --------------------------------------------------------------- analysis --
+
+class ProfileCalculator:
+    """`ProfileCalculator` (src/analysis.rs:58-63, 90-105)."""
+
+    def __init__(self, q, options: ComputeOptions = None):
+        q = f64(q).reshape(-1)
+        if (not np.all(np.isfinite(q))) or np.any(q < 0) or np.any(np.diff(q) < 0):
+            raise InvalidInput("q grid must be finite, non-negative, and non-decreasing")
+        self.q = q
+        self.options = options or ComputeOptions()
+
+    def calculate(self, structure: Structure, ctx: Context = None) -> ScatteringCurve:
+        return compute_curve(structure, self.q, self.options, ctx)

@@ -1,0 +1,753 @@
+// C-ABI entry points: contexts, structures, the Debye path and compute_curve.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <sstream>
+
+#include "capi_internal.hpp"
+#include "../../include/crabsaxs_tables.h"
+
+namespace csx {
+
+static thread_local std::string g_error;
+
+csx_status fail(csx_status st, const std::string& msg) {
+  g_error = msg;
+  return st;
+}
+
+csx_status hip_fail(hipError_t e, const char* what) {
+  const csx_status st = e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation ? CSX_OUT_OF_MEMORY
+                        : e == hipErrorNoDevice || e == hipErrorInvalidDevice   ? CSX_NO_DEVICE
+                                                                                 : CSX_DEVICE_ERROR;
+  return fail(st, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// Scope of one API call on a device: current device, a leased stream, an
+// optional kernel timer merged into the context statistics at the end.
+class Call {
+ public:
+  explicit Call(csx_ctx* ctx) : ctx_(ctx) {}
+  ~Call() {
+    if (stream_) {
+      std::lock_guard<std::mutex> lock(ctx_->mu);
+      ctx_->free_streams.push_back(stream_);
+    }
+  }
+  csx_status begin() {
+    hipError_t e = hipSetDevice(ctx_->device);
+    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+    {
+      std::lock_guard<std::mutex> lock(ctx_->mu);
+      if (!ctx_->free_streams.empty()) {
+        stream_ = ctx_->free_streams.back();
+        ctx_->free_streams.pop_back();
+      }
+    }
+    if (!stream_) {
+      e = hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking);
+      if (e != hipSuccess) {
+        stream_ = nullptr;
+        return hip_fail(e, "hipStreamCreate");
+      }
+    }
+    if (ctx_->timing.load()) timer_.reset(new KernelTimer);
+    return CSX_OK;
+  }
+  // Wait for the stream and fold kernel timings into the context.
+  csx_status finish() {
+    hipError_t e = hipStreamSynchronize(stream_);
+    if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
+    if (timer_) {
+      std::lock_guard<std::mutex> lock(ctx_->mu);
+      for (auto& p : timer_->pending) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, p.start, p.stop) == hipSuccess) {
+          auto& st = ctx_->stats[p.name];
+          st.first += 1;
+          st.second += ms;
+        }
+      }
+      timer_.reset();
+    }
+    return CSX_OK;
+  }
+  hipStream_t stream() const { return stream_; }
+  KernelTimer* timer() const { return timer_.get(); }
+  csx_ctx* ctx() const { return ctx_; }
+
+ private:
+  csx_ctx* ctx_;
+  hipStream_t stream_ = nullptr;
+  std::unique_ptr<KernelTimer> timer_;
+};
+
+#define CSX_HIP(expr, what)                          \
+  do {                                               \
+    hipError_t e_ = (expr);                          \
+    if (e_ != hipSuccess) return hip_fail(e_, what); \
+  } while (0)
+#define CSX_TRY(expr)                \
+  do {                               \
+    csx_status st_ = (expr);         \
+    if (st_ != CSX_OK) return st_;   \
+  } while (0)
+
+// ------------------------------------------------------------ point sets --
+
+static std::string set_key(const csx_solvent_params* p) {
+  if (!p || p->hydration_model == CSX_HYDRATION_ANALYTIC) return "atoms";
+  std::ostringstream k;
+  k.precision(17);
+  if (p->hydration_model == CSX_HYDRATION_GRID)
+    k << "grid:" << p->shell_thickness_angstrom;
+  else
+    k << "adaptive:" << p->hydration_grid_spacing_angstrom;
+  return k.str();
+}
+
+static csx_status build_point_set(csx_structure* s, const csx_solvent_params* p, PointSet& ps) {
+  const int T = static_cast<int>(s->types.size());
+  const size_t n = s->n;
+  if (p && p->hydration_model != CSX_HYDRATION_ANALYTIC) {
+    Grid g = p->hydration_model == CSX_HYDRATION_GRID
+                 ? build_hydration_grid(s->xyz.data(), s->element.data(), n, 1.0, 1.4,
+                                        p->shell_thickness_angstrom)
+                 : build_adaptive_hydration_grid(s->xyz.data(), n,
+                                                 p->hydration_grid_spacing_angstrom, 3.0, 12.0);
+    if (!g.error.empty()) return fail(CSX_INVALID_INPUT, g.error);
+    ps.site_xyz = std::move(g.sites);
+    ps.site_volume = g.voxel_volume;
+    ps.nsites = static_cast<int>(ps.site_xyz.size() / 3);
+  }
+  const bool with_sites = p && p->hydration_model != CSX_HYDRATION_ANALYTIC;
+  ps.natoms = static_cast<int>(n);
+  ps.ntypes = T + (with_sites ? 1 : 0);
+  const int TT = ps.ntypes;
+  const size_t npts = n + ps.nsites;
+  if (npts > static_cast<size_t>(INT32_MAX / 2)) return fail(CSX_INVALID_INPUT, "too many points");
+
+  // Type-sorted FP32 layout.
+  std::vector<size_t> order(n);
+  std::iota(order.begin(), order.end(), size_t(0));
+  std::stable_sort(order.begin(), order.end(),
+                   [&](size_t a, size_t b) { return s->type_of[a] < s->type_of[b]; });
+  std::vector<float> pts32(4 * npts);
+  std::vector<int> seg_start(TT + 1, 0);
+  ps.self.assign(TT, 0.0);
+  ps.wsum.assign(TT, 0.0);
+  for (size_t k = 0; k < n; ++k) {
+    const size_t i = order[k];
+    const int t = s->type_of[i];
+    for (int a = 0; a < 3; ++a) pts32[4 * k + a] = static_cast<float>(s->xyz[3 * i + a] - s->centre[a]);
+    pts32[4 * k + 3] = static_cast<float>(s->occ[i]);
+    seg_start[t + 1] += 1;
+    ps.self[t] += s->occ[i] * s->occ[i];
+    ps.wsum[t] += s->occ[i];
+  }
+  for (int k = 0; k < ps.nsites; ++k) {
+    for (int a = 0; a < 3; ++a)
+      pts32[4 * (n + k) + a] = static_cast<float>(ps.site_xyz[3 * k + a] - s->centre[a]);
+    pts32[4 * (n + k) + 3] = 1.0f;
+  }
+  if (with_sites) {
+    seg_start[T + 1] = ps.nsites;
+    ps.self[T] = ps.nsites;
+    ps.wsum[T] = ps.nsites;
+  }
+  for (int t = 0; t < TT; ++t) seg_start[t + 1] += seg_start[t];
+
+  // Work items in bin-major order; bins are (a <= b) row-major.
+  const int kI = 256;
+  const int chunk = s->ctx->tuning.j_chunk;
+  std::vector<int> items;
+  std::vector<int> bin_first;
+  for (int a = 0; a < TT; ++a)
+    for (int b = a; b < TT; ++b) {
+      bin_first.push_back(static_cast<int>(items.size() / 4));
+      for (int i0 = seg_start[a]; i0 < seg_start[a + 1]; i0 += kI) {
+        const int ilen = std::min(kI, seg_start[a + 1] - i0);
+        const int jlo = std::max(seg_start[b], i0), jhi = seg_start[b + 1];
+        for (int j0 = jlo; j0 < jhi; j0 += chunk) {
+          items.insert(items.end(), {i0, ilen, j0, std::min(chunk, jhi - j0)});
+        }
+      }
+    }
+  bin_first.push_back(static_cast<int>(items.size() / 4));
+  ps.nitems = static_cast<int>(items.size() / 4);
+  ps.nbins = TT * (TT + 1) / 2;
+  ps.j_chunk = chunk;
+
+  // FP64 layout in original order.
+  std::vector<double> pts64(4 * npts, 0.0), occ64(npts);
+  std::vector<int> type64(npts);
+  for (size_t i = 0; i < n; ++i) {
+    for (int a = 0; a < 3; ++a) pts64[4 * i + a] = s->xyz[3 * i + a];
+    occ64[i] = s->occ[i];
+    type64[i] = s->type_of[i];
+  }
+  for (int k = 0; k < ps.nsites; ++k) {
+    for (int a = 0; a < 3; ++a) pts64[4 * (n + k) + a] = ps.site_xyz[3 * k + a];
+    occ64[n + k] = 1.0;
+    type64[n + k] = T;
+  }
+  CSX_HIP(ps.pts32.upload(pts32.data(), pts32.size() * sizeof(float)), "upload points");
+  CSX_HIP(ps.items.upload(items.data(), items.size() * sizeof(int)), "upload items");
+  CSX_HIP(ps.bin_first.upload(bin_first.data(), bin_first.size() * sizeof(int)), "upload bins");
+  CSX_HIP(ps.pts64.upload(pts64.data(), pts64.size() * sizeof(double)), "upload points");
+  CSX_HIP(ps.type64.upload(type64.data(), type64.size() * sizeof(int)), "upload types");
+  CSX_HIP(ps.occ64.upload(occ64.data(), occ64.size() * sizeof(double)), "upload occupancy");
+  return CSX_OK;
+}
+
+static csx_status get_point_set(csx_structure* s, const csx_solvent_params* p, PointSet** out) {
+  std::lock_guard<std::mutex> lock(s->mu);
+  const std::string key = set_key(p);
+  auto it = s->sets.find(key);
+  if (it != s->sets.end() && it->second->j_chunk == s->ctx->tuning.j_chunk) {
+    *out = it->second.get();
+    return CSX_OK;
+  }
+  std::unique_ptr<PointSet> ps(new PointSet);
+  CSX_TRY(build_point_set(s, p, *ps));
+  *out = ps.get();
+  s->sets[key] = std::move(ps);
+  return CSX_OK;
+}
+
+// --------------------------------------------------------- pair tables --
+
+// P[b*nq + k] = sum_{i<j in bin b} occ_i occ_j sinc(q_k r_ij).
+static csx_status pair_tables_f32(Call& call, const PointSet& ps, const double* q, size_t nq,
+                                  std::vector<double>& P) {
+  const Tuning tune = call.ctx()->tuning;
+  const int qt = tune.q_tile;
+  const size_t nq_pad = (nq + qt - 1) / qt * qt;
+  std::vector<float> qrev(nq_pad, 0.0f);
+  for (size_t k = 0; k < nq; ++k) qrev[k] = static_cast<float>(q[k] / (2.0 * kPi));
+  const hipStream_t st = call.stream();
+  DevBuf d_q, d_partial, d_out;
+  CSX_HIP(d_q.alloc(nq_pad * sizeof(float), st), "hipMallocAsync");
+  CSX_HIP(d_partial.alloc(size_t(ps.nitems) * nq_pad * sizeof(double), st), "hipMallocAsync");
+  CSX_HIP(d_out.alloc(size_t(ps.nbins) * nq_pad * sizeof(double), st), "hipMallocAsync");
+  CSX_HIP(hipMemcpyAsync(d_q.as<void>(), qrev.data(), nq_pad * sizeof(float),
+                         hipMemcpyHostToDevice, st),
+          "hipMemcpyAsync");
+  CSX_HIP(launch_debye_pairs(ps.pts32.as<float4>(), ps.items.as<int4>(), ps.nitems,
+                             ps.bin_first.as<int>(), ps.nbins, d_q.as<float>(),
+                             static_cast<int>(nq_pad), qt, tune.sin_mode, d_partial.as<double>(),
+                             d_out.as<double>(), st, call.timer()),
+          "debye_pairs_f32");
+  std::vector<double> raw(size_t(ps.nbins) * nq_pad);
+  CSX_HIP(hipMemcpyAsync(raw.data(), d_out.as<void>(), raw.size() * sizeof(double),
+                         hipMemcpyDeviceToHost, st),
+          "hipMemcpyAsync");
+  d_q.reset();
+  d_partial.reset();
+  d_out.reset();
+  CSX_TRY(call.finish());
+  P.assign(size_t(ps.nbins) * nq, 0.0);
+  const int TT = ps.ntypes;
+  int b = 0;
+  for (int a = 0; a < TT; ++a)
+    for (int c = a; c < TT; ++c, ++b)
+      for (size_t k = 0; k < nq; ++k) {
+        const double qe = 2.0 * kPi * static_cast<double>(qrev[k]);
+        if (qe == 0.0) {
+          P[b * nq + k] = a == c ? 0.5 * (ps.wsum[a] * ps.wsum[a] - ps.self[a])
+                                 : ps.wsum[a] * ps.wsum[c];
+        } else {
+          P[b * nq + k] = raw[size_t(b) * nq_pad + k] / qe;
+        }
+      }
+  return CSX_OK;
+}
+
+// out[k] = sum_a X_a Y_a self_a + sum_{a<=b} (X_a Y_b + X_b Y_a) P_ab.
+static void contract(const PointSet& ps, const std::vector<double>& P, size_t nq,
+                     const std::vector<double>& X, const std::vector<double>& Y, double* out) {
+  const int TT = ps.ntypes;
+  for (size_t k = 0; k < nq; ++k) {
+    double sum = 0.0;
+    for (int a = 0; a < TT; ++a) sum += X[a * nq + k] * Y[a * nq + k] * ps.self[a];
+    int b = 0;
+    for (int a = 0; a < TT; ++a)
+      for (int c = a; c < TT; ++c, ++b)
+        sum += (X[a * nq + k] * Y[c * nq + k] + X[c * nq + k] * Y[a * nq + k]) * P[b * nq + k];
+    out[k] = sum;
+  }
+}
+
+// Per-type Debye amplitudes [TT][nq] (v, e, h); the site type carries only
+// the water amplitude of solvent.rs:560-562 in h.
+static void type_amplitudes(const csx_structure* s, const PointSet& ps, const double* q, size_t nq,
+                            bool implicit_h, const csx_solvent_params* p, std::vector<double>& V,
+                            std::vector<double>& E, std::vector<double>& H) {
+  const int T = static_cast<int>(s->types.size());
+  const int TT = ps.ntypes;
+  V.assign(size_t(TT) * nq, 0.0);
+  E.assign(size_t(TT) * nq, 0.0);
+  H.assign(size_t(TT) * nq, 0.0);
+  const bool grid = p && p->hydration_model != CSX_HYDRATION_ANALYTIC;
+  for (int t = 0; t < T; ++t)
+    for (size_t k = 0; k < nq; ++k) {
+      if (!p) {
+        V[t * nq + k] = type_form_factor(s->types[t], implicit_h, q[k]);
+        continue;
+      }
+      const Amplitudes a = debye_amplitudes(s->types[t], implicit_h, q[k],
+                                            p->shell_thickness_angstrom, p->bulk_density_e_per_a3);
+      V[t * nq + k] = a.v;
+      E[t * nq + k] = a.e;
+      H[t * nq + k] = grid ? 0.0 : a.h;
+    }
+  if (grid)
+    for (size_t k = 0; k < nq; ++k)
+      H[T * nq + k] = p->bulk_density_e_per_a3 / 10.0 * ps.site_volume * water_form_factor(q[k]);
+}
+
+// FP64 plain Debye over per-point amplitudes: 1 channel -> I, 3 -> 6 terms.
+static csx_status plain_f64(Call& call, const PointSet& ps, const double* q, size_t nq, int nch,
+                            const std::vector<double>* tables[3], double* out) {
+  const hipStream_t st = call.stream();
+  const int npts = nch == 1 ? ps.natoms : ps.natoms + ps.nsites;
+  const int TT = ps.ntypes;
+  const int nterms = nch == 1 ? 1 : 6;
+  if (npts == 0) {
+    std::fill(out, out + nterms * nq, 0.0);
+    return CSX_OK;
+  }
+  std::vector<double> table(size_t(nch) * TT * nq);
+  for (int c = 0; c < nch; ++c)
+    std::copy(tables[c]->begin(), tables[c]->end(), table.begin() + size_t(c) * TT * nq);
+  const int nblk = (npts + 255) / 256;
+  DevBuf d_table, d_amp, d_q, d_partial, d_out;
+  CSX_HIP(d_table.alloc(table.size() * sizeof(double), st), "hipMallocAsync");
+  CSX_HIP(d_amp.alloc(size_t(nch) * nq * npts * sizeof(double), st), "hipMallocAsync");
+  CSX_HIP(d_q.alloc(nq * sizeof(double), st), "hipMallocAsync");
+  CSX_HIP(d_partial.alloc(size_t(nq) * nblk * nterms * sizeof(double), st), "hipMallocAsync");
+  CSX_HIP(d_out.alloc(size_t(nterms) * nq * sizeof(double), st), "hipMallocAsync");
+  CSX_HIP(hipMemcpyAsync(d_table.as<void>(), table.data(), table.size() * sizeof(double),
+                         hipMemcpyHostToDevice, st),
+          "hipMemcpyAsync");
+  CSX_HIP(hipMemcpyAsync(d_q.as<void>(), q, nq * sizeof(double), hipMemcpyHostToDevice, st),
+          "hipMemcpyAsync");
+  CSX_HIP(launch_expand_amplitudes(d_table.as<double>(), ps.type64.as<int>(), ps.occ64.as<double>(),
+                                   npts, TT, static_cast<int>(nq), nch, d_amp.as<double>(), st),
+          "expand_amplitudes");
+  CSX_HIP(launch_debye_plain(ps.pts64.as<double4>(), d_amp.as<double>(), d_q.as<double>(), npts,
+                             static_cast<int>(nq), nch, nterms, d_partial.as<double>(),
+                             d_out.as<double>(), st, call.timer()),
+          "debye_plain_f64");
+  CSX_HIP(hipMemcpyAsync(out, d_out.as<void>(), size_t(nterms) * nq * sizeof(double),
+                         hipMemcpyDeviceToHost, st),
+          "hipMemcpyAsync");
+  d_table.reset();
+  d_amp.reset();
+  d_q.reset();
+  d_partial.reset();
+  d_out.reset();
+  return call.finish();
+}
+
+static csx_status check_structure(const csx_structure* s, const void* q, size_t nq,
+                                  const void* out) {
+  if (!s) return fail(CSX_INVALID_INPUT, "structure handle is null");
+  if (nq > 0 && (!q || !out)) return fail(CSX_INVALID_INPUT, "q and output buffers are required");
+  if (nq > static_cast<size_t>(INT32_MAX / 64)) return fail(CSX_INVALID_INPUT, "too many q values");
+  return CSX_OK;
+}
+
+static csx_status debye_basis(csx_structure* s, const double* q, size_t nq,
+                              const csx_solvent_params* p, bool implicit_h, int precision,
+                              double* out) {
+  Call call(s->ctx);
+  CSX_TRY(call.begin());
+  PointSet* ps = nullptr;
+  CSX_TRY(get_point_set(s, p, &ps));
+  std::vector<double> V, E, H;
+  type_amplitudes(s, *ps, q, nq, implicit_h, p, V, E, H);
+  if (precision == CSX_FP64_PLAIN) {
+    const std::vector<double>* tabs[3] = {&V, &E, &H};
+    return plain_f64(call, *ps, q, nq, 3, tabs, out);
+  }
+  std::vector<double> P;
+  CSX_TRY(pair_tables_f32(call, *ps, q, nq, P));
+  contract(*ps, P, nq, V, V, out + 0 * nq);
+  contract(*ps, P, nq, E, E, out + 1 * nq);
+  contract(*ps, P, nq, H, H, out + 2 * nq);
+  contract(*ps, P, nq, V, E, out + 3 * nq);
+  contract(*ps, P, nq, V, H, out + 4 * nq);
+  contract(*ps, P, nq, E, H, out + 5 * nq);
+  return CSX_OK;
+}
+
+static void combine(const double* basis, const double* q, size_t nq, double c1, double c2,
+                    double* out) {
+  for (size_t i = 0; i < nq; ++i) {
+    const double g = excluded_volume_scale(q[i], c1);
+    const double v = basis[i] + g * g * basis[nq + i] + c2 * c2 * basis[2 * nq + i] -
+                     2.0 * g * basis[3 * nq + i] + 2.0 * c2 * basis[4 * nq + i] -
+                     2.0 * g * c2 * basis[5 * nq + i];
+    out[i] = std::max(v, 0.0);
+  }
+}
+
+static bool precision_ok(int precision) {
+  return precision == CSX_FP64_PLAIN || precision == CSX_FP32_COMPENSATED;
+}
+
+}  // namespace csx
+
+using namespace csx;
+
+// ================================================================ C ABI ==
+
+extern "C" {
+
+const char* csx_last_error(void) { return g_error.c_str(); }
+
+void csx_solvent_default(csx_solvent_params* out) {
+  if (!out) return;
+  out->c1 = 1.0;
+  out->c2 = 0.0;
+  out->shell_thickness_angstrom = 3.0;
+  out->bulk_density_e_per_a3 = 0.334;
+  out->hydration_model = CSX_HYDRATION_ANALYTIC;
+  out->hydration_grid_spacing_angstrom = 4.0;
+}
+
+void csx_compute_options_default(csx_compute_options* out) {
+  if (!out) return;
+  out->method = CSX_METHOD_AUTO;
+  out->implicit_hydrogen = 1;
+  out->has_solvent = 0;
+  csx_solvent_default(&out->solvent);
+  out->l_max = 0;
+  out->precision = CSX_FP32_COMPENSATED;
+}
+
+int csx_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+csx_status csx_ctx_create(int device, csx_ctx** out) {
+  if (!out) return fail(CSX_INVALID_INPUT, "output pointer is null");
+  *out = nullptr;
+  int n = 0;
+  const hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess || n == 0) return fail(CSX_NO_DEVICE, "no HIP device is available");
+  if (device < 0 || device >= n) return fail(CSX_NO_DEVICE, "device index out of range");
+  csx_ctx* ctx = new (std::nothrow) csx_ctx;
+  if (!ctx) return fail(CSX_OUT_OF_MEMORY, "context allocation failed");
+  ctx->device = device;
+  *out = ctx;
+  return CSX_OK;
+}
+
+void csx_ctx_destroy(csx_ctx* ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->device);
+  for (hipStream_t s : ctx->free_streams) (void)hipStreamDestroy(s);
+  delete ctx;
+}
+
+csx_status csx_ctx_set_timing(csx_ctx* ctx, int enabled) {
+  if (!ctx) return fail(CSX_INVALID_INPUT, "context is null");
+  ctx->timing.store(enabled != 0);
+  return CSX_OK;
+}
+
+csx_status csx_ctx_kernel_time(csx_ctx* ctx, const char* name, int64_t* launches, double* total_ms) {
+  if (!ctx || !name) return fail(CSX_INVALID_INPUT, "context and name are required");
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  auto it = ctx->stats.find(name);
+  if (launches) *launches = it == ctx->stats.end() ? 0 : it->second.first;
+  if (total_ms) *total_ms = it == ctx->stats.end() ? 0.0 : it->second.second;
+  return CSX_OK;
+}
+
+csx_status csx_ctx_reset_timing(csx_ctx* ctx) {
+  if (!ctx) return fail(CSX_INVALID_INPUT, "context is null");
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  ctx->stats.clear();
+  return CSX_OK;
+}
+
+csx_status csx_ctx_set_tuning(csx_ctx* ctx, int q_tile, int sin_mode, int j_chunk) {
+  if (!ctx) return fail(CSX_INVALID_INPUT, "context is null");
+  if (q_tile != 8 && q_tile != 16 && q_tile != 32)
+    return fail(CSX_INVALID_INPUT, "q_tile must be 8, 16 or 32");
+  if (sin_mode != 0 && sin_mode != 1) return fail(CSX_INVALID_INPUT, "sin_mode must be 0 or 1");
+  if (j_chunk < 256 || j_chunk % 256 != 0 || j_chunk > (1 << 20))
+    return fail(CSX_INVALID_INPUT, "j_chunk must be a multiple of 256 in [256, 2^20]");
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  ctx->tuning.q_tile = q_tile;
+  ctx->tuning.sin_mode = sin_mode;
+  ctx->tuning.j_chunk = j_chunk;
+  return CSX_OK;
+}
+
+int csx_element_from_symbol(const char* symbol) {
+  return symbol ? element_from_symbol(symbol) : CSX_EL_OTHER;
+}
+
+int csx_united_atom_group(const char* residue, const char* atom_name) {
+  if (!residue || !atom_name) return -1;
+  return united_atom_group(residue, atom_name);
+}
+
+double csx_form_factor(int element, int group, double occupancy, int implicit_h, double q) {
+  if (element < 0 || element >= CSX_NUM_ELEMENTS || group < -1 || group >= CSX_NUM_UA_GROUPS)
+    return 0.0;
+  return occupancy * type_form_factor({element, group}, implicit_h != 0, q);
+}
+
+double csx_excluded_volume(int element, int group, int implicit_h) {
+  if (element < 0 || element >= CSX_NUM_ELEMENTS || group < -1 || group >= CSX_NUM_UA_GROUPS)
+    return 0.0;
+  return type_excluded_volume({element, group}, implicit_h != 0);
+}
+
+double csx_water_form_factor(double q) { return water_form_factor(q); }
+
+double csx_excluded_volume_scale(double q, double radius_scale) {
+  return excluded_volume_scale(q, radius_scale);
+}
+
+csx_status csx_structure_create(csx_ctx* ctx, size_t n, const double* xyz, const uint8_t* element,
+                                const int8_t* group, const double* occupancy,
+                                csx_structure** out) {
+  if (!ctx || !out) return fail(CSX_INVALID_INPUT, "context and output pointer are required");
+  *out = nullptr;
+  if (n > 0 && (!xyz || !element || !occupancy))
+    return fail(CSX_INVALID_INPUT, "xyz, element and occupancy arrays are required");
+  if (n > static_cast<size_t>(INT32_MAX / 4)) return fail(CSX_INVALID_INPUT, "too many atoms");
+  std::unique_ptr<csx_structure> s(new (std::nothrow) csx_structure);
+  if (!s) return fail(CSX_OUT_OF_MEMORY, "structure allocation failed");
+  s->ctx = ctx;
+  s->n = n;
+  s->xyz.assign(xyz, xyz + 3 * n);
+  s->occ.assign(occupancy, occupancy + n);
+  s->element.assign(element, element + n);
+  s->group.assign(n, -1);
+  if (group) s->group.assign(group, group + n);
+  for (size_t i = 0; i < n; ++i) {
+    if (s->element[i] >= CSX_NUM_ELEMENTS) return fail(CSX_INVALID_INPUT, "element code out of range");
+    if (s->group[i] < -1 || s->group[i] >= CSX_NUM_UA_GROUPS)
+      return fail(CSX_INVALID_INPUT, "united-atom group out of range");
+  }
+  // Type partition: unique (element, group) keys in sorted order.
+  std::vector<TypeKey> keys(n);
+  for (size_t i = 0; i < n; ++i) keys[i] = {s->element[i], s->group[i]};
+  s->types = keys;
+  std::sort(s->types.begin(), s->types.end());
+  s->types.erase(std::unique(s->types.begin(), s->types.end()), s->types.end());
+  s->type_of.resize(n);
+  for (size_t i = 0; i < n; ++i)
+    s->type_of[i] = static_cast<int>(std::lower_bound(s->types.begin(), s->types.end(), keys[i]) -
+                                     s->types.begin());
+  // Centroid and maximum radius, as compute_curve does (scattering.rs:157-167).
+  for (size_t i = 0; i < n; ++i)
+    for (int a = 0; a < 3; ++a) s->centre[a] += s->xyz[3 * i + a];
+  for (int a = 0; a < 3; ++a) s->centre[a] /= static_cast<double>(std::max<size_t>(n, 1));
+  for (size_t i = 0; i < n; ++i) {
+    const double dx = s->xyz[3 * i] - s->centre[0], dy = s->xyz[3 * i + 1] - s->centre[1],
+                 dz = s->xyz[3 * i + 2] - s->centre[2];
+    s->max_radius = std::max(s->max_radius, std::sqrt(dx * dx + dy * dy + dz * dz));
+  }
+  const hipError_t e = hipSetDevice(ctx->device);
+  if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+  PointSet* ps = nullptr;
+  CSX_TRY(get_point_set(s.get(), nullptr, &ps));
+  *out = s.release();
+  return CSX_OK;
+}
+
+void csx_structure_destroy(csx_structure* s) {
+  if (!s) return;
+  (void)hipSetDevice(s->ctx->device);
+  delete s;
+}
+
+size_t csx_structure_atom_count(const csx_structure* s) { return s ? s->n : 0; }
+
+static csx_status copy_grid(const Grid& g, double* sites, size_t capacity, size_t* count,
+                            double* volume) {
+  if (!g.error.empty()) return fail(CSX_INVALID_INPUT, g.error);
+  const size_t ns = g.sites.size() / 3;
+  if (count) *count = ns;
+  if (volume) *volume = g.voxel_volume;
+  if (sites && capacity) std::copy(g.sites.begin(), g.sites.begin() + 3 * std::min(capacity, ns), sites);
+  return CSX_OK;
+}
+
+csx_status csx_build_adaptive_hydration_grid(const double* xyz, size_t n, double spacing,
+                                             double clearance, double padding, double* sites,
+                                             size_t capacity, size_t* count,
+                                             double* voxel_volume) {
+  if (n > 0 && !xyz) return fail(CSX_INVALID_INPUT, "xyz is required");
+  return copy_grid(build_adaptive_hydration_grid(xyz, n, spacing, clearance, padding), sites,
+                   capacity, count, voxel_volume);
+}
+
+csx_status csx_build_hydration_grid(const double* xyz, const uint8_t* element, size_t n,
+                                    double spacing, double probe_radius, double shell_thickness,
+                                    double* sites, size_t capacity, size_t* count,
+                                    double* voxel_volume) {
+  if (n > 0 && (!xyz || !element)) return fail(CSX_INVALID_INPUT, "xyz and element are required");
+  for (size_t i = 0; i < n; ++i)
+    if (element[i] >= CSX_NUM_ELEMENTS) return fail(CSX_INVALID_INPUT, "element code out of range");
+  return copy_grid(build_hydration_grid(xyz, element, n, spacing, probe_radius, shell_thickness),
+                   sites, capacity, count, voxel_volume);
+}
+
+csx_status csx_generate_shell_sites(const double* xyz, size_t n, double spacing, double inner,
+                                    double outer, double* sites, size_t capacity, size_t* count) {
+  if (n > 0 && !xyz) return fail(CSX_INVALID_INPUT, "xyz is required");
+  return copy_grid(shell_sites(xyz, n, spacing, inner, outer), sites, capacity, count, nullptr);
+}
+
+csx_status csx_generate_globule(size_t n, double radius, double elongation, uint64_t seed,
+                                double* xyz, uint8_t* element) {
+  if (n > 0 && (!xyz || !element)) return fail(CSX_INVALID_INPUT, "output arrays are required");
+  if (!(radius > 0.0) || !(elongation > 0.0) || !std::isfinite(radius) || !std::isfinite(elongation))
+    return fail(CSX_INVALID_INPUT, "radius and elongation must be finite and positive");
+  generate_globule(n, radius, elongation, seed, xyz, element);
+  return CSX_OK;
+}
+
+csx_status csx_generate_domain_chain(size_t n, int domains, uint64_t seed, double* xyz,
+                                     uint8_t* element) {
+  if (n > 0 && (!xyz || !element)) return fail(CSX_INVALID_INPUT, "output arrays are required");
+  if (domains < 1 || domains > 16) return fail(CSX_INVALID_INPUT, "domains must be in [1, 16]");
+  generate_domain_chain(n, domains, seed, xyz, element);
+  return CSX_OK;
+}
+
+csx_status csx_debye_intensity(csx_structure* s, const double* q, size_t nq, int implicit_h,
+                               int precision, double* out) {
+  CSX_TRY(check_structure(s, q, nq, out));
+  if (!precision_ok(precision)) return fail(CSX_INVALID_INPUT, "unknown precision");
+  if (nq == 0) return CSX_OK;
+  Call call(s->ctx);
+  CSX_TRY(call.begin());
+  PointSet* ps = nullptr;
+  CSX_TRY(get_point_set(s, nullptr, &ps));
+  std::vector<double> V, E, H;
+  type_amplitudes(s, *ps, q, nq, implicit_h != 0, nullptr, V, E, H);
+  if (precision == CSX_FP64_PLAIN) {
+    const std::vector<double>* tabs[3] = {&V, &V, &V};
+    CSX_TRY(plain_f64(call, *ps, q, nq, 1, tabs, out));
+  } else {
+    std::vector<double> P;
+    CSX_TRY(pair_tables_f32(call, *ps, q, nq, P));
+    contract(*ps, P, nq, V, V, out);
+  }
+  for (size_t k = 0; k < nq; ++k) out[k] = std::max(out[k], 0.0);
+  return CSX_OK;
+}
+
+csx_status csx_debye_solvent_basis(csx_structure* s, const double* q, size_t nq,
+                                   const csx_solvent_params* p, int implicit_h, int precision,
+                                   double* out) {
+  CSX_TRY(check_structure(s, q, nq, out));
+  if (!p) return fail(CSX_INVALID_INPUT, "solvent parameters are required");
+  if (!precision_ok(precision)) return fail(CSX_INVALID_INPUT, "unknown precision");
+  if (nq == 0) return CSX_OK;
+  return debye_basis(s, q, nq, p, implicit_h != 0, precision, out);
+}
+
+csx_status csx_solvent_corrected_intensity(csx_structure* s, const double* q, size_t nq,
+                                           const csx_solvent_params* p, int implicit_h,
+                                           int precision, double* out) {
+  CSX_TRY(check_structure(s, q, nq, out));
+  if (!p) return fail(CSX_INVALID_INPUT, "solvent parameters are required");
+  if (!precision_ok(precision)) return fail(CSX_INVALID_INPUT, "unknown precision");
+  if (nq == 0) return CSX_OK;
+  std::vector<double> basis(6 * nq);
+  CSX_TRY(debye_basis(s, q, nq, p, implicit_h != 0, precision, basis.data()));
+  combine(basis.data(), q, nq, p->c1, p->c2, out);
+  return CSX_OK;
+}
+
+csx_status csx_debye_pair_tables(csx_structure* s, const double* q, size_t nq,
+                                 const csx_solvent_params* p, double* out, double* self_terms,
+                                 size_t* n_types) {
+  if (!s) return fail(CSX_INVALID_INPUT, "structure handle is null");
+  if (nq > 0 && !q) return fail(CSX_INVALID_INPUT, "q is required");
+  PointSet* ps = nullptr;
+  {
+    const hipError_t e = hipSetDevice(s->ctx->device);
+    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+  }
+  CSX_TRY(get_point_set(s, p, &ps));
+  if (n_types) *n_types = static_cast<size_t>(ps->ntypes);
+  if (self_terms) std::copy(ps->self.begin(), ps->self.end(), self_terms);
+  if (!out || nq == 0) return CSX_OK;
+  Call call(s->ctx);
+  CSX_TRY(call.begin());
+  std::vector<double> P;
+  CSX_TRY(pair_tables_f32(call, *ps, q, nq, P));
+  std::copy(P.begin(), P.end(), out);
+  return CSX_OK;
+}
+
+csx_status csx_combine_solvent_basis(const double* basis, const double* q, size_t nq, double c1,
+                                     double c2, double* out) {
+  if (nq > 0 && (!basis || !q || !out)) return fail(CSX_INVALID_INPUT, "buffers are required");
+  combine(basis, q, nq, c1, c2, out);
+  return CSX_OK;
+}
+
+csx_status csx_compute_curve(csx_structure* s, const double* q, size_t nq,
+                             const csx_compute_options* o, double* out, int* method_used) {
+  CSX_TRY(check_structure(s, q, nq, out));
+  if (!o) return fail(CSX_INVALID_INPUT, "options are required");
+  for (size_t k = 0; k < nq; ++k)
+    if (!std::isfinite(q[k]) || q[k] < 0.0 || (k > 0 && q[k] < q[k - 1]))
+      return fail(CSX_INVALID_INPUT, "q values must be finite, non-negative, and non-decreasing");
+  const csx_solvent_params* p = o->has_solvent ? &o->solvent : nullptr;
+  if (p) {
+    const bool ok = std::isfinite(p->c1) && p->c1 > 0.0 && std::isfinite(p->c2) &&
+                    std::isfinite(p->shell_thickness_angstrom) &&
+                    p->shell_thickness_angstrom > 0.0 &&
+                    std::isfinite(p->bulk_density_e_per_a3) && p->bulk_density_e_per_a3 > 0.0 &&
+                    std::isfinite(p->hydration_grid_spacing_angstrom) &&
+                    p->hydration_grid_spacing_angstrom > 0.0;
+    if (!ok)
+      return fail(CSX_INVALID_INPUT,
+                  "solvent parameters must be finite with positive dimensions, density, and c1");
+    if (p->hydration_model < 0 || p->hydration_model > 2)
+      return fail(CSX_INVALID_INPUT, "unknown hydration model");
+  }
+  if (o->l_max != 0 && (o->l_max < 8 || o->l_max > 64))
+    return fail(CSX_INVALID_INPUT, "multipole l_max must be between 8 and 64");
+  if (!precision_ok(o->precision)) return fail(CSX_INVALID_INPUT, "unknown precision");
+  int method = o->method;
+  if (method == CSX_METHOD_AUTO) {
+    if (p && p->hydration_model != CSX_HYDRATION_ANALYTIC) method = CSX_METHOD_MULTIPOLE;
+    else if (s->n < 500) method = CSX_METHOD_DEBYE;
+    else method = CSX_METHOD_MULTIPOLE;
+  }
+  if (method != CSX_METHOD_DEBYE && method != CSX_METHOD_MULTIPOLE)
+    return fail(CSX_INVALID_INPUT, "unknown scattering method");
+  if (method_used) *method_used = method;
+  if (method == CSX_METHOD_DEBYE) {
+    if (!p) return csx_debye_intensity(s, q, nq, o->implicit_hydrogen, o->precision, out);
+    return csx_solvent_corrected_intensity(s, q, nq, p, o->implicit_hydrogen, o->precision, out);
+  }
+  double qmax = 0.0;
+  for (size_t k = 0; k < nq; ++k) qmax = std::max(qmax, q[k]);
+  const double order = std::ceil(qmax * s->max_radius);
+  const int l_max = o->l_max != 0 ? o->l_max
+                                  : static_cast<int>(std::min(std::max(order, 0.0) + 8.0, 64.0) < 8.0
+                                                         ? 8.0
+                                                         : std::min(std::max(order, 0.0) + 8.0, 64.0));
+  return multipole_dispatch(s, q, nq, l_max, o->implicit_hydrogen != 0, p, o->precision, out);
+}
+
+}  // extern "C"

@@ -1,0 +1,301 @@
+// Debye pair-sum kernels for gfx950 (CDNA4).
+//
+//  debye_plain_f64   FP64 restatement of the reference pair loop
+//                    (scattering.rs:100-106, solvent.rs:533-558): per-atom
+//                    amplitudes, r_ij, sin and division for every pair and q.
+//                    One workgroup per (q, 256-row block); rows i of the block
+//                    sweep all j > i through LDS tiles.  This is the FP64
+//                    baseline the fast path is measured against.
+//  debye_pairs_f32   Per-type pair tables.  A work item pairs a type-pure block
+//                    of 256 "i" atoms (one per lane) with a chunk of "j" atoms
+//                    of one type; r_ij is computed once per pair and swept over
+//                    QT q values held in scalar registers.  The term
+//                    occ_j sin(q r)/r is accumulated in FP32 per LDS tile
+//                    (<= 256 pairs per lane), folded into FP64 per tile, and
+//                    the workgroup writes one FP64 partial per (item, q); a
+//                    second kernel sums partials per type bin in a fixed order,
+//                    so results are bitwise reproducible run to run.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "kernels.hpp"
+
+namespace csx {
+namespace {
+
+constexpr int kBlock = 256;
+
+// Fixed-order FP64 sum over a 256-thread workgroup; valid in thread 0.
+template <int NV>
+__device__ inline void block_sum_f64(double (&v)[NV], double* scratch /* [4][NV] */) {
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+    double x = v[k];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
+    v[k] = x;
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) scratch[wave * NV + k] = v[k];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k)
+      v[k] = ((scratch[k] + scratch[NV + k]) + scratch[2 * NV + k]) + scratch[3 * NV + k];
+  }
+}
+
+__device__ inline double sinc64(double x) { return fabs(x) < 1e-8 ? 1.0 : sin(x) / x; }
+
+// ---------------------------------------------------------------- FP64 --
+// NCH amplitude channels per point (1: vacuum f; 3: v, e, h), NT output
+// terms (1: I; 6: vv, ee, hh, ve, vh, eh).  amp is [NCH][nq][npts].
+template <int NCH, int NT>
+__global__ __launch_bounds__(kBlock) void debye_plain_f64(const double4* __restrict__ pts,
+                                                          const double* __restrict__ amp,
+                                                          const double* __restrict__ qv, int npts,
+                                                          int nq, double* __restrict__ partial) {
+  __shared__ double4 tile_p[kBlock];
+  __shared__ double tile_a[NCH][kBlock];
+  __shared__ double scratch[4 * NT];
+  const int blk = blockIdx.x, qi = blockIdx.y;
+  const int i = blk * kBlock + threadIdx.x;
+  const double q = qv[qi];
+  const bool vi = i < npts;
+  const double4 pi = vi ? pts[i] : make_double4(0, 0, 0, 0);
+  double ai[NCH];
+#pragma unroll
+  for (int c = 0; c < NCH; ++c) ai[c] = vi ? amp[(size_t(c) * nq + qi) * npts + i] : 0.0;
+  double acc[NT];
+  if (NT == 1) {
+    acc[0] = ai[0] * ai[0];
+  } else {
+    acc[0] = ai[0] * ai[0];
+    acc[1] = ai[1] * ai[1];
+    acc[2] = ai[2] * ai[2];
+    acc[3] = ai[0] * ai[1];
+    acc[4] = ai[0] * ai[2];
+    acc[5] = ai[1] * ai[2];
+  }
+  for (int t0 = blk * kBlock; t0 < npts; t0 += kBlock) {
+    __syncthreads();
+    const int j = t0 + threadIdx.x;
+    tile_p[threadIdx.x] = j < npts ? pts[j] : make_double4(0, 0, 0, 0);
+#pragma unroll
+    for (int c = 0; c < NCH; ++c)
+      tile_a[c][threadIdx.x] = j < npts ? amp[(size_t(c) * nq + qi) * npts + j] : 0.0;
+    __syncthreads();
+    const int jn = min(kBlock, npts - t0);
+    for (int t = 0; t < jn; ++t) {
+      if (!vi || t0 + t <= i) continue;
+      const double4 pj = tile_p[t];
+      const double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+      const double r = sqrt(dx * dx + dy * dy + dz * dz);
+      const double s = sinc64(q * r);
+      if (NT == 1) {
+        acc[0] += 2.0 * ai[0] * tile_a[0][t] * s;
+      } else {
+        const double vj = tile_a[0][t], ej = tile_a[1][t], hj = tile_a[NCH - 1][t];
+        acc[0] += 2.0 * ai[0] * vj * s;
+        acc[1] += 2.0 * ai[1] * ej * s;
+        acc[2] += 2.0 * ai[2] * hj * s;
+        acc[3] += (ai[0] * ej + vj * ai[1]) * s;
+        acc[4] += (ai[0] * hj + vj * ai[2]) * s;
+        acc[5] += (ai[1] * hj + ej * ai[2]) * s;
+      }
+    }
+  }
+  block_sum_f64<NT>(acc, scratch);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < NT; ++k) partial[(size_t(qi) * gridDim.x + blk) * NT + k] = acc[k];
+  }
+}
+
+// out[k*nq + qi] = sum over blocks of partial[qi][blk][k], fixed order.
+__global__ void reduce_plain_partials(const double* __restrict__ partial, int nblk, int nq, int nt,
+                                      double* __restrict__ out) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= nq * nt) return;
+  const int qi = idx / nt, k = idx % nt;
+  double s = 0.0;
+  for (int b = 0; b < nblk; ++b) s += partial[(size_t(qi) * nblk + b) * nt + k];
+  out[size_t(k) * nq + qi] = s;
+}
+
+// amp[c][qi][i] = occ_i * table[c][type_i][qi].
+__global__ void expand_amplitudes(const double* __restrict__ table, const int* __restrict__ type,
+                                  const double* __restrict__ occ, int npts, int ntypes, int nq,
+                                  int nch, double* __restrict__ amp) {
+  const size_t total = size_t(nch) * nq * npts;
+  for (size_t idx = blockIdx.x * size_t(blockDim.x) + threadIdx.x; idx < total;
+       idx += size_t(gridDim.x) * blockDim.x) {
+    const int i = int(idx % npts);
+    const size_t cq = idx / npts;
+    const int qi = int(cq % nq), c = int(cq / nq);
+    amp[idx] = occ[i] * table[(size_t(c) * ntypes + type[i]) * nq + qi];
+  }
+}
+
+// ---------------------------------------------------------------- FP32 --
+
+// sin(2 pi u) with an exact reduction in revolutions: g = u - rint(u) lies in
+// [-1/2, 1/2] and is exact (|u| < 2^22), so only the product q*r before it is
+// rounded; then an odd degree-13 polynomial (max error 1.2e-9 in exact
+// arithmetic, fitted to sin(2 pi g) on [-1/2, 1/2]).
+__device__ inline float sin2pi_soft(float u) {
+  const float g = u - __builtin_rintf(u);
+  const float g2 = g * g;
+  float p = 3.1996936798095703f;
+  p = __builtin_fmaf(p, g2, -14.868611335754395f);
+  p = __builtin_fmaf(p, g2, 42.01616668701172f);
+  p = __builtin_fmaf(p, g2, -76.70155334472656f);
+  p = __builtin_fmaf(p, g2, 81.60502624511719f);
+  p = __builtin_fmaf(p, g2, -41.341697692871094f);
+  p = __builtin_fmaf(p, g2, 6.2831854820251465f);
+  return g * p;
+}
+
+// Hardware V_SIN_F32 (argument in revolutions) after the same exact
+// reduction.
+__device__ inline float sin2pi_hw(float u) {
+  const float g = u - __builtin_rintf(u);
+  return __builtin_amdgcn_sinf(g);
+}
+
+template <int QT, int SINMODE>
+__global__ __launch_bounds__(kBlock) void debye_pairs_f32(const float4* __restrict__ pts,
+                                                          const int4* __restrict__ items,
+                                                          const float* __restrict__ qrev,
+                                                          int nq_pad,
+                                                          double* __restrict__ partial) {
+  __shared__ float4 tile[kBlock];
+  __shared__ double scratch[4 * QT];
+  const int4 item = items[blockIdx.x];  // i0, ilen, j0, jlen
+  const int qc = blockIdx.y * QT;
+  const int i = item.x + threadIdx.x;
+  const bool vi = threadIdx.x < item.y;
+  const float4 pi = vi ? pts[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+  float qk[QT];
+#pragma unroll
+  for (int k = 0; k < QT; ++k) qk[k] = qrev[qc + k];
+  float acc[QT];
+  double acc64[QT];
+#pragma unroll
+  for (int k = 0; k < QT; ++k) {
+    acc[k] = 0.f;
+    acc64[k] = 0.0;
+  }
+  const int jend = item.z + item.w;
+  for (int t0 = item.z; t0 < jend; t0 += kBlock) {
+    __syncthreads();
+    const int j = t0 + threadIdx.x;
+    tile[threadIdx.x] = j < jend ? pts[j] : make_float4(0.f, 0.f, 0.f, 0.f);
+    __syncthreads();
+    const int jn = min(kBlock, jend - t0);
+    for (int t = 0; t < jn; ++t) {
+      const float4 pj = tile[t];
+      const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+      const float d2 = fmaxf(__builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz)), 1e-30f);
+      const float r = __builtin_sqrtf(d2);
+      const float w = (t0 + t > i) ? pj.w * __builtin_amdgcn_rcpf(r) : 0.f;
+#pragma unroll
+      for (int k = 0; k < QT; ++k) {
+        const float u = qk[k] * r;
+        const float s = SINMODE == 0 ? sin2pi_soft(u) : sin2pi_hw(u);
+        acc[k] = __builtin_fmaf(w, s, acc[k]);
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < QT; ++k) {
+      acc64[k] += double(acc[k]);
+      acc[k] = 0.f;
+    }
+  }
+  const double wi = double(pi.w);
+#pragma unroll
+  for (int k = 0; k < QT; ++k) acc64[k] *= wi;
+  block_sum_f64<QT>(acc64, scratch);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < QT; ++k) partial[size_t(blockIdx.x) * nq_pad + qc + k] = acc64[k];
+  }
+}
+
+// out[b][q] = sum_{items of bin b, in order} partial[item][q].
+__global__ void reduce_bins(const double* __restrict__ partial, const int* __restrict__ bin_first,
+                            int nq_pad, double* __restrict__ out) {
+  const int b = blockIdx.y;
+  const int qi = blockIdx.x * blockDim.x + threadIdx.x;
+  if (qi >= nq_pad) return;
+  double s = 0.0;
+  for (int it = bin_first[b]; it < bin_first[b + 1]; ++it) s += partial[size_t(it) * nq_pad + qi];
+  out[size_t(b) * nq_pad + qi] = s;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------- launchers --
+
+hipError_t launch_debye_plain(const double4* pts, const double* amp, const double* q, int npts,
+                              int nq, int nch, int nterms, double* partial, double* out,
+                              hipStream_t stream, KernelTimer* timer) {
+  const int nblk = (npts + kBlock - 1) / kBlock;
+  dim3 grid(nblk, nq);
+  TimerScope scope(timer, "debye_plain_f64", stream);
+  if (nch == 1 && nterms == 1)
+    debye_plain_f64<1, 1><<<grid, kBlock, 0, stream>>>(pts, amp, q, npts, nq, partial);
+  else if (nch == 3 && nterms == 6)
+    debye_plain_f64<3, 6><<<grid, kBlock, 0, stream>>>(pts, amp, q, npts, nq, partial);
+  else
+    return hipErrorInvalidValue;
+  scope.stop();
+  const int total = nq * nterms;
+  reduce_plain_partials<<<(total + 255) / 256, 256, 0, stream>>>(partial, nblk, nq, nterms, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_expand_amplitudes(const double* table, const int* type, const double* occ,
+                                    int npts, int ntypes, int nq, int nch, double* amp,
+                                    hipStream_t stream) {
+  const size_t total = size_t(nch) * nq * npts;
+  const int blocks = int(std::min<size_t>((total + 255) / 256, 65536));
+  expand_amplitudes<<<blocks, 256, 0, stream>>>(table, type, occ, npts, ntypes, nq, nch, amp);
+  return hipGetLastError();
+}
+
+template <int QT, int SM>
+static void launch_pairs_t(dim3 grid, const float4* pts, const int4* items, const float* qrev,
+                           int nq_pad, double* partial, hipStream_t stream) {
+  debye_pairs_f32<QT, SM><<<grid, kBlock, 0, stream>>>(pts, items, qrev, nq_pad, partial);
+}
+
+hipError_t launch_debye_pairs(const float4* pts, const int4* items, int nitems, const int* bin_first,
+                              int nbins, const float* qrev, int nq_pad, int qt, int sin_mode,
+                              double* partial, double* out, hipStream_t stream,
+                              KernelTimer* timer) {
+  if (nq_pad % qt != 0) return hipErrorInvalidValue;
+  dim3 grid(nitems, nq_pad / qt);
+  {
+    TimerScope scope(timer, "debye_pairs_f32", stream);
+    if (nitems > 0) {
+      if (qt == 8 && sin_mode == 0) launch_pairs_t<8, 0>(grid, pts, items, qrev, nq_pad, partial, stream);
+      else if (qt == 8 && sin_mode == 1) launch_pairs_t<8, 1>(grid, pts, items, qrev, nq_pad, partial, stream);
+      else if (qt == 16 && sin_mode == 0) launch_pairs_t<16, 0>(grid, pts, items, qrev, nq_pad, partial, stream);
+      else if (qt == 16 && sin_mode == 1) launch_pairs_t<16, 1>(grid, pts, items, qrev, nq_pad, partial, stream);
+      else if (qt == 32 && sin_mode == 0) launch_pairs_t<32, 0>(grid, pts, items, qrev, nq_pad, partial, stream);
+      else if (qt == 32 && sin_mode == 1) launch_pairs_t<32, 1>(grid, pts, items, qrev, nq_pad, partial, stream);
+      else return hipErrorInvalidValue;
+    }
+    scope.stop();
+  }
+  dim3 rgrid((nq_pad + 255) / 256, nbins);
+  reduce_bins<<<rgrid, 256, 0, stream>>>(partial, bin_first, nq_pad, out);
+  return hipGetLastError();
+}
+
+}  // namespace csx

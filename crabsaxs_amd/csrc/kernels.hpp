@@ -1,0 +1,65 @@
+// Internal interface between the C-ABI host code and the HIP kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <utility>
+#include <vector>
+
+namespace csx {
+
+// Per-call collection of kernel durations measured with HIP events recorded
+// on the launch stream.  Events are read after the call's stream has been
+// synchronised, so timing adds no synchronisation of its own.
+struct KernelTimer {
+  struct Pending {
+    std::string name;
+    hipEvent_t start, stop;
+  };
+  std::vector<Pending> pending;
+  ~KernelTimer() {
+    for (auto& p : pending) {
+      (void)hipEventDestroy(p.start);
+      (void)hipEventDestroy(p.stop);
+    }
+  }
+};
+
+class TimerScope {
+ public:
+  TimerScope(KernelTimer* t, const char* name, hipStream_t s) : timer_(t), stream_(s) {
+    if (!timer_) return;
+    KernelTimer::Pending p{name, nullptr, nullptr};
+    if (hipEventCreate(&p.start) != hipSuccess || hipEventCreate(&p.stop) != hipSuccess) {
+      timer_ = nullptr;
+      return;
+    }
+    (void)hipEventRecord(p.start, stream_);
+    timer_->pending.push_back(p);
+    index_ = timer_->pending.size() - 1;
+  }
+  void stop() {
+    if (timer_ && !stopped_) (void)hipEventRecord(timer_->pending[index_].stop, stream_);
+    stopped_ = true;
+  }
+  ~TimerScope() { stop(); }
+
+ private:
+  KernelTimer* timer_;
+  hipStream_t stream_;
+  size_t index_ = 0;
+  bool stopped_ = false;
+};
+
+hipError_t launch_debye_plain(const double4* pts, const double* amp, const double* q, int npts,
+                              int nq, int nch, int nterms, double* partial, double* out,
+                              hipStream_t stream, KernelTimer* timer);
+hipError_t launch_expand_amplitudes(const double* table, const int* type, const double* occ,
+                                    int npts, int ntypes, int nq, int nch, double* amp,
+                                    hipStream_t stream);
+hipError_t launch_debye_pairs(const float4* pts, const int4* items, int nitems, const int* bin_first,
+                              int nbins, const float* qrev, int nq_pad, int qt, int sin_mode,
+                              double* partial, double* out, hipStream_t stream, KernelTimer* timer);
+
+}  // namespace csx

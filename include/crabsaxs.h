@@ -1,0 +1,253 @@
+/*
+ * crabsaxs.h -- C ABI of the MI355X-native crabSAXS profile engine.
+ *
+ * The reference (crabSAXS, Rust) has no FFI layer: its seam is the set of
+ * `pub` functions of src/scattering.rs, src/solvent.rs and src/spherical.rs
+ * that the fitting and analysis layers call.  Each compute entry point below
+ * replaces one of them; the comment on each names the function it replaces
+ * (file:line in the reference).  A Rust `-sys` crate binds these 1:1 (the
+ * bindings are given in INTEGRATION.md).
+ *
+ * Conventions
+ *   - Every function returns a csx_status; on failure a message is kept in a
+ *     thread-local buffer, read with csx_last_error().
+ *   - Inputs are borrowed (const pointers), outputs are caller-owned buffers
+ *     of the documented size.  The library never hands out memory it
+ *     allocated except through opaque handles released by *_destroy.
+ *   - Status codes mirror `SaxsError` (src/error.rs:4-25).
+ *   - Intensities are clamped at >= 0 exactly where the reference clamps;
+ *     basis terms and pair tables are not clamped.
+ *   - Functions are reentrant: handles may be shared between host threads
+ *     (the reference calls its seam from rayon workers, ensemble.rs:244-256).
+ *   - Coordinates are Angstrom, q is 1/Angstrom.
+ */
+#ifndef CRABSAXS_H
+#define CRABSAXS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    CSX_OK = 0,
+    CSX_INVALID_INPUT = 1,    /* SaxsError::InvalidInput */
+    CSX_NOT_CONVERGED = 2,    /* SaxsError::FitDidNotConverge */
+    CSX_NO_DEVICE = 3,        /* no usable HIP device */
+    CSX_DEVICE_ERROR = 4,     /* a HIP runtime call failed */
+    CSX_OUT_OF_MEMORY = 5     /* device or host allocation failed */
+} csx_status;
+
+/* Pair-sum arithmetic. */
+typedef enum {
+    /* FP64 per-atom form factors, r_ij, sin and division per pair and q:
+     * the reference loop of scattering.rs:100-106 run on the GPU. */
+    CSX_FP64_PLAIN = 0,
+    /* FP32 pair terms swept over all q for each r_ij, cascade-compensated
+     * (FP32 inside a tile of <= 256 pairs, FP64 across tiles and blocks),
+     * accumulated into per-type tables S_ab(q). */
+    CSX_FP32_COMPENSATED = 1
+} csx_precision;
+
+/* HydrationModel, src/solvent.rs:17-25. */
+typedef enum {
+    CSX_HYDRATION_ANALYTIC = 0,
+    CSX_HYDRATION_GRID = 1,
+    CSX_HYDRATION_ADAPTIVE_GRID = 2
+} csx_hydration_model;
+
+/* ScatteringMethod, src/scattering.rs:16-21. */
+typedef enum {
+    CSX_METHOD_AUTO = 0,
+    CSX_METHOD_DEBYE = 1,
+    CSX_METHOD_MULTIPOLE = 2
+} csx_method;
+
+/* SolventParams, src/solvent.rs:27-50 (defaults: csx_solvent_default). */
+typedef struct {
+    double c1;
+    double c2;
+    double shell_thickness_angstrom;
+    double bulk_density_e_per_a3;
+    int32_t hydration_model;               /* csx_hydration_model */
+    double hydration_grid_spacing_angstrom;
+} csx_solvent_params;
+
+/* ComputeOptions, src/scattering.rs:28-47.  l_max = 0 means "not given". */
+typedef struct {
+    int32_t method;           /* csx_method */
+    int32_t implicit_hydrogen;/* HydrogenMode::Implicit = 1, Explicit = 0 */
+    int32_t has_solvent;      /* Option<SolventParams>::is_some */
+    csx_solvent_params solvent;
+    int32_t l_max;            /* MultipoleOptions::l_max; 0 = None */
+    int32_t precision;        /* csx_precision of the pair sums */
+} csx_compute_options;
+
+typedef struct csx_ctx csx_ctx;
+typedef struct csx_structure csx_structure;
+
+/* ---- library / context ------------------------------------------------ */
+
+/* Message of the last failed call on this thread ("" if none). */
+const char *csx_last_error(void);
+
+/* SolventParams::default() (solvent.rs:39-50). */
+void csx_solvent_default(csx_solvent_params *out);
+
+/* ComputeOptions::default() (scattering.rs:37-47) with FP32_COMPENSATED. */
+void csx_compute_options_default(csx_compute_options *out);
+
+/* Number of visible HIP devices (0 when none; never fails). */
+int csx_device_count(void);
+
+/* Create a context bound to one HIP device.  CSX_NO_DEVICE when absent. */
+csx_status csx_ctx_create(int device, csx_ctx **out);
+void csx_ctx_destroy(csx_ctx *ctx);
+
+/* Kernel timing: when enabled, the library brackets every launch of the
+ * named kernels with HIP events on the stream they run on and accumulates
+ * their durations.  `name` is one of "debye_pairs_f32", "debye_plain_f64",
+ * "multipole_coeff", "debye_adjoint_f32", "ensemble_pairs_f32". */
+csx_status csx_ctx_set_timing(csx_ctx *ctx, int enabled);
+csx_status csx_ctx_kernel_time(csx_ctx *ctx, const char *name,
+                               int64_t *launches, double *total_ms);
+csx_status csx_ctx_reset_timing(csx_ctx *ctx);
+
+/* Launch shape of the FP32 pair kernel (no reference counterpart; chosen by
+ * the autotuner): q_tile in {8, 16, 32} q values per pair sweep, sin_mode 0
+ * (software polynomial) or 1 (hardware V_SIN_F32), j_chunk atoms per work
+ * item (multiple of 256). */
+csx_status csx_ctx_set_tuning(csx_ctx *ctx, int q_tile, int sin_mode,
+                              int j_chunk);
+
+/* ---- host typing (src/formfactor.rs) ---------------------------------- */
+
+/* Element::from_symbol (src/structure.rs:109-127): CSX_EL_* code. */
+int csx_element_from_symbol(const char *symbol);
+
+/* united_atom_group (src/formfactor.rs:357-417): CSX_UA_* or -1. */
+int csx_united_atom_group(const char *residue_name, const char *atom_name);
+
+/* form_factor_for_atom (src/formfactor.rs:425-439) for one atom. */
+double csx_form_factor(int element, int group, double occupancy,
+                       int implicit_hydrogen, double q);
+
+/* excluded_volume_for_atom (src/solvent.rs:428-438). */
+double csx_excluded_volume(int element, int group, int implicit_hydrogen);
+
+/* water_form_factor (src/solvent.rs:227-231). */
+double csx_water_form_factor(double q);
+
+/* excluded_volume_scale (src/solvent.rs:448-452). */
+double csx_excluded_volume_scale(double q, double radius_scale);
+
+/* ---- structures --------------------------------------------------------- */
+
+/* Upload one structure (SoA).  xyz: n*3 doubles; element: CSX_EL_*;
+ * group: CSX_UA_* or -1 (from csx_united_atom_group); occupancy: n doubles.
+ * The structure keeps an FP64 and a type-sorted FP32 copy in HBM.
+ * Replaces the borrowed `&Structure` of every seam function. */
+csx_status csx_structure_create(csx_ctx *ctx, size_t n, const double *xyz,
+                                const uint8_t *element, const int8_t *group,
+                                const double *occupancy,
+                                csx_structure **out);
+void csx_structure_destroy(csx_structure *s);
+size_t csx_structure_atom_count(const csx_structure *s);
+
+/* ---- hydration grids (src/solvent.rs:121-401), host side --------------- */
+
+/* build_adaptive_hydration_grid (solvent.rs:121-223) with
+ * AdaptiveHydrationOptions{spacing, clearance, padding}.
+ * Writes at most `capacity` sites (3 doubles each) and the true count to
+ * *count; call with capacity 0 to size the buffer. */
+csx_status csx_build_adaptive_hydration_grid(
+    const double *xyz, size_t n, double spacing, double clearance,
+    double padding, double *sites, size_t capacity, size_t *count,
+    double *voxel_volume);
+
+/* build_hydration_grid (solvent.rs:259-401) with SolventGridOptions. */
+csx_status csx_build_hydration_grid(
+    const double *xyz, const uint8_t *element, size_t n, double spacing,
+    double probe_radius, double shell_thickness, double *sites,
+    size_t capacity, size_t *count, double *voxel_volume);
+
+/* ---- Debye path ---------------------------------------------------------- */
+
+/* debye_intensity_with_hydrogens (src/scattering.rs:86-110).  out: nq. */
+csx_status csx_debye_intensity(csx_structure *s, const double *q, size_t nq,
+                               int implicit_hydrogen, int precision,
+                               double *out);
+
+/* debye_solvent_basis (src/solvent.rs:504-584).  out: 6*nq, rows vv, ee,
+ * hh, ve, vh, eh (term-major, like the reference's [Vec<f64>; 6]). */
+csx_status csx_debye_solvent_basis(csx_structure *s, const double *q,
+                                   size_t nq, const csx_solvent_params *p,
+                                   int implicit_hydrogen, int precision,
+                                   double *out);
+
+/* solvent_corrected_intensity (src/solvent.rs:490-500).  out: nq. */
+csx_status csx_solvent_corrected_intensity(csx_structure *s, const double *q,
+                                           size_t nq,
+                                           const csx_solvent_params *p,
+                                           int implicit_hydrogen,
+                                           int precision, double *out);
+
+/* Per-type pair tables S_ab(q) (no reference counterpart; the reusable
+ * half of debye_solvent_basis, solvent.rs:533-577).  Atoms are typed by
+ * (element, united-atom group); with a Grid/AdaptiveGrid solvent model the
+ * hydration sites of that model form one extra, last type (occupancy 1).
+ * For the T types, out[b*nq + k] =
+ *   sum_{i<j, type(i)=a, type(j)=c} occ_i occ_j sinc(q_k r_ij)
+ * for bin b of (a <= c) in row-major upper-triangular order (pairs ordered
+ * by type), computed with CSX_FP32_COMPENSATED arithmetic, and
+ * self_terms[a] = sum_{type(i)=a} occ_i^2.  *n_types receives T.
+ * Sizes: out >= nq*T*(T+1)/2, self_terms >= T.  p may be NULL (atoms only);
+ * pass out = NULL to query T. */
+csx_status csx_debye_pair_tables(csx_structure *s, const double *q,
+                                 size_t nq, const csx_solvent_params *p,
+                                 double *out, double *self_terms,
+                                 size_t *n_types);
+
+/* combine_solvent_basis (src/spherical.rs:567-584).  basis: 6*nq. */
+csx_status csx_combine_solvent_basis(const double *basis, const double *q,
+                                     size_t nq, double c1, double c2,
+                                     double *out);
+
+/* compute_curve (src/scattering.rs:112-212): validation, Auto method
+ * choice, adaptive l_max and dispatch.  out: nq; *method_used receives
+ * CSX_METHOD_DEBYE or CSX_METHOD_MULTIPOLE. */
+csx_status csx_compute_curve(csx_structure *s, const double *q, size_t nq,
+                             const csx_compute_options *options, double *out,
+                             int *method_used);
+
+/* ---- synthetic workloads (no reference counterpart) -------------------- */
+
+/* Seeded synthetic globular protein: self-avoiding C-alpha random walk with
+ * 3.8 A steps (3.0 A exclusion) confined to an ellipsoid with semi-axes
+ * (radius, radius, radius*elongation), 7 further atoms per residue at
+ * 1.2-1.8 A from their C-alpha, elements drawn C 0.63 / N 0.17 / O 0.19 /
+ * S 0.01, occupancy 1, centred on the centroid.  Deterministic for a seed
+ * on every platform.  xyz: 3n, element: n. */
+csx_status csx_generate_globule(size_t n, double radius, double elongation,
+                                uint64_t seed, double *xyz,
+                                uint8_t *element);
+
+/* Multi-domain chain: `domains` globules (2-4) joined by flexible linkers of
+ * C-alpha random walk; same atom placement and element rules. */
+csx_status csx_generate_domain_chain(size_t n, int domains, uint64_t seed,
+                                     double *xyz, uint8_t *element);
+
+/* Hydration pseudo-atoms: grid points of spacing `spacing` whose nearest
+ * atom lies in (inner, outer].  Same sizing protocol as the grid builders. */
+csx_status csx_generate_shell_sites(const double *xyz, size_t n,
+                                    double spacing, double inner,
+                                    double outer, double *sites,
+                                    size_t capacity, size_t *count);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CRABSAXS_H */

@@ -1,0 +1,33 @@
+import os
+import subprocess
+import sys
+from pathlib import Path
+
+import pytest
+
+ROOT = Path(__file__).resolve().parents[1]
+sys.path.insert(0, str(ROOT))
+sys.path.insert(0, str(ROOT / "tests"))
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (HIP device)")
+    config.addinivalue_line("markers", "slow: long-running CPU test")
+
+
+@pytest.fixture(scope="session", autouse=True)
+def _built_artifacts():
+    """Build the oracle (and, in a container with hipcc, the library) when missing."""
+    lib = ROOT / "crabsaxs_amd" / "libcrabsaxs_hip.so"
+    if not lib.exists() and os.path.exists("/opt/rocm/bin/hipcc"):
+        subprocess.run([sys.executable, "-m", "crabsaxs_amd.build"], cwd=ROOT, check=True)
+    oracle = ROOT / "oracle" / "liboracle.so"
+    if not oracle.exists():
+        subprocess.run(["make", "-s", "-C", str(ROOT / "oracle")], check=True)
+    yield
+
+
+@pytest.fixture(scope="session")
+def gpu_ctx():
+    import crabsaxs_amd as csx
+    return csx.default_context()

@@ -1,0 +1,200 @@
+"""GPU parity of the Debye path (through the C ABI) against the FP64 oracle.
+
+Bars: FP64_PLAIN within 1e-12 relative of the oracle; FP32_COMPENSATED within
+1e-5 maximum relative error per q (the north-star gate), both against the
+oracle at oracle-feasible sizes and against the FP64 GPU kernel at the full
+20,000-atom, 500-q precision-study size.
+"""
+import json
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import crabsaxs_amd as csx
+import oracle_py as O
+
+pytestmark = pytest.mark.gpu
+GOLD = json.loads((Path(__file__).parent / "golden" / "debye_golden.json").read_text())
+F32 = csx.Precision.FP32_COMPENSATED
+F64 = csx.Precision.FP64_PLAIN
+
+
+def rel(a, b):
+    return np.abs(a - b) / np.maximum(np.abs(b), 1e-300)
+
+
+def structure(name):
+    s = GOLD["structures"][name]
+    return csx.Structure.from_arrays(s["xyz"], s["element"])
+
+
+def oracle_atoms(s):
+    return O.Atoms(s.xyz, s.element, s.group, s.occupancy)
+
+
+@pytest.mark.parametrize("name", sorted(GOLD["structures"]))
+def test_debye_golden_vectors(gpu_ctx, name):
+    s = structure(name)
+    for tag in ("low", "high"):
+        q = np.array(GOLD[f"q_{tag}"])
+        want = np.array(GOLD["structures"][name][f"debye_{tag}"])
+        got64 = csx.debye_intensity(s, q, precision=F64)
+        got32 = csx.debye_intensity(s, q, precision=F32)
+        print(name, tag, rel(got64, want).max(), rel(got32, want).max(), flush=True)
+        assert rel(got64, want).max() < 1e-12
+        assert rel(got32, want).max() < 1e-5
+        basis = np.array(GOLD["structures"][name][f"basis_{tag}"])
+        b64 = csx.debye_solvent_basis(s, q, csx.SolventParams(), precision=F64)
+        b32 = csx.debye_solvent_basis(s, q, csx.SolventParams(), precision=F32)
+        scale = np.abs(basis).max(axis=1, keepdims=True)
+        assert (np.abs(b64 - basis) / scale).max() < 1e-12
+        assert (np.abs(b32 - basis) / scale).max() < 1e-5
+
+
+def test_two_atom_closed_form(gpu_ctx):
+    # scattering.rs:250-257
+    s = csx.Structure([csx.Atom(pos=(0, 0, 0), residue_name="TST"),
+                       csx.Atom(pos=(2, 0, 0), residue_name="TST", atom_name="C2")])
+    f = csx.form_factor_for_atom(csx.Atom(pos=(0, 0, 0), residue_name="TST"), True, 0.2)
+    want = 2 * f * f * (1 + np.sin(0.4) / 0.4)
+    for p in (F64, F32):
+        got = csx.debye_intensity(s, [0.2], precision=p)[0]
+        assert abs(got - want) < 1e-10 * want if p == F64 else abs(got - want) < 1e-6 * want
+
+
+@pytest.mark.parametrize("seed", [11, 12])
+def test_globule_1200_matches_oracle(gpu_ctx, seed):
+    """1,200-atom globule (R = 16 A), q = 0.005..0.5 (100 points)."""
+    s = csx.generate_globule(1200, 16.0, 1.0, seed=seed)
+    q = np.arange(1, 101) * 0.005
+    want = O.debye_intensity(oracle_atoms(s), q)
+    got64 = csx.debye_intensity(s, q, precision=F64)
+    got32 = csx.debye_intensity(s, q, precision=F32)
+    print("1200 f64", rel(got64, want).max(), "f32", rel(got32, want).max(), flush=True)
+    assert rel(got64, want).max() < 1e-12
+    assert rel(got32, want).max() < 1e-5
+
+
+@pytest.mark.parametrize("model", [csx.HydrationModel.Analytic, csx.HydrationModel.AdaptiveGrid,
+                                   csx.HydrationModel.Grid])
+def test_solvent_basis_matches_oracle(gpu_ctx, model):
+    s = csx.generate_globule(400, 12.0, 1.0, seed=4)
+    q = np.linspace(0.0, 0.5, 41)
+    p = csx.SolventParams(c1=1.03, c2=0.4, hydration_model=model,
+                          hydration_grid_spacing_angstrom=4.0)
+    sites, vol = None, 0.0
+    if model == csx.HydrationModel.AdaptiveGrid:
+        sites, vol = O.adaptive_grid(s.xyz, spacing=4.0)
+    elif model == csx.HydrationModel.Grid:
+        sites, vol = O.exterior_grid(s.xyz, s.element, spacing=1.0, shell=3.0)
+    want = O.debye_solvent_basis(oracle_atoms(s), q, c1=1.03, c2=0.4, sites=sites,
+                                 voxel_volume=vol)
+    scale = np.abs(want).max(axis=1, keepdims=True)
+    for prec, tol in ((F64, 1e-12), (F32, 1e-5)):
+        got = csx.debye_solvent_basis(s, q, p, precision=prec)
+        err = (np.abs(got - want) / scale).max()
+        print(model.name, prec.name, err, flush=True)
+        assert err < tol
+        inten = csx.solvent_corrected_intensity(s, q, p, precision=prec)
+        ref = O.combine_solvent_basis(want, q, 1.03, 0.4)
+        assert rel(inten, ref).max() < tol * 10
+
+
+def test_pair_tables_recontract(gpu_ctx):
+    """S_ab tables re-contract to the vacuum intensity with the type amplitudes."""
+    s = csx.generate_globule(600, 12.0, 1.0, seed=9)
+    q = np.linspace(0, 0.8, 33)
+    S, self_terms = csx.debye_pair_tables(s, q)
+    types = sorted(set(int(e) for e in s.element))
+    f = np.array([[csx.form_factor_for_atom(csx.Atom(pos=(0, 0, 0), element=csx.Element(t)),
+                                            True, qq) for qq in q] for t in types])
+    T = len(types)
+    inten = (f ** 2 * self_terms[:, None]).sum(axis=0)
+    b = 0
+    for a in range(T):
+        for c in range(a, T):
+            inten = inten + 2 * f[a] * f[c] * S[b]
+            b += 1
+    want = csx.debye_intensity(s, q, precision=F64)
+    assert rel(inten, want).max() < 1e-5
+    # q = 0 row is exact: sum_{i<j} occ_i occ_j per bin
+    counts = np.array([np.sum(s.element == t) for t in types], dtype=float)
+    b = 0
+    for a in range(T):
+        for c in range(a, T):
+            exp0 = counts[a] * (counts[a] - 1) / 2 if a == c else counts[a] * counts[c]
+            assert S[b, 0] == exp0
+            b += 1
+
+
+def test_edge_cases(gpu_ctx):
+    empty = csx.Structure.from_arrays(np.zeros((0, 3)), np.zeros(0, dtype=np.uint8))
+    q = np.array([0.0, 0.1, 0.2])
+    for p in (F64, F32):
+        assert np.array_equal(csx.debye_intensity(empty, q, precision=p), np.zeros(3))
+    one = csx.Structure.from_arrays([[1.0, 2.0, 3.0]], [1])
+    f = [csx.form_factor_for_atom(csx.Atom(pos=(0, 0, 0)), True, qq) for qq in q]
+    for p in (F64, F32):
+        assert np.allclose(csx.debye_intensity(one, q, precision=p), np.square(f), rtol=1e-12)
+    # coincident atoms: sinc(0) = 1 for the pair
+    dup = csx.Structure.from_arrays([[0.5, 0.5, 0.5]] * 3 + [[4.0, 0.0, 0.0]], [1, 1, 3, 2])
+    qq = np.array([0.0, 1e-9, 0.05, 0.7])
+    want = O.debye_intensity(oracle_atoms(dup), qq)
+    for p, tol in ((F64, 1e-12), (F32, 1e-6)):
+        assert rel(csx.debye_intensity(dup, qq, precision=p), want).max() < tol
+    # fractional occupancy and hydrogens in both modes
+    atoms = [csx.Atom(pos=(0, 0, 0), element=csx.Element.C, residue_name="ALA", atom_name="CB",
+                      occupancy=0.5),
+             csx.Atom(pos=(1.1, 0, 0), element=csx.Element.H, residue_name="ALA",
+                      atom_name="HB1"),
+             csx.Atom(pos=(0, 1.5, 0.3), element=csx.Element.N, residue_name="ALA",
+                      atom_name="N", occupancy=0.8)]
+    s = csx.Structure(atoms)
+    qv = np.linspace(0, 0.9, 19)
+    for imp in (True, False):
+        want = O.debye_intensity(oracle_atoms(s), qv, implicit_h=imp)
+        for p, tol in ((F64, 1e-12), (F32, 1e-6)):
+            got = csx.debye_intensity_with_hydrogens(s, qv, imp, precision=p)
+            assert rel(got, want).max() < tol
+
+
+def test_deterministic(gpu_ctx):
+    s = csx.generate_globule(3000, 20.0, 1.0, seed=5)
+    q = np.linspace(0, 1, 64)
+    a = csx.debye_intensity(s, q)
+    b = csx.debye_intensity(s, q)
+    assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("q_tile,sin_mode,j_chunk", [(8, 0, 256), (32, 0, 4096), (16, 1, 2048)])
+def test_tuning_variants_agree(gpu_ctx, q_tile, sin_mode, j_chunk):
+    s = csx.generate_globule(1500, 18.0, 1.0, seed=6)
+    q = np.linspace(0, 1.0, 77)
+    want = csx.debye_intensity(s, q, precision=F64)
+    gpu_ctx.set_tuning(q_tile, sin_mode, j_chunk)
+    try:
+        s2 = csx.Structure.from_arrays(s.xyz, s.element)
+        got = csx.debye_intensity(s2, q)
+    finally:
+        gpu_ctx.set_tuning(16, 0, 2048)
+    err = rel(got, want).max()
+    print("tuning", q_tile, sin_mode, j_chunk, err, flush=True)
+    assert err < (1e-5 if sin_mode == 0 else 1e-4)
+
+
+def test_precision_study_20k_against_fp64_gpu(gpu_ctx):
+    """20,000 atoms, R = 35 A, q = linspace(0, 1, 500): FP32 vs FP64 GPU Debye."""
+    s = csx.generate_globule(20000, 35.0, 1.0, seed=1)
+    q = np.linspace(0.0, 1.0, 500)
+    ref = csx.debye_intensity(s, q, precision=F64)
+    got = csx.debye_intensity(s, q, precision=F32)
+    err = rel(got, ref)
+    print("20k vacuum max", err.max(), "mean", err.mean(), flush=True)
+    assert err.max() < 1e-5
+    p = csx.SolventParams()
+    ref = csx.solvent_corrected_intensity(s, q, p, precision=F64)
+    got = csx.solvent_corrected_intensity(s, q, p, precision=F32)
+    err = rel(got, ref)
+    print("20k solvent max", err.max(), "mean", err.mean(), flush=True)
+    assert err.max() < 1e-5
