@@ -4,7 +4,9 @@ The public names mirror the reference crate's API (src/lib.rs:16-50).
 """
 from .api import (  # noqa: F401
     Atom, ComputeOptions, Context, CurveMetadata, DeviceError, Element, FitDidNotConverge,
-    HydrationGrid, HydrationModel, HydrogenMode, InvalidInput, MultipoleOptions, NoDevice,
+    HydrationGrid, HydrationModel, HydrogenMode, InvalidInput, MultipoleOptions,
+    MultipoleParams, NoDevice, multipole_intensity, multipole_solvent_basis,
+    multipole_solvent_intensity,
     OutOfMemory, Precision, ProfileCalculator, SaxsError, ScatteringCurve, ScatteringMethod,
     SolventParams, Structure, build_adaptive_hydration_grid, build_hydration_grid,
     combine_solvent_basis, compute_curve, debye_intensity, debye_intensity_with_hydrogens,

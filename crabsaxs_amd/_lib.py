@@ -81,6 +81,13 @@ SIGNATURES = {
                                         c_size_t_p]),
     "csx_combine_solvent_basis": (C.c_int, [c_double_p, c_double_p, C.c_size_t, C.c_double,
                                             C.c_double, c_double_p]),
+    "csx_multipole_intensity": (C.c_int, [C.c_void_p, c_double_p, C.c_size_t, C.c_int,
+                                          C.c_int, c_double_p]),
+    "csx_multipole_solvent_intensity": (C.c_int, [C.c_void_p, c_double_p, C.c_size_t, C.c_int,
+                                                  C.c_int, C.POINTER(SolventParamsC),
+                                                  c_double_p]),
+    "csx_multipole_solvent_basis": (C.c_int, [C.c_void_p, c_double_p, C.c_size_t, C.c_int,
+                                              C.c_int, C.POINTER(SolventParamsC), c_double_p]),
     "csx_compute_curve": (C.c_int, [C.c_void_p, c_double_p, C.c_size_t,
                                     C.POINTER(ComputeOptionsC), c_double_p,
                                     C.POINTER(C.c_int)]),

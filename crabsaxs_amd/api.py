@@ -372,6 +372,49 @@ def debye_pair_tables(structure: Structure, q_values, params: SolventParams = No
     return out.reshape(nb, q.size), self_terms
 
 
+@dataclass
+class MultipoleParams:
+    """`MultipoleParams` (src/spherical.rs:19-31)."""
+    l_max: int = 20
+    hydrogen_mode: HydrogenMode = HydrogenMode.Implicit
+
+
+def multipole_intensity(structure: Structure, q_values, params: MultipoleParams = None,
+                        ctx: Context = None) -> np.ndarray:
+    """`multipole_intensity` (src/spherical.rs:266-312)."""
+    params = params or MultipoleParams()
+    q = f64(q_values).reshape(-1)
+    out = _out(q.size)
+    _check(_lib.load().csx_multipole_intensity(
+        structure.handle(ctx), dptr(q), q.size, int(params.l_max),
+        int(params.hydrogen_mode == HydrogenMode.Implicit), dptr(out)))
+    return out
+
+
+def multipole_solvent_intensity(structure: Structure, q_values, params: MultipoleParams,
+                                solvent: SolventParams, ctx: Context = None) -> np.ndarray:
+    """`multipole_solvent_intensity` (src/spherical.rs:408-470)."""
+    q = f64(q_values).reshape(-1)
+    out = _out(q.size)
+    p = solvent.to_c()
+    _check(_lib.load().csx_multipole_solvent_intensity(
+        structure.handle(ctx), dptr(q), q.size, int(params.l_max),
+        int(params.hydrogen_mode == HydrogenMode.Implicit), C.byref(p), dptr(out)))
+    return out
+
+
+def multipole_solvent_basis(structure: Structure, q_values, params: MultipoleParams,
+                            solvent: SolventParams, ctx: Context = None) -> np.ndarray:
+    """`multipole_solvent_basis` (src/spherical.rs:474-565): array [6, nq]."""
+    q = f64(q_values).reshape(-1)
+    out = _out(6 * q.size)
+    p = solvent.to_c()
+    _check(_lib.load().csx_multipole_solvent_basis(
+        structure.handle(ctx), dptr(q), q.size, int(params.l_max),
+        int(params.hydrogen_mode == HydrogenMode.Implicit), C.byref(p), dptr(out)))
+    return out.reshape(6, q.size)
+
+
 def combine_solvent_basis(basis, q_values, c1: float, c2: float) -> np.ndarray:
     """`combine_solvent_basis` (src/spherical.rs:567-584)."""
     q = f64(q_values).reshape(-1)

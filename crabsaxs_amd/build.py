@@ -32,7 +32,9 @@ SOURCES = [
     ("physics.cpp", "host"),
     ("capi.cpp", "host"),
     ("multipole.cpp", "host"),
+    ("fit.cpp", "host"),
     ("debye.hip", "device"),
+    ("multipole.hip", "device"),
 ]
 
 
@@ -51,7 +53,7 @@ def build(verbose: bool = False) -> Path:
         objs = []
         for name, kind in SOURCES:
             src = CSRC / name
-            obj = Path(tmp) / (src.stem + ".o")
+            obj = Path(tmp) / (src.name + ".o")
             if kind == "device":
                 cmd = [HIPCC, "-x", "hip", *COMMON, *DEVICE, "-c", str(src), "-o", str(obj)]
             else:

@@ -24,76 +24,6 @@ csx_status hip_fail(hipError_t e, const char* what) {
   return fail(st, std::string(what) + ": " + hipGetErrorString(e));
 }
 
-// Scope of one API call on a device: current device, a leased stream, an
-// optional kernel timer merged into the context statistics at the end.
-class Call {
- public:
-  explicit Call(csx_ctx* ctx) : ctx_(ctx) {}
-  ~Call() {
-    if (stream_) {
-      std::lock_guard<std::mutex> lock(ctx_->mu);
-      ctx_->free_streams.push_back(stream_);
-    }
-  }
-  csx_status begin() {
-    hipError_t e = hipSetDevice(ctx_->device);
-    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
-    {
-      std::lock_guard<std::mutex> lock(ctx_->mu);
-      if (!ctx_->free_streams.empty()) {
-        stream_ = ctx_->free_streams.back();
-        ctx_->free_streams.pop_back();
-      }
-    }
-    if (!stream_) {
-      e = hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking);
-      if (e != hipSuccess) {
-        stream_ = nullptr;
-        return hip_fail(e, "hipStreamCreate");
-      }
-    }
-    if (ctx_->timing.load()) timer_.reset(new KernelTimer);
-    return CSX_OK;
-  }
-  // Wait for the stream and fold kernel timings into the context.
-  csx_status finish() {
-    hipError_t e = hipStreamSynchronize(stream_);
-    if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
-    if (timer_) {
-      std::lock_guard<std::mutex> lock(ctx_->mu);
-      for (auto& p : timer_->pending) {
-        float ms = 0.f;
-        if (hipEventElapsedTime(&ms, p.start, p.stop) == hipSuccess) {
-          auto& st = ctx_->stats[p.name];
-          st.first += 1;
-          st.second += ms;
-        }
-      }
-      timer_.reset();
-    }
-    return CSX_OK;
-  }
-  hipStream_t stream() const { return stream_; }
-  KernelTimer* timer() const { return timer_.get(); }
-  csx_ctx* ctx() const { return ctx_; }
-
- private:
-  csx_ctx* ctx_;
-  hipStream_t stream_ = nullptr;
-  std::unique_ptr<KernelTimer> timer_;
-};
-
-#define CSX_HIP(expr, what)                          \
-  do {                                               \
-    hipError_t e_ = (expr);                          \
-    if (e_ != hipSuccess) return hip_fail(e_, what); \
-  } while (0)
-#define CSX_TRY(expr)                \
-  do {                               \
-    csx_status st_ = (expr);         \
-    if (st_ != CSX_OK) return st_;   \
-  } while (0)
-
 // ------------------------------------------------------------ point sets --
 
 static std::string set_key(const csx_solvent_params* p) {
@@ -201,7 +131,7 @@ static csx_status build_point_set(csx_structure* s, const csx_solvent_params* p,
   return CSX_OK;
 }
 
-static csx_status get_point_set(csx_structure* s, const csx_solvent_params* p, PointSet** out) {
+csx_status get_point_set(csx_structure* s, const csx_solvent_params* p, PointSet** out) {
   std::lock_guard<std::mutex> lock(s->mu);
   const std::string key = set_key(p);
   auto it = s->sets.find(key);

@@ -77,10 +77,13 @@ class DevArray {
   ~DevArray() {
     if (p_) (void)hipFree(p_);
   }
-  hipError_t upload(const void* host, size_t bytes) {
+  hipError_t alloc(size_t bytes) {
     if (p_) (void)hipFree(p_);
     p_ = nullptr;
-    hipError_t e = hipMalloc(&p_, bytes == 0 ? 16 : bytes);
+    return hipMalloc(&p_, bytes == 0 ? 16 : bytes);
+  }
+  hipError_t upload(const void* host, size_t bytes) {
+    hipError_t e = alloc(bytes);
     if (e != hipSuccess) return e;
     return bytes ? hipMemcpy(p_, host, bytes, hipMemcpyHostToDevice) : hipSuccess;
   }
@@ -91,9 +94,20 @@ class DevArray {
   void* p_ = nullptr;
 };
 
+// Multipole geometry of one point set at one l_max: radii, types and
+// occupancies in original order (atoms, then sites) and the FP32 real
+// spherical harmonics in MFMA B-operand tiles.
+struct MultipoleGeometry {
+  int L = -1, ntiles = 0, npad = 0, npts = 0, max_l_range = 1;
+  double rmax_atoms = 0.0, rmax_sites = 0.0;
+  DevArray radius, type, occ, Y, tile_l;
+};
+
 // One set of scattering points (atoms, optionally followed by hydration
 // sites as an extra type) laid out for both Debye kernels.
 struct PointSet {
+  std::mutex geom_mu;
+  std::map<int, std::unique_ptr<MultipoleGeometry>> geoms;  // by l_max
   int ntypes = 0;         // atom types (+1 when sites are present)
   int natoms = 0, nsites = 0;
   double site_volume = 0.0;
@@ -126,3 +140,81 @@ struct csx_structure {
   std::mutex mu;
   std::map<std::string, std::unique_ptr<csx::PointSet>> sets;
 };
+
+namespace csx {
+
+// Scope of one API call on a device: current device, a leased stream, an
+// optional kernel timer merged into the context statistics at the end.
+class Call {
+ public:
+  explicit Call(csx_ctx* ctx) : ctx_(ctx) {}
+  ~Call() {
+    if (stream_) {
+      std::lock_guard<std::mutex> lock(ctx_->mu);
+      ctx_->free_streams.push_back(stream_);
+    }
+  }
+  csx_status begin() {
+    hipError_t e = hipSetDevice(ctx_->device);
+    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+    {
+      std::lock_guard<std::mutex> lock(ctx_->mu);
+      if (!ctx_->free_streams.empty()) {
+        stream_ = ctx_->free_streams.back();
+        ctx_->free_streams.pop_back();
+      }
+    }
+    if (!stream_) {
+      e = hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking);
+      if (e != hipSuccess) {
+        stream_ = nullptr;
+        return hip_fail(e, "hipStreamCreate");
+      }
+    }
+    if (ctx_->timing.load()) timer_.reset(new KernelTimer);
+    return CSX_OK;
+  }
+  // Wait for the stream and fold kernel timings into the context.
+  csx_status finish() {
+    hipError_t e = hipStreamSynchronize(stream_);
+    if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
+    if (timer_) {
+      std::lock_guard<std::mutex> lock(ctx_->mu);
+      for (auto& p : timer_->pending) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, p.start, p.stop) == hipSuccess) {
+          auto& st = ctx_->stats[p.name];
+          st.first += 1;
+          st.second += ms;
+        }
+      }
+      timer_.reset();
+    }
+    return CSX_OK;
+  }
+  hipStream_t stream() const { return stream_; }
+  KernelTimer* timer() const { return timer_.get(); }
+  csx_ctx* ctx() const { return ctx_; }
+
+ private:
+  csx_ctx* ctx_;
+  hipStream_t stream_ = nullptr;
+  std::unique_ptr<KernelTimer> timer_;
+};
+
+#define CSX_HIP(expr, what)                          \
+  do {                                               \
+    hipError_t e_ = (expr);                          \
+    if (e_ != hipSuccess) return hip_fail(e_, what); \
+  } while (0)
+#define CSX_TRY(expr)                \
+  do {                               \
+    csx_status st_ = (expr);         \
+    if (st_ != CSX_OK) return st_;   \
+  } while (0)
+
+// Point set of a structure for a solvent model (atoms only when p is NULL or
+// analytic), built and uploaded on first use.
+csx_status get_point_set(csx_structure* s, const csx_solvent_params* p, PointSet** out);
+
+}  // namespace csx

@@ -293,8 +293,10 @@ hipError_t launch_debye_pairs(const float4* pts, const int4* items, int nitems, 
     }
     scope.stop();
   }
-  dim3 rgrid((nq_pad + 255) / 256, nbins);
-  reduce_bins<<<rgrid, 256, 0, stream>>>(partial, bin_first, nq_pad, out);
+  if (nbins > 0) {
+    dim3 rgrid((nq_pad + 255) / 256, nbins);
+    reduce_bins<<<rgrid, 256, 0, stream>>>(partial, bin_first, nq_pad, out);
+  }
   return hipGetLastError();
 }
 

@@ -1,13 +1,275 @@
-// Multipole path (spherical.rs): host dispatch.
+// Multipole path (spherical.rs:266-565): geometry cache, per-q expansion
+// orders, amplitude tables and the C-ABI entry points.
+#include <algorithm>
+#include <cmath>
+
 #include "capi_internal.hpp"
 
 namespace csx {
 
+int multipole_tiles_per_group();
+size_t multipole_coeff_lds_bytes(int l_range, int nch);
+hipError_t launch_multipole_geometry(const double* xyz, int npts, const double* centre, int l_max,
+                                     const double* norms, const int* tile_first, int npad,
+                                     float* radius, float* Y, hipStream_t stream);
+hipError_t launch_multipole_coeff(const float* radius, const int* type, const float* occ, int npts,
+                                  int npad, const float* Y, int ntiles, const int* tile_l,
+                                  int max_l_range, const float* amp, const float* qv,
+                                  const int* qlmax, int nqp, int nch, int chunk, float* partial,
+                                  double* out, hipStream_t stream, KernelTimer* timer);
+
+namespace {
+
+constexpr int kChunk = 2048;  // points per coefficient workgroup
+
+// harmonic_norms (spherical.rs:33-51).
+std::vector<double> harmonic_norms(int L) {
+  std::vector<double> norms((L + 1) * (L + 1), 0.0);
+  for (int l = 0; l <= L; ++l)
+    for (int m = 0; m <= l; ++m) {
+      double ratio = 1.0;
+      for (int k = l - m + 1; k <= l + m; ++k) ratio /= static_cast<double>(k);
+      norms[l * l + l + m] = std::sqrt((static_cast<double>(2 * l + 1) / (4.0 * kPi)) * ratio);
+    }
+  return norms;
+}
+
+// adaptive_l_max (spherical.rs:201-203).
+int adaptive_l_max(double q, double max_radius, int L) {
+  const double o = std::ceil(q * max_radius);
+  long v = (o > 1e6 ? 1000000L : static_cast<long>(std::max(o, 0.0))) + 8;
+  v = std::max<long>(v, std::min(L, 8));
+  return static_cast<int>(std::min<long>(v, L));
+}
+
+}  // namespace
+
+static csx_status get_geometry(Call& call, csx_structure* s, PointSet& ps, int L,
+                               MultipoleGeometry** out) {
+  std::lock_guard<std::mutex> lock(ps.geom_mu);
+  auto& slot = ps.geoms[L];
+  if (slot) {
+    *out = slot.get();
+    return CSX_OK;
+  }
+  std::unique_ptr<MultipoleGeometry> g(new MultipoleGeometry);
+  g->L = L;
+  const int n = ps.natoms, ns = ps.nsites;
+  g->npts = n + ns;
+  g->npad = (g->npts + 15) / 16 * 16;
+  std::vector<int> tile_first(L + 2), tile_l;
+  for (int l = 0; l <= L; ++l) {
+    tile_first[l] = static_cast<int>(tile_l.size());
+    for (int t = 0; t < (2 * l + 1 + 15) / 16; ++t) tile_l.push_back(l);
+  }
+  tile_first[L + 1] = static_cast<int>(tile_l.size());
+  g->ntiles = static_cast<int>(tile_l.size());
+  const int tpg = multipole_tiles_per_group();
+  for (int g0 = 0; g0 < g->ntiles; g0 += tpg) {
+    const int g1 = std::min(g0 + tpg, g->ntiles);
+    g->max_l_range = std::max(g->max_l_range, tile_l[g1 - 1] - tile_l[g0] + 1);
+  }
+  std::vector<double> xyz(3 * static_cast<size_t>(g->npts));
+  std::vector<int> type(g->npts);
+  std::vector<float> occ(g->npts);
+  for (int i = 0; i < n; ++i) {
+    for (int a = 0; a < 3; ++a) xyz[3 * i + a] = s->xyz[3 * i + a];
+    type[i] = s->type_of[i];
+    occ[i] = static_cast<float>(s->occ[i]);
+  }
+  const int T = static_cast<int>(s->types.size());
+  for (int k = 0; k < ns; ++k) {
+    for (int a = 0; a < 3; ++a) xyz[3 * (n + k) + a] = ps.site_xyz[3 * k + a];
+    type[n + k] = T;
+    occ[n + k] = 1.0f;
+  }
+  for (int i = 0; i < g->npts; ++i) {
+    double r2 = 0.0;
+    for (int a = 0; a < 3; ++a) {
+      const double d = xyz[3 * i + a] - s->centre[a];
+      r2 += d * d;
+    }
+    double& rm = i < n ? g->rmax_atoms : g->rmax_sites;
+    rm = std::max(rm, std::sqrt(r2));
+  }
+  const std::vector<double> norms = harmonic_norms(L);
+  const hipStream_t st = call.stream();
+  DevBuf d_xyz, d_centre, d_norms, d_first;
+  CSX_HIP(d_xyz.alloc(xyz.size() * sizeof(double), st), "hipMallocAsync");
+  CSX_HIP(d_centre.alloc(3 * sizeof(double), st), "hipMallocAsync");
+  CSX_HIP(d_norms.alloc(norms.size() * sizeof(double), st), "hipMallocAsync");
+  CSX_HIP(d_first.alloc(tile_first.size() * sizeof(int), st), "hipMallocAsync");
+  CSX_HIP(hipMemcpyAsync(d_xyz.as<void>(), xyz.data(), xyz.size() * sizeof(double),
+                         hipMemcpyHostToDevice, st), "hipMemcpyAsync");
+  CSX_HIP(hipMemcpyAsync(d_centre.as<void>(), s->centre, 3 * sizeof(double),
+                         hipMemcpyHostToDevice, st), "hipMemcpyAsync");
+  CSX_HIP(hipMemcpyAsync(d_norms.as<void>(), norms.data(), norms.size() * sizeof(double),
+                         hipMemcpyHostToDevice, st), "hipMemcpyAsync");
+  CSX_HIP(hipMemcpyAsync(d_first.as<void>(), tile_first.data(), tile_first.size() * sizeof(int),
+                         hipMemcpyHostToDevice, st), "hipMemcpyAsync");
+  {
+    std::vector<float> zero_r(g->npad, 0.0f);
+    CSX_HIP(g->radius.upload(zero_r.data(), zero_r.size() * sizeof(float)), "upload radii");
+  }
+  CSX_HIP(g->type.upload(type.data(), type.size() * sizeof(int)), "upload types");
+  CSX_HIP(g->occ.upload(occ.data(), occ.size() * sizeof(float)), "upload occupancy");
+  CSX_HIP(g->tile_l.upload(tile_l.data(), tile_l.size() * sizeof(int)), "upload tiles");
+  if (multipole_coeff_lds_bytes(g->max_l_range, 3) > 64 * 1024)
+    return fail(CSX_INVALID_INPUT, "multipole l_max too large for the LDS layout");
+  const size_t ybytes = static_cast<size_t>(g->ntiles) * g->npad * 16 * sizeof(float);
+  CSX_HIP(g->Y.alloc(ybytes), "hipMalloc harmonics");
+  CSX_HIP(hipMemsetAsync(g->Y.as<void>(), 0, ybytes, st), "hipMemsetAsync");
+  CSX_HIP(launch_multipole_geometry(d_xyz.as<double>(), g->npts, d_centre.as<double>(), L,
+                                    d_norms.as<double>(), d_first.as<int>(), g->npad,
+                                    g->radius.as<float>(), g->Y.as<float>(), st),
+          "multipole_geometry");
+  d_xyz.reset();
+  d_centre.reset();
+  d_norms.reset();
+  d_first.reset();
+  CSX_HIP(hipStreamSynchronize(st), "hipStreamSynchronize");
+  *out = g.get();
+  slot = std::move(g);
+  return CSX_OK;
+}
+
+// mode 0: vacuum intensity; 1: solvent intensity; 2: six-term basis.
+static csx_status multipole_run(csx_structure* s, const double* q, size_t nq, int L,
+                                bool implicit_h, const csx_solvent_params* p, int mode,
+                                double* out) {
+  const int nterms = mode == 2 ? 6 : 1;
+  if (s->n == 0) {
+    std::fill(out, out + nterms * nq, 0.0);
+    return CSX_OK;
+  }
+  Call call(s->ctx);
+  CSX_TRY(call.begin());
+  PointSet* ps = nullptr;
+  CSX_TRY(get_point_set(s, mode == 0 ? nullptr : p, &ps));
+  MultipoleGeometry* g = nullptr;
+  CSX_TRY(get_geometry(call, s, *ps, L, &g));
+  const bool grid = mode != 0 && p->hydration_model != CSX_HYDRATION_ANALYTIC;
+  const int nch = mode == 2 ? 3 : 1;
+  const size_t nqp = (nq + 15) / 16 * 16;
+  const int T = static_cast<int>(s->types.size());
+  const int TT = ps->ntypes;
+  std::vector<float> amp(static_cast<size_t>(TT) * nqp * nch, 0.0f);
+  std::vector<float> qv(nqp, 0.0f);
+  std::vector<int> qlmax(nqp, 0);
+  const double rmax = grid ? std::max(g->rmax_atoms, g->rmax_sites) : g->rmax_atoms;
+  for (size_t k = 0; k < nq; ++k) {
+    qv[k] = static_cast<float>(q[k]);
+    qlmax[k] = adaptive_l_max(q[k], rmax, L);
+    const double gscale = mode == 1 ? excluded_volume_scale(q[k], p->c1) : 0.0;
+    for (int t = 0; t < T; ++t) {
+      float* a = &amp[(static_cast<size_t>(t) * nqp + k) * nch];
+      if (mode == 0) {
+        a[0] = static_cast<float>(type_form_factor(s->types[t], implicit_h, q[k]));
+        continue;
+      }
+      const Amplitudes c = multipole_amplitudes(s->types[t], implicit_h, q[k],
+                                                p->shell_thickness_angstrom,
+                                                p->bulk_density_e_per_a3);
+      if (mode == 1) {
+        a[0] = static_cast<float>(c.v - gscale * c.e + (grid ? 0.0 : p->c2 * c.h));
+      } else {
+        a[0] = static_cast<float>(c.v);
+        a[1] = static_cast<float>(c.e);
+        a[2] = static_cast<float>(grid ? 0.0 : c.h);
+      }
+    }
+    if (grid) {
+      const double site = p->bulk_density_e_per_a3 / 10.0 * ps->site_volume * water_form_factor(q[k]);
+      float* a = &amp[(static_cast<size_t>(T) * nqp + k) * nch];
+      if (mode == 1) a[0] = static_cast<float>(p->c2 * site);
+      else a[2] = static_cast<float>(site);
+    }
+  }
+  const int npts = mode == 0 ? ps->natoms : g->npts;
+  const int nchunk = (g->npad + kChunk - 1) / kChunk;
+  const hipStream_t st = call.stream();
+  DevBuf d_amp, d_q, d_l, d_partial, d_out;
+  CSX_HIP(d_amp.alloc(amp.size() * sizeof(float), st), "hipMallocAsync");
+  CSX_HIP(d_q.alloc(nqp * sizeof(float), st), "hipMallocAsync");
+  CSX_HIP(d_l.alloc(nqp * sizeof(int), st), "hipMallocAsync");
+  CSX_HIP(d_partial.alloc(static_cast<size_t>(nchunk) * nch * nqp * g->ntiles * 16 * sizeof(float), st),
+          "hipMallocAsync");
+  CSX_HIP(d_out.alloc(nqp * nterms * sizeof(double), st), "hipMallocAsync");
+  CSX_HIP(hipMemcpyAsync(d_amp.as<void>(), amp.data(), amp.size() * sizeof(float),
+                         hipMemcpyHostToDevice, st), "hipMemcpyAsync");
+  CSX_HIP(hipMemcpyAsync(d_q.as<void>(), qv.data(), nqp * sizeof(float), hipMemcpyHostToDevice, st),
+          "hipMemcpyAsync");
+  CSX_HIP(hipMemcpyAsync(d_l.as<void>(), qlmax.data(), nqp * sizeof(int), hipMemcpyHostToDevice, st),
+          "hipMemcpyAsync");
+  CSX_HIP(launch_multipole_coeff(g->radius.as<float>(), g->type.as<int>(), g->occ.as<float>(), npts,
+                                 g->npad, g->Y.as<float>(), g->ntiles, g->tile_l.as<int>(),
+                                 g->max_l_range, d_amp.as<float>(), d_q.as<float>(), d_l.as<int>(),
+                                 static_cast<int>(nqp), nch, kChunk, d_partial.as<float>(),
+                                 d_out.as<double>(), st, call.timer()),
+          "multipole_coeff");
+  std::vector<double> res(nqp * nterms);
+  CSX_HIP(hipMemcpyAsync(res.data(), d_out.as<void>(), res.size() * sizeof(double),
+                         hipMemcpyDeviceToHost, st), "hipMemcpyAsync");
+  d_amp.reset();
+  d_q.reset();
+  d_l.reset();
+  d_partial.reset();
+  d_out.reset();
+  CSX_TRY(call.finish());
+  const double scale = 4.0 * kPi;
+  for (size_t k = 0; k < nq; ++k)
+    for (int t = 0; t < nterms; ++t) {
+      const double v = scale * res[k * nterms + t];
+      out[t * nq + k] = mode == 2 ? v : std::max(v, 0.0);
+    }
+  return CSX_OK;
+}
+
+static csx_status check(csx_structure* s, const double* q, size_t nq, int L, const void* out) {
+  if (!s) return fail(CSX_INVALID_INPUT, "structure handle is null");
+  if (nq > 0 && (!q || !out)) return fail(CSX_INVALID_INPUT, "q and output buffers are required");
+  if (L < 0 || L > 64) return fail(CSX_INVALID_INPUT, "multipole l_max must be between 0 and 64");
+  if (nq > (1u << 20)) return fail(CSX_INVALID_INPUT, "too many q values");
+  return CSX_OK;
+}
+
 csx_status multipole_dispatch(csx_structure* s, const double* q, size_t nq, int l_max,
                               bool implicit_h, const csx_solvent_params* p, int precision,
                               double* out) {
-  (void)s; (void)q; (void)nq; (void)l_max; (void)implicit_h; (void)p; (void)precision; (void)out;
-  return fail(CSX_INVALID_INPUT, "multipole path is not built yet");
+  (void)precision;  // the multipole contraction has one (FP32 MFMA) arithmetic
+  CSX_TRY(check(s, q, nq, l_max, out));
+  if (nq == 0) return CSX_OK;
+  return multipole_run(s, q, nq, l_max, implicit_h, p, p ? 1 : 0, out);
 }
 
 }  // namespace csx
+
+using namespace csx;
+
+extern "C" {
+
+csx_status csx_multipole_intensity(csx_structure* s, const double* q, size_t nq, int l_max,
+                                   int implicit_h, double* out) {
+  CSX_TRY(check(s, q, nq, l_max, out));
+  if (nq == 0) return CSX_OK;
+  return multipole_run(s, q, nq, l_max, implicit_h != 0, nullptr, 0, out);
+}
+
+csx_status csx_multipole_solvent_intensity(csx_structure* s, const double* q, size_t nq,
+                                           int l_max, int implicit_h,
+                                           const csx_solvent_params* p, double* out) {
+  CSX_TRY(check(s, q, nq, l_max, out));
+  if (!p) return fail(CSX_INVALID_INPUT, "solvent parameters are required");
+  if (nq == 0) return CSX_OK;
+  return multipole_run(s, q, nq, l_max, implicit_h != 0, p, 1, out);
+}
+
+csx_status csx_multipole_solvent_basis(csx_structure* s, const double* q, size_t nq, int l_max,
+                                       int implicit_h, const csx_solvent_params* p, double* out) {
+  CSX_TRY(check(s, q, nq, l_max, out));
+  if (!p) return fail(CSX_INVALID_INPUT, "solvent parameters are required");
+  if (nq == 0) return CSX_OK;
+  return multipole_run(s, q, nq, l_max, implicit_h != 0, p, 2, out);
+}
+
+}  // extern "C"

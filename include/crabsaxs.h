@@ -215,12 +215,161 @@ csx_status csx_combine_solvent_basis(const double *basis, const double *q,
                                      size_t nq, double c1, double c2,
                                      double *out);
 
+/* ---- multipole path ------------------------------------------------------ */
+/* All three run the FP32 MFMA contraction with an exact j_l (FP64 upward
+ * recurrence for l <= q r, FP64 downward ratio recurrence above) and the
+ * per-q expansion order clamp(ceil(q R_max) + 8, min(8, l_max), l_max) of
+ * spherical.rs:201-203.  0 <= l_max <= 64.  Empty structure -> zeros. */
+
+/* multipole_intensity (src/spherical.rs:266-312).  out: nq. */
+csx_status csx_multipole_intensity(csx_structure *s, const double *q,
+                                   size_t nq, int l_max,
+                                   int implicit_hydrogen, double *out);
+
+/* multipole_solvent_intensity (src/spherical.rs:408-470).  out: nq. */
+csx_status csx_multipole_solvent_intensity(csx_structure *s, const double *q,
+                                           size_t nq, int l_max,
+                                           int implicit_hydrogen,
+                                           const csx_solvent_params *p,
+                                           double *out);
+
+/* multipole_solvent_basis (src/spherical.rs:474-565).  out: 6*nq rows vv,
+ * ee, hh, ve, vh, eh. */
+csx_status csx_multipole_solvent_basis(csx_structure *s, const double *q,
+                                       size_t nq, int l_max,
+                                       int implicit_hydrogen,
+                                       const csx_solvent_params *p,
+                                       double *out);
+
 /* compute_curve (src/scattering.rs:112-212): validation, Auto method
  * choice, adaptive l_max and dispatch.  out: nq; *method_used receives
  * CSX_METHOD_DEBYE or CSX_METHOD_MULTIPOLE. */
 csx_status csx_compute_curve(csx_structure *s, const double *q, size_t nq,
                              const csx_compute_options *options, double *out,
                              int *method_used);
+
+/* ---- fitting (src/fit.rs) ---------------------------------------------- */
+
+/* FitParams, fit.rs:11-17. */
+typedef struct {
+    double scale;
+    double background;
+    double c1;
+    double c2;
+} csx_fit_params;
+
+/* BackgroundMode (fit.rs:41-47) and ScaleMode (fit.rs:29-34). */
+typedef enum {
+    CSX_BACKGROUND_FIT = 0,
+    CSX_BACKGROUND_FIXED_ZERO = 1,
+    CSX_BACKGROUND_FIXED = 2
+} csx_background_mode;
+typedef enum { CSX_SCALE_FIT = 0, CSX_SCALE_FIXED = 1 } csx_scale_mode;
+
+/* Optimiser over (c1, c2): the reference's Nelder-Mead (fit.rs:385-418) or
+ * a projected quasi-Newton on the analytic gradient d chi^2/d(c1, c2) (no
+ * reference counterpart). */
+typedef enum {
+    CSX_OPTIMIZER_NELDER_MEAD = 0,
+    CSX_OPTIMIZER_GRADIENT = 1
+} csx_optimizer;
+
+/* FitOptions, fit.rs:49-82 (defaults: csx_fit_options_default). */
+typedef struct {
+    int32_t method;              /* csx_method */
+    int32_t implicit_hydrogen;
+    double c1_bounds[2];
+    double c2_bounds[2];
+    int32_t max_iterations;
+    int32_t q_sampling_stride;
+    csx_solvent_params solvent;
+    int32_t background_mode;     /* csx_background_mode */
+    double background_value;     /* for CSX_BACKGROUND_FIXED */
+    int32_t scale_mode;          /* csx_scale_mode */
+    double scale_value;          /* for CSX_SCALE_FIXED */
+    int32_t optimizer;           /* csx_optimizer */
+    int32_t precision;           /* csx_precision of Debye pair sums */
+} csx_fit_options;
+
+/* FitResult, fit.rs:19-27 (fitted curve returned separately). */
+typedef struct {
+    csx_fit_params params;
+    double chi2;
+    int64_t iterations;
+    int32_t converged;
+    char termination[96];
+} csx_fit_result;
+
+void csx_fit_options_default(csx_fit_options *out);
+
+/* solve_scale_background (fit.rs:200-244) incl. solve_linear (137-198):
+ * out3 = (scale, background, reduced chi^2). */
+csx_status csx_solve_scale_background(const double *theory,
+                                      const double *intensity,
+                                      const double *sigma, size_t n,
+                                      int scale_mode, double scale_value,
+                                      int background_mode,
+                                      double background_value, double *out3);
+
+/* cubic_hermite_interpolate (fit.rs:93-123). */
+csx_status csx_cubic_hermite_interpolate(const double *x, const double *y,
+                                         size_t n, const double *targets,
+                                         size_t nt, double *out);
+
+/* The stride-sampled, Hermite-interpolated six-term basis that
+ * fit_to_experimental_with_options minimises over (fit.rs:329-384).
+ * basis_out: 6*n on the experimental q grid. */
+csx_status csx_fit_basis(csx_structure *s, const double *q, size_t n,
+                         const csx_fit_options *options, double *basis_out);
+
+/* fit_to_experimental_with_options (fit.rs:288-454).  fitted: n or NULL. */
+csx_status csx_fit_to_experimental(csx_structure *s, const double *q,
+                                   const double *intensity,
+                                   const double *sigma, size_t n,
+                                   const csx_fit_params *initial,
+                                   const csx_fit_options *options,
+                                   csx_fit_result *result, double *fitted);
+
+/* The same fit from a precomputed basis (6*n); no reference counterpart. */
+csx_status csx_fit_from_basis(const double *basis, const double *q,
+                              const double *intensity, const double *sigma,
+                              size_t n, const csx_fit_params *initial,
+                              const csx_fit_options *options,
+                              csx_fit_result *result, double *fitted);
+
+/* chi^2 (reduced, N-1 dof as fit.rs:185-197) and its gradient
+ * d chi^2 / d(scale, background, c1, c2) at the given parameters, from the
+ * six-term basis (no reference counterpart).  grad4: 4 doubles. */
+csx_status csx_chi2_grad(const double *basis, const double *q,
+                         const double *intensity, const double *sigma,
+                         size_t n, const csx_fit_params *params,
+                         double *chi2, double *grad4);
+
+/* ---- ensembles (src/ensemble.rs) ---------------------------------------- */
+
+/* project_simplex (ensemble.rs:449-465). */
+csx_status csx_project_simplex(const double *values, size_t m, double *out);
+
+/* optimize_weights (ensemble.rs:317-381) on m curves (m*n, row-major):
+ * weights[m]; out3 = (scale, background, reduced chi^2); fitted: n or NULL. */
+csx_status csx_optimize_weights(const double *curves, size_t m,
+                                const double *intensity, const double *sigma,
+                                size_t n, const csx_fit_options *options,
+                                int uniform, double regularization,
+                                double *weights, double *out3,
+                                double *fitted);
+
+/* EnsembleFitter::fit_core (ensemble.rs:234-299): the 5 x 5 (c1, c2)
+ * candidate grid re-contracts per-conformer six-term bases (m*6*n) instead
+ * of recomputing profiles; best candidate's weights[m], params, reduced
+ * chi^2 and fitted curve (n or NULL). */
+csx_status csx_ensemble_fit_core(const double *bases, size_t m,
+                                 const double *q, const double *intensity,
+                                 const double *sigma, size_t n,
+                                 const csx_fit_options *options, int uniform,
+                                 double regularization, double *weights,
+                                 csx_fit_params *params, double *chi2,
+                                 double *fitted);
 
 /* ---- synthetic workloads (no reference counterpart) -------------------- */
 

@@ -91,7 +91,10 @@ def test_solvent_basis_matches_oracle(gpu_ctx, model):
     want = O.debye_solvent_basis(oracle_atoms(s), q, c1=1.03, c2=0.4, sites=sites,
                                  voxel_volume=vol)
     scale = np.abs(want).max(axis=1, keepdims=True)
-    for prec, tol in ((F64, 1e-12), (F32, 1e-5)):
+    # FP64 differs from the oracle only by summation order and the last bit of
+    # sin: ~1e-16 per term, over 8e4 atom pairs and up to 1e8 site pairs
+    # (1 A exterior grid) with cancellation between terms.
+    for prec, tol in ((F64, 1e-9), (F32, 1e-5)):
         got = csx.debye_solvent_basis(s, q, p, precision=prec)
         err = (np.abs(got - want) / scale).max()
         print(model.name, prec.name, err, flush=True)

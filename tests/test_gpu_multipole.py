@@ -1,0 +1,130 @@
+"""GPU parity of the multipole path (FP32 MFMA contraction, exact j_l)."""
+import json
+import math
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import crabsaxs_amd as csx
+import oracle_py as O
+
+pytestmark = pytest.mark.gpu
+GOLD = json.loads((Path(__file__).parent / "golden" / "debye_golden.json").read_text())
+
+
+def rel(a, b):
+    return np.abs(a - b) / np.maximum(np.abs(b), 1e-300)
+
+
+def gold(name):
+    s = GOLD["structures"][name]
+    return csx.Structure.from_arrays(s["xyz"], s["element"])
+
+
+def oatoms(s):
+    return O.Atoms(s.xyz, s.element, s.group, s.occupancy)
+
+
+def test_two_atoms_match_debye(gpu_ctx):
+    # spherical.rs:596-634 and 637-678 (L = 32, 1e-6)
+    for xyz, el, q in [([[0, 0, 0], [2, 0, 0]], [1, 1], [0.1, 0.2, 0.5]),
+                       ([[-20, 0, 0], [20, 0, 0]], [1, 3], [0.2, 0.35, 0.5])]:
+        s = csx.Structure.from_arrays(xyz, el)
+        m = csx.multipole_intensity(s, q, csx.MultipoleParams(l_max=32))
+        d = csx.debye_intensity(s, q, precision=csx.Precision.FP64_PLAIN)
+        print("two atoms", rel(m, d).max(), flush=True)
+        assert rel(m, d).max() < 2e-6
+
+
+def test_helix_nrmse(gpu_ctx):
+    # spherical.rs:681-727 (24 atoms, L = 24, NRMSE <= 1e-3) and parity with
+    # the reference's own Miller j_l, accurate at these q r.
+    s = gold("helix24")
+    q = np.arange(1, 101) * 0.005
+    m = csx.multipole_intensity(s, q, csx.MultipoleParams(l_max=24))
+    d = csx.debye_intensity(s, q, precision=csx.Precision.FP64_PLAIN)
+    assert math.sqrt(((m - d) ** 2).sum() / (d ** 2).sum()) <= 1e-3
+    ref = O.multipole_intensity(oatoms(s), q, 24, exact=False)
+    print("helix vs reference-miller oracle", rel(m, ref).max(), flush=True)
+    assert rel(m, ref).max() < 1e-5
+
+
+def test_compute_curve_auto_multipole_spiral(gpu_ctx):
+    # scattering.rs:312-365: adaptive l_max multipole within 1e-3 (rel-L2) of Debye
+    s = gold("spiral96")
+    q = np.arange(1, 101) * 0.005
+    c = csx.compute_curve(s, q, csx.ComputeOptions(method=csx.ScatteringMethod.Multipole))
+    assert c.metadata.method == "Multipole"
+    d = csx.debye_intensity(s, q, precision=csx.Precision.FP64_PLAIN)
+    assert math.sqrt(((c.intensity - d) ** 2).sum() / (d ** 2).sum()) <= 1e-3
+
+
+def test_adaptive_grid_basis_matches_debye(gpu_ctx):
+    # spherical.rs:730-787
+    s = gold("eight")
+    q = np.arange(1, 51) * 0.01
+    p = csx.SolventParams(hydration_model=csx.HydrationModel.AdaptiveGrid,
+                          hydration_grid_spacing_angstrom=8.0)
+    m = csx.multipole_solvent_basis(s, q, csx.MultipoleParams(l_max=32), p)
+    d = csx.debye_solvent_basis(s, q, p, precision=csx.Precision.FP64_PLAIN)
+    for t in range(6):
+        err = math.sqrt(((m[t] - d[t]) ** 2).sum() / max((d[t] ** 2).sum(), 1e-30))
+        assert err <= 1e-3, (t, err)
+
+
+@pytest.mark.parametrize("l_max", [15, 25, 40])
+def test_globule_matches_exact_oracle(gpu_ctx, l_max):
+    s = csx.generate_globule(1200, 16.0, 1.0, seed=21)
+    q = np.linspace(0.005, 0.5, 60)
+    got = csx.multipole_intensity(s, q, csx.MultipoleParams(l_max=l_max))
+    want = O.multipole_intensity(oatoms(s), q, l_max, exact=True)
+    print("globule L", l_max, rel(got, want).max(), flush=True)
+    assert rel(got, want).max() < 1e-5
+
+
+@pytest.mark.parametrize("model", [csx.HydrationModel.Analytic, csx.HydrationModel.AdaptiveGrid])
+def test_solvent_paths_match_exact_oracle(gpu_ctx, model):
+    s = csx.generate_globule(700, 14.0, 1.0, seed=22)
+    q = np.linspace(0.0, 0.5, 41)
+    p = csx.SolventParams(c1=1.02, c2=0.6, hydration_model=model,
+                          hydration_grid_spacing_angstrom=4.0)
+    sites, vol = (None, 0.0)
+    if model == csx.HydrationModel.AdaptiveGrid:
+        sites, vol = O.adaptive_grid(s.xyz, spacing=4.0)
+    mp = csx.MultipoleParams(l_max=30)
+    want_i = O.multipole_solvent(oatoms(s), q, 30, c1=1.02, c2=0.6, sites=sites,
+                                 voxel_volume=vol, exact=True)
+    want_b = O.multipole_solvent(oatoms(s), q, 30, c1=1.02, c2=0.6, sites=sites,
+                                 voxel_volume=vol, basis=True, exact=True)
+    got_i = csx.multipole_solvent_intensity(s, q, mp, p)
+    got_b = csx.multipole_solvent_basis(s, q, mp, p)
+    print(model.name, rel(got_i, want_i).max(), flush=True)
+    assert rel(got_i, want_i).max() < 1e-5
+    scale = np.abs(want_b).max(axis=1, keepdims=True)
+    assert (np.abs(got_b - want_b) / scale).max() < 1e-5
+
+
+def test_empty_and_single(gpu_ctx):
+    empty = csx.Structure.from_arrays(np.zeros((0, 3)), np.zeros(0, dtype=np.uint8))
+    q = np.array([0.0, 0.1])
+    assert np.array_equal(csx.multipole_intensity(empty, q), np.zeros(2))
+    one = csx.Structure.from_arrays([[3.0, -1.0, 2.0]], [3])
+    f = np.array([csx.form_factor_for_atom(csx.Atom(pos=(0, 0, 0), element=csx.Element.O),
+                                           True, v) for v in q])
+    assert rel(csx.multipole_intensity(one, q, csx.MultipoleParams(l_max=8)), f * f).max() < 1e-6
+
+
+def test_prolate_with_shell_sites_large(gpu_ctx):
+    """10,000-atom 2:1 prolate globule with 3 A shell pseudo-atoms, L = 25,
+    against the exact-j_l FP64 oracle on a q subset."""
+    s = csx.generate_globule(10000, 30.0 / 2 ** (1 / 3), 2.0, seed=23)
+    sites = csx.generate_shell_sites(s, 3.0, 3.0, 6.0)
+    xyz = np.vstack([s.xyz, sites])
+    el = np.concatenate([s.element, np.full(len(sites), 3, dtype=np.uint8)])
+    full = csx.Structure.from_arrays(xyz, el)
+    q = np.linspace(0.005, 0.5, 24)
+    got = csx.multipole_intensity(full, q, csx.MultipoleParams(l_max=25))
+    want = O.multipole_intensity(oatoms(full), q, 25, exact=True)
+    print("prolate", len(full), rel(got, want).max(), flush=True)
+    assert rel(got, want).max() < 1e-5
