@@ -41,7 +41,21 @@ def parse():
     ap.add_argument("--sin-mode", type=int, default=None)
     ap.add_argument("--q-tile", type=int, default=None)
     ap.add_argument("--j-chunk", type=int, default=None)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only run a device-to-device copy of BYTES (HBM counter calibration)")
     return ap.parse_args()
+
+
+def copy_calibration(nbytes: int) -> None:
+    """One elementwise device copy of a known size: under rocprofv3 --pmc its
+    FETCH_SIZE / WRITE_SIZE give the counter's bytes-per-KiB scale."""
+    import torch
+    src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes": nbytes, "read": nbytes, "written": nbytes}))
 
 
 def cpu_baseline(structure, q, threads):
@@ -64,6 +78,9 @@ def cpu_baseline(structure, q, threads):
 
 def main():
     args = parse()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not "
                                    "part of it (use --gpus 1)"}))
@@ -71,7 +88,7 @@ def main():
     import crabsaxs_amd as csx
 
     ctx = csx.default_context()
-    tune = dict(q_tile=16, sin_mode=0, j_chunk=2048)
+    tune = dict(q_tile=64, sin_mode=2, j_chunk=512)
     if args.sin_mode is not None:
         tune["sin_mode"] = args.sin_mode
     if args.q_tile is not None:

@@ -205,7 +205,7 @@ class Context:
         _check(_lib.load().csx_ctx_kernel_time(self.ptr, name.encode(), C.byref(n), C.byref(ms)))
         return int(n.value), float(ms.value)
 
-    def set_tuning(self, q_tile: int = 16, sin_mode: int = 0, j_chunk: int = 2048) -> None:
+    def set_tuning(self, q_tile: int = 64, sin_mode: int = 2, j_chunk: int = 512) -> None:
         _check(_lib.load().csx_ctx_set_tuning(self.ptr, q_tile, sin_mode, j_chunk))
 
     def close(self):

@@ -72,6 +72,9 @@ static csx_status build_point_set(csx_structure* s, const csx_solvent_params* p,
     const int t = s->type_of[i];
     for (int a = 0; a < 3; ++a) pts32[4 * k + a] = static_cast<float>(s->xyz[3 * i + a] - s->centre[a]);
     pts32[4 * k + 3] = static_cast<float>(s->occ[i]);
+    ps.max_radius = std::max(ps.max_radius, std::sqrt(double(pts32[4 * k]) * pts32[4 * k] +
+                                                      double(pts32[4 * k + 1]) * pts32[4 * k + 1] +
+                                                      double(pts32[4 * k + 2]) * pts32[4 * k + 2]));
     seg_start[t + 1] += 1;
     ps.self[t] += s->occ[i] * s->occ[i];
     ps.wsum[t] += s->occ[i];
@@ -80,6 +83,9 @@ static csx_status build_point_set(csx_structure* s, const csx_solvent_params* p,
     for (int a = 0; a < 3; ++a)
       pts32[4 * (n + k) + a] = static_cast<float>(ps.site_xyz[3 * k + a] - s->centre[a]);
     pts32[4 * (n + k) + 3] = 1.0f;
+    const float* p3 = &pts32[4 * (n + k)];
+    ps.max_radius = std::max(ps.max_radius, std::sqrt(double(p3[0]) * p3[0] + double(p3[1]) * p3[1] +
+                                                      double(p3[2]) * p3[2]));
   }
   if (with_sites) {
     seg_start[T + 1] = ps.nsites;
@@ -155,7 +161,13 @@ static csx_status pair_tables_f32(Call& call, const PointSet& ps, const double* 
   const int qt = tune.q_tile;
   const size_t nq_pad = (nq + qt - 1) / qt * qt;
   std::vector<float> qrev(nq_pad, 0.0f);
-  for (size_t k = 0; k < nq; ++k) qrev[k] = static_cast<float>(q[k] / (2.0 * kPi));
+  double umax = 0.0;
+  for (size_t k = 0; k < nq; ++k) {
+    qrev[k] = static_cast<float>(q[k] / (2.0 * kPi));
+    umax = std::max(umax, std::fabs(static_cast<double>(qrev[k])) * 2.0 * ps.max_radius);
+  }
+  // The unreduced hardware sine takes |q r / 2 pi| <= 256 revolutions.
+  const int sin_mode = tune.sin_mode == 2 && !(umax <= 250.0) ? 1 : tune.sin_mode;
   const hipStream_t st = call.stream();
   DevBuf d_q, d_partial, d_out;
   CSX_HIP(d_q.alloc(nq_pad * sizeof(float), st), "hipMallocAsync");
@@ -166,7 +178,7 @@ static csx_status pair_tables_f32(Call& call, const PointSet& ps, const double* 
           "hipMemcpyAsync");
   CSX_HIP(launch_debye_pairs(ps.pts32.as<float4>(), ps.items.as<int4>(), ps.nitems,
                              ps.bin_first.as<int>(), ps.nbins, d_q.as<float>(),
-                             static_cast<int>(nq_pad), qt, tune.sin_mode, d_partial.as<double>(),
+                             static_cast<int>(nq_pad), qt, sin_mode, d_partial.as<double>(),
                              d_out.as<double>(), st, call.timer()),
           "debye_pairs_f32");
   std::vector<double> raw(size_t(ps.nbins) * nq_pad);
@@ -409,9 +421,9 @@ csx_status csx_ctx_reset_timing(csx_ctx* ctx) {
 
 csx_status csx_ctx_set_tuning(csx_ctx* ctx, int q_tile, int sin_mode, int j_chunk) {
   if (!ctx) return fail(CSX_INVALID_INPUT, "context is null");
-  if (q_tile != 8 && q_tile != 16 && q_tile != 32)
-    return fail(CSX_INVALID_INPUT, "q_tile must be 8, 16 or 32");
-  if (sin_mode != 0 && sin_mode != 1) return fail(CSX_INVALID_INPUT, "sin_mode must be 0 or 1");
+  if (q_tile != 8 && q_tile != 16 && q_tile != 32 && q_tile != 64)
+    return fail(CSX_INVALID_INPUT, "q_tile must be 8, 16, 32 or 64");
+  if (sin_mode < 0 || sin_mode > 2) return fail(CSX_INVALID_INPUT, "sin_mode must be 0, 1 or 2");
   if (j_chunk < 256 || j_chunk % 256 != 0 || j_chunk > (1 << 20))
     return fail(CSX_INVALID_INPUT, "j_chunk must be a multiple of 256 in [256, 2^20]");
   std::lock_guard<std::mutex> lock(ctx->mu);

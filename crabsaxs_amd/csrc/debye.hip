@@ -167,6 +167,17 @@ __device__ inline float sin2pi_hw(float u) {
   return __builtin_amdgcn_sinf(g);
 }
 
+// Hardware V_SIN_F32 on the unreduced argument; its own reduction takes the
+// fraction exactly, valid for |u| <= 256 revolutions (checked on the host).
+__device__ inline float sin2pi_hw_direct(float u) { return __builtin_amdgcn_sinf(u); }
+
+template <int SINMODE>
+__device__ inline float sin2pi(float u) {
+  if (SINMODE == 0) return sin2pi_soft(u);
+  if (SINMODE == 1) return sin2pi_hw(u);
+  return sin2pi_hw_direct(u);
+}
+
 template <int QT, int SINMODE>
 __global__ __launch_bounds__(kBlock) void debye_pairs_f32(const float4* __restrict__ pts,
                                                           const int4* __restrict__ items,
@@ -183,13 +194,14 @@ __global__ __launch_bounds__(kBlock) void debye_pairs_f32(const float4* __restri
   float qk[QT];
 #pragma unroll
   for (int k = 0; k < QT; ++k) qk[k] = qrev[qc + k];
+  // Per lane: FP32 tile sums acc[k] (<= 256 pairs).  After each tile they are
+  // weighted by occ_i, summed over the wave (fixed butterfly) and lane k adds
+  // the wave's tile sum for q_k to its FP64 running sum `run`.
   float acc[QT];
-  double acc64[QT];
 #pragma unroll
-  for (int k = 0; k < QT; ++k) {
-    acc[k] = 0.f;
-    acc64[k] = 0.0;
-  }
+  for (int k = 0; k < QT; ++k) acc[k] = 0.f;
+  double run = 0.0;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int jend = item.z + item.w;
   for (int t0 = item.z; t0 < jend; t0 += kBlock) {
     __syncthreads();
@@ -201,28 +213,33 @@ __global__ __launch_bounds__(kBlock) void debye_pairs_f32(const float4* __restri
       const float4 pj = tile[t];
       const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
       const float d2 = fmaxf(__builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz)), 1e-30f);
+      // Correctly rounded sqrt: the raw V_SQRT_F32 approximation biases r,
+      // i.e. shifts every phase q r the same way, which measurably moves
+      // I(q) where it is steep (once per pair, so the cost is amortised).
       const float r = __builtin_sqrtf(d2);
       const float w = (t0 + t > i) ? pj.w * __builtin_amdgcn_rcpf(r) : 0.f;
 #pragma unroll
       for (int k = 0; k < QT; ++k) {
         const float u = qk[k] * r;
-        const float s = SINMODE == 0 ? sin2pi_soft(u) : sin2pi_hw(u);
+        const float s = sin2pi<SINMODE>(u);
         acc[k] = __builtin_fmaf(w, s, acc[k]);
       }
     }
 #pragma unroll
     for (int k = 0; k < QT; ++k) {
-      acc64[k] += double(acc[k]);
+      float v = acc[k] * pi.w;
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+      if (lane == k) run += double(v);
       acc[k] = 0.f;
     }
   }
-  const double wi = double(pi.w);
-#pragma unroll
-  for (int k = 0; k < QT; ++k) acc64[k] *= wi;
-  block_sum_f64<QT>(acc64, scratch);
-  if (threadIdx.x == 0) {
-#pragma unroll
-    for (int k = 0; k < QT; ++k) partial[size_t(blockIdx.x) * nq_pad + qc + k] = acc64[k];
+  if (lane < QT) scratch[wave * QT + lane] = run;
+  __syncthreads();
+  if (threadIdx.x < QT) {
+    const int k = threadIdx.x;
+    partial[size_t(blockIdx.x) * nq_pad + qc + k] =
+        ((scratch[k] + scratch[QT + k]) + scratch[2 * QT + k]) + scratch[3 * QT + k];
   }
 }
 
@@ -283,13 +300,16 @@ hipError_t launch_debye_pairs(const float4* pts, const int4* items, int nitems, 
   {
     TimerScope scope(timer, "debye_pairs_f32", stream);
     if (nitems > 0) {
-      if (qt == 8 && sin_mode == 0) launch_pairs_t<8, 0>(grid, pts, items, qrev, nq_pad, partial, stream);
-      else if (qt == 8 && sin_mode == 1) launch_pairs_t<8, 1>(grid, pts, items, qrev, nq_pad, partial, stream);
-      else if (qt == 16 && sin_mode == 0) launch_pairs_t<16, 0>(grid, pts, items, qrev, nq_pad, partial, stream);
-      else if (qt == 16 && sin_mode == 1) launch_pairs_t<16, 1>(grid, pts, items, qrev, nq_pad, partial, stream);
-      else if (qt == 32 && sin_mode == 0) launch_pairs_t<32, 0>(grid, pts, items, qrev, nq_pad, partial, stream);
-      else if (qt == 32 && sin_mode == 1) launch_pairs_t<32, 1>(grid, pts, items, qrev, nq_pad, partial, stream);
+#define CSX_PAIRS_CASE(QT_, SM_) \
+  else if (qt == QT_ && sin_mode == SM_) launch_pairs_t<QT_, SM_>(grid, pts, items, qrev, nq_pad, partial, stream);
+      if (false) {
+      }
+      CSX_PAIRS_CASE(8, 0) CSX_PAIRS_CASE(8, 1) CSX_PAIRS_CASE(8, 2)
+      CSX_PAIRS_CASE(16, 0) CSX_PAIRS_CASE(16, 1) CSX_PAIRS_CASE(16, 2)
+      CSX_PAIRS_CASE(32, 0) CSX_PAIRS_CASE(32, 1) CSX_PAIRS_CASE(32, 2)
+      CSX_PAIRS_CASE(64, 0) CSX_PAIRS_CASE(64, 1) CSX_PAIRS_CASE(64, 2)
       else return hipErrorInvalidValue;
+#undef CSX_PAIRS_CASE
     }
     scope.stop();
   }

@@ -116,9 +116,11 @@ csx_status csx_ctx_kernel_time(csx_ctx *ctx, const char *name,
 csx_status csx_ctx_reset_timing(csx_ctx *ctx);
 
 /* Launch shape of the FP32 pair kernel (no reference counterpart; chosen by
- * the autotuner): q_tile in {8, 16, 32} q values per pair sweep, sin_mode 0
- * (software polynomial) or 1 (hardware V_SIN_F32), j_chunk atoms per work
- * item (multiple of 256). */
+ * the autotuner): q_tile in {8, 16, 32, 64} q values per pair sweep;
+ * sin_mode 0 (software polynomial after exact reduction in revolutions),
+ * 1 (hardware V_SIN_F32 after the same reduction) or 2 (V_SIN_F32 on the
+ * unreduced argument; used only when |q| r_max / pi <= 250 and otherwise
+ * replaced by 1); j_chunk atoms per work item (multiple of 256). */
 csx_status csx_ctx_set_tuning(csx_ctx *ctx, int q_tile, int sin_mode,
                               int j_chunk);
 

@@ -170,7 +170,8 @@ def test_deterministic(gpu_ctx):
     assert np.array_equal(a, b)
 
 
-@pytest.mark.parametrize("q_tile,sin_mode,j_chunk", [(8, 0, 256), (32, 0, 4096), (16, 1, 2048)])
+@pytest.mark.parametrize("q_tile,sin_mode,j_chunk", [(8, 0, 256), (32, 0, 4096), (16, 1, 2048),
+                                                     (64, 2, 512), (64, 0, 512)])
 def test_tuning_variants_agree(gpu_ctx, q_tile, sin_mode, j_chunk):
     s = csx.generate_globule(1500, 18.0, 1.0, seed=6)
     q = np.linspace(0, 1.0, 77)
@@ -180,10 +181,10 @@ def test_tuning_variants_agree(gpu_ctx, q_tile, sin_mode, j_chunk):
         s2 = csx.Structure.from_arrays(s.xyz, s.element)
         got = csx.debye_intensity(s2, q)
     finally:
-        gpu_ctx.set_tuning(16, 0, 2048)
+        gpu_ctx.set_tuning()
     err = rel(got, want).max()
     print("tuning", q_tile, sin_mode, j_chunk, err, flush=True)
-    assert err < (1e-5 if sin_mode == 0 else 1e-4)
+    assert err < 1e-5
 
 
 def test_precision_study_20k_against_fp64_gpu(gpu_ctx):
