@@ -38,6 +38,37 @@ class ComputeOptionsC(C.Structure):
     ]
 
 
+class FitParamsC(C.Structure):
+    _fields_ = [("scale", C.c_double), ("background", C.c_double), ("c1", C.c_double),
+                ("c2", C.c_double)]
+
+
+class FitOptionsC(C.Structure):
+    _fields_ = [
+        ("method", C.c_int32),
+        ("implicit_hydrogen", C.c_int32),
+        ("c1_bounds", C.c_double * 2),
+        ("c2_bounds", C.c_double * 2),
+        ("max_iterations", C.c_int32),
+        ("q_sampling_stride", C.c_int32),
+        ("solvent", SolventParamsC),
+        ("background_mode", C.c_int32),
+        ("background_value", C.c_double),
+        ("scale_mode", C.c_int32),
+        ("scale_value", C.c_double),
+        ("optimizer", C.c_int32),
+        ("precision", C.c_int32),
+    ]
+
+
+class FitResultC(C.Structure):
+    _fields_ = [("params", FitParamsC), ("chi2", C.c_double), ("iterations", C.c_int64),
+                ("converged", C.c_int32), ("termination", C.c_char * 96)]
+
+
+_fo = C.POINTER(FitOptionsC)
+_fp = C.POINTER(FitParamsC)
+
 # name -> (restype, argtypes); every symbol declared in include/crabsaxs.h.
 SIGNATURES = {
     "csx_last_error": (C.c_char_p, []),
@@ -91,6 +122,31 @@ SIGNATURES = {
     "csx_compute_curve": (C.c_int, [C.c_void_p, c_double_p, C.c_size_t,
                                     C.POINTER(ComputeOptionsC), c_double_p,
                                     C.POINTER(C.c_int)]),
+    "csx_debye_grad_x": (C.c_int, [C.c_void_p, c_double_p, C.c_size_t, c_double_p,
+                                   C.POINTER(SolventParamsC), C.c_int, c_double_p]),
+    "csx_debye_jacobian": (C.c_int, [C.c_void_p, c_double_p, C.c_size_t,
+                                     C.POINTER(SolventParamsC), C.c_int, c_double_p]),
+    "csx_fit_options_default": (None, [_fo]),
+    "csx_solve_scale_background": (C.c_int, [c_double_p, c_double_p, c_double_p, C.c_size_t,
+                                             C.c_int, C.c_double, C.c_int, C.c_double,
+                                             c_double_p]),
+    "csx_cubic_hermite_interpolate": (C.c_int, [c_double_p, c_double_p, C.c_size_t, c_double_p,
+                                                C.c_size_t, c_double_p]),
+    "csx_fit_basis": (C.c_int, [C.c_void_p, c_double_p, C.c_size_t, _fo, c_double_p]),
+    "csx_fit_to_experimental": (C.c_int, [C.c_void_p, c_double_p, c_double_p, c_double_p,
+                                          C.c_size_t, _fp, _fo, C.POINTER(FitResultC),
+                                          c_double_p]),
+    "csx_fit_from_basis": (C.c_int, [c_double_p, c_double_p, c_double_p, c_double_p, C.c_size_t,
+                                     _fp, _fo, C.POINTER(FitResultC), c_double_p]),
+    "csx_chi2_grad": (C.c_int, [c_double_p, c_double_p, c_double_p, c_double_p, C.c_size_t, _fp,
+                                c_double_p, c_double_p]),
+    "csx_project_simplex": (C.c_int, [c_double_p, C.c_size_t, c_double_p]),
+    "csx_optimize_weights": (C.c_int, [c_double_p, C.c_size_t, c_double_p, c_double_p,
+                                       C.c_size_t, _fo, C.c_int, C.c_double, c_double_p,
+                                       c_double_p, c_double_p]),
+    "csx_ensemble_fit_core": (C.c_int, [c_double_p, C.c_size_t, c_double_p, c_double_p,
+                                        c_double_p, C.c_size_t, _fo, C.c_int, C.c_double,
+                                        c_double_p, _fp, c_double_p, c_double_p]),
     "csx_generate_globule": (C.c_int, [C.c_size_t, C.c_double, C.c_double, C.c_uint64,
                                        c_double_p, C.POINTER(C.c_uint8)]),
     "csx_generate_domain_chain": (C.c_int, [C.c_size_t, C.c_int, C.c_uint64, c_double_p,

@@ -372,6 +372,33 @@ def debye_pair_tables(structure: Structure, q_values, params: SolventParams = No
     return out.reshape(nb, q.size), self_terms
 
 
+def debye_grad_x(structure: Structure, q_values, weights, params: SolventParams = None,
+                 implicit_hydrogen: bool = True, ctx: Context = None) -> np.ndarray:
+    """sum_q weights[q] dI(q)/dx for every atom: array [n, 3] (adjoint kernel)."""
+    q = f64(q_values).reshape(-1)
+    w = f64(weights).reshape(-1)
+    if w.size != q.size:
+        raise InvalidInput("weights and q differ in length")
+    out = _out(3 * len(structure))
+    pc = params.to_c() if params is not None else None
+    _check(_lib.load().csx_debye_grad_x(structure.handle(ctx), dptr(q), q.size, dptr(w),
+                                        C.byref(pc) if pc is not None else None,
+                                        int(implicit_hydrogen), dptr(out)))
+    return out.reshape(len(structure), 3)
+
+
+def debye_jacobian(structure: Structure, q_values, params: SolventParams = None,
+                   implicit_hydrogen: bool = True, ctx: Context = None) -> np.ndarray:
+    """dI(q)/dx: array [nq, n, 3] (adjoint kernel)."""
+    q = f64(q_values).reshape(-1)
+    out = _out(q.size * 3 * len(structure))
+    pc = params.to_c() if params is not None else None
+    _check(_lib.load().csx_debye_jacobian(structure.handle(ctx), dptr(q), q.size,
+                                          C.byref(pc) if pc is not None else None,
+                                          int(implicit_hydrogen), dptr(out)))
+    return out.reshape(q.size, len(structure), 3)
+
+
 @dataclass
 class MultipoleParams:
     """`MultipoleParams` (src/spherical.rs:19-31)."""

@@ -33,8 +33,10 @@ SOURCES = [
     ("capi.cpp", "host"),
     ("multipole.cpp", "host"),
     ("fit.cpp", "host"),
+    ("adjoint.cpp", "host"),
     ("debye.hip", "device"),
     ("multipole.hip", "device"),
+    ("adjoint.hip", "device"),
 ]
 
 

@@ -128,6 +128,18 @@ static csx_status build_point_set(csx_structure* s, const csx_solvent_params* p,
     occ64[n + k] = 1.0;
     type64[n + k] = T;
   }
+  std::vector<int> orig32(npts), type32(npts);
+  for (size_t k = 0; k < n; ++k) {
+    orig32[k] = static_cast<int>(order[k]);
+    type32[k] = s->type_of[order[k]];
+  }
+  for (int k = 0; k < ps.nsites; ++k) {
+    orig32[n + k] = static_cast<int>(n) + k;
+    type32[n + k] = T;
+  }
+  CSX_HIP(ps.orig32.upload(orig32.data(), orig32.size() * sizeof(int)), "upload order");
+  CSX_HIP(ps.type32.upload(type32.data(), type32.size() * sizeof(int)), "upload types");
+  CSX_HIP(ps.seg32.upload(seg_start.data(), seg_start.size() * sizeof(int)), "upload segments");
   CSX_HIP(ps.pts32.upload(pts32.data(), pts32.size() * sizeof(float)), "upload points");
   CSX_HIP(ps.items.upload(items.data(), items.size() * sizeof(int)), "upload items");
   CSX_HIP(ps.bin_first.upload(bin_first.data(), bin_first.size() * sizeof(int)), "upload bins");

@@ -122,6 +122,9 @@ struct PointSet {
   DevArray bin_first;         // int[nbins + 1]
   int nitems = 0, nbins = 0, j_chunk = 0;
   double max_radius = 0.0;    // of the FP32 points about the centroid
+  DevArray orig32;            // int[npts]: original index of each sorted point
+  DevArray type32;            // int[npts]: type of each sorted point
+  DevArray seg32;             // int[ntypes + 1]: type segments of the sorted order
   // FP64 plain layout: original order, atoms then sites.
   DevArray pts64;             // double4[natoms + nsites]
   DevArray type64;            // int[natoms + nsites]

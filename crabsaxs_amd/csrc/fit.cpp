@@ -238,6 +238,20 @@ static OptResult nelder_mead(const Objective& f, double x1, double x2, double st
 static OptResult gradient_fit(const Objective& f, double x1, double x2, int max_iters) {
   const double lo[2] = {f.b1[0], f.b2[0]}, hi[2] = {f.b1[1], f.b2[1]};
   const double span[2] = {hi[0] - lo[0], hi[1] - lo[1]};
+  // Start from the best of the initial point and a 5 x 5 grid over the box:
+  // where the fitted scale is clamped at zero chi^2 is flat in (c1, c2) and a
+  // gradient method cannot leave that region by itself.
+  double best = f.cost(x1, x2);
+  for (int i = 0; i < 5; ++i)
+    for (int j = 0; j < 5; ++j) {
+      const double a = lo[0] + span[0] * i / 4.0, b = lo[1] + span[1] * j / 4.0;
+      const double c = f.cost(a, b);
+      if (c < best) {
+        best = c;
+        x1 = a;
+        x2 = b;
+      }
+    }
   double x[2] = {x1, x2}, g[2];
   double fx = f.cost_grad(x[0], x[1], g);
   double H[2][2] = {{span[0] * span[0], 0}, {0, span[1] * span[1]}};  // inverse Hessian

@@ -250,6 +250,25 @@ csx_status csx_compute_curve(csx_structure *s, const double *q, size_t nq,
                              const csx_compute_options *options, double *out,
                              int *method_used);
 
+/* ---- adjoint (no reference counterpart) -------------------------------- */
+/* Derivatives of the Debye intensity of debye_intensity_with_hydrogens
+ * (p == NULL) or of solvent_corrected_intensity with Analytic hydration
+ * (solvent.rs:490-500, c1 and c2 fixed) with respect to the atom
+ * coordinates, before the >= 0 clamp: FP32 pair terms, FP64 accumulation.
+ * Grid hydration models are rejected (their sites are not differentiable). */
+
+/* grad[3*i + c] = sum_q weights[q] dI(q)/dx_{i,c}  (vector-Jacobian product;
+ * with weights = d chi^2 / d I(q) this is d chi^2 / dx). */
+csx_status csx_debye_grad_x(csx_structure *s, const double *q, size_t nq,
+                            const double *weights,
+                            const csx_solvent_params *p,
+                            int implicit_hydrogen, double *grad);
+
+/* jac[(k*n + i)*3 + c] = dI(q_k)/dx_{i,c}: nq * n * 3 doubles. */
+csx_status csx_debye_jacobian(csx_structure *s, const double *q, size_t nq,
+                              const csx_solvent_params *p,
+                              int implicit_hydrogen, double *jac);
+
 /* ---- fitting (src/fit.rs) ---------------------------------------------- */
 
 /* FitParams, fit.rs:11-17. */

@@ -366,6 +366,69 @@ void or_combine_solvent_basis(const double *b, const double *q, size_t nq, doubl
     }
 }
 
+/* ---------------------------------------------------- Debye gradient -- */
+
+struct grad_ctx {
+    const double *xyz, *occ, *q, *w;
+    const int *el, *grp;
+    size_t n;
+    int implicit_h, solvent;
+    double c1, c2, t, rho;
+    double *per_q; /* nq * 3n */
+};
+
+/* d I(q)/dx of the Debye sum with amplitudes A_i = v - G e + c2 h (solvent)
+ * or A_i = v: 2 sum_j A_i A_j q sinc'(q r_ij) (x_i - x_j) / r_ij, restated
+ * from the derivative of scattering.rs:100-106 / solvent.rs:533-558. */
+static void grad_task(void *p, size_t k) {
+    const struct grad_ctx *c = (const struct grad_ctx *)p;
+    const double q = c->q[k];
+    const size_t n = c->n;
+    double *g = c->per_q + k * 3 * n;
+    for (size_t i = 0; i < 3 * n; ++i) g[i] = 0.0;
+    if (q == 0.0) return;
+    double *A = (double *)malloc(sizeof(double) * (n ? n : 1));
+    struct basis_ctx b = {c->xyz, c->occ, c->q, NULL, c->el, c->grp, n, 0, 0, c->c1, c->c2,
+                          c->t, c->rho, 0.0, 0, c->implicit_h, NULL};
+    double *v = (double *)malloc(sizeof(double) * 3 * (n ? n : 1));
+    amplitudes(&b, q, v, v + n, v + 2 * n);
+    const double G = or_excluded_volume_scale(q, c->c1);
+    for (size_t i = 0; i < n; ++i)
+        A[i] = c->solvent ? v[i] - G * v[n + i] + c->c2 * v[2 * n + i] : v[i];
+    for (size_t i = 0; i < n; ++i)
+        for (size_t j = i + 1; j < n; ++j) {
+            const double d[3] = {c->xyz[3 * i] - c->xyz[3 * j], c->xyz[3 * i + 1] - c->xyz[3 * j + 1],
+                                 c->xyz[3 * i + 2] - c->xyz[3 * j + 2]};
+            const double r = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+            if (r < 1e-10) continue;
+            const double u = q * r;
+            const double dsinc = (u * cos(u) - sin(u)) / (u * u);
+            const double f = 2.0 * A[i] * A[j] * q * dsinc / r;
+            for (int a = 0; a < 3; ++a) {
+                g[3 * i + a] += f * d[a];
+                g[3 * j + a] -= f * d[a];
+            }
+        }
+    free(A);
+    free(v);
+}
+
+void or_debye_grad_x(const double *xyz, const int *element, const int *group, const double *occ,
+                     size_t n, const double *q, size_t nq, const double *weights, int solvent,
+                     double c1, double c2, double thickness, double rho, int implicit_h,
+                     int threads, double *grad, double *jac) {
+    double *per_q = jac ? jac : (double *)malloc(sizeof(double) * (nq ? nq : 1) * 3 * (n ? n : 1));
+    struct grad_ctx c = {xyz, occ, q, weights, element, group, n, implicit_h, solvent,
+                         c1, c2, thickness, rho, per_q};
+    parallel_q(grad_task, &c, nq, threads);
+    if (grad) {
+        for (size_t i = 0; i < 3 * n; ++i) grad[i] = 0.0;
+        for (size_t k = 0; k < nq; ++k)
+            for (size_t i = 0; i < 3 * n; ++i) grad[i] += weights[k] * per_q[k * 3 * n + i];
+    }
+    if (!jac) free(per_q);
+}
+
 /* -------------------------------------------------------------- grids -- */
 
 struct box {

@@ -46,6 +46,15 @@ void or_debye_solvent_basis(const double *xyz, const int *element, const int *gr
                             double voxel_volume, int implicit_h, int threads,
                             double *out);
 
+/* Coordinate derivatives of the Debye intensity (vacuum, or Analytic solvent
+ * with amplitude v - G(c1) e + c2 h): jac[k][i][c] = dI(q_k)/dx_{i,c} (may be
+ * NULL) and grad[i][c] = sum_k weights[k] jac[k][i][c] (may be NULL). */
+void or_debye_grad_x(const double *xyz, const int *element, const int *group,
+                     const double *occ, size_t n, const double *q, size_t nq,
+                     const double *weights, int solvent, double c1, double c2,
+                     double thickness, double rho, int implicit_h, int threads,
+                     double *grad, double *jac);
+
 /* combine_solvent_basis, spherical.rs:567-584. */
 void or_combine_solvent_basis(const double *basis, const double *q, size_t nq,
                               double c1, double c2, double *out);

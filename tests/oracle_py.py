@@ -74,6 +74,9 @@ def lib() -> C.CDLL:
                                           C.c_double, C.c_int, C.c_double, C.c_int, C.c_double,
                                           _d, _d]
         L.or_project_simplex.argtypes = [_d, C.c_size_t, _d]
+        L.or_debye_grad_x.argtypes = [_d, _i, _i, _d, C.c_size_t, _d, C.c_size_t, _d, C.c_int,
+                                      C.c_double, C.c_double, C.c_double, C.c_double, C.c_int,
+                                      C.c_int, _d, _d]
         _lib = L
     return _lib
 
@@ -128,6 +131,23 @@ def debye_solvent_basis(atoms: Atoms, q, c1=1.0, c2=0.0, thickness=3.0, rho=0.33
                                  _p(s), s.shape[0] if s.ndim == 2 else 0, voxel_volume,
                                  int(implicit_h), threads, _p(out))
     return out.reshape(6, q.size)
+
+
+def debye_grad_x(atoms: Atoms, q, weights=None, solvent=None, implicit_h=True, threads=0,
+                 jacobian=False):
+    """solvent: None (vacuum) or (c1, c2, thickness, rho)."""
+    q = _f(q)
+    w = _f(np.ones(q.size) if weights is None else weights)
+    n = len(atoms)
+    grad = np.zeros(3 * n)
+    jac = np.zeros(q.size * 3 * n) if jacobian else None
+    c1, c2, t, rho = solvent if solvent is not None else (1.0, 0.0, 3.0, 0.334)
+    lib().or_debye_grad_x(*atoms.args(), _p(q), q.size, _p(w), int(solvent is not None), c1, c2,
+                          t, rho, int(implicit_h), threads, _p(grad),
+                          _p(jac) if jacobian else None)
+    if jacobian:
+        return jac.reshape(q.size, n, 3)
+    return grad.reshape(n, 3)
 
 
 def combine_solvent_basis(basis, q, c1, c2):

@@ -1,0 +1,156 @@
+"""Host-side fitting and ensemble arithmetic of the library (no GPU needed),
+checked against the FP64 oracle, finite differences and the reference's own
+fit tests (fit.rs:489-660) with bases computed by the oracle."""
+import math
+
+import numpy as np
+import pytest
+
+import crabsaxs_amd as csx
+import oracle_py as O
+
+
+def curve(q, y, s):
+    return csx.ExperimentalCurve(np.asarray(q), np.asarray(y), np.asarray(s))
+
+
+@pytest.mark.parametrize("scale_mode,bg_mode", [(0, 0), (0, 1), (0, 2), (1, 0), (1, 1), (1, 2)])
+def test_solve_scale_background_bitwise(scale_mode, bg_mode):
+    rng = np.random.default_rng(1)
+    x = rng.uniform(1, 100, 40)
+    y = 1.3 * x + 2 + rng.normal(size=40)
+    s = rng.uniform(0.5, 2, 40)
+    sm = csx.ScaleMode.Fixed(1.25) if scale_mode else csx.ScaleMode.Fit
+    bm = [csx.BackgroundMode.Fit, csx.BackgroundMode.FixedZero, csx.BackgroundMode.Fixed(1.5)][bg_mode]
+    got = csx.solve_scale_background(x, curve(x, y, s), sm, bm)
+    want = O.solve_scale_background(x, y, s, scale_mode, 1.25, bg_mode, 1.5)
+    assert got == tuple(float(v) for v in want)
+
+
+def test_hermite_simplex_weights_bitwise():
+    rng = np.random.default_rng(2)
+    x = np.sort(rng.uniform(0, 1, 12))
+    y = rng.normal(size=12)
+    t = np.linspace(x[0], x[-1], 57)
+    assert np.array_equal(csx.cubic_hermite_interpolate(x, y, t), O.cubic_hermite(x, y, t))
+    v = rng.normal(size=9)
+    assert np.array_equal(csx.project_simplex(v), O.project_simplex(v))
+    q = np.linspace(0.01, 0.5, 60)
+    curves = np.stack([np.exp(-(q * r) ** 2 / 3) * 1e4 for r in (10, 15, 20, 30)])
+    yv = 0.7 * curves[0] + 0.3 * curves[3] + 3.0
+    sg = 0.01 * yv
+    for uniform in (False, True):
+        w, sc, bg, chi, fitted = csx.optimize_weights(curves, curve(q, yv, sg), uniform=uniform,
+                                                      regularization=0.1)
+        ow, (osc, obg, ochi) = O.optimize_weights(curves, yv, sg, uniform=uniform,
+                                                  regularization=0.1)
+        assert np.array_equal(w, ow) and (sc, bg, chi) == (osc, obg, ochi)
+        assert np.allclose(fitted, sc * (w @ curves) + bg, rtol=1e-14)
+
+
+def synthetic_basis(q):
+    atoms = O.Atoms([[0, 0, 0], [2, 0, 0]], [1, 3])
+    return O.debye_solvent_basis(atoms, q, implicit_h=False)
+
+
+def test_chi2_gradient_matches_finite_differences():
+    q = np.linspace(0.01, 0.5, 50)
+    b = synthetic_basis(q)
+    y = 1.7 * O.combine_solvent_basis(b, q, 1.03, 0.4) + 0.02
+    e = curve(q, y * (1 + 0.01 * np.sin(40 * q)), 0.01 * y + 1e-3)
+    p = csx.FitParams(scale=1.6, background=0.05, c1=1.01, c2=0.3)
+    chi, g = csx.chi2_grad(b, e, p)
+    h = [1e-6, 1e-8, 1e-7, 1e-7]
+    for k, name in enumerate(["scale", "background", "c1", "c2"]):
+        up, dn = csx.FitParams(**vars(p)), csx.FitParams(**vars(p))
+        setattr(up, name, getattr(p, name) + h[k])
+        setattr(dn, name, getattr(p, name) - h[k])
+        fd = (csx.chi2_grad(b, e, up)[0] - csx.chi2_grad(b, e, dn)[0]) / (2 * h[k])
+        assert abs(fd - g[k]) <= 1e-5 * max(abs(fd), 1e-3), (name, fd, g[k])
+
+
+@pytest.mark.parametrize("optimizer", [csx.Optimizer.NelderMead, csx.Optimizer.Gradient])
+def test_exact_recovery_from_basis(optimizer):
+    q = np.linspace(0.01, 0.5, 80)
+    b = synthetic_basis(q)
+    y = 1.7 * O.combine_solvent_basis(b, q, 1.03, 0.4) + 0.02
+    e = curve(q, y, np.full(q.size, 0.01))
+    opts = csx.FitOptions(optimizer=optimizer, max_iterations=200)
+    r = csx.fit_from_basis(b, e, csx.FitParams(), opts)
+    assert abs(r.params.c1 - 1.03) < 1e-3 and abs(r.params.c2 - 0.4) < 1e-2, r
+    assert abs(r.params.scale - 1.7) < 1e-2 and r.chi2 < 1e-4, r
+
+
+def lcg_noise(n):
+    """The deterministic LCG + Box-Muller noise of fit.rs:555-566."""
+    state = 0x5EED
+    out = []
+    for _ in range(n):
+        vals = []
+        for _ in range(2):
+            state = (state * 6364136223846793005 + 1) & 0xFFFFFFFFFFFFFFFF
+            vals.append(max((state >> 11) / float(1 << 53), 2.2250738585072014e-308))
+        u1, u2 = vals
+        out.append(math.sqrt(-2.0 * math.log(u1)) * math.cos(2.0 * math.pi * u2))
+    return np.array(out)
+
+
+@pytest.mark.parametrize("optimizer", [csx.Optimizer.NelderMead, csx.Optimizer.Gradient])
+def test_recovers_solvent_parameters_with_deterministic_noise(optimizer):
+    # fit.rs:536-599 with the oracle's multipole basis (two atoms, explicit H)
+    atoms = O.Atoms([[0, 0, 0], [2, 0, 0]], [1, 3])
+    q = np.arange(1, 81) * 0.006
+    raw = O.multipole_solvent(atoms, q, 8, c1=1.02, c2=0.55, implicit_h=False, exact=False)
+    y = 1.4e-4 * raw + 0.03 + 0.0001 * lcg_noise(q.size)
+    e = curve(q, y, np.full(q.size, 0.002))
+    basis = O.multipole_solvent(atoms, q, 9, basis=True, implicit_h=False, exact=False)
+    opts = csx.FitOptions(method=csx.ScatteringMethod.Multipole,
+                          hydrogen_mode=csx.HydrogenMode.Explicit, max_iterations=100,
+                          q_sampling_stride=1, optimizer=optimizer)
+    r = csx.fit_from_basis(basis, e, csx.FitParams(1.4e-4, 0.03, 1.02, 0.55), opts)
+    if optimizer == csx.Optimizer.Gradient:
+        # chi^2 has a flat c1/c2 valley here; the gradient fit must reach a chi^2
+        # no worse than Nelder-Mead's (it may stop elsewhere in the valley).
+        opts.optimizer = csx.Optimizer.NelderMead
+        nm = csx.fit_from_basis(basis, e, csx.FitParams(1.4e-4, 0.03, 1.02, 0.55), opts)
+        assert r.chi2 <= nm.chi2 * (1 + 1e-6) and r.chi2 < 0.01, (r, nm)
+        return
+    assert abs(r.params.c1 - 1.02) < 0.02, r
+    assert abs(r.params.c2 - 0.55) < 0.08, r
+    assert abs(r.params.scale - 1.4e-4) < 0.4e-4, r
+    assert abs(r.params.background - 0.03) < 0.01, r
+    assert r.chi2 < 0.01, r
+
+
+def test_nelder_mead_respects_bounds_and_penalty():
+    q = np.linspace(0.01, 0.5, 64)
+    b = synthetic_basis(q)
+    y = 1.0 * O.combine_solvent_basis(b, q, 1.05, 4.0) + 0.0
+    e = curve(q, y, np.full(q.size, 0.01))
+    r = csx.fit_from_basis(b, e, csx.FitParams(c1=2.0, c2=9.0), csx.FitOptions())
+    assert 0.95 <= r.params.c1 <= 1.05 and -2.0 <= r.params.c2 <= 4.0
+    assert r.iterations <= 64
+
+
+def test_invalid_fit_options_raise():
+    q = np.linspace(0.01, 0.5, 10)
+    b = synthetic_basis(q)
+    e = curve(q, np.ones(10), np.ones(10))
+    with pytest.raises(csx.InvalidInput):
+        csx.fit_from_basis(b, e, csx.FitParams(), csx.FitOptions(c1_bounds=(1.1, 1.0)))
+    with pytest.raises(csx.InvalidInput):
+        csx.fit_from_basis(b, e, csx.FitParams(), csx.FitOptions(max_iterations=0))
+    with pytest.raises(csx.InvalidInput):
+        csx.fit_from_basis(b, curve(q, np.ones(10), np.zeros(10)), csx.FitParams(), csx.FitOptions())
+
+
+def test_compare_curves_and_quality_presets():
+    q = np.linspace(0.01, 0.3, 30)
+    y = np.exp(-q * 10)
+    e = curve(q, y, np.full(30, 0.01))
+    c = csx.compare_curves(e, y * 1.01)
+    assert c.r_factor == pytest.approx(0.01, rel=1e-9) and c.pearson_correlation > 0.999
+    assert csx.Quality.Fast.fit_options(csx.FitOptions()).max_iterations == 32
+    assert csx.Quality.Accurate.fit_options(csx.FitOptions()).q_sampling_stride == 1
+    with pytest.raises(csx.InvalidInput):
+        csx.compare_curves(e, y[:5])

@@ -256,3 +256,34 @@ def test_optimize_weights_properties():
     chi_want = np.sum(((s_want * x + b_want - y) / sigma) ** 2) / (len(q) - 1)
     assert w1[0] == 1.0 and abs(s1 - s_want) < 1e-12 * s_want and abs(b1 - b_want) < 1e-9
     assert abs(c1 - chi_want) < 1e-9 * chi_want
+
+
+@pytest.mark.parametrize("solvent", [None, (1.03, 0.7, 3.0, 0.334)])
+def test_oracle_gradient_matches_central_differences(solvent):
+    rng = np.random.default_rng(8)
+    xyz = rng.normal(size=(12, 3)) * 4
+    el = rng.integers(1, 5, size=12)
+    q = np.array([0.01, 0.1, 0.35, 0.8])
+    jac = O.debye_grad_x(O.Atoms(xyz, el), q, solvent=solvent, jacobian=True)
+
+    def intensity(x):
+        a = O.Atoms(x, el)
+        if solvent is None:
+            return O.debye_intensity(a, q)
+        c1, c2, t, rho = solvent
+        return O.combine_solvent_basis(O.debye_solvent_basis(a, q, c1, c2, t, rho), q, c1, c2)
+
+    h = 1e-4
+    for i in (0, 5, 11):
+        for c in range(3):
+            up, dn = xyz.copy(), xyz.copy()
+            up[i, c] += h
+            dn[i, c] -= h
+            fd = (intensity(up) - intensity(dn)) / (2 * h)
+            scale = np.abs(jac[:, :, :]).max(axis=(1, 2))
+            assert np.all(np.abs(fd - jac[:, i, c]) <= 1e-6 * scale + 1e-9), (i, c, fd, jac[:, i, c])
+    w = rng.normal(size=q.size)
+    g = O.debye_grad_x(O.Atoms(xyz, el), q, weights=w, solvent=solvent)
+    assert np.allclose(g, np.einsum("k,kic->ic", w, jac), rtol=1e-12, atol=1e-12)
+    # translation invariance: the gradients sum to zero
+    assert np.abs(jac.sum(axis=1)).max() <= 1e-10 * np.abs(jac).max()
