@@ -122,6 +122,13 @@ SIGNATURES = {
     "csx_compute_curve": (C.c_int, [C.c_void_p, c_double_p, C.c_size_t,
                                     C.POINTER(ComputeOptionsC), c_double_p,
                                     C.POINTER(C.c_int)]),
+    "csx_ctx_set_persistence": (C.c_int, [C.c_void_p, C.c_int]),
+    "csx_ctx_device_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.c_char_p, C.c_size_t]),
+    "csx_batch_debye_intensity": (C.c_int, [C.POINTER(C.c_void_p), C.c_size_t, c_double_p,
+                                            C.c_size_t, C.c_int, c_double_p]),
+    "csx_batch_debye_solvent_basis": (C.c_int, [C.POINTER(C.c_void_p), C.c_size_t, c_double_p,
+                                                C.c_size_t, C.POINTER(SolventParamsC), C.c_int,
+                                                c_double_p]),
     "csx_debye_grad_x": (C.c_int, [C.c_void_p, c_double_p, C.c_size_t, c_double_p,
                                    C.POINTER(SolventParamsC), C.c_int, c_double_p]),
     "csx_debye_jacobian": (C.c_int, [C.c_void_p, c_double_p, C.c_size_t,

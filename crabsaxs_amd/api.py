@@ -208,6 +208,19 @@ class Context:
     def set_tuning(self, q_tile: int = 64, sin_mode: int = 2, j_chunk: int = 512) -> None:
         _check(_lib.load().csx_ctx_set_tuning(self.ptr, q_tile, sin_mode, j_chunk))
 
+    def cu_count(self) -> int:
+        n = C.c_int()
+        _check(_lib.load().csx_ctx_device_info(self.ptr, C.byref(n), None, 0))
+        return int(n.value)
+
+    def arch(self) -> str:
+        buf = C.create_string_buffer(256)
+        _check(_lib.load().csx_ctx_device_info(self.ptr, None, buf, 256))
+        return buf.value.decode()
+
+    def set_persistence(self, workgroups_per_cu: int = 0) -> None:
+        _check(_lib.load().csx_ctx_set_persistence(self.ptr, workgroups_per_cu))
+
     def close(self):
         if self.ptr:
             _lib.load().csx_ctx_destroy(self.ptr)
@@ -370,6 +383,36 @@ def debye_pair_tables(structure: Structure, q_values, params: SolventParams = No
     _check(lib.csx_debye_pair_tables(h, dptr(q), q.size, pp, dptr(out), dptr(self_terms),
                                      C.byref(nt)))
     return out.reshape(nb, q.size), self_terms
+
+
+def _handles(structures: Sequence[Structure], ctx):
+    arr = (C.c_void_p * len(structures))()
+    for k, s in enumerate(structures):
+        arr[k] = s.handle(ctx).value
+    return arr
+
+
+def batch_debye_intensity(structures: Sequence[Structure], q_values,
+                          implicit_hydrogen: bool = True, ctx: Context = None) -> np.ndarray:
+    """Vacuum Debye profiles of many structures from one kernel launch: [m, nq]."""
+    q = f64(q_values).reshape(-1)
+    out = _out(len(structures) * q.size)
+    _check(_lib.load().csx_batch_debye_intensity(_handles(structures, ctx), len(structures),
+                                                 dptr(q), q.size, int(implicit_hydrogen),
+                                                 dptr(out)))
+    return out.reshape(len(structures), q.size)
+
+
+def batch_debye_solvent_basis(structures: Sequence[Structure], q_values, params: SolventParams,
+                              implicit_hydrogen: bool = True, ctx: Context = None) -> np.ndarray:
+    """Six-term Debye bases of many structures from one kernel launch: [m, 6, nq]."""
+    q = f64(q_values).reshape(-1)
+    out = _out(len(structures) * 6 * q.size)
+    p = params.to_c()
+    _check(_lib.load().csx_batch_debye_solvent_basis(_handles(structures, ctx), len(structures),
+                                                     dptr(q), q.size, C.byref(p),
+                                                     int(implicit_hydrogen), dptr(out)))
+    return out.reshape(len(structures), 6, q.size)
 
 
 def debye_grad_x(structure: Structure, q_values, weights, params: SolventParams = None,

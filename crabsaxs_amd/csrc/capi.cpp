@@ -111,6 +111,8 @@ static csx_status build_point_set(csx_structure* s, const csx_solvent_params* p,
       }
     }
   bin_first.push_back(static_cast<int>(items.size() / 4));
+  ps.h_items = items;
+  ps.h_bin_first = bin_first;
   ps.nitems = static_cast<int>(items.size() / 4);
   ps.nbins = TT * (TT + 1) / 2;
   ps.j_chunk = chunk;
@@ -166,55 +168,132 @@ csx_status get_point_set(csx_structure* s, const csx_solvent_params* p, PointSet
 
 // --------------------------------------------------------- pair tables --
 
-// P[b*nq + k] = sum_{i<j in bin b} occ_i occ_j sinc(q_k r_ij).
-static csx_status pair_tables_f32(Call& call, const PointSet& ps, const double* q, size_t nq,
-                                  std::vector<double>& P) {
+// Pair tables of one or more point sets from one launch of the pair kernel
+// (a batch concatenates the sets' points, work items and bins):
+// P[c][b*nq + k] = sum_{i<j in bin b of set c} occ_i occ_j sinc(q_k r_ij).
+static csx_status pair_tables_batch(Call& call, const std::vector<const PointSet*>& sets,
+                                    const double* q, size_t nq,
+                                    std::vector<std::vector<double>>& P) {
   const Tuning tune = call.ctx()->tuning;
   const int qt = tune.q_tile;
   const size_t nq_pad = (nq + qt - 1) / qt * qt;
+  double rmax = 0.0;
+  size_t npts = 0, nitems = 0, nbins = 0;
+  for (const PointSet* ps : sets) {
+    rmax = std::max(rmax, ps->max_radius);
+    npts += static_cast<size_t>(ps->natoms + ps->nsites);
+    nitems += static_cast<size_t>(ps->nitems);
+    nbins += static_cast<size_t>(ps->nbins);
+  }
+  if (npts > static_cast<size_t>(INT32_MAX / 2) || nitems > static_cast<size_t>(INT32_MAX / 2) ||
+      nbins > static_cast<size_t>(INT32_MAX / 2))
+    return fail(CSX_INVALID_INPUT, "batch too large for one launch");
   std::vector<float> qrev(nq_pad, 0.0f);
   double umax = 0.0;
   for (size_t k = 0; k < nq; ++k) {
     qrev[k] = static_cast<float>(q[k] / (2.0 * kPi));
-    umax = std::max(umax, std::fabs(static_cast<double>(qrev[k])) * 2.0 * ps.max_radius);
+    umax = std::max(umax, std::fabs(static_cast<double>(qrev[k])) * 2.0 * rmax);
   }
   // The unreduced hardware sine takes |q r / 2 pi| <= 256 revolutions.
   const int sin_mode = tune.sin_mode == 2 && !(umax <= 250.0) ? 1 : tune.sin_mode;
   const hipStream_t st = call.stream();
-  DevBuf d_q, d_partial, d_out;
+  DevBuf d_q, d_partial, d_out, d_counter, d_pts, d_items, d_bins;
   CSX_HIP(d_q.alloc(nq_pad * sizeof(float), st), "hipMallocAsync");
-  CSX_HIP(d_partial.alloc(size_t(ps.nitems) * nq_pad * sizeof(double), st), "hipMallocAsync");
-  CSX_HIP(d_out.alloc(size_t(ps.nbins) * nq_pad * sizeof(double), st), "hipMallocAsync");
+  CSX_HIP(d_partial.alloc(nitems * nq_pad * sizeof(double), st), "hipMallocAsync");
+  CSX_HIP(d_out.alloc(nbins * nq_pad * sizeof(double), st), "hipMallocAsync");
+  CSX_HIP(d_counter.alloc(sizeof(int), st), "hipMallocAsync");
   CSX_HIP(hipMemcpyAsync(d_q.as<void>(), qrev.data(), nq_pad * sizeof(float),
                          hipMemcpyHostToDevice, st),
           "hipMemcpyAsync");
-  CSX_HIP(launch_debye_pairs(ps.pts32.as<float4>(), ps.items.as<int4>(), ps.nitems,
-                             ps.bin_first.as<int>(), ps.nbins, d_q.as<float>(),
-                             static_cast<int>(nq_pad), qt, sin_mode, d_partial.as<double>(),
-                             d_out.as<double>(), st, call.timer()),
+  const float4* pts = nullptr;
+  const int4* items = nullptr;
+  const int* bins = nullptr;
+  std::vector<int> h_items, h_bins;
+  if (sets.size() == 1) {
+    pts = sets[0]->pts32.as<float4>();
+    items = sets[0]->items.as<int4>();
+    bins = sets[0]->bin_first.as<int>();
+  } else {
+    CSX_HIP(d_pts.alloc(npts * sizeof(float4), st), "hipMallocAsync");
+    h_items.reserve(4 * nitems);
+    h_bins.reserve(nbins + 1);
+    size_t off = 0, item_off = 0;
+    for (const PointSet* ps : sets) {
+      const size_t np = static_cast<size_t>(ps->natoms + ps->nsites);
+      CSX_HIP(hipMemcpyAsync(d_pts.as<float4>() + off, ps->pts32.as<void>(), np * sizeof(float4),
+                             hipMemcpyDeviceToDevice, st),
+              "hipMemcpyAsync");
+      for (size_t k = 0; k < ps->h_items.size(); k += 4) {
+        h_items.push_back(ps->h_items[k] + static_cast<int>(off));
+        h_items.push_back(ps->h_items[k + 1]);
+        h_items.push_back(ps->h_items[k + 2] + static_cast<int>(off));
+        h_items.push_back(ps->h_items[k + 3]);
+      }
+      for (int b = 0; b < ps->nbins; ++b)
+        h_bins.push_back(ps->h_bin_first[b] + static_cast<int>(item_off));
+      off += np;
+      item_off += static_cast<size_t>(ps->nitems);
+    }
+    h_bins.push_back(static_cast<int>(item_off));
+    CSX_HIP(d_items.alloc(h_items.size() * sizeof(int), st), "hipMallocAsync");
+    CSX_HIP(d_bins.alloc(h_bins.size() * sizeof(int), st), "hipMallocAsync");
+    CSX_HIP(hipMemcpyAsync(d_items.as<void>(), h_items.data(), h_items.size() * sizeof(int),
+                           hipMemcpyHostToDevice, st),
+            "hipMemcpyAsync");
+    CSX_HIP(hipMemcpyAsync(d_bins.as<void>(), h_bins.data(), h_bins.size() * sizeof(int),
+                           hipMemcpyHostToDevice, st),
+            "hipMemcpyAsync");
+    pts = d_pts.as<float4>();
+    items = d_items.as<int4>();
+    bins = d_bins.as<int>();
+  }
+  const int persist_wgs = tune.persist > 0 ? tune.persist * call.ctx()->cu_count : 0;
+  CSX_HIP(launch_debye_pairs(pts, items, static_cast<int>(nitems), bins, static_cast<int>(nbins),
+                             d_q.as<float>(), static_cast<int>(nq_pad), qt, sin_mode, persist_wgs,
+                             d_counter.as<int>(), d_partial.as<double>(), d_out.as<double>(), st,
+                             call.timer()),
           "debye_pairs_f32");
-  std::vector<double> raw(size_t(ps.nbins) * nq_pad);
+  std::vector<double> raw(nbins * nq_pad);
   CSX_HIP(hipMemcpyAsync(raw.data(), d_out.as<void>(), raw.size() * sizeof(double),
                          hipMemcpyDeviceToHost, st),
           "hipMemcpyAsync");
   d_q.reset();
   d_partial.reset();
   d_out.reset();
+  d_counter.reset();
+  d_pts.reset();
+  d_items.reset();
+  d_bins.reset();
   CSX_TRY(call.finish());
-  P.assign(size_t(ps.nbins) * nq, 0.0);
-  const int TT = ps.ntypes;
-  int b = 0;
-  for (int a = 0; a < TT; ++a)
-    for (int c = a; c < TT; ++c, ++b)
-      for (size_t k = 0; k < nq; ++k) {
-        const double qe = 2.0 * kPi * static_cast<double>(qrev[k]);
-        if (qe == 0.0) {
-          P[b * nq + k] = a == c ? 0.5 * (ps.wsum[a] * ps.wsum[a] - ps.self[a])
-                                 : ps.wsum[a] * ps.wsum[c];
-        } else {
-          P[b * nq + k] = raw[size_t(b) * nq_pad + k] / qe;
+  P.assign(sets.size(), std::vector<double>());
+  size_t bin_off = 0;
+  for (size_t c = 0; c < sets.size(); ++c) {
+    const PointSet& ps = *sets[c];
+    std::vector<double>& Pc = P[c];
+    Pc.assign(size_t(ps.nbins) * nq, 0.0);
+    const int TT = ps.ntypes;
+    int b = 0;
+    for (int a = 0; a < TT; ++a)
+      for (int t = a; t < TT; ++t, ++b)
+        for (size_t k = 0; k < nq; ++k) {
+          const double qe = 2.0 * kPi * static_cast<double>(qrev[k]);
+          if (qe == 0.0) {
+            Pc[b * nq + k] = a == t ? 0.5 * (ps.wsum[a] * ps.wsum[a] - ps.self[a])
+                                    : ps.wsum[a] * ps.wsum[t];
+          } else {
+            Pc[b * nq + k] = raw[(bin_off + b) * nq_pad + k] / qe;
+          }
         }
-      }
+    bin_off += static_cast<size_t>(ps.nbins);
+  }
+  return CSX_OK;
+}
+
+static csx_status pair_tables_f32(Call& call, const PointSet& ps, const double* q, size_t nq,
+                                  std::vector<double>& P) {
+  std::vector<std::vector<double>> all;
+  CSX_TRY(pair_tables_batch(call, {&ps}, q, nq, all));
+  P = std::move(all[0]);
   return CSX_OK;
 }
 
@@ -398,6 +477,10 @@ csx_status csx_ctx_create(int device, csx_ctx** out) {
   csx_ctx* ctx = new (std::nothrow) csx_ctx;
   if (!ctx) return fail(CSX_OUT_OF_MEMORY, "context allocation failed");
   ctx->device = device;
+  int cus = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess &&
+      cus > 0)
+    ctx->cu_count = cus;
   *out = ctx;
   return CSX_OK;
 }
@@ -647,6 +730,83 @@ csx_status csx_debye_pair_tables(csx_structure* s, const double* q, size_t nq,
   std::vector<double> P;
   CSX_TRY(pair_tables_f32(call, *ps, q, nq, P));
   std::copy(P.begin(), P.end(), out);
+  return CSX_OK;
+}
+
+// Batched Debye profiles: one launch of the pair kernel over every
+// conformer's work items.  p == NULL: vacuum intensities (out: m*nq);
+// otherwise six-term bases (out: m*6*nq).
+static csx_status batch_debye(csx_structure* const* structures, size_t m, const double* q,
+                              size_t nq, const csx_solvent_params* p, bool implicit_h,
+                              double* out) {
+  if (m == 0) return CSX_OK;
+  if (!structures || (nq > 0 && (!q || !out)))
+    return fail(CSX_INVALID_INPUT, "structures, q and output buffers are required");
+  csx_ctx* ctx = structures[0] ? structures[0]->ctx : nullptr;
+  for (size_t c = 0; c < m; ++c)
+    if (!structures[c] || structures[c]->ctx != ctx)
+      return fail(CSX_INVALID_INPUT, "all structures must be non-null and share one context");
+  if (nq == 0) return CSX_OK;
+  Call call(ctx);
+  CSX_TRY(call.begin());
+  std::vector<const PointSet*> sets(m);
+  for (size_t c = 0; c < m; ++c) {
+    PointSet* ps = nullptr;
+    CSX_TRY(get_point_set(structures[c], p, &ps));
+    sets[c] = ps;
+  }
+  std::vector<std::vector<double>> P;
+  CSX_TRY(pair_tables_batch(call, sets, q, nq, P));
+  for (size_t c = 0; c < m; ++c) {
+    std::vector<double> V, E, H;
+    type_amplitudes(structures[c], *sets[c], q, nq, implicit_h, p, V, E, H);
+    if (!p) {
+      double* o = out + c * nq;
+      contract(*sets[c], P[c], nq, V, V, o);
+      for (size_t k = 0; k < nq; ++k) o[k] = std::max(o[k], 0.0);
+      continue;
+    }
+    double* o = out + c * 6 * nq;
+    contract(*sets[c], P[c], nq, V, V, o + 0 * nq);
+    contract(*sets[c], P[c], nq, E, E, o + 1 * nq);
+    contract(*sets[c], P[c], nq, H, H, o + 2 * nq);
+    contract(*sets[c], P[c], nq, V, E, o + 3 * nq);
+    contract(*sets[c], P[c], nq, V, H, o + 4 * nq);
+    contract(*sets[c], P[c], nq, E, H, o + 5 * nq);
+  }
+  return CSX_OK;
+}
+
+csx_status csx_batch_debye_intensity(csx_structure* const* structures, size_t m, const double* q,
+                                     size_t nq, int implicit_h, double* out) {
+  return batch_debye(structures, m, q, nq, nullptr, implicit_h != 0, out);
+}
+
+csx_status csx_batch_debye_solvent_basis(csx_structure* const* structures, size_t m,
+                                         const double* q, size_t nq, const csx_solvent_params* p,
+                                         int implicit_h, double* out) {
+  if (!p) return fail(CSX_INVALID_INPUT, "solvent parameters are required");
+  return batch_debye(structures, m, q, nq, p, implicit_h != 0, out);
+}
+
+csx_status csx_ctx_device_info(csx_ctx* ctx, int* cu_count, char* arch, size_t arch_len) {
+  if (!ctx) return fail(CSX_INVALID_INPUT, "context is null");
+  if (cu_count) *cu_count = ctx->cu_count;
+  if (arch && arch_len) {
+    hipDeviceProp_t prop;
+    const hipError_t e = hipGetDeviceProperties(&prop, ctx->device);
+    if (e != hipSuccess) return hip_fail(e, "hipGetDeviceProperties");
+    std::snprintf(arch, arch_len, "%s", prop.gcnArchName);
+  }
+  return CSX_OK;
+}
+
+csx_status csx_ctx_set_persistence(csx_ctx* ctx, int workgroups_per_cu) {
+  if (!ctx) return fail(CSX_INVALID_INPUT, "context is null");
+  if (workgroups_per_cu < 0 || workgroups_per_cu > 64)
+    return fail(CSX_INVALID_INPUT, "workgroups_per_cu must be in [0, 64]");
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  ctx->tuning.persist = workgroups_per_cu;
   return CSX_OK;
 }
 

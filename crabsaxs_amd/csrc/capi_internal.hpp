@@ -32,6 +32,7 @@ struct Tuning {
   int q_tile = 64;     // q values per sweep of a pair
   int sin_mode = 2;    // 0 software polynomial, 1 reduced + V_SIN_F32, 2 V_SIN_F32
   int j_chunk = 512;   // j atoms per work item
+  int persist = 0;     // persistent launch with this many workgroups per CU (0: grid)
 };
 
 }  // namespace csx
@@ -43,6 +44,7 @@ struct csx_ctx {
   std::atomic<bool> timing{false};
   std::map<std::string, std::pair<int64_t, double>> stats;
   csx::Tuning tuning;
+  int cu_count = 256;
 };
 
 namespace csx {
@@ -125,6 +127,7 @@ struct PointSet {
   DevArray orig32;            // int[npts]: original index of each sorted point
   DevArray type32;            // int[npts]: type of each sorted point
   DevArray seg32;             // int[ntypes + 1]: type segments of the sorted order
+  std::vector<int> h_items, h_bin_first;  // host copies (batch concatenation)
   // FP64 plain layout: original order, atoms then sites.
   DevArray pts64;             // double4[natoms + nsites]
   DevArray type64;            // int[natoms + nsites]

@@ -178,16 +178,13 @@ __device__ inline float sin2pi(float u) {
   return sin2pi_hw_direct(u);
 }
 
+// One work item (type-pure block of 256 i atoms x chunk of j atoms) for the
+// QT q values starting at qc; writes partial_row[qc .. qc + QT).
 template <int QT, int SINMODE>
-__global__ __launch_bounds__(kBlock) void debye_pairs_f32(const float4* __restrict__ pts,
-                                                          const int4* __restrict__ items,
-                                                          const float* __restrict__ qrev,
-                                                          int nq_pad,
-                                                          double* __restrict__ partial) {
+__device__ __attribute__((always_inline)) inline void pair_item(const float4* __restrict__ pts, const int4 item, const int qc,
+                                 const float* __restrict__ qrev, double* __restrict__ partial_row) {
   __shared__ float4 tile[kBlock];
   __shared__ double scratch[4 * QT];
-  const int4 item = items[blockIdx.x];  // i0, ilen, j0, jlen
-  const int qc = blockIdx.y * QT;
   const int i = item.x + threadIdx.x;
   const bool vi = threadIdx.x < item.y;
   const float4 pi = vi ? pts[i] : make_float4(0.f, 0.f, 0.f, 0.f);
@@ -238,16 +235,50 @@ __global__ __launch_bounds__(kBlock) void debye_pairs_f32(const float4* __restri
   __syncthreads();
   if (threadIdx.x < QT) {
     const int k = threadIdx.x;
-    partial[size_t(blockIdx.x) * nq_pad + qc + k] =
+    partial_row[qc + k] =
         ((scratch[k] + scratch[QT + k]) + scratch[2 * QT + k]) + scratch[3 * QT + k];
   }
 }
 
+// One workgroup per (work item, q chunk).
+template <int QT, int SINMODE>
+__global__ __launch_bounds__(kBlock) void debye_pairs_f32(const float4* __restrict__ pts,
+                                                          const int4* __restrict__ items,
+                                                          const float* __restrict__ qrev,
+                                                          int nq_pad,
+                                                          double* __restrict__ partial) {
+  pair_item<QT, SINMODE>(pts, items[blockIdx.x], blockIdx.y * QT, qrev,
+                         partial + size_t(blockIdx.x) * nq_pad);
+}
+
+// Persistent form: a fixed grid of workgroups takes (work item, q chunk)
+// tasks from a global counter until all `total` tasks are done; every
+// workgroup exits once the counter passes `total`.
+template <int QT, int SINMODE>
+__global__ __launch_bounds__(kBlock) void debye_pairs_f32_persistent(
+    const float4* __restrict__ pts, const int4* __restrict__ items, const float* __restrict__ qrev,
+    int nq_pad, int total, int* __restrict__ counter, double* __restrict__ partial) {
+  __shared__ int task;
+  const int nqc = nq_pad / QT;
+  while (true) {
+    __syncthreads();
+    if (threadIdx.x == 0) task = atomicAdd(counter, 1);
+    __syncthreads();
+    const int w = task;
+    if (w >= total) break;
+    const int it = w / nqc;
+    pair_item<QT, SINMODE>(pts, items[it], (w - it * nqc) * QT, qrev,
+                           partial + size_t(it) * nq_pad);
+  }
+}
+
+__global__ void reset_counter(int* counter) { *counter = 0; }
+
 // out[b][q] = sum_{items of bin b, in order} partial[item][q].
 __global__ void reduce_bins(const double* __restrict__ partial, const int* __restrict__ bin_first,
                             int nq_pad, double* __restrict__ out) {
-  const int b = blockIdx.y;
-  const int qi = blockIdx.x * blockDim.x + threadIdx.x;
+  const int b = blockIdx.x;
+  const int qi = blockIdx.y * blockDim.x + threadIdx.x;
   if (qi >= nq_pad) return;
   double s = 0.0;
   for (int it = bin_first[b]; it < bin_first[b + 1]; ++it) s += partial[size_t(it) * nq_pad + qi];
@@ -286,22 +317,35 @@ hipError_t launch_expand_amplitudes(const double* table, const int* type, const 
 }
 
 template <int QT, int SM>
-static void launch_pairs_t(dim3 grid, const float4* pts, const int4* items, const float* qrev,
-                           int nq_pad, double* partial, hipStream_t stream) {
-  debye_pairs_f32<QT, SM><<<grid, kBlock, 0, stream>>>(pts, items, qrev, nq_pad, partial);
+static void launch_pairs_t(int nitems, int persist_wgs, int* counter, const float4* pts,
+                           const int4* items, const float* qrev, int nq_pad, double* partial,
+                           hipStream_t stream) {
+  const int nqc = nq_pad / QT;
+  if (persist_wgs > 0) {
+    const int total = nitems * nqc;
+    reset_counter<<<1, 1, 0, stream>>>(counter);
+    debye_pairs_f32_persistent<QT, SM><<<std::min(persist_wgs, total), kBlock, 0, stream>>>(
+        pts, items, qrev, nq_pad, total, counter, partial);
+  } else {
+    debye_pairs_f32<QT, SM><<<dim3(nitems, nqc), kBlock, 0, stream>>>(pts, items, qrev, nq_pad,
+                                                                       partial);
+  }
 }
 
 hipError_t launch_debye_pairs(const float4* pts, const int4* items, int nitems, const int* bin_first,
                               int nbins, const float* qrev, int nq_pad, int qt, int sin_mode,
-                              double* partial, double* out, hipStream_t stream,
-                              KernelTimer* timer) {
+                              int persist_wgs, int* counter, double* partial, double* out,
+                              hipStream_t stream, KernelTimer* timer) {
   if (nq_pad % qt != 0) return hipErrorInvalidValue;
-  dim3 grid(nitems, nq_pad / qt);
+  if (persist_wgs > 0 && (!counter || static_cast<long long>(nitems) * (nq_pad / qt) > INT32_MAX))
+    persist_wgs = 0;
   {
     TimerScope scope(timer, "debye_pairs_f32", stream);
     if (nitems > 0) {
-#define CSX_PAIRS_CASE(QT_, SM_) \
-  else if (qt == QT_ && sin_mode == SM_) launch_pairs_t<QT_, SM_>(grid, pts, items, qrev, nq_pad, partial, stream);
+#define CSX_PAIRS_CASE(QT_, SM_)                                                              \
+  else if (qt == QT_ && sin_mode == SM_) launch_pairs_t<QT_, SM_>(nitems, persist_wgs, counter, \
+                                                                  pts, items, qrev, nq_pad,    \
+                                                                  partial, stream);
       if (false) {
       }
       CSX_PAIRS_CASE(8, 0) CSX_PAIRS_CASE(8, 1) CSX_PAIRS_CASE(8, 2)
@@ -314,7 +358,7 @@ hipError_t launch_debye_pairs(const float4* pts, const int4* items, int nitems, 
     scope.stop();
   }
   if (nbins > 0) {
-    dim3 rgrid((nq_pad + 255) / 256, nbins);
+    dim3 rgrid(nbins, (nq_pad + 255) / 256);
     reduce_bins<<<rgrid, 256, 0, stream>>>(partial, bin_first, nq_pad, out);
   }
   return hipGetLastError();

@@ -58,8 +58,11 @@ hipError_t launch_debye_plain(const double4* pts, const double* amp, const doubl
 hipError_t launch_expand_amplitudes(const double* table, const int* type, const double* occ,
                                     int npts, int ntypes, int nq, int nch, double* amp,
                                     hipStream_t stream);
+// persist_wgs > 0: persistent launch of that many workgroups over a task
+// counter (device int, reset by the launcher); 0: one workgroup per task.
 hipError_t launch_debye_pairs(const float4* pts, const int4* items, int nitems, const int* bin_first,
                               int nbins, const float* qrev, int nq_pad, int qt, int sin_mode,
-                              double* partial, double* out, hipStream_t stream, KernelTimer* timer);
+                              int persist_wgs, int* counter, double* partial, double* out,
+                              hipStream_t stream, KernelTimer* timer);
 
 }  // namespace csx

@@ -447,10 +447,18 @@ class EnsembleFitter:
         """Per-conformer bases [m, 6, n] (computed once)."""
         if self._bases is None:
             import copy
+            from .api import batch_debye_solvent_basis
             o = copy.deepcopy(self.options.fit)
             o.q_sampling_stride = 1
-            self._bases = np.stack([fit_basis(c, self.experimental.q, o, ctx)
-                                    for c in self.conformers])
+            if (o.method == ScatteringMethod.Debye
+                    and o.precision == Precision.FP32_COMPENSATED):
+                # every conformer's pair tables from one launch
+                self._bases = batch_debye_solvent_basis(
+                    self.conformers, self.experimental.q, o.solvent,
+                    o.hydrogen_mode == HydrogenMode.Implicit, ctx)
+            else:
+                self._bases = np.stack([fit_basis(c, self.experimental.q, o, ctx)
+                                        for c in self.conformers])
         return self._bases
 
     def _fit_core(self, selected, data: ExperimentalCurve, ctx):

@@ -1,0 +1,63 @@
+"""Seeded synthetic workloads of the benchmark configurations.
+
+  globule_1200      1,200 atoms, R = 16 A, q = 0.005..0.5 step 0.005 (100)
+  precision_20k     20,000 atoms, R = 35 A, q = linspace(0, 1, 500)
+  prolate_100k      100,000 atoms in a 2:1 prolate spheroid of the volume of a
+                    60 A sphere (semi-axes 47.6, 47.6, 95.2 A) plus hydration
+                    pseudo-atoms on a 3 A grid 3-6 A from the nearest atom;
+                    q = linspace(0.005, 0.5, 200)
+  fit_target_20k    precision_20k structure, 300 q = linspace(0.01, 1, 300),
+                    target with scale 2.3, bg 1e-3, c1 1.02, c2 2.0 and
+                    Gaussian noise sigma = 1% I + 1e-4 I(0)
+  ensemble_512      512 conformers, atom counts uniform in [2000, 30000],
+                    alternately a globule (density of the 20k globule) or a
+                    2-4 domain chain; q = linspace(0.005, 0.5, 200)
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from .api import (Element, SolventParams, Structure, generate_domain_chain, generate_globule,
+                  generate_shell_sites, solvent_corrected_intensity)
+
+PRECISION_Q = np.linspace(0.0, 1.0, 500)
+
+
+def globule_1200(seed: int = 0):
+    return generate_globule(1200, 16.0, 1.0, seed=seed), np.arange(1, 101) * 0.005
+
+
+def precision_20k(seed: int = 1):
+    return generate_globule(20000, 35.0, 1.0, seed=seed), PRECISION_Q.copy()
+
+
+def prolate_100k(seed: int = 2, n: int = 100000):
+    s = generate_globule(n, 60.0 / 2 ** (1 / 3), 2.0, seed=seed)
+    sites = generate_shell_sites(s, 3.0, 3.0, 6.0)
+    xyz = np.vstack([s.xyz, sites])
+    el = np.concatenate([s.element, np.full(len(sites), int(Element.O), dtype=np.uint8)])
+    return Structure.from_arrays(xyz, el), np.linspace(0.005, 0.5, 200), len(sites)
+
+
+def fit_target_20k(seed: int = 1, noise_seed: int = 7, ctx=None):
+    s, _ = precision_20k(seed)
+    q = np.linspace(0.01, 1.0, 300)
+    truth = SolventParams(c1=1.02, c2=2.0)
+    clean = 2.3 * solvent_corrected_intensity(s, q, truth, ctx=ctx) + 1e-3
+    i0 = clean[0]
+    sigma = 0.01 * clean + 1e-4 * i0
+    rng = np.random.default_rng(noise_seed)
+    return s, q, clean + sigma * rng.normal(size=q.size), sigma
+
+
+def ensemble_512(m: int = 512, seed: int = 4, lo: int = 2000, hi: int = 30000):
+    rng = np.random.default_rng(seed)
+    sizes = rng.integers(lo, hi + 1, size=m)
+    conformers = []
+    for k, n in enumerate(sizes):
+        if k % 2 == 0:
+            radius = 35.0 * (n / 20000.0) ** (1 / 3)
+            conformers.append(generate_globule(int(n), radius, 1.0, seed=seed * 100003 + k))
+        else:
+            conformers.append(generate_domain_chain(int(n), int(2 + k % 3), seed=seed * 100003 + k))
+    return conformers, np.linspace(0.005, 0.5, 200)

@@ -124,6 +124,16 @@ csx_status csx_ctx_reset_timing(csx_ctx *ctx);
 csx_status csx_ctx_set_tuning(csx_ctx *ctx, int q_tile, int sin_mode,
                               int j_chunk);
 
+/* Persistent launch of the FP32 pair kernel: a fixed grid of
+ * workgroups_per_cu x CU workgroups drains a global task counter (0 = one
+ * workgroup per task).  No reference counterpart. */
+csx_status csx_ctx_set_persistence(csx_ctx *ctx, int workgroups_per_cu);
+
+/* Compute-unit count and architecture name (e.g. "gfx950:...") of the
+ * context's device; either output may be NULL. */
+csx_status csx_ctx_device_info(csx_ctx *ctx, int *cu_count, char *arch,
+                               size_t arch_len);
+
 /* ---- host typing (src/formfactor.rs) ---------------------------------- */
 
 /* Element::from_symbol (src/structure.rs:109-127): CSX_EL_* code. */
@@ -211,6 +221,20 @@ csx_status csx_debye_pair_tables(csx_structure *s, const double *q,
                                  size_t nq, const csx_solvent_params *p,
                                  double *out, double *self_terms,
                                  size_t *n_types);
+
+/* Batched conformer profiles (no reference counterpart; replaces the per-
+ * conformer compute_curve calls of EnsembleFitter::fit_core,
+ * ensemble.rs:244-261): the work items of all m structures (any sizes, one
+ * context) go through one launch of the FP32 pair kernel.
+ * Vacuum intensities, out: m*nq (debye_intensity_with_hydrogens each). */
+csx_status csx_batch_debye_intensity(csx_structure *const *structures,
+                                     size_t m, const double *q, size_t nq,
+                                     int implicit_hydrogen, double *out);
+/* Six-term bases, out: m*6*nq (debye_solvent_basis each). */
+csx_status csx_batch_debye_solvent_basis(csx_structure *const *structures,
+                                         size_t m, const double *q, size_t nq,
+                                         const csx_solvent_params *p,
+                                         int implicit_hydrogen, double *out);
 
 /* combine_solvent_basis (src/spherical.rs:567-584).  basis: 6*nq. */
 csx_status csx_combine_solvent_basis(const double *basis, const double *q,

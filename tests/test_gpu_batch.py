@@ -1,0 +1,74 @@
+"""Batched conformer profiles, the persistent launch and the autotuner."""
+import numpy as np
+import pytest
+
+import crabsaxs_amd as csx
+from crabsaxs_amd import autotune
+
+pytestmark = pytest.mark.gpu
+
+
+def conformers():
+    return [csx.generate_globule(n, 10.0 + n / 400, 1.0, seed=60 + n) for n in (300, 1700, 2500)] + \
+           [csx.generate_domain_chain(3200, 3, seed=9)]
+
+
+def test_batch_matches_single_bitwise(gpu_ctx):
+    cs = conformers()
+    q = np.linspace(0.0, 0.8, 90)
+    p = csx.SolventParams(c1=1.01, c2=0.5)
+    got = csx.batch_debye_solvent_basis(cs, q, p)
+    got_i = csx.batch_debye_intensity(cs, q)
+    for k, c in enumerate(cs):
+        assert np.array_equal(got[k], csx.debye_solvent_basis(c, q, p))
+        assert np.array_equal(got_i[k], csx.debye_intensity(c, q))
+    ref = csx.debye_intensity(cs[1], q, precision=csx.Precision.FP64_PLAIN)
+    assert np.max(np.abs(got_i[1] - ref) / ref) < 1e-5
+
+
+def test_persistent_launch_is_bitwise_identical(gpu_ctx):
+    cs = conformers()
+    q = np.linspace(0.0, 1.0, 130)
+    base = csx.batch_debye_intensity(cs, q)
+    try:
+        for wgs in (1, 4, 8):
+            gpu_ctx.set_persistence(wgs)
+            assert np.array_equal(csx.batch_debye_intensity(cs, q), base)
+    finally:
+        gpu_ctx.set_persistence(0)
+
+
+def test_autotuner_picks_a_measured_shape(gpu_ctx, tmp_path, monkeypatch):
+    monkeypatch.setattr(autotune, "CACHE", tmp_path / "tuning.json")
+    s = csx.generate_globule(4000, 20.0, 1.0, seed=3)
+    q = np.linspace(0.0, 1.0, 128)
+    want = csx.debye_intensity(s, q, precision=csx.Precision.FP64_PLAIN)
+    try:
+        best = autotune.autotune(s, q, gpu_ctx, candidates={"q_tile": (32, 64),
+                                                            "j_chunk": (256, 512),
+                                                            "persist": (0, 4)})
+        assert best["tried"] == 8 and best["kernel_ms"] > 0 and not best["cached"]
+        got = csx.debye_intensity(s, q)
+        assert np.max(np.abs(got - want) / want) < 1e-5
+        again = autotune.autotune(s, q, gpu_ctx)
+        assert again["cached"] and again["q_tile"] == best["q_tile"]
+    finally:
+        gpu_ctx.set_tuning()
+        gpu_ctx.set_persistence(0)
+
+
+def test_ensemble_fitter_uses_batched_bases(gpu_ctx):
+    cs = conformers()
+    q = np.linspace(0.01, 0.4, 70)
+    p = csx.SolventParams(c1=1.0, c2=1.0)
+    curves = csx.batch_debye_solvent_basis(cs, q, p)
+    mix = 0.5 * csx.combine_solvent_basis(curves[0], q, 1.0, 1.0) + \
+        0.5 * csx.combine_solvent_basis(curves[2], q, 1.0, 1.0)
+    e = csx.ExperimentalCurve(q, 2.0 * mix + 1.0, 0.01 * mix + 1.0)
+    opts = csx.EnsembleOptions(fit=csx.FitOptions(method=csx.ScatteringMethod.Debye))
+    fitter = csx.EnsembleFitter(cs, ["a", "b", "c", "d"], e, opts)
+    b = fitter.bases()
+    for k, c in enumerate(cs):
+        assert np.array_equal(b[k], csx.debye_solvent_basis(c, q, csx.SolventParams()))
+    r = fitter.fit()
+    assert abs(np.sum(r.weights) - 1) < 1e-12
