@@ -1,16 +1,31 @@
 """Benchmark of the crabSAXS profile hot path on one MI355X.
 
-Workload (one "step"): the Debye precision-study profile -- a seeded 20,000-atom
-globule (R = 35 A), q = linspace(0, 1, 500), default solvent parameters --
-computed with the FP32-compensated per-type pair-table kernel and contracted to
-I(q).  Inputs are resident in HBM before timing starts (the structure is
-uploaded once); each step is one C-ABI call: q upload, pair kernel, FP64 bin
-reduction, contraction, I(q) download.
+Headline workload (one "step"): the Debye precision-study profile -- a seeded
+20,000-atom globule (R = 35 A), q = linspace(0, 1, 500), default solvent
+parameters -- computed with the FP32-compensated per-type pair-table kernel and
+contracted to I(q).  Inputs are resident in HBM before timing starts (the
+structure is uploaded once); each step is one C-ABI call: q upload, pair
+kernel, FP64 bin reduction, contraction, I(q) download.
 
 Prints one JSON line: metric pair.q evals/s (whole job), the dominant kernel's
-roofline (timed with HIP events on its own stream inside the library), the FP64
-GPU Debye baseline with the speedup and max relative error against it, and the
-FP64 CPU oracle timed on a bounded sample on the host cores.
+roofline (timed with HIP events on its own stream inside the library; HBM
+traffic per launch from the committed rocprofv3 counter summary), the FP64 GPU
+Debye baseline with the speedup and max relative error against it, the FP64
+CPU oracle timed on a bounded sample on the host cores, and the other legs of
+the metric:
+
+  ensemble    512 conformers (2,000-30,000 atoms, globules and domain chains),
+              200 q, every conformer's six-term basis from one pair-kernel
+              launch: conformer profiles/s, error against FP64 GPU Debye on
+              sampled conformers, and the weights recovered from a
+              5-conformer Dirichlet(1) mixture with 1% noise
+  multipole   100,000-atom 2:1 prolate globule + hydration pseudo-atoms on a
+              3 A grid 3-6 A out, 200 q, l_max 15 / 25 / 40, against FP64 GPU
+              Debye on the same points
+  adjoint     20k structure, 300 q: dI/dx vector-Jacobian product throughput,
+              checked against a central difference of FP64 GPU Debye; the
+              Nelder-Mead and gradient fits of a noisy target (scale 2.3,
+              bg 1e-3, c1 1.02, c2 2.0) timed to their chi2
 """
 from __future__ import annotations
 
@@ -29,6 +44,9 @@ sys.path.insert(0, str(ROOT))
 N_ATOMS, RADIUS, N_Q, SEED = 20000, 35.0, 500, 1
 VALU_OPS_PER_PAIRQ = 4          # x = q r, sin x, * 1/(q r), accumulate
 PEAK_VALU_OPS = 256 * 64 * 2.4e9  # CUs x lanes x clock (unpacked FP32 issue)
+PEAK_FP32_MFMA = 157.3e12
+TRAFFIC_FILE = ROOT / "profiles" / "hbm_traffic.json"
+LEGS = ("fp64", "cpu", "ensemble", "multipole", "adjoint")
 
 
 def parse():
@@ -36,14 +54,26 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--legs", default=",".join(LEGS),
+                    help="comma-separated extra legs to run: " + ",".join(LEGS) + " (or none)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 GPU baseline")
     ap.add_argument("--sin-mode", type=int, default=None)
     ap.add_argument("--q-tile", type=int, default=None)
     ap.add_argument("--j-chunk", type=int, default=None)
+    ap.add_argument("--conformers", type=int, default=512)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run a device-to-device copy of BYTES (HBM counter calibration)")
-    return ap.parse_args()
+    a = ap.parse_args()
+    a.legs = set() if a.legs in ("", "none") else set(a.legs.split(","))
+    if a.no_cpu:
+        a.legs.discard("cpu")
+    if a.no_fp64:
+        a.legs.discard("fp64")
+    unknown = a.legs - set(LEGS)
+    if unknown:
+        ap.error(f"unknown legs {sorted(unknown)}")
+    return a
 
 
 def copy_calibration(nbytes: int) -> None:
@@ -56,6 +86,20 @@ def copy_calibration(nbytes: int) -> None:
     dst.copy_(src)
     torch.cuda.synchronize()
     print(json.dumps({"copy_calibration_bytes": nbytes, "read": nbytes, "written": nbytes}))
+
+
+def traffic_per_launch(kernel: str):
+    """HBM bytes per launch of `kernel` from the committed counter summary:
+    FETCH_SIZE and WRITE_SIZE (KiB per dispatch) times the bytes per KiB the
+    same counters read for a copy of known size."""
+    try:
+        t = json.loads(TRAFFIC_FILE.read_text())
+        k = t["kernels"][kernel]
+        cal = t["calibration"]
+        return (k["fetch_kib"] * cal["fetch_bytes_per_kib"]
+                + k["write_kib"] * cal["write_bytes_per_kib"])
+    except (OSError, KeyError, ValueError, TypeError):
+        return None
 
 
 def cpu_baseline(structure, q, threads):
@@ -76,6 +120,171 @@ def cpu_baseline(structure, q, threads):
                       f"one q per thread"}
 
 
+def timed(ctx, kernel, fn, repeats=1):
+    """Wall time per call and the kernel's HIP-event time per launch."""
+    ctx.reset_timing()
+    ctx.set_timing(True)
+    t0 = time.perf_counter()
+    for _ in range(repeats):
+        out = fn()
+    dt = (time.perf_counter() - t0) / repeats
+    ctx.set_timing(False)
+    n, ms = ctx.kernel_time(kernel)
+    return out, dt, ms / max(n, 1), n / repeats
+
+
+def rel_err(got, want):
+    e = np.abs(got - want) / np.abs(want)
+    return float(e.max()), float(e.mean())
+
+
+def ensemble_leg(csx, ctx, m, log):
+    from crabsaxs_amd import workloads
+    F64 = csx.Precision.FP64_PLAIN
+    t0 = time.perf_counter()
+    conformers, q = workloads.ensemble_512(m=m)
+    log(f"ensemble: generated {m} conformers in {time.perf_counter() - t0:.1f} s")
+    for c in conformers:
+        c.handle(ctx)
+    sizes = np.array([len(c) for c in conformers])
+    pairq = float(np.sum(sizes * (sizes - 1) / 2)) * q.size
+    p = csx.SolventParams()
+    csx.batch_debye_solvent_basis(conformers, q, p, ctx=ctx)  # point sets, warm-up
+    launch = ctx.last_launch()
+    reps = 3
+    bases, dt, kms, launches = timed(ctx, "debye_pairs_f32",
+                                     lambda: csx.batch_debye_solvent_basis(conformers, q, p,
+                                                                           ctx=ctx), reps)
+    log(f"ensemble: {dt:.3f} s per batch, kernel {kms:.1f} ms")
+    # accuracy on the smallest, median and largest conformer
+    order = np.argsort(sizes)
+    worst = 0.0
+    checked = []
+    for k in (order[0], order[m // 2], order[-1]):
+        got = csx.combine_solvent_basis(bases[k], q, 1.02, 1.0)
+        want = csx.solvent_corrected_intensity(conformers[k], q, csx.SolventParams(c1=1.02, c2=1.0),
+                                               precision=F64, ctx=ctx)
+        e, _ = rel_err(got, want)
+        worst = max(worst, e)
+        checked.append(int(sizes[k]))
+    # weights: Dirichlet(1) mixture of 5 conformers, 1% Gaussian noise
+    rng = np.random.default_rng(2024)
+    pick = np.sort(rng.choice(m, size=5, replace=False))
+    w_true = rng.dirichlet(np.ones(5))
+    # (c1, c2) = (1, 1) is one of the 5 x 5 candidates the ensemble fit tries
+    prof = np.stack([csx.combine_solvent_basis(bases[k], q, 1.0, 1.0) for k in pick])
+    mix = w_true @ prof
+    sigma = 0.01 * mix
+    y = mix + sigma * rng.normal(size=q.size)
+    exp = csx.ExperimentalCurve(q, y, sigma)
+    weights = {"conformers": pick.tolist(), "true": w_true.round(4).tolist()}
+    for solver in (csx.WeightSolver.ProjectedGradient, csx.WeightSolver.NNLS):
+        opts = csx.EnsembleOptions(fit=csx.FitOptions(method=csx.ScatteringMethod.Debye),
+                                   weight_solver=solver)
+        fitter = csx.EnsembleFitter(conformers, [str(k) for k in range(m)], exp, opts)
+        fitter._bases = bases  # the batch above is exactly what bases() computes
+        t1 = time.perf_counter()
+        r = fitter.fit(ctx)
+        fit_s = time.perf_counter() - t1
+        w_all = np.zeros(m)
+        w_all[[int(n) for n in r.conformers]] = r.weights
+        weights[solver.name] = {"recovered": w_all[pick].round(4).tolist(),
+                                "weight_elsewhere": float(w_all.sum() - w_all[pick].sum()),
+                                "reduced_chi2": r.reduced_chi2, "c1": r.params.c1,
+                                "c2": r.params.c2, "fit_s": fit_s}
+        log(f"ensemble: {solver.name} weights {weights[solver.name]}")
+    return {
+        "workload": f"{m} conformers, atoms U[2000, 30000] (mean {sizes.mean():.0f}), "
+                    "globules and 2-4 domain chains, q=linspace(0.005,0.5,200), one launch",
+        "value": m / dt, "unit": "conformer profiles/s", "pair_q_per_s": pairq / dt,
+        "batch_s": dt, "kernel_ms": kms, "launches_per_batch": launches,
+        "kernel_pair_q_per_s": pairq / (kms / 1e3),
+        "valu_frac": pairq / (kms / 1e3) * VALU_OPS_PER_PAIRQ / PEAK_VALU_OPS,
+        "launch": launch,
+        "max_rel_err_vs_fp64_gpu": worst, "checked_atoms": checked,
+        "weights": weights,
+    }
+
+
+def multipole_leg(csx, ctx, log):
+    from crabsaxs_amd import workloads
+    F64 = csx.Precision.FP64_PLAIN
+    s, q, nsites = workloads.prolate_100k()
+    n = len(s)
+    pairq = n * (n - 1) / 2 * q.size
+    s.handle(ctx)
+    ref, dt64, ms64, _ = timed(ctx, "debye_plain_f64",
+                               lambda: csx.debye_intensity(s, q, precision=F64, ctx=ctx))
+    log(f"multipole: FP64 Debye {dt64:.2f} s on {n} points")
+    csx.debye_intensity(s, q, ctx=ctx)
+    d32, dt32, ms32, _ = timed(ctx, "debye_pairs_f32", lambda: csx.debye_intensity(s, q, ctx=ctx), 2)
+    out = {"workload": f"100,000-atom 2:1 prolate globule + {nsites} hydration pseudo-atoms "
+                       "(3 A grid, 3-6 A out), q=linspace(0.005,0.5,200), vacuum I(q)",
+           "points": n, "fp64_debye": {"pair_q_per_s": pairq / dt64, "kernel_ms": ms64},
+           "fp32_debye": {"pair_q_per_s": pairq / dt32, "kernel_ms": ms32,
+                          "max_rel_err": rel_err(d32, ref)[0]},
+           "l_max": {}}
+    for L in (15, 25, 40):
+        mp = csx.MultipoleParams(l_max=L)
+        csx.multipole_intensity(s, q, mp, ctx=ctx)
+        got, dt, kms, _ = timed(ctx, "multipole_coeff",
+                                lambda: csx.multipole_intensity(s, q, mp, ctx=ctx), 3)
+        e_max, e_mean = rel_err(got, ref)
+        flops = 2.0 * n * q.size * (L + 1) ** 2  # one real-harmonic channel
+        out["l_max"][str(L)] = {"wall_ms": dt * 1e3, "kernel_ms": kms,
+                                "equiv_pair_q_per_s": pairq / dt,
+                                "speedup_vs_fp64_debye": dt64 / dt,
+                                "mfma_tflops_algorithmic": flops / (kms / 1e3) / 1e12,
+                                "max_rel_err": e_max, "mean_rel_err": e_mean}
+        log(f"multipole: L={L} {dt * 1e3:.1f} ms, max rel err {e_max:.2e}")
+    return out
+
+
+def adjoint_leg(csx, ctx, log):
+    from crabsaxs_amd import workloads
+    F64 = csx.Precision.FP64_PLAIN
+    s, q, y, sigma = workloads.fit_target_20k(ctx=ctx)
+    n = len(s)
+    pairq = n * (n - 1) / 2 * q.size
+    truth = csx.SolventParams(c1=1.02, c2=2.0)
+    rng = np.random.default_rng(5)
+    w = rng.normal(size=q.size) / np.maximum(csx.solvent_corrected_intensity(s, q, truth, ctx=ctx),
+                                             1e-30)
+    csx.debye_grad_x(s, q, w, truth, ctx=ctx)
+    g, dt, kms, _ = timed(ctx, "debye_adjoint_f32", lambda: csx.debye_grad_x(s, q, w, truth, ctx=ctx),
+                          3)
+    # directional derivative against a central difference of FP64 GPU Debye
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d)
+    h = 1e-4
+    xyz = s.xyz
+
+    def shifted(sign):
+        t = csx.Structure.from_arrays(xyz + sign * h * d, s.element, s.occupancy, s.group)
+        return w @ csx.solvent_corrected_intensity(t, q, truth, precision=F64, ctx=ctx)
+    fd = (shifted(1.0) - shifted(-1.0)) / (2 * h)
+    ad = float(np.sum(g * d))
+    log(f"adjoint: {kms:.1f} ms, directional {ad:.6e} vs FD {fd:.6e}")
+    e = csx.ExperimentalCurve(q, y, sigma)
+    fits = {}
+    for opt in (csx.Optimizer.NelderMead, csx.Optimizer.Gradient):
+        o = csx.FitOptions(method=csx.ScatteringMethod.Debye, optimizer=opt, q_sampling_stride=1,
+                           max_iterations=200)
+        csx.fit_to_experimental_with_options(s, e, csx.FitParams(), o, ctx)  # point sets
+        t0 = time.perf_counter()
+        r = csx.fit_to_experimental_with_options(s, e, csx.FitParams(), o, ctx)
+        fits[opt.name] = {"s": time.perf_counter() - t0, "reduced_chi2": r.chi2,
+                          "iterations": r.iterations,
+                          "params": [r.params.scale, r.params.background, r.params.c1, r.params.c2]}
+    return {"workload": "20,000-atom globule, q=linspace(0.01,1,300), target scale 2.3, bg 1e-3, "
+                        "c1 1.02, c2 2.0, noise 1% I + 1e-4 I(0)",
+            "vjp_kernel_ms": kms, "vjp_wall_ms": dt * 1e3,
+            "vjp_pair_q_per_s": pairq / (kms / 1e3),
+            "directional_derivative": {"adjoint": ad, "central_difference_fp64": fd,
+                                       "h_angstrom": h, "rel_diff": abs(ad - fd) / abs(fd)},
+            "fits": fits}
+
+
 def main():
     args = parse()
     if args.copy_calibration:
@@ -87,8 +296,11 @@ def main():
         sys.exit(2)
     import crabsaxs_amd as csx
 
+    def log(msg):
+        print(msg, file=sys.stderr, flush=True)
+
     ctx = csx.default_context()
-    tune = dict(q_tile=64, sin_mode=2, j_chunk=512)
+    tune = dict(q_tile=0, sin_mode=2, j_chunk=512)
     if args.sin_mode is not None:
         tune["sin_mode"] = args.sin_mode
     if args.q_tile is not None:
@@ -113,6 +325,8 @@ def main():
     dt = time.perf_counter() - t0
     ctx.set_timing(False)
     launches, kernel_ms = ctx.kernel_time("debye_pairs_f32")
+    launch = ctx.last_launch()
+    log(f"headline: {dt / args.steps * 1e3:.2f} ms per step, launch {launch}")
     pairq = N_ATOMS * (N_ATOMS - 1) / 2 * N_Q
     ms_per_step = dt / args.steps * 1e3
     value = pairq * args.steps / dt
@@ -135,27 +349,32 @@ def main():
         "config": {"workload": "Debye precision study: seeded 20,000-atom globule, R=35 A, "
                                "q=linspace(0,1,500), default solvent, per-type S_ab tables",
                    "atoms": N_ATOMS, "q_points": N_Q, "pairs": N_ATOMS * (N_ATOMS - 1) // 2,
-                   "tuning": tune},
+                   "tuning": tune, "launch": launch},
         "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": PEAK_VALU_OPS / 1e12,
-                     "unit": "T VALU-op/s", "frac": achieved / PEAK_VALU_OPS, "traffic": None,
+                     "unit": "T VALU-op/s", "frac": achieved / PEAK_VALU_OPS,
+                     "traffic": traffic_per_launch("debye_pairs_f32"),
                      "kernel": "debye_pairs_f32", "kernel_ms": avg_kernel_s * 1e3,
                      "ops_per_pairq": VALU_OPS_PER_PAIRQ},
     }
-    if not args.no_fp64:
-        ctx.reset_timing()
-        ctx.set_timing(True)
-        t1 = time.perf_counter()
-        ref = csx.solvent_corrected_intensity(s, q, params, precision=F64, ctx=ctx)
-        dt64 = time.perf_counter() - t1
-        ctx.set_timing(False)
-        _, ms64 = ctx.kernel_time("debye_plain_f64")
-        err = np.abs(out - ref) / np.abs(ref)
+    if "fp64" in args.legs:
+        ref, dt64, ms64, _ = timed(ctx, "debye_plain_f64",
+                                   lambda: csx.solvent_corrected_intensity(s, q, params,
+                                                                           precision=F64, ctx=ctx))
+        e_max, e_mean = rel_err(out, ref)
         result["fp64_gpu"] = {"value": pairq / dt64, "unit": "pair.q evals/s",
                               "kernel_ms": ms64, "speedup": value / (pairq / dt64),
-                              "max_rel_err": float(err.max()), "mean_rel_err": float(err.mean())}
-    if not args.no_cpu:
+                              "max_rel_err": e_max, "mean_rel_err": e_mean}
+        log(f"fp64: {dt64:.3f} s, max rel err {e_max:.2e}")
+    if "cpu" in args.legs:
         threads = min(os.cpu_count() or 1, 16)
         result["cpu_baseline"] = cpu_baseline(s, q, threads)
+        log("cpu baseline done")
+    if "ensemble" in args.legs:
+        result["ensemble"] = ensemble_leg(csx, ctx, args.conformers, log)
+    if "multipole" in args.legs:
+        result["multipole"] = multipole_leg(csx, ctx, log)
+    if "adjoint" in args.legs:
+        result["adjoint"] = adjoint_leg(csx, ctx, log)
     print(json.dumps(result), flush=True)
 
 

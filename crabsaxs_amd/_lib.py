@@ -124,6 +124,8 @@ SIGNATURES = {
                                     C.POINTER(C.c_int)]),
     "csx_ctx_set_persistence": (C.c_int, [C.c_void_p, C.c_int]),
     "csx_ctx_device_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.c_char_p, C.c_size_t]),
+    "csx_ctx_last_launch": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                      C.POINTER(C.c_int)]),
     "csx_batch_debye_intensity": (C.c_int, [C.POINTER(C.c_void_p), C.c_size_t, c_double_p,
                                             C.c_size_t, C.c_int, c_double_p]),
     "csx_batch_debye_solvent_basis": (C.c_int, [C.POINTER(C.c_void_p), C.c_size_t, c_double_p,
@@ -151,9 +153,11 @@ SIGNATURES = {
     "csx_optimize_weights": (C.c_int, [c_double_p, C.c_size_t, c_double_p, c_double_p,
                                        C.c_size_t, _fo, C.c_int, C.c_double, c_double_p,
                                        c_double_p, c_double_p]),
+    "csx_nnls_weights": (C.c_int, [c_double_p, C.c_size_t, c_double_p, c_double_p, C.c_size_t,
+                                   _fo, c_double_p, c_double_p, c_double_p]),
     "csx_ensemble_fit_core": (C.c_int, [c_double_p, C.c_size_t, c_double_p, c_double_p,
                                         c_double_p, C.c_size_t, _fo, C.c_int, C.c_double,
-                                        c_double_p, _fp, c_double_p, c_double_p]),
+                                        C.c_int, c_double_p, _fp, c_double_p, c_double_p]),
     "csx_generate_globule": (C.c_int, [C.c_size_t, C.c_double, C.c_double, C.c_uint64,
                                        c_double_p, C.POINTER(C.c_uint8)]),
     "csx_generate_domain_chain": (C.c_int, [C.c_size_t, C.c_int, C.c_uint64, c_double_p,

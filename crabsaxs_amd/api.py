@@ -205,8 +205,15 @@ class Context:
         _check(_lib.load().csx_ctx_kernel_time(self.ptr, name.encode(), C.byref(n), C.byref(ms)))
         return int(n.value), float(ms.value)
 
-    def set_tuning(self, q_tile: int = 64, sin_mode: int = 2, j_chunk: int = 512) -> None:
+    def set_tuning(self, q_tile: int = 0, sin_mode: int = 2, j_chunk: int = 512) -> None:
+        """Pair-kernel launch shape; q_tile 0 picks the tile per call."""
         _check(_lib.load().csx_ctx_set_tuning(self.ptr, q_tile, sin_mode, j_chunk))
+
+    def last_launch(self) -> dict:
+        """Shape of the most recent pair-kernel launch on this context."""
+        v = [C.c_int() for _ in range(3)]
+        _check(_lib.load().csx_ctx_last_launch(self.ptr, *(C.byref(x) for x in v)))
+        return {"q_tile": v[0].value, "sin_mode": v[1].value, "persistent_workgroups": v[2].value}
 
     def cu_count(self) -> int:
         n = C.c_int()

@@ -171,12 +171,38 @@ csx_status get_point_set(csx_structure* s, const csx_solvent_params* p, PointSet
 // Pair tables of one or more point sets from one launch of the pair kernel
 // (a batch concatenates the sets' points, work items and bins):
 // P[c][b*nq + k] = sum_{i<j in bin b of set c} occ_i occ_j sinc(q_k r_ij).
+// Tiles instantiated for every sine mode, and those of sine mode 2 only.
+static const int kTilesAll[] = {8, 16, 32, 64};
+static const int kTilesHw[] = {8, 16, 24, 32, 40, 48, 56, 64};
+
+// q values per pair sweep: the tuned tile if it is instantiated for the sine
+// mode, else the tile with the least padded work ceil(nq / QT) (QT + c).  The
+// per-pair work (distance, correctly rounded sqrt, reciprocal, LDS read)
+// costs about c = 4.5 q evaluations (measured: 4.7 VALU cycles per pair.q at
+// QT = 64 against 4 for the q loop alone).
+static int choose_q_tile(int tuned, int sin_mode, size_t nq) {
+  const int* tiles = sin_mode == 2 ? kTilesHw : kTilesAll;
+  const int ntiles = sin_mode == 2 ? 8 : 4;
+  for (int k = 0; k < ntiles; ++k)
+    if (tiles[k] == tuned) return tuned;
+  int best = tiles[0];
+  double best_cost = 1e300;
+  for (int k = 0; k < ntiles; ++k) {
+    const double chunks = static_cast<double>((nq + tiles[k] - 1) / tiles[k]);
+    const double cost = chunks * (tiles[k] + 4.5);
+    if (cost < best_cost) best = tiles[k], best_cost = cost;
+  }
+  return best;
+}
+
 static csx_status pair_tables_batch(Call& call, const std::vector<const PointSet*>& sets,
                                     const double* q, size_t nq,
                                     std::vector<std::vector<double>>& P) {
-  const Tuning tune = call.ctx()->tuning;
-  const int qt = tune.q_tile;
-  const size_t nq_pad = (nq + qt - 1) / qt * qt;
+  Tuning tune;
+  {
+    std::lock_guard<std::mutex> lock(call.ctx()->mu);
+    tune = call.ctx()->tuning;
+  }
   double rmax = 0.0;
   size_t npts = 0, nitems = 0, nbins = 0;
   for (const PointSet* ps : sets) {
@@ -188,14 +214,16 @@ static csx_status pair_tables_batch(Call& call, const std::vector<const PointSet
   if (npts > static_cast<size_t>(INT32_MAX / 2) || nitems > static_cast<size_t>(INT32_MAX / 2) ||
       nbins > static_cast<size_t>(INT32_MAX / 2))
     return fail(CSX_INVALID_INPUT, "batch too large for one launch");
-  std::vector<float> qrev(nq_pad, 0.0f);
   double umax = 0.0;
-  for (size_t k = 0; k < nq; ++k) {
-    qrev[k] = static_cast<float>(q[k] / (2.0 * kPi));
-    umax = std::max(umax, std::fabs(static_cast<double>(qrev[k])) * 2.0 * rmax);
-  }
+  for (size_t k = 0; k < nq; ++k)
+    umax = std::max(umax, std::fabs(static_cast<double>(static_cast<float>(q[k] / (2.0 * kPi)))) *
+                              2.0 * rmax);
   // The unreduced hardware sine takes |q r / 2 pi| <= 256 revolutions.
   const int sin_mode = tune.sin_mode == 2 && !(umax <= 250.0) ? 1 : tune.sin_mode;
+  const int qt = choose_q_tile(tune.q_tile, sin_mode, nq);
+  const size_t nq_pad = (nq + qt - 1) / qt * qt;
+  std::vector<float> qrev(nq_pad, 0.0f);
+  for (size_t k = 0; k < nq; ++k) qrev[k] = static_cast<float>(q[k] / (2.0 * kPi));
   const hipStream_t st = call.stream();
   DevBuf d_q, d_partial, d_out, d_counter, d_pts, d_items, d_bins;
   CSX_HIP(d_q.alloc(nq_pad * sizeof(float), st), "hipMallocAsync");
@@ -248,6 +276,12 @@ static csx_status pair_tables_batch(Call& call, const std::vector<const PointSet
     bins = d_bins.as<int>();
   }
   const int persist_wgs = tune.persist > 0 ? tune.persist * call.ctx()->cu_count : 0;
+  {
+    std::lock_guard<std::mutex> lock(call.ctx()->mu);
+    call.ctx()->last_launch[0] = qt;
+    call.ctx()->last_launch[1] = sin_mode;
+    call.ctx()->last_launch[2] = persist_wgs;
+  }
   CSX_HIP(launch_debye_pairs(pts, items, static_cast<int>(nitems), bins, static_cast<int>(nbins),
                              d_q.as<float>(), static_cast<int>(nq_pad), qt, sin_mode, persist_wgs,
                              d_counter.as<int>(), d_partial.as<double>(), d_out.as<double>(), st,
@@ -516,8 +550,8 @@ csx_status csx_ctx_reset_timing(csx_ctx* ctx) {
 
 csx_status csx_ctx_set_tuning(csx_ctx* ctx, int q_tile, int sin_mode, int j_chunk) {
   if (!ctx) return fail(CSX_INVALID_INPUT, "context is null");
-  if (q_tile != 8 && q_tile != 16 && q_tile != 32 && q_tile != 64)
-    return fail(CSX_INVALID_INPUT, "q_tile must be 8, 16, 32 or 64");
+  if (q_tile < 0 || q_tile > 64 || q_tile % 8 != 0)
+    return fail(CSX_INVALID_INPUT, "q_tile must be 0 (automatic) or a multiple of 8 up to 64");
   if (sin_mode < 0 || sin_mode > 2) return fail(CSX_INVALID_INPUT, "sin_mode must be 0, 1 or 2");
   if (j_chunk < 256 || j_chunk % 256 != 0 || j_chunk > (1 << 20))
     return fail(CSX_INVALID_INPUT, "j_chunk must be a multiple of 256 in [256, 2^20]");
@@ -525,6 +559,15 @@ csx_status csx_ctx_set_tuning(csx_ctx* ctx, int q_tile, int sin_mode, int j_chun
   ctx->tuning.q_tile = q_tile;
   ctx->tuning.sin_mode = sin_mode;
   ctx->tuning.j_chunk = j_chunk;
+  return CSX_OK;
+}
+
+csx_status csx_ctx_last_launch(csx_ctx* ctx, int* q_tile, int* sin_mode, int* persistent_wgs) {
+  if (!ctx) return fail(CSX_INVALID_INPUT, "context is null");
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  if (q_tile) *q_tile = ctx->last_launch[0];
+  if (sin_mode) *sin_mode = ctx->last_launch[1];
+  if (persistent_wgs) *persistent_wgs = ctx->last_launch[2];
   return CSX_OK;
 }
 

@@ -29,7 +29,7 @@ csx_status multipole_dispatch(csx_structure* s, const double* q, size_t nq, int 
 // on MI355X for a 20,000-atom x 500-q profile (all shapes meet the same
 // accuracy); the autotuner can replace them per workload.
 struct Tuning {
-  int q_tile = 64;     // q values per sweep of a pair
+  int q_tile = 0;      // q values per sweep of a pair (0: automatic)
   int sin_mode = 2;    // 0 software polynomial, 1 reduced + V_SIN_F32, 2 V_SIN_F32
   int j_chunk = 512;   // j atoms per work item
   int persist = 0;     // persistent launch with this many workgroups per CU (0: grid)
@@ -45,6 +45,7 @@ struct csx_ctx {
   std::map<std::string, std::pair<int64_t, double>> stats;
   csx::Tuning tuning;
   int cu_count = 256;
+  int last_launch[3] = {0, 0, 0};  // q tile, sin mode, persistent workgroups
 };
 
 namespace csx {

@@ -529,6 +529,188 @@ static WeightFit optimize_weights(const std::vector<const double*>& curves, cons
   return r;
 }
 
+// Least squares min |E[:, P] z - f| over the columns P of the n x m
+// column-major E, by Householder QR of the n x |P| submatrix.  Returns false
+// when a column is (numerically) dependent on the others.
+static bool passive_lsq(const std::vector<double>& E, size_t n, const std::vector<size_t>& P,
+                        const std::vector<double>& f, std::vector<double>& z) {
+  const size_t k = P.size();
+  if (k > n) return false;
+  std::vector<double> A(n * k), b(f);
+  for (size_t c = 0; c < k; ++c) std::copy(&E[P[c] * n], &E[P[c] * n] + n, &A[c * n]);
+  std::vector<double> diag(k);
+  for (size_t c = 0; c < k; ++c) {
+    double norm = 0.0;
+    for (size_t r = c; r < n; ++r) norm += A[c * n + r] * A[c * n + r];
+    norm = std::sqrt(norm);
+    if (!(norm > 1e-12)) return false;  // columns are unit-norm: rank loss
+    const double alpha = A[c * n + c] > 0.0 ? -norm : norm;
+    A[c * n + c] -= alpha;
+    double vnorm = 0.0;
+    for (size_t r = c; r < n; ++r) vnorm += A[c * n + r] * A[c * n + r];
+    for (size_t cc = c + 1; cc < k; ++cc) {
+      double dot = 0.0;
+      for (size_t r = c; r < n; ++r) dot += A[c * n + r] * A[cc * n + r];
+      const double t = 2.0 * dot / vnorm;
+      for (size_t r = c; r < n; ++r) A[cc * n + r] -= t * A[c * n + r];
+    }
+    double dot = 0.0;
+    for (size_t r = c; r < n; ++r) dot += A[c * n + r] * b[r];
+    const double t = 2.0 * dot / vnorm;
+    for (size_t r = c; r < n; ++r) b[r] -= t * A[c * n + r];
+    diag[c] = alpha;
+  }
+  z.assign(k, 0.0);
+  for (size_t c = k; c-- > 0;) {
+    double v = b[c];
+    for (size_t cc = c + 1; cc < k; ++cc) v -= A[cc * n + c] * z[cc];
+    z[c] = v / diag[c];
+  }
+  return true;
+}
+
+// Lawson-Hanson active-set NNLS (Lawson & Hanson, "Solving Least Squares
+// Problems", 1974, ch. 23): min |E a - f| subject to a >= 0.  E is n x m
+// column-major with unit-norm (or zero) columns.
+static std::vector<double> nnls(const std::vector<double>& E, size_t n, size_t m,
+                                const std::vector<double>& f) {
+  std::vector<double> a(m, 0.0), res(f), grad(m), z;
+  std::vector<char> passive(m, 0), blocked(m, 0);
+  double fnorm = 0.0;
+  for (double v : f) fnorm += v * v;
+  const double tol = 1e-12 * std::max(std::sqrt(fnorm), 1e-300);
+  auto gradient = [&]() {
+    for (size_t p = 0; p < n; ++p) {
+      double v = f[p];
+      for (size_t c = 0; c < m; ++c)
+        if (a[c] != 0.0) v -= E[c * n + p] * a[c];
+      res[p] = v;
+    }
+    for (size_t c = 0; c < m; ++c) {
+      double v = 0.0;
+      for (size_t p = 0; p < n; ++p) v += E[c * n + p] * res[p];
+      grad[c] = v;
+    }
+  };
+  gradient();
+  for (size_t outer = 0; outer < 3 * m + 10; ++outer) {
+    size_t t = m;
+    double best = tol;
+    for (size_t c = 0; c < m; ++c)
+      if (!passive[c] && !blocked[c] && grad[c] > best) best = grad[c], t = c;
+    if (t == m) break;
+    passive[t] = 1;
+    for (size_t inner = 0; inner < 3 * m + 10; ++inner) {
+      std::vector<size_t> P;
+      for (size_t c = 0; c < m; ++c)
+        if (passive[c]) P.push_back(c);
+      if (!passive_lsq(E, n, P, f, z)) {
+        // the new column is dependent on the passive set: keep it out
+        passive[t] = 0;
+        blocked[t] = 1;
+        break;
+      }
+      bool feasible = true;
+      for (size_t k = 0; k < P.size(); ++k)
+        if (z[k] <= 0.0) feasible = false;
+      if (feasible) {
+        for (size_t k = 0; k < P.size(); ++k) a[P[k]] = z[k];
+        break;
+      }
+      if (inner == 0 && z[std::find(P.begin(), P.end(), t) - P.begin()] <= 0.0) {
+        // rounding made the entering coefficient non-positive: drop it
+        passive[t] = 0;
+        blocked[t] = 1;
+        break;
+      }
+      // step from a towards z until the first passive coefficient hits 0
+      double alpha = 2.0;
+      size_t kmin = 0;
+      for (size_t k = 0; k < P.size(); ++k)
+        if (z[k] <= 0.0) {
+          const double r = a[P[k]] / (a[P[k]] - z[k]);
+          if (r < alpha) alpha = r, kmin = k;
+        }
+      for (size_t k = 0; k < P.size(); ++k) {
+        a[P[k]] += alpha * (z[k] - a[P[k]]);
+        if (k == kmin || a[P[k]] <= 0.0) {
+          a[P[k]] = 0.0;
+          passive[P[k]] = 0;
+        }
+      }
+    }
+    gradient();
+  }
+  return a;
+}
+
+// Weights by NNLS: with background b eliminated (fitted: projected out of
+// the sigma-weighted system; fixed: subtracted), a = scale * weights >= 0
+// minimises sum ((X a + b - y) / sigma)^2; scale = sum a.
+static csx_status nnls_weights(const std::vector<const double*>& curves, const double* y,
+                               const double* s, size_t n, const csx_fit_options* o,
+                               WeightFit& out) {
+  if (o->scale_mode != CSX_SCALE_FIT)
+    return fail(CSX_INVALID_INPUT, "NNLS weights need a fitted scale");
+  const size_t m = curves.size();
+  std::vector<double> E(n * m), f(n), u(n);
+  const bool fit_bg = o->background_mode == CSX_BACKGROUND_FIT;
+  const double bg0 = o->background_mode == CSX_BACKGROUND_FIXED ? o->background_value : 0.0;
+  double uu = 0.0;
+  for (size_t p = 0; p < n; ++p) {
+    u[p] = 1.0 / s[p];
+    uu += u[p] * u[p];
+  }
+  auto project = [&](double* v) {  // remove the component along u
+    if (!fit_bg) return;
+    double d = 0.0;
+    for (size_t p = 0; p < n; ++p) d += v[p] * u[p];
+    for (size_t p = 0; p < n; ++p) v[p] -= d / uu * u[p];
+  };
+  for (size_t p = 0; p < n; ++p) f[p] = (y[p] - bg0) / s[p];
+  project(f.data());
+  std::vector<double> norm(m, 0.0);
+  for (size_t c = 0; c < m; ++c) {
+    double* col = &E[c * n];
+    for (size_t p = 0; p < n; ++p) col[p] = curves[c][p] / s[p];
+    project(col);
+    double v = 0.0;
+    for (size_t p = 0; p < n; ++p) v += col[p] * col[p];
+    norm[c] = std::sqrt(v);
+    for (size_t p = 0; p < n; ++p) col[p] = norm[c] > 0.0 ? col[p] / norm[c] : 0.0;
+  }
+  std::vector<double> a = nnls(E, n, m, f);
+  double scale = 0.0;
+  for (size_t c = 0; c < m; ++c) {
+    a[c] = norm[c] > 0.0 ? a[c] / norm[c] : 0.0;
+    scale += a[c];
+  }
+  out.weights.assign(m, 1.0 / static_cast<double>(m));
+  if (scale > 0.0)
+    for (size_t c = 0; c < m; ++c) out.weights[c] = a[c] / scale;
+  // Joint optimum: scale = sum a and the background that goes with it (not
+  // the sequential scale-then-background solve of the projected-gradient
+  // path, which is biased when the background is non-zero).
+  std::vector<double> model(n, 0.0);
+  double num = 0.0;
+  for (size_t p = 0; p < n; ++p) {
+    double v = 0.0;
+    for (size_t c = 0; c < m; ++c) v += curves[c][p] * a[c];
+    model[p] = v;
+    num += (y[p] - v) * u[p] * u[p];
+  }
+  const double bg = fit_bg ? num / uu : bg0;
+  double chi = 0.0;
+  out.fitted.resize(n);
+  for (size_t p = 0; p < n; ++p) {
+    out.fitted[p] = model[p] + bg;
+    const double r = (out.fitted[p] - y[p]) / s[p];
+    chi += r * r;
+  }
+  out.lin = {scale, bg, chi / (n > 1 ? static_cast<double>(n - 1) : 1.0)};
+  return CSX_OK;
+}
+
 }  // namespace csx
 
 using namespace csx;
@@ -667,16 +849,38 @@ csx_status csx_optimize_weights(const double* curves, size_t m, const double* in
   return CSX_OK;
 }
 
+csx_status csx_nnls_weights(const double* curves, size_t m, const double* intensity,
+                            const double* sigma, size_t n, const csx_fit_options* o,
+                            double* weights, double* out3, double* fitted) {
+  if (!curves || !intensity || !sigma || !o || !weights || !out3 || m == 0 || n == 0)
+    return fail(CSX_INVALID_INPUT, "buffers are required");
+  std::vector<const double*> cv(m);
+  for (size_t c = 0; c < m; ++c) cv[c] = curves + c * n;
+  WeightFit w;
+  CSX_TRY(nnls_weights(cv, intensity, sigma, n, o, w));
+  std::copy(w.weights.begin(), w.weights.end(), weights);
+  out3[0] = w.lin.scale;
+  out3[1] = w.lin.bg;
+  out3[2] = w.lin.chi;
+  if (fitted) std::copy(w.fitted.begin(), w.fitted.end(), fitted);
+  return CSX_OK;
+}
+
 // EnsembleFitter::fit_core (ensemble.rs:234-299) from per-conformer bases:
 // every (c1, c2) candidate of the 5 x 5 grid re-contracts the cached bases
 // instead of recomputing profiles.  bases: m * 6 * n.
 csx_status csx_ensemble_fit_core(const double* bases, size_t m, const double* q,
                                  const double* intensity, const double* sigma, size_t n,
                                  const csx_fit_options* o, int uniform, double regularization,
-                                 double* weights, csx_fit_params* params, double* chi2,
-                                 double* fitted) {
+                                 int weight_solver, double* weights, csx_fit_params* params,
+                                 double* chi2, double* fitted) {
   if (!bases || !q || !intensity || !sigma || !o || !weights || !params || !chi2 || m == 0 || n == 0)
     return fail(CSX_INVALID_INPUT, "buffers are required");
+  if (weight_solver != CSX_WEIGHTS_PROJECTED_GRADIENT && weight_solver != CSX_WEIGHTS_NNLS)
+    return fail(CSX_INVALID_INPUT, "unknown weight solver");
+  const bool use_nnls = weight_solver == CSX_WEIGHTS_NNLS && !uniform;
+  if (use_nnls && regularization != 0.0)
+    return fail(CSX_INVALID_INPUT, "NNLS weights take no regularization");
   std::vector<double> curves(m * n);
   std::vector<const double*> cv(m);
   double best = INFINITY;
@@ -689,7 +893,11 @@ csx_status csx_ensemble_fit_core(const double* bases, size_t m, const double* q,
         combine_with_grad(bases + c * 6 * n, q, n, c1, c2, curves.data() + c * n, nullptr, nullptr);
         cv[c] = curves.data() + c * n;
       }
-      const WeightFit w = optimize_weights(cv, intensity, sigma, n, o, uniform != 0, regularization);
+      WeightFit w;
+      if (use_nnls)
+        CSX_TRY(nnls_weights(cv, intensity, sigma, n, o, w));
+      else
+        w = optimize_weights(cv, intensity, sigma, n, o, uniform != 0, regularization);
       if (!have || w.lin.chi < best) {
         have = true;
         best = w.lin.chi;

@@ -371,6 +371,28 @@ def optimize_weights(curves, experimental: ExperimentalCurve, options: FitOption
     return w, float(out3[0]), float(out3[1]), float(out3[2]), fitted
 
 
+def nnls_weights(curves, experimental: ExperimentalCurve, options: FitOptions = None):
+    """Exact non-negative least-squares weights (Lawson-Hanson active set):
+    (weights, scale, bg, chi2, fitted) like `optimize_weights`.  No reference
+    counterpart; needs a fitted scale."""
+    options = options or FitOptions()
+    cv = f64(curves)
+    m, n = cv.shape
+    e = experimental
+    w, out3, fitted = np.zeros(m), np.zeros(3), np.zeros(n)
+    o = options.to_c()
+    _check(_lib.load().csx_nnls_weights(dptr(cv), m, dptr(e.intensity), dptr(e.sigma), n,
+                                        C.byref(o), dptr(w), dptr(out3), dptr(fitted)))
+    return w, float(out3[0]), float(out3[1]), float(out3[2]), fitted
+
+
+class WeightSolver(enum.IntEnum):
+    """Ensemble weight solver: the reference's projected gradient
+    (ensemble.rs:317-381) or exact non-negative least squares."""
+    ProjectedGradient = 0
+    NNLS = 1
+
+
 @dataclass
 class EnsembleOptions:
     """`EnsembleOptions` (src/ensemble.rs:13-38)."""
@@ -382,6 +404,7 @@ class EnsembleOptions:
     bootstrap: int = 0
     seed: int = 0xC0FFEE
     fit: FitOptions = field(default_factory=FitOptions)
+    weight_solver: WeightSolver = WeightSolver.ProjectedGradient
 
 
 @dataclass
@@ -470,7 +493,8 @@ class EnsembleFitter:
         uniform = self.options.uniform or not self.options.fit_weights
         _check(_lib.load().csx_ensemble_fit_core(dptr(b), m, dptr(data.q), dptr(data.intensity),
                                                  dptr(data.sigma), n, C.byref(o), int(uniform),
-                                                 self.options.regularization, dptr(w), C.byref(p),
+                                                 self.options.regularization,
+                                                 int(self.options.weight_solver), dptr(w), C.byref(p),
                                                  C.byref(chi), dptr(fitted)))
         return w, FitParams(p.scale, p.background, p.c1, p.c2), float(chi.value), fitted
 

@@ -116,13 +116,22 @@ csx_status csx_ctx_kernel_time(csx_ctx *ctx, const char *name,
 csx_status csx_ctx_reset_timing(csx_ctx *ctx);
 
 /* Launch shape of the FP32 pair kernel (no reference counterpart; chosen by
- * the autotuner): q_tile in {8, 16, 32, 64} q values per pair sweep;
- * sin_mode 0 (software polynomial after exact reduction in revolutions),
- * 1 (hardware V_SIN_F32 after the same reduction) or 2 (V_SIN_F32 on the
- * unreduced argument; used only when |q| r_max / pi <= 250 and otherwise
- * replaced by 1); j_chunk atoms per work item (multiple of 256). */
+ * the autotuner): q_tile q values per pair sweep, 0 (automatic: the tile
+ * with the least padded work for the call's q count) or a multiple of 8 up
+ * to 64 (sin modes 0 and 1 have the tiles 8, 16, 32 and 64; another tile
+ * falls back to the automatic choice there); sin_mode 0 (software
+ * polynomial after exact reduction in revolutions), 1 (hardware V_SIN_F32
+ * after the same reduction) or 2 (V_SIN_F32 on the unreduced argument; used
+ * only when |q| r_max / pi <= 250 and otherwise replaced by 1); j_chunk
+ * atoms per work item (multiple of 256). */
 csx_status csx_ctx_set_tuning(csx_ctx *ctx, int q_tile, int sin_mode,
                               int j_chunk);
+
+/* Shape of the most recent FP32 pair-kernel launch on the context (after
+ * the automatic tile choice and the sine-mode fallback); 0s before the
+ * first launch.  Any output may be NULL. */
+csx_status csx_ctx_last_launch(csx_ctx *ctx, int *q_tile, int *sin_mode,
+                               int *persistent_workgroups);
 
 /* Persistent launch of the FP32 pair kernel: a fixed grid of
  * workgroups_per_cu x CU workgroups drains a global task counter (0 = one
@@ -404,17 +413,38 @@ csx_status csx_optimize_weights(const double *curves, size_t m,
                                 double *weights, double *out3,
                                 double *fitted);
 
+/* Weight solvers of an ensemble fit. */
+typedef enum {
+    CSX_WEIGHTS_PROJECTED_GRADIENT = 0, /* optimize_weights (ensemble.rs:317-381) */
+    CSX_WEIGHTS_NNLS = 1 /* exact non-negative least squares (no reference
+                          * counterpart): scale * weights = the non-negative
+                          * coefficients a >= 0 minimising chi^2 with the
+                          * background eliminated, by the Lawson-Hanson active
+                          * set method; needs a fitted scale and zero
+                          * regularization */
+} csx_weight_solver;
+
+/* Non-negative least-squares weights (CSX_WEIGHTS_NNLS) with the outputs of
+ * csx_optimize_weights: weights[m] (sum 1, or uniform when every
+ * coefficient is zero), out3 = (scale, background, reduced chi^2), fitted: n
+ * or NULL.  No reference counterpart. */
+csx_status csx_nnls_weights(const double *curves, size_t m,
+                            const double *intensity, const double *sigma,
+                            size_t n, const csx_fit_options *options,
+                            double *weights, double *out3, double *fitted);
+
 /* EnsembleFitter::fit_core (ensemble.rs:234-299): the 5 x 5 (c1, c2)
  * candidate grid re-contracts per-conformer six-term bases (m*6*n) instead
  * of recomputing profiles; best candidate's weights[m], params, reduced
- * chi^2 and fitted curve (n or NULL). */
+ * chi^2 and fitted curve (n or NULL).  weight_solver: csx_weight_solver
+ * (the reference uses CSX_WEIGHTS_PROJECTED_GRADIENT). */
 csx_status csx_ensemble_fit_core(const double *bases, size_t m,
                                  const double *q, const double *intensity,
                                  const double *sigma, size_t n,
                                  const csx_fit_options *options, int uniform,
-                                 double regularization, double *weights,
-                                 csx_fit_params *params, double *chi2,
-                                 double *fitted);
+                                 double regularization, int weight_solver,
+                                 double *weights, csx_fit_params *params,
+                                 double *chi2, double *fitted);
 
 /* ---- synthetic workloads (no reference counterpart) -------------------- */
 

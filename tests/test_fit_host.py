@@ -154,3 +154,66 @@ def test_compare_curves_and_quality_presets():
     assert csx.Quality.Accurate.fit_options(csx.FitOptions()).q_sampling_stride == 1
     with pytest.raises(csx.InvalidInput):
         csx.compare_curves(e, y[:5])
+
+
+def _nnls_reference(curves, y, sigma, fit_bg=True):
+    """scipy.optimize.nnls (Lawson-Hanson) on the same background-eliminated,
+    sigma-weighted system."""
+    from scipy.optimize import nnls as scipy_nnls
+    u = 1.0 / sigma
+    E = (curves / sigma).T.copy()
+    f = y / sigma
+    if fit_bg:
+        P = np.eye(len(y)) - np.outer(u, u) / (u @ u)
+        E, f = P @ E, P @ f
+    a, _ = scipy_nnls(E, f, maxiter=50 * E.shape[1])
+    return a
+
+
+@pytest.mark.parametrize("fit_bg", [True, False])
+def test_nnls_weights_match_scipy_and_recover_mixtures(fit_bg):
+    rng = np.random.default_rng(17)
+    q = np.linspace(0.01, 0.5, 120)
+    m = 40
+    # smooth positive curves of varying decay: strongly correlated columns
+    rg = rng.uniform(10.0, 40.0, size=m)
+    curves = np.stack([np.exp(-(q * r) ** 2 / 3.0) * (1 + 0.1 * np.sin(q * r * 3)) for r in rg])
+    truth = np.zeros(m)
+    pick = rng.choice(m, size=4, replace=False)
+    truth[pick] = rng.dirichlet(np.ones(4))
+    bg = 0.02 if fit_bg else 0.0
+    y = 3.0 * truth @ curves + bg
+    sigma = 0.01 * y
+    opts = csx.FitOptions(background_mode=csx.BackgroundMode.Fit if fit_bg
+                          else csx.BackgroundMode.FixedZero)
+    w, scale, b, chi, fitted = csx.nnls_weights(curves, curve(q, y, sigma), opts)
+    assert abs(np.sum(w) - 1) < 1e-12 and np.all(w >= 0)
+    # noise-free data: an exact fit
+    assert chi < 1e-12, chi
+    assert np.allclose(fitted, y, rtol=1e-7)
+    # same optimum as scipy's NNLS on the same system
+    a = _nnls_reference(curves, y, sigma, fit_bg)
+    got = scale * w
+    assert np.allclose(curves.T @ got, curves.T @ a, rtol=1e-7)
+    # noisy data: chi2 of the exact NNLS optimum
+    y2 = y + sigma * rng.normal(size=q.size)
+    w2, s2, b2, chi2, _ = csx.nnls_weights(curves, curve(q, y2, sigma), opts)
+    a2 = _nnls_reference(curves, y2, sigma, fit_bg)
+    u = 1.0 / sigma
+    r_ref = (y2 - curves.T @ a2) / sigma
+    if fit_bg:
+        r_ref = r_ref - (r_ref @ u) / (u @ u) * u
+    ref_chi = float(r_ref @ r_ref) / (q.size - 1)
+    assert chi2 <= ref_chi * (1 + 1e-9) + 1e-15
+    # the reference's projected gradient never beats the exact optimum
+    _, _, _, chi_pg, _ = csx.optimize_weights(curves, curve(q, y2, sigma), opts)
+    assert chi2 <= chi_pg * (1 + 1e-12)
+
+
+def test_nnls_weights_reject_fixed_scale_and_regularization():
+    q = np.linspace(0.01, 0.3, 20)
+    curves = np.stack([np.exp(-q * k) for k in (1.0, 2.0, 3.0)])
+    y = curves[0] + 0.5 * curves[2]
+    with pytest.raises(csx.InvalidInput):
+        csx.nnls_weights(curves, curve(q, y, 0.01 * y),
+                         csx.FitOptions(scale_mode=csx.ScaleMode.Fixed(1.0)))

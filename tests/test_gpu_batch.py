@@ -72,3 +72,42 @@ def test_ensemble_fitter_uses_batched_bases(gpu_ctx):
         assert np.array_equal(b[k], csx.debye_solvent_basis(c, q, csx.SolventParams()))
     r = fitter.fit()
     assert abs(np.sum(r.weights) - 1) < 1e-12
+
+
+def test_every_q_tile_gives_identical_sums(gpu_ctx):
+    """Each q value's sum runs over the same pairs in the same order whatever
+    the tile, so every instantiated tile (and the automatic choice) agrees
+    bitwise; 200 q makes the automatic choice a 40-wide tile."""
+    s = csx.generate_globule(2300, 15.0, 1.0, seed=12)
+    q = np.linspace(0.005, 0.5, 200)
+    try:
+        gpu_ctx.set_tuning(64, 2, 512)
+        base = csx.debye_intensity(s, q)
+        for qt in (0, 8, 16, 24, 32, 40, 48, 56):
+            gpu_ctx.set_tuning(qt, 2, 512)
+            assert np.array_equal(csx.debye_intensity(s, q), base), qt
+        assert gpu_ctx.last_launch()["q_tile"] == 56
+        gpu_ctx.set_tuning(0, 2, 512)
+        csx.debye_intensity(s, q)
+        assert gpu_ctx.last_launch() == {"q_tile": 40, "sin_mode": 2, "persistent_workgroups": 0}
+        gpu_ctx.set_tuning(40, 1, 512)  # no 40-wide tile for the reduced sine: automatic
+        csx.debye_intensity(s, q)
+        assert gpu_ctx.last_launch()["q_tile"] in (8, 16, 32, 64)
+    finally:
+        gpu_ctx.set_tuning()
+
+
+def test_ensemble_nnls_recovers_weights(gpu_ctx):
+    cs = conformers() + [csx.generate_globule(2100, 13.0, 1.4, seed=77)]
+    q = np.linspace(0.01, 0.45, 90)
+    curves = csx.batch_debye_solvent_basis(cs, q, csx.SolventParams())
+    prof = np.stack([csx.combine_solvent_basis(b, q, 1.0, 1.0) for b in curves])
+    w_true = np.array([0.0, 0.3, 0.0, 0.0, 0.7])
+    y = 2.0 * w_true @ prof + 5.0
+    e = csx.ExperimentalCurve(q, y, 0.01 * y)
+    opts = csx.EnsembleOptions(fit=csx.FitOptions(method=csx.ScatteringMethod.Debye),
+                               weight_solver=csx.WeightSolver.NNLS)
+    r = csx.EnsembleFitter(cs, list("abcde"), e, opts).fit()
+    w = dict(zip(r.conformers, r.weights))
+    assert abs(w.get("b", 0) - 0.3) < 1e-6 and abs(w.get("e", 0) - 0.7) < 1e-6, w
+    assert abs(r.params.scale - 2.0) < 1e-6 and r.reduced_chi2 < 1e-10
