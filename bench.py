@@ -251,7 +251,7 @@ def adjoint_leg(csx, ctx, log):
     w = rng.normal(size=q.size) / np.maximum(csx.solvent_corrected_intensity(s, q, truth, ctx=ctx),
                                              1e-30)
     csx.debye_grad_x(s, q, w, truth, ctx=ctx)
-    g, dt, kms, _ = timed(ctx, "debye_adjoint_f32", lambda: csx.debye_grad_x(s, q, w, truth, ctx=ctx),
+    g, dt, kms, _ = timed(ctx, "debye_vjp_f32", lambda: csx.debye_grad_x(s, q, w, truth, ctx=ctx),
                           3)
     # directional derivative against a central difference of FP64 GPU Debye
     d = rng.normal(size=(n, 3))

@@ -209,6 +209,10 @@ class Context:
         """Pair-kernel launch shape; q_tile 0 picks the tile per call."""
         _check(_lib.load().csx_ctx_set_tuning(self.ptr, q_tile, sin_mode, j_chunk))
 
+    def set_workspace_limit(self, nbytes: int = 2 << 30) -> None:
+        """Bytes of adjoint partial sums per launch (larger runs split)."""
+        _check(_lib.load().csx_ctx_set_workspace_limit(self.ptr, int(nbytes)))
+
     def last_launch(self) -> dict:
         """Shape of the most recent pair-kernel launch on this context."""
         v = [C.c_int() for _ in range(3)]

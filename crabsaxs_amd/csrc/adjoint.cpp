@@ -6,11 +6,167 @@
 
 namespace csx {
 
-hipError_t launch_debye_adjoint(const float4* pts, const int* orig, const int* type, int npts,
-                                const int* seg_start, int ntypes, const float* qrev,
-                                const float* Wp, int nq_pad, bool jacobian, double* out,
-                                hipStream_t stream, KernelTimer* timer);
-int adjoint_q_tile(bool jacobian);
+hipError_t launch_debye_jacobian(const float4* pts, const int* orig, const int* type, int npts,
+                                 const int* seg_start, int ntypes, const float* qrev,
+                                 const float* Wp, int nq_pad, double* out, hipStream_t stream,
+                                 KernelTimer* timer);
+hipError_t launch_debye_vjp(const float4* pts, const int4* blocks, int nb, const int2* items,
+                            size_t nitems, const int* orig, int ntypes, const float* qrev,
+                            const float* AB, int nq_pad, int qt, int sin_mode, int qsplit,
+                            size_t items_per_launch, double* part, double* gacc, double* out,
+                            hipStream_t stream, KernelTimer* timer);
+int jacobian_q_tile();
+int vjp_q_tile(size_t nq);
+
+namespace {
+
+constexpr int kBlock = 256;
+
+// Per-type amplitudes A_t(q_k) of the intensity being differentiated.
+std::vector<double> type_amplitudes_for(const csx_structure* s, const csx_solvent_params* p,
+                                        bool implicit_h, const double* q, size_t nq, int T) {
+  std::vector<double> A(static_cast<size_t>(T) * nq);
+  for (size_t k = 0; k < nq; ++k) {
+    const double g = p ? excluded_volume_scale(q[k], p->c1) : 0.0;
+    for (int t = 0; t < T; ++t) {
+      if (!p) {
+        A[t * nq + k] = type_form_factor(s->types[t], implicit_h, q[k]);
+      } else {
+        const Amplitudes a = debye_amplitudes(s->types[t], implicit_h, q[k],
+                                              p->shell_thickness_angstrom, p->bulk_density_e_per_a3);
+        A[t * nq + k] = a.v - g * a.e + p->c2 * a.h;
+      }
+    }
+  }
+  return A;
+}
+
+}  // namespace
+
+static csx_status vjp(Call& call, const csx_structure* s, PointSet* ps, const double* q, size_t nq,
+                      const double* weights, const csx_solvent_params* p, bool implicit_h,
+                      double* out) {
+  const int T = ps->ntypes;
+  const size_t n = s->n;
+  // type-pure blocks of 256 sorted atoms and the block pairs I <= J
+  std::vector<int> blocks;
+  for (int t = 0; t < T; ++t)
+    for (int i0 = ps->h_seg[t]; i0 < ps->h_seg[t + 1]; i0 += kBlock)
+      blocks.insert(blocks.end(), {i0, std::min(kBlock, ps->h_seg[t + 1] - i0), t, 0});
+  const int nb = static_cast<int>(blocks.size() / 4);
+  std::vector<int> items;
+  items.reserve(static_cast<size_t>(nb) * (nb + 1));
+  for (int I = 0; I < nb; ++I)
+    for (int J = I; J < nb; ++J) items.insert(items.end(), {I, J});
+  const size_t nitems = items.size() / 2;
+  const int qt = vjp_q_tile(nq);
+  const size_t nq_pad = (nq + qt - 1) / qt * qt;
+  const std::vector<double> A = type_amplitudes_for(s, p, implicit_h, q, nq, T);
+  // AB[a][b][0][k] = qe_k B_ab(k), AB[a][b][1][k] = B_ab(k) = w_k 2 A_a A_b / qe_k, where
+  // qe_k = 2 pi (q_k / 2 pi)_f32 is the q the kernel's phase uses.
+  std::vector<float> qrev(nq_pad, 0.0f), AB(static_cast<size_t>(T) * T * 2 * nq_pad, 0.0f);
+  double umax = 0.0;
+  for (size_t k = 0; k < nq; ++k) {
+    qrev[k] = static_cast<float>(q[k] / (2.0 * kPi));
+    umax = std::max(umax, std::fabs(static_cast<double>(qrev[k])) * 2.0 * ps->max_radius);
+    const double qe = 2.0 * kPi * static_cast<double>(qrev[k]);
+    if (qe == 0.0) continue;
+    for (int a = 0; a < T; ++a)
+      for (int b = 0; b < T; ++b) {
+        const double B = 2.0 * weights[k] * A[a * nq + k] * A[b * nq + k] / qe;
+        float* row = &AB[((static_cast<size_t>(a) * T + b) * 2) * nq_pad];
+        row[k] = static_cast<float>(qe * B);
+        row[nq_pad + k] = static_cast<float>(B);
+      }
+  }
+  const int sin_mode = umax <= 250.0 ? 2 : 1;
+  const int nqc = static_cast<int>(nq_pad / qt);
+  // enough workgroups to fill the device: split the q chunks of small problems
+  const int qsplit = static_cast<int>(std::min<size_t>(
+      static_cast<size_t>(nqc), std::max<size_t>(1, (16384 + nitems - 1) / nitems)));
+  const size_t per_item = static_cast<size_t>(qsplit) * 2 * 3 * kBlock * sizeof(double);
+  // partial sums of one launch stay within the context's workspace limit
+  size_t budget;
+  {
+    std::lock_guard<std::mutex> lock(call.ctx()->mu);
+    budget = call.ctx()->tuning.workspace_limit;
+  }
+  const size_t ipl = std::max<size_t>(1, std::min(nitems, budget / per_item));
+  const hipStream_t st = call.stream();
+  DevBuf d_q, d_ab, d_blocks, d_items, d_part, d_acc, d_out;
+  CSX_HIP(d_q.alloc(nq_pad * sizeof(float), st), "hipMallocAsync");
+  CSX_HIP(d_ab.alloc(AB.size() * sizeof(float), st), "hipMallocAsync");
+  CSX_HIP(d_blocks.alloc(blocks.size() * sizeof(int), st), "hipMallocAsync");
+  CSX_HIP(d_items.alloc(items.size() * sizeof(int), st), "hipMallocAsync");
+  CSX_HIP(d_part.alloc(ipl * per_item, st), "hipMallocAsync");
+  CSX_HIP(d_acc.alloc(static_cast<size_t>(nb) * 3 * kBlock * sizeof(double), st), "hipMallocAsync");
+  CSX_HIP(d_out.alloc(n * 3 * sizeof(double), st), "hipMallocAsync");
+  CSX_HIP(hipMemcpyAsync(d_q.as<void>(), qrev.data(), qrev.size() * sizeof(float),
+                         hipMemcpyHostToDevice, st), "hipMemcpyAsync");
+  CSX_HIP(hipMemcpyAsync(d_ab.as<void>(), AB.data(), AB.size() * sizeof(float),
+                         hipMemcpyHostToDevice, st), "hipMemcpyAsync");
+  CSX_HIP(hipMemcpyAsync(d_blocks.as<void>(), blocks.data(), blocks.size() * sizeof(int),
+                         hipMemcpyHostToDevice, st), "hipMemcpyAsync");
+  CSX_HIP(hipMemcpyAsync(d_items.as<void>(), items.data(), items.size() * sizeof(int),
+                         hipMemcpyHostToDevice, st), "hipMemcpyAsync");
+  CSX_HIP(launch_debye_vjp(ps->pts32.as<float4>(), d_blocks.as<int4>(), nb, d_items.as<int2>(),
+                           nitems, ps->orig32.as<int>(), T, d_q.as<float>(), d_ab.as<float>(),
+                           static_cast<int>(nq_pad), qt, sin_mode, qsplit, ipl, d_part.as<double>(),
+                           d_acc.as<double>(), d_out.as<double>(), st, call.timer()),
+          "debye_vjp_f32");
+  CSX_HIP(hipMemcpyAsync(out, d_out.as<void>(), n * 3 * sizeof(double), hipMemcpyDeviceToHost, st),
+          "hipMemcpyAsync");
+  d_q.reset();
+  d_ab.reset();
+  d_blocks.reset();
+  d_items.reset();
+  d_part.reset();
+  d_acc.reset();
+  d_out.reset();
+  return call.finish();
+}
+
+static csx_status jacobian_run(Call& call, const csx_structure* s, PointSet* ps, const double* q,
+                               size_t nq, const csx_solvent_params* p, bool implicit_h,
+                               double* out) {
+  const int T = ps->ntypes;
+  const size_t n = s->n;
+  const int qt = jacobian_q_tile();
+  const size_t nq_pad = (nq + qt - 1) / qt * qt;
+  const std::vector<double> A = type_amplitudes_for(s, p, implicit_h, q, nq, T);
+  std::vector<float> qrev(nq_pad, 0.0f), Wp(static_cast<size_t>(T) * T * nq_pad, 0.0f);
+  for (size_t k = 0; k < nq; ++k) {
+    qrev[k] = static_cast<float>(q[k] / (2.0 * kPi));
+    // The kernel's u = 2 pi (q/2pi)_f32 r: use the same effective q in W'.
+    const double qe = 2.0 * kPi * static_cast<double>(qrev[k]);
+    if (qe == 0.0) continue;
+    for (int a = 0; a < T; ++a)
+      for (int b = 0; b < T; ++b)
+        Wp[(static_cast<size_t>(a) * T + b) * nq_pad + k] =
+            static_cast<float>(2.0 * A[a * nq + k] * A[b * nq + k] / qe);
+  }
+  const hipStream_t st = call.stream();
+  const size_t out_elems = nq_pad * 3 * n;
+  DevBuf d_q, d_w, d_out;
+  CSX_HIP(d_q.alloc(nq_pad * sizeof(float), st), "hipMallocAsync");
+  CSX_HIP(d_w.alloc(Wp.size() * sizeof(float), st), "hipMallocAsync");
+  CSX_HIP(d_out.alloc(out_elems * sizeof(double), st), "hipMallocAsync");
+  CSX_HIP(hipMemcpyAsync(d_q.as<void>(), qrev.data(), qrev.size() * sizeof(float),
+                         hipMemcpyHostToDevice, st), "hipMemcpyAsync");
+  CSX_HIP(hipMemcpyAsync(d_w.as<void>(), Wp.data(), Wp.size() * sizeof(float),
+                         hipMemcpyHostToDevice, st), "hipMemcpyAsync");
+  CSX_HIP(launch_debye_jacobian(ps->pts32.as<float4>(), ps->orig32.as<int>(), ps->type32.as<int>(),
+                                static_cast<int>(n), ps->seg32.as<int>(), T, d_q.as<float>(),
+                                d_w.as<float>(), static_cast<int>(nq_pad), d_out.as<double>(), st,
+                                call.timer()),
+          "debye_jacobian_f32");
+  CSX_HIP(hipMemcpyAsync(out, d_out.as<void>(), nq * 3 * n * sizeof(double), hipMemcpyDeviceToHost,
+                         st), "hipMemcpyAsync");
+  d_q.reset();
+  d_w.reset();
+  d_out.reset();
+  return call.finish();
+}
 
 // out: jacobian ? nq x n x 3 : n x 3 (weights applied).
 static csx_status adjoint(csx_structure* s, const double* q, size_t nq, const double* weights,
@@ -28,62 +184,8 @@ static csx_status adjoint(csx_structure* s, const double* q, size_t nq, const do
   CSX_TRY(call.begin());
   PointSet* ps = nullptr;
   CSX_TRY(get_point_set(s, nullptr, &ps));
-  const int T = ps->ntypes;
-  const int qt = adjoint_q_tile(jacobian);
-  const size_t nq_pad = (nq + qt - 1) / qt * qt;
-  std::vector<float> qrev(nq_pad, 0.0f), Wp(static_cast<size_t>(T) * T * nq_pad, 0.0f);
-  std::vector<double> A(static_cast<size_t>(T) * nq);
-  for (size_t k = 0; k < nq; ++k) {
-    qrev[k] = static_cast<float>(q[k] / (2.0 * kPi));
-    const double g = p ? excluded_volume_scale(q[k], p->c1) : 0.0;
-    for (int t = 0; t < T; ++t) {
-      if (!p) {
-        A[t * nq + k] = type_form_factor(s->types[t], implicit_h, q[k]);
-      } else {
-        const Amplitudes a = debye_amplitudes(s->types[t], implicit_h, q[k],
-                                              p->shell_thickness_angstrom, p->bulk_density_e_per_a3);
-        A[t * nq + k] = a.v - g * a.e + p->c2 * a.h;
-      }
-    }
-    // The kernel's u = 2 pi (q/2pi)_f32 r: use the same effective q in W'.
-    const double qe = 2.0 * kPi * static_cast<double>(qrev[k]);
-    if (qe == 0.0) continue;
-    const double w = jacobian ? 1.0 : weights[k];
-    for (int a = 0; a < T; ++a)
-      for (int b = 0; b < T; ++b)
-        Wp[(static_cast<size_t>(a) * T + b) * nq_pad + k] =
-            static_cast<float>(2.0 * w * A[a * nq + k] * A[b * nq + k] / qe);
-  }
-  const hipStream_t st = call.stream();
-  const size_t nqc = nq_pad / qt;
-  const size_t out_elems = jacobian ? nq_pad * 3 * n : nqc * 3 * n;
-  DevBuf d_q, d_w, d_out;
-  CSX_HIP(d_q.alloc(nq_pad * sizeof(float), st), "hipMallocAsync");
-  CSX_HIP(d_w.alloc(Wp.size() * sizeof(float), st), "hipMallocAsync");
-  CSX_HIP(d_out.alloc(out_elems * sizeof(double), st), "hipMallocAsync");
-  CSX_HIP(hipMemcpyAsync(d_q.as<void>(), qrev.data(), qrev.size() * sizeof(float),
-                         hipMemcpyHostToDevice, st), "hipMemcpyAsync");
-  CSX_HIP(hipMemcpyAsync(d_w.as<void>(), Wp.data(), Wp.size() * sizeof(float),
-                         hipMemcpyHostToDevice, st), "hipMemcpyAsync");
-  CSX_HIP(launch_debye_adjoint(ps->pts32.as<float4>(), ps->orig32.as<int>(), ps->type32.as<int>(),
-                               static_cast<int>(n), ps->seg32.as<int>(), T, d_q.as<float>(),
-                               d_w.as<float>(), static_cast<int>(nq_pad), jacobian,
-                               d_out.as<double>(), st, call.timer()),
-          "debye_adjoint_f32");
-  std::vector<double> res(out_elems);
-  CSX_HIP(hipMemcpyAsync(res.data(), d_out.as<void>(), out_elems * sizeof(double),
-                         hipMemcpyDeviceToHost, st), "hipMemcpyAsync");
-  d_q.reset();
-  d_w.reset();
-  d_out.reset();
-  CSX_TRY(call.finish());
-  if (jacobian) {
-    std::copy(res.begin(), res.begin() + nq * 3 * n, out);
-  } else {
-    for (size_t c = 0; c < nqc; ++c)
-      for (size_t k = 0; k < 3 * n; ++k) out[k] += res[c * 3 * n + k];
-  }
-  return CSX_OK;
+  return jacobian ? jacobian_run(call, s, ps, q, nq, p, implicit_h, out)
+                  : vjp(call, s, ps, q, nq, weights, p, implicit_h, out);
 }
 
 }  // namespace csx

@@ -5,16 +5,24 @@
 // I(q) = sum_{i,j} A_i A_j sinc(q r_ij), A_i = occ_i A_{type(i)}(q), so
 //   dI/dx_i = sum_{j != i} W'_{ab}(q) occ_i occ_j (u cos u - sin u) / r^3 (x_i - x_j)
 // with u = q r_ij and W'_{ab}(q) = 2 A_a(q) A_b(q) / q (zero at q = 0).
-// debye_adjoint_f32: a workgroup owns 256 consecutive type-sorted atoms (one
-// per lane) and a chunk of QT q values and sweeps every other atom j through
-// LDS tiles, one type segment at a time (W' of the lane's type and the
-// segment's type for the chunk is held in registers).  sin and cos come from
-// V_SIN_F32 / V_COS_F32 in revolutions after an exact reduction; FP32 tile
-// sums are folded into FP64 per tile.
-//  JAC = false: the q weights w_q are folded into W' on the host; the kernel
-//               writes out[qchunk][atom][3] (summed over chunks on the host).
-//  JAC = true : out[q][atom][3], the full Jacobian.
+//
+// debye_vjp_f32 (sum_q w_q dI(q)/dx): every unordered pair once.  Work items
+// are pairs (I <= J) of type-pure 256-atom blocks; lane l owns atom I_l and
+// at step t of a sweep meets atom J_((t + l) mod 256), so the lanes of a wave
+// touch 64 distinct j slots and the j-side sums go into per-wave LDS arrays
+// without bank conflicts or races (fixed order: deterministic).  With the host
+// tables B_ab(q) = w_q W'_ab(q) and A_ab(q) = q B_ab(q) the q loop is
+//   v = q r / 2 pi,  C += A cos 2 pi v,  S += B sin 2 pi v,  F = r C - S,
+// i.e. two FMAs and one V_SIN_F32 / V_COS_F32 pair per pair.q.  FP32 sums
+// per q chunk are folded into FP64 partials per (item, side), which a second
+// kernel adds per block in fixed order.
+//
+// debye_jacobian_f32 (dI(q)/dx for every q): a workgroup owns 256
+// type-sorted atoms and QT q values and sweeps all other atoms (each pair is
+// visited from both ends; the output is q x n x 3 anyway).
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
 
 #include "kernels.hpp"
 
@@ -22,13 +30,163 @@ namespace csx {
 namespace {
 
 constexpr int kAdjBlock = 256;
+constexpr float kTwoPi = 6.283185307179586f;
 
-template <int QT, bool JAC>
-__global__ __launch_bounds__(kAdjBlock) void debye_adjoint_f32(
+// sin and cos of 2 pi v, v in revolutions: SM 2 takes V_SIN_F32 / V_COS_F32
+// on v itself (|v| <= 256, checked on the host), SM 1 after the exact
+// reduction v - rint(v).
+template <int SM>
+__device__ inline void sincos_rev(float v, float& s, float& c) {
+  const float f = SM == 2 ? v : v - __builtin_rintf(v);
+  s = __builtin_amdgcn_sinf(f);
+  c = __builtin_amdgcn_cosf(f);
+}
+
+template <int QT, int SM>
+__global__ __launch_bounds__(kAdjBlock) void debye_vjp_f32(
+    const float4* __restrict__ pts, const int4* __restrict__ blocks /* start, len, type, 0 */,
+    const int2* __restrict__ items /* (I, J), I <= J */, int ntypes,
+    const float* __restrict__ qrev, const float* __restrict__ AB /* [T][T][2][nq_pad] */,
+    int nq_pad, int qsplit, double* __restrict__ part /* [items][qsplit][2][3][256] */) {
+  __shared__ float4 tile[kAdjBlock];
+  __shared__ float accj[4][3][kAdjBlock];
+  __shared__ float qab[3][QT];
+  const int tid = threadIdx.x, wave = tid >> 6;
+  const int2 it = items[blockIdx.x];
+  const int4 bi = blocks[it.x], bj = blocks[it.y];
+  const bool diag = it.x == it.y;
+  const float4 zero = make_float4(0.f, 0.f, 0.f, 0.f);
+  const float4 pi = tid < bi.y ? pts[bi.x + tid] : zero;
+  tile[tid] = tid < bj.y ? pts[bj.x + tid] : zero;
+#pragma unroll
+  for (int w = 0; w < 4; ++w)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) accj[w][c][tid] = 0.f;
+  const float* A = AB + (size_t(bi.z) * ntypes + bj.z) * 2 * nq_pad;
+  const float* B = A + nq_pad;
+  double gi64[3] = {0.0, 0.0, 0.0}, gj64[3] = {0.0, 0.0, 0.0};
+  // this workgroup's share of the q chunks
+  const int nqc = nq_pad / QT;
+  const int c0 = int(blockIdx.y) * nqc / qsplit, c1 = (int(blockIdx.y) + 1) * nqc / qsplit;
+  __syncthreads();
+  for (int qc = c0 * QT; qc < c1 * QT; qc += QT) {
+    // The chunk's q, A and B pass through LDS so that they sit in VGPRs:
+    // as uniform global loads they would take 3 QT SGPRs and spill at QT 32.
+    if (tid < QT) {
+      qab[0][tid] = qrev[qc + tid];
+      qab[1][tid] = A[qc + tid];
+      qab[2][tid] = B[qc + tid];
+    }
+    __syncthreads();
+    float qk[QT], ak[QT], bk[QT];
+#pragma unroll
+    for (int k = 0; k < QT; ++k) {
+      qk[k] = qab[0][k];
+      ak[k] = qab[1][k];
+      bk[k] = qab[2][k];
+    }
+    float gi[3] = {0.f, 0.f, 0.f};
+    float4 pnext = tile[tid];  // the next step's j, read one step ahead
+    for (int t = 0; t < kAdjBlock; ++t) {
+      const int jj = (t + tid) & (kAdjBlock - 1);
+      const float4 pj = pnext;
+      pnext = tile[(t + 1 + tid) & (kAdjBlock - 1)];
+      const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+      const float d2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
+      // coincident points have no gradient; a diagonal block counts j > i only
+      const bool use = d2 > 1e-20f && (!diag || jj > tid);
+      const float r = __builtin_sqrtf(use ? d2 : 1.0f);
+      const float ir = __builtin_amdgcn_rcpf(r);
+      const float wgt = use ? pi.w * pj.w * ir * ir * ir : 0.f;
+      float C = 0.f, S = 0.f;
+#pragma unroll
+      for (int k = 0; k < QT; ++k) {
+        float sn, cs;
+        sincos_rev<SM>(qk[k] * r, sn, cs);
+        C = __builtin_fmaf(ak[k], cs, C);
+        S = __builtin_fmaf(bk[k], sn, S);
+      }
+      const float h = __builtin_fmaf(r, C, -S) * wgt;
+      gi[0] = __builtin_fmaf(h, dx, gi[0]);
+      gi[1] = __builtin_fmaf(h, dy, gi[1]);
+      gi[2] = __builtin_fmaf(h, dz, gi[2]);
+      // Plain read-modify-write: slot jj belongs to this lane at this step
+      // and LDS operations of one wave complete in order (LDS atomics here
+      // measured 2.2x slower than the whole kernel without them).
+      accj[wave][0][jj] -= h * dx;
+      accj[wave][1][jj] -= h * dy;
+      accj[wave][2][jj] -= h * dz;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      gi64[c] += double(gi[c]);
+      gj64[c] += ((double(accj[0][c][tid]) + double(accj[1][c][tid])) + double(accj[2][c][tid])) +
+                 double(accj[3][c][tid]);
+#pragma unroll
+      for (int w = 0; w < 4; ++w) accj[w][c][tid] = 0.f;
+    }
+    __syncthreads();
+  }
+  double* out = part + (size_t(blockIdx.x) * qsplit + blockIdx.y) * 2 * 3 * kAdjBlock;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    out[c * kAdjBlock + tid] = gi64[c];
+    out[(3 + c) * kAdjBlock + tid] = gj64[c];
+  }
+}
+
+// Position of block pair (I, J), I <= J, in the row-major upper triangle.
+__device__ inline size_t item_index(int I, int J, int nb) {
+  const size_t i = size_t(I);
+  return i * size_t(nb) - (i * (i > 0 ? i - 1 : 0)) / 2 + size_t(J - I);
+}
+
+// gacc[B][c][l] += the partials of items [k0, k1) that hold block B: the
+// i side of (B, J), J ascending, then the j side of (I, B), I ascending.
+__global__ void reduce_vjp(const double* __restrict__ part, size_t k0, size_t k1, int qsplit,
+                           int nb, double* __restrict__ gacc) {
+  const int Bk = blockIdx.x, tid = threadIdx.x;
+  double g[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) g[c] = gacc[(size_t(Bk) * 3 + c) * kAdjBlock + tid];
+  for (int side = 0; side < 2; ++side) {
+    for (int o = side == 0 ? Bk : 0; o < (side == 0 ? nb : Bk + 1); ++o) {
+      const size_t k = side == 0 ? item_index(Bk, o, nb) : item_index(o, Bk, nb);
+      if (k < k0 || k >= k1) continue;
+      for (int s = 0; s < qsplit; ++s) {
+        const double* p = part + (((k - k0) * qsplit + s) * 2 + side) * 3 * kAdjBlock;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) g[c] += p[c * kAdjBlock + tid];
+      }
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < 3; ++c) gacc[(size_t(Bk) * 3 + c) * kAdjBlock + tid] = g[c];
+}
+
+__global__ void zero_f64(double* p, size_t n) {
+  const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0.0;
+}
+
+// out[orig atom][c] = gacc of its block slot.
+__global__ void scatter_vjp(const double* __restrict__ gacc, const int4* __restrict__ blocks,
+                            const int* __restrict__ orig, double* __restrict__ out) {
+  const int Bk = blockIdx.x, tid = threadIdx.x;
+  const int4 b = blocks[Bk];
+  if (tid >= b.y) return;
+  const int o = orig[b.x + tid];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) out[size_t(o) * 3 + c] = gacc[(size_t(Bk) * 3 + c) * kAdjBlock + tid];
+}
+
+template <int QT>
+__global__ __launch_bounds__(kAdjBlock) void debye_jacobian_f32(
     const float4* __restrict__ pts, const int* __restrict__ orig, const int* __restrict__ type,
     int npts, const int* __restrict__ seg_start, int ntypes, const float* __restrict__ qrev,
     const float* __restrict__ Wp /* [ntypes][ntypes][nq_pad] */, int nq_pad,
-    double* __restrict__ out) {
+    double* __restrict__ out /* [nq_pad][npts][3] */) {
   __shared__ float4 tile[kAdjBlock];
   const int i = blockIdx.x * kAdjBlock + threadIdx.x;
   const int qc = blockIdx.y * QT;
@@ -38,17 +196,15 @@ __global__ __launch_bounds__(kAdjBlock) void debye_adjoint_f32(
   float qk[QT];
 #pragma unroll
   for (int k = 0; k < QT; ++k) qk[k] = qrev[qc + k];
-  constexpr int NACC = JAC ? QT : 1;
-  double g64[NACC][3];
-  float g[NACC][3];
+  double g64[QT][3];
+  float g[QT][3];
 #pragma unroll
-  for (int k = 0; k < NACC; ++k)
+  for (int k = 0; k < QT; ++k)
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       g64[k][c] = 0.0;
       g[k][c] = 0.f;
     }
-  const float two_pi = 6.283185307179586f;
   for (int b = 0; b < ntypes; ++b) {
     float w[QT];
 #pragma unroll
@@ -69,33 +225,20 @@ __global__ __launch_bounds__(kAdjBlock) void debye_adjoint_f32(
         const float r = __builtin_sqrtf(use ? d2 : 1.0f);
         const float ir = __builtin_amdgcn_rcpf(r);
         const float ir3 = use ? pj.w * ir * ir * ir : 0.f;
-        float F = 0.f;
 #pragma unroll
         for (int k = 0; k < QT; ++k) {
-          const float v = qk[k] * r;                 // revolutions
-          const float f = v - __builtin_rintf(v);    // exact reduction
-          const float s = __builtin_amdgcn_sinf(f);
-          const float c = __builtin_amdgcn_cosf(f);
-          const float u = two_pi * v;                // radians
-          const float term = __builtin_fmaf(u, c, -s);
-          if (JAC) {
-            const float h = w[k] * term * ir3;
-            g[k % NACC][0] = __builtin_fmaf(h, dx, g[k % NACC][0]);
-            g[k % NACC][1] = __builtin_fmaf(h, dy, g[k % NACC][1]);
-            g[k % NACC][2] = __builtin_fmaf(h, dz, g[k % NACC][2]);
-          } else {
-            F = __builtin_fmaf(w[k], term, F);
-          }
-        }
-        if (!JAC) {
-          const float h = F * ir3;
-          g[0][0] = __builtin_fmaf(h, dx, g[0][0]);
-          g[0][1] = __builtin_fmaf(h, dy, g[0][1]);
-          g[0][2] = __builtin_fmaf(h, dz, g[0][2]);
+          const float v = qk[k] * r;  // revolutions
+          float sn, cs;
+          sincos_rev<1>(v, sn, cs);
+          const float term = __builtin_fmaf(kTwoPi * v, cs, -sn);
+          const float h = w[k] * term * ir3;
+          g[k][0] = __builtin_fmaf(h, dx, g[k][0]);
+          g[k][1] = __builtin_fmaf(h, dy, g[k][1]);
+          g[k][2] = __builtin_fmaf(h, dz, g[k][2]);
         }
       }
 #pragma unroll
-      for (int k = 0; k < NACC; ++k)
+      for (int k = 0; k < QT; ++k)
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
           g64[k][c] += double(g[k][c]);
@@ -106,37 +249,72 @@ __global__ __launch_bounds__(kAdjBlock) void debye_adjoint_f32(
   if (!vi) return;
   const double oi = double(pi.w);
   const int o = orig[i];
-  if (JAC) {
 #pragma unroll
-    for (int k = 0; k < QT; ++k)
+  for (int k = 0; k < QT; ++k)
 #pragma unroll
-      for (int c = 0; c < 3; ++c) out[(size_t(qc + k) * npts + o) * 3 + c] = oi * g64[k][c];
-  } else {
-#pragma unroll
-    for (int c = 0; c < 3; ++c) out[(size_t(blockIdx.y) * npts + o) * 3 + c] = oi * g64[0][c];
-  }
+    for (int c = 0; c < 3; ++c) out[(size_t(qc + k) * npts + o) * 3 + c] = oi * g64[k][c];
 }
+
+constexpr int kJacQT = 8;
 
 }  // namespace
 
-hipError_t launch_debye_adjoint(const float4* pts, const int* orig, const int* type, int npts,
-                                const int* seg_start, int ntypes, const float* qrev,
-                                const float* Wp, int nq_pad, bool jacobian, double* out,
-                                hipStream_t stream, KernelTimer* timer) {
+int jacobian_q_tile() { return kJacQT; }
+
+// 16 or 32 q per pair sweep, whichever needs fewer cycles: about 7 per pair.q
+// (one multiply, V_SIN and V_COS at two cycles each, two FMAs) plus about 45
+// per pair (distance, correctly rounded sqrt, weights, the three sums).
+int vjp_q_tile(size_t nq) {
+  const size_t c16 = (nq + 15) / 16 * (16 * 7 + 45), c32 = (nq + 31) / 32 * (32 * 7 + 45);
+  return c32 < c16 ? 32 : 16;
+}
+
+hipError_t launch_debye_jacobian(const float4* pts, const int* orig, const int* type, int npts,
+                                 const int* seg_start, int ntypes, const float* qrev,
+                                 const float* Wp, int nq_pad, double* out, hipStream_t stream,
+                                 KernelTimer* timer) {
   if (npts <= 0) return hipSuccess;
-  const int qt = jacobian ? 8 : 32;
-  if (nq_pad % qt != 0) return hipErrorInvalidValue;
-  dim3 grid((npts + kAdjBlock - 1) / kAdjBlock, nq_pad / qt);
-  TimerScope scope(timer, "debye_adjoint_f32", stream);
-  if (jacobian)
-    debye_adjoint_f32<8, true><<<grid, kAdjBlock, 0, stream>>>(pts, orig, type, npts, seg_start,
-                                                               ntypes, qrev, Wp, nq_pad, out);
-  else
-    debye_adjoint_f32<32, false><<<grid, kAdjBlock, 0, stream>>>(pts, orig, type, npts, seg_start,
-                                                                 ntypes, qrev, Wp, nq_pad, out);
+  if (nq_pad % kJacQT != 0) return hipErrorInvalidValue;
+  dim3 grid((npts + kAdjBlock - 1) / kAdjBlock, nq_pad / kJacQT);
+  TimerScope scope(timer, "debye_jacobian_f32", stream);
+  debye_jacobian_f32<kJacQT><<<grid, kAdjBlock, 0, stream>>>(pts, orig, type, npts, seg_start,
+                                                             ntypes, qrev, Wp, nq_pad, out);
   return hipGetLastError();
 }
 
-int adjoint_q_tile(bool jacobian) { return jacobian ? 8 : 32; }
+hipError_t launch_debye_vjp(const float4* pts, const int4* blocks, int nb, const int2* items,
+                            size_t nitems, const int* orig, int ntypes, const float* qrev,
+                            const float* AB, int nq_pad, int qt, int sin_mode, int qsplit,
+                            size_t items_per_launch, double* part, double* gacc, double* out,
+                            hipStream_t stream, KernelTimer* timer) {
+  if (nb <= 0) return hipSuccess;
+  if ((qt != 16 && qt != 32) || nq_pad % qt != 0 || qsplit < 1 || qsplit > nq_pad / qt ||
+      items_per_launch == 0)
+    return hipErrorInvalidValue;
+  const size_t nacc = size_t(nb) * 3 * kAdjBlock;
+  zero_f64<<<(nacc + 255) / 256, 256, 0, stream>>>(gacc, nacc);
+  for (size_t k0 = 0; k0 < nitems; k0 += items_per_launch) {
+    const size_t k1 = std::min(nitems, k0 + items_per_launch);
+    dim3 grid(static_cast<unsigned>(k1 - k0), qsplit);
+    {
+      TimerScope scope(timer, "debye_vjp_f32", stream);
+#define CSX_VJP(QT_, SM_)                                                               \
+  debye_vjp_f32<QT_, SM_><<<grid, kAdjBlock, 0, stream>>>(pts, blocks, items + k0, ntypes, qrev, \
+                                                          AB, nq_pad, qsplit, part)
+      if (qt == 32 && sin_mode == 2)
+        CSX_VJP(32, 2);
+      else if (qt == 32)
+        CSX_VJP(32, 1);
+      else if (sin_mode == 2)
+        CSX_VJP(16, 2);
+      else
+        CSX_VJP(16, 1);
+#undef CSX_VJP
+    }
+    reduce_vjp<<<nb, kAdjBlock, 0, stream>>>(part, k0, k1, qsplit, nb, gacc);
+  }
+  scatter_vjp<<<nb, kAdjBlock, 0, stream>>>(gacc, blocks, orig, out);
+  return hipGetLastError();
+}
 
 }  // namespace csx

@@ -142,6 +142,7 @@ static csx_status build_point_set(csx_structure* s, const csx_solvent_params* p,
   CSX_HIP(ps.orig32.upload(orig32.data(), orig32.size() * sizeof(int)), "upload order");
   CSX_HIP(ps.type32.upload(type32.data(), type32.size() * sizeof(int)), "upload types");
   CSX_HIP(ps.seg32.upload(seg_start.data(), seg_start.size() * sizeof(int)), "upload segments");
+  ps.h_seg = seg_start;
   CSX_HIP(ps.pts32.upload(pts32.data(), pts32.size() * sizeof(float)), "upload points");
   CSX_HIP(ps.items.upload(items.data(), items.size() * sizeof(int)), "upload items");
   CSX_HIP(ps.bin_first.upload(bin_first.data(), bin_first.size() * sizeof(int)), "upload bins");
@@ -559,6 +560,14 @@ csx_status csx_ctx_set_tuning(csx_ctx* ctx, int q_tile, int sin_mode, int j_chun
   ctx->tuning.q_tile = q_tile;
   ctx->tuning.sin_mode = sin_mode;
   ctx->tuning.j_chunk = j_chunk;
+  return CSX_OK;
+}
+
+csx_status csx_ctx_set_workspace_limit(csx_ctx* ctx, size_t bytes) {
+  if (!ctx) return fail(CSX_INVALID_INPUT, "context is null");
+  if (bytes < (size_t(1) << 20)) return fail(CSX_INVALID_INPUT, "workspace limit must be >= 1 MiB");
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  ctx->tuning.workspace_limit = bytes;
   return CSX_OK;
 }
 

@@ -33,6 +33,7 @@ struct Tuning {
   int sin_mode = 2;    // 0 software polynomial, 1 reduced + V_SIN_F32, 2 V_SIN_F32
   int j_chunk = 512;   // j atoms per work item
   int persist = 0;     // persistent launch with this many workgroups per CU (0: grid)
+  size_t workspace_limit = size_t(2) << 30;  // bytes of adjoint partials per launch
 };
 
 }  // namespace csx
@@ -129,6 +130,7 @@ struct PointSet {
   DevArray type32;            // int[npts]: type of each sorted point
   DevArray seg32;             // int[ntypes + 1]: type segments of the sorted order
   std::vector<int> h_items, h_bin_first;  // host copies (batch concatenation)
+  std::vector<int> h_seg;                  // host copy of the type segments
   // FP64 plain layout: original order, atoms then sites.
   DevArray pts64;             // double4[natoms + nsites]
   DevArray type64;            // int[natoms + nsites]

@@ -108,8 +108,9 @@ void csx_ctx_destroy(csx_ctx *ctx);
 
 /* Kernel timing: when enabled, the library brackets every launch of the
  * named kernels with HIP events on the stream they run on and accumulates
- * their durations.  `name` is one of "debye_pairs_f32", "debye_plain_f64",
- * "multipole_coeff", "debye_adjoint_f32", "ensemble_pairs_f32". */
+ * their durations.  `name` is one of "debye_pairs_f32" (every FP32 pair
+ * table launch, batches included), "debye_plain_f64", "multipole_coeff",
+ * "debye_vjp_f32", "debye_jacobian_f32". */
 csx_status csx_ctx_set_timing(csx_ctx *ctx, int enabled);
 csx_status csx_ctx_kernel_time(csx_ctx *ctx, const char *name,
                                int64_t *launches, double *total_ms);
@@ -132,6 +133,11 @@ csx_status csx_ctx_set_tuning(csx_ctx *ctx, int q_tile, int sin_mode,
  * first launch.  Any output may be NULL. */
 csx_status csx_ctx_last_launch(csx_ctx *ctx, int *q_tile, int *sin_mode,
                                int *persistent_workgroups);
+
+/* Upper bound (bytes, >= 1 MiB; default 2 GiB) on the partial-sum
+ * workspace of one coordinate-adjoint launch; larger problems run as
+ * several launches.  No reference counterpart. */
+csx_status csx_ctx_set_workspace_limit(csx_ctx *ctx, size_t bytes);
 
 /* Persistent launch of the FP32 pair kernel: a fixed grid of
  * workgroups_per_cu x CU workgroups drains a global task counter (0 = one
