@@ -19,7 +19,8 @@ from .api import (  # noqa: F401
 from .fitting import (  # noqa: F401
     BackgroundMode, ComparisonResult, EnsembleFitter, EnsembleOptions, EnsembleResult,
     ExperimentalCurve, FitOptions, FitParams, FitResult, Optimizer, Quality, SaxsFitter,
-    SaxsScorer, ScaleMode, ScoreOptions, ScoreResult, WeightInterval, WeightSolver, chi2_grad,
+    SaxsScorer, ScaleMode, ScoreOptions, ScoreResult, WeightInterval, WeightSolver,
+    bootstrap_indices, chi2_grad,
     compare_curves, cubic_hermite_interpolate, fit_basis, fit_from_basis, fit_to_experimental,
     fit_to_experimental_with_options, nnls_weights, optimize_weights, project_simplex,
     solve_scale_background,

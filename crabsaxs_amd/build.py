@@ -34,6 +34,7 @@ SOURCES = [
     ("multipole.cpp", "host"),
     ("fit.cpp", "host"),
     ("adjoint.cpp", "host"),
+    ("rng.cpp", "host"),
     ("debye.hip", "device"),
     ("multipole.hip", "device"),
     ("adjoint.hip", "device"),
@@ -71,5 +72,21 @@ def build(verbose: bool = False) -> Path:
     return LIB
 
 
+def build_cpp_suite() -> Path:
+    """The C++ API test program tests/cpp/reference_suite (host compiler,
+    header-only API over the library just built)."""
+    root = PKG.parent
+    src = root / "tests" / "cpp" / "reference_suite.cpp"
+    out = src.with_suffix("")
+    cxx = shutil.which("g++") or "c++"
+    tmp = str(out) + ".tmp"
+    _run([cxx, "-std=c++17", "-O1", "-Wall", "-Wextra", f"-I{root / 'include'}", str(src),
+          "-o", tmp, f"-L{PKG}", "-lcrabsaxs_hip", "-Wl,-rpath,$ORIGIN/../../crabsaxs_amd",
+          f"-Wl,-rpath-link,{ROCM}/lib"])
+    os.replace(tmp, out)
+    return out
+
+
 if __name__ == "__main__":
     print(build(verbose="-v" in sys.argv))
+    print(build_cpp_suite())

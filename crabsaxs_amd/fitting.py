@@ -386,6 +386,15 @@ def nnls_weights(curves, experimental: ExperimentalCurve, options: FitOptions = 
     return w, float(out3[0]), float(out3[1]), float(out3[2]), fitted
 
 
+def bootstrap_indices(seed: int, n: int, replicates: int) -> np.ndarray:
+    """The resampling draws of EnsembleFitter::fit (ensemble.rs:184-216):
+    ChaCha8Rng::seed_from_u64(seed), gen_range(0..n); [replicates, n]."""
+    out = np.zeros(replicates * n, dtype=np.uint64)
+    _check(_lib.load().csx_bootstrap_indices(int(seed) & (2**64 - 1), n, replicates,
+                                             out.ctypes.data_as(C.POINTER(C.c_uint64))))
+    return out.reshape(replicates, n).astype(np.int64)
+
+
 class WeightSolver(enum.IntEnum):
     """Ensemble weight solver: the reference's projected gradient
     (ensemble.rs:317-381) or exact non-negative least squares."""
@@ -521,14 +530,12 @@ class EnsembleFitter:
         intervals = [WeightInterval(x, x, x) for x in w]
         boot_chi = []
         if opts.bootstrap > 0:
-            # Resampling stream: numpy PCG64 seeded with opts.seed (the
-            # reference draws from ChaCha8; the procedure is the same).
-            rng = np.random.Generator(np.random.PCG64(opts.seed))
             e = self.experimental
             resid = (e.intensity - fitted) / e.sigma
             samples = [[] for _ in selected]
-            for _ in range(opts.bootstrap):
-                pick = rng.integers(0, len(resid), size=len(e))
+            picks = bootstrap_indices(opts.seed, len(e), opts.bootstrap)
+            for r in range(opts.bootstrap):
+                pick = picks[r]
                 synth = ExperimentalCurve(e.q, fitted + resid[pick] * e.sigma, e.sigma,
                                           e.has_errors)
                 rw, _, rchi, _ = self._fit_core(selected, synth, ctx)

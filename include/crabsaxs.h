@@ -452,6 +452,19 @@ csx_status csx_ensemble_fit_core(const double *bases, size_t m,
                                  double *weights, csx_fit_params *params,
                                  double *chi2, double *fitted);
 
+/* The bootstrap resampling indices of EnsembleFitter::fit
+ * (ensemble.rs:184-216): out[r*n + i] = the r-th replicate's draw for point
+ * i from ChaCha8Rng::seed_from_u64(seed) and gen_range(0..n), in the
+ * reference's order (replicate-major).  out: replicates * n. */
+csx_status csx_bootstrap_indices(uint64_t seed, size_t n, size_t replicates,
+                                 uint64_t *out);
+
+/* The ChaCha block function under that stream (RFC 7539 section 2.3 with a
+ * 64-bit block counter in state words 12-13 and a 64-bit stream in 14-15):
+ * 16 output words for key[8], `rounds` (even) rounds. */
+csx_status csx_chacha_block(const uint32_t *key, uint64_t counter,
+                            uint64_t stream, int rounds, uint32_t *out);
+
 /* ---- synthetic workloads (no reference counterpart) -------------------- */
 
 /* Seeded synthetic globular protein: self-avoiding C-alpha random walk with

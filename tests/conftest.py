@@ -21,6 +21,11 @@ def _built_artifacts():
     lib = ROOT / "crabsaxs_amd" / "libcrabsaxs_hip.so"
     if not lib.exists() and os.path.exists("/opt/rocm/bin/hipcc"):
         subprocess.run([sys.executable, "-m", "crabsaxs_amd.build"], cwd=ROOT, check=True)
+    suite = ROOT / "tests" / "cpp" / "reference_suite"
+    if lib.exists() and not suite.exists():
+        subprocess.run([sys.executable, "-c",
+                        "from crabsaxs_amd.build import build_cpp_suite; build_cpp_suite()"],
+                       cwd=ROOT, check=True)
     oracle = ROOT / "oracle" / "liboracle.so"
     if not oracle.exists():
         subprocess.run(["make", "-s", "-C", str(ROOT / "oracle")], check=True)

@@ -217,3 +217,44 @@ def test_nnls_weights_reject_fixed_scale_and_regularization():
     with pytest.raises(csx.InvalidInput):
         csx.nnls_weights(curves, curve(q, y, 0.01 * y),
                          csx.FitOptions(scale_mode=csx.ScaleMode.Fixed(1.0)))
+
+
+def _chacha(key, counter, stream, rounds):
+    import ctypes as C
+    from crabsaxs_amd import _lib
+    k = (C.c_uint32 * 8)(*key)
+    out = (C.c_uint32 * 16)()
+    assert _lib.load().csx_chacha_block(k, counter, stream, rounds, out) == 0
+    return b"".join(int(w).to_bytes(4, "little") for w in out)
+
+
+def test_chacha_block_matches_published_vectors():
+    """The block function under the bootstrap stream: RFC 7539 section 2.3.2
+    (key 00..1f, counter 1, nonce 00:00:00:09:00:00:00:4a:00:00:00:00),
+    RFC 7539 A.1 #1 (all-zero key, nonce, counter) and the all-zero-key
+    ChaCha8 keystream block."""
+    key = [int.from_bytes(bytes(range(4 * i, 4 * i + 4)), "little") for i in range(8)]
+    got = _chacha(key, 0x0900000000000001, 0x4A000000, 20)
+    assert got.hex() == ("10f1e7e4d13b5915500fdd1fa32071c4c7d1f4c733c068030422aa9ac3d46c4e"
+                         "d2826446079faa0914c2d705d98b02a2b5129cd1de164eb9cbd083e8a2503c4e")
+    assert _chacha([0] * 8, 0, 0, 20).hex() == (
+        "76b8e0ada0f13d90405d6ae55386bd28bdd219b8a08ded1aa836efcc8b770dc7"
+        "da41597c5157488d7724e03fb8d84a376a43b8f41518a11cc387b669b2ee6586")
+    assert _chacha([0] * 8, 0, 0, 8).hex() == (
+        "3e00ef2f895f40d67f5bb8e81f09a5a12c840ec3ce9a7f3b181be188ef711a1e"
+        "984ce172b9216f419f445367456d5619314a42a3da86b001387bfdb80e0cfe42")
+
+
+def test_bootstrap_indices_stream():
+    a = csx.bootstrap_indices(0xC0FFEE, 150, 40)
+    assert a.shape == (40, 150) and a.min() >= 0 and a.max() < 150
+    assert np.array_equal(a, csx.bootstrap_indices(0xC0FFEE, 150, 40))
+    # replicate-major: fewer replicates are a prefix of the same stream
+    assert np.array_equal(csx.bootstrap_indices(0xC0FFEE, 150, 3), a[:3])
+    assert not np.array_equal(csx.bootstrap_indices(1, 150, 3), a[:3])
+    # uniform: chi-square over 150 cells of 6000 draws
+    counts = np.bincount(a.ravel(), minlength=150)
+    chi = np.sum((counts - 40.0) ** 2 / 40.0)
+    assert chi < 220, chi
+    # n = 1 always draws 0
+    assert np.all(csx.bootstrap_indices(7, 1, 5) == 0)
