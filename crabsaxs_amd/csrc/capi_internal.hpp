@@ -105,9 +105,10 @@ class DevArray {
 // occupancies in original order (atoms, then sites) and the FP32 real
 // spherical harmonics in MFMA B-operand tiles.
 struct MultipoleGeometry {
-  int L = -1, ntiles = 0, npad = 0, npts = 0, max_l_range = 1;
+  int L = -1, ntiles = 0, npad = 0, npts = 0;
   double rmax_atoms = 0.0, rmax_sites = 0.0;
   DevArray radius, type, occ, Y, tile_l;
+  std::vector<int> h_tile_l;  // l of each column tile
 };
 
 // One set of scattering points (atoms, optionally followed by hydration

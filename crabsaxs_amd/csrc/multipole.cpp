@@ -7,20 +7,29 @@
 
 namespace csx {
 
-int multipole_tiles_per_group();
+int multipole_chunk();
+int multipole_tiles_per_group(int nch);
 size_t multipole_coeff_lds_bytes(int l_range, int nch);
 hipError_t launch_multipole_geometry(const double* xyz, int npts, const double* centre, int l_max,
-                                     const double* norms, const int* tile_first, int npad,
-                                     float* radius, float* Y, hipStream_t stream);
+                                     const double* norms, const int* tile_first, int ntiles,
+                                     int npad, float* radius, float* Y, hipStream_t stream);
 hipError_t launch_multipole_coeff(const float* radius, const int* type, const float* occ, int npts,
                                   int npad, const float* Y, int ntiles, const int* tile_l,
                                   int max_l_range, const float* amp, const float* qv,
-                                  const int* qlmax, int nqp, int nch, int chunk, float* partial,
-                                  double* out, hipStream_t stream, KernelTimer* timer);
+                                  const int* qlmax, int nqp, int nch, float* partial, double* out,
+                                  hipStream_t stream, KernelTimer* timer);
 
 namespace {
 
-constexpr int kChunk = 2048;  // points per coefficient workgroup
+// Largest l range of a column-tile group (the LDS rows one workgroup needs).
+int max_group_l_range(const std::vector<int>& tile_l, int tiles_per_group) {
+  int r = 1;
+  for (size_t g0 = 0; g0 < tile_l.size(); g0 += tiles_per_group) {
+    const size_t g1 = std::min(tile_l.size(), g0 + tiles_per_group);
+    r = std::max(r, tile_l[g1 - 1] - tile_l[g0] + 1);
+  }
+  return r;
+}
 
 // harmonic_norms (spherical.rs:33-51).
 std::vector<double> harmonic_norms(int L) {
@@ -64,11 +73,7 @@ static csx_status get_geometry(Call& call, csx_structure* s, PointSet& ps, int L
   }
   tile_first[L + 1] = static_cast<int>(tile_l.size());
   g->ntiles = static_cast<int>(tile_l.size());
-  const int tpg = multipole_tiles_per_group();
-  for (int g0 = 0; g0 < g->ntiles; g0 += tpg) {
-    const int g1 = std::min(g0 + tpg, g->ntiles);
-    g->max_l_range = std::max(g->max_l_range, tile_l[g1 - 1] - tile_l[g0] + 1);
-  }
+  g->h_tile_l = tile_l;
   std::vector<double> xyz(3 * static_cast<size_t>(g->npts));
   std::vector<int> type(g->npts);
   std::vector<float> occ(g->npts);
@@ -114,13 +119,14 @@ static csx_status get_geometry(Call& call, csx_structure* s, PointSet& ps, int L
   CSX_HIP(g->type.upload(type.data(), type.size() * sizeof(int)), "upload types");
   CSX_HIP(g->occ.upload(occ.data(), occ.size() * sizeof(float)), "upload occupancy");
   CSX_HIP(g->tile_l.upload(tile_l.data(), tile_l.size() * sizeof(int)), "upload tiles");
-  if (multipole_coeff_lds_bytes(g->max_l_range, 3) > 64 * 1024)
-    return fail(CSX_INVALID_INPUT, "multipole l_max too large for the LDS layout");
+  for (int nch : {1, 3})
+    if (multipole_coeff_lds_bytes(max_group_l_range(tile_l, multipole_tiles_per_group(nch)),
+                                  nch) > 64 * 1024)
+      return fail(CSX_INVALID_INPUT, "multipole l_max too large for the LDS layout");
   const size_t ybytes = static_cast<size_t>(g->ntiles) * g->npad * 16 * sizeof(float);
   CSX_HIP(g->Y.alloc(ybytes), "hipMalloc harmonics");
-  CSX_HIP(hipMemsetAsync(g->Y.as<void>(), 0, ybytes, st), "hipMemsetAsync");
   CSX_HIP(launch_multipole_geometry(d_xyz.as<double>(), g->npts, d_centre.as<double>(), L,
-                                    d_norms.as<double>(), d_first.as<int>(), g->npad,
+                                    d_norms.as<double>(), d_first.as<int>(), g->ntiles, g->npad,
                                     g->radius.as<float>(), g->Y.as<float>(), st),
           "multipole_geometry");
   d_xyz.reset();
@@ -186,7 +192,7 @@ static csx_status multipole_run(csx_structure* s, const double* q, size_t nq, in
     }
   }
   const int npts = mode == 0 ? ps->natoms : g->npts;
-  const int nchunk = (g->npad + kChunk - 1) / kChunk;
+  const int nchunk = (g->npad + multipole_chunk() - 1) / multipole_chunk();
   const hipStream_t st = call.stream();
   DevBuf d_amp, d_q, d_l, d_partial, d_out;
   CSX_HIP(d_amp.alloc(amp.size() * sizeof(float), st), "hipMallocAsync");
@@ -201,10 +207,11 @@ static csx_status multipole_run(csx_structure* s, const double* q, size_t nq, in
           "hipMemcpyAsync");
   CSX_HIP(hipMemcpyAsync(d_l.as<void>(), qlmax.data(), nqp * sizeof(int), hipMemcpyHostToDevice, st),
           "hipMemcpyAsync");
+  const int l_range = max_group_l_range(g->h_tile_l, multipole_tiles_per_group(nch));
   CSX_HIP(launch_multipole_coeff(g->radius.as<float>(), g->type.as<int>(), g->occ.as<float>(), npts,
                                  g->npad, g->Y.as<float>(), g->ntiles, g->tile_l.as<int>(),
-                                 g->max_l_range, d_amp.as<float>(), d_q.as<float>(), d_l.as<int>(),
-                                 static_cast<int>(nqp), nch, kChunk, d_partial.as<float>(),
+                                 l_range, d_amp.as<float>(), d_q.as<float>(), d_l.as<int>(),
+                                 static_cast<int>(nqp), nch, d_partial.as<float>(),
                                  d_out.as<double>(), st, call.timer()),
           "multipole_coeff");
   std::vector<double> res(nqp * nterms);

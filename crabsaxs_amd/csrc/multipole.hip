@@ -3,24 +3,31 @@
 //
 //  multipole_geometry  once per (point set, l_max): FP64 real spherical
 //                      harmonics by the associated-Legendre recurrence of
-//                      spherical.rs:54-93, stored FP32 in MFMA B-operand tiles
-//                      Y[tile][point][16]: each l owns ceil((2l+1)/16) tiles of
-//                      16 columns (column c of l is m = c - l; padding is 0).
-//  multipole_coeff     coefficients C_c[q][lm] = sum_i A_c,i(q) j_l(q r_i) Y_lm,i
-//                      as an FP32 MFMA contraction (v_mfma_f32_16x16x4_f32):
-//                      rows = 16 q values, columns = 16 m of one l, k = 4
-//                      points.  A workgroup owns a chunk of points, a block of
-//                      16 q and a group of column tiles; per 16-point slab its
-//                      256 threads first generate j_l(q r_i) for (16 points x
-//                      16 q) into LDS -- FP32 sin/cos, FP64 upward recurrence
-//                      for l <= q r, FP64 downward ratio recurrence (Miller)
-//                      above, valid for every x -- and the amplitudes; then
-//                      each wave runs the MFMAs of its tiles.  Partial
+//                      spherical.rs:54-93, stored FP32 as MFMA B operands.
+//                      Each l owns ceil((2l+1)/16) column tiles of 16 (column
+//                      c of l is m = c - l; padding is 0).  A tile is stored
+//                      in slabs of 16 points, each slab 256 floats laid out so
+//                      that lane 16k + n of a wave reads its four k-step values
+//                      (points 4 ks + k, column n) with one 16-byte load.
+//  multipole_coeff     C_c[q][lm] = sum_i A_c,i(q) j_l(q r_i) Y_lm,i as FP32
+//                      MFMA (v_mfma_f32_16x16x4f32: 16 q rows x 16 columns,
+//                      k = 4 points).  A workgroup owns a chunk of points, a
+//                      block of 16 q and a group of column tiles (4 waves x
+//                      MAXT accumulators).  Per slab of 16 points, each thread
+//                      generates j_l(q r) for one (point, q) and every l of
+//                      the group into LDS (FP32 sin/cos; FP64 upward
+//                      recurrence for l <= q r; FP64 Miller downward
+//                      recurrence from l + 14 above it, normalised on the
+//                      upward value, two passes so no step divides), then the
+//                      waves run the MFMAs of their tiles.  Tiles above the
+//                      block's largest per-q order are skipped.  Partial
 //                      coefficients per chunk are written in FP32.
-//  multipole_reduce    FP64 sum of the chunk partials per (q, column) and the
-//                      quadratic forms sum_lm C_a C_b (1 or 6 per q).
+//  multipole_reduce    FP64 sum of the chunk partials per (q, column) over the
+//                      columns with l <= that q's order, and the quadratic
+//                      forms sum_lm C_a C_b (1 or 6 per q).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 
 #include "kernels.hpp"
@@ -30,10 +37,14 @@ namespace {
 
 typedef float v4f __attribute__((ext_vector_type(4)));
 
-constexpr int kSlab = 16;    // points per j_l generation step
-constexpr int kFlush = 4;    // slabs per MFMA accumulation run
+constexpr int kSlab = 16;    // points per slab (4 MFMA k-steps of 4)
 constexpr int kQB = 16;      // q values per MFMA row block
 constexpr int kThreads = 256;
+constexpr int kChunk = 512;  // points per workgroup
+
+// Offset of (point p of a slab, column n) inside a 256-float tile slab: lane
+// 16 k + n holds the values of points 4 ks + k, ks = 0..3, contiguously.
+__device__ inline int slab_offset(int p, int n) { return ((p & 3) * 16 + n) * 4 + (p >> 2); }
 
 // ------------------------------------------------------------ geometry --
 __global__ void multipole_geometry(const double* __restrict__ xyz, int npts,
@@ -56,9 +67,10 @@ __global__ void multipole_geometry(const double* __restrict__ xyz, int npts,
   double sm = 0.0, cm = 1.0, pmm = 1.0;
   const double sqrt2 = 1.4142135623730951;
   // Y(l, column) with column = l + m (cos part) or l - m (sin part).
+  const int nslab = npad / kSlab;
   auto store = [&](int l, int col, double v) {
     const int t = tile_first[l] + col / 16;
-    Y[(size_t(t) * npad + i) * 16 + (col & 15)] = float(v);
+    Y[(size_t(t) * nslab + i / kSlab) * 256 + slab_offset(i % kSlab, col & 15)] = float(v);
   };
   for (int m = 0; m <= l_max; ++m) {
     if (m > 0) {
@@ -91,184 +103,207 @@ __global__ void multipole_geometry(const double* __restrict__ xyz, int npts,
   }
 }
 
-// j_l(x) for l in [l_lo, l_hi] (zero above lmax) into w[l - l_lo].
-__device__ inline void spherical_bessel_range(float xf, int lmax, int l_lo, int l_hi, double* w,
-                                              int stride) {
-  for (int l = l_lo; l <= l_hi; ++l) w[(l - l_lo) * stride] = 0.0;
-  if (lmax < l_lo) return;
-  if (xf < 1e-8f) {
-    if (l_lo == 0) w[0] = 1.0;
-    return;
-  }
-  const double x = double(xf);
-  float sf, cf;
-  sincosf(xf, &sf, &cf);
-  const double s = double(sf), c = double(cf);
-  const double ix = 1.0 / x;
-  const double j0 = s * ix;
-  const double j1 = s * ix * ix - c * ix;
-  const int top = min(lmax, l_hi);
-  const int lu = min(lmax, int(floorf(xf)));
-  // Upward recurrence, stable for l <= x.
-  double jm = j0, jc = j1, jlu = j0;
-  if (l_lo == 0) w[0] = j0;
-  if (lu >= 1) {
-    if (l_lo <= 1 && 1 <= top) w[(1 - l_lo) * stride] = j1;
-    jlu = j1;
-    for (int l = 1; l < lu; ++l) {
-      const double jn = double(2 * l + 1) * ix * jc - jm;
-      jm = jc;
-      jc = jn;
-      if (l + 1 >= l_lo && l + 1 <= top) w[(l + 1 - l_lo) * stride] = jn;
+
+__global__ void fill_zero(float* __restrict__ p, size_t n) {
+  const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0.f;
+}
+
+// One (point, q) column of spherical Bessel values: X[(l - l_lo) * 256] =
+// j_l(x) for l in [l_lo, min(lq, l_hi)] and 0 for the rest of [l_lo, l_hi].
+// l <= floor(x): upward recurrence from j_0, j_1 (stable there).  Above:
+// Miller's downward recurrence f_{l-1} = (2l+1)/x f_l - f_{l+1} started at
+// top + 14 (f = 1e-250, 0), normalised by f at floor(x) against the upward
+// value; pass 1 finds the normalisation, pass 2 repeats the recurrence and
+// stores.  Values computed before an overflow rescale are below 1e-250 of
+// the normalising value and stored as 0.
+__device__ inline void bessel_column(float xf, int lq, int l_lo, int l_hi, float* X) {
+  const int top = min(lq, l_hi);
+  int l_zero = l_lo;  // first l to clear
+  if (top >= l_lo) {
+    if (!(xf >= 1e-8f)) {
+      if (l_lo == 0) X[0] = 1.f;
+      l_zero = max(l_lo, 1);
+    } else {
+      const double x = double(xf);
+      float sf, cf;
+      sincosf(xf, &sf, &cf);
+      const double ix = 1.0 / x;
+      const double j0 = double(sf) * ix;
+      const double j1 = (double(sf) * ix - double(cf)) * ix;
+      const int lu = min(top, int(floorf(xf)));
+      if (l_lo == 0) X[0] = float(j0);
+      double jlu = j0;
+      if (lu >= 1) {
+        if (l_lo <= 1) X[(1 - l_lo) * 256] = float(j1);
+        double jm = j0, jc = j1, c = 3.0 * ix;  // c = (2l + 1) / x at l = 1
+        for (int l = 1; l < lu; ++l) {
+          const double jn = c * jc - jm;
+          jm = jc;
+          jc = jn;
+          c += 2.0 * ix;
+          if (l + 1 >= l_lo) X[(l + 1 - l_lo) * 256] = float(jn);
+        }
+        jlu = jc;
+      }
+      if (top > lu) {
+        const int S = top + 14;
+        // pass 1: normalisation and the last rescale
+        double fp = 0.0, f = 1e-250, c = double(2 * S + 1) * ix;
+        int last = S + 1;
+        for (int l = S; l > lu; --l) {
+          const double fn = c * f - fp;
+          fp = f;
+          f = fn;
+          c -= 2.0 * ix;
+          if (fabs(f) > 1e250) {
+            f *= 1e-250;
+            fp *= 1e-250;
+            last = l - 1;
+          }
+        }
+        const double norm = jlu / f;
+        // pass 2: the same recurrence, storing l in [max(lu + 1, l_lo), top]
+        fp = 0.0;
+        f = 1e-250;
+        c = double(2 * S + 1) * ix;
+        const int lo = max(lu + 1, l_lo);
+        for (int l = S; l >= lo; --l) {
+          if (l <= top) X[(l - l_lo) * 256] = l <= last ? float(norm * f) : 0.f;
+          const double fn = c * f - fp;
+          fp = f;
+          f = fn;
+          c -= 2.0 * ix;
+          if (fabs(f) > 1e250) {
+            f *= 1e-250;
+            fp *= 1e-250;
+          }
+        }
+      }
+      l_zero = top + 1;
     }
-    jlu = jc;
   }
-  if (top <= lu) return;
-  // Downward ratio recurrence rho_l = j_l / j_{l-1} from lmax + 14, stable for
-  // l > x; the ratios of the stored range are parked in w, those below it
-  // are folded into one product.
-  const int start = max(l_lo, lu + 1);
-  double rho = x / double(2 * (lmax + 14) + 1);
-  double below = 1.0;
-  for (int l = lmax + 13; l > lu; --l) {
-    rho = x / (double(2 * l + 1) - x * rho);
-    if (l <= top && l >= start) w[(l - l_lo) * stride] = rho;
-    else if (l < start) below *= rho;
-  }
-  double v = jlu * below;  // j_{start-1}
-  for (int l = start; l <= top; ++l) {
-    v *= w[(l - l_lo) * stride];
-    w[(l - l_lo) * stride] = v;
-  }
+  for (int l = max(l_zero, l_lo); l <= l_hi; ++l) X[(l - l_lo) * 256] = 0.f;
 }
 
 // partial[((chunk * NCH + c) * nqp + q) * ncol + col]
-template <int NCH, int TPW>
-__global__ __launch_bounds__(kThreads) void multipole_coeff(
+// Two waves per SIMD: caps registers at 256 (the accumulators take 4 MAXT NCH),
+// which also stops the scheduler from hoisting every tile's operand loads.
+template <int NCH, int MAXT>
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) void multipole_coeff(
     const float* __restrict__ radius, const int* __restrict__ type, const float* __restrict__ occ,
     int npts, int npad, const float* __restrict__ Y, int ntiles, const int* __restrict__ tile_l,
     const float* __restrict__ amp /* [ntypes][nqp][NCH] */, const float* __restrict__ qv,
-    const int* __restrict__ qlmax, int nqp, int chunk, float* __restrict__ partial) {
-  extern __shared__ double smem[];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int c0 = blockIdx.x * chunk;
-  const int c1 = min(c0 + chunk, npad);
-  const int qb = blockIdx.y * kQB;
-  const int tpg = 4 * TPW;
-  const int g0 = blockIdx.z * tpg;
-  const int g1 = min(g0 + tpg, ntiles);
-  const int l_lo = tile_l[g0], l_hi = tile_l[g1 - 1];
+    const int* __restrict__ qlmax, int nqp, int nchunk, int ngroups,
+    float* __restrict__ partial) {
+  extern __shared__ float smem[];
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  // XCD-aware order: workgroups go to the 8 XCDs round robin, so give each
+  // XCD a contiguous run of logical ids; the q blocks of one point chunk
+  // (which read the same harmonic slabs) then share an L2.
+  const int nqb = nqp / kQB;
+  const int total = nqb * nchunk * ngroups;
+  const int b = blockIdx.x;
+  const int per = total / 8, extra = total % 8, xcd = b % 8;
+  const int logical = xcd * per + min(xcd, extra) + b / 8;
+  const int qb = logical % nqb;
+  const int chunk = (logical / nqb) % nchunk;
+  const int grp = logical / (nqb * nchunk);
+  const int q0 = qb * kQB;
+  int Lb = 0;
+  for (int k = 0; k < kQB; ++k) Lb = max(Lb, qlmax[q0 + k]);
+  const int g0 = grp * 4 * MAXT;
+  const int g1 = min(g0 + 4 * MAXT, ntiles);
+  const int l_lo = tile_l[g0];
+  if (l_lo > Lb) return;  // every tile of the group is above this block's orders
+  const int l_hi = min(tile_l[g1 - 1], Lb);
   const int lr = l_hi - l_lo + 1;
-  const int stride = (lr | 1);  // odd stride: conflict-free 8-byte LDS rows
-  double* W = smem;                                           // [16 pt][16 q][stride]
-  float* A = reinterpret_cast<float*>(smem + kSlab * kQB * stride);  // [16 pt][16 q][NCH]
+  float* X = smem;              // [lr][4 k][16 q][4 ks]
+  float* AMP = smem + lr * 256; // [16 p][16 q][NCH]
 
-  int my_tiles[TPW];
-  int my_l[TPW];
+  v4f acc[MAXT][NCH];
 #pragma unroll
-  for (int t = 0; t < TPW; ++t) {
-    const int tile = g0 + wave + 4 * t;
-    my_tiles[t] = tile < g1 ? tile : -1;
-    my_l[t] = tile < g1 ? tile_l[tile] : 0;
-  }
-  // Two-level accumulation: the MFMA accumulator covers kFlush slabs (64
-  // points) and is then added into an FP32 running sum, so the rounding of a
-  // long same-sign accumulation (forward scattering) does not build up.
-  v4f acc[TPW][NCH], tot[TPW][NCH];
+  for (int t = 0; t < MAXT; ++t)
 #pragma unroll
-  for (int t = 0; t < TPW; ++t)
-#pragma unroll
-    for (int c = 0; c < NCH; ++c) {
-      acc[t][c] = v4f{0.f, 0.f, 0.f, 0.f};
-      tot[t][c] = v4f{0.f, 0.f, 0.f, 0.f};
-    }
-  int slabs = 0;
+    for (int c = 0; c < NCH; ++c) acc[t][c] = v4f{0.f, 0.f, 0.f, 0.f};
 
-  // phase-1 roles: one (point, q) pair per thread
-  const int gq = threadIdx.x % kQB, gp = threadIdx.x / kQB;
-  const float q_gen = qv[qb + gq];
-  const int lq_gen = qlmax[qb + gq];
-  // phase-2 roles (MFMA lane layout): row = q, k = point
-  const int mrow = lane % 16, mk = lane / 16;
+  // generation role: point 4 ks + wave of the slab, q = lane / 4, ks = lane % 4
+  const int gq = lane >> 2, gp = 4 * (lane & 3) + wave;
+  const float qg = qv[q0 + gq];
+  const int lg = qlmax[q0 + gq];
+  // MFMA role: lane = 16 k + n
+  const int mk = lane >> 4, mn = lane & 15;
+  const int nslab = npad / kSlab;
+  const int s0 = chunk * (kChunk / kSlab), s1 = min(s0 + kChunk / kSlab, nslab);
 
-  for (int s0 = c0; s0 < c1; s0 += kSlab) {
+  for (int s = s0; s < s1; ++s) {
+    __syncthreads();  // the previous slab's operands have been read
+    const int i = s * kSlab + gp;
+    const bool vi = i < npts;
+    bessel_column(vi ? qg * radius[i] : 0.f, vi ? lg : -1, l_lo, l_hi, X + wave * 64 + lane);
     {
-      const int i = s0 + gp;
-      double* w = W + (gp * kQB + gq) * stride;
-      if (i < npts) {
-        spherical_bessel_range(q_gen * radius[i], lq_gen, l_lo, l_hi, w, 1);
-        const float o = occ[i];
-        const int ty = type[i];
+      const float o = vi ? occ[i] : 0.f;
+      const int ty = vi ? type[i] : 0;
 #pragma unroll
-        for (int c = 0; c < NCH; ++c)
-          A[(gp * kQB + gq) * NCH + c] = o * amp[(size_t(ty) * nqp + qb + gq) * NCH + c];
-      } else {
-        for (int l = 0; l < lr; ++l) w[l] = 0.0;
-#pragma unroll
-        for (int c = 0; c < NCH; ++c) A[(gp * kQB + gq) * NCH + c] = 0.f;
-      }
+      for (int c = 0; c < NCH; ++c)
+        AMP[(gp * kQB + gq) * NCH + c] = o * amp[(size_t(ty) * nqp + q0 + gq) * NCH + c];
     }
     __syncthreads();
+    float am[NCH][4];
 #pragma unroll
-    for (int ks = 0; ks < kSlab / 4; ++ks) {
-      const int p = 4 * ks + mk;
-      float am[NCH];
+    for (int ks = 0; ks < 4; ++ks)
 #pragma unroll
-      for (int c = 0; c < NCH; ++c) am[c] = A[(p * kQB + mrow) * NCH + c];
-      const double* wrow = W + (p * kQB + mrow) * stride - l_lo;
+      for (int c = 0; c < NCH; ++c) am[c][ks] = AMP[((4 * ks + mk) * kQB + mn) * NCH + c];
 #pragma unroll
-      for (int t = 0; t < TPW; ++t) {
-        if (my_tiles[t] < 0) continue;
-        const float jv = float(wrow[my_l[t]]);
-        const float b = Y[(size_t(my_tiles[t]) * npad + s0 + p) * 16 + mrow];
+    for (int t = 0; t < MAXT; ++t) {
+      const int tile = g0 + wave + 4 * t;
+      if (tile >= g1) continue;
+      const int l = tile_l[tile];
+      if (l > l_hi) continue;
+      const v4f a = *reinterpret_cast<const v4f*>(X + ((l - l_lo) * 4 + mk) * 64 + mn * 4);
+      const v4f y = *reinterpret_cast<const v4f*>(Y + (size_t(tile) * nslab + s) * 256 + lane * 4);
 #pragma unroll
-        for (int c = 0; c < NCH; ++c)
-          acc[t][c] = __builtin_amdgcn_mfma_f32_16x16x4f32(jv * am[c], b, acc[t][c], 0, 0, 0);
-      }
-    }
-    __syncthreads();
-    if (++slabs == kFlush) {
-      slabs = 0;
+      for (int c = 0; c < NCH; ++c)
 #pragma unroll
-      for (int t = 0; t < TPW; ++t)
-#pragma unroll
-        for (int c = 0; c < NCH; ++c) {
-          tot[t][c] += acc[t][c];
-          acc[t][c] = v4f{0.f, 0.f, 0.f, 0.f};
-        }
+        for (int ks = 0; ks < 4; ++ks)
+          acc[t][c] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[ks] * am[c][ks], y[ks], acc[t][c],
+                                                           0, 0, 0);
     }
   }
-#pragma unroll
-  for (int t = 0; t < TPW; ++t)
-#pragma unroll
-    for (int c = 0; c < NCH; ++c) tot[t][c] += acc[t][c];
   const int ncol = ntiles * 16;
 #pragma unroll
-  for (int t = 0; t < TPW; ++t) {
-    if (my_tiles[t] < 0) continue;
+  for (int t = 0; t < MAXT; ++t) {
+    const int tile = g0 + wave + 4 * t;
+    if (tile >= g1 || tile_l[tile] > l_hi) continue;
 #pragma unroll
     for (int c = 0; c < NCH; ++c)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int q = qb + 4 * mk + r;
-        partial[((size_t(blockIdx.x) * NCH + c) * nqp + q) * ncol + my_tiles[t] * 16 + mrow] =
-            tot[t][c][r];
+        const int q = q0 + 4 * mk + r;
+        partial[((size_t(chunk) * NCH + c) * nqp + q) * ncol + tile * 16 + mn] = acc[t][c][r];
       }
   }
 }
 
 // out[q * NT + k]: NCH = 1 -> sum C^2; NCH = 3 -> vv, ee, hh, ve, vh, eh.
+// Only the columns with l <= qlmax[q] are read (the others were not written
+// for this q's block, or hold zero coefficients).
 template <int NCH>
 __global__ __launch_bounds__(256) void multipole_reduce(const float* __restrict__ partial,
+                                                        const int* __restrict__ tile_l,
+                                                        const int* __restrict__ qlmax,
                                                         int nchunk, int nqp, int ncol,
                                                         double* __restrict__ out) {
   constexpr int NT = NCH == 1 ? 1 : 6;
   __shared__ double scratch[4 * NT];
   const int q = blockIdx.x;
+  const int lq = qlmax[q];
   double s[NT];
 #pragma unroll
   for (int k = 0; k < NT; ++k) s[k] = 0.0;
   for (int col = threadIdx.x; col < ncol; col += blockDim.x) {
+    if (tile_l[col / 16] > lq) continue;
     double cc[NCH];
 #pragma unroll
     for (int c = 0; c < NCH; ++c) {
@@ -307,21 +342,28 @@ __global__ __launch_bounds__(256) void multipole_reduce(const float* __restrict_
           ((scratch[k] + scratch[NT + k]) + scratch[2 * NT + k]) + scratch[3 * NT + k];
 }
 
-constexpr int kTPW = 8;
+// Accumulator tiles per wave: 4 x MAXT tiles per workgroup group.
+constexpr int kMaxT1 = 32;  // one channel: 128 accumulator registers
+constexpr int kMaxT3 = 12;  // three channels: 144
 
 }  // namespace
 
-int multipole_tiles_per_group() { return 4 * kTPW; }
+int multipole_chunk() { return kChunk; }
+
+int multipole_tiles_per_group(int nch) { return 4 * (nch == 1 ? kMaxT1 : kMaxT3); }
 
 size_t multipole_coeff_lds_bytes(int l_range, int nch) {
-  const int stride = l_range | 1;
-  return size_t(kSlab) * kQB * stride * sizeof(double) + size_t(kSlab) * kQB * nch * sizeof(float);
+  return size_t(l_range) * 256 * sizeof(float) + size_t(kSlab) * kQB * nch * sizeof(float);
 }
 
 hipError_t launch_multipole_geometry(const double* xyz, int npts, const double* centre,
                                      int l_max, const double* norms, const int* tile_first,
-                                     int npad, float* radius, float* Y, hipStream_t stream) {
+                                     int ntiles, int npad, float* radius, float* Y,
+                                     hipStream_t stream) {
   if (npts <= 0) return hipSuccess;
+  if (npad % kSlab != 0) return hipErrorInvalidValue;
+  const size_t ny = size_t(ntiles) * npad * 16;
+  fill_zero<<<(ny + 255) / 256, 256, 0, stream>>>(Y, ny);
   multipole_geometry<<<(npts + 127) / 128, 128, 0, stream>>>(xyz, npts, centre, l_max, norms,
                                                              tile_first, npad, radius, Y);
   return hipGetLastError();
@@ -330,30 +372,33 @@ hipError_t launch_multipole_geometry(const double* xyz, int npts, const double* 
 hipError_t launch_multipole_coeff(const float* radius, const int* type, const float* occ,
                                   int npts, int npad, const float* Y, int ntiles,
                                   const int* tile_l, int max_l_range, const float* amp,
-                                  const float* qv, const int* qlmax, int nqp, int nch, int chunk,
+                                  const float* qv, const int* qlmax, int nqp, int nch,
                                   float* partial, double* out, hipStream_t stream,
                                   KernelTimer* timer) {
-  if (nqp % kQB != 0 || chunk % kSlab != 0 || npad % kSlab != 0) return hipErrorInvalidValue;
-  const int nchunk = (npad + chunk - 1) / chunk;
-  const int ngroups = (ntiles + 4 * kTPW - 1) / (4 * kTPW);
-  dim3 grid(nchunk, nqp / kQB, ngroups);
+  if (nqp % kQB != 0 || npad % kSlab != 0 || ntiles <= 0 || (nch != 1 && nch != 3))
+    return hipErrorInvalidValue;
+  const int nchunk = (npad + kChunk - 1) / kChunk;
+  const int tpg = multipole_tiles_per_group(nch);
+  const int ngroups = (ntiles + tpg - 1) / tpg;
+  const long long total = (long long)(nqp / kQB) * nchunk * ngroups;
+  if (total > INT32_MAX) return hipErrorInvalidValue;
   const size_t lds = multipole_coeff_lds_bytes(max_l_range, nch);
   {
     TimerScope scope(timer, "multipole_coeff", stream);
     if (nch == 1)
-      multipole_coeff<1, kTPW><<<grid, kThreads, lds, stream>>>(
-          radius, type, occ, npts, npad, Y, ntiles, tile_l, amp, qv, qlmax, nqp, chunk, partial);
-    else if (nch == 3)
-      multipole_coeff<3, kTPW><<<grid, kThreads, lds, stream>>>(
-          radius, type, occ, npts, npad, Y, ntiles, tile_l, amp, qv, qlmax, nqp, chunk, partial);
+      multipole_coeff<1, kMaxT1><<<unsigned(total), kThreads, lds, stream>>>(
+          radius, type, occ, npts, npad, Y, ntiles, tile_l, amp, qv, qlmax, nqp, nchunk, ngroups,
+          partial);
     else
-      return hipErrorInvalidValue;
+      multipole_coeff<3, kMaxT3><<<unsigned(total), kThreads, lds, stream>>>(
+          radius, type, occ, npts, npad, Y, ntiles, tile_l, amp, qv, qlmax, nqp, nchunk, ngroups,
+          partial);
   }
   const int ncol = ntiles * 16;
   if (nch == 1)
-    multipole_reduce<1><<<nqp, 256, 0, stream>>>(partial, nchunk, nqp, ncol, out);
+    multipole_reduce<1><<<nqp, 256, 0, stream>>>(partial, tile_l, qlmax, nchunk, nqp, ncol, out);
   else
-    multipole_reduce<3><<<nqp, 256, 0, stream>>>(partial, nchunk, nqp, ncol, out);
+    multipole_reduce<3><<<nqp, 256, 0, stream>>>(partial, tile_l, qlmax, nchunk, nqp, ncol, out);
   return hipGetLastError();
 }
 
