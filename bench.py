@@ -211,6 +211,7 @@ def multipole_leg(csx, ctx, log):
     F64 = csx.Precision.FP64_PLAIN
     s, q, nsites = workloads.prolate_100k()
     n = len(s)
+    s_rmax = float(np.max(np.linalg.norm(s.xyz - s.xyz.mean(axis=0), axis=1)))
     pairq = n * (n - 1) / 2 * q.size
     s.handle(ctx)
     ref, dt64, ms64, _ = timed(ctx, "debye_plain_f64",
@@ -230,11 +231,14 @@ def multipole_leg(csx, ctx, log):
         got, dt, kms, _ = timed(ctx, "multipole_coeff",
                                 lambda: csx.multipole_intensity(s, q, mp, ctx=ctx), 3)
         e_max, e_mean = rel_err(got, ref)
-        flops = 2.0 * n * q.size * (L + 1) ** 2  # one real-harmonic channel
+        # 2 N sum_q (L_q + 1)^2 with the per-q order clamp(ceil(q R) + 8, min(8, L), L)
+        lq = np.clip(np.ceil(q * s_rmax) + 8, min(8, L), L)
+        flops = 2.0 * n * float(np.sum((lq + 1) ** 2))
         out["l_max"][str(L)] = {"wall_ms": dt * 1e3, "kernel_ms": kms,
                                 "equiv_pair_q_per_s": pairq / dt,
                                 "speedup_vs_fp64_debye": dt64 / dt,
                                 "mfma_tflops_algorithmic": flops / (kms / 1e3) / 1e12,
+                                "mfma_frac": flops / (kms / 1e3) / PEAK_FP32_MFMA,
                                 "max_rel_err": e_max, "mean_rel_err": e_mean}
         log(f"multipole: L={L} {dt * 1e3:.1f} ms, max rel err {e_max:.2e}")
     return out

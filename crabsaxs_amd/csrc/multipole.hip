@@ -187,8 +187,8 @@ __device__ inline void bessel_column(float xf, int lq, int l_lo, int l_hi, float
 }
 
 // partial[((chunk * NCH + c) * nqp + q) * ncol + col]
-// Two waves per SIMD: caps registers at 256 (the accumulators take 4 MAXT NCH),
-// which also stops the scheduler from hoisting every tile's operand loads.
+// At least two waves per SIMD: caps registers at 256, which also stops the
+// scheduler from hoisting every tile's operand loads.
 template <int NCH, int MAXT>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) void multipole_coeff(
     const float* __restrict__ radius, const int* __restrict__ type, const float* __restrict__ occ,
@@ -342,9 +342,13 @@ __global__ __launch_bounds__(256) void multipole_reduce(const float* __restrict_
           ((scratch[k] + scratch[NT + k]) + scratch[2 * NT + k]) + scratch[3 * NT + k];
 }
 
-// Accumulator tiles per wave: 4 x MAXT tiles per workgroup group.
-constexpr int kMaxT1 = 32;  // one channel: 128 accumulator registers
-constexpr int kMaxT3 = 12;  // three channels: 144
+// Accumulator tiles per wave (4 x MAXT tiles per workgroup group), from a
+// sweep at 108k points x 200 q: one channel 16 (L = 15/25/40: 0.78/1.27/2.46
+// ms against 1.16/1.88/3.10 at 32), three channels 8 (L = 25: 2.59 ms
+// against 4.02 at 12).  Fewer accumulators raise occupancy, which hides the
+// harmonic-tile loads, at the cost of generating j_l once per tile group.
+constexpr int kMaxT1 = 16;
+constexpr int kMaxT3 = 8;
 
 }  // namespace
 
