@@ -149,6 +149,12 @@ def ensemble_leg(csx, ctx, m, log):
     sizes = np.array([len(c) for c in conformers])
     pairq = float(np.sum(sizes * (sizes - 1) / 2)) * q.size
     p = csx.SolventParams()
+    from crabsaxs_amd import autotune
+    t0 = time.perf_counter()
+    tuned = autotune.autotune_batch(conformers, q, ctx, sample=48, params=p,
+                                    candidates={"q_tile": (0, 32), "j_chunk": (256, 512, 1024),
+                                                "persist": (0, 2, 4)})
+    log(f"ensemble: autotuned {tuned} in {time.perf_counter() - t0:.1f} s")
     csx.batch_debye_solvent_basis(conformers, q, p, ctx=ctx)  # point sets, warm-up
     launch = ctx.last_launch()
     reps = 3
@@ -177,7 +183,10 @@ def ensemble_leg(csx, ctx, m, log):
     sigma = 0.01 * mix
     y = mix + sigma * rng.normal(size=q.size)
     exp = csx.ExperimentalCurve(q, y, sigma)
-    weights = {"conformers": pick.tolist(), "true": w_true.round(4).tolist()}
+    # the noise floor: chi2 of the true mixture with its scale and background fitted
+    truth_chi2 = csx.solve_scale_background(mix, exp)[2]
+    weights = {"conformers": pick.tolist(), "true": w_true.round(4).tolist(),
+               "true_weights_reduced_chi2": truth_chi2}
     for solver in (csx.WeightSolver.ProjectedGradient, csx.WeightSolver.NNLS):
         opts = csx.EnsembleOptions(fit=csx.FitOptions(method=csx.ScatteringMethod.Debye),
                                    weight_solver=solver)
@@ -200,7 +209,7 @@ def ensemble_leg(csx, ctx, m, log):
         "batch_s": dt, "kernel_ms": kms, "launches_per_batch": launches,
         "kernel_pair_q_per_s": pairq / (kms / 1e3),
         "valu_frac": pairq / (kms / 1e3) * VALU_OPS_PER_PAIRQ / PEAK_VALU_OPS,
-        "launch": launch,
+        "launch": launch, "autotuned": tuned,
         "max_rel_err_vs_fp64_gpu": worst, "checked_atoms": checked,
         "weights": weights,
     }
@@ -375,6 +384,8 @@ def main():
         log("cpu baseline done")
     if "ensemble" in args.legs:
         result["ensemble"] = ensemble_leg(csx, ctx, args.conformers, log)
+        ctx.set_tuning(**tune)  # the other legs run with the headline shape
+        ctx.set_persistence(0)
     if "multipole" in args.legs:
         result["multipole"] = multipole_leg(csx, ctx, log)
     if "adjoint" in args.legs:

@@ -61,6 +61,45 @@ def apply(ctx: Context, shape: dict) -> None:
     ctx.set_persistence(shape["persist"])
 
 
+def _time_shapes(run, ctx: Context, cand: dict, repeats: int):
+    results = []
+    for qt, sm, jc, pe in itertools.product(cand["q_tile"], cand["sin_mode"], cand["j_chunk"],
+                                            cand["persist"]):
+        shape = {"q_tile": qt, "sin_mode": sm, "j_chunk": jc, "persist": pe}
+        apply(ctx, shape)
+        run()  # point layouts for this j_chunk, warm-up
+        ctx.reset_timing()
+        ctx.set_timing(True)
+        for _ in range(repeats):
+            run()
+        ctx.set_timing(False)
+        n, ms = ctx.kernel_time("debye_pairs_f32")
+        results.append((ms / max(n, 1), shape))
+    return results
+
+
+def autotune_batch(structures: Sequence[Structure], q, ctx: Context = None,
+                   candidates: Optional[dict] = None, sample: int = 64, repeats: int = 2,
+                   params: SolventParams = None) -> dict:
+    """Pick and apply the fastest pair-kernel shape for a conformer batch.
+
+    Times csx_batch_debye_solvent_basis on an evenly spaced sample of the
+    batch (sizes spread like the whole set); the chosen shape is applied to
+    the context and returned with its kernel time on the sample."""
+    ctx = ctx or default_context()
+    q = np.asarray(q, dtype=np.float64)
+    params = params or SolventParams()
+    order = np.argsort([len(s) for s in structures])
+    pick = [structures[int(i)] for i in order[np.linspace(0, len(order) - 1,
+                                                          min(sample, len(order))).astype(int)]]
+    cand = dict(DEFAULT_CANDIDATES, **(candidates or {}))
+    results = _time_shapes(lambda: _api.batch_debye_solvent_basis(pick, q, params, ctx=ctx), ctx,
+                           cand, repeats)
+    best_ms, best = min(results, key=lambda t: t[0])
+    apply(ctx, best)
+    return dict(best, kernel_ms=best_ms, sample=len(pick), tried=len(results))
+
+
 def autotune(structure: Structure, q, ctx: Context = None,
              candidates: Optional[dict] = None, repeats: int = 3,
              use_cache: bool = True, params: SolventParams = None) -> dict:
@@ -74,19 +113,8 @@ def autotune(structure: Structure, q, ctx: Context = None,
         return dict(table[key], cached=True)
     cand = dict(DEFAULT_CANDIDATES, **(candidates or {}))
     params = params or SolventParams()
-    results = []
-    for qt, sm, jc, pe in itertools.product(cand["q_tile"], cand["sin_mode"], cand["j_chunk"],
-                                            cand["persist"]):
-        shape = {"q_tile": qt, "sin_mode": sm, "j_chunk": jc, "persist": pe}
-        apply(ctx, shape)
-        solvent_corrected_intensity(structure, q, params, ctx=ctx)  # build layout, warm up
-        ctx.reset_timing()
-        ctx.set_timing(True)
-        for _ in range(repeats):
-            solvent_corrected_intensity(structure, q, params, ctx=ctx)
-        ctx.set_timing(False)
-        n, ms = ctx.kernel_time("debye_pairs_f32")
-        results.append((ms / max(n, 1), shape))
+    results = _time_shapes(lambda: solvent_corrected_intensity(structure, q, params, ctx=ctx),
+                           ctx, cand, repeats)
     best_ms, best = min(results, key=lambda t: t[0])
     best = dict(best, kernel_ms=best_ms)
     apply(ctx, best)

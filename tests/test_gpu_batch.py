@@ -111,3 +111,21 @@ def test_ensemble_nnls_recovers_weights(gpu_ctx):
     w = dict(zip(r.conformers, r.weights))
     assert abs(w.get("b", 0) - 0.3) < 1e-6 and abs(w.get("e", 0) - 0.7) < 1e-6, w
     assert abs(r.params.scale - 2.0) < 1e-6 and r.reduced_chi2 < 1e-10
+
+
+def test_batch_autotuner_applies_a_measured_shape(gpu_ctx):
+    cs = conformers()
+    q = np.linspace(0.005, 0.5, 60)
+    try:
+        best = autotune.autotune_batch(cs, q, gpu_ctx, sample=3,
+                                       candidates={"q_tile": (0, 16), "j_chunk": (256,),
+                                                   "persist": (0, 2)})
+        assert best["tried"] == 4 and best["sample"] == 3 and best["kernel_ms"] > 0
+        tuned = csx.batch_debye_intensity(cs, q)
+        gpu_ctx.set_tuning(0, 2, 256)
+        gpu_ctx.set_persistence(0)
+        # with the same j chunk every shape sums the same pairs in the same order
+        assert np.array_equal(tuned, csx.batch_debye_intensity(cs, q))
+    finally:
+        gpu_ctx.set_tuning()
+        gpu_ctx.set_persistence(0)
