@@ -178,6 +178,34 @@ __device__ inline float sin2pi(float u) {
   return sin2pi_hw_direct(u);
 }
 
+// The pairs of lane atom pi with tile[0 .. jn): acc[k] += occ_j / r *
+// sin(2 pi qk[k] r).  r = d2 * rsq(d2) refined by one Newton step (the raw
+// square-root approximations are biased, which shifts every phase q r the
+// same way and measurably moves I(q) where it is steep; the step removes the
+// bias at 4 VALU cycles instead of the ~16 of the correctly rounded square
+// root); 1/r is the rsq itself.  MASK keeps only j > i (the tile overlapping
+// the lane's own block in a diagonal work item).
+template <int QT, int SINMODE, bool MASK>
+__device__ __attribute__((always_inline)) inline void tile_sweep(const float4* tile, int jn,
+                                                                 const float4 pi, int jt_minus_i,
+                                                                 const float* qk, float* acc) {
+  for (int t = 0; t < jn; ++t) {
+    const float4 pj = tile[t];
+    const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+    const float d2 = fmaxf(__builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz)), 1e-30f);
+    const float y = __builtin_amdgcn_rsqf(d2);
+    const float r0 = d2 * y;
+    const float r = __builtin_fmaf(__builtin_fmaf(-r0, r0, d2), 0.5f * y, r0);
+    const float w = (!MASK || t + jt_minus_i > 0) ? pj.w * y : 0.f;
+#pragma unroll
+    for (int k = 0; k < QT; ++k) {
+      const float u = qk[k] * r;
+      const float s = sin2pi<SINMODE>(u);
+      acc[k] = __builtin_fmaf(w, s, acc[k]);
+    }
+  }
+}
+
 // One work item (type-pure block of 256 i atoms x chunk of j atoms) for the
 // QT q values starting at qc; writes partial_row[qc .. qc + QT).
 template <int QT, int SINMODE>
@@ -206,22 +234,11 @@ __device__ __attribute__((always_inline)) inline void pair_item(const float4* __
     tile[threadIdx.x] = j < jend ? pts[j] : make_float4(0.f, 0.f, 0.f, 0.f);
     __syncthreads();
     const int jn = min(kBlock, jend - t0);
-    for (int t = 0; t < jn; ++t) {
-      const float4 pj = tile[t];
-      const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-      const float d2 = fmaxf(__builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz)), 1e-30f);
-      // Correctly rounded sqrt: the raw V_SQRT_F32 approximation biases r,
-      // i.e. shifts every phase q r the same way, which measurably moves
-      // I(q) where it is steep (once per pair, so the cost is amortised).
-      const float r = __builtin_sqrtf(d2);
-      const float w = (t0 + t > i) ? pj.w * __builtin_amdgcn_rcpf(r) : 0.f;
-#pragma unroll
-      for (int k = 0; k < QT; ++k) {
-        const float u = qk[k] * r;
-        const float s = sin2pi<SINMODE>(u);
-        acc[k] = __builtin_fmaf(w, s, acc[k]);
-      }
-    }
+    // only a tile overlapping the i block needs the j > i mask
+    if (t0 < item.x + item.y && t0 + kBlock > item.x)
+      tile_sweep<QT, SINMODE, true>(tile, jn, pi, t0 - i, qk, acc);
+    else
+      tile_sweep<QT, SINMODE, false>(tile, jn, pi, t0 - i, qk, acc);
 #pragma unroll
     for (int k = 0; k < QT; ++k) {
       float v = acc[k] * pi.w;
