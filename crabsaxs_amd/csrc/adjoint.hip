@@ -95,8 +95,11 @@ __global__ __launch_bounds__(kAdjBlock) void debye_vjp_f32(
       const float d2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
       // coincident points have no gradient; a diagonal block counts j > i only
       const bool use = d2 > 1e-20f && (!diag || jj > tid);
-      const float r = __builtin_sqrtf(use ? d2 : 1.0f);
-      const float ir = __builtin_amdgcn_rcpf(r);
+      // r = d2 rsq(d2) with one Newton step (unbiased; see the pair kernel)
+      const float dd = use ? d2 : 1.0f;
+      const float ir = __builtin_amdgcn_rsqf(dd);
+      const float r0 = dd * ir;
+      const float r = __builtin_fmaf(__builtin_fmaf(-r0, r0, dd), 0.5f * ir, r0);
       const float wgt = use ? pi.w * pj.w * ir * ir * ir : 0.f;
       float C = 0.f, S = 0.f;
 #pragma unroll
