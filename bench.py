@@ -234,7 +234,7 @@ def multipole_leg(csx, ctx, log):
            "fp32_debye": {"pair_q_per_s": pairq / dt32, "kernel_ms": ms32,
                           "max_rel_err": rel_err(d32, ref)[0]},
            "l_max": {}}
-    for L in (15, 25, 40):
+    for L in (15, 25, 40, 64):  # the config's orders, then the largest supported
         mp = csx.MultipoleParams(l_max=L)
         csx.multipole_intensity(s, q, mp, ctx=ctx)
         got, dt, kms, _ = timed(ctx, "multipole_coeff",

@@ -129,3 +129,17 @@ def test_batch_autotuner_applies_a_measured_shape(gpu_ctx):
     finally:
         gpu_ctx.set_tuning()
         gpu_ctx.set_persistence(0)
+
+
+@pytest.mark.parametrize("model", [csx.HydrationModel.AdaptiveGrid, csx.HydrationModel.Grid])
+def test_batch_with_grid_hydration_matches_single_bitwise(gpu_ctx, model):
+    """Grid hydration sites form an extra point type per structure; the batch
+    concatenates them with the atoms and keeps every bin separate."""
+    cs = conformers()[:3]
+    q = np.linspace(0.0, 0.6, 50)
+    p = csx.SolventParams(c1=1.01, c2=0.7, hydration_model=model,
+                          hydration_grid_spacing_angstrom=4.0 if model == csx.HydrationModel.AdaptiveGrid
+                          else 2.0)
+    got = csx.batch_debye_solvent_basis(cs, q, p)
+    for k, c in enumerate(cs):
+        assert np.array_equal(got[k], csx.debye_solvent_basis(c, q, p))
