@@ -38,6 +38,7 @@ SOURCES = [
     ("debye.hip", "device"),
     ("multipole.hip", "device"),
     ("adjoint.hip", "device"),
+    ("ensemble.hip", "device-exact"),
 ]
 
 
@@ -57,8 +58,11 @@ def build(verbose: bool = False) -> Path:
         for name, kind in SOURCES:
             src = CSRC / name
             obj = Path(tmp) / (src.name + ".o")
-            if kind == "device":
-                cmd = [HIPCC, "-x", "hip", *COMMON, *DEVICE, "-c", str(src), "-o", str(obj)]
+            if kind.startswith("device"):
+                # device-exact: no FMA contraction, so FP64 results match the
+                # host restatement bit for bit
+                extra = ["-ffp-contract=off"] if kind == "device-exact" else []
+                cmd = [HIPCC, "-x", "hip", *COMMON, *DEVICE, *extra, "-c", str(src), "-o", str(obj)]
             else:
                 cmd = [HIPCC, "-x", "c++", *COMMON, *HOST_ONLY, "-c", str(src), "-o", str(obj)]
             if verbose:

@@ -911,4 +911,89 @@ csx_status csx_ensemble_fit_core(const double* bases, size_t m, const double* q,
   return CSX_OK;
 }
 
+csx_status csx_ensemble_fit_core_device(csx_ctx* ctx, const double* bases, size_t m,
+                                        const double* q, const double* intensity,
+                                        const double* sigma, size_t n, const csx_fit_options* o,
+                                        int uniform, double regularization, int weight_solver,
+                                        double* weights, csx_fit_params* params, double* chi2,
+                                        double* fitted) {
+  if (!ctx) return fail(CSX_INVALID_INPUT, "context is null");
+  if (weight_solver == CSX_WEIGHTS_NNLS || (weight_solver == CSX_WEIGHTS_PROJECTED_GRADIENT &&
+                                            (!bases || !q || !intensity || !sigma || !o ||
+                                             !weights || !params || !chi2 || m == 0 || n == 0)))
+    return csx_ensemble_fit_core(bases, m, q, intensity, sigma, n, o, uniform, regularization,
+                                 weight_solver, weights, params, chi2, fitted);
+  if (weight_solver != CSX_WEIGHTS_PROJECTED_GRADIENT)
+    return fail(CSX_INVALID_INPUT, "unknown weight solver");
+  if (m > (1u << 20) || n > (1u << 20) || m * n > (size_t(1) << 28))
+    return fail(CSX_INVALID_INPUT, "ensemble too large for the device fit");
+  constexpr int kCand = 25;
+  std::vector<double> g(kCand * n), c2v(kCand), c1v(kCand);
+  for (int i = 0; i < 5; ++i) {
+    const double c1 = o->c1_bounds[0] + (o->c1_bounds[1] - o->c1_bounds[0]) * i / 4.0;
+    for (int j = 0; j < 5; ++j) {
+      const double c2 = o->c2_bounds[0] + (o->c2_bounds[1] - o->c2_bounds[0]) * j / 4.0;
+      const int k = 5 * i + j;
+      c1v[k] = c1;
+      c2v[k] = c2;
+      for (size_t p = 0; p < n; ++p) g[k * n + p] = excluded_volume_scale(q[p], c1);
+    }
+  }
+  int mpad = 2;
+  while (static_cast<size_t>(mpad) < m) mpad <<= 1;
+  Call call(ctx);
+  CSX_TRY(call.begin());
+  const hipStream_t st = call.stream();
+  DevBuf d_b, d_g, d_c2, d_y, d_s, d_curves, d_ws, d_sort, d_w, d_lin, d_fit;
+  CSX_HIP(d_b.alloc(m * 6 * n * sizeof(double), st), "hipMallocAsync");
+  CSX_HIP(d_g.alloc(g.size() * sizeof(double), st), "hipMallocAsync");
+  CSX_HIP(d_c2.alloc(kCand * sizeof(double), st), "hipMallocAsync");
+  CSX_HIP(d_y.alloc(n * sizeof(double), st), "hipMallocAsync");
+  CSX_HIP(d_s.alloc(n * sizeof(double), st), "hipMallocAsync");
+  CSX_HIP(d_curves.alloc(kCand * m * n * sizeof(double), st), "hipMallocAsync");
+  CSX_HIP(d_ws.alloc(kCand * (3 * m + 2 * n) * sizeof(double), st), "hipMallocAsync");
+  CSX_HIP(d_sort.alloc(static_cast<size_t>(kCand) * mpad * sizeof(double), st), "hipMallocAsync");
+  CSX_HIP(d_w.alloc(kCand * m * sizeof(double), st), "hipMallocAsync");
+  CSX_HIP(d_lin.alloc(3 * kCand * sizeof(double), st), "hipMallocAsync");
+  CSX_HIP(d_fit.alloc(kCand * n * sizeof(double), st), "hipMallocAsync");
+  CSX_HIP(hipMemcpyAsync(d_b.as<void>(), bases, m * 6 * n * sizeof(double), hipMemcpyHostToDevice,
+                         st), "hipMemcpyAsync");
+  CSX_HIP(hipMemcpyAsync(d_g.as<void>(), g.data(), g.size() * sizeof(double),
+                         hipMemcpyHostToDevice, st), "hipMemcpyAsync");
+  CSX_HIP(hipMemcpyAsync(d_c2.as<void>(), c2v.data(), kCand * sizeof(double),
+                         hipMemcpyHostToDevice, st), "hipMemcpyAsync");
+  CSX_HIP(hipMemcpyAsync(d_y.as<void>(), intensity, n * sizeof(double), hipMemcpyHostToDevice, st),
+          "hipMemcpyAsync");
+  CSX_HIP(hipMemcpyAsync(d_s.as<void>(), sigma, n * sizeof(double), hipMemcpyHostToDevice, st),
+          "hipMemcpyAsync");
+  CSX_HIP(launch_ensemble_candidates(d_b.as<double>(), d_g.as<double>(), d_c2.as<double>(),
+                                     static_cast<int>(m), static_cast<int>(n), kCand,
+                                     d_y.as<double>(), d_s.as<double>(), uniform != 0,
+                                     regularization, o->scale_mode, o->scale_value,
+                                     o->background_mode, o->background_value,
+                                     d_curves.as<double>(), d_ws.as<double>(), d_sort.as<double>(),
+                                     mpad, d_w.as<double>(), d_lin.as<double>(),
+                                     d_fit.as<double>(), st),
+          "ensemble_fit_core");
+  std::vector<double> w(kCand * m), lin(3 * kCand), fit(kCand * n);
+  CSX_HIP(hipMemcpyAsync(w.data(), d_w.as<void>(), w.size() * sizeof(double),
+                         hipMemcpyDeviceToHost, st), "hipMemcpyAsync");
+  CSX_HIP(hipMemcpyAsync(lin.data(), d_lin.as<void>(), lin.size() * sizeof(double),
+                         hipMemcpyDeviceToHost, st), "hipMemcpyAsync");
+  CSX_HIP(hipMemcpyAsync(fit.data(), d_fit.as<void>(), fit.size() * sizeof(double),
+                         hipMemcpyDeviceToHost, st), "hipMemcpyAsync");
+  for (DevBuf* b : {&d_b, &d_g, &d_c2, &d_y, &d_s, &d_curves, &d_ws, &d_sort, &d_w, &d_lin, &d_fit})
+    b->reset();
+  CSX_TRY(call.finish());
+  // the first candidate with the lowest chi^2, as the host loop picks it
+  int best = 0;
+  for (int k = 1; k < kCand; ++k)
+    if (lin[3 * k + 2] < lin[3 * best + 2]) best = k;
+  std::copy(w.begin() + best * m, w.begin() + (best + 1) * m, weights);
+  *params = {lin[3 * best], lin[3 * best + 1], c1v[best], c2v[best]};
+  *chi2 = lin[3 * best + 2];
+  if (fitted) std::copy(fit.begin() + best * n, fit.begin() + (best + 1) * n, fitted);
+  return CSX_OK;
+}
+
 }  // extern "C"

@@ -52,6 +52,15 @@ class TimerScope {
   bool stopped_ = false;
 };
 
+// EnsembleFitter::fit_core candidates (ensemble.hip): profiles of every
+// (c1, c2) candidate and their projected-gradient weight loops.
+hipError_t launch_ensemble_candidates(const double* bases, const double* g, const double* c2v,
+                                      int m, int n, int ncand, const double* y, const double* s,
+                                      int uniform, double reg, int scale_mode, double scale_value,
+                                      int bg_mode, double bg_value, double* curves, double* ws,
+                                      double* sortbuf, int mpad, double* w_out, double* lin_out,
+                                      double* fitted_out, hipStream_t stream);
+
 hipError_t launch_debye_plain(const double4* pts, const double* amp, const double* q, int npts,
                               int nq, int nch, int nterms, double* partial, double* out,
                               hipStream_t stream, KernelTimer* timer);

@@ -494,17 +494,20 @@ class EnsembleFitter:
         return self._bases
 
     def _fit_core(self, selected, data: ExperimentalCurve, ctx):
+        """fit_core on the GPU (csx_ensemble_fit_core_device; bitwise the
+        host restatement's result)."""
+        from .api import default_context
+        ctx = ctx or default_context()
         b = np.ascontiguousarray(self.bases(ctx)[selected])
         m, n = len(selected), len(data)
         w, chi, fitted = np.zeros(m), C.c_double(), np.zeros(n)
         p = FitParamsC()
         o = self.options.fit.to_c()
         uniform = self.options.uniform or not self.options.fit_weights
-        _check(_lib.load().csx_ensemble_fit_core(dptr(b), m, dptr(data.q), dptr(data.intensity),
-                                                 dptr(data.sigma), n, C.byref(o), int(uniform),
-                                                 self.options.regularization,
-                                                 int(self.options.weight_solver), dptr(w), C.byref(p),
-                                                 C.byref(chi), dptr(fitted)))
+        _check(_lib.load().csx_ensemble_fit_core_device(
+            ctx.ptr, dptr(b), m, dptr(data.q), dptr(data.intensity), dptr(data.sigma), n,
+            C.byref(o), int(uniform), self.options.regularization,
+            int(self.options.weight_solver), dptr(w), C.byref(p), C.byref(chi), dptr(fitted)))
         return w, FitParams(p.scale, p.background, p.c1, p.c2), float(chi.value), fitted
 
     def fit(self, ctx: Context = None) -> EnsembleResult:

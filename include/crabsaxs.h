@@ -465,6 +465,22 @@ csx_status csx_bootstrap_indices(uint64_t seed, size_t n, size_t replicates,
 csx_status csx_chacha_block(const uint32_t *key, uint64_t counter,
                             uint64_t stream, int rounds, uint32_t *out);
 
+/* csx_ensemble_fit_core on the GPU of `ctx`: the 25 candidate profile sets
+ * and their projected-gradient weight loops run on the device (one
+ * workgroup per candidate), with every order-dependent sum in the host
+ * order, so the outputs are bitwise those of csx_ensemble_fit_core.
+ * weight_solver CSX_WEIGHTS_NNLS runs on the host (it is the same call).
+ * No reference counterpart. */
+csx_status csx_ensemble_fit_core_device(csx_ctx *ctx, const double *bases,
+                                        size_t m, const double *q,
+                                        const double *intensity,
+                                        const double *sigma, size_t n,
+                                        const csx_fit_options *options,
+                                        int uniform, double regularization,
+                                        int weight_solver, double *weights,
+                                        csx_fit_params *params, double *chi2,
+                                        double *fitted);
+
 /* ---- synthetic workloads (no reference counterpart) -------------------- */
 
 /* Seeded synthetic globular protein: self-avoiding C-alpha random walk with

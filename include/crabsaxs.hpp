@@ -1073,8 +1073,8 @@ class EnsembleFitter {
     c.fitted.resize(n);
     const csx_fit_options o = options.fit.to_c();
     csx_fit_params p;
-    detail::check(csx_ensemble_fit_core(
-        b.data(), m, data.q.data(), data.intensity.data(), data.sigma.data(), n, &o,
+    detail::check(csx_ensemble_fit_core_device(
+        ctx.get(), b.data(), m, data.q.data(), data.intensity.data(), data.sigma.data(), n, &o,
         (options.uniform || !options.fit_weights) ? 1 : 0, options.regularization,
         static_cast<int>(options.weight_solver), c.weights.data(), &p, &c.chi2,
         c.fitted.data()));

@@ -121,3 +121,47 @@ def test_ensemble_recovers_mixture(gpu_ctx):
     direct = csx.solvent_corrected_intensity(conformers[2], q, csx.SolventParams(c1=0.975, c2=-0.5))
     recon = csx.combine_solvent_basis(b[2], q, 0.975, -0.5)
     assert np.max(np.abs(recon - direct) / np.abs(direct)) < 1e-12
+
+
+def _fit_core(b, e, opts, uniform, reg, solver, ctx=None):
+    import ctypes as C
+    from crabsaxs_amd import _lib
+    from crabsaxs_amd._lib import FitParamsC, dptr
+    m, n = b.shape[0], len(e)
+    w, chi, fitted, p = np.zeros(m), C.c_double(), np.zeros(n), FitParamsC()
+    o = opts.to_c()
+    args = (dptr(np.ascontiguousarray(b)), m, dptr(e.q), dptr(e.intensity), dptr(e.sigma), n,
+            C.byref(o), int(uniform), reg, int(solver), dptr(w), C.byref(p), C.byref(chi),
+            dptr(fitted))
+    L = _lib.load()
+    st = L.csx_ensemble_fit_core_device(ctx.ptr, *args) if ctx else L.csx_ensemble_fit_core(*args)
+    assert st == 0
+    return w, (p.scale, p.background, p.c1, p.c2), chi.value, fitted
+
+
+@pytest.mark.parametrize("m,n,uniform,reg,bg", [(37, 90, False, 0.0, "Fit"), (512, 200, False, 0.0, "Fit"),
+                                               (5, 64, False, 0.3, "FixedZero"), (8, 50, True, 0.0, "Fit"),
+                                               (300, 120, False, 0.0, "Fixed")])
+def test_device_fit_core_is_bitwise_the_host_fit_core(gpu_ctx, m, n, uniform, reg, bg):
+    """The 25-candidate weight fit on the GPU against the host restatement
+    (which is bitwise the oracle's optimize_weights): identical bits."""
+    rng = np.random.default_rng(m * 1000 + n)
+    q = np.linspace(0.01, 0.5, n)
+    base = np.exp(-np.outer(rng.uniform(5, 30, m), q) ** 2 / 3.0)
+    b = np.zeros((m, 6, n))
+    b[:, 0] = 1e4 * base
+    b[:, 1] = 2e3 * base * np.exp(-q ** 2)
+    b[:, 2] = 50 * base
+    b[:, 3] = 4e3 * base
+    b[:, 4] = 300 * base
+    b[:, 5] = 90 * base
+    y = 1.7 * (b[:m // 2 + 1, 0].mean(axis=0)) + 3.0
+    e = csx.ExperimentalCurve(q, y + 0.01 * y * rng.normal(size=n), 0.01 * y + 0.1)
+    mode = {"Fit": csx.BackgroundMode.Fit, "FixedZero": csx.BackgroundMode.FixedZero,
+            "Fixed": csx.BackgroundMode.Fixed(2.5)}[bg]
+    opts = csx.FitOptions(background_mode=mode)
+    host = _fit_core(b, e, opts, uniform, reg, csx.WeightSolver.ProjectedGradient)
+    dev = _fit_core(b, e, opts, uniform, reg, csx.WeightSolver.ProjectedGradient, gpu_ctx)
+    assert np.array_equal(host[0], dev[0])
+    assert host[1] == dev[1] and host[2] == dev[2]
+    assert np.array_equal(host[3], dev[3])
