@@ -3,7 +3,8 @@
 The public names mirror the reference crate's API (src/lib.rs:16-50).
 """
 from .api import (  # noqa: F401
-    Atom, ComputeOptions, Context, CurveMetadata, DeviceError, Element, FitDidNotConverge,
+    Atom, ComputeOptions, Context, CoordinateFeatures, CurveMetadata, DeviceError, Element,
+    FitDidNotConverge, PairDistribution, coordinate_features,
     batch_debye_intensity, batch_debye_solvent_basis,
     HydrationGrid, HydrationModel, HydrogenMode, InvalidInput, MultipoleOptions,
     MultipoleParams, NoDevice, multipole_intensity, multipole_solvent_basis,

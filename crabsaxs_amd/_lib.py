@@ -163,6 +163,9 @@ SIGNATURES = {
                                                c_double_p, c_double_p, C.c_size_t, _fo, C.c_int,
                                                C.c_double, C.c_int, c_double_p, _fp, c_double_p,
                                                c_double_p]),
+    "csx_coordinate_features": (C.c_int, [C.c_void_p, C.c_size_t, c_double_p, c_double_p,
+                                          c_double_p, c_double_p, C.POINTER(C.c_size_t),
+                                          c_double_p, c_double_p, C.POINTER(C.c_int)]),
     "csx_bootstrap_indices": (C.c_int, [C.c_uint64, C.c_size_t, C.c_size_t,
                                         C.POINTER(C.c_uint64)]),
     "csx_chacha_block": (C.c_int, [C.POINTER(C.c_uint32), C.c_uint64, C.c_uint64, C.c_int,

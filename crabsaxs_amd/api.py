@@ -590,6 +590,40 @@ def generate_shell_sites(structure: Structure, spacing: float, inner: float,
 
 # --------------------------------------------------------------- analysis --
 
+@dataclass
+class PairDistribution:
+    """`PairDistribution` (src/metrics.rs:32-37)."""
+    r: np.ndarray
+    p: np.ndarray
+    rg: Optional[float]
+    dmax: Optional[float]
+
+
+@dataclass
+class CoordinateFeatures:
+    """`CoordinateFeatures` (src/metrics.rs:61-65)."""
+    rg: float
+    dmax: float
+    p_r: PairDistribution
+
+
+def coordinate_features(structure: Structure, bins: int = 160,
+                        ctx: Context = None) -> CoordinateFeatures:
+    """`coordinate_features` (src/metrics.rs:236-270), PrOptions.bins = `bins`;
+    the pair histogram runs on the GPU."""
+    nb = max(int(bins), 16)
+    r, p = np.zeros(nb), np.zeros(nb)
+    rg, dmax, prg, pdmax = C.c_double(), C.c_double(), C.c_double(), C.c_double()
+    count, flags = C.c_size_t(), C.c_int()
+    _check(_lib.load().csx_coordinate_features(structure.handle(ctx), int(bins), C.byref(rg),
+                                               C.byref(dmax), dptr(r), dptr(p), C.byref(count),
+                                               C.byref(prg), C.byref(pdmax), C.byref(flags)))
+    k = count.value
+    pr = PairDistribution(r[:k].copy(), p[:k].copy(), prg.value if flags.value & 1 else None,
+                          pdmax.value if flags.value & 2 else None)
+    return CoordinateFeatures(rg.value, dmax.value, pr)
+
+
 class ProfileCalculator:
     """`ProfileCalculator` (src/analysis.rs:58-63, 90-105)."""
 

@@ -35,10 +35,12 @@ SOURCES = [
     ("fit.cpp", "host"),
     ("adjoint.cpp", "host"),
     ("rng.cpp", "host"),
+    ("pr.cpp", "host"),
     ("debye.hip", "device"),
     ("multipole.hip", "device"),
     ("adjoint.hip", "device"),
     ("ensemble.hip", "device-exact"),
+    ("pr.hip", "device-exact"),
 ]
 
 

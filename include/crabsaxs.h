@@ -308,6 +308,23 @@ csx_status csx_debye_jacobian(csx_structure *s, const double *q, size_t nq,
                               const csx_solvent_params *p,
                               int implicit_hydrogen, double *jac);
 
+/* ---- coordinate features (src/metrics.rs) ------------------------------- */
+
+/* coordinate_features (metrics.rs:236-270) with coordinate_pr
+ * (metrics.rs:475-548) and normalize_distribution (metrics.rs:550-584):
+ * weights |form_factor_for_atom(atom, implicit, 0)| floored at 1e-6;
+ * *rg, *dmax the weighted radius of gyration and maximum pair distance;
+ * the distribution on max(bins, 16) bins (at most 1536): *n_out points
+ * (1 in the degenerate cases of the reference) written to r and p (each of
+ * max(bins, 16) doubles), with its own rg / dmax in *pr_rg / *pr_dmax and
+ * *pr_flags bit 0 / bit 1 set when they are present (Some).  Dmax and every
+ * pair's bin are exactly the reference's (FP64 distances in its order); the
+ * histogram's FP64 sums run in a different order. */
+csx_status csx_coordinate_features(csx_structure *s, size_t bins, double *rg,
+                                   double *dmax, double *r, double *p,
+                                   size_t *n_out, double *pr_rg,
+                                   double *pr_dmax, int *pr_flags);
+
 /* ---- fitting (src/fit.rs) ---------------------------------------------- */
 
 /* FitParams, fit.rs:11-17. */

@@ -582,6 +582,45 @@ inline std::vector<double> debye_grad_x(const Structure& s, const std::vector<do
   return out;
 }
 
+// ----------------------------------------------------------------- metrics
+// PrOptions, PairDistribution, CoordinateFeatures (src/metrics.rs:15-37, 61-65).
+
+struct PrOptions {
+  size_t bins = 160;
+  std::optional<double> max_r;  // used by the experimental transform only
+};
+
+struct PairDistribution {
+  std::vector<double> r, p;
+  std::optional<double> rg, dmax;
+};
+
+struct CoordinateFeatures {
+  double rg = 0.0, dmax = 0.0;
+  PairDistribution p_r;
+};
+
+// coordinate_features (metrics.rs:236-270): weighted Rg and Dmax of the
+// coordinates and their pair-distance distribution (pair histogram on the GPU).
+inline CoordinateFeatures coordinate_features(const Structure& s, const PrOptions& o = {},
+                                              Context& ctx = Context::default_context()) {
+  const size_t nb = std::max<size_t>(o.bins, 16);
+  std::vector<double> r(nb), p(nb);
+  size_t n_out = 0;
+  double pr_rg = 0.0, pr_dmax = 0.0;
+  int flags = 0;
+  CoordinateFeatures f;
+  detail::check(csx_coordinate_features(s.handle(ctx), o.bins, &f.rg, &f.dmax, r.data(), p.data(),
+                                        &n_out, &pr_rg, &pr_dmax, &flags));
+  r.resize(n_out);
+  p.resize(n_out);
+  f.p_r.r = std::move(r);
+  f.p_r.p = std::move(p);
+  if (flags & 1) f.p_r.rg = pr_rg;
+  if (flags & 2) f.p_r.dmax = pr_dmax;
+  return f;
+}
+
 // --------------------------------------------------------------------- io
 // ExperimentalCurve (src/io.rs:8-16, 113-141).
 

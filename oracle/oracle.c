@@ -28,6 +28,10 @@
  *   or_cubic_hermite            reproduces cubics exactly              (pinned)
  *   or_optimize_weights         simplex projection brute force, weight
  *                               recovery on noiseless mixtures        (pinned)
+ *   or_debye_grad_x             central differences of or_debye_*     (pinned)
+ *   or_coordinate_features      numpy brute force (exact bins, sums to 1e-12),
+ *                               two-atom Dmax = 4 of metrics.rs:678-684,
+ *                               sum of the histogram = sum of w_i w_j  (pinned)
  */
 #include "oracle.h"
 
@@ -1130,4 +1134,66 @@ void or_optimize_weights(const double *curves, size_t m, const double *y, const 
     free(res);
     free(g);
     free(prop);
+}
+
+/* ------------------------------------------------- coordinate features -- */
+
+void or_coordinate_features(const double *xyz, const int *element, const int *group,
+                            const double *occ, size_t n, size_t bins, double *rg,
+                            double *dmax, double *hist) {
+    /* metrics.rs:245-266: weights, centre, Rg */
+    double *w = (double *)malloc(sizeof(double) * (n ? n : 1));
+    double total = 0.0;
+    for (size_t i = 0; i < n; ++i) {
+        double f = fabs(or_form_factor(element[i], group ? group[i] : -1, occ[i], 1, 0.0));
+        w[i] = f > 1.0e-6 ? f : 1.0e-6;
+        total += w[i];
+    }
+    double c[3] = {0.0, 0.0, 0.0};
+    for (size_t i = 0; i < n; ++i)
+        for (int a = 0; a < 3; ++a) c[a] = c[a] + xyz[3 * i + a] * w[i];
+    for (int a = 0; a < 3; ++a) c[a] = c[a] / total;
+    double acc = 0.0;
+    for (size_t i = 0; i < n; ++i) {
+        double dx = xyz[3 * i] - c[0], dy = xyz[3 * i + 1] - c[1], dz = xyz[3 * i + 2] - c[2];
+        acc += w[i] * (dx * dx + dy * dy + dz * dz);
+    }
+    *rg = sqrt(acc / total);
+    /* metrics.rs:481-491: Dmax from squared distances */
+    double dmax2 = 0.0;
+    for (size_t l = 0; l < n; ++l)
+        for (size_t r = l + 1; r < n; ++r) {
+            double dx = xyz[3 * l] - xyz[3 * r], dy = xyz[3 * l + 1] - xyz[3 * r + 1],
+                   dz = xyz[3 * l + 2] - xyz[3 * r + 2];
+            double d2 = dx * dx + dy * dy + dz * dz;
+            if (d2 > dmax2) dmax2 = d2;
+        }
+    if (!(dmax2 > 0.0)) {
+        *dmax = 0.0;
+        free(w);
+        return;
+    }
+    *dmax = sqrt(dmax2);
+    if (bins < 16) bins = 16;
+    double step = *dmax / (double)bins;
+    /* metrics.rs:508-531: 64-row chunks, each histogram in pair order, then
+     * the chunks summed in order */
+    double *part = (double *)calloc(bins, sizeof(double));
+    for (size_t b = 0; b < bins; ++b) hist[b] = 0.0;
+    for (size_t start = 0; start < n; start += 64) {
+        size_t end = start + 64 < n ? start + 64 : n;
+        for (size_t b = 0; b < bins; ++b) part[b] = 0.0;
+        for (size_t l = start; l < end; ++l)
+            for (size_t r = l + 1; r < n; ++r) {
+                double dx = xyz[3 * l] - xyz[3 * r], dy = xyz[3 * l + 1] - xyz[3 * r + 1],
+                       dz = xyz[3 * l + 2] - xyz[3 * r + 2];
+                double d = sqrt(dx * dx + dy * dy + dz * dz);
+                double f = floor(d / step);
+                size_t b = f < (double)(bins - 1) ? (size_t)f : bins - 1;
+                part[b] += w[l] * w[r];
+            }
+        for (size_t b = 0; b < bins; ++b) hist[b] += part[b];
+    }
+    free(part);
+    free(w);
 }

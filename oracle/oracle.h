@@ -55,6 +55,13 @@ void or_debye_grad_x(const double *xyz, const int *element, const int *group,
                      double thickness, double rho, int implicit_h, int threads,
                      double *grad, double *jac);
 
+/* coordinate_features (metrics.rs:236-270): weighted *rg and *dmax, and the
+ * unnormalised pair histogram of coordinate_pr (metrics.rs:475-548) in its
+ * chunked order into hist[max(bins,16)] (untouched when dmax is 0). */
+void or_coordinate_features(const double *xyz, const int *element, const int *group,
+                            const double *occ, size_t n, size_t bins, double *rg,
+                            double *dmax, double *hist);
+
 /* combine_solvent_basis, spherical.rs:567-584. */
 void or_combine_solvent_basis(const double *basis, const double *q, size_t nq,
                               double c1, double c2, double *out);

@@ -527,6 +527,22 @@ CSX_TEST(profile_calculator_scorer_and_fitter_drop_in, true) {
   CHECK(std::fabs(score.chi2 - score.reduced_chi2 * 119.0) <= 1e-9 * std::max(score.chi2, 1.0));
 }
 
+CSX_TEST(coordinate_features_report_pair_distance, true) {
+  // metrics.rs:678-684
+  const Structure s({atom(0, 0, 0, Element::C, "C1"), atom(4, 0, 0, Element::C, "C2")});
+  const CoordinateFeatures f = coordinate_features(s);
+  CHECK(std::fabs(f.dmax - 4.0) < 1e-12);
+  CHECK(f.rg > 1.0);
+  CHECK(!f.p_r.p.empty());
+  bool threw = false;
+  try {
+    coordinate_features(Structure(std::vector<Atom>{}));
+  } catch (const InvalidInput&) {
+    threw = true;
+  }
+  CHECK(threw);
+}
+
 CSX_TEST(ensemble_fitter_recovers_a_mixture, true) {
   std::vector<Structure> conformers;
   for (int k = 0; k < 4; ++k) {

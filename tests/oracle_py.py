@@ -77,6 +77,7 @@ def lib() -> C.CDLL:
         L.or_debye_grad_x.argtypes = [_d, _i, _i, _d, C.c_size_t, _d, C.c_size_t, _d, C.c_int,
                                       C.c_double, C.c_double, C.c_double, C.c_double, C.c_int,
                                       C.c_int, _d, _d]
+        L.or_coordinate_features.argtypes = [_d, _i, _i, _d, C.c_size_t, C.c_size_t, _d, _d, _d]
         _lib = L
     return _lib
 
@@ -148,6 +149,38 @@ def debye_grad_x(atoms: Atoms, q, weights=None, solvent=None, implicit_h=True, t
     if jacobian:
         return jac.reshape(q.size, n, 3)
     return grad.reshape(n, 3)
+
+
+def coordinate_features(atoms: Atoms, bins=160):
+    """(rg, dmax, unnormalised histogram or None) of coordinate_features /
+    coordinate_pr (metrics.rs:236-270, 475-548)."""
+    nb = max(int(bins), 16)
+    hist = np.zeros(nb)
+    rg, dmax = C.c_double(), C.c_double()
+    lib().or_coordinate_features(*atoms.args(), nb, C.byref(rg), C.byref(dmax), _p(hist))
+    return rg.value, dmax.value, (hist if dmax.value > 0 else None)
+
+
+def normalize_distribution(r, p):
+    """normalize_distribution (metrics.rs:550-584): (r, p, rg, dmax) or None."""
+    r, p = list(map(float, r)), list(map(float, p))
+    if len(r) != len(p) or not p:
+        return None
+    area = 0.0
+    for k in range(len(r) - 1):
+        area += 0.5 * (r[k + 1] - r[k]) * (p[k] + p[k + 1])
+    if not np.isfinite(area) or area <= 0.0:
+        return None
+    p = [v / area for v in p]
+    moment = 0.0
+    for k in range(len(r) - 1):
+        moment += 0.5 * (r[k + 1] - r[k]) * (r[k] * r[k] * p[k] + r[k + 1] * r[k + 1] * p[k + 1])
+    rg = float(np.sqrt(max(moment / 2.0, 0.0)))
+    maximum = 0.0
+    for v in p:
+        maximum = max(maximum, v)
+    dmax = next((r[k] for k in range(len(p) - 1, -1, -1) if p[k] > maximum * 0.01), None)
+    return np.array(r), np.array(p), (rg if np.isfinite(rg) else None), dmax
 
 
 def combine_solvent_basis(basis, q, c1, c2):

@@ -202,3 +202,34 @@ def test_precision_study_20k_against_fp64_gpu(gpu_ctx):
     err = rel(got, ref)
     print("20k solvent max", err.max(), "mean", err.mean(), flush=True)
     assert err.max() < 1e-5
+
+
+@pytest.mark.parametrize("n", [1200, 20000])
+def test_coordinate_features_match_oracle(gpu_ctx, n):
+    """P(r) pair histogram on the GPU against the oracle's restatement of
+    coordinate_pr: Dmax and Rg bitwise, the normalised distribution to 1e-12
+    (only the FP64 summation order differs)."""
+    s = csx.generate_globule(n, 35.0 * (n / 20000) ** (1 / 3), 1.0, seed=17)
+    f = csx.coordinate_features(s, 160)
+    rg, dmax, hist = O.coordinate_features(O.Atoms(s.xyz, s.element), 160)
+    assert f.dmax == dmax and f.rg == rg
+    r = (np.arange(160) + 0.5) * (dmax / 160)  # metrics.rs:529-531
+    r_o, p_o, rg_o, dmax_o = O.normalize_distribution(r, hist)
+    assert np.array_equal(f.p_r.r, r_o)
+    assert np.max(np.abs(f.p_r.p - p_o)) <= 1e-12 * np.max(p_o)
+    assert abs(f.p_r.rg - rg_o) <= 1e-12 * rg_o and f.p_r.dmax == dmax_o
+
+
+def test_coordinate_features_reference_fixture_and_edges(gpu_ctx):
+    # metrics.rs:651-684
+    two = csx.Structure.from_arrays([[0, 0, 0], [4, 0, 0]], [1, 1])
+    f = csx.coordinate_features(two)
+    assert abs(f.dmax - 4.0) < 1e-12 and f.rg > 1.0 and f.p_r.p.size > 0
+    one = csx.Structure.from_arrays([[1.0, 2.0, 3.0]], [1])
+    f1 = csx.coordinate_features(one)
+    assert f1.dmax == 0.0 and list(f1.p_r.r) == [0.0] and list(f1.p_r.p) == [1.0]
+    assert f1.p_r.rg == 0.0 and f1.p_r.dmax == 0.0
+    with pytest.raises(csx.InvalidInput):
+        csx.coordinate_features(csx.Structure.from_arrays(np.zeros((0, 3)), np.zeros(0, np.uint8)))
+    with pytest.raises(csx.InvalidInput):
+        csx.coordinate_features(two, bins=5000)

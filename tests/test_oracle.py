@@ -287,3 +287,53 @@ def test_oracle_gradient_matches_central_differences(solvent):
     assert np.allclose(g, np.einsum("k,kic->ic", w, jac), rtol=1e-12, atol=1e-12)
     # translation invariance: the gradients sum to zero
     assert np.abs(jac.sum(axis=1)).max() <= 1e-10 * np.abs(jac).max()
+
+
+def _pr_brute(xyz, w, bins):
+    """numpy brute force of coordinate_pr: the same FP64 operations per pair
+    (so the same bins), summed in a different order."""
+    n = len(xyz)
+    il, ir = np.triu_indices(n, 1)
+    dx = xyz[il, 0] - xyz[ir, 0]
+    dy = xyz[il, 1] - xyz[ir, 1]
+    dz = xyz[il, 2] - xyz[ir, 2]
+    d2 = dx * dx + dy * dy + dz * dz
+    dmax = np.sqrt(d2.max())
+    step = dmax / bins
+    b = np.minimum(np.floor(np.sqrt(d2) / step), bins - 1).astype(int)
+    return dmax, np.bincount(b, weights=w[il] * w[ir], minlength=bins), (il, ir, b)
+
+
+def test_coordinate_pr_reference_fixture():
+    # metrics.rs:651-684: two GLY carbons 4 A apart
+    atoms = O.Atoms([[0, 0, 0], [4, 0, 0]], [1, 1],
+                    group=[O.lib().or_united_atom_group(b"GLY", b"C1")] * 2)
+    rg, dmax, hist = O.coordinate_features(atoms, 160)
+    assert abs(dmax - 4.0) < 1e-12 and rg > 1.0
+    r = (np.arange(160) + 0.5) * dmax / 160
+    norm = O.normalize_distribution(r, hist)
+    assert norm is not None and len(norm[1]) > 0
+    # the single pair lands in the last bin (floor(4 / step) = 160 -> 159)
+    assert np.count_nonzero(hist) == 1 and hist[-1] > 0
+
+
+@pytest.mark.parametrize("n,bins", [(300, 160), (517, 16), (64, 300)])
+def test_coordinate_pr_matches_brute_force(n, bins):
+    rng = np.random.default_rng(n + bins)
+    xyz = rng.normal(scale=9.0, size=(n, 3))
+    el = rng.choice([1, 2, 3, 4], size=n)
+    atoms = O.Atoms(xyz, el)
+    rg, dmax, hist = O.coordinate_features(atoms, bins)
+    w = np.array([max(abs(O.lib().or_form_factor(int(e), -1, 1.0, 1, 0.0)), 1e-6) for e in el])
+    dmax_b, hist_b, _ = _pr_brute(xyz, w, max(bins, 16))
+    assert dmax == dmax_b
+    assert np.allclose(hist, hist_b, rtol=1e-12, atol=0)
+    assert abs(hist.sum() - (w.sum() ** 2 - (w * w).sum()) / 2) <= 1e-12 * hist.sum()
+    c = (xyz * w[:, None]).sum(axis=0) / w.sum()
+    assert abs(rg - np.sqrt((w * ((xyz - c) ** 2).sum(axis=1)).sum() / w.sum())) < 1e-12 * rg
+
+
+def test_coordinate_pr_degenerate():
+    rg, dmax, hist = O.coordinate_features(O.Atoms([[1.0, 2.0, 3.0]], [1]), 160)
+    # (the weighted centre x w / w rounds, so Rg is ~1e-16, as in the reference)
+    assert dmax == 0.0 and hist is None and rg < 1e-12
