@@ -93,12 +93,11 @@ SIGNATURES = {
                                        C.POINTER(C.c_void_p)]),
     "csx_structure_destroy": (None, [C.c_void_p]),
     "csx_structure_atom_count": (C.c_size_t, [C.c_void_p]),
-    "csx_build_adaptive_hydration_grid": (C.c_int, [c_double_p, C.c_size_t, C.c_double,
-                                                    C.c_double, C.c_double, c_double_p,
-                                                    C.c_size_t, c_size_t_p, c_double_p]),
-    "csx_build_hydration_grid": (C.c_int, [c_double_p, C.POINTER(C.c_uint8), C.c_size_t,
-                                           C.c_double, C.c_double, C.c_double, c_double_p,
-                                           C.c_size_t, c_size_t_p, c_double_p]),
+    "csx_build_adaptive_hydration_grid": (C.c_int, [C.c_void_p, C.c_double, C.c_double,
+                                                    C.c_double, c_double_p, C.c_size_t,
+                                                    c_size_t_p, c_double_p]),
+    "csx_build_hydration_grid": (C.c_int, [C.c_void_p, C.c_double, C.c_double, C.c_double,
+                                           c_double_p, C.c_size_t, c_size_t_p, c_double_p]),
     "csx_debye_intensity": (C.c_int, [C.c_void_p, c_double_p, C.c_size_t, C.c_int, C.c_int,
                                       c_double_p]),
     "csx_debye_solvent_basis": (C.c_int, [C.c_void_p, c_double_p, C.c_size_t,

@@ -539,20 +539,23 @@ def _grid(call, *args) -> HydrationGrid:
 
 def build_adaptive_hydration_grid(structure: Structure, spacing_angstrom: float = 4.0,
                                   molecular_clearance_angstrom: float = 3.0,
-                                  padding_angstrom: float = 12.0) -> HydrationGrid:
-    """`build_adaptive_hydration_grid` (src/solvent.rs:121-223)."""
+                                  padding_angstrom: float = 12.0,
+                                  ctx: Context = None) -> HydrationGrid:
+    """`build_adaptive_hydration_grid` (src/solvent.rs:121-223), rasterised on
+    the GPU."""
     lib = _lib.load()
-    return _grid(lib.csx_build_adaptive_hydration_grid, dptr(structure.xyz), len(structure),
+    return _grid(lib.csx_build_adaptive_hydration_grid, structure.handle(ctx),
                  spacing_angstrom, molecular_clearance_angstrom, padding_angstrom)
 
 
 def build_hydration_grid(structure: Structure, spacing_angstrom: float = 1.0,
                          probe_radius_angstrom: float = 1.4,
-                         shell_thickness_angstrom: float = 3.0) -> HydrationGrid:
-    """`build_hydration_grid` (src/solvent.rs:259-401)."""
+                         shell_thickness_angstrom: float = 3.0,
+                         ctx: Context = None) -> HydrationGrid:
+    """`build_hydration_grid` (src/solvent.rs:259-401): rasterisation, exterior
+    flood fill and shell on the GPU."""
     lib = _lib.load()
-    return _grid(lib.csx_build_hydration_grid, dptr(structure.xyz),
-                 structure.element.ctypes.data_as(C.POINTER(C.c_uint8)), len(structure),
+    return _grid(lib.csx_build_hydration_grid, structure.handle(ctx),
                  spacing_angstrom, probe_radius_angstrom, shell_thickness_angstrom)
 
 

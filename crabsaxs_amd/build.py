@@ -36,11 +36,13 @@ SOURCES = [
     ("adjoint.cpp", "host"),
     ("rng.cpp", "host"),
     ("pr.cpp", "host"),
+    ("grid.cpp", "host"),
     ("debye.hip", "device"),
     ("multipole.hip", "device"),
     ("adjoint.hip", "device"),
     ("ensemble.hip", "device-exact"),
     ("pr.hip", "device-exact"),
+    ("grid.hip", "device-exact"),
 ]
 
 

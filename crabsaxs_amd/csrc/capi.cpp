@@ -41,14 +41,13 @@ static csx_status build_point_set(csx_structure* s, const csx_solvent_params* p,
   const int T = static_cast<int>(s->types.size());
   const size_t n = s->n;
   if (p && p->hydration_model != CSX_HYDRATION_ANALYTIC) {
-    Grid g = p->hydration_model == CSX_HYDRATION_GRID
-                 ? build_hydration_grid(s->xyz.data(), s->element.data(), n, 1.0, 1.4,
-                                        p->shell_thickness_angstrom)
-                 : build_adaptive_hydration_grid(s->xyz.data(), n,
-                                                 p->hydration_grid_spacing_angstrom, 3.0, 12.0);
-    if (!g.error.empty()) return fail(CSX_INVALID_INPUT, g.error);
-    ps.site_xyz = std::move(g.sites);
-    ps.site_volume = g.voxel_volume;
+    const Grid* g = nullptr;
+    CSX_TRY(p->hydration_model == CSX_HYDRATION_GRID
+                ? structure_grid(s, kExteriorGrid, 1.0, 1.4, p->shell_thickness_angstrom, &g)
+                : structure_grid(s, kAdaptiveGrid, p->hydration_grid_spacing_angstrom, 3.0, 12.0,
+                                 &g));
+    ps.site_xyz = g->sites;
+    ps.site_volume = g->voxel_volume;
     ps.nsites = static_cast<int>(ps.site_xyz.size() / 3);
   }
   const bool with_sites = p && p->hydration_model != CSX_HYDRATION_ANALYTIC;
@@ -672,26 +671,6 @@ static csx_status copy_grid(const Grid& g, double* sites, size_t capacity, size_
   if (volume) *volume = g.voxel_volume;
   if (sites && capacity) std::copy(g.sites.begin(), g.sites.begin() + 3 * std::min(capacity, ns), sites);
   return CSX_OK;
-}
-
-csx_status csx_build_adaptive_hydration_grid(const double* xyz, size_t n, double spacing,
-                                             double clearance, double padding, double* sites,
-                                             size_t capacity, size_t* count,
-                                             double* voxel_volume) {
-  if (n > 0 && !xyz) return fail(CSX_INVALID_INPUT, "xyz is required");
-  return copy_grid(build_adaptive_hydration_grid(xyz, n, spacing, clearance, padding), sites,
-                   capacity, count, voxel_volume);
-}
-
-csx_status csx_build_hydration_grid(const double* xyz, const uint8_t* element, size_t n,
-                                    double spacing, double probe_radius, double shell_thickness,
-                                    double* sites, size_t capacity, size_t* count,
-                                    double* voxel_volume) {
-  if (n > 0 && (!xyz || !element)) return fail(CSX_INVALID_INPUT, "xyz and element are required");
-  for (size_t i = 0; i < n; ++i)
-    if (element[i] >= CSX_NUM_ELEMENTS) return fail(CSX_INVALID_INPUT, "element code out of range");
-  return copy_grid(build_hydration_grid(xyz, element, n, spacing, probe_radius, shell_thickness),
-                   sites, capacity, count, voxel_volume);
 }
 
 csx_status csx_generate_shell_sites(const double* xyz, size_t n, double spacing, double inner,

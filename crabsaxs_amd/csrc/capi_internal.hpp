@@ -153,6 +153,7 @@ struct csx_structure {
   double max_radius = 0.0;          // from the centroid
   std::mutex mu;
   std::map<std::string, std::unique_ptr<csx::PointSet>> sets;
+  std::map<std::string, csx::Grid> grids;  // hydration grids by kind and options
 };
 
 namespace csx {
@@ -230,5 +231,13 @@ class Call {
 // Point set of a structure for a solvent model (atoms only when p is NULL or
 // analytic), built and uploaded on first use.
 csx_status get_point_set(csx_structure* s, const csx_solvent_params* p, PointSet** out);
+
+// Hydration grid of a structure, built on its device on first use and kept
+// with the structure; the caller holds s->mu.  kAdaptiveGrid takes (spacing,
+// clearance, padding), kExteriorGrid (spacing, probe radius, shell thickness).
+constexpr int kAdaptiveGrid = 0;
+constexpr int kExteriorGrid = 1;
+csx_status structure_grid(csx_structure* s, int kind, double spacing, double a, double b,
+                          const Grid** out);
 
 }  // namespace csx

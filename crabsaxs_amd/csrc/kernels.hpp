@@ -52,6 +52,34 @@ class TimerScope {
   bool stopped_ = false;
 };
 
+// Hydration grids (grid.hip).  A grid of nx * ny * nz voxels (at most
+// INT32_MAX) with voxel (x, y, z) centred at lo + (x h, y h, z h); byte masks
+// are padded to grid_padded_voxels(count) and zero beyond the grid.
+struct GridGeom {
+  double lo[3];
+  double h;
+  int nx, ny, nz;
+};
+struct ElementRadii {
+  double r[16];
+};
+long long grid_padded_voxels(long long count);
+hipError_t launch_grid_fill(uint8_t* p, long long padded, hipStream_t stream);
+hipError_t launch_grid_raster(const double* xyz, const uint8_t* element, int n,
+                              const ElementRadii& radii, int per_element, double extra,
+                              const GridGeom& g, uint8_t* mask, int value, int exterior_only,
+                              hipStream_t stream, KernelTimer* timer);
+// Unknown voxels (0) connected to the grid boundary become exterior (2).
+hipError_t launch_grid_flood(uint8_t* state, const GridGeom& g, int* labels, uint8_t* exterior,
+                             hipStream_t stream, KernelTimer* timer);
+// mode 0 keeps a[v] == 3, mode 1 keeps a[v] != 0 && b[v] == 0; counts has
+// padded / 4096 entries, offsets one more (offsets[last] = kept voxels).
+hipError_t launch_grid_count(int mode, const uint8_t* a, const uint8_t* b, long long padded,
+                             long long* counts, long long* offsets, hipStream_t stream);
+hipError_t launch_grid_emit(int mode, const uint8_t* a, const uint8_t* b, long long padded,
+                            const GridGeom& g, const long long* offsets, double* sites,
+                            hipStream_t stream);
+
 // EnsembleFitter::fit_core candidates (ensemble.hip): profiles of every
 // (c1, c2) candidate and their projected-gradient weight loops.
 hipError_t launch_ensemble_candidates(const double* bases, const double* g, const double* c2v,

@@ -6,7 +6,6 @@
 #include <cctype>
 #include <cmath>
 #include <cstring>
-#include <deque>
 #include <limits>
 #include <random>
 #include <unordered_map>
@@ -226,43 +225,46 @@ Amplitudes multipole_amplitudes(const TypeKey& t, bool implicit_h, double q, dou
 
 // ---------------------------------------------------------------- grids --
 
-namespace {
-
-struct Box {
-  double lo[3], hi[3];
-  size_t dim[3];
-};
-
-bool make_box(const double* xyz, size_t n, double margin, double spacing, Box& box) {
+bool grid_box(const double* xyz, size_t n, double margin, double spacing, GridBox& box) {
   for (int a = 0; a < 3; ++a) box.lo[a] = box.hi[a] = xyz[a];
   for (size_t i = 1; i < n; ++i)
     for (int a = 0; a < 3; ++a) {
       box.lo[a] = std::min(box.lo[a], xyz[3 * i + a]);
       box.hi[a] = std::max(box.hi[a], xyz[3 * i + a]);
     }
+  box.count = 1;
   for (int a = 0; a < 3; ++a) {
     box.lo[a] -= margin;
     box.hi[a] += margin;
     const double cells = std::ceil((box.hi[a] - box.lo[a]) / spacing - 1e-10);
     if (!(cells < 1e15)) return false;
     box.dim[a] = static_cast<size_t>(cells) + 1;
+    if (box.count > std::numeric_limits<size_t>::max() / box.dim[a]) return false;
+    box.count *= box.dim[a];
   }
   return true;
 }
 
-bool checked_volume(const Box& b, size_t& count) {
-  count = 1;
-  for (int a = 0; a < 3; ++a) {
-    if (b.dim[a] != 0 && count > std::numeric_limits<size_t>::max() / b.dim[a]) return false;
-    count *= b.dim[a];
+double adaptive_site_volume(double spacing) {
+  if (spacing > 4.0) {
+    const double h = spacing / 2.0;
+    return 2.0 * (h * h * h);
   }
-  return true;
+  return spacing * spacing * spacing;
 }
+
+double adaptive_shell_width(double rg) {
+  if (rg <= 15.0) return 3.0;
+  if (rg >= 20.0) return 5.0;
+  return 3.0 + 0.4 * (rg - 15.0);
+}
+
+namespace {
 
 // Visit every voxel whose centre lies within `radius` of `p`, with the bound
 // rounding of solvent.rs:187-199.
 template <class F>
-void for_voxels_within(const Box& b, double spacing, const double* p, double radius, F&& f) {
+void for_voxels_within(const GridBox& b, double spacing, const double* p, double radius, F&& f) {
   size_t lo[3], hi[3];
   for (int a = 0; a < 3; ++a) {
     const double lower = (p[a] - radius - b.lo[a]) / spacing;
@@ -283,163 +285,38 @@ void for_voxels_within(const Box& b, double spacing, const double* p, double rad
       }
 }
 
-void emit_sites(const Box& b, double spacing, const std::vector<uint8_t>& keep, Grid& g) {
-  for (size_t z = 0; z < b.dim[2]; ++z)
-    for (size_t y = 0; y < b.dim[1]; ++y)
-      for (size_t x = 0; x < b.dim[0]; ++x)
-        if (keep[(z * b.dim[1] + y) * b.dim[0] + x]) {
-          g.sites.push_back(b.lo[0] + static_cast<double>(x) * spacing);
-          g.sites.push_back(b.lo[1] + static_cast<double>(y) * spacing);
-          g.sites.push_back(b.lo[2] + static_cast<double>(z) * spacing);
-        }
-}
+}  // namespace
 
-double adaptive_site_volume(double spacing) {
-  if (spacing > 4.0) {
-    const double h = spacing / 2.0;
-    return 2.0 * (h * h * h);
-  }
-  return spacing * spacing * spacing;
-}
-
-double adaptive_shell_width(double rg) {
-  if (rg <= 15.0) return 3.0;
-  if (rg >= 20.0) return 5.0;
-  return 3.0 + 0.4 * (rg - 15.0);
-}
-
-// Sites whose nearest atom lies in (inner, outer], rasterised on a box with
-// the given margin: the two-mask scheme of solvent.rs:182-218.
-Grid two_radius_shell(const double* xyz, size_t n, double spacing, double inner, double outer,
-                      double margin, double volume) {
+// Synthetic shell sites for workloads: grid points whose nearest atom lies in
+// (inner, outer], from the two-mask scheme of solvent.rs:182-218 on the host.
+Grid shell_sites(const double* xyz, size_t n, double spacing, double inner, double outer) {
+  if (n == 0) return Grid{};
   Grid g;
-  g.voxel_volume = volume;
-  Box b;
-  size_t count = 0;
-  if (!make_box(xyz, n, margin, spacing, b) || !checked_volume(b, count)) {
-    g.error = "adaptive hydration grid is too large";
+  if (!std::isfinite(spacing) || spacing <= 0.0 || !(inner >= 0.0) || !(outer > inner)) {
+    g.error = "shell-site spacing and radii must be finite with outer > inner >= 0";
     return g;
   }
-  std::vector<uint8_t> shell, molecular;
-  try {
-    shell.assign(count, 0);
-    molecular.assign(count, 0);
-  } catch (const std::bad_alloc&) {
-    g.error = "adaptive hydration grid is too large";
+  g.voxel_volume = spacing * spacing * spacing;
+  GridBox b;
+  if (!grid_box(xyz, n, outer + spacing, spacing, b) || b.count > (size_t(1) << 31)) {
+    g.error = "shell-site grid is too large";
     return g;
   }
+  std::vector<uint8_t> shell(b.count, 0), molecular(b.count, 0);
   for (size_t i = 0; i < n; ++i)
     for_voxels_within(b, spacing, xyz + 3 * i, outer, [&](size_t c) { shell[c] = 1; });
   for (size_t i = 0; i < n; ++i)
     for_voxels_within(b, spacing, xyz + 3 * i, inner, [&](size_t c) { molecular[c] = 1; });
-  for (size_t c = 0; c < count; ++c) shell[c] = shell[c] && !molecular[c];
-  emit_sites(b, spacing, shell, g);
-  return g;
-}
-
-}  // namespace
-
-Grid build_adaptive_hydration_grid(const double* xyz, size_t n, double spacing, double clearance,
-                                   double padding) {
-  if (n == 0) {
-    Grid g;
-    g.voxel_volume = adaptive_site_volume(spacing);
-    return g;
-  }
-  if (!std::isfinite(spacing) || spacing <= 0.0 || !std::isfinite(clearance) || clearance <= 0.0 ||
-      !std::isfinite(padding) || padding < 0.0) {
-    Grid g;
-    g.error = "adaptive hydration-grid dimensions must be finite and positive";
-    return g;
-  }
-  double c[3] = {0, 0, 0};
-  for (size_t i = 0; i < n; ++i)
-    for (int a = 0; a < 3; ++a) c[a] += xyz[3 * i + a];
-  for (int a = 0; a < 3; ++a) c[a] /= static_cast<double>(n);
-  double ss = 0.0;
-  for (size_t i = 0; i < n; ++i) {
-    const double dx = xyz[3 * i] - c[0], dy = xyz[3 * i + 1] - c[1], dz = xyz[3 * i + 2] - c[2];
-    ss += dx * dx + dy * dy + dz * dz;
-  }
-  const double rg = std::sqrt(ss / static_cast<double>(n));
-  const double outer = clearance + adaptive_shell_width(rg);
-  return two_radius_shell(xyz, n, spacing, clearance, outer, padding,
-                          adaptive_site_volume(spacing));
-}
-
-Grid shell_sites(const double* xyz, size_t n, double spacing, double inner, double outer) {
-  if (n == 0) return Grid{};
-  if (!std::isfinite(spacing) || spacing <= 0.0 || !(inner >= 0.0) || !(outer > inner)) {
-    Grid g;
-    g.error = "shell-site spacing and radii must be finite with outer > inner >= 0";
-    return g;
-  }
-  return two_radius_shell(xyz, n, spacing, inner, outer, outer + spacing,
-                          spacing * spacing * spacing);
-}
-
-Grid build_hydration_grid(const double* xyz, const uint8_t* element, size_t n, double spacing,
-                          double probe, double shell) {
-  Grid g;
-  g.voxel_volume = spacing * spacing * spacing;
-  if (n == 0) return g;
-  if (!std::isfinite(spacing) || spacing <= 0.0 || !std::isfinite(probe) || probe < 0.0 ||
-      !std::isfinite(shell) || shell <= 0.0) {
-    g.error = "solvent-grid dimensions must be finite and positive";
-    return g;
-  }
-  double vdw_max = 0.0;
-  for (size_t i = 0; i < n; ++i) vdw_max = std::max(vdw_max, CSX_VDW_RADIUS[element[i]]);
-  const double margin = vdw_max + probe + shell + spacing;
-  Box b;
-  size_t count = 0;
-  if (!make_box(xyz, n, margin, spacing, b) || !checked_volume(b, count)) {
-    g.error = "solvent grid is too large";
-    return g;
-  }
-  if (count > 250000000) {
-    g.error = "solvent grid requires " + std::to_string(count) + " voxels; limit is 250000000";
-    return g;
-  }
-  // 0 unknown, 1 molecular, 2 exterior, 3 hydration shell.
-  std::vector<uint8_t> state(count, 0);
-  for (size_t i = 0; i < n; ++i)
-    for_voxels_within(b, spacing, xyz + 3 * i, CSX_VDW_RADIUS[element[i]] + probe,
-                      [&](size_t c) { state[c] = 1; });
-  const size_t nx = b.dim[0], ny = b.dim[1], nz = b.dim[2];
-  std::deque<size_t> queue;
-  for (size_t z = 0; z < nz; ++z)
-    for (size_t y = 0; y < ny; ++y)
-      for (size_t x = 0; x < nx; ++x) {
-        const bool boundary = x == 0 || x + 1 == nx || y == 0 || y + 1 == ny || z == 0 || z + 1 == nz;
-        if (!boundary) continue;
-        const size_t c = (z * ny + y) * nx + x;
-        if (state[c] == 0) {
-          state[c] = 2;
-          queue.push_back(c);
+  for (size_t z = 0; z < b.dim[2]; ++z)
+    for (size_t y = 0; y < b.dim[1]; ++y)
+      for (size_t x = 0; x < b.dim[0]; ++x) {
+        const size_t c = (z * b.dim[1] + y) * b.dim[0] + x;
+        if (shell[c] && !molecular[c]) {
+          g.sites.push_back(b.lo[0] + static_cast<double>(x) * spacing);
+          g.sites.push_back(b.lo[1] + static_cast<double>(y) * spacing);
+          g.sites.push_back(b.lo[2] + static_cast<double>(z) * spacing);
         }
       }
-  while (!queue.empty()) {
-    const size_t c = queue.front();
-    queue.pop_front();
-    const size_t x = c % nx, y = (c / nx) % ny, z = c / (nx * ny);
-    const size_t nb[6] = {x > 0 ? c - 1 : c,           x + 1 < nx ? c + 1 : c,
-                          y > 0 ? c - nx : c,          y + 1 < ny ? c + nx : c,
-                          z > 0 ? c - nx * ny : c,     z + 1 < nz ? c + nx * ny : c};
-    for (size_t k : nb)
-      if (state[k] == 0) {
-        state[k] = 2;
-        queue.push_back(k);
-      }
-  }
-  for (size_t i = 0; i < n; ++i)
-    for_voxels_within(b, spacing, xyz + 3 * i, CSX_VDW_RADIUS[element[i]] + probe + shell,
-                      [&](size_t c) {
-                        if (state[c] == 2) state[c] = 3;
-                      });
-  std::vector<uint8_t> keep(count);
-  for (size_t c = 0; c < count; ++c) keep[c] = state[c] == 3;
-  emit_sites(b, spacing, keep, g);
   return g;
 }
 

@@ -49,18 +49,26 @@ Amplitudes debye_amplitudes(const TypeKey& t, bool implicit_h, double q,
 Amplitudes multipole_amplitudes(const TypeKey& t, bool implicit_h, double q,
                                 double shell_thickness, double rho);
 
-// Hydration grids.  Both return sites in (z, y, x) scan order, exactly as the
-// reference emits them.  Errors are reported as a non-empty string.
+// Grid sites in (z, y, x) scan order, as the reference emits them; errors are
+// reported as a non-empty string.
 struct Grid {
   std::vector<double> sites;  // 3 per site
   double voxel_volume = 0.0;
   std::string error;
 };
-Grid build_adaptive_hydration_grid(const double* xyz, size_t n, double spacing,
-                                   double clearance, double padding);
-Grid build_hydration_grid(const double* xyz, const uint8_t* element, size_t n,
-                          double spacing, double probe, double shell);
-// Grid points whose nearest atom is in (inner, outer].
+// Voxel box of the hydration grids (solvent.rs:166-181, 286-306): the
+// atoms' bounding box grown by `margin`, ceil((hi - lo) / h - 1e-10) + 1
+// voxels per axis, count = their product.  False when the count overflows.
+struct GridBox {
+  double lo[3], hi[3];
+  size_t dim[3];
+  size_t count;
+};
+bool grid_box(const double* xyz, size_t n, double margin, double spacing, GridBox& box);
+double adaptive_site_volume(double spacing);  // solvent.rs:106-115
+double adaptive_shell_width(double rg);       // solvent.rs:96-104
+// Grid points whose nearest atom is in (inner, outer] (synthetic workloads;
+// host-side).
 Grid shell_sites(const double* xyz, size_t n, double spacing, double inner,
                  double outer);
 

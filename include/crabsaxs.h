@@ -183,22 +183,26 @@ csx_status csx_structure_create(csx_ctx *ctx, size_t n, const double *xyz,
 void csx_structure_destroy(csx_structure *s);
 size_t csx_structure_atom_count(const csx_structure *s);
 
-/* ---- hydration grids (src/solvent.rs:121-401), host side --------------- */
+/* ---- hydration grids (src/solvent.rs:121-401) ---------------------------- */
 
 /* build_adaptive_hydration_grid (solvent.rs:121-223) with
- * AdaptiveHydrationOptions{spacing, clearance, padding}.
- * Writes at most `capacity` sites (3 doubles each) and the true count to
- * *count; call with capacity 0 to size the buffer. */
+ * AdaptiveHydrationOptions{spacing, clearance, padding}, rasterised on the
+ * structure's device.  Sites are bitwise the reference's, in its (z, y, x)
+ * scan order.  Writes at most `capacity` sites (3 doubles each) and the true
+ * count to *count; call with capacity 0 to size the buffer (the grid is kept
+ * with the structure, so the second call does not rebuild it).  Grids of
+ * more than 2^31 - 2^16 voxels are rejected. */
 csx_status csx_build_adaptive_hydration_grid(
-    const double *xyz, size_t n, double spacing, double clearance,
-    double padding, double *sites, size_t capacity, size_t *count,
-    double *voxel_volume);
+    csx_structure *s, double spacing, double clearance, double padding,
+    double *sites, size_t capacity, size_t *count, double *voxel_volume);
 
-/* build_hydration_grid (solvent.rs:259-401) with SolventGridOptions. */
+/* build_hydration_grid (solvent.rs:259-401) with SolventGridOptions: the
+ * probe-expanded rasterisation, the exterior flood fill and the shell, on
+ * the structure's device; same output protocol. */
 csx_status csx_build_hydration_grid(
-    const double *xyz, const uint8_t *element, size_t n, double spacing,
-    double probe_radius, double shell_thickness, double *sites,
-    size_t capacity, size_t *count, double *voxel_volume);
+    csx_structure *s, double spacing, double probe_radius,
+    double shell_thickness, double *sites, size_t capacity, size_t *count,
+    double *voxel_volume);
 
 /* ---- Debye path ---------------------------------------------------------- */
 

@@ -360,15 +360,6 @@ struct SolventGridOptions {
 };
 
 namespace detail {
-inline std::vector<double> coordinates(const Structure& s) {
-  std::vector<double> xyz(3 * s.atoms.size());
-  for (size_t i = 0; i < s.atoms.size(); ++i) {
-    xyz[3 * i] = s.atoms[i].pos.x;
-    xyz[3 * i + 1] = s.atoms[i].pos.y;
-    xyz[3 * i + 2] = s.atoms[i].pos.z;
-  }
-  return xyz;
-}
 template <class Call>
 HydrationGrid grid(Call call) {
   size_t count = 0;
@@ -384,24 +375,24 @@ HydrationGrid grid(Call call) {
 }
 }  // namespace detail
 
+// Both grids are built on the structure's device (bitwise the reference's
+// sites, in its scan order).
 inline HydrationGrid build_adaptive_hydration_grid(const Structure& s,
-                                                   const AdaptiveHydrationOptions& o) {
-  const std::vector<double> xyz = detail::coordinates(s);
+                                                   const AdaptiveHydrationOptions& o,
+                                                   Context& ctx = Context::default_context()) {
+  csx_structure* h = s.handle(ctx);
   return detail::grid([&](double* sites, size_t cap, size_t* count, double* vol) {
-    return csx_build_adaptive_hydration_grid(xyz.data(), s.atoms.size(), o.spacing_angstrom,
-                                             o.molecular_clearance_angstrom, o.padding_angstrom,
-                                             sites, cap, count, vol);
+    return csx_build_adaptive_hydration_grid(h, o.spacing_angstrom, o.molecular_clearance_angstrom,
+                                             o.padding_angstrom, sites, cap, count, vol);
   });
 }
 
-inline HydrationGrid build_hydration_grid(const Structure& s, const SolventGridOptions& o) {
-  const std::vector<double> xyz = detail::coordinates(s);
-  std::vector<uint8_t> el(s.atoms.size());
-  for (size_t i = 0; i < el.size(); ++i) el[i] = static_cast<uint8_t>(s.atoms[i].element);
+inline HydrationGrid build_hydration_grid(const Structure& s, const SolventGridOptions& o,
+                                          Context& ctx = Context::default_context()) {
+  csx_structure* h = s.handle(ctx);
   return detail::grid([&](double* sites, size_t cap, size_t* count, double* vol) {
-    return csx_build_hydration_grid(xyz.data(), el.data(), s.atoms.size(), o.spacing_angstrom,
-                                    o.probe_radius_angstrom, o.shell_thickness_angstrom, sites,
-                                    cap, count, vol);
+    return csx_build_hydration_grid(h, o.spacing_angstrom, o.probe_radius_angstrom,
+                                    o.shell_thickness_angstrom, sites, cap, count, vol);
   });
 }
 

@@ -608,8 +608,10 @@ long or_exterior_grid(const double *xyz, const int *element, size_t n, double h,
     }
     ctx.value = 3;
     ctx.only_exterior = 1;
+    /* radius_extra = probe + shell is summed first (solvent.rs:386-391, 319) */
+    const double extra = probe + shell;
     for (size_t i = 0; i < n; ++i)
-        rasterize(&b, h, xyz + 3 * i, CSX_VDW_RADIUS[element[i]] + probe + shell, set_state, &ctx);
+        rasterize(&b, h, xyz + 3 * i, CSX_VDW_RADIUS[element[i]] + extra, set_state, &ctx);
     for (size_t c = 0; c < count; ++c) state[c] = state[c] == 3;
     const long ns = emit(&b, h, state, sites, cap);
     free(state);

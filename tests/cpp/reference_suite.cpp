@@ -249,7 +249,7 @@ CSX_TEST(vacuum_minus_solvent_is_finite_for_structure_fixture, true) {
   CHECK(curve[0] > 0.0);
 }
 
-CSX_TEST(hydration_grid_is_deterministic_and_exterior, false) {
+CSX_TEST(hydration_grid_is_deterministic_and_exterior, true) {
   const Structure s({carbon_oxygen().atoms[0]});
   const SolventGridOptions o;
   const HydrationGrid first = build_hydration_grid(s, o), second = build_hydration_grid(s, o);
@@ -269,7 +269,7 @@ CSX_TEST(hydration_grid_is_deterministic_and_exterior, false) {
   }
 }
 
-CSX_TEST(adaptive_hydration_grid_is_deterministic, false) {
+CSX_TEST(adaptive_hydration_grid_is_deterministic, true) {
   const AdaptiveHydrationOptions o;
   const HydrationGrid first = build_adaptive_hydration_grid(carbon_oxygen(), o);
   const HydrationGrid second = build_adaptive_hydration_grid(carbon_oxygen(), o);
@@ -281,13 +281,13 @@ CSX_TEST(adaptive_hydration_grid_is_deterministic, false) {
           first.sites[k].z == second.sites[k].z);
 }
 
-CSX_TEST(adaptive_fast_grid_halves_linear_point_density_weight, false) {
+CSX_TEST(adaptive_fast_grid_halves_linear_point_density_weight, true) {
   AdaptiveHydrationOptions o;
   o.spacing_angstrom = 8.0;
   CHECK(build_adaptive_hydration_grid(carbon_oxygen(), o).voxel_volume_a3 == 2.0 * 64.0);
 }
 
-CSX_TEST(hydration_grids_are_translation_invariant, false) {
+CSX_TEST(hydration_grids_are_translation_invariant, true) {
   const Structure s = carbon_oxygen();
   const Vector3 shift{3.17, -8.4, 2.6};
   Structure moved = s;

@@ -112,44 +112,6 @@ def two_atoms():
                                    atom_name="O")])
 
 
-def test_grids_match_oracle_and_reference_properties():
-    s = two_atoms()
-    g1 = csx.build_adaptive_hydration_grid(s)
-    g2 = csx.build_adaptive_hydration_grid(s)
-    ref, vol = O.adaptive_grid(s.xyz)
-    assert len(g1.sites) > 0 and np.array_equal(g1.sites, g2.sites)
-    assert np.array_equal(g1.sites, ref) and g1.voxel_volume_a3 == vol == 64.0
-    fast = csx.build_adaptive_hydration_grid(s, spacing_angstrom=8.0)
-    assert fast.voxel_volume_a3 == 2.0 * 4.0 ** 3  # solvent.rs:728-738
-    one = csx.Structure([csx.Atom(pos=(-1, 0, 0), element=csx.Element.C, residue_name="TST")])
-    ext = csx.build_hydration_grid(one)
-    ref_e, vol_e = O.exterior_grid(one.xyz, one.element)
-    assert np.array_equal(ext.sites, ref_e) and ext.voxel_volume_a3 == vol_e == 1.0
-    d = np.linalg.norm(ext.sites - one.xyz[0], axis=1)  # solvent.rs:688-705
-    assert len(d) > 0 and np.all(d > 1.70 + 1.4) and np.all(d <= 1.70 + 1.4 + 3.0 + 1e-12)
-
-
-def test_grids_translation_invariant():
-    s = two_atoms()
-    shift = np.array([3.17, -8.4, 2.6])
-    t = csx.Structure.from_arrays(s.xyz + shift, s.element, s.occupancy, s.group)
-    for build in (csx.build_adaptive_hydration_grid, csx.build_hydration_grid):
-        a, b = build(s).sites, build(t).sites
-        assert a.shape == b.shape
-        assert np.abs(b - a - shift).max() < 1e-12
-
-
-def test_grid_on_generated_structure_matches_oracle():
-    s = csx.generate_globule(400, 12.0, 1.0, seed=3)
-    for h in (3.0, 4.0, 8.0):
-        a = csx.build_adaptive_hydration_grid(s, spacing_angstrom=h)
-        ref, _ = O.adaptive_grid(s.xyz, spacing=h)
-        assert np.array_equal(a.sites, ref)
-    e = csx.build_hydration_grid(s, spacing_angstrom=1.5)
-    ref_e, _ = O.exterior_grid(s.xyz, s.element, spacing=1.5)
-    assert np.array_equal(e.sites, ref_e)
-
-
 def test_generator_deterministic_and_shaped():
     a = csx.generate_globule(1200, 16.0, 1.0, seed=7)
     b = csx.generate_globule(1200, 16.0, 1.0, seed=7)

@@ -337,3 +337,54 @@ def test_coordinate_pr_degenerate():
     rg, dmax, hist = O.coordinate_features(O.Atoms([[1.0, 2.0, 3.0]], [1]), 160)
     # (the weighted centre x w / w rounds, so Rg is ~1e-16, as in the reference)
     assert dmax == 0.0 and hist is None and rg < 1e-12
+
+
+# ------------------------------------------------------------------ grids --
+# The oracle's grids against an independent brute force (tests/grid_cases.py)
+# and the reference's grid tests (solvent.rs:688-771).
+
+import grid_cases as G  # noqa: E402
+
+
+@pytest.mark.parametrize("spacing", [1.0, 3.0, 4.0, 8.0])
+def test_adaptive_grid_matches_brute_force(spacing):
+    xyz, _ = G.globule(250, 11.0, seed=4)
+    ref, vol = O.adaptive_grid(xyz, spacing=spacing)
+    assert len(ref) > 0 and np.array_equal(ref, G.adaptive_grid(xyz, spacing=spacing))
+    assert vol == (2.0 * (spacing / 2) ** 3 if spacing > 4.0 else spacing ** 3)
+
+
+@pytest.mark.parametrize("spacing,probe,shell", [(1.0, 1.4, 3.0), (1.5, 1.2, 2.0),
+                                                 (0.8, 0.0, 1.0)])
+def test_exterior_grid_matches_brute_force(spacing, probe, shell):
+    xyz, el = G.globule(250, 11.0, seed=5)
+    ref, vol = O.exterior_grid(xyz, el, spacing=spacing, probe=probe, shell=shell)
+    assert len(ref) > 0 and vol == spacing ** 3
+    assert np.array_equal(ref, G.exterior_grid(xyz, el, spacing, probe, shell))
+
+
+def test_exterior_grid_excludes_enclosed_void_and_adaptive_keeps_it():
+    xyz, el = G.hollow_sphere(9.0)
+    ext, _ = O.exterior_grid(xyz, el)
+    assert np.array_equal(ext, G.exterior_grid(xyz, el))
+    r = np.linalg.norm(ext, axis=1)
+    assert r.min() > 9.0 and len(ext) > 0  # no site inside the sealed shell
+    ada, _ = O.adaptive_grid(xyz, spacing=1.0)
+    assert np.array_equal(ada, G.adaptive_grid(xyz, spacing=1.0))
+    assert np.linalg.norm(ada, axis=1).min() < 6.0 + 1e-9  # voids are included
+
+
+def test_grids_reference_properties():
+    one = np.array([[-1.0, 0.0, 0.0]])
+    ext, vol = O.exterior_grid(one, [1])  # solvent.rs:688-705
+    d = np.linalg.norm(ext - one[0], axis=1)
+    assert len(d) > 0 and vol == 1.0
+    assert np.all(d > 1.70 + 1.4) and np.all(d <= 1.70 + 1.4 + 3.0 + 1e-12)
+    two = np.array([[-1.0, 0.0, 0.0], [1.0, 0.0, 0.0]])
+    a, vol = O.adaptive_grid(two)  # solvent.rs:716-726
+    assert len(a) > 0 and vol == 64.0
+    assert O.adaptive_grid(two, spacing=8.0)[1] == 2.0 * 4.0 ** 3  # solvent.rs:728-738
+    shift = np.array([3.17, -8.4, 2.6])  # solvent.rs:740-771
+    for build in (lambda x: O.adaptive_grid(x)[0], lambda x: O.exterior_grid(x, [1, 3])[0]):
+        g0, g1 = build(two), build(two + shift)
+        assert g0.shape == g1.shape and np.abs(g1 - g0 - shift).max() < 1e-12
