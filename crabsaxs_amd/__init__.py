@@ -5,6 +5,8 @@ The public names mirror the reference crate's API (src/lib.rs:16-50).
 from .api import (  # noqa: F401
     Atom, ComputeOptions, Context, CoordinateFeatures, CurveMetadata, DeviceError, Element,
     FitDidNotConverge, PairDistribution, coordinate_features,
+    AltlocPolicy, IoError, OccupancyPolicy, ParseError, ParseOptions, ResidueKind,
+    UnsupportedElement, residue_kind,
     batch_debye_intensity, batch_debye_solvent_basis,
     HydrationGrid, HydrationModel, HydrogenMode, InvalidInput, MultipoleOptions,
     MultipoleParams, NoDevice, multipole_intensity, multipole_solvent_basis,
@@ -24,5 +26,5 @@ from .fitting import (  # noqa: F401
     bootstrap_indices, chi2_grad,
     compare_curves, cubic_hermite_interpolate, fit_basis, fit_from_basis, fit_to_experimental,
     fit_to_experimental_with_options, nnls_weights, optimize_weights, project_simplex,
-    solve_scale_background,
+    solve_scale_background, write_curve_csv, write_curve_dat,
 )

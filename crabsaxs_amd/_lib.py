@@ -27,6 +27,35 @@ class SolventParamsC(C.Structure):
     ]
 
 
+class ParseOptionsC(C.Structure):
+    _fields_ = [
+        ("include_hetatm", C.c_int32),
+        ("include_waters", C.c_int32),
+        ("include_glycans", C.c_int32),
+        ("model_index", C.c_size_t),
+        ("allow_unknown_elements", C.c_int32),
+        ("occupancy_policy", C.c_int32),
+        ("altloc_policy", C.c_int32),
+        ("chains", C.c_char_p),
+    ]
+
+
+class AtomRecordC(C.Structure):
+    _fields_ = [
+        ("x", C.c_double),
+        ("y", C.c_double),
+        ("z", C.c_double),
+        ("occupancy", C.c_double),
+        ("element", C.c_int32),
+        ("group", C.c_int32),
+        ("is_hetatm", C.c_int32),
+        ("residue_id", C.c_int64),
+        ("atom_name", C.c_char * 16),
+        ("residue_name", C.c_char * 16),
+        ("chain_id", C.c_char * 16),
+    ]
+
+
 class ComputeOptionsC(C.Structure):
     _fields_ = [
         ("method", C.c_int32),
@@ -93,6 +122,20 @@ SIGNATURES = {
                                        C.POINTER(C.c_void_p)]),
     "csx_structure_destroy": (None, [C.c_void_p]),
     "csx_structure_atom_count": (C.c_size_t, [C.c_void_p]),
+    "csx_residue_kind": (C.c_int, [C.c_char_p]),
+    "csx_parse_options_default": (None, [C.c_void_p]),
+    "csx_read_structure_file": (C.c_int, [C.c_char_p, C.c_void_p, C.POINTER(C.c_void_p)]),
+    "csx_parse_structure": (C.c_int, [C.c_char_p, C.c_size_t, C.c_int, C.c_void_p,
+                                      C.POINTER(C.c_void_p)]),
+    "csx_atoms_count": (C.c_size_t, [C.c_void_p]),
+    "csx_atoms_get": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "csx_atoms_arrays": (C.c_int, [C.c_void_p, c_double_p, C.POINTER(C.c_uint8),
+                                   C.POINTER(C.c_int8), c_double_p]),
+    "csx_atoms_destroy": (None, [C.c_void_p]),
+    "csx_read_dat_file": (C.c_int, [C.c_char_p, c_double_p, c_double_p, c_double_p, C.c_size_t,
+                                    c_size_t_p, C.POINTER(C.c_int)]),
+    "csx_parse_dat": (C.c_int, [C.c_char_p, C.c_size_t, c_double_p, c_double_p, c_double_p,
+                                C.c_size_t, c_size_t_p, C.POINTER(C.c_int)]),
     "csx_build_adaptive_hydration_grid": (C.c_int, [C.c_void_p, C.c_double, C.c_double,
                                                     C.c_double, c_double_p, C.c_size_t,
                                                     c_size_t_p, c_double_p]),

@@ -46,7 +46,20 @@ class OutOfMemory(SaxsError):
     pass
 
 
-_STATUS = {1: InvalidInput, 2: FitDidNotConverge, 3: NoDevice, 4: DeviceError, 5: OutOfMemory}
+class ParseError(SaxsError):
+    """`SaxsError::StructureParse` / `SaxsError::ExperimentalDataParse`."""
+
+
+class UnsupportedElement(SaxsError):
+    pass
+
+
+class IoError(SaxsError):
+    pass
+
+
+_STATUS = {1: InvalidInput, 2: FitDidNotConverge, 3: NoDevice, 4: DeviceError, 5: OutOfMemory,
+           6: ParseError, 7: UnsupportedElement, 8: IoError}
 
 
 def _check(status: int) -> None:
@@ -118,6 +131,59 @@ class Atom:
     residue_id: int = 1
 
 
+class AltlocPolicy(enum.IntEnum):
+    """`AltlocPolicy` (src/structure.rs:13-21)."""
+    Highest = 0
+    First = 1
+    All = 2
+
+
+class OccupancyPolicy(enum.IntEnum):
+    """`OccupancyPolicy` (src/structure.rs:166-175)."""
+    Reported = 0
+    SelectedConformerFull = 1
+
+
+class ResidueKind(enum.IntEnum):
+    """`ResidueKind` (src/structure.rs:24-33)."""
+    Polymer = 0
+    Glycan = 1
+    Water = 2
+    Other = 3
+
+
+def residue_kind(name: str) -> ResidueKind:
+    """`residue_kind` (src/structure.rs:34-88)."""
+    return ResidueKind(_lib.load().csx_residue_kind(name.encode()))
+
+
+@dataclass
+class ParseOptions:
+    """`ParseOptions` (src/structure.rs:177-202)."""
+    include_hetatm: bool = False
+    include_waters: bool = False
+    include_glycans: bool = True
+    model_index: int = 0
+    allow_unknown_elements: bool = False
+    occupancy_policy: OccupancyPolicy = OccupancyPolicy.Reported
+    altloc_policy: AltlocPolicy = AltlocPolicy.Highest
+    chains: Optional[Sequence[str]] = None
+
+    def _c(self) -> "_lib.ParseOptionsC":
+        return _lib.ParseOptionsC(
+            int(self.include_hetatm), int(self.include_waters), int(self.include_glycans),
+            int(self.model_index), int(self.allow_unknown_elements), int(self.occupancy_policy),
+            int(self.altloc_policy),
+            None if self.chains is None else ",".join(self.chains).encode())
+
+
+def _read_atoms(path, options: Optional[ParseOptions]):
+    h = C.c_void_p()
+    opts = (options or ParseOptions())._c()
+    _check(_lib.load().csx_read_structure_file(str(path).encode(), C.byref(opts), C.byref(h)))
+    return h
+
+
 class Structure:
     """`Structure` (src/structure.rs:163-165), held as SoA arrays.
 
@@ -135,8 +201,58 @@ class Structure:
             [lib.csx_united_atom_group(a.residue_name.encode(), a.atom_name.encode())
              for a in atoms], dtype=np.int8)
         self.occupancy = f64([a.occupancy for a in atoms])
+        self.atoms = atoms
         self._handle = None
         self._ctx = None
+
+    @classmethod
+    def from_pdb_file(cls, path, options: "ParseOptions" = None) -> "Structure":
+        """`Structure::from_pdb_file[_with_options]` (src/structure.rs:205-326):
+        PDB or mmCIF."""
+        return cls._from_atoms_handle(_read_atoms(path, options))
+
+    from_pdb_file_with_options = from_pdb_file
+
+    @classmethod
+    def from_pdb_file_models_with_options(cls, path, options: "ParseOptions" = None):
+        """Every model of a file (src/structure.rs:330-346)."""
+        options = options or ParseOptions()
+        models = []
+        index = 0
+        while True:
+            try:
+                models.append(cls.from_pdb_file(path, replace(options, model_index=index)))
+            except SaxsError:
+                if index > 0 and models:
+                    break
+                raise
+            index += 1
+        return models
+
+    @classmethod
+    def parse(cls, text: str, format: int = 0, options: "ParseOptions" = None) -> "Structure":
+        """A PDB (format 1) or mmCIF (2) text; 0 detects the format."""
+        h = C.c_void_p()
+        data = text.encode()
+        opts = (options or ParseOptions())._c()
+        _check(_lib.load().csx_parse_structure(data, len(data), format, C.byref(opts),
+                                               C.byref(h)))
+        return cls._from_atoms_handle(h)
+
+    @classmethod
+    def _from_atoms_handle(cls, h) -> "Structure":
+        lib = _lib.load()
+        try:
+            n = lib.csx_atoms_count(h)
+            recs = (_lib.AtomRecordC * max(n, 1))()
+            _check(lib.csx_atoms_get(h, recs))
+        finally:
+            lib.csx_atoms_destroy(h)
+        atoms = [Atom(pos=(r.x, r.y, r.z), element=Element(r.element), occupancy=r.occupancy,
+                      is_hetatm=bool(r.is_hetatm), atom_name=r.atom_name.decode(),
+                      residue_name=r.residue_name.decode(), chain_id=r.chain_id.decode(),
+                      residue_id=int(r.residue_id)) for r in recs[:n]]
+        return cls(atoms)
 
     @classmethod
     def from_arrays(cls, xyz, element, occupancy=None, group=None) -> "Structure":
@@ -147,6 +263,7 @@ class Structure:
         s.occupancy = f64(np.ones(n) if occupancy is None else occupancy).reshape(n)
         s.group = np.ascontiguousarray(
             np.full(n, -1, dtype=np.int8) if group is None else np.asarray(group, dtype=np.int8))
+        s.atoms = None
         s._handle = None
         s._ctx = None
         return s

@@ -37,6 +37,7 @@ SOURCES = [
     ("rng.cpp", "host"),
     ("pr.cpp", "host"),
     ("grid.cpp", "host"),
+    ("ingest.cpp", "host"),
     ("debye.hip", "device"),
     ("multipole.hip", "device"),
     ("adjoint.hip", "device"),

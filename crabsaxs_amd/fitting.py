@@ -42,6 +42,54 @@ class ExperimentalCurve:
     def q_unit(self) -> str:
         return "angstrom^-1"
 
+    @classmethod
+    def from_dat_file(cls, path) -> "ExperimentalCurve":
+        """`ExperimentalCurve::from_dat_file` (src/io.rs:19-112)."""
+        return cls._read(_lib.load().csx_read_dat_file, str(path).encode())
+
+    @classmethod
+    def parse_dat(cls, text: str) -> "ExperimentalCurve":
+        """The same rules applied to text in memory."""
+        data = text.encode()
+        return cls._read(_lib.load().csx_parse_dat, data, len(data))
+
+    @classmethod
+    def _read(cls, call, *args) -> "ExperimentalCurve":
+        n, has = C.c_size_t(), C.c_int()
+        _check(call(*args, None, None, None, 0, C.byref(n), C.byref(has)))
+        q, i, s = np.zeros(n.value), np.zeros(n.value), np.zeros(n.value)
+        _check(call(*args, dptr(q), dptr(i), dptr(s), n.value, C.byref(n), C.byref(has)))
+        return cls(q, i, s, bool(has.value))
+
+
+def write_curve_dat(path, q, intensity) -> None:
+    """`write_curve_dat` (src/io.rs:143-155): a header and `{:.8e}` columns."""
+    _write_curve(path, q, intensity, "# q (1/A) intensity", " ")
+
+
+def write_curve_csv(path, q, intensity) -> None:
+    """`write_curve_csv` (src/io.rs:157-169)."""
+    _write_curve(path, q, intensity, "q,intensity", ",")
+
+
+def _write_curve(path, q, intensity, header, sep) -> None:
+    q, intensity = np.asarray(q, float), np.asarray(intensity, float)
+    if q.shape != intensity.shape:
+        raise InvalidInput("q and intensity lengths differ")
+    with open(path, "w") as f:
+        f.write(header + "\n")
+        for a, b in zip(q, intensity):
+            f.write(f"{_rust_exp(a)}{sep}{_rust_exp(b)}\n")
+
+
+def _rust_exp(v: float) -> str:
+    """Rust's `{:.8e}`: mantissa with 8 decimals, exponent without sign
+    padding or '+' (1.00000000e-2, 2.50000000e1)."""
+    if not np.isfinite(v):
+        return "NaN" if np.isnan(v) else ("inf" if v > 0 else "-inf")
+    m, e = f"{v:.8e}".split("e")
+    return f"{m}e{int(e)}"
+
 
 class BackgroundMode:
     """`BackgroundMode` (src/fit.rs:41-47): Fit, FixedZero or Fixed(value)."""

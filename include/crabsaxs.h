@@ -37,7 +37,10 @@ typedef enum {
     CSX_NOT_CONVERGED = 2,    /* SaxsError::FitDidNotConverge */
     CSX_NO_DEVICE = 3,        /* no usable HIP device */
     CSX_DEVICE_ERROR = 4,     /* a HIP runtime call failed */
-    CSX_OUT_OF_MEMORY = 5     /* device or host allocation failed */
+    CSX_OUT_OF_MEMORY = 5,    /* device or host allocation failed */
+    CSX_PARSE_ERROR = 6,      /* SaxsError::StructureParse / ExperimentalDataParse */
+    CSX_UNSUPPORTED_ELEMENT = 7, /* SaxsError::UnsupportedElement */
+    CSX_IO_ERROR = 8          /* SaxsError::Io (a file could not be read) */
 } csx_status;
 
 /* Pair-sum arithmetic. */
@@ -182,6 +185,85 @@ csx_status csx_structure_create(csx_ctx *ctx, size_t n, const double *xyz,
                                 csx_structure **out);
 void csx_structure_destroy(csx_structure *s);
 size_t csx_structure_atom_count(const csx_structure *s);
+
+/* ---- structure and curve ingest (src/structure.rs, src/io.rs) ---------- */
+
+/* ResidueKind (structure.rs:24-33) and residue_kind (structure.rs:34-88). */
+enum {
+    CSX_RESIDUE_POLYMER = 0,
+    CSX_RESIDUE_GLYCAN = 1,
+    CSX_RESIDUE_WATER = 2,
+    CSX_RESIDUE_OTHER = 3
+};
+int csx_residue_kind(const char *residue_name);
+
+/* AltlocPolicy (structure.rs:13-21), OccupancyPolicy (structure.rs:166-175). */
+enum { CSX_ALTLOC_HIGHEST = 0, CSX_ALTLOC_FIRST = 1, CSX_ALTLOC_ALL = 2 };
+enum { CSX_OCCUPANCY_REPORTED = 0, CSX_OCCUPANCY_SELECTED_FULL = 1 };
+/* Coordinate file formats of csx_parse_structure. */
+enum { CSX_FORMAT_AUTO = 0, CSX_FORMAT_PDB = 1, CSX_FORMAT_MMCIF = 2 };
+
+/* ParseOptions (structure.rs:177-202).  chains: NULL for every chain, else
+ * a comma-separated list of chain ids (ParseOptions::chains). */
+typedef struct {
+    int include_hetatm;
+    int include_waters;
+    int include_glycans;
+    size_t model_index;
+    int allow_unknown_elements;
+    int occupancy_policy;
+    int altloc_policy;
+    const char *chains;
+} csx_parse_options;
+void csx_parse_options_default(csx_parse_options *out);
+
+/* One selected atom (Atom, structure.rs:150-160), with its united-atom group
+ * (csx_united_atom_group of its residue and atom names). */
+typedef struct {
+    double x, y, z, occupancy;
+    int element;
+    int group;
+    int is_hetatm;
+    int64_t residue_id;
+    char atom_name[16];
+    char residue_name[16];
+    char chain_id[16];
+} csx_atom_record;
+
+typedef struct csx_atoms csx_atoms;
+
+/* Structure::from_pdb_file_with_options (structure.rs:209-326): PDB or
+ * mmCIF (by extension .pdb/.ent or .cif/.mmcif, else by content), model
+ * `model_index`, the reference's residue filters, element check, altloc and
+ * occupancy policies and (chain, residue number, atom name) order.
+ * Errors: CSX_IO_ERROR, CSX_PARSE_ERROR (malformed records, missing model),
+ * CSX_UNSUPPORTED_ELEMENT.  Free with csx_atoms_destroy. */
+csx_status csx_read_structure_file(const char *path,
+                                   const csx_parse_options *options,
+                                   csx_atoms **out);
+/* The same from text in memory (format: CSX_FORMAT_*). */
+csx_status csx_parse_structure(const char *text, size_t len, int format,
+                               const csx_parse_options *options,
+                               csx_atoms **out);
+size_t csx_atoms_count(const csx_atoms *a);
+/* Copy the csx_atoms_count(a) records. */
+csx_status csx_atoms_get(const csx_atoms *a, csx_atom_record *out);
+/* The SoA upload arrays of csx_structure_create (any pointer may be NULL). */
+csx_status csx_atoms_arrays(const csx_atoms *a, double *xyz, uint8_t *element,
+                            int8_t *group, double *occupancy);
+void csx_atoms_destroy(csx_atoms *a);
+
+/* ExperimentalCurve::from_dat_file (io.rs:19-112): comment and header
+ * lines, ',' separators, the 1/nm header and q > 1 heuristics (q scaled by
+ * 0.1), I = sigma = 0 sentinel rows, sigma 1 for two-column rows
+ * (*has_errors 0), validation with the reference's messages.  Writes at most
+ * `capacity` rows and the true count to *count (capacity 0 sizes). */
+csx_status csx_read_dat_file(const char *path, double *q, double *intensity,
+                             double *sigma, size_t capacity, size_t *count,
+                             int *has_errors);
+csx_status csx_parse_dat(const char *text, size_t len, double *q,
+                         double *intensity, double *sigma, size_t capacity,
+                         size_t *count, int *has_errors);
 
 /* ---- hydration grids (src/solvent.rs:121-401) ---------------------------- */
 

@@ -16,6 +16,7 @@
 #include <array>
 #include <cmath>
 #include <cstdint>
+#include <cstdio>
 #include <cstring>
 #include <memory>
 #include <mutex>
@@ -57,6 +58,19 @@ class OutOfMemory : public SaxsError {
  public:
   using SaxsError::SaxsError;
 };
+// StructureParse and ExperimentalDataParse.
+class ParseError : public SaxsError {
+ public:
+  using SaxsError::SaxsError;
+};
+class UnsupportedElement : public SaxsError {
+ public:
+  using SaxsError::SaxsError;
+};
+class IoError : public SaxsError {
+ public:
+  using SaxsError::SaxsError;
+};
 
 namespace detail {
 inline void check(csx_status s) {
@@ -67,6 +81,9 @@ inline void check(csx_status s) {
     case CSX_NOT_CONVERGED: throw FitDidNotConverge(msg);
     case CSX_NO_DEVICE: throw NoDevice(msg);
     case CSX_OUT_OF_MEMORY: throw OutOfMemory(msg);
+    case CSX_PARSE_ERROR: throw ParseError(msg);
+    case CSX_UNSUPPORTED_ELEMENT: throw UnsupportedElement(msg);
+    case CSX_IO_ERROR: throw IoError(msg);
     default: throw DeviceError(msg);
   }
 }
@@ -99,6 +116,8 @@ struct Atom {
   std::string chain_id = "A";
   long residue_id = 0;
 };
+
+struct ParseOptions;
 
 // A HIP device context (one per GPU; reentrant).
 class Context {
@@ -179,6 +198,14 @@ class Structure {
 
   size_t atom_count() const { return atoms.size(); }
 
+  // PDB / mmCIF input (src/structure.rs:205-346); defined with ParseOptions.
+  static Structure from_pdb_file(const std::string& path);
+  static Structure from_pdb_file_with_options(const std::string& path, const ParseOptions& o);
+  static std::vector<Structure> from_pdb_file_models_with_options(const std::string& path,
+                                                                  const ParseOptions& o);
+  // PDB (CSX_FORMAT_PDB) or mmCIF (CSX_FORMAT_MMCIF) text; CSX_FORMAT_AUTO detects.
+  static Structure parse(const std::string& text, int format, const ParseOptions& o);
+
   // The device copy of this structure's current atoms on `ctx`: uploaded on
   // first use and again whenever the atoms have changed since.
   csx_structure* handle(Context& ctx) const {
@@ -248,6 +275,114 @@ class Structure {
     return h;
   }
 };
+
+// ------------------------------------------------------------------ ingest
+// AltlocPolicy, ResidueKind, OccupancyPolicy, ParseOptions
+// (src/structure.rs:13-202).
+
+enum class AltlocPolicy { Highest = CSX_ALTLOC_HIGHEST, First = CSX_ALTLOC_FIRST, All = CSX_ALTLOC_ALL };
+enum class OccupancyPolicy {
+  Reported = CSX_OCCUPANCY_REPORTED,
+  SelectedConformerFull = CSX_OCCUPANCY_SELECTED_FULL
+};
+enum class ResidueKind {
+  Polymer = CSX_RESIDUE_POLYMER,
+  Glycan = CSX_RESIDUE_GLYCAN,
+  Water = CSX_RESIDUE_WATER,
+  Other = CSX_RESIDUE_OTHER
+};
+
+inline ResidueKind residue_kind(const std::string& name) {
+  return static_cast<ResidueKind>(csx_residue_kind(name.c_str()));
+}
+
+struct ParseOptions {
+  bool include_hetatm = false;
+  bool include_waters = false;
+  bool include_glycans = true;
+  size_t model_index = 0;
+  bool allow_unknown_elements = false;
+  OccupancyPolicy occupancy_policy = OccupancyPolicy::Reported;
+  AltlocPolicy altloc_policy = AltlocPolicy::Highest;
+  std::optional<std::vector<std::string>> chains;
+};
+
+namespace detail {
+// csx_parse_options of o; `chains` holds the comma-joined chain list.
+inline csx_parse_options parse_options(const ParseOptions& o, std::string& chains) {
+  csx_parse_options c;
+  csx_parse_options_default(&c);
+  c.include_hetatm = o.include_hetatm;
+  c.include_waters = o.include_waters;
+  c.include_glycans = o.include_glycans;
+  c.model_index = o.model_index;
+  c.allow_unknown_elements = o.allow_unknown_elements;
+  c.occupancy_policy = static_cast<int>(o.occupancy_policy);
+  c.altloc_policy = static_cast<int>(o.altloc_policy);
+  if (o.chains) {
+    for (size_t i = 0; i < o.chains->size(); ++i) chains += (i ? "," : "") + (*o.chains)[i];
+    c.chains = chains.c_str();
+  }
+  return c;
+}
+
+// Structure of a parsed atom set (takes ownership of h).
+inline Structure structure_of(csx_atoms* h) {
+  std::unique_ptr<csx_atoms, void (*)(csx_atoms*)> owner(h, csx_atoms_destroy);
+  std::vector<csx_atom_record> recs(csx_atoms_count(h));
+  check(csx_atoms_get(h, recs.data()));
+  std::vector<Atom> atoms(recs.size());
+  for (size_t i = 0; i < recs.size(); ++i) {
+    const csx_atom_record& r = recs[i];
+    atoms[i].pos = {r.x, r.y, r.z};
+    atoms[i].element = static_cast<Element>(r.element);
+    atoms[i].occupancy = r.occupancy;
+    atoms[i].is_hetatm = r.is_hetatm != 0;
+    atoms[i].atom_name = r.atom_name;
+    atoms[i].residue_name = r.residue_name;
+    atoms[i].chain_id = r.chain_id;
+    atoms[i].residue_id = static_cast<long>(r.residue_id);
+  }
+  return Structure(std::move(atoms));
+}
+}  // namespace detail
+
+inline Structure Structure::from_pdb_file_with_options(const std::string& path,
+                                                       const ParseOptions& o) {
+  std::string chains;
+  const csx_parse_options c = detail::parse_options(o, chains);
+  csx_atoms* h = nullptr;
+  detail::check(csx_read_structure_file(path.c_str(), &c, &h));
+  return detail::structure_of(h);
+}
+
+inline Structure Structure::from_pdb_file(const std::string& path) {
+  return from_pdb_file_with_options(path, ParseOptions{});
+}
+
+inline std::vector<Structure> Structure::from_pdb_file_models_with_options(
+    const std::string& path, const ParseOptions& o) {
+  std::vector<Structure> models;
+  for (size_t index = 0;; ++index) {
+    ParseOptions m = o;
+    m.model_index = index;
+    try {
+      models.push_back(from_pdb_file_with_options(path, m));
+    } catch (const SaxsError&) {
+      if (index > 0 && !models.empty()) break;
+      throw;
+    }
+  }
+  return models;
+}
+
+inline Structure Structure::parse(const std::string& text, int format, const ParseOptions& o) {
+  std::string chains;
+  const csx_parse_options c = detail::parse_options(o, chains);
+  csx_atoms* h = nullptr;
+  detail::check(csx_parse_structure(text.data(), text.size(), format, &c, &h));
+  return detail::structure_of(h);
+}
 
 // ------------------------------------------------------------- form factors
 
@@ -622,6 +757,18 @@ struct ExperimentalCurve {
   size_t len() const { return q.size(); }
   bool is_empty() const { return q.empty(); }
   const char* q_unit() const { return "angstrom^-1"; }
+  // ExperimentalCurve::from_dat_file (src/io.rs:19-112).
+  static ExperimentalCurve from_dat_file(const std::string& path) {
+    return read([&](double* q, double* i, double* s, size_t cap, size_t* n, int* has) {
+      return csx_read_dat_file(path.c_str(), q, i, s, cap, n, has);
+    });
+  }
+  // The same rules applied to text in memory.
+  static ExperimentalCurve parse_dat(const std::string& text) {
+    return read([&](double* q, double* i, double* s, size_t cap, size_t* n, int* has) {
+      return csx_parse_dat(text.data(), text.size(), q, i, s, cap, n, has);
+    });
+  }
   ExperimentalCurve through_qmax(double qmax) const {
     ExperimentalCurve c;
     c.has_errors = has_errors;
@@ -633,7 +780,55 @@ struct ExperimentalCurve {
       }
     return c;
   }
+
+ private:
+  template <class Call>
+  static ExperimentalCurve read(Call call) {
+    size_t n = 0;
+    int has = 0;
+    detail::check(call(nullptr, nullptr, nullptr, 0, &n, &has));
+    ExperimentalCurve c;
+    c.q.resize(n);
+    c.intensity.resize(n);
+    c.sigma.resize(n);
+    detail::check(call(c.q.data(), c.intensity.data(), c.sigma.data(), n, &n, &has));
+    c.has_errors = has != 0;
+    return c;
+  }
 };
+
+namespace detail {
+// Rust's `{:.8e}`: 8 decimals, exponent without '+' or leading zeros.
+inline std::string rust_exp(double v) {
+  if (std::isnan(v)) return "NaN";
+  if (std::isinf(v)) return v > 0 ? "inf" : "-inf";
+  char buf[64];
+  std::snprintf(buf, sizeof buf, "%.8e", v);
+  std::string s(buf);
+  const size_t e = s.find('e');
+  return s.substr(0, e + 1) + std::to_string(std::stoi(s.substr(e + 1)));
+}
+inline void write_curve(const std::string& path, const std::vector<double>& q,
+                        const std::vector<double>& intensity, const char* header, char sep) {
+  if (q.size() != intensity.size()) throw InvalidInput("q and intensity lengths differ");
+  std::FILE* f = std::fopen(path.c_str(), "w");
+  if (!f) throw IoError("cannot write " + path);
+  std::fprintf(f, "%s\n", header);
+  for (size_t i = 0; i < q.size(); ++i)
+    std::fprintf(f, "%s%c%s\n", rust_exp(q[i]).c_str(), sep, rust_exp(intensity[i]).c_str());
+  std::fclose(f);
+}
+}  // namespace detail
+
+// write_curve_dat / write_curve_csv (src/io.rs:143-169).
+inline void write_curve_dat(const std::string& path, const std::vector<double>& q,
+                            const std::vector<double>& intensity) {
+  detail::write_curve(path, q, intensity, "# q (1/A) intensity", ' ');
+}
+inline void write_curve_csv(const std::string& path, const std::vector<double>& q,
+                            const std::vector<double>& intensity) {
+  detail::write_curve(path, q, intensity, "q,intensity", ',');
+}
 
 // -------------------------------------------------------------------- fit
 // src/fit.rs:11-135.

@@ -1,5 +1,6 @@
 // The reference's own unit tests for the profile path (src/formfactor.rs,
-// src/scattering.rs, src/solvent.rs, src/spherical.rs, src/fit.rs), restated
+// src/scattering.rs, src/solvent.rs, src/spherical.rs, src/fit.rs,
+// src/structure.rs, src/io.rs, tests/end_to_end.rs), restated
 // over the C++ API of include/crabsaxs.hpp, plus drop-in checks of the
 // analysis and ensemble wrappers.  Each test names the reference test it
 // restates (file:line).  Tests marked GPU compute profiles and need a device.
@@ -72,6 +73,11 @@ Atom atom(double x, double y, double z, Element e, const std::string& name,
   a.residue_id = residue_id;
   return a;
 }
+
+// Fixture files of the reference's tests (examples/data), next to this suite
+// under tests/golden/structures.
+std::string g_data_dir = "tests/golden/structures/";
+std::string data_file(const char* name) { return g_data_dir + name; }
 
 double element_factor(Element e, double q) {  // coefficients_for(e).evaluate(q)
   return csx_form_factor(static_cast<int>(e), -1, 1.0, 0, q);
@@ -235,11 +241,7 @@ CSX_TEST(solvent_basis_recombines_to_direct_debye_curve, true) {
 }
 
 CSX_TEST(vacuum_minus_solvent_is_finite_for_structure_fixture, true) {
-  // examples/data/test_structure.pdb as Structure::from_pdb_file reads it
-  // (structure.rs:195-300): model 1, the altloc-A CA (highest occupancy,
-  // 0.80) and N; the HOH HETATM is dropped by default.
-  Structure s({atom(0.1, 0, 0, Element::C, "CA", "ALA"), atom(1.33, 0, 0, Element::N, "N", "ALA")});
-  s.atoms[0].occupancy = 0.8;
+  const Structure s = Structure::from_pdb_file(data_file("test_structure.pdb"));
   const std::vector<double> q = range_q(0, 100, 0.005);
   SolventParams p;
   p.c1 = 1.0;
@@ -543,6 +545,83 @@ CSX_TEST(coordinate_features_report_pair_distance, true) {
   CHECK(threw);
 }
 
+// ------------------------------------------------------ structure.rs:349-441
+
+CSX_TEST(parses_known_atom_count_and_altloc, false) {
+  const Structure s = Structure::from_pdb_file(data_file("test_structure.pdb"));
+  CHECK(s.atom_count() == 2);
+  for (const Atom& a : s.atoms)
+    if (a.atom_name == "CA") CHECK(std::fabs(a.occupancy - 0.8) < 1e-12);
+}
+
+CSX_TEST(selected_conformer_can_be_treated_as_full_occupancy, false) {
+  ParseOptions o;
+  o.occupancy_policy = OccupancyPolicy::SelectedConformerFull;
+  const Structure s = Structure::from_pdb_file_with_options(data_file("test_structure.pdb"), o);
+  for (const Atom& a : s.atoms)
+    if (a.atom_name == "CA") CHECK(a.occupancy == 1.0);
+}
+
+CSX_TEST(applies_hetatm_water_and_model_policies, false) {
+  const std::string path = data_file("test_structure.pdb");
+  ParseOptions het;
+  het.include_hetatm = true;
+  CHECK(Structure::from_pdb_file_with_options(path, het).atom_count() == 2);
+  ParseOptions water = het;
+  water.include_waters = true;
+  CHECK(Structure::from_pdb_file_with_options(path, water).atom_count() == 3);
+  ParseOptions second;
+  second.model_index = 1;
+  const Structure m = Structure::from_pdb_file_with_options(path, second);
+  CHECK(m.atom_count() == 1 && m.atoms[0].pos.x == 9.0);
+}
+
+CSX_TEST(parses_first_model_from_mmcif, false) {
+  const Structure s = Structure::from_pdb_file(data_file("test_structure.cif"));
+  CHECK(s.atom_count() == 2);
+  CHECK(s.atoms[0].chain_id == "A");
+}
+
+CSX_TEST(malformed_structure_returns_an_error, false) {
+  CHECK(throws<SaxsError>([] { Structure::from_pdb_file(data_file("malformed.pdb")); }));
+}
+
+// ------------------------------------------------------------ io.rs:171-212
+
+CSX_TEST(parses_validated_three_column_curve, false) {
+  const ExperimentalCurve c =
+      ExperimentalCurve::parse_dat("header\n# comment\n0.01 2.0 0.1\n0.02,1.5,0.2\n0.03 0.0 0.0\n");
+  CHECK((c.q == std::vector<double>{0.01, 0.02}));
+  CHECK((c.sigma == std::vector<double>{0.1, 0.2}));
+}
+
+CSX_TEST(accepts_two_column_data_and_rejects_nonpositive_sigma_and_unsorted_q, false) {
+  const ExperimentalCurve c = ExperimentalCurve::parse_dat("0.01 2.0\n0.02 1.0\n");
+  CHECK(!c.has_errors);
+  CHECK((c.sigma == std::vector<double>{1.0, 1.0}));
+  for (const char* bad : {"0.01 2.0 0\n", "0.02 2.0 0.1\n0.01 1.0 0.1\n", "-0.01 2.0 0.1\n"})
+    CHECK(throws<ParseError>([&] { ExperimentalCurve::parse_dat(bad); }));
+}
+
+CSX_TEST(normalizes_nm_inverse_angstrom_header, false) {
+  const ExperimentalCurve c =
+      ExperimentalCurve::parse_dat("# q (1/nm) I sigma\n1.0 2.0 0.1\n2.0 1.0 0.1\n");
+  CHECK((c.q == std::vector<double>{0.1, 0.2}));
+  CHECK(std::string(c.q_unit()) == "angstrom^-1");
+  const ExperimentalCurve f = ExperimentalCurve::from_dat_file(data_file("test_experimental.dat"));
+  CHECK(f.len() == 5 && f.has_errors);
+}
+
+// ------------------------------------------------------ tests/end_to_end.rs:3-10
+
+CSX_TEST(compute_curve_for_example_structure_is_well_formed, true) {
+  const Structure s = Structure::from_pdb_file(data_file("small.pdb"));
+  const std::vector<double> q = range_q(0, 99, 0.005);
+  const std::vector<double> curve = debye_intensity(s, q);
+  CHECK(curve.size() == q.size());
+  for (double v : curve) CHECK(std::isfinite(v) && v >= 0.0);
+}
+
 CSX_TEST(ensemble_fitter_recovers_a_mixture, true) {
   std::vector<Structure> conformers;
   for (int k = 0; k < 4; ++k) {
@@ -588,6 +667,9 @@ CSX_TEST(ensemble_fitter_recovers_a_mixture, true) {
 
 int main(int argc, char** argv) {
   const std::string mode = argc > 1 ? argv[1] : "--all";
+  const std::string self = argv[0];
+  const size_t slash = self.rfind('/');
+  if (slash != std::string::npos) g_data_dir = self.substr(0, slash) + "/../golden/structures/";
   int run = 0, failed = 0;
   for (const TestCase& t : registry()) {
     if (mode == "--list") {
