@@ -515,6 +515,10 @@ csx_status csx_ctx_create(int device, csx_ctx** out) {
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess &&
       cus > 0)
     ctx->cu_count = cus;
+  int lds = 0;
+  if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess &&
+      lds > 0)
+    ctx->lds_per_workgroup = static_cast<size_t>(lds);
   *out = ctx;
   return CSX_OK;
 }

@@ -46,6 +46,7 @@ struct csx_ctx {
   std::map<std::string, std::pair<int64_t, double>> stats;
   csx::Tuning tuning;
   int cu_count = 256;
+  size_t lds_per_workgroup = 64 * 1024;  // the device's per-workgroup LDS limit
   int last_launch[3] = {0, 0, 0};  // q tile, sin mode, persistent workgroups
 };
 

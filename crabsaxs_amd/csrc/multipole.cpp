@@ -1,6 +1,7 @@
 // Multipole path (spherical.rs:266-565): geometry cache, per-q expansion
 // orders, amplitude tables and the C-ABI entry points.
 #include <algorithm>
+#include <numeric>
 #include <cmath>
 
 #include "capi_internal.hpp"
@@ -74,17 +75,39 @@ static csx_status get_geometry(Call& call, csx_structure* s, PointSet& ps, int L
   tile_first[L + 1] = static_cast<int>(tile_l.size());
   g->ntiles = static_cast<int>(tile_l.size());
   g->h_tile_l = tile_l;
+  // Atoms, then hydration sites (the vacuum path reads the atoms only), each
+  // segment ordered by distance from the centre: the points of a slab then
+  // have nearly equal radii, so the lanes generating j_l(q r) for them run
+  // the Bessel recurrences with nearly equal trip counts.  The sums over
+  // points do not depend on the order.
+  auto centre_r2 = [&](const double* p) {
+    double r2 = 0.0;
+    for (int a = 0; a < 3; ++a) r2 += (p[a] - s->centre[a]) * (p[a] - s->centre[a]);
+    return r2;
+  };
+  auto by_radius = [&](const double* pts, int count) {
+    std::vector<double> r2(count);
+    for (int i = 0; i < count; ++i) r2[i] = centre_r2(pts + 3 * i);
+    std::vector<int> order(count);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return r2[a] < r2[b]; });
+    return order;
+  };
+  const std::vector<int> atom_order = by_radius(s->xyz.data(), n);
+  const std::vector<int> site_order = by_radius(ps.site_xyz.data(), ns);
   std::vector<double> xyz(3 * static_cast<size_t>(g->npts));
   std::vector<int> type(g->npts);
   std::vector<float> occ(g->npts);
-  for (int i = 0; i < n; ++i) {
-    for (int a = 0; a < 3; ++a) xyz[3 * i + a] = s->xyz[3 * i + a];
-    type[i] = s->type_of[i];
-    occ[i] = static_cast<float>(s->occ[i]);
+  for (int k = 0; k < n; ++k) {
+    const int i = atom_order[k];
+    for (int a = 0; a < 3; ++a) xyz[3 * k + a] = s->xyz[3 * i + a];
+    type[k] = s->type_of[i];
+    occ[k] = static_cast<float>(s->occ[i]);
   }
   const int T = static_cast<int>(s->types.size());
   for (int k = 0; k < ns; ++k) {
-    for (int a = 0; a < 3; ++a) xyz[3 * (n + k) + a] = ps.site_xyz[3 * k + a];
+    const int i = site_order[k];
+    for (int a = 0; a < 3; ++a) xyz[3 * (n + k) + a] = ps.site_xyz[3 * i + a];
     type[n + k] = T;
     occ[n + k] = 1.0f;
   }
@@ -121,8 +144,8 @@ static csx_status get_geometry(Call& call, csx_structure* s, PointSet& ps, int L
   CSX_HIP(g->tile_l.upload(tile_l.data(), tile_l.size() * sizeof(int)), "upload tiles");
   for (int nch : {1, 3})
     if (multipole_coeff_lds_bytes(max_group_l_range(tile_l, multipole_tiles_per_group(nch)),
-                                  nch) > 64 * 1024)
-      return fail(CSX_INVALID_INPUT, "multipole l_max too large for the LDS layout");
+                                  nch) > s->ctx->lds_per_workgroup)
+      return fail(CSX_INVALID_INPUT, "multipole l_max too large for the device's LDS");
   const size_t ybytes = static_cast<size_t>(g->ntiles) * g->npad * 16 * sizeof(float);
   CSX_HIP(g->Y.alloc(ybytes), "hipMalloc harmonics");
   CSX_HIP(launch_multipole_geometry(d_xyz.as<double>(), g->npts, d_centre.as<double>(), L,

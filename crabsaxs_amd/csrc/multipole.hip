@@ -46,6 +46,14 @@ constexpr int kChunk = 512;  // points per workgroup
 // 16 k + n holds the values of points 4 ks + k, ks = 0..3, contiguously.
 __device__ inline int slab_offset(int p, int n) { return ((p & 3) * 16 + n) * 4 + (p >> 2); }
 
+// j_l operands of one slab in LDS: X[l][k][q][ks] = j_l(q r) for point
+// 4 ks + k, rows of 16 q x 4 ks padded to kRow floats.  kRow = 72 (8 mod 32
+// banks) puts the stores of the first 32 lanes of a generating wave
+// (4 k x 2 q x 4 ks) on distinct banks; rows stay 16-byte aligned for the
+// MFMA operand reads.
+constexpr int kRow = 72;
+constexpr int kXL = 4 * kRow;  // floats per l
+
 // ------------------------------------------------------------ geometry --
 __global__ void multipole_geometry(const double* __restrict__ xyz, int npts,
                                    const double* __restrict__ centre3, int l_max,
@@ -109,14 +117,18 @@ __global__ void fill_zero(float* __restrict__ p, size_t n) {
   if (i < n) p[i] = 0.f;
 }
 
-// One (point, q) column of spherical Bessel values: X[(l - l_lo) * 256] =
+// One (point, q) column of spherical Bessel values: X[(l - l_lo) * kXL] =
 // j_l(x) for l in [l_lo, min(lq, l_hi)] and 0 for the rest of [l_lo, l_hi].
 // l <= floor(x): upward recurrence from j_0, j_1 (stable there).  Above:
 // Miller's downward recurrence f_{l-1} = (2l+1)/x f_l - f_{l+1} started at
-// top + 14 (f = 1e-250, 0), normalised by f at floor(x) against the upward
-// value; pass 1 finds the normalisation, pass 2 repeats the recurrence and
-// stores.  Values computed before an overflow rescale are below 1e-250 of
-// the normalising value and stored as 0.
+// S = top + 14 (f = 1e-250, 0) and normalised by f at floor(x) against the
+// upward value.  Pass 1 runs it down to floor(x) for the normalisation and
+// keeps the state at `top`; pass 2 resumes from that state and stores.
+// Where the growth bound ((2S+1)/x)^(S - floor(x)) stays below 2^1800 the
+// recurrence cannot leave the double range from 1e-250 and runs without
+// overflow tests.  Otherwise f is rescaled by 1e-250 on overflow and values
+// computed before the last rescale (below 1e-250 of the normalising value)
+// are stored as 0.
 __device__ inline void bessel_column(float xf, int lq, int l_lo, int l_hi, float* X) {
   const int top = min(lq, l_hi);
   int l_zero = l_lo;  // first l to clear
@@ -135,55 +147,85 @@ __device__ inline void bessel_column(float xf, int lq, int l_lo, int l_hi, float
       if (l_lo == 0) X[0] = float(j0);
       double jlu = j0;
       if (lu >= 1) {
-        if (l_lo <= 1) X[(1 - l_lo) * 256] = float(j1);
+        if (l_lo <= 1) X[(1 - l_lo) * kXL] = float(j1);
         double jm = j0, jc = j1, c = 3.0 * ix;  // c = (2l + 1) / x at l = 1
         for (int l = 1; l < lu; ++l) {
           const double jn = c * jc - jm;
           jm = jc;
           jc = jn;
           c += 2.0 * ix;
-          if (l + 1 >= l_lo) X[(l + 1 - l_lo) * 256] = float(jn);
+          if (l + 1 >= l_lo) X[(l + 1 - l_lo) * kXL] = float(jn);
         }
         jlu = jc;
       }
       if (top > lu) {
         const int S = top + 14;
-        // pass 1: normalisation and the last rescale
-        double fp = 0.0, f = 1e-250, c = double(2 * S + 1) * ix;
-        int last = S + 1;
-        for (int l = S; l > lu; --l) {
-          const double fn = c * f - fp;
-          fp = f;
-          f = fn;
-          c -= 2.0 * ix;
-          if (fabs(f) > 1e250) {
-            f *= 1e-250;
-            fp *= 1e-250;
-            last = l - 1;
-          }
-        }
-        const double norm = jlu / f;
-        // pass 2: the same recurrence, storing l in [max(lu + 1, l_lo), top]
-        fp = 0.0;
-        f = 1e-250;
-        c = double(2 * S + 1) * ix;
         const int lo = max(lu + 1, l_lo);
-        for (int l = S; l >= lo; --l) {
-          if (l <= top) X[(l - l_lo) * 256] = l <= last ? float(norm * f) : 0.f;
-          const double fn = c * f - fp;
-          fp = f;
-          f = fn;
-          c -= 2.0 * ix;
-          if (fabs(f) > 1e250) {
-            f *= 1e-250;
-            fp *= 1e-250;
+        const bool safe = float(S - lu) * __log2f(float(2 * S + 1) / xf) < 1800.f;
+        if (safe) {
+          // state at index l: f = f_l, fp = f_{l+1}, c = (2l + 1) / x
+          double fp = 0.0, f = 1e-250, c = double(2 * S + 1) * ix;
+          for (int l = S; l > top; --l) {
+            const double fn = c * f - fp;
+            fp = f;
+            f = fn;
+            c -= 2.0 * ix;
+          }
+          const double f_top = f, fp_top = fp, c_top = c;
+          for (int l = top; l > lu; --l) {
+            const double fn = c * f - fp;
+            fp = f;
+            f = fn;
+            c -= 2.0 * ix;
+          }
+          const double norm = jlu / f;
+          f = f_top;
+          fp = fp_top;
+          c = c_top;
+          for (int l = top; l >= lo; --l) {
+            X[(l - l_lo) * kXL] = float(norm * f);
+            const double fn = c * f - fp;
+            fp = f;
+            f = fn;
+            c -= 2.0 * ix;
+          }
+        } else {
+          // pass 1: normalisation and the last rescale
+          double fp = 0.0, f = 1e-250, c = double(2 * S + 1) * ix;
+          int last = S + 1;
+          for (int l = S; l > lu; --l) {
+            const double fn = c * f - fp;
+            fp = f;
+            f = fn;
+            c -= 2.0 * ix;
+            if (fabs(f) > 1e250) {
+              f *= 1e-250;
+              fp *= 1e-250;
+              last = l - 1;
+            }
+          }
+          const double norm = jlu / f;
+          // pass 2: the same recurrence, storing l in [lo, top]
+          fp = 0.0;
+          f = 1e-250;
+          c = double(2 * S + 1) * ix;
+          for (int l = S; l >= lo; --l) {
+            if (l <= top) X[(l - l_lo) * kXL] = l <= last ? float(norm * f) : 0.f;
+            const double fn = c * f - fp;
+            fp = f;
+            f = fn;
+            c -= 2.0 * ix;
+            if (fabs(f) > 1e250) {
+              f *= 1e-250;
+              fp *= 1e-250;
+            }
           }
         }
       }
       l_zero = top + 1;
     }
   }
-  for (int l = max(l_zero, l_lo); l <= l_hi; ++l) X[(l - l_lo) * 256] = 0.f;
+  for (int l = max(l_zero, l_lo); l <= l_hi; ++l) X[(l - l_lo) * kXL] = 0.f;
 }
 
 // partial[((chunk * NCH + c) * nqp + q) * ncol + col]
@@ -219,8 +261,8 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
   if (l_lo > Lb) return;  // every tile of the group is above this block's orders
   const int l_hi = min(tile_l[g1 - 1], Lb);
   const int lr = l_hi - l_lo + 1;
-  float* X = smem;              // [lr][4 k][16 q][4 ks]
-  float* AMP = smem + lr * 256; // [16 p][16 q][NCH]
+  float* X = smem;              // [lr][4 k][16 q][4 ks], rows of kRow
+  float* AMP = smem + lr * kXL; // [16 p][16 q][NCH]
 
   v4f acc[MAXT][NCH];
 #pragma unroll
@@ -228,8 +270,10 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
 #pragma unroll
     for (int c = 0; c < NCH; ++c) acc[t][c] = v4f{0.f, 0.f, 0.f, 0.f};
 
-  // generation role: point 4 ks + wave of the slab, q = lane / 4, ks = lane % 4
-  const int gq = lane >> 2, gp = 4 * (lane & 3) + wave;
+  // generation role: wave w takes q = 4 w + lane / 16 and point lane % 16 of
+  // the slab.  Points are sorted by radius, so the lanes of a wave see
+  // nearly equal q r and run the recurrences with little divergence.
+  const int gq = 4 * wave + (lane >> 4), gp = lane & 15;
   const float qg = qv[q0 + gq];
   const int lg = qlmax[q0 + gq];
   // MFMA role: lane = 16 k + n
@@ -241,7 +285,8 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
     __syncthreads();  // the previous slab's operands have been read
     const int i = s * kSlab + gp;
     const bool vi = i < npts;
-    bessel_column(vi ? qg * radius[i] : 0.f, vi ? lg : -1, l_lo, l_hi, X + wave * 64 + lane);
+    bessel_column(vi ? qg * radius[i] : 0.f, vi ? lg : -1, l_lo, l_hi,
+                  X + (gp & 3) * kRow + gq * 4 + (gp >> 2));
     {
       const float o = vi ? occ[i] : 0.f;
       const int ty = vi ? type[i] : 0;
@@ -261,7 +306,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
       if (tile >= g1) continue;
       const int l = tile_l[tile];
       if (l > l_hi) continue;
-      const v4f a = *reinterpret_cast<const v4f*>(X + ((l - l_lo) * 4 + mk) * 64 + mn * 4);
+      const v4f a = *reinterpret_cast<const v4f*>(X + ((l - l_lo) * 4 + mk) * kRow + mn * 4);
       const v4f y = *reinterpret_cast<const v4f*>(Y + (size_t(tile) * nslab + s) * 256 + lane * 4);
 #pragma unroll
       for (int c = 0; c < NCH; ++c)
@@ -357,7 +402,7 @@ int multipole_chunk() { return kChunk; }
 int multipole_tiles_per_group(int nch) { return 4 * (nch == 1 ? kMaxT1 : kMaxT3); }
 
 size_t multipole_coeff_lds_bytes(int l_range, int nch) {
-  return size_t(l_range) * 256 * sizeof(float) + size_t(kSlab) * kQB * nch * sizeof(float);
+  return size_t(l_range) * kXL * sizeof(float) + size_t(kSlab) * kQB * nch * sizeof(float);
 }
 
 hipError_t launch_multipole_geometry(const double* xyz, int npts, const double* centre,
