@@ -64,6 +64,9 @@ def parse():
     ap.add_argument("--conformers", type=int, default=512)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run a device-to-device copy of BYTES (HBM counter calibration)")
+    ap.add_argument("--multipole-profile", type=int, default=0, metavar="L",
+                    help="only run the multipole workload at l_max L (1 + 3 launches), "
+                         "for counter collection")
     a = ap.parse_args()
     a.legs = set() if a.legs in ("", "none") else set(a.legs.split(","))
     if a.no_cpu:
@@ -86,6 +89,27 @@ def copy_calibration(nbytes: int) -> None:
     dst.copy_(src)
     torch.cuda.synchronize()
     print(json.dumps({"copy_calibration_bytes": nbytes, "read": nbytes, "written": nbytes}))
+
+
+def multipole_profile(L: int) -> None:
+    """The multipole workload's vacuum intensity at one l_max, 1 warm-up and 3
+    timed launches and nothing else, so that counters of a profiler run
+    belong to that order only."""
+    import crabsaxs_amd as csx
+    from crabsaxs_amd import workloads
+    ctx = csx.default_context()
+    s, q, _ = workloads.prolate_100k()
+    s.handle(ctx)
+    mp = csx.MultipoleParams(l_max=L)
+    csx.multipole_intensity(s, q, mp, ctx=ctx)
+    ctx.reset_timing()
+    ctx.set_timing(True)
+    for _ in range(3):
+        csx.multipole_intensity(s, q, mp, ctx=ctx)
+    ctx.set_timing(False)
+    n, ms = ctx.kernel_time("multipole_coeff")
+    print(json.dumps({"multipole_profile_l_max": L, "points": len(s), "q": int(q.size),
+                      "multipole_coeff_ms": ms / n}))
 
 
 def traffic_per_launch(kernel: str):
@@ -302,6 +326,9 @@ def main():
     args = parse()
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
+        return
+    if args.multipole_profile:
+        multipole_profile(args.multipole_profile)
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not "

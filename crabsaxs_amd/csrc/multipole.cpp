@@ -17,8 +17,8 @@ hipError_t launch_multipole_geometry(const double* xyz, int npts, const double* 
 hipError_t launch_multipole_coeff(const float* radius, const int* type, const float* occ, int npts,
                                   int npad, const float* Y, int ntiles, const int* tile_l,
                                   int max_l_range, const float* amp, const float* qv,
-                                  const int* qlmax, int nqp, int nch, float* partial, double* out,
-                                  hipStream_t stream, KernelTimer* timer);
+                                  const int* qlmax, int nqp, int nch, float* partial, double* coef,
+                                  double* out, hipStream_t stream, KernelTimer* timer);
 
 namespace {
 
@@ -217,13 +217,15 @@ static csx_status multipole_run(csx_structure* s, const double* q, size_t nq, in
   const int npts = mode == 0 ? ps->natoms : g->npts;
   const int nchunk = (g->npad + multipole_chunk() - 1) / multipole_chunk();
   const hipStream_t st = call.stream();
-  DevBuf d_amp, d_q, d_l, d_partial, d_out;
+  DevBuf d_amp, d_q, d_l, d_partial, d_coef, d_out;
   CSX_HIP(d_amp.alloc(amp.size() * sizeof(float), st), "hipMallocAsync");
   CSX_HIP(d_q.alloc(nqp * sizeof(float), st), "hipMallocAsync");
   CSX_HIP(d_l.alloc(nqp * sizeof(int), st), "hipMallocAsync");
   CSX_HIP(d_partial.alloc(static_cast<size_t>(nchunk) * nch * nqp * g->ntiles * 16 * sizeof(float), st),
           "hipMallocAsync");
   CSX_HIP(d_out.alloc(nqp * nterms * sizeof(double), st), "hipMallocAsync");
+  CSX_HIP(d_coef.alloc(static_cast<size_t>(nch) * nqp * g->ntiles * 16 * sizeof(double), st),
+          "hipMallocAsync");
   CSX_HIP(hipMemcpyAsync(d_amp.as<void>(), amp.data(), amp.size() * sizeof(float),
                          hipMemcpyHostToDevice, st), "hipMemcpyAsync");
   CSX_HIP(hipMemcpyAsync(d_q.as<void>(), qv.data(), nqp * sizeof(float), hipMemcpyHostToDevice, st),
@@ -235,7 +237,7 @@ static csx_status multipole_run(csx_structure* s, const double* q, size_t nq, in
                                  g->npad, g->Y.as<float>(), g->ntiles, g->tile_l.as<int>(),
                                  l_range, d_amp.as<float>(), d_q.as<float>(), d_l.as<int>(),
                                  static_cast<int>(nqp), nch, d_partial.as<float>(),
-                                 d_out.as<double>(), st, call.timer()),
+                                 d_coef.as<double>(), d_out.as<double>(), st, call.timer()),
           "multipole_coeff");
   std::vector<double> res(nqp * nterms);
   CSX_HIP(hipMemcpyAsync(res.data(), d_out.as<void>(), res.size() * sizeof(double),
@@ -244,6 +246,7 @@ static csx_status multipole_run(csx_structure* s, const double* q, size_t nq, in
   d_q.reset();
   d_l.reset();
   d_partial.reset();
+  d_coef.reset();
   d_out.reset();
   CSX_TRY(call.finish());
   const double scale = 4.0 * kPi;

@@ -46,13 +46,14 @@ constexpr int kChunk = 512;  // points per workgroup
 // 16 k + n holds the values of points 4 ks + k, ks = 0..3, contiguously.
 __device__ inline int slab_offset(int p, int n) { return ((p & 3) * 16 + n) * 4 + (p >> 2); }
 
-// j_l operands of one slab in LDS: X[l][k][q][ks] = j_l(q r) for point
-// 4 ks + k, rows of 16 q x 4 ks padded to kRow floats.  kRow = 72 (8 mod 32
-// banks) puts the stores of the first 32 lanes of a generating wave
-// (4 k x 2 q x 4 ks) on distinct banks; rows stay 16-byte aligned for the
-// MFMA operand reads.
-constexpr int kRow = 72;
-constexpr int kXL = 4 * kRow;  // floats per l
+// j_l operands of one slab in LDS: 256 floats per l, q-major.  The value for
+// (q, point 4 ks + k) sits at q * 16 + 4 xslot(q, k) + ks, so the MFMA lane
+// (k, n = q) reads its four k-step values with one 16-byte load.  The slot
+// is k XOR two bits of q: the 16-byte reads of 16 consecutive lanes (64
+// banks) or of 8 (32 banks) then fall on distinct banks, and a generating
+// wave (4 q x 16 points) stores a permutation of 64 consecutive words.
+constexpr int kXL = 256;  // floats per l
+__device__ inline int xslot(int q, int k) { return k ^ (((q >> 1) ^ (q >> 3)) & 3); }
 
 // ------------------------------------------------------------ geometry --
 __global__ void multipole_geometry(const double* __restrict__ xyz, int npts,
@@ -261,7 +262,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
   if (l_lo > Lb) return;  // every tile of the group is above this block's orders
   const int l_hi = min(tile_l[g1 - 1], Lb);
   const int lr = l_hi - l_lo + 1;
-  float* X = smem;              // [lr][4 k][16 q][4 ks], rows of kRow
+  float* X = smem;              // [lr][16 q][4 slots][4 ks]
   float* AMP = smem + lr * kXL; // [16 p][16 q][NCH]
 
   v4f acc[MAXT][NCH];
@@ -286,7 +287,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
     const int i = s * kSlab + gp;
     const bool vi = i < npts;
     bessel_column(vi ? qg * radius[i] : 0.f, vi ? lg : -1, l_lo, l_hi,
-                  X + (gp & 3) * kRow + gq * 4 + (gp >> 2));
+                  X + gq * 16 + 4 * xslot(gq, gp & 3) + (gp >> 2));
     {
       const float o = vi ? occ[i] : 0.f;
       const int ty = vi ? type[i] : 0;
@@ -306,7 +307,8 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
       if (tile >= g1) continue;
       const int l = tile_l[tile];
       if (l > l_hi) continue;
-      const v4f a = *reinterpret_cast<const v4f*>(X + ((l - l_lo) * 4 + mk) * kRow + mn * 4);
+      const v4f a =
+          *reinterpret_cast<const v4f*>(X + (l - l_lo) * kXL + mn * 16 + 4 * xslot(mn, mk));
       const v4f y = *reinterpret_cast<const v4f*>(Y + (size_t(tile) * nslab + s) * 256 + lane * 4);
 #pragma unroll
       for (int c = 0; c < NCH; ++c)
@@ -331,14 +333,36 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
   }
 }
 
-// out[q * NT + k]: NCH = 1 -> sum C^2; NCH = 3 -> vv, ee, hh, ve, vh, eh.
-// Only the columns with l <= qlmax[q] are read (the others were not written
-// for this q's block, or hold zero coefficients).
+// coef[(c * nqp + q) * ncol + col] = FP64 sum over chunks, in chunk order,
+// of the partial coefficients; 0 for the columns with l > qlmax[q] (not
+// written for this q's block).  One thread per (q, column).
 template <int NCH>
-__global__ __launch_bounds__(256) void multipole_reduce(const float* __restrict__ partial,
+__global__ __launch_bounds__(256) void multipole_sum_chunks(const float* __restrict__ partial,
+                                                            const int* __restrict__ tile_l,
+                                                            const int* __restrict__ qlmax,
+                                                            int nchunk, int nqp, int ncol,
+                                                            double* __restrict__ coef) {
+  const int col = blockIdx.x * blockDim.x + threadIdx.x;
+  const int q = blockIdx.y;
+  if (col >= ncol) return;
+  const bool live = tile_l[col / 16] <= qlmax[q];
+#pragma unroll
+  for (int c = 0; c < NCH; ++c) {
+    double v = 0.0;
+    if (live)
+      for (int ch = 0; ch < nchunk; ++ch)
+        v += double(partial[((size_t(ch) * NCH + c) * nqp + q) * ncol + col]);
+    coef[(size_t(c) * nqp + q) * ncol + col] = v;
+  }
+}
+
+// out[q * NT + k]: NCH = 1 -> sum C^2; NCH = 3 -> vv, ee, hh, ve, vh, eh,
+// over the columns with l <= qlmax[q].
+template <int NCH>
+__global__ __launch_bounds__(256) void multipole_reduce(const double* __restrict__ coef,
                                                         const int* __restrict__ tile_l,
                                                         const int* __restrict__ qlmax,
-                                                        int nchunk, int nqp, int ncol,
+                                                        int nqp, int ncol,
                                                         double* __restrict__ out) {
   constexpr int NT = NCH == 1 ? 1 : 6;
   __shared__ double scratch[4 * NT];
@@ -351,12 +375,7 @@ __global__ __launch_bounds__(256) void multipole_reduce(const float* __restrict_
     if (tile_l[col / 16] > lq) continue;
     double cc[NCH];
 #pragma unroll
-    for (int c = 0; c < NCH; ++c) {
-      double v = 0.0;
-      for (int ch = 0; ch < nchunk; ++ch)
-        v += double(partial[((size_t(ch) * NCH + c) * nqp + q) * ncol + col]);
-      cc[c] = v;
-    }
+    for (int c = 0; c < NCH; ++c) cc[c] = coef[(size_t(c) * nqp + q) * ncol + col];
     if (NCH == 1) {
       s[0] += cc[0] * cc[0];
     } else {
@@ -387,11 +406,12 @@ __global__ __launch_bounds__(256) void multipole_reduce(const float* __restrict_
           ((scratch[k] + scratch[NT + k]) + scratch[2 * NT + k]) + scratch[3 * NT + k];
 }
 
-// Accumulator tiles per wave (4 x MAXT tiles per workgroup group), from a
-// sweep at 108k points x 200 q: one channel 16 (L = 15/25/40: 0.78/1.27/2.46
-// ms against 1.16/1.88/3.10 at 32), three channels 8 (L = 25: 2.59 ms
-// against 4.02 at 12).  Fewer accumulators raise occupancy, which hides the
-// harmonic-tile loads, at the cost of generating j_l once per tile group.
+// Accumulator tiles per wave (4 x MAXT tiles per tile group), from a sweep
+// at 108k points x 200 q: one channel 16 (L = 15/25/40: 0.59/0.98/2.02 ms;
+// 8: 0.55/1.16/2.20; 12: 0.57/1.61/2.35), three channels 8 (L = 25: 2.17
+// ms; 6: 2.32; 4: 2.14).  Fewer accumulators raise occupancy, which hides
+// the harmonic-tile loads, at the cost of generating j_l once per tile
+// group.
 constexpr int kMaxT1 = 16;
 constexpr int kMaxT3 = 8;
 
@@ -422,7 +442,7 @@ hipError_t launch_multipole_coeff(const float* radius, const int* type, const fl
                                   int npts, int npad, const float* Y, int ntiles,
                                   const int* tile_l, int max_l_range, const float* amp,
                                   const float* qv, const int* qlmax, int nqp, int nch,
-                                  float* partial, double* out, hipStream_t stream,
+                                  float* partial, double* coef, double* out, hipStream_t stream,
                                   KernelTimer* timer) {
   if (nqp % kQB != 0 || npad % kSlab != 0 || ntiles <= 0 || (nch != 1 && nch != 3))
     return hipErrorInvalidValue;
@@ -444,10 +464,16 @@ hipError_t launch_multipole_coeff(const float* radius, const int* type, const fl
           partial);
   }
   const int ncol = ntiles * 16;
-  if (nch == 1)
-    multipole_reduce<1><<<nqp, 256, 0, stream>>>(partial, tile_l, qlmax, nchunk, nqp, ncol, out);
-  else
-    multipole_reduce<3><<<nqp, 256, 0, stream>>>(partial, tile_l, qlmax, nchunk, nqp, ncol, out);
+  const dim3 sum_grid((ncol + 255) / 256, nqp);
+  if (nch == 1) {
+    multipole_sum_chunks<1><<<sum_grid, 256, 0, stream>>>(partial, tile_l, qlmax, nchunk, nqp,
+                                                          ncol, coef);
+    multipole_reduce<1><<<nqp, 256, 0, stream>>>(coef, tile_l, qlmax, nqp, ncol, out);
+  } else {
+    multipole_sum_chunks<3><<<sum_grid, 256, 0, stream>>>(partial, tile_l, qlmax, nchunk, nqp,
+                                                          ncol, coef);
+    multipole_reduce<3><<<nqp, 256, 0, stream>>>(coef, tile_l, qlmax, nqp, ncol, out);
+  }
   return hipGetLastError();
 }
 
