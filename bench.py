@@ -26,6 +26,9 @@ the metric:
               checked against a central difference of FP64 GPU Debye; the
               Nelder-Mead and gradient fits of a noisy target (scale 2.3,
               bg 1e-3, c1 1.02, c2 2.0) timed to their chi2
+  structure   the hydration-grid builders (exterior at 1 A, adaptive at 1 A
+              and 4 A) on the 20k and 100k structures, and the coordinate
+              P(r) pair histogram of the 20k structure
 """
 from __future__ import annotations
 
@@ -46,7 +49,7 @@ VALU_OPS_PER_PAIRQ = 4          # x = q r, sin x, * 1/(q r), accumulate
 PEAK_VALU_OPS = 256 * 64 * 2.4e9  # CUs x lanes x clock (unpacked FP32 issue)
 PEAK_FP32_MFMA = 157.3e12
 TRAFFIC_FILE = ROOT / "profiles" / "hbm_traffic.json"
-LEGS = ("fp64", "cpu", "ensemble", "multipole", "adjoint")
+LEGS = ("fp64", "cpu", "ensemble", "multipole", "adjoint", "structure")
 
 
 def parse():
@@ -277,6 +280,48 @@ def multipole_leg(csx, ctx, log):
     return out
 
 
+def structure_leg(csx, ctx, log):
+    """Hydration grids (built on the GPU; each timing on a fresh structure
+    handle, since grids are kept with the structure) and the coordinate P(r)
+    pair histogram."""
+    from crabsaxs_amd import workloads
+    s20, _ = workloads.precision_20k()
+    s100, _, nsites = workloads.prolate_100k()
+
+    def fresh(s):
+        t = csx.Structure.from_arrays(s.xyz, s.element, s.occupancy, s.group)
+        t.handle(ctx)
+        return t
+
+    builds = (("exterior_1A", lambda t: csx.build_hydration_grid(t, 1.0, ctx=ctx)),
+              ("adaptive_1A", lambda t: csx.build_adaptive_hydration_grid(t, 1.0, ctx=ctx)),
+              ("adaptive_4A", lambda t: csx.build_adaptive_hydration_grid(t, 4.0, ctx=ctx)))
+    grids = {}
+    for name, s in (("20k", s20), (f"100k+{nsites}", s100)):
+        for kind, build in builds:
+            build(fresh(s))  # warm-up
+            best, sites = float("inf"), 0
+            for _ in range(3):
+                t = fresh(s)
+                t0 = time.perf_counter()
+                g = build(t)
+                best = min(best, time.perf_counter() - t0)
+                sites = len(g.sites)
+            grids[f"{name}_{kind}"] = {"ms": best * 1e3, "sites": sites}
+            log(f"structure: {name} {kind} grid {best * 1e3:.2f} ms, {sites} sites")
+    t = fresh(s20)
+    n = len(t)
+    csx.coordinate_features(t, 160, ctx=ctx)
+    f, dt, kms, _ = timed(ctx, "pr_hist", lambda: csx.coordinate_features(t, 160, ctx=ctx), 3)
+    pairs = n * (n - 1) / 2
+    log(f"structure: P(r) of {n} atoms {dt * 1e3:.2f} ms (histogram kernel {kms:.3f} ms)")
+    return {"workload": "20,000-atom precision-study globule and the 100,000-atom prolate "
+                        "structure with its hydration pseudo-atoms",
+            "grids_wall": grids,
+            "p_r": {"atoms": n, "bins": 160, "wall_ms": dt * 1e3, "hist_kernel_ms": kms,
+                    "pairs_per_s": pairs / (kms / 1e3), "dmax": f.dmax, "rg": f.rg}}
+
+
 def adjoint_leg(csx, ctx, log):
     from crabsaxs_amd import workloads
     F64 = csx.Precision.FP64_PLAIN
@@ -417,6 +462,8 @@ def main():
         result["multipole"] = multipole_leg(csx, ctx, log)
     if "adjoint" in args.legs:
         result["adjoint"] = adjoint_leg(csx, ctx, log)
+    if "structure" in args.legs:
+        result["structure"] = structure_leg(csx, ctx, log)
     print(json.dumps(result), flush=True)
 
 

@@ -11,9 +11,11 @@
 // The exterior flood fill of solvent.rs:341-375 (breadth-first from the
 // boundary through unknown voxels, six-connected) marks exactly the unknown
 // voxels whose connected component touches the grid boundary.  Here that set
-// comes from a union-find labelling of the unknown voxels (one pass of unions
-// over +x/+y/+z neighbours, lock-free with atomicMin on parent labels, then
-// one flattening pass) and a flag per root.
+// comes from a union-find labelling of the unknown voxels: every x-run of
+// unknown voxels starts as one tree under its first voxel (one pass per row),
+// runs are joined along y and z by lock-free unions (atomicMin on parent
+// labels, path halving in every find), a flattening pass points every voxel
+// at its root, and the roots of boundary voxels are flagged.
 //
 // Sites are emitted in linear voxel order (z, y, x), the reference's scan
 // order, by a three-kernel compaction: counts per 4096-voxel block, one
@@ -94,14 +96,19 @@ inline __device__ int uf_load(const int* L, int a) {
   return __hip_atomic_load(L + a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Root of a's tree.  Labels only decrease (L[a] <= a), so every chain ends.
-inline __device__ int uf_find(const int* L, int a) {
-  int p = uf_load(L, a);
-  while (p != a) {
-    a = p;
-    p = uf_load(L, a);
+// Root of a's tree, halving the path on the way: a is re-pointed at its
+// grandparent with atomicMin, which keeps every label pointing into the same
+// set and never undoes a smaller link made concurrently.  Labels only
+// decrease (L[a] <= a), so every chain ends.
+inline __device__ int uf_find(int* L, int a) {
+  for (;;) {
+    const int p = uf_load(L, a);
+    if (p == a) return a;
+    const int g = uf_load(L, p);
+    if (g == p) return p;
+    atomicMin(L + a, g);
+    a = g;
   }
-  return a;
 }
 
 // Join the trees of a and b: the larger root is linked below the smaller.
@@ -125,19 +132,38 @@ inline __device__ void uf_unite(int* L, int a, int b) {
 
 inline __device__ int thread_voxel() { return blockIdx.x * blockDim.x + threadIdx.x; }
 
-__global__ void cc_init(const uint8_t* __restrict__ state, int count, int* __restrict__ L) {
-  const int c = thread_voxel();
-  if (c < count) L[c] = state[c] == 0 ? c : -1;
+// One thread per x-row: L[c] = first voxel of c's run of unknown voxels, -1
+// for the other voxels.
+__global__ void cc_rows(const uint8_t* __restrict__ state, GridGeom g, int* __restrict__ L) {
+  const int row = blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= g.ny * g.nz) return;
+  const int c0 = row * g.nx;
+  int start = -1;
+  for (int x = 0; x < g.nx; ++x) {
+    const int c = c0 + x;
+    if (state[c] == 0) {
+      if (start < 0) start = c;
+      L[c] = start;
+    } else {
+      start = -1;
+      L[c] = -1;
+    }
+  }
 }
 
+// Join the runs along y and z.  A voxel whose left neighbour and that
+// neighbour's y (z) neighbour are both unknown skips the y (z) union: the two
+// runs are joined by the first voxel of that overlap.
 __global__ void cc_merge(const uint8_t* __restrict__ state, GridGeom g, int count, int* L) {
   const int c = thread_voxel();
   if (c >= count || state[c] != 0) return;
   const int plane = g.nx * g.ny;
   const int x = c % g.nx, y = (c / g.nx) % g.ny, z = c / plane;
-  if (x + 1 < g.nx && state[c + 1] == 0) uf_unite(L, c, c + 1);
-  if (y + 1 < g.ny && state[c + g.nx] == 0) uf_unite(L, c, c + g.nx);
-  if (z + 1 < g.nz && state[c + plane] == 0) uf_unite(L, c, c + plane);
+  const bool left = x > 0 && state[c - 1] == 0;
+  if (y + 1 < g.ny && state[c + g.nx] == 0 && !(left && state[c - 1 + g.nx] == 0))
+    uf_unite(L, c, c + g.nx);
+  if (z + 1 < g.nz && state[c + plane] == 0 && !(left && state[c - 1 + plane] == 0))
+    uf_unite(L, c, c + plane);
 }
 
 // L[c] = root of c; roots of boundary voxels are flagged exterior.
@@ -285,7 +311,8 @@ hipError_t launch_grid_flood(uint8_t* state, const GridGeom& g, int* labels, uin
   const int n = static_cast<int>(count);
   const unsigned blocks = blocks_for(count, kThreads);
   TimerScope scope(timer, "grid_flood", stream);
-  cc_init<<<blocks, kThreads, 0, stream>>>(state, n, labels);
+  cc_rows<<<blocks_for(static_cast<long long>(g.ny) * g.nz, kThreads), kThreads, 0, stream>>>(
+      state, g, labels);
   cc_merge<<<blocks, kThreads, 0, stream>>>(state, g, n, labels);
   cc_flatten<<<blocks, kThreads, 0, stream>>>(state, g, n, labels, exterior);
   cc_apply<<<blocks, kThreads, 0, stream>>>(state, n, labels, exterior);
