@@ -66,21 +66,22 @@ __global__ __launch_bounds__(kThreads) void grid_raster(const double* __restrict
   axis_bounds(py, radius, g.lo[1], g.h, g.ny, y0, y1);
   axis_bounds(pz, radius, g.lo[2], g.h, g.nz, z0, z1);
   if (x1 < x0 || y1 < y0 || z1 < z0) return;
-  const unsigned wx = x1 - x0 + 1, wy = y1 - y0 + 1;
-  const unsigned total = wx * wy * static_cast<unsigned>(z1 - z0 + 1);  // a box within the grid
+  const unsigned wx = x1 - x0 + 1, wy = y1 - y0 + 1;  // a box within the grid
   const double r2 = radius * radius;
   const uint8_t v = static_cast<uint8_t>(value);
-  for (unsigned k = lane; k < total; k += 64) {
+  const int plane = g.nx * g.ny;
+  // lanes take (x, y) columns of the box and walk z; dx^2 + dy^2 is formed
+  // once per column, which is the reference's left-to-right order
+  for (unsigned k = lane; k < wx * wy; k += 64) {
     const int x = x0 + static_cast<int>(k % wx);
-    const unsigned t = k / wx;
-    const int y = y0 + static_cast<int>(t % wy);
-    const int z = z0 + static_cast<int>(t / wy);
+    const int y = y0 + static_cast<int>(k / wx);
     const double dx = (g.lo[0] + static_cast<double>(x) * g.h) - px;
     const double dy = (g.lo[1] + static_cast<double>(y) * g.h) - py;
-    const double dz = (g.lo[2] + static_cast<double>(z) * g.h) - pz;
-    if (dx * dx + dy * dy + dz * dz <= r2 + 1e-10) {
-      const int c = voxel_index(g, x, y, z);
-      if (!exterior_only || mask[c] == 2) mask[c] = v;
+    const double dxy = dx * dx + dy * dy;
+    int c = voxel_index(g, x, y, z0);
+    for (int z = z0; z <= z1; ++z, c += plane) {
+      const double dz = (g.lo[2] + static_cast<double>(z) * g.h) - pz;
+      if (dxy + dz * dz <= r2 + 1e-10 && (!exterior_only || mask[c] == 2)) mask[c] = v;
     }
   }
 }
