@@ -1,5 +1,6 @@
 // C-ABI entry points: contexts, structures, the Debye path and compute_curve.
 #include <algorithm>
+#include <functional>
 #include <cmath>
 #include <cstring>
 #include <numeric>
@@ -195,9 +196,12 @@ static int choose_q_tile(int tuned, int sin_mode, size_t nq) {
   return best;
 }
 
+// `overlap`, when given, runs on the host while the pair kernel runs (after
+// the launch, before the blocking download of the tables).
 static csx_status pair_tables_batch(Call& call, const std::vector<const PointSet*>& sets,
                                     const double* q, size_t nq,
-                                    std::vector<std::vector<double>>& P) {
+                                    std::vector<std::vector<double>>& P,
+                                    const std::function<void()>& overlap = nullptr) {
   Tuning tune;
   {
     std::lock_guard<std::mutex> lock(call.ctx()->mu);
@@ -287,6 +291,7 @@ static csx_status pair_tables_batch(Call& call, const std::vector<const PointSet
                              d_counter.as<int>(), d_partial.as<double>(), d_out.as<double>(), st,
                              call.timer()),
           "debye_pairs_f32");
+  if (overlap) overlap();
   std::vector<double> raw(nbins * nq_pad);
   CSX_HIP(hipMemcpyAsync(raw.data(), d_out.as<void>(), raw.size() * sizeof(double),
                          hipMemcpyDeviceToHost, st),
@@ -324,9 +329,10 @@ static csx_status pair_tables_batch(Call& call, const std::vector<const PointSet
 }
 
 static csx_status pair_tables_f32(Call& call, const PointSet& ps, const double* q, size_t nq,
-                                  std::vector<double>& P) {
+                                  std::vector<double>& P,
+                                  const std::function<void()>& overlap = nullptr) {
   std::vector<std::vector<double>> all;
-  CSX_TRY(pair_tables_batch(call, {&ps}, q, nq, all));
+  CSX_TRY(pair_tables_batch(call, {&ps}, q, nq, all, overlap));
   P = std::move(all[0]);
   return CSX_OK;
 }
@@ -434,13 +440,14 @@ static csx_status debye_basis(csx_structure* s, const double* q, size_t nq,
   PointSet* ps = nullptr;
   CSX_TRY(get_point_set(s, p, &ps));
   std::vector<double> V, E, H;
-  type_amplitudes(s, *ps, q, nq, implicit_h, p, V, E, H);
+  auto amplitudes = [&] { type_amplitudes(s, *ps, q, nq, implicit_h, p, V, E, H); };
   if (precision == CSX_FP64_PLAIN) {
+    amplitudes();
     const std::vector<double>* tabs[3] = {&V, &E, &H};
     return plain_f64(call, *ps, q, nq, 3, tabs, out);
   }
   std::vector<double> P;
-  CSX_TRY(pair_tables_f32(call, *ps, q, nq, P));
+  CSX_TRY(pair_tables_f32(call, *ps, q, nq, P, amplitudes));
   contract(*ps, P, nq, V, V, out + 0 * nq);
   contract(*ps, P, nq, E, E, out + 1 * nq);
   contract(*ps, P, nq, H, H, out + 2 * nq);
@@ -710,13 +717,14 @@ csx_status csx_debye_intensity(csx_structure* s, const double* q, size_t nq, int
   PointSet* ps = nullptr;
   CSX_TRY(get_point_set(s, nullptr, &ps));
   std::vector<double> V, E, H;
-  type_amplitudes(s, *ps, q, nq, implicit_h != 0, nullptr, V, E, H);
+  auto amplitudes = [&] { type_amplitudes(s, *ps, q, nq, implicit_h != 0, nullptr, V, E, H); };
   if (precision == CSX_FP64_PLAIN) {
+    amplitudes();
     const std::vector<double>* tabs[3] = {&V, &V, &V};
     CSX_TRY(plain_f64(call, *ps, q, nq, 1, tabs, out));
   } else {
     std::vector<double> P;
-    CSX_TRY(pair_tables_f32(call, *ps, q, nq, P));
+    CSX_TRY(pair_tables_f32(call, *ps, q, nq, P, amplitudes));
     contract(*ps, P, nq, V, V, out);
   }
   for (size_t k = 0; k < nq; ++k) out[k] = std::max(out[k], 0.0);
@@ -790,24 +798,25 @@ static csx_status batch_debye(csx_structure* const* structures, size_t m, const 
     CSX_TRY(get_point_set(structures[c], p, &ps));
     sets[c] = ps;
   }
-  std::vector<std::vector<double>> P;
-  CSX_TRY(pair_tables_batch(call, sets, q, nq, P));
+  std::vector<std::vector<double>> P, V(m), E(m), H(m);
+  CSX_TRY(pair_tables_batch(call, sets, q, nq, P, [&] {
+    for (size_t c = 0; c < m; ++c)
+      type_amplitudes(structures[c], *sets[c], q, nq, implicit_h, p, V[c], E[c], H[c]);
+  }));
   for (size_t c = 0; c < m; ++c) {
-    std::vector<double> V, E, H;
-    type_amplitudes(structures[c], *sets[c], q, nq, implicit_h, p, V, E, H);
     if (!p) {
       double* o = out + c * nq;
-      contract(*sets[c], P[c], nq, V, V, o);
+      contract(*sets[c], P[c], nq, V[c], V[c], o);
       for (size_t k = 0; k < nq; ++k) o[k] = std::max(o[k], 0.0);
       continue;
     }
     double* o = out + c * 6 * nq;
-    contract(*sets[c], P[c], nq, V, V, o + 0 * nq);
-    contract(*sets[c], P[c], nq, E, E, o + 1 * nq);
-    contract(*sets[c], P[c], nq, H, H, o + 2 * nq);
-    contract(*sets[c], P[c], nq, V, E, o + 3 * nq);
-    contract(*sets[c], P[c], nq, V, H, o + 4 * nq);
-    contract(*sets[c], P[c], nq, E, H, o + 5 * nq);
+    contract(*sets[c], P[c], nq, V[c], V[c], o + 0 * nq);
+    contract(*sets[c], P[c], nq, E[c], E[c], o + 1 * nq);
+    contract(*sets[c], P[c], nq, H[c], H[c], o + 2 * nq);
+    contract(*sets[c], P[c], nq, V[c], E[c], o + 3 * nq);
+    contract(*sets[c], P[c], nq, V[c], H[c], o + 4 * nq);
+    contract(*sets[c], P[c], nq, E[c], H[c], o + 5 * nq);
   }
   return CSX_OK;
 }
