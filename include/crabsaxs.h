@@ -5,8 +5,8 @@
  * `pub` functions of src/scattering.rs, src/solvent.rs and src/spherical.rs
  * that the fitting and analysis layers call.  Each compute entry point below
  * replaces one of them; the comment on each names the function it replaces
- * (file:line in the reference).  A Rust `-sys` crate binds these 1:1 (the
- * bindings are given in INTEGRATION.md).
+ * (file:line in the reference).  A Rust `-sys` crate binds these 1:1 with
+ * `extern "C"` declarations that mirror this header.
  *
  * Conventions
  *   - Every function returns a csx_status; on failure a message is kept in a
