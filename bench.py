@@ -3,9 +3,11 @@
 Headline workload (one "step"): the Debye precision-study profile -- a seeded
 20,000-atom globule (R = 35 A), q = linspace(0, 1, 500), default solvent
 parameters -- computed with the FP32-compensated per-type pair-table kernel and
-contracted to I(q).  Inputs are resident in HBM before timing starts (the
-structure is uploaded once); each step is one C-ABI call: q upload, pair
-kernel, FP64 bin reduction, contraction, I(q) download.
+contracted to I(q), in the launch shape that the project's autotuner picks for
+this workload before timing starts (--no-autotune: the library default).
+Inputs are resident in HBM before timing starts (the structure is uploaded
+once); each step is one C-ABI call: q upload, pair kernel, FP64 bin
+reduction, contraction, I(q) download.
 
 Prints one JSON line: metric pair.q evals/s (whole job), the dominant kernel's
 roofline (timed with HIP events on its own stream inside the library; HBM
@@ -64,6 +66,11 @@ def parse():
     ap.add_argument("--sin-mode", type=int, default=None)
     ap.add_argument("--q-tile", type=int, default=None)
     ap.add_argument("--j-chunk", type=int, default=None)
+    ap.add_argument("--persist", type=int, default=None,
+                    help="persistent workgroups per CU for the pair kernel (0: grid launch)")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="run the headline with the library's default launch shape instead "
+                         "of the autotuned one")
     ap.add_argument("--conformers", type=int, default=512)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run a device-to-device copy of BYTES (HBM counter calibration)")
@@ -385,19 +392,32 @@ def main():
         print(msg, file=sys.stderr, flush=True)
 
     ctx = csx.default_context()
-    tune = dict(q_tile=0, sin_mode=2, j_chunk=512)
-    if args.sin_mode is not None:
-        tune["sin_mode"] = args.sin_mode
-    if args.q_tile is not None:
-        tune["q_tile"] = args.q_tile
-    if args.j_chunk is not None:
-        tune["j_chunk"] = args.j_chunk
-    ctx.set_tuning(**tune)
+    tune = dict(q_tile=0, sin_mode=2, j_chunk=512, persist=0)
+    explicit = {k: v for k, v in (("sin_mode", args.sin_mode), ("q_tile", args.q_tile),
+                                  ("j_chunk", args.j_chunk), ("persist", args.persist))
+                if v is not None}
+    tune.update(explicit)
 
+    def apply_tune():
+        ctx.set_tuning(tune["q_tile"], tune["sin_mode"], tune["j_chunk"])
+        ctx.set_persistence(tune["persist"])
+
+    apply_tune()
     s = csx.generate_globule(N_ATOMS, RADIUS, 1.0, seed=SEED)
     q = np.linspace(0.0, 1.0, N_Q)
     params = csx.SolventParams()
     s.handle(ctx)  # upload before timing
+    tuned = None
+    if not explicit and not args.no_autotune:
+        # the project's autotuner picks the pair kernel's launch shape for this
+        # workload (every shape computes the same sums); timed steps use it
+        from crabsaxs_amd import autotune
+        t0 = time.perf_counter()
+        tuned = autotune.autotune(s, q, ctx, use_cache=False, params=params, repeats=2)
+        tune = {k: tuned[k] for k in ("q_tile", "sin_mode", "j_chunk", "persist")}
+        log(f"headline: autotuned {tune} ({tuned['tried']} shapes, "
+            f"{time.perf_counter() - t0:.1f} s)")
+        apply_tune()
     F32, F64 = csx.Precision.FP32_COMPENSATED, csx.Precision.FP64_PLAIN
 
     for _ in range(args.warmup):
@@ -434,7 +454,7 @@ def main():
         "config": {"workload": "Debye precision study: seeded 20,000-atom globule, R=35 A, "
                                "q=linspace(0,1,500), default solvent, per-type S_ab tables",
                    "atoms": N_ATOMS, "q_points": N_Q, "pairs": N_ATOMS * (N_ATOMS - 1) // 2,
-                   "tuning": tune, "launch": launch},
+                   "tuning": tune, "autotuned": tuned is not None, "launch": launch},
         "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": PEAK_VALU_OPS / 1e12,
                      "unit": "T VALU-op/s", "frac": achieved / PEAK_VALU_OPS,
                      "traffic": traffic_per_launch("debye_pairs_f32"),
@@ -456,8 +476,7 @@ def main():
         log("cpu baseline done")
     if "ensemble" in args.legs:
         result["ensemble"] = ensemble_leg(csx, ctx, args.conformers, log)
-        ctx.set_tuning(**tune)  # the other legs run with the headline shape
-        ctx.set_persistence(0)
+        apply_tune()  # the other legs run with the headline shape
     if "multipole" in args.legs:
         result["multipole"] = multipole_leg(csx, ctx, log)
     if "adjoint" in args.legs:
