@@ -177,6 +177,9 @@ class ParseOptions:
             None if self.chains is None else ",".join(self.chains).encode())
 
 
+_handle_lock = threading.Lock()  # creation of Structure device handles
+
+
 def _read_atoms(path, options: Optional[ParseOptions]):
     h = C.c_void_p()
     opts = (options or ParseOptions())._c()
@@ -202,8 +205,7 @@ class Structure:
              for a in atoms], dtype=np.int8)
         self.occupancy = f64([a.occupancy for a in atoms])
         self.atoms = atoms
-        self._handle = None
-        self._ctx = None
+        self._handles = {}
 
     @classmethod
     def from_pdb_file(cls, path, options: "ParseOptions" = None) -> "Structure":
@@ -264,8 +266,7 @@ class Structure:
         s.group = np.ascontiguousarray(
             np.full(n, -1, dtype=np.int8) if group is None else np.asarray(group, dtype=np.int8))
         s.atoms = None
-        s._handle = None
-        s._ctx = None
+        s._handles = {}
         return s
 
     def __len__(self) -> int:
@@ -276,23 +277,31 @@ class Structure:
         return len(self)
 
     def handle(self, ctx: "Context" = None):
+        """The device copy on `ctx`, uploaded on first use.  One handle per
+        context, created under a lock and kept until the structure is freed,
+        so threads may share a structure."""
         ctx = ctx or default_context()
-        if self._handle is None or self._ctx is not ctx:
+        entry = self._handles.get(id(ctx))
+        if entry is not None and entry[0] is ctx:
+            return entry[1]
+        with _handle_lock:
+            entry = self._handles.get(id(ctx))
+            if entry is not None and entry[0] is ctx:
+                return entry[1]
             h = C.c_void_p()
-            lib = _lib.load()
-            _check(lib.csx_structure_create(
+            _check(_lib.load().csx_structure_create(
                 ctx.ptr, len(self), dptr(self.xyz),
                 self.element.ctypes.data_as(C.POINTER(C.c_uint8)),
                 self.group.ctypes.data_as(C.POINTER(C.c_int8)), dptr(self.occupancy),
                 C.byref(h)))
-            self._free()
-            self._handle, self._ctx = h, ctx
-        return self._handle
+            self._handles[id(ctx)] = (ctx, h)
+            return h
 
     def _free(self):
-        if getattr(self, "_handle", None) is not None:
-            _lib.load().csx_structure_destroy(self._handle)
-            self._handle = None
+        handles = getattr(self, "_handles", {})
+        for _, h in handles.values():
+            _lib.load().csx_structure_destroy(h)
+        handles.clear()
 
     def __del__(self):
         try:
