@@ -182,7 +182,7 @@ class Structure {
   Structure& operator=(const Structure& o) {
     atoms = o.atoms;
     std::lock_guard<std::mutex> lock(state_->mu);
-    state_->upload.reset();
+    state_->uploads.clear();
     return *this;
   }
   Structure(Structure&& o) : atoms(std::move(o.atoms)), state_(std::move(o.state_)) {
@@ -192,7 +192,7 @@ class Structure {
     atoms = std::move(o.atoms);
     std::swap(state_, o.state_);
     std::lock_guard<std::mutex> lock(o.state_->mu);
-    o.state_->upload.reset();
+    o.state_->uploads.clear();
     return *this;
   }
 
@@ -207,12 +207,16 @@ class Structure {
   static Structure parse(const std::string& text, int format, const ParseOptions& o);
 
   // The device copy of this structure's current atoms on `ctx`: uploaded on
-  // first use and again whenever the atoms have changed since.
+  // first use and again whenever the atoms have changed since.  One copy per
+  // context, so threads may share a structure across contexts; changing the
+  // atoms while another thread computes on the structure is a data race.
   csx_structure* handle(Context& ctx) const {
     const uint64_t fp = fingerprint();
     std::lock_guard<std::mutex> lock(state_->mu);
-    Upload* u = state_->upload.get();
-    if (u && u->ctx == ctx.get() && u->fingerprint == fp && u->n == atoms.size()) return u->h;
+    std::unique_ptr<Upload>* slot = nullptr;
+    for (auto& u : state_->uploads)
+      if (u->ctx == ctx.get()) slot = &u;
+    if (slot && (*slot)->fingerprint == fp && (*slot)->n == atoms.size()) return (*slot)->h;
     const size_t n = atoms.size();
     std::vector<double> xyz(3 * n), occ(n);
     std::vector<uint8_t> el(n);
@@ -233,8 +237,12 @@ class Structure {
     fresh->ctx = ctx.get();
     fresh->fingerprint = fp;
     fresh->n = n;
-    state_->upload = std::move(fresh);
-    return state_->upload->h;
+    csx_structure* h = fresh->h;
+    if (slot)
+      *slot = std::move(fresh);
+    else
+      state_->uploads.push_back(std::move(fresh));
+    return h;
   }
 
  private:
@@ -249,7 +257,7 @@ class Structure {
   };
   struct State {
     std::mutex mu;
-    std::unique_ptr<Upload> upload;
+    std::vector<std::unique_ptr<Upload>> uploads;  // one per context
   };
   std::unique_ptr<State> state_{new State};
 

@@ -15,6 +15,7 @@
 #include <cstring>
 #include <functional>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "crabsaxs.hpp"
@@ -620,6 +621,38 @@ CSX_TEST(compute_curve_for_example_structure_is_well_formed, true) {
   const std::vector<double> curve = debye_intensity(s, q);
   CHECK(curve.size() == q.size());
   for (double v : curve) CHECK(std::isfinite(v) && v >= 0.0);
+}
+
+// The seam is called from rayon workers (ensemble.rs:244-256): threads sharing
+// structures get bitwise the sequential profiles.
+CSX_TEST(concurrent_calls_match_sequential, true) {
+  std::vector<Structure> shared, fresh;
+  for (int k = 0; k < 3; ++k) {
+    std::vector<Atom> atoms;
+    for (int i = 0; i < 300 + 200 * k; ++i)
+      atoms.push_back(atom(9.0 * std::cos(0.37 * i), 9.0 * std::sin(0.37 * i), 0.3 * i - 40.0,
+                           i % 5 ? Element::C : Element::O, "A" + std::to_string(i)));
+    shared.emplace_back(atoms);
+    fresh.emplace_back(atoms);
+  }
+  const std::vector<double> q = range_q(1, 60, 0.008);
+  SolventParams p;
+  p.c1 = 1.01;
+  p.c2 = 0.5;
+  std::vector<std::vector<double>> want;
+  for (const Structure& s : fresh) want.push_back(solvent_corrected_intensity(s, q, p, true));
+  std::vector<std::vector<double>> got(12);
+  std::vector<std::thread> threads;
+  for (int i = 0; i < 12; ++i)
+    threads.emplace_back([&, i] {
+      try {
+        got[i] = solvent_corrected_intensity(shared[i % 3], q, p, true);
+      } catch (const std::exception&) {
+        got[i].clear();  // reported by the comparison below
+      }
+    });
+  for (std::thread& t : threads) t.join();
+  for (int i = 0; i < 12; ++i) CHECK(got[i] == want[i % 3]);
 }
 
 CSX_TEST(ensemble_fitter_recovers_a_mixture, true) {
