@@ -9,14 +9,14 @@
 namespace csx {
 
 int multipole_chunk();
-int multipole_tiles_per_group(int nch);
-size_t multipole_coeff_lds_bytes(int l_range, int nch);
+int multipole_tiles_per_group(int nch, int ntiles);
+size_t multipole_coeff_lds_bytes(int l_range, int nch, int ntypes);
 hipError_t launch_multipole_geometry(const double* xyz, int npts, const double* centre, int l_max,
                                      const double* norms, const int* tile_first, int ntiles,
                                      int npad, float* radius, float* Y, hipStream_t stream);
 hipError_t launch_multipole_coeff(const float* radius, const int* type, const float* occ, int npts,
                                   int npad, const float* Y, int ntiles, const int* tile_l,
-                                  int max_l_range, const float* amp, const float* qv,
+                                  int max_l_range, const float* amp, int ntypes, const float* qv,
                                   const int* qlmax, int nqp, int nch, float* partial, double* coef,
                                   double* out, hipStream_t stream, KernelTimer* timer);
 
@@ -143,8 +143,9 @@ static csx_status get_geometry(Call& call, csx_structure* s, PointSet& ps, int L
   CSX_HIP(g->occ.upload(occ.data(), occ.size() * sizeof(float)), "upload occupancy");
   CSX_HIP(g->tile_l.upload(tile_l.data(), tile_l.size() * sizeof(int)), "upload tiles");
   for (int nch : {1, 3})
-    if (multipole_coeff_lds_bytes(max_group_l_range(tile_l, multipole_tiles_per_group(nch)),
-                                  nch) > s->ctx->lds_per_workgroup)
+    if (multipole_coeff_lds_bytes(
+            max_group_l_range(tile_l, multipole_tiles_per_group(nch, g->ntiles)), nch,
+            ps.ntypes) > s->ctx->lds_per_workgroup)
       return fail(CSX_INVALID_INPUT, "multipole l_max too large for the device's LDS");
   const size_t ybytes = static_cast<size_t>(g->ntiles) * g->npad * 16 * sizeof(float);
   CSX_HIP(g->Y.alloc(ybytes), "hipMalloc harmonics");
@@ -232,10 +233,10 @@ static csx_status multipole_run(csx_structure* s, const double* q, size_t nq, in
           "hipMemcpyAsync");
   CSX_HIP(hipMemcpyAsync(d_l.as<void>(), qlmax.data(), nqp * sizeof(int), hipMemcpyHostToDevice, st),
           "hipMemcpyAsync");
-  const int l_range = max_group_l_range(g->h_tile_l, multipole_tiles_per_group(nch));
+  const int l_range = max_group_l_range(g->h_tile_l, multipole_tiles_per_group(nch, g->ntiles));
   CSX_HIP(launch_multipole_coeff(g->radius.as<float>(), g->type.as<int>(), g->occ.as<float>(), npts,
                                  g->npad, g->Y.as<float>(), g->ntiles, g->tile_l.as<int>(),
-                                 l_range, d_amp.as<float>(), d_q.as<float>(), d_l.as<int>(),
+                                 l_range, d_amp.as<float>(), TT, d_q.as<float>(), d_l.as<int>(),
                                  static_cast<int>(nqp), nch, d_partial.as<float>(),
                                  d_coef.as<double>(), d_out.as<double>(), st, call.timer()),
           "multipole_coeff");

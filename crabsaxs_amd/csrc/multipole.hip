@@ -233,10 +233,10 @@ __device__ inline void bessel_column(float xf, int lq, int l_lo, int l_hi, float
 // At least two waves per SIMD: caps registers at 256, which also stops the
 // scheduler from hoisting every tile's operand loads.
 template <int NCH, int MAXT>
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) void multipole_coeff(
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(NCH == 1 ? 4 : 3))) void multipole_coeff(
     const float* __restrict__ radius, const int* __restrict__ type, const float* __restrict__ occ,
     int npts, int npad, const float* __restrict__ Y, int ntiles, const int* __restrict__ tile_l,
-    const float* __restrict__ amp /* [ntypes][nqp][NCH] */, const float* __restrict__ qv,
+    const float* __restrict__ amp /* [ntypes][nqp][NCH] */, int ntypes, const float* __restrict__ qv,
     const int* __restrict__ qlmax, int nqp, int nchunk, int ngroups,
     float* __restrict__ partial) {
   extern __shared__ float smem[];
@@ -262,8 +262,23 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
   if (l_lo > Lb) return;  // every tile of the group is above this block's orders
   const int l_hi = min(tile_l[g1 - 1], Lb);
   const int lr = l_hi - l_lo + 1;
-  float* X = smem;              // [lr][16 q][4 slots][4 ks]
-  float* AMP = smem + lr * kXL; // [16 p][16 q][NCH]
+  float* X = smem;                        // [lr][16 q][4 slots][4 ks]
+  float* AMP = smem + lr * kXL;           // [16 p][16 q][NCH]
+  float* ATAB = AMP + kSlab * kQB * NCH;  // [ntypes][16 q][NCH]: this block's amplitudes
+  float* PR = ATAB + ntypes * kQB * NCH;  // [kChunk] radius, occupancy, type of the
+  float* PO = PR + kChunk;                //   chunk's points (0 past npts)
+  int* PT = reinterpret_cast<int*>(PO + kChunk);
+  for (int k = threadIdx.x; k < ntypes * kQB * NCH; k += kThreads) {
+    const int t = k / (kQB * NCH), rem = k - t * (kQB * NCH);
+    ATAB[k] = amp[(size_t(t) * nqp + q0) * NCH + rem];
+  }
+  for (int k = threadIdx.x; k < kChunk; k += kThreads) {
+    const int i = chunk * kChunk + k;
+    const bool v = i < npts;
+    PR[k] = v ? radius[i] : 0.f;
+    PO[k] = v ? occ[i] : 0.f;
+    PT[k] = v ? type[i] : 0;
+  }
 
   v4f acc[MAXT][NCH];
 #pragma unroll
@@ -282,47 +297,76 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
   const int nslab = npad / kSlab;
   const int s0 = chunk * (kChunk / kSlab), s1 = min(s0 + kChunk / kSlab, nslab);
 
+  // This wave's tiles are g0 + wave + 4 t.  tile_l is non-decreasing, so the
+  // live ones (inside the group and at most l_hi) are t < nt; xoff[t] is the
+  // LDS offset of tile t's order.  Both are wave-uniform.
+  int nt = 0;
+  int xoff[MAXT];
+#pragma unroll
+  for (int t = 0; t < MAXT; ++t) {
+    const int tile = g0 + wave + 4 * t;
+    const int l = tile < g1 ? tile_l[tile] : l_hi + 1;
+    if (l <= l_hi) nt = t + 1;
+    xoff[t] = (min(l, l_hi) - l_lo) * kXL + mn * 16 + 4 * xslot(mn, mk);
+  }
+  // Harmonic tile rows read by this wave; tiles past the live ones read the
+  // last live (or, with none, the last existing) tile, so every load in the
+  // loop is unconditional and in bounds.
+  size_t yrow[MAXT];
+#pragma unroll
+  for (int t = 0; t < MAXT; ++t)
+    yrow[t] = size_t(min(g0 + wave + 4 * max(min(t, nt - 1), 0), ntiles - 1)) * nslab * 256;
+
+  // Harmonic tiles are read in batches of kB: the loads of batch b + 1 are in
+  // flight while batch b runs its MFMAs.
+  constexpr int kB = 2, kNB = MAXT / kB;
+  static_assert(MAXT % kB == 0, "whole batches of tiles");
   for (int s = s0; s < s1; ++s) {
     __syncthreads();  // the previous slab's operands have been read
-    const int i = s * kSlab + gp;
-    const bool vi = i < npts;
-    bessel_column(vi ? qg * radius[i] : 0.f, vi ? lg : -1, l_lo, l_hi,
+    const int k = (s - s0) * kSlab + gp;
+    const bool vi = s * kSlab + gp < npts;
+    bessel_column(vi ? qg * PR[k] : 0.f, vi ? lg : -1, l_lo, l_hi,
                   X + gq * 16 + 4 * xslot(gq, gp & 3) + (gp >> 2));
-    {
-      const float o = vi ? occ[i] : 0.f;
-      const int ty = vi ? type[i] : 0;
 #pragma unroll
-      for (int c = 0; c < NCH; ++c)
-        AMP[(gp * kQB + gq) * NCH + c] = o * amp[(size_t(ty) * nqp + q0 + gq) * NCH + c];
-    }
+    for (int c = 0; c < NCH; ++c)
+      AMP[(gp * kQB + gq) * NCH + c] = PO[k] * ATAB[(PT[k] * kQB + gq) * NCH + c];
     __syncthreads();
+    const float* Ys = Y + size_t(s) * 256 + lane * 4;
+    v4f yb[2][kB];
+#pragma unroll
+    for (int u = 0; u < kB; ++u) yb[0][u] = *reinterpret_cast<const v4f*>(Ys + yrow[u]);
     float am[NCH][4];
 #pragma unroll
     for (int ks = 0; ks < 4; ++ks)
 #pragma unroll
       for (int c = 0; c < NCH; ++c) am[c][ks] = AMP[((4 * ks + mk) * kQB + mn) * NCH + c];
 #pragma unroll
-    for (int t = 0; t < MAXT; ++t) {
-      const int tile = g0 + wave + 4 * t;
-      if (tile >= g1) continue;
-      const int l = tile_l[tile];
-      if (l > l_hi) continue;
-      const v4f a =
-          *reinterpret_cast<const v4f*>(X + (l - l_lo) * kXL + mn * 16 + 4 * xslot(mn, mk));
-      const v4f y = *reinterpret_cast<const v4f*>(Y + (size_t(tile) * nslab + s) * 256 + lane * 4);
+    for (int b = 0; b < kNB; ++b) {
+      if (b + 1 < kNB) {
 #pragma unroll
-      for (int c = 0; c < NCH; ++c)
+        for (int u = 0; u < kB; ++u)
+          yb[(b + 1) & 1][u] = *reinterpret_cast<const v4f*>(Ys + yrow[(b + 1) * kB + u]);
+      }
 #pragma unroll
-        for (int ks = 0; ks < 4; ++ks)
-          acc[t][c] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[ks] * am[c][ks], y[ks], acc[t][c],
-                                                           0, 0, 0);
+      for (int u = 0; u < kB; ++u) {
+        const int t = b * kB + u;
+        if (t >= nt) continue;
+        const v4f a = *reinterpret_cast<const v4f*>(X + xoff[t]);
+        const v4f y = yb[b & 1][u];
+#pragma unroll
+        for (int c = 0; c < NCH; ++c)
+#pragma unroll
+          for (int ks = 0; ks < 4; ++ks)
+            acc[t][c] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[ks] * am[c][ks], y[ks], acc[t][c],
+                                                             0, 0, 0);
+      }
     }
   }
   const int ncol = ntiles * 16;
 #pragma unroll
   for (int t = 0; t < MAXT; ++t) {
+    if (t >= nt) continue;
     const int tile = g0 + wave + 4 * t;
-    if (tile >= g1 || tile_l[tile] > l_hi) continue;
 #pragma unroll
     for (int c = 0; c < NCH; ++c)
 #pragma unroll
@@ -407,12 +451,12 @@ __global__ __launch_bounds__(256) void multipole_reduce(const double* __restrict
           ((scratch[k] + scratch[NT + k]) + scratch[2 * NT + k]) + scratch[3 * NT + k];
 }
 
-// Accumulator tiles per wave (4 x MAXT tiles per tile group), from a sweep
-// at 108k points x 200 q: one channel 16 (L = 15/25/40: 0.59/0.98/2.02 ms;
-// 8: 0.55/1.16/2.20; 12: 0.57/1.61/2.35), three channels 8 (L = 25: 2.17
-// ms; 6: 2.32; 4: 2.14).  Fewer accumulators raise occupancy, which hides
-// the harmonic-tile loads, at the cost of generating j_l once per tile
-// group.
+// Accumulator tiles per wave (4 x MAXT tiles per tile group): 16 for one
+// channel and 8 for three, or half that where the half still covers the
+// tiles in as many groups.  Fewer accumulators raise occupancy; more groups
+// generate j_l once more each.  At 108k points x 200 q, one channel, MAXT 8
+// vs 16: L = 15 (one group either way) 0.46 vs 0.52 ms; L = 25 (two groups
+// vs one) 0.99 vs 0.81 ms.
 constexpr int kMaxT1 = 16;
 constexpr int kMaxT3 = 8;
 
@@ -420,10 +464,13 @@ constexpr int kMaxT3 = 8;
 
 int multipole_chunk() { return kChunk; }
 
-int multipole_tiles_per_group(int nch) { return 4 * (nch == 1 ? kMaxT1 : kMaxT3); }
+int multipole_tiles_per_group(int nch, int ntiles) {
+  const int big = 4 * (nch == 1 ? kMaxT1 : kMaxT3), small = big / 2;
+  return (ntiles + small - 1) / small == (ntiles + big - 1) / big ? small : big;
+}
 
-size_t multipole_coeff_lds_bytes(int l_range, int nch) {
-  return size_t(l_range) * kXL * sizeof(float) + size_t(kSlab) * kQB * nch * sizeof(float);
+size_t multipole_coeff_lds_bytes(int l_range, int nch, int ntypes) {
+  return (size_t(l_range) * kXL + size_t(kSlab + ntypes) * kQB * nch + 3 * kChunk) * sizeof(float);
 }
 
 hipError_t launch_multipole_geometry(const double* xyz, int npts, const double* centre,
@@ -442,27 +489,32 @@ hipError_t launch_multipole_geometry(const double* xyz, int npts, const double* 
 hipError_t launch_multipole_coeff(const float* radius, const int* type, const float* occ,
                                   int npts, int npad, const float* Y, int ntiles,
                                   const int* tile_l, int max_l_range, const float* amp,
-                                  const float* qv, const int* qlmax, int nqp, int nch,
+                                  int ntypes, const float* qv, const int* qlmax, int nqp, int nch,
                                   float* partial, double* coef, double* out, hipStream_t stream,
                                   KernelTimer* timer) {
   if (nqp % kQB != 0 || npad % kSlab != 0 || ntiles <= 0 || (nch != 1 && nch != 3))
     return hipErrorInvalidValue;
   const int nchunk = (npad + kChunk - 1) / kChunk;
-  const int tpg = multipole_tiles_per_group(nch);
+  const int tpg = multipole_tiles_per_group(nch, ntiles);
   const int ngroups = (ntiles + tpg - 1) / tpg;
   const long long total = (long long)(nqp / kQB) * nchunk * ngroups;
   if (total > INT32_MAX) return hipErrorInvalidValue;
-  const size_t lds = multipole_coeff_lds_bytes(max_l_range, nch);
+  const size_t lds = multipole_coeff_lds_bytes(max_l_range, nch, ntypes);
   {
     TimerScope scope(timer, "multipole_coeff", stream);
-    if (nch == 1)
-      multipole_coeff<1, kMaxT1><<<unsigned(total), kThreads, lds, stream>>>(
-          radius, type, occ, npts, npad, Y, ntiles, tile_l, amp, qv, qlmax, nqp, nchunk, ngroups,
-          partial);
+#define CSX_MP_LAUNCH(NCH, MAXT)                                                              \
+  multipole_coeff<NCH, MAXT><<<unsigned(total), kThreads, lds, stream>>>(                       \
+      radius, type, occ, npts, npad, Y, ntiles, tile_l, amp, ntypes, qv, qlmax, nqp, nchunk,     \
+      ngroups, partial)
+    if (nch == 1 && tpg == 4 * kMaxT1)
+      CSX_MP_LAUNCH(1, kMaxT1);
+    else if (nch == 1)
+      CSX_MP_LAUNCH(1, kMaxT1 / 2);
+    else if (tpg == 4 * kMaxT3)
+      CSX_MP_LAUNCH(3, kMaxT3);
     else
-      multipole_coeff<3, kMaxT3><<<unsigned(total), kThreads, lds, stream>>>(
-          radius, type, occ, npts, npad, Y, ntiles, tile_l, amp, qv, qlmax, nqp, nchunk, ngroups,
-          partial);
+      CSX_MP_LAUNCH(3, kMaxT3 / 2);
+#undef CSX_MP_LAUNCH
   }
   const int ncol = ntiles * 16;
   const dim3 sum_grid((ncol + 255) / 256, nqp);
