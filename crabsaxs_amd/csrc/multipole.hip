@@ -129,13 +129,14 @@ __global__ void fill_zero(float* __restrict__ p, size_t n) {
 // recurrence cannot leave the double range from 1e-250 and runs without
 // overflow tests.  Otherwise f is rescaled by 1e-250 on overflow and values
 // computed before the last rescale (below 1e-250 of the normalising value)
-// are stored as 0.
-__device__ inline void bessel_column(float xf, int lq, int l_lo, int l_hi, float* X) {
+// are stored as 0.  Every stored value is the FP32 j_l times `scale` (one
+// FP32 multiply; 1 leaves it unchanged).
+__device__ inline void bessel_column(float xf, int lq, int l_lo, int l_hi, float scale, float* X) {
   const int top = min(lq, l_hi);
   int l_zero = l_lo;  // first l to clear
   if (top >= l_lo) {
     if (!(xf >= 1e-8f)) {
-      if (l_lo == 0) X[0] = 1.f;
+      if (l_lo == 0) X[0] = scale;
       l_zero = max(l_lo, 1);
     } else {
       const double x = double(xf);
@@ -145,17 +146,17 @@ __device__ inline void bessel_column(float xf, int lq, int l_lo, int l_hi, float
       const double j0 = double(sf) * ix;
       const double j1 = (double(sf) * ix - double(cf)) * ix;
       const int lu = min(top, int(floorf(xf)));
-      if (l_lo == 0) X[0] = float(j0);
+      if (l_lo == 0) X[0] = float(j0) * scale;
       double jlu = j0;
       if (lu >= 1) {
-        if (l_lo <= 1) X[(1 - l_lo) * kXL] = float(j1);
+        if (l_lo <= 1) X[(1 - l_lo) * kXL] = float(j1) * scale;
         double jm = j0, jc = j1, c = 3.0 * ix;  // c = (2l + 1) / x at l = 1
         for (int l = 1; l < lu; ++l) {
           const double jn = c * jc - jm;
           jm = jc;
           jc = jn;
           c += 2.0 * ix;
-          if (l + 1 >= l_lo) X[(l + 1 - l_lo) * kXL] = float(jn);
+          if (l + 1 >= l_lo) X[(l + 1 - l_lo) * kXL] = float(jn) * scale;
         }
         jlu = jc;
       }
@@ -184,7 +185,7 @@ __device__ inline void bessel_column(float xf, int lq, int l_lo, int l_hi, float
           fp = fp_top;
           c = c_top;
           for (int l = top; l >= lo; --l) {
-            X[(l - l_lo) * kXL] = float(norm * f);
+            X[(l - l_lo) * kXL] = float(norm * f) * scale;
             const double fn = c * f - fp;
             fp = f;
             f = fn;
@@ -211,7 +212,7 @@ __device__ inline void bessel_column(float xf, int lq, int l_lo, int l_hi, float
           f = 1e-250;
           c = double(2 * S + 1) * ix;
           for (int l = S; l >= lo; --l) {
-            if (l <= top) X[(l - l_lo) * kXL] = l <= last ? float(norm * f) : 0.f;
+            if (l <= top) X[(l - l_lo) * kXL] = l <= last ? float(norm * f) * scale : 0.f;
             const double fn = c * f - fp;
             fp = f;
             f = fn;
@@ -325,21 +326,28 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(NCH ==
     __syncthreads();  // the previous slab's operands have been read
     const int k = (s - s0) * kSlab + gp;
     const bool vi = s * kSlab + gp < npts;
-    bessel_column(vi ? qg * PR[k] : 0.f, vi ? lg : -1, l_lo, l_hi,
-                  X + gq * 16 + 4 * xslot(gq, gp & 3) + (gp >> 2));
+    // One channel: the amplitude is multiplied into the stored j_l values, so
+    // the MFMA A operand is read as is.  Three channels: the A operand is
+    // j_l times the channel's amplitude, formed per MFMA.
+    float ap[NCH];
 #pragma unroll
-    for (int c = 0; c < NCH; ++c)
-      AMP[(gp * kQB + gq) * NCH + c] = PO[k] * ATAB[(PT[k] * kQB + gq) * NCH + c];
+    for (int c = 0; c < NCH; ++c) ap[c] = PO[k] * ATAB[(PT[k] * kQB + gq) * NCH + c];
+    bessel_column(vi ? qg * PR[k] : 0.f, vi ? lg : -1, l_lo, l_hi, NCH == 1 ? ap[0] : 1.f,
+                  X + gq * 16 + 4 * xslot(gq, gp & 3) + (gp >> 2));
+    if (NCH > 1)
+#pragma unroll
+      for (int c = 0; c < NCH; ++c) AMP[(gp * kQB + gq) * NCH + c] = ap[c];
     __syncthreads();
     const float* Ys = Y + size_t(s) * 256 + lane * 4;
     v4f yb[2][kB];
 #pragma unroll
     for (int u = 0; u < kB; ++u) yb[0][u] = *reinterpret_cast<const v4f*>(Ys + yrow[u]);
     float am[NCH][4];
+    if (NCH > 1)
 #pragma unroll
-    for (int ks = 0; ks < 4; ++ks)
+      for (int ks = 0; ks < 4; ++ks)
 #pragma unroll
-      for (int c = 0; c < NCH; ++c) am[c][ks] = AMP[((4 * ks + mk) * kQB + mn) * NCH + c];
+        for (int c = 0; c < NCH; ++c) am[c][ks] = AMP[((4 * ks + mk) * kQB + mn) * NCH + c];
 #pragma unroll
     for (int b = 0; b < kNB; ++b) {
       if (b + 1 < kNB) {
@@ -357,8 +365,8 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(NCH ==
         for (int c = 0; c < NCH; ++c)
 #pragma unroll
           for (int ks = 0; ks < 4; ++ks)
-            acc[t][c] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[ks] * am[c][ks], y[ks], acc[t][c],
-                                                             0, 0, 0);
+            acc[t][c] = __builtin_amdgcn_mfma_f32_16x16x4f32(NCH == 1 ? a[ks] : a[ks] * am[c][ks],
+                                                             y[ks], acc[t][c], 0, 0, 0);
       }
     }
   }
