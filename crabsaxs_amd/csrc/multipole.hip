@@ -118,6 +118,15 @@ __global__ void fill_zero(float* __restrict__ p, size_t n) {
   if (i < n) p[i] = 0.f;
 }
 
+// 1 / a for a positive normal double whose reciprocal is normal: the
+// hardware estimate refined by two Newton steps (within an ulp; the correctly
+// rounded division sequence costs twice the instructions).
+__device__ inline double recip(double a) {
+  double r = __builtin_amdgcn_rcp(a);
+  r = fma(r, fma(-a, r, 1.0), r);
+  return fma(r, fma(-a, r, 1.0), r);
+}
+
 // One (point, q) column of spherical Bessel values: X[(l - l_lo) * kXL] =
 // j_l(x) for l in [l_lo, min(lq, l_hi)] and 0 for the rest of [l_lo, l_hi].
 // l <= floor(x): upward recurrence from j_0, j_1 (stable there).  Above:
@@ -142,7 +151,7 @@ __device__ inline void bessel_column(float xf, int lq, int l_lo, int l_hi, float
       const double x = double(xf);
       float sf, cf;
       sincosf(xf, &sf, &cf);
-      const double ix = 1.0 / x;
+      const double ix = recip(x);
       const double j0 = double(sf) * ix;
       const double j1 = (double(sf) * ix - double(cf)) * ix;
       const int lu = min(top, int(floorf(xf)));
@@ -163,34 +172,44 @@ __device__ inline void bessel_column(float xf, int lq, int l_lo, int l_hi, float
       if (top > lu) {
         const int S = top + 14;
         const int lo = max(lu + 1, l_lo);
-        const bool safe = float(S - lu) * __log2f(float(2 * S + 1) / xf) < 1800.f;
+        const bool safe = float(S - lu) * (__log2f(float(2 * S + 1)) - __log2f(xf)) < 1800.f;
         if (safe) {
-          // state at index l: f = f_l, fp = f_{l+1}, c = (2l + 1) / x
+          // state at index l: f = f_l, fp = f_{l+1}, c = (2l + 1) / x.  The
+          // loops take two steps per trip, f and fp trading roles, so the
+          // state needs no register moves.
+          const double d = 2.0 * ix;
           double fp = 0.0, f = 1e-250, c = double(2 * S + 1) * ix;
-          for (int l = S; l > top; --l) {
-            const double fn = c * f - fp;
-            fp = f;
-            f = fn;
-            c -= 2.0 * ix;
+#pragma unroll
+          for (int k = 0; k < 14; k += 2) {  // S down to top
+            fp = c * f - fp;
+            c -= d;
+            f = c * fp - f;
+            c -= d;
           }
           const double f_top = f, fp_top = fp, c_top = c;
-          for (int l = top; l > lu; --l) {
-            const double fn = c * f - fp;
-            fp = f;
-            f = fn;
-            c -= 2.0 * ix;
+          int l = top;
+          for (; l - 1 > lu; l -= 2) {
+            fp = c * f - fp;
+            c -= d;
+            f = c * fp - f;
+            c -= d;
           }
-          const double norm = jlu / f;
+          if (l > lu) f = c * f - fp;
+          // f_lu > 0 (j_l(x) > 0 for l >= floor(x)), and on this path it is at
+          // most 2^1800 times 1e-250, so its reciprocal is a normal double.
+          const double ns = jlu * recip(f) * double(scale);
           f = f_top;
           fp = fp_top;
           c = c_top;
-          for (int l = top; l >= lo; --l) {
-            X[(l - l_lo) * kXL] = float(norm * f) * scale;
-            const double fn = c * f - fp;
-            fp = f;
-            f = fn;
-            c -= 2.0 * ix;
+          for (l = top; l - 1 >= lo; l -= 2) {
+            X[(l - l_lo) * kXL] = float(ns * f);
+            fp = c * f - fp;
+            c -= d;
+            X[(l - 1 - l_lo) * kXL] = float(ns * fp);
+            f = c * fp - f;
+            c -= d;
           }
+          if (l >= lo) X[(l - l_lo) * kXL] = float(ns * f);
         } else {
           // pass 1: normalisation and the last rescale
           double fp = 0.0, f = 1e-250, c = double(2 * S + 1) * ix;
