@@ -15,7 +15,7 @@
 //                      block of 16 q and a group of column tiles (4 waves x
 //                      MAXT accumulators).  Per slab of 16 points, each thread
 //                      generates j_l(q r) for one (point, q) and every l of
-//                      the group into LDS (FP32 sin/cos; FP64 upward
+//                      the group into LDS (FP32 sin/cos; FP32 upward
 //                      recurrence for l <= q r; FP64 Miller downward
 //                      recurrence from l + 14 above it, normalised on the
 //                      upward value, two passes so no step divides), then the
@@ -129,7 +129,8 @@ __device__ inline double recip(double a) {
 
 // One (point, q) column of spherical Bessel values: X[(l - l_lo) * kXL] =
 // j_l(x) for l in [l_lo, min(lq, l_hi)] and 0 for the rest of [l_lo, l_hi].
-// l <= floor(x): upward recurrence from j_0, j_1 (stable there).  Above:
+// l <= floor(x): FP32 upward recurrence from j_0, j_1 (formed in FP64 from
+// the FP32 sine and cosine; the recurrence is stable there).  Above:
 // Miller's downward recurrence f_{l-1} = (2l+1)/x f_l - f_{l+1} started at
 // S = top + 14 (f = 1e-250, 0) and normalised by f at floor(x) against the
 // upward value.  Pass 1 runs it down to floor(x) for the normalisation and
@@ -159,15 +160,18 @@ __device__ inline void bessel_column(float xf, int lq, int l_lo, int l_hi, float
       double jlu = j0;
       if (lu >= 1) {
         if (l_lo <= 1) X[(1 - l_lo) * kXL] = float(j1) * scale;
-        double jm = j0, jc = j1, c = 3.0 * ix;  // c = (2l + 1) / x at l = 1
+        // FP32 upward recurrence; (2l + 1) / x is formed from the exact odd
+        // integer every step, so the coefficient carries one rounding.
+        const float ixf = float(ix);
+        float jm = float(j0), jc = float(j1), odd = 3.f;
         for (int l = 1; l < lu; ++l) {
-          const double jn = c * jc - jm;
+          const float jn = (odd * ixf) * jc - jm;
           jm = jc;
           jc = jn;
-          c += 2.0 * ix;
-          if (l + 1 >= l_lo) X[(l + 1 - l_lo) * kXL] = float(jn) * scale;
+          odd += 2.f;
+          if (l + 1 >= l_lo) X[(l + 1 - l_lo) * kXL] = jn * scale;
         }
-        jlu = jc;
+        jlu = double(jc);
       }
       if (top > lu) {
         const int S = top + 14;

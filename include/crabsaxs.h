@@ -343,8 +343,8 @@ csx_status csx_combine_solvent_basis(const double *basis, const double *q,
                                      double *out);
 
 /* ---- multipole path ------------------------------------------------------ */
-/* All three run the FP32 MFMA contraction with an exact j_l (FP64 upward
- * recurrence for l <= q r, FP64 downward ratio recurrence above) and the
+/* All three run the FP32 MFMA contraction with an exact j_l (FP32 upward
+ * recurrence for l <= q r, FP64 Miller downward recurrence above) and the
  * per-q expansion order clamp(ceil(q R_max) + 8, min(8, l_max), l_max) of
  * spherical.rs:201-203.  0 <= l_max <= 64.  Empty structure -> zeros. */
 
