@@ -378,11 +378,16 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(NCH ==
         for (int u = 0; u < kB; ++u)
           yb[(b + 1) & 1][u] = *reinterpret_cast<const v4f*>(Ys + yrow[(b + 1) * kB + u]);
       }
+      // both tiles' A operands are read before either tile's MFMAs (offsets
+      // clamped to the group's orders, so the reads are in bounds)
+      v4f xa[kB];
+#pragma unroll
+      for (int u = 0; u < kB; ++u) xa[u] = *reinterpret_cast<const v4f*>(X + xoff[b * kB + u]);
 #pragma unroll
       for (int u = 0; u < kB; ++u) {
         const int t = b * kB + u;
         if (t >= nt) continue;
-        const v4f a = *reinterpret_cast<const v4f*>(X + xoff[t]);
+        const v4f a = xa[u];
         const v4f y = yb[b & 1][u];
 #pragma unroll
         for (int c = 0; c < NCH; ++c)
