@@ -73,7 +73,7 @@ def test_adaptive_grid_basis_matches_debye(gpu_ctx):
         assert err <= 1e-3, (t, err)
 
 
-@pytest.mark.parametrize("l_max", [15, 25, 40])
+@pytest.mark.parametrize("l_max", [15, 25, 40, 64])
 def test_globule_matches_exact_oracle(gpu_ctx, l_max):
     s = csx.generate_globule(1200, 16.0, 1.0, seed=21)
     q = np.linspace(0.005, 0.5, 60)
@@ -103,6 +103,27 @@ def test_solvent_paths_match_exact_oracle(gpu_ctx, model):
     assert rel(got_i, want_i).max() < 1e-5
     scale = np.abs(want_b).max(axis=1, keepdims=True)
     assert (np.abs(got_b - want_b) / scale).max() < 1e-5
+
+
+@pytest.mark.parametrize("l_max", [4, 11])
+@pytest.mark.parametrize("model", [csx.HydrationModel.Analytic, csx.HydrationModel.AdaptiveGrid])
+def test_low_order_basis_matches_exact_oracle(gpu_ctx, model, l_max):
+    # at most 16 column tiles: the three-channel kernel with 4 accumulators
+    # per wave (one tile group)
+    s = csx.generate_globule(500, 12.0, 1.0, seed=24)
+    q = np.linspace(0.0, 0.3, 33)
+    p = csx.SolventParams(c1=0.98, c2=1.4, hydration_model=model,
+                          hydration_grid_spacing_angstrom=4.0)
+    sites, vol = (None, 0.0)
+    if model == csx.HydrationModel.AdaptiveGrid:
+        sites, vol = O.adaptive_grid(s.xyz, spacing=4.0)
+    want = O.multipole_solvent(oatoms(s), q, l_max, c1=0.98, c2=1.4, sites=sites,
+                               voxel_volume=vol, basis=True, exact=True)
+    got = csx.multipole_solvent_basis(s, q, csx.MultipoleParams(l_max=l_max), p)
+    scale = np.abs(want).max(axis=1, keepdims=True)
+    err = (np.abs(got - want) / scale).max()
+    print(model.name, "L", l_max, err, flush=True)
+    assert err < 1e-5
 
 
 def test_empty_and_single(gpu_ctx):
