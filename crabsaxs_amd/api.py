@@ -281,6 +281,8 @@ class Structure:
         context, created under a lock and kept until the structure is freed,
         so threads may share a structure."""
         ctx = ctx or default_context()
+        if not ctx.ptr:
+            raise InvalidInput("the context has been closed")
         entry = self._handles.get(id(ctx))
         if entry is not None and entry[0] is ctx:
             return entry[1]
@@ -359,6 +361,8 @@ class Context:
         _check(_lib.load().csx_ctx_set_persistence(self.ptr, workgroups_per_cu))
 
     def close(self):
+        """Free the context and the device temporaries it keeps.  Structures
+        used on it may outlive it but are not used on it again."""
         if self.ptr:
             _lib.load().csx_ctx_destroy(self.ptr)
             self.ptr = C.c_void_p()

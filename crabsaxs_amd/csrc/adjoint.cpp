@@ -93,7 +93,8 @@ static csx_status vjp(Call& call, const csx_structure* s, PointSet* ps, const do
   }
   const size_t ipl = std::max<size_t>(1, std::min(nitems, budget / per_item));
   const hipStream_t st = call.stream();
-  DevBuf d_q, d_ab, d_blocks, d_items, d_part, d_acc, d_out;
+  Scratch d_q(call), d_ab(call), d_blocks(call), d_items(call), d_part(call), d_acc(call),
+      d_out(call);
   CSX_HIP(d_q.alloc(nq_pad * sizeof(float), st), "hipMallocAsync");
   CSX_HIP(d_ab.alloc(AB.size() * sizeof(float), st), "hipMallocAsync");
   CSX_HIP(d_blocks.alloc(blocks.size() * sizeof(int), st), "hipMallocAsync");
@@ -147,7 +148,7 @@ static csx_status jacobian_run(Call& call, const csx_structure* s, PointSet* ps,
   }
   const hipStream_t st = call.stream();
   const size_t out_elems = nq_pad * 3 * n;
-  DevBuf d_q, d_w, d_out;
+  Scratch d_q(call), d_w(call), d_out(call);
   CSX_HIP(d_q.alloc(nq_pad * sizeof(float), st), "hipMallocAsync");
   CSX_HIP(d_w.alloc(Wp.size() * sizeof(float), st), "hipMallocAsync");
   CSX_HIP(d_out.alloc(out_elems * sizeof(double), st), "hipMallocAsync");

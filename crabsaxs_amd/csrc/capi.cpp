@@ -229,7 +229,8 @@ static csx_status pair_tables_batch(Call& call, const std::vector<const PointSet
   std::vector<float> qrev(nq_pad, 0.0f);
   for (size_t k = 0; k < nq; ++k) qrev[k] = static_cast<float>(q[k] / (2.0 * kPi));
   const hipStream_t st = call.stream();
-  DevBuf d_q, d_partial, d_out, d_counter, d_pts, d_items, d_bins;
+  Scratch d_q(call), d_partial(call), d_out(call), d_counter(call), d_pts(call), d_items(call),
+      d_bins(call);
   CSX_HIP(d_q.alloc(nq_pad * sizeof(float), st), "hipMallocAsync");
   CSX_HIP(d_partial.alloc(nitems * nq_pad * sizeof(double), st), "hipMallocAsync");
   CSX_HIP(d_out.alloc(nbins * nq_pad * sizeof(double), st), "hipMallocAsync");
@@ -395,7 +396,7 @@ static csx_status plain_f64(Call& call, const PointSet& ps, const double* q, siz
   for (int c = 0; c < nch; ++c)
     std::copy(tables[c]->begin(), tables[c]->end(), table.begin() + size_t(c) * TT * nq);
   const int nblk = (npts + 255) / 256;
-  DevBuf d_table, d_amp, d_q, d_partial, d_out;
+  Scratch d_table(call), d_amp(call), d_q(call), d_partial(call), d_out(call);
   CSX_HIP(d_table.alloc(table.size() * sizeof(double), st), "hipMallocAsync");
   CSX_HIP(d_amp.alloc(size_t(nch) * nq * npts * sizeof(double), st), "hipMallocAsync");
   CSX_HIP(d_q.alloc(nq * sizeof(double), st), "hipMallocAsync");
@@ -534,6 +535,9 @@ void csx_ctx_destroy(csx_ctx* ctx) {
   if (!ctx) return;
   (void)hipSetDevice(ctx->device);
   for (hipStream_t s : ctx->free_streams) (void)hipStreamDestroy(s);
+  for (auto& entry : ctx->scratch)
+    for (auto& slot : entry.second)
+      if (slot.first) (void)hipFree(slot.first);
   delete ctx;
 }
 
@@ -628,6 +632,7 @@ csx_status csx_structure_create(csx_ctx* ctx, size_t n, const double* xyz, const
   std::unique_ptr<csx_structure> s(new (std::nothrow) csx_structure);
   if (!s) return fail(CSX_OUT_OF_MEMORY, "structure allocation failed");
   s->ctx = ctx;
+  s->device = ctx->device;
   s->n = n;
   s->xyz.assign(xyz, xyz + 3 * n);
   s->occ.assign(occupancy, occupancy + n);
@@ -668,7 +673,7 @@ csx_status csx_structure_create(csx_ctx* ctx, size_t n, const double* xyz, const
 
 void csx_structure_destroy(csx_structure* s) {
   if (!s) return;
-  (void)hipSetDevice(s->ctx->device);
+  (void)hipSetDevice(s->device);
   delete s;
 }
 

@@ -48,6 +48,9 @@ struct csx_ctx {
   int cu_count = 256;
   size_t lds_per_workgroup = 64 * 1024;  // the device's per-workgroup LDS limit
   int last_launch[3] = {0, 0, 0};  // q tile, sin mode, persistent workgroups
+  // Grow-only device buffers kept per stream (see csx::Scratch); freed with
+  // the context.
+  std::map<hipStream_t, std::vector<std::pair<void*, size_t>>> scratch;
 };
 
 namespace csx {
@@ -144,6 +147,7 @@ struct PointSet {
 
 struct csx_structure {
   csx_ctx* ctx = nullptr;
+  int device = 0;  // the context's device; destroy needs no live context
   size_t n = 0;
   std::vector<double> xyz, occ;
   std::vector<uint8_t> element;
@@ -211,11 +215,69 @@ class Call {
   hipStream_t stream() const { return stream_; }
   KernelTimer* timer() const { return timer_.get(); }
   csx_ctx* ctx() const { return ctx_; }
+  // The call's next grow-only buffer of at least `bytes` (see Scratch).
+  hipError_t take_scratch(size_t bytes, void** out) {
+    if (!cache_) {
+      std::lock_guard<std::mutex> lock(ctx_->mu);
+      cache_ = &ctx_->scratch[stream_];  // map nodes are stable
+    }
+    if (next_ == cache_->size()) cache_->push_back({nullptr, 0});
+    auto& slot = (*cache_)[next_++];
+    if (slot.second < bytes) {
+      if (slot.first) {
+        // hipFree waits for the device, so no queued work still uses it
+        const hipError_t e = hipFree(slot.first);
+        slot = {nullptr, 0};
+        if (e != hipSuccess) return e;
+      }
+      const size_t rounded = (bytes + (size_t(1) << 20) - 1) >> 20 << 20;
+      const hipError_t e = hipMalloc(&slot.first, rounded);
+      if (e != hipSuccess) {
+        slot.first = nullptr;
+        return e;
+      }
+      slot.second = rounded;
+    }
+    *out = slot.first;
+    return hipSuccess;
+  }
 
  private:
   csx_ctx* ctx_;
   hipStream_t stream_ = nullptr;
   std::unique_ptr<KernelTimer> timer_;
+  std::vector<std::pair<void*, size_t>>* cache_ = nullptr;
+  size_t next_ = 0;
+};
+
+// Device buffer of one call taken from grow-only buffers kept with the
+// call's stream, so that a call allocates and frees nothing once its sizes
+// have been seen.  A stream serves one call at a time, so the buffers are the
+// call's alone, and work that an earlier call queued on the same stream runs
+// before this call's.  Requests above kScratchCacheLimit are allocated and
+// freed on the stream like DevBuf, so a context does not keep them.
+class Scratch {
+ public:
+  static constexpr size_t kScratchCacheLimit = size_t(1) << 30;
+  explicit Scratch(Call& call) : call_(call) {}
+  Scratch(const Scratch&) = delete;
+  Scratch& operator=(const Scratch&) = delete;
+  hipError_t alloc(size_t bytes, hipStream_t s) {
+    reset();
+    if (bytes > kScratchCacheLimit) return big_.alloc(bytes, s);
+    return call_.take_scratch(bytes == 0 ? 16 : bytes, &p_);
+  }
+  void reset() {
+    big_.reset();
+    p_ = nullptr;
+  }
+  template <class T>
+  T* as() const { return static_cast<T*>(p_ ? p_ : big_.as<void>()); }
+
+ private:
+  Call& call_;
+  void* p_ = nullptr;
+  DevBuf big_;
 };
 
 #define CSX_HIP(expr, what)                          \

@@ -944,7 +944,8 @@ csx_status csx_ensemble_fit_core_device(csx_ctx* ctx, const double* bases, size_
   Call call(ctx);
   CSX_TRY(call.begin());
   const hipStream_t st = call.stream();
-  DevBuf d_b, d_g, d_c2, d_y, d_s, d_curves, d_ws, d_sort, d_w, d_lin, d_fit;
+  Scratch d_b(call), d_g(call), d_c2(call), d_y(call), d_s(call), d_curves(call), d_ws(call),
+      d_sort(call), d_w(call), d_lin(call), d_fit(call);
   CSX_HIP(d_b.alloc(m * 6 * n * sizeof(double), st), "hipMallocAsync");
   CSX_HIP(d_g.alloc(g.size() * sizeof(double), st), "hipMallocAsync");
   CSX_HIP(d_c2.alloc(kCand * sizeof(double), st), "hipMallocAsync");
@@ -982,7 +983,7 @@ csx_status csx_ensemble_fit_core_device(csx_ctx* ctx, const double* bases, size_
                          hipMemcpyDeviceToHost, st), "hipMemcpyAsync");
   CSX_HIP(hipMemcpyAsync(fit.data(), d_fit.as<void>(), fit.size() * sizeof(double),
                          hipMemcpyDeviceToHost, st), "hipMemcpyAsync");
-  for (DevBuf* b : {&d_b, &d_g, &d_c2, &d_y, &d_s, &d_curves, &d_ws, &d_sort, &d_w, &d_lin, &d_fit})
+  for (Scratch* b : {&d_b, &d_g, &d_c2, &d_y, &d_s, &d_curves, &d_ws, &d_sort, &d_w, &d_lin, &d_fit})
     b->reset();
   CSX_TRY(call.finish());
   // the first candidate with the lowest chi^2, as the host loop picks it

@@ -218,7 +218,7 @@ static csx_status multipole_run(csx_structure* s, const double* q, size_t nq, in
   const int npts = mode == 0 ? ps->natoms : g->npts;
   const int nchunk = (g->npad + multipole_chunk() - 1) / multipole_chunk();
   const hipStream_t st = call.stream();
-  DevBuf d_amp, d_q, d_l, d_partial, d_coef, d_out;
+  Scratch d_amp(call), d_q(call), d_l(call), d_partial(call), d_coef(call), d_out(call);
   CSX_HIP(d_amp.alloc(amp.size() * sizeof(float), st), "hipMallocAsync");
   CSX_HIP(d_q.alloc(nqp * sizeof(float), st), "hipMallocAsync");
   CSX_HIP(d_l.alloc(nqp * sizeof(int), st), "hipMallocAsync");

@@ -105,7 +105,7 @@ csx_status csx_coordinate_features(csx_structure* s, size_t bins_in, double* rg,
   Call call(s->ctx);
   CSX_TRY(call.begin());
   const hipStream_t st = call.stream();
-  DevBuf d_pts, d_items, d_max, d_partial, d_hist;
+  Scratch d_pts(call), d_items(call), d_max(call), d_partial(call), d_hist(call);
   CSX_HIP(d_pts.alloc(pts.size() * sizeof(double), st), "hipMallocAsync");
   CSX_HIP(d_items.alloc(items.size() * sizeof(int), st), "hipMallocAsync");
   CSX_HIP(d_max.alloc(sizeof(unsigned long long), st), "hipMallocAsync");

@@ -105,7 +105,11 @@ void csx_compute_options_default(csx_compute_options *out);
 /* Number of visible HIP devices (0 when none; never fails). */
 int csx_device_count(void);
 
-/* Create a context bound to one HIP device.  CSX_NO_DEVICE when absent. */
+/* Create a context bound to one HIP device.  CSX_NO_DEVICE when absent.
+ * A context keeps, per stream, the device temporaries of its calls (grown on
+ * demand, each at most 1 GiB; larger ones are freed after the call) until it
+ * is destroyed.  Structures created on a context are not used after its
+ * destroy; they may be destroyed before or after it. */
 csx_status csx_ctx_create(int device, csx_ctx **out);
 void csx_ctx_destroy(csx_ctx *ctx);
 

@@ -38,3 +38,28 @@ def test_threads_sharing_structures_match_sequential(gpu_ctx):
     for k, res in got:
         for a, b in zip(res, sequential[k]):
             assert np.array_equal(a, b)
+
+
+def test_reused_and_regrown_scratch_give_identical_results():
+    """A context keeps each stream's temporaries and grows them on demand: a
+    small call, a larger one (the buffers are regrown) and the small call
+    again give bitwise the first result, on a fresh context and its destroy."""
+    ctx = csx.Context(0)
+    try:
+        small = csx.generate_globule(600, 12.0, 1.0, seed=50)
+        large = csx.generate_globule(5000, 26.0, 1.0, seed=51)
+        q_small = np.linspace(0.01, 0.4, 24)
+        q_large = np.linspace(0.0, 0.8, 300)
+
+        def run(s, q):
+            return [csx.debye_intensity(s, q, ctx=ctx),
+                    csx.multipole_intensity(s, q, csx.MultipoleParams(l_max=30), ctx=ctx),
+                    csx.coordinate_features(s, 48, ctx=ctx).p_r.p]
+
+        first = run(small, q_small)
+        run(large, q_large)
+        again = run(small, q_small)
+        for a, b in zip(first, again):
+            assert np.array_equal(a, b)
+    finally:
+        ctx.close()
