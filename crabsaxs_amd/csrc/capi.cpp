@@ -58,11 +58,12 @@ static csx_status build_point_set(csx_structure* s, const csx_solvent_params* p,
   const size_t npts = n + ps.nsites;
   if (npts > static_cast<size_t>(INT32_MAX / 2)) return fail(CSX_INVALID_INPUT, "too many points");
 
-  // Type-sorted FP32 layout.
-  std::vector<size_t> order(n);
-  std::iota(order.begin(), order.end(), size_t(0));
-  std::stable_sort(order.begin(), order.end(),
-                   [&](size_t a, size_t b) { return s->type_of[a] < s->type_of[b]; });
+  // Type-sorted FP32 layout: a counting sort, stable, so atoms of one type
+  // keep their input order.
+  std::vector<size_t> order(n), next(T + 1, 0);
+  for (size_t i = 0; i < n; ++i) ++next[s->type_of[i] + 1];
+  for (int t = 0; t < T; ++t) next[t + 1] += next[t];
+  for (size_t i = 0; i < n; ++i) order[next[s->type_of[i]]++] = i;
   std::vector<float> pts32(4 * npts);
   std::vector<int> seg_start(TT + 1, 0);
   ps.self.assign(TT, 0.0);
@@ -644,16 +645,20 @@ csx_status csx_structure_create(csx_ctx* ctx, size_t n, const double* xyz, const
     if (s->group[i] < -1 || s->group[i] >= CSX_NUM_UA_GROUPS)
       return fail(CSX_INVALID_INPUT, "united-atom group out of range");
   }
-  // Type partition: unique (element, group) keys in sorted order.
-  std::vector<TypeKey> keys(n);
-  for (size_t i = 0; i < n; ++i) keys[i] = {s->element[i], s->group[i]};
-  s->types = keys;
-  std::sort(s->types.begin(), s->types.end());
-  s->types.erase(std::unique(s->types.begin(), s->types.end()), s->types.end());
+  // Type partition: the (element, group) keys present, in sorted order,
+  // from a table over the small key range (element, then group from -1).
+  constexpr int kGroups = CSX_NUM_UA_GROUPS + 1;
+  std::vector<int> slot(static_cast<size_t>(CSX_NUM_ELEMENTS) * kGroups, -1);
+  auto key_of = [&](size_t i) { return size_t(s->element[i]) * kGroups + (s->group[i] + 1); };
+  for (size_t i = 0; i < n; ++i) slot[key_of(i)] = 0;
+  for (int e = 0; e < CSX_NUM_ELEMENTS; ++e)
+    for (int g = 0; g < kGroups; ++g)
+      if (slot[size_t(e) * kGroups + g] == 0) {
+        slot[size_t(e) * kGroups + g] = static_cast<int>(s->types.size()) + 1;
+        s->types.push_back({e, g - 1});
+      }
   s->type_of.resize(n);
-  for (size_t i = 0; i < n; ++i)
-    s->type_of[i] = static_cast<int>(std::lower_bound(s->types.begin(), s->types.end(), keys[i]) -
-                                     s->types.begin());
+  for (size_t i = 0; i < n; ++i) s->type_of[i] = slot[key_of(i)] - 1;
   // Centroid and maximum radius, as compute_curve does (scattering.rs:157-167).
   for (size_t i = 0; i < n; ++i)
     for (int a = 0; a < 3; ++a) s->centre[a] += s->xyz[3 * i + a];

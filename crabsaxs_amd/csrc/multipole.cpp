@@ -1,6 +1,8 @@
 // Multipole path (spherical.rs:266-565): geometry cache, per-q expansion
 // orders, amplitude tables and the C-ABI entry points.
 #include <algorithm>
+#include <cstdint>
+#include <cstring>
 #include <numeric>
 #include <cmath>
 
@@ -80,17 +82,29 @@ static csx_status get_geometry(Call& call, csx_structure* s, PointSet& ps, int L
   // have nearly equal radii, so the lanes generating j_l(q r) for them run
   // the Bessel recurrences with nearly equal trip counts.  The sums over
   // points do not depend on the order.
-  auto centre_r2 = [&](const double* p) {
-    double r2 = 0.0;
-    for (int a = 0; a < 3; ++a) r2 += (p[a] - s->centre[a]) * (p[a] - s->centre[a]);
-    return r2;
-  };
+  // The order is by FP32 r^2, ties by index: an LSD radix sort (3 passes of
+  // 11 bits) of the non-negative FP32 bit patterns, which order like the
+  // values, on keys that start in index order.
   auto by_radius = [&](const double* pts, int count) {
-    std::vector<double> r2(count);
-    for (int i = 0; i < count; ++i) r2[i] = centre_r2(pts + 3 * i);
+    std::vector<uint64_t> key(count), tmp(count);
+    for (int i = 0; i < count; ++i) {
+      double r2 = 0.0;
+      for (int a = 0; a < 3; ++a) r2 += (pts[3 * i + a] - s->centre[a]) * (pts[3 * i + a] - s->centre[a]);
+      const float f = static_cast<float>(r2);
+      uint32_t bits;
+      std::memcpy(&bits, &f, sizeof bits);
+      key[i] = (uint64_t(bits) << 32) | uint32_t(i);
+    }
+    for (int pass = 0; pass < 3; ++pass) {
+      const int shift = 32 + 11 * pass;
+      std::vector<int> first(2049, 0);
+      for (int i = 0; i < count; ++i) ++first[((key[i] >> shift) & 2047) + 1];
+      for (int b = 0; b < 2048; ++b) first[b + 1] += first[b];
+      for (int i = 0; i < count; ++i) tmp[first[(key[i] >> shift) & 2047]++] = key[i];
+      key.swap(tmp);
+    }
     std::vector<int> order(count);
-    std::iota(order.begin(), order.end(), 0);
-    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return r2[a] < r2[b]; });
+    for (int i = 0; i < count; ++i) order[i] = static_cast<int>(key[i] & 0xffffffffu);
     return order;
   };
   const std::vector<int> atom_order = by_radius(s->xyz.data(), n);
