@@ -105,13 +105,15 @@ class DevArray {
   void* p_ = nullptr;
 };
 
-// Multipole geometry of one point set at one l_max: radii, types and
-// occupancies in original order (atoms, then sites) and the FP32 real
-// spherical harmonics in MFMA B-operand tiles.
+// Multipole geometry of one point set at one l_max: the points in segments
+// by type (atoms, then sites), each ordered by radius and padded to whole
+// slabs, and the FP32 real spherical harmonics in MFMA B-operand tiles.
 struct MultipoleGeometry {
-  int L = -1, ntiles = 0, npad = 0, npts = 0;
+  int L = -1, ntiles = 0, npad = 0, npts = 0, nchunk = 0;
   double rmax_atoms = 0.0, rmax_sites = 0.0;
-  DevArray radius, type, occ, Y, tile_l;
+  // Per padded point: radius, occupancy (0 for padding); chunks: (first
+  // slab, end slab, type) of each one-type chunk of the coefficient kernel.
+  DevArray radius, occ, chunks, Y, tile_l;
   std::vector<int> h_tile_l;  // l of each column tile
 };
 

@@ -11,18 +11,21 @@
 namespace csx {
 
 int multipole_chunk();
-int multipole_tiles_per_group(int nch, int ntiles);
-size_t multipole_coeff_lds_bytes(int l_range, int nch, int ntypes);
+int multipole_tiles_per_group(int ntiles);
+size_t multipole_coeff_lds_bytes(int l_range, int nch);
 hipError_t launch_multipole_geometry(const double* xyz, int npts, const double* centre, int l_max,
                                      const double* norms, const int* tile_first, int ntiles,
                                      int npad, float* radius, float* Y, hipStream_t stream);
-hipError_t launch_multipole_coeff(const float* radius, const int* type, const float* occ, int npts,
-                                  int npad, const float* Y, int ntiles, const int* tile_l,
-                                  int max_l_range, const float* amp, int ntypes, const float* qv,
-                                  const int* qlmax, int nqp, int nch, float* partial, double* coef,
-                                  double* out, hipStream_t stream, KernelTimer* timer);
+hipError_t launch_multipole_coeff(const float* radius, const float* occ, const int* chunks,
+                                  int nchunk, int npad, const float* Y, int ntiles,
+                                  const int* tile_l, int max_l_range, const float* amp,
+                                  const float* qv, const int* qlmax, int nqp, int nch,
+                                  float* partial, double* coef, double* out, hipStream_t stream,
+                                  KernelTimer* timer);
 
 namespace {
+
+constexpr int kSlabPoints = 16;  // points per harmonic-tile slab
 
 // Largest l range of a column-tile group (the LDS rows one workgroup needs).
 int max_group_l_range(const std::vector<int>& tile_l, int tiles_per_group) {
@@ -68,7 +71,6 @@ static csx_status get_geometry(Call& call, csx_structure* s, PointSet& ps, int L
   g->L = L;
   const int n = ps.natoms, ns = ps.nsites;
   g->npts = n + ns;
-  g->npad = (g->npts + 15) / 16 * 16;
   std::vector<int> tile_first(L + 2), tile_l;
   for (int l = 0; l <= L; ++l) {
     tile_first[l] = static_cast<int>(tile_l.size());
@@ -77,58 +79,81 @@ static csx_status get_geometry(Call& call, csx_structure* s, PointSet& ps, int L
   tile_first[L + 1] = static_cast<int>(tile_l.size());
   g->ntiles = static_cast<int>(tile_l.size());
   g->h_tile_l = tile_l;
-  // Atoms, then hydration sites (the vacuum path reads the atoms only), each
-  // segment ordered by distance from the centre: the points of a slab then
-  // have nearly equal radii, so the lanes generating j_l(q r) for them run
-  // the Bessel recurrences with nearly equal trip counts.  The sums over
-  // points do not depend on the order.
-  // The order is by FP32 r^2, ties by index: an LSD radix sort (3 passes of
-  // 11 bits) of the non-negative FP32 bit patterns, which order like the
-  // values, on keys that start in index order.
-  auto by_radius = [&](const double* pts, int count) {
-    std::vector<uint64_t> key(count), tmp(count);
-    for (int i = 0; i < count; ++i) {
-      double r2 = 0.0;
-      for (int a = 0; a < 3; ++a) r2 += (pts[3 * i + a] - s->centre[a]) * (pts[3 * i + a] - s->centre[a]);
-      const float f = static_cast<float>(r2);
-      uint32_t bits;
-      std::memcpy(&bits, &f, sizeof bits);
-      key[i] = (uint64_t(bits) << 32) | uint32_t(i);
-    }
-    for (int pass = 0; pass < 3; ++pass) {
-      const int shift = 32 + 11 * pass;
-      std::vector<int> first(2049, 0);
-      for (int i = 0; i < count; ++i) ++first[((key[i] >> shift) & 2047) + 1];
-      for (int b = 0; b < 2048; ++b) first[b + 1] += first[b];
-      for (int i = 0; i < count; ++i) tmp[first[(key[i] >> shift) & 2047]++] = key[i];
-      key.swap(tmp);
-    }
-    std::vector<int> order(count);
-    for (int i = 0; i < count; ++i) order[i] = static_cast<int>(key[i] & 0xffffffffu);
-    return order;
-  };
-  const std::vector<int> atom_order = by_radius(s->xyz.data(), n);
-  const std::vector<int> site_order = by_radius(ps.site_xyz.data(), ns);
-  std::vector<double> xyz(3 * static_cast<size_t>(g->npts));
-  std::vector<int> type(g->npts);
-  std::vector<float> occ(g->npts);
-  for (int k = 0; k < n; ++k) {
-    const int i = atom_order[k];
-    for (int a = 0; a < 3; ++a) xyz[3 * k + a] = s->xyz[3 * i + a];
-    type[k] = s->type_of[i];
-    occ[k] = static_cast<float>(s->occ[i]);
-  }
+  // Points (atoms, then hydration sites as type T) in segments by type, each
+  // segment ordered by distance from the centre and padded to whole slabs of
+  // 16 with zero-occupancy points at the centre.  Every chunk of the
+  // coefficient kernel then holds one type, whose amplitudes factor out of
+  // its sum, and the lanes generating j_l(q r) for one slab see nearly equal
+  // radii (nearly equal recurrence trip counts).  The sums over points do not
+  // depend on the order.
+  // The order is by (type, FP32 r^2, index): an LSD radix sort of keys that
+  // start in index order, 3 passes of 11 bits over the non-negative FP32 bit
+  // patterns (which order like the values), then a stable pass by type.
   const int T = static_cast<int>(s->types.size());
-  for (int k = 0; k < ns; ++k) {
-    const int i = site_order[k];
-    for (int a = 0; a < 3; ++a) xyz[3 * (n + k) + a] = ps.site_xyz[3 * i + a];
-    type[n + k] = T;
-    occ[n + k] = 1.0f;
+  const int npts = g->npts;
+  auto point = [&](int i) { return i < n ? &s->xyz[3 * size_t(i)] : &ps.site_xyz[3 * size_t(i - n)]; };
+  auto type_of = [&](int i) { return i < n ? s->type_of[i] : T; };
+  std::vector<uint64_t> key(npts), tmp(npts);
+  for (int i = 0; i < npts; ++i) {
+    const double* p = point(i);
+    double r2 = 0.0;
+    for (int a = 0; a < 3; ++a) r2 += (p[a] - s->centre[a]) * (p[a] - s->centre[a]);
+    const float f = static_cast<float>(r2);
+    uint32_t bits;
+    std::memcpy(&bits, &f, sizeof bits);
+    key[i] = (uint64_t(bits) << 32) | uint32_t(i);
   }
-  for (int i = 0; i < g->npts; ++i) {
+  for (int pass = 0; pass < 3; ++pass) {
+    const int shift = 32 + 11 * pass;
+    std::vector<int> first(2049, 0);
+    for (int i = 0; i < npts; ++i) ++first[((key[i] >> shift) & 2047) + 1];
+    for (int b = 0; b < 2048; ++b) first[b + 1] += first[b];
+    for (int i = 0; i < npts; ++i) tmp[first[(key[i] >> shift) & 2047]++] = key[i];
+    key.swap(tmp);
+  }
+  std::vector<int> seg(T + 2, 0);  // segment starts in the unpadded order
+  for (int i = 0; i < npts; ++i) ++seg[type_of(i) + 1];
+  for (int t = 0; t <= T; ++t) seg[t + 1] += seg[t];
+  {
+    std::vector<int> next(seg.begin(), seg.end() - 1);
+    for (int i = 0; i < npts; ++i) {
+      const int idx = static_cast<int>(key[i] & 0xffffffffu);
+      tmp[next[type_of(idx)]++] = key[i];
+    }
+    key.swap(tmp);
+  }
+  // Padded layout and the chunk table (first slab, end slab, type).
+  std::vector<int> chunks;
+  std::vector<double> xyz;
+  std::vector<float> occ;
+  int slab = 0;
+  for (int t = 0; t <= T; ++t) {
+    const int count = seg[t + 1] - seg[t];
+    if (count == 0) continue;
+    const int slabs = (count + kSlabPoints - 1) / kSlabPoints;
+    for (int k = 0; k < slabs * kSlabPoints; ++k) {
+      if (k < count) {
+        const int idx = static_cast<int>(key[seg[t] + k] & 0xffffffffu);
+        const double* p = point(idx);
+        xyz.insert(xyz.end(), p, p + 3);
+        occ.push_back(idx < n ? static_cast<float>(s->occ[idx]) : 1.0f);
+      } else {
+        xyz.insert(xyz.end(), s->centre, s->centre + 3);
+        occ.push_back(0.0f);
+      }
+    }
+    const int per = multipole_chunk() / kSlabPoints;
+    for (int c = 0; c < slabs; c += per)
+      chunks.insert(chunks.end(), {slab + c, slab + std::min(slabs, c + per), t, 0});
+    slab += slabs;
+  }
+  g->npad = slab * kSlabPoints;
+  g->nchunk = static_cast<int>(chunks.size() / 4);
+  for (int i = 0; i < npts; ++i) {
+    const double* p = point(i);
     double r2 = 0.0;
     for (int a = 0; a < 3; ++a) {
-      const double d = xyz[3 * i + a] - s->centre[a];
+      const double d = p[a] - s->centre[a];
       r2 += d * d;
     }
     double& rm = i < n ? g->rmax_atoms : g->rmax_sites;
@@ -149,21 +174,17 @@ static csx_status get_geometry(Call& call, csx_structure* s, PointSet& ps, int L
                          hipMemcpyHostToDevice, st), "hipMemcpyAsync");
   CSX_HIP(hipMemcpyAsync(d_first.as<void>(), tile_first.data(), tile_first.size() * sizeof(int),
                          hipMemcpyHostToDevice, st), "hipMemcpyAsync");
-  {
-    std::vector<float> zero_r(g->npad, 0.0f);
-    CSX_HIP(g->radius.upload(zero_r.data(), zero_r.size() * sizeof(float)), "upload radii");
-  }
-  CSX_HIP(g->type.upload(type.data(), type.size() * sizeof(int)), "upload types");
+  CSX_HIP(g->radius.alloc(g->npad * sizeof(float)), "hipMalloc radii");
   CSX_HIP(g->occ.upload(occ.data(), occ.size() * sizeof(float)), "upload occupancy");
+  CSX_HIP(g->chunks.upload(chunks.data(), chunks.size() * sizeof(int)), "upload chunks");
   CSX_HIP(g->tile_l.upload(tile_l.data(), tile_l.size() * sizeof(int)), "upload tiles");
   for (int nch : {1, 3})
-    if (multipole_coeff_lds_bytes(
-            max_group_l_range(tile_l, multipole_tiles_per_group(nch, g->ntiles)), nch,
-            ps.ntypes) > s->ctx->lds_per_workgroup)
+    if (multipole_coeff_lds_bytes(max_group_l_range(tile_l, multipole_tiles_per_group(g->ntiles)),
+                                  nch) > s->ctx->lds_per_workgroup)
       return fail(CSX_INVALID_INPUT, "multipole l_max too large for the device's LDS");
   const size_t ybytes = static_cast<size_t>(g->ntiles) * g->npad * 16 * sizeof(float);
   CSX_HIP(g->Y.alloc(ybytes), "hipMalloc harmonics");
-  CSX_HIP(launch_multipole_geometry(d_xyz.as<double>(), g->npts, d_centre.as<double>(), L,
+  CSX_HIP(launch_multipole_geometry(d_xyz.as<double>(), g->npad, d_centre.as<double>(), L,
                                     d_norms.as<double>(), d_first.as<int>(), g->ntiles, g->npad,
                                     g->radius.as<float>(), g->Y.as<float>(), st),
           "multipole_geometry");
@@ -229,8 +250,7 @@ static csx_status multipole_run(csx_structure* s, const double* q, size_t nq, in
       else a[2] = static_cast<float>(site);
     }
   }
-  const int npts = mode == 0 ? ps->natoms : g->npts;
-  const int nchunk = (g->npad + multipole_chunk() - 1) / multipole_chunk();
+  const int nchunk = g->nchunk;
   const hipStream_t st = call.stream();
   Scratch d_amp(call), d_q(call), d_l(call), d_partial(call), d_coef(call), d_out(call);
   CSX_HIP(d_amp.alloc(amp.size() * sizeof(float), st), "hipMallocAsync");
@@ -247,12 +267,13 @@ static csx_status multipole_run(csx_structure* s, const double* q, size_t nq, in
           "hipMemcpyAsync");
   CSX_HIP(hipMemcpyAsync(d_l.as<void>(), qlmax.data(), nqp * sizeof(int), hipMemcpyHostToDevice, st),
           "hipMemcpyAsync");
-  const int l_range = max_group_l_range(g->h_tile_l, multipole_tiles_per_group(nch, g->ntiles));
-  CSX_HIP(launch_multipole_coeff(g->radius.as<float>(), g->type.as<int>(), g->occ.as<float>(), npts,
-                                 g->npad, g->Y.as<float>(), g->ntiles, g->tile_l.as<int>(),
-                                 l_range, d_amp.as<float>(), TT, d_q.as<float>(), d_l.as<int>(),
-                                 static_cast<int>(nqp), nch, d_partial.as<float>(),
-                                 d_coef.as<double>(), d_out.as<double>(), st, call.timer()),
+  const int l_range = max_group_l_range(g->h_tile_l, multipole_tiles_per_group(g->ntiles));
+  CSX_HIP(launch_multipole_coeff(g->radius.as<float>(), g->occ.as<float>(), g->chunks.as<int>(),
+                                 nchunk, g->npad, g->Y.as<float>(), g->ntiles,
+                                 g->tile_l.as<int>(), l_range, d_amp.as<float>(),
+                                 d_q.as<float>(), d_l.as<int>(), static_cast<int>(nqp), nch,
+                                 d_partial.as<float>(), d_coef.as<double>(), d_out.as<double>(),
+                                 st, call.timer()),
           "multipole_coeff");
   std::vector<double> res(nqp * nterms);
   CSX_HIP(hipMemcpyAsync(res.data(), d_out.as<void>(), res.size() * sizeof(double),

@@ -11,15 +11,17 @@
 //                      (points 4 ks + k, column n) with one 16-byte load.
 //  multipole_coeff     C_c[q][lm] = sum_i A_c,i(q) j_l(q r_i) Y_lm,i as FP32
 //                      MFMA (v_mfma_f32_16x16x4f32: 16 q rows x 16 columns,
-//                      k = 4 points).  A workgroup owns a chunk of points, a
-//                      block of 16 q and a group of column tiles (4 waves x
-//                      MAXT accumulators).  Per slab of 16 points, each thread
-//                      generates j_l(q r) for one (point, q) and every l of
-//                      the group into LDS (FP32 sin/cos; FP32 upward
-//                      recurrence for l <= q r; FP64 Miller downward
-//                      recurrence from l + 14 above it, normalised on the
-//                      upward value, two passes so no step divides), then the
-//                      waves run the MFMAs of their tiles.  Tiles above the
+//                      k = 4 points).  A workgroup owns a chunk of up to 512
+//                      points of one type, a block of 16 q and a group of
+//                      column tiles (4 waves x MAXT accumulators); the type's
+//                      amplitudes A_c(q) factor out of the chunk's sum and
+//                      multiply it once at the end.  Per slab of 16 points,
+//                      each thread generates occ_i j_l(q r_i) for one
+//                      (point, q) and every l of the group into LDS (FP32
+//                      sin/cos; FP32 upward recurrence for l <= q r; FP64
+//                      Miller downward recurrence from l + 14 above it,
+//                      normalised on the upward value, two passes so no step
+//                      divides), then the waves run the MFMAs of their tiles.  Tiles above the
 //                      block's largest per-q order are skipped.  Partial
 //                      coefficients per chunk are written in FP32.
 //  multipole_reduce    FP64 sum of the chunk partials per (q, column) over the
@@ -253,16 +255,17 @@ __device__ inline void bessel_column(float xf, int lq, int l_lo, int l_hi, float
   for (int l = max(l_zero, l_lo); l <= l_hi; ++l) X[(l - l_lo) * kXL] = 0.f;
 }
 
-// partial[((chunk * NCH + c) * nqp + q) * ncol + col]
-// At least two waves per SIMD: caps registers at 256, which also stops the
-// scheduler from hoisting every tile's operand loads.
+// partial[((chunk * NCH + c) * nqp + q) * ncol + col]: a chunk holds points
+// of one type, so the sum over its points of j_l occ Y is formed once and
+// multiplied by the type's NCH amplitudes at q on the way out.
+// At least four waves per SIMD (128 registers).
 template <int NCH, int MAXT>
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(NCH == 1 ? 4 : 3))) void multipole_coeff(
-    const float* __restrict__ radius, const int* __restrict__ type, const float* __restrict__ occ,
-    int npts, int npad, const float* __restrict__ Y, int ntiles, const int* __restrict__ tile_l,
-    const float* __restrict__ amp /* [ntypes][nqp][NCH] */, int ntypes, const float* __restrict__ qv,
-    const int* __restrict__ qlmax, int nqp, int nchunk, int ngroups,
-    float* __restrict__ partial) {
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) void multipole_coeff(
+    const float* __restrict__ radius, const float* __restrict__ occ,
+    const int4* __restrict__ chunks /* (first slab, end slab, type) */, int nchunk, int nslab,
+    const float* __restrict__ Y, int ntiles, const int* __restrict__ tile_l,
+    const float* __restrict__ amp /* [ntypes][nqp][NCH] */, const float* __restrict__ qv,
+    const int* __restrict__ qlmax, int nqp, int ngroups, float* __restrict__ partial) {
   extern __shared__ float smem[];
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -286,29 +289,22 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(NCH ==
   if (l_lo > Lb) return;  // every tile of the group is above this block's orders
   const int l_hi = min(tile_l[g1 - 1], Lb);
   const int lr = l_hi - l_lo + 1;
-  float* X = smem;                        // [lr][16 q][4 slots][4 ks]
-  float* AMP = smem + lr * kXL;           // [16 p][16 q][NCH]
-  float* ATAB = AMP + kSlab * kQB * NCH;  // [ntypes][16 q][NCH]: this block's amplitudes
-  float* PR = ATAB + ntypes * kQB * NCH;  // [kChunk] radius, occupancy, type of the
-  float* PO = PR + kChunk;                //   chunk's points (0 past npts)
-  int* PT = reinterpret_cast<int*>(PO + kChunk);
-  for (int k = threadIdx.x; k < ntypes * kQB * NCH; k += kThreads) {
-    const int t = k / (kQB * NCH), rem = k - t * (kQB * NCH);
-    ATAB[k] = amp[(size_t(t) * nqp + q0) * NCH + rem];
-  }
-  for (int k = threadIdx.x; k < kChunk; k += kThreads) {
-    const int i = chunk * kChunk + k;
-    const bool v = i < npts;
-    PR[k] = v ? radius[i] : 0.f;
-    PO[k] = v ? occ[i] : 0.f;
-    PT[k] = v ? type[i] : 0;
+  const int4 ck = chunks[chunk];
+  const int s0 = ck.x, s1 = ck.y, ty = ck.z;
+  float* X = smem;                   // [lr][16 q][4 slots][4 ks]
+  float* ATAB = X + lr * kXL;        // [16 q][NCH]: the chunk type's amplitudes
+  float* PR = ATAB + kQB * NCH;      // [kChunk] radius and occupancy of the
+  float* PO = PR + kChunk;           //   chunk's points
+  for (int k = threadIdx.x; k < kQB * NCH; k += kThreads)
+    ATAB[k] = amp[(size_t(ty) * nqp + q0) * NCH + k];
+  for (int k = threadIdx.x; k < (s1 - s0) * kSlab; k += kThreads) {
+    PR[k] = radius[s0 * kSlab + k];
+    PO[k] = occ[s0 * kSlab + k];
   }
 
-  v4f acc[MAXT][NCH];
+  v4f acc[MAXT];
 #pragma unroll
-  for (int t = 0; t < MAXT; ++t)
-#pragma unroll
-    for (int c = 0; c < NCH; ++c) acc[t][c] = v4f{0.f, 0.f, 0.f, 0.f};
+  for (int t = 0; t < MAXT; ++t) acc[t] = v4f{0.f, 0.f, 0.f, 0.f};
 
   // generation role: wave w takes q = 4 w + lane / 16 and point lane % 16 of
   // the slab.  Points are sorted by radius, so the lanes of a wave see
@@ -318,8 +314,6 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(NCH ==
   const int lg = qlmax[q0 + gq];
   // MFMA role: lane = 16 k + n
   const int mk = lane >> 4, mn = lane & 15;
-  const int nslab = npad / kSlab;
-  const int s0 = chunk * (kChunk / kSlab), s1 = min(s0 + kChunk / kSlab, nslab);
 
   // This wave's tiles are g0 + wave + 4 t.  tile_l is non-decreasing, so the
   // live ones (inside the group and at most l_hi) are t < nt; xoff[t] is the
@@ -347,30 +341,15 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(NCH ==
   static_assert(MAXT % kB == 0, "whole batches of tiles");
   for (int s = s0; s < s1; ++s) {
     __syncthreads();  // the previous slab's operands have been read
+    // j_l times the point's occupancy (padding points have occupancy 0)
     const int k = (s - s0) * kSlab + gp;
-    const bool vi = s * kSlab + gp < npts;
-    // One channel: the amplitude is multiplied into the stored j_l values, so
-    // the MFMA A operand is read as is.  Three channels: the A operand is
-    // j_l times the channel's amplitude, formed per MFMA.
-    float ap[NCH];
-#pragma unroll
-    for (int c = 0; c < NCH; ++c) ap[c] = PO[k] * ATAB[(PT[k] * kQB + gq) * NCH + c];
-    bessel_column(vi ? qg * PR[k] : 0.f, vi ? lg : -1, l_lo, l_hi, NCH == 1 ? ap[0] : 1.f,
+    bessel_column(qg * PR[k], lg, l_lo, l_hi, PO[k],
                   X + gq * 16 + 4 * xslot(gq, gp & 3) + (gp >> 2));
-    if (NCH > 1)
-#pragma unroll
-      for (int c = 0; c < NCH; ++c) AMP[(gp * kQB + gq) * NCH + c] = ap[c];
     __syncthreads();
     const float* Ys = Y + size_t(s) * 256 + lane * 4;
     v4f yb[2][kB];
 #pragma unroll
     for (int u = 0; u < kB; ++u) yb[0][u] = *reinterpret_cast<const v4f*>(Ys + yrow[u]);
-    float am[NCH][4];
-    if (NCH > 1)
-#pragma unroll
-      for (int ks = 0; ks < 4; ++ks)
-#pragma unroll
-        for (int c = 0; c < NCH; ++c) am[c][ks] = AMP[((4 * ks + mk) * kQB + mn) * NCH + c];
 #pragma unroll
     for (int b = 0; b < kNB; ++b) {
       if (b + 1 < kNB) {
@@ -390,11 +369,8 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(NCH ==
         const v4f a = xa[u];
         const v4f y = yb[b & 1][u];
 #pragma unroll
-        for (int c = 0; c < NCH; ++c)
-#pragma unroll
-          for (int ks = 0; ks < 4; ++ks)
-            acc[t][c] = __builtin_amdgcn_mfma_f32_16x16x4f32(NCH == 1 ? a[ks] : a[ks] * am[c][ks],
-                                                             y[ks], acc[t][c], 0, 0, 0);
+        for (int ks = 0; ks < 4; ++ks)
+          acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[ks], y[ks], acc[t], 0, 0, 0);
       }
     }
   }
@@ -408,7 +384,8 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(NCH ==
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int q = q0 + 4 * mk + r;
-        partial[((size_t(chunk) * NCH + c) * nqp + q) * ncol + tile * 16 + mn] = acc[t][c][r];
+        partial[((size_t(chunk) * NCH + c) * nqp + q) * ncol + tile * 16 + mn] =
+            acc[t][r] * ATAB[(4 * mk + r) * NCH + c];
       }
   }
 }
@@ -487,26 +464,24 @@ __global__ __launch_bounds__(256) void multipole_reduce(const double* __restrict
           ((scratch[k] + scratch[NT + k]) + scratch[2 * NT + k]) + scratch[3 * NT + k];
 }
 
-// Accumulator tiles per wave (4 x MAXT tiles per tile group): 16 for one
-// channel and 8 for three, or half that where the half still covers the
-// tiles in as many groups.  Fewer accumulators raise occupancy; more groups
-// generate j_l once more each.  At 108k points x 200 q, one channel, MAXT 8
-// vs 16: L = 15 (one group either way) 0.46 vs 0.52 ms; L = 25 (two groups
-// vs one) 0.99 vs 0.81 ms.
-constexpr int kMaxT1 = 16;
-constexpr int kMaxT3 = 8;
+// Accumulator tiles per wave (4 x MAXT tiles per tile group): 16, or 8
+// where 8 still covers the tiles in as many groups.  Fewer accumulators
+// raise occupancy; more groups generate j_l once more each.  At 108k points
+// x 200 q, MAXT 8 vs 16: L = 15 (one group either way) 0.46 vs 0.52 ms;
+// L = 25 (two groups vs one) 0.99 vs 0.81 ms.
+constexpr int kMaxT = 16;
 
 }  // namespace
 
 int multipole_chunk() { return kChunk; }
 
-int multipole_tiles_per_group(int nch, int ntiles) {
-  const int big = 4 * (nch == 1 ? kMaxT1 : kMaxT3), small = big / 2;
+int multipole_tiles_per_group(int ntiles) {
+  const int big = 4 * kMaxT, small = big / 2;
   return (ntiles + small - 1) / small == (ntiles + big - 1) / big ? small : big;
 }
 
-size_t multipole_coeff_lds_bytes(int l_range, int nch, int ntypes) {
-  return (size_t(l_range) * kXL + size_t(kSlab + ntypes) * kQB * nch + 3 * kChunk) * sizeof(float);
+size_t multipole_coeff_lds_bytes(int l_range, int nch) {
+  return (size_t(l_range) * kXL + size_t(kQB) * nch + 2 * kChunk) * sizeof(float);
 }
 
 hipError_t launch_multipole_geometry(const double* xyz, int npts, const double* centre,
@@ -522,34 +497,34 @@ hipError_t launch_multipole_geometry(const double* xyz, int npts, const double* 
   return hipGetLastError();
 }
 
-hipError_t launch_multipole_coeff(const float* radius, const int* type, const float* occ,
-                                  int npts, int npad, const float* Y, int ntiles,
+hipError_t launch_multipole_coeff(const float* radius, const float* occ, const int* chunks,
+                                  int nchunk, int npad, const float* Y, int ntiles,
                                   const int* tile_l, int max_l_range, const float* amp,
-                                  int ntypes, const float* qv, const int* qlmax, int nqp, int nch,
+                                  const float* qv, const int* qlmax, int nqp, int nch,
                                   float* partial, double* coef, double* out, hipStream_t stream,
                                   KernelTimer* timer) {
-  if (nqp % kQB != 0 || npad % kSlab != 0 || ntiles <= 0 || (nch != 1 && nch != 3))
+  if (nqp % kQB != 0 || npad % kSlab != 0 || ntiles <= 0 || nchunk <= 0 || (nch != 1 && nch != 3))
     return hipErrorInvalidValue;
-  const int nchunk = (npad + kChunk - 1) / kChunk;
-  const int tpg = multipole_tiles_per_group(nch, ntiles);
+  const int tpg = multipole_tiles_per_group(ntiles);
   const int ngroups = (ntiles + tpg - 1) / tpg;
   const long long total = (long long)(nqp / kQB) * nchunk * ngroups;
   if (total > INT32_MAX) return hipErrorInvalidValue;
-  const size_t lds = multipole_coeff_lds_bytes(max_l_range, nch, ntypes);
+  const size_t lds = multipole_coeff_lds_bytes(max_l_range, nch);
+  const int4* ck = reinterpret_cast<const int4*>(chunks);
+  const int nslab = npad / kSlab;
   {
     TimerScope scope(timer, "multipole_coeff", stream);
-#define CSX_MP_LAUNCH(NCH, MAXT)                                                              \
-  multipole_coeff<NCH, MAXT><<<unsigned(total), kThreads, lds, stream>>>(                       \
-      radius, type, occ, npts, npad, Y, ntiles, tile_l, amp, ntypes, qv, qlmax, nqp, nchunk,     \
-      ngroups, partial)
-    if (nch == 1 && tpg == 4 * kMaxT1)
-      CSX_MP_LAUNCH(1, kMaxT1);
+#define CSX_MP_LAUNCH(NCH, MAXT)                                                             \
+  multipole_coeff<NCH, MAXT><<<unsigned(total), kThreads, lds, stream>>>(                      \
+      radius, occ, ck, nchunk, nslab, Y, ntiles, tile_l, amp, qv, qlmax, nqp, ngroups, partial)
+    if (nch == 1 && tpg == 4 * kMaxT)
+      CSX_MP_LAUNCH(1, kMaxT);
     else if (nch == 1)
-      CSX_MP_LAUNCH(1, kMaxT1 / 2);
-    else if (tpg == 4 * kMaxT3)
-      CSX_MP_LAUNCH(3, kMaxT3);
+      CSX_MP_LAUNCH(1, kMaxT / 2);
+    else if (tpg == 4 * kMaxT)
+      CSX_MP_LAUNCH(3, kMaxT);
     else
-      CSX_MP_LAUNCH(3, kMaxT3 / 2);
+      CSX_MP_LAUNCH(3, kMaxT / 2);
 #undef CSX_MP_LAUNCH
   }
   const int ncol = ntiles * 16;
