@@ -236,8 +236,14 @@ static csx_status pair_tables_batch(Call& call, const std::vector<const PointSet
   CSX_HIP(d_partial.alloc(nitems * nq_pad * sizeof(double), st), "hipMallocAsync");
   CSX_HIP(d_out.alloc(nbins * nq_pad * sizeof(double), st), "hipMallocAsync");
   CSX_HIP(d_counter.alloc(sizeof(int), st), "hipMallocAsync");
-  CSX_HIP(hipMemcpyAsync(d_q.as<void>(), qrev.data(), nq_pad * sizeof(float),
-                         hipMemcpyHostToDevice, st),
+  // q and the tables go through the stream's pinned staging, so both copies
+  // are asynchronous.
+  void* h_q = nullptr;
+  void* h_out = nullptr;
+  CSX_HIP(call.take_pinned(0, nq_pad * sizeof(float), &h_q), "hipHostMalloc");
+  CSX_HIP(call.take_pinned(1, nbins * nq_pad * sizeof(double), &h_out), "hipHostMalloc");
+  std::memcpy(h_q, qrev.data(), nq_pad * sizeof(float));
+  CSX_HIP(hipMemcpyAsync(d_q.as<void>(), h_q, nq_pad * sizeof(float), hipMemcpyHostToDevice, st),
           "hipMemcpyAsync");
   const float4* pts = nullptr;
   const int4* items = nullptr;
@@ -293,11 +299,10 @@ static csx_status pair_tables_batch(Call& call, const std::vector<const PointSet
                              d_counter.as<int>(), d_partial.as<double>(), d_out.as<double>(), st,
                              call.timer()),
           "debye_pairs_f32");
-  if (overlap) overlap();
-  std::vector<double> raw(nbins * nq_pad);
-  CSX_HIP(hipMemcpyAsync(raw.data(), d_out.as<void>(), raw.size() * sizeof(double),
+  CSX_HIP(hipMemcpyAsync(h_out, d_out.as<void>(), nbins * nq_pad * sizeof(double),
                          hipMemcpyDeviceToHost, st),
           "hipMemcpyAsync");
+  if (overlap) overlap();
   d_q.reset();
   d_partial.reset();
   d_out.reset();
@@ -306,6 +311,7 @@ static csx_status pair_tables_batch(Call& call, const std::vector<const PointSet
   d_items.reset();
   d_bins.reset();
   CSX_TRY(call.finish());
+  const double* raw = static_cast<const double*>(h_out);
   P.assign(sets.size(), std::vector<double>());
   size_t bin_off = 0;
   for (size_t c = 0; c < sets.size(); ++c) {
@@ -539,6 +545,9 @@ void csx_ctx_destroy(csx_ctx* ctx) {
   for (auto& entry : ctx->scratch)
     for (auto& slot : entry.second)
       if (slot.first) (void)hipFree(slot.first);
+  for (auto& entry : ctx->pinned)
+    for (auto& slot : entry.second)
+      if (slot.first) (void)hipHostFree(slot.first);
   delete ctx;
 }
 

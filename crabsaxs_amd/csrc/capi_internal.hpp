@@ -48,9 +48,9 @@ struct csx_ctx {
   int cu_count = 256;
   size_t lds_per_workgroup = 64 * 1024;  // the device's per-workgroup LDS limit
   int last_launch[3] = {0, 0, 0};  // q tile, sin mode, persistent workgroups
-  // Grow-only device buffers kept per stream (see csx::Scratch); freed with
-  // the context.
-  std::map<hipStream_t, std::vector<std::pair<void*, size_t>>> scratch;
+  // Grow-only device buffers kept per stream (see csx::Scratch), and
+  // grow-only pinned host staging buffers per stream; freed with the context.
+  std::map<hipStream_t, std::vector<std::pair<void*, size_t>>> scratch, pinned;
 };
 
 namespace csx {
@@ -172,6 +172,9 @@ class Call {
   explicit Call(csx_ctx* ctx) : ctx_(ctx) {}
   ~Call() {
     if (stream_) {
+      // A stream goes back to the pool idle (a call that returned early may
+      // have left work queued), so the next call may overwrite its staging.
+      if (hipStreamQuery(stream_) == hipErrorNotReady) (void)hipStreamSynchronize(stream_);
       std::lock_guard<std::mutex> lock(ctx_->mu);
       ctx_->free_streams.push_back(stream_);
     }
@@ -234,6 +237,31 @@ class Call {
       }
       const size_t rounded = (bytes + (size_t(1) << 20) - 1) >> 20 << 20;
       const hipError_t e = hipMalloc(&slot.first, rounded);
+      if (e != hipSuccess) {
+        slot.first = nullptr;
+        return e;
+      }
+      slot.second = rounded;
+    }
+    *out = slot.first;
+    return hipSuccess;
+  }
+  // Pinned host staging buffer `index` of the call's stream, at least
+  // `bytes` (grow-only).  Copies through it are asynchronous; the stream is
+  // idle when a call starts, so the buffer is free to overwrite.
+  hipError_t take_pinned(size_t index, size_t bytes, void** out) {
+    std::vector<std::pair<void*, size_t>>* pins;
+    {
+      std::lock_guard<std::mutex> lock(ctx_->mu);
+      pins = &ctx_->pinned[stream_];
+    }
+    if (pins->size() <= index) pins->resize(index + 1, {nullptr, 0});
+    auto& slot = (*pins)[index];
+    if (slot.second < bytes) {
+      if (slot.first) (void)hipHostFree(slot.first);
+      slot = {nullptr, 0};
+      const size_t rounded = (bytes + 4095) / 4096 * 4096;
+      const hipError_t e = hipHostMalloc(&slot.first, rounded, hipHostMallocDefault);
       if (e != hipSuccess) {
         slot.first = nullptr;
         return e;

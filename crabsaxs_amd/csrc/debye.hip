@@ -123,7 +123,7 @@ __global__ void reduce_plain_partials(const double* __restrict__ partial, int nb
   if (idx >= nq * nt) return;
   const int qi = idx / nt, k = idx % nt;
   double s = 0.0;
-#pragma unroll 8
+#pragma unroll 32
   for (int b = 0; b < nblk; ++b) s += partial[(size_t(qi) * nblk + b) * nt + k];
   out[size_t(k) * nq + qi] = s;
 }
@@ -299,8 +299,8 @@ __global__ void reduce_bins(const double* __restrict__ partial, const int* __res
   const int qi = blockIdx.y * blockDim.x + threadIdx.x;
   if (qi >= nq_pad) return;
   double s = 0.0;
-  // unrolled so that the independent loads issue ahead of the in-order sum
-#pragma unroll 8
+  // unrolled so that 32 independent loads issue ahead of the in-order sum
+#pragma unroll 32
   for (int it = bin_first[b]; it < bin_first[b + 1]; ++it) s += partial[size_t(it) * nq_pad + qi];
   out[size_t(b) * nq_pad + qi] = s;
 }

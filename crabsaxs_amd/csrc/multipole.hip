@@ -407,7 +407,7 @@ __global__ __launch_bounds__(256) void multipole_sum_chunks(const float* __restr
   for (int c = 0; c < NCH; ++c) {
     double v = 0.0;
     if (live)
-#pragma unroll 8
+#pragma unroll 32
       for (int ch = 0; ch < nchunk; ++ch)
         v += double(partial[((size_t(ch) * NCH + c) * nqp + q) * ncol + col]);
     coef[(size_t(c) * nqp + q) * ncol + col] = v;

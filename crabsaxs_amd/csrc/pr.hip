@@ -87,7 +87,7 @@ __global__ void pr_reduce(const double* __restrict__ partial, int nitems, int bi
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= bins) return;
   double s = 0.0;
-#pragma unroll 8
+#pragma unroll 32
   for (int k = 0; k < nitems; ++k) s += partial[size_t(k) * bins + b];
   out[b] = s;
 }
