@@ -175,7 +175,7 @@ csx_status get_point_set(csx_structure* s, const csx_solvent_params* p, PointSet
 // P[c][b*nq + k] = sum_{i<j in bin b of set c} occ_i occ_j sinc(q_k r_ij).
 // Tiles instantiated for every sine mode, and those of sine mode 2 only.
 static const int kTilesAll[] = {8, 16, 32, 64};
-static const int kTilesHw[] = {8, 16, 24, 32, 40, 48, 56, 64};
+static const int kTilesHw[] = {8, 16, 24, 32, 40, 48, 50, 56, 63, 64};
 
 // q values per pair sweep: the tuned tile if it is instantiated for the sine
 // mode, else the tile with the least padded work ceil(nq / QT) (QT + c).  The
@@ -184,7 +184,7 @@ static const int kTilesHw[] = {8, 16, 24, 32, 40, 48, 56, 64};
 // QT = 64 against 4 for the q loop alone).
 static int choose_q_tile(int tuned, int sin_mode, size_t nq) {
   const int* tiles = sin_mode == 2 ? kTilesHw : kTilesAll;
-  const int ntiles = sin_mode == 2 ? 8 : 4;
+  const int ntiles = sin_mode == 2 ? 10 : 4;
   for (int k = 0; k < ntiles; ++k)
     if (tiles[k] == tuned) return tuned;
   int best = tiles[0];
@@ -575,8 +575,9 @@ csx_status csx_ctx_reset_timing(csx_ctx* ctx) {
 
 csx_status csx_ctx_set_tuning(csx_ctx* ctx, int q_tile, int sin_mode, int j_chunk) {
   if (!ctx) return fail(CSX_INVALID_INPUT, "context is null");
-  if (q_tile < 0 || q_tile > 64 || q_tile % 8 != 0)
-    return fail(CSX_INVALID_INPUT, "q_tile must be 0 (automatic) or a multiple of 8 up to 64");
+  if (q_tile < 0 || q_tile > 64 || (q_tile % 8 != 0 && q_tile != 50 && q_tile != 63))
+    return fail(CSX_INVALID_INPUT,
+                "q_tile must be 0 (automatic), a multiple of 8 up to 64, 50 or 63");
   if (sin_mode < 0 || sin_mode > 2) return fail(CSX_INVALID_INPUT, "sin_mode must be 0, 1 or 2");
   if (j_chunk < 256 || j_chunk % 256 != 0 || j_chunk > (1 << 20))
     return fail(CSX_INVALID_INPUT, "j_chunk must be a multiple of 256 in [256, 2^20]");

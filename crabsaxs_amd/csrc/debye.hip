@@ -372,7 +372,8 @@ hipError_t launch_debye_pairs(const float4* pts, const int4* items, int nitems, 
       CSX_PAIRS_CASE(16, 0) CSX_PAIRS_CASE(16, 1) CSX_PAIRS_CASE(16, 2)
       CSX_PAIRS_CASE(32, 0) CSX_PAIRS_CASE(32, 1) CSX_PAIRS_CASE(32, 2)
       CSX_PAIRS_CASE(64, 0) CSX_PAIRS_CASE(64, 1) CSX_PAIRS_CASE(64, 2)
-      CSX_PAIRS_CASE(24, 2) CSX_PAIRS_CASE(40, 2) CSX_PAIRS_CASE(48, 2) CSX_PAIRS_CASE(56, 2)
+      CSX_PAIRS_CASE(24, 2) CSX_PAIRS_CASE(40, 2) CSX_PAIRS_CASE(48, 2) CSX_PAIRS_CASE(50, 2)
+      CSX_PAIRS_CASE(56, 2) CSX_PAIRS_CASE(63, 2)
       else return hipErrorInvalidValue;
 #undef CSX_PAIRS_CASE
     }

@@ -125,9 +125,10 @@ csx_status csx_ctx_reset_timing(csx_ctx *ctx);
 
 /* Launch shape of the FP32 pair kernel (no reference counterpart; chosen by
  * the autotuner): q_tile q values per pair sweep, 0 (automatic: the tile
- * with the least padded work for the call's q count) or a multiple of 8 up
- * to 64 (sin modes 0 and 1 have the tiles 8, 16, 32 and 64; another tile
- * falls back to the automatic choice there); sin_mode 0 (software
+ * with the least padded work for the call's q count), a multiple of 8 up
+ * to 64, 50 or 63 (sin mode 2 has 8, 16, 24, 32, 40, 48, 50, 56, 63 and 64;
+ * sin modes 0 and 1 have 8, 16, 32 and 64; another tile falls back to the
+ * automatic choice); sin_mode 0 (software
  * polynomial after exact reduction in revolutions), 1 (hardware V_SIN_F32
  * after the same reduction) or 2 (V_SIN_F32 on the unreduced argument; used
  * only when |q| r_max / pi <= 250 and otherwise replaced by 1); j_chunk

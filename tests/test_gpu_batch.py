@@ -77,19 +77,20 @@ def test_ensemble_fitter_uses_batched_bases(gpu_ctx):
 def test_every_q_tile_gives_identical_sums(gpu_ctx):
     """Each q value's sum runs over the same pairs in the same order whatever
     the tile, so every instantiated tile (and the automatic choice) agrees
-    bitwise; 200 q makes the automatic choice a 40-wide tile."""
+    bitwise; 200 q makes the automatic choice a 50-wide tile (four sweeps,
+    no padding)."""
     s = csx.generate_globule(2300, 15.0, 1.0, seed=12)
     q = np.linspace(0.005, 0.5, 200)
     try:
         gpu_ctx.set_tuning(64, 2, 512)
         base = csx.debye_intensity(s, q)
-        for qt in (0, 8, 16, 24, 32, 40, 48, 56):
+        for qt in (0, 8, 16, 24, 32, 40, 48, 50, 56, 63):
             gpu_ctx.set_tuning(qt, 2, 512)
             assert np.array_equal(csx.debye_intensity(s, q), base), qt
-        assert gpu_ctx.last_launch()["q_tile"] == 56
+        assert gpu_ctx.last_launch()["q_tile"] == 63
         gpu_ctx.set_tuning(0, 2, 512)
         csx.debye_intensity(s, q)
-        assert gpu_ctx.last_launch() == {"q_tile": 40, "sin_mode": 2, "persistent_workgroups": 0}
+        assert gpu_ctx.last_launch() == {"q_tile": 50, "sin_mode": 2, "persistent_workgroups": 0}
         gpu_ctx.set_tuning(40, 1, 512)  # no 40-wide tile for the reduced sine: automatic
         csx.debye_intensity(s, q)
         assert gpu_ctx.last_launch()["q_tile"] in (8, 16, 32, 64)
