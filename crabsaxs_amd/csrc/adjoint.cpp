@@ -16,7 +16,7 @@ hipError_t launch_debye_vjp(const float4* pts, const int4* blocks, int nb, const
                             size_t items_per_launch, double* part, double* gacc, double* out,
                             hipStream_t stream, KernelTimer* timer);
 int jacobian_q_tile();
-int vjp_q_tile(size_t nq);
+int vjp_q_tile(size_t nq, int sin_mode);
 
 namespace {
 
@@ -59,16 +59,20 @@ static csx_status vjp(Call& call, const csx_structure* s, PointSet* ps, const do
   for (int I = 0; I < nb; ++I)
     for (int J = I; J < nb; ++J) items.insert(items.end(), {I, J});
   const size_t nitems = items.size() / 2;
-  const int qt = vjp_q_tile(nq);
+  // The unreduced hardware sine takes |q r / 2 pi| <= 256 revolutions.
+  double umax = 0.0;
+  for (size_t k = 0; k < nq; ++k)
+    umax = std::max(umax, std::fabs(static_cast<double>(static_cast<float>(q[k] / (2.0 * kPi)))) *
+                              2.0 * ps->max_radius);
+  const int sin_mode = umax <= 250.0 ? 2 : 1;
+  const int qt = vjp_q_tile(nq, sin_mode);
   const size_t nq_pad = (nq + qt - 1) / qt * qt;
   const std::vector<double> A = type_amplitudes_for(s, p, implicit_h, q, nq, T);
   // AB[a][b][0][k] = qe_k B_ab(k), AB[a][b][1][k] = B_ab(k) = w_k 2 A_a A_b / qe_k, where
   // qe_k = 2 pi (q_k / 2 pi)_f32 is the q the kernel's phase uses.
   std::vector<float> qrev(nq_pad, 0.0f), AB(static_cast<size_t>(T) * T * 2 * nq_pad, 0.0f);
-  double umax = 0.0;
   for (size_t k = 0; k < nq; ++k) {
     qrev[k] = static_cast<float>(q[k] / (2.0 * kPi));
-    umax = std::max(umax, std::fabs(static_cast<double>(qrev[k])) * 2.0 * ps->max_radius);
     const double qe = 2.0 * kPi * static_cast<double>(qrev[k]);
     if (qe == 0.0) continue;
     for (int a = 0; a < T; ++a)
@@ -79,7 +83,6 @@ static csx_status vjp(Call& call, const csx_structure* s, PointSet* ps, const do
         row[nq_pad + k] = static_cast<float>(B);
       }
   }
-  const int sin_mode = umax <= 250.0 ? 2 : 1;
   const int nqc = static_cast<int>(nq_pad / qt);
   // enough workgroups to fill the device: split the q chunks of small problems
   const int qsplit = static_cast<int>(std::min<size_t>(

@@ -267,9 +267,22 @@ int jacobian_q_tile() { return kJacQT; }
 // 16 or 32 q per pair sweep, whichever needs fewer cycles: about 7 per pair.q
 // (one multiply, V_SIN and V_COS at two cycles each, two FMAs) plus about 45
 // per pair (distance, correctly rounded sqrt, weights, the three sums).
-int vjp_q_tile(size_t nq) {
-  const size_t c16 = (nq + 15) / 16 * (16 * 7 + 45), c32 = (nq + 31) / 32 * (32 * 7 + 45);
-  return c32 < c16 ? 32 : 16;
+// The tile with the least modelled work: ceil(nq / QT) sweeps of QT q at 7
+// VALU cycles each plus 45 per pair, times 1.055 for the tiles of 48 and up
+// (two waves per SIMD instead of three; measured at 20k atoms x 300 q:
+// QT 32 / 48 / 60 / 64 = 13.3 / 14.0 / 12.1 / 12.9 ms).  Sine mode 1 has the
+// tiles 16 and 32 only.
+int vjp_q_tile(size_t nq, int sin_mode) {
+  static const int kTiles[] = {16, 32, 48, 60, 64};
+  const int ntiles = sin_mode == 2 ? 5 : 2;
+  int best = 16;
+  double best_cost = 1e300;
+  for (int k = 0; k < ntiles; ++k) {
+    const int qt = kTiles[k];
+    const double cost = double((nq + qt - 1) / qt) * (7.0 * qt + 45.0) * (qt >= 48 ? 1.055 : 1.0);
+    if (cost < best_cost) best = qt, best_cost = cost;
+  }
+  return best;
 }
 
 hipError_t launch_debye_jacobian(const float4* pts, const int* orig, const int* type, int npts,
@@ -291,8 +304,9 @@ hipError_t launch_debye_vjp(const float4* pts, const int4* blocks, int nb, const
                             size_t items_per_launch, double* part, double* gacc, double* out,
                             hipStream_t stream, KernelTimer* timer) {
   if (nb <= 0) return hipSuccess;
-  if ((qt != 16 && qt != 32) || nq_pad % qt != 0 || qsplit < 1 || qsplit > nq_pad / qt ||
-      items_per_launch == 0)
+  const bool hw_only = qt == 48 || qt == 60 || qt == 64;  // sine mode 2 only
+  if ((qt != 16 && qt != 32 && !(hw_only && sin_mode == 2)) || nq_pad % qt != 0 || qsplit < 1 ||
+      qsplit > nq_pad / qt || items_per_launch == 0)
     return hipErrorInvalidValue;
   const size_t nacc = size_t(nb) * 3 * kAdjBlock;
   zero_f64<<<(nacc + 255) / 256, 256, 0, stream>>>(gacc, nacc);
@@ -304,7 +318,13 @@ hipError_t launch_debye_vjp(const float4* pts, const int4* blocks, int nb, const
 #define CSX_VJP(QT_, SM_)                                                               \
   debye_vjp_f32<QT_, SM_><<<grid, kAdjBlock, 0, stream>>>(pts, blocks, items + k0, ntypes, qrev, \
                                                           AB, nq_pad, qsplit, part)
-      if (qt == 32 && sin_mode == 2)
+      if (qt == 64)
+        CSX_VJP(64, 2);
+      else if (qt == 60)
+        CSX_VJP(60, 2);
+      else if (qt == 48)
+        CSX_VJP(48, 2);
+      else if (qt == 32 && sin_mode == 2)
         CSX_VJP(32, 2);
       else if (qt == 32)
         CSX_VJP(32, 1);
