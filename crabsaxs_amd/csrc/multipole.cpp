@@ -261,12 +261,24 @@ static csx_status multipole_run(csx_structure* s, const double* q, size_t nq, in
   CSX_HIP(d_out.alloc(nqp * nterms * sizeof(double), st), "hipMallocAsync");
   CSX_HIP(d_coef.alloc(static_cast<size_t>(nch) * nqp * g->ntiles * 16 * sizeof(double), st),
           "hipMallocAsync");
-  CSX_HIP(hipMemcpyAsync(d_amp.as<void>(), amp.data(), amp.size() * sizeof(float),
-                         hipMemcpyHostToDevice, st), "hipMemcpyAsync");
-  CSX_HIP(hipMemcpyAsync(d_q.as<void>(), qv.data(), nqp * sizeof(float), hipMemcpyHostToDevice, st),
+  // Inputs and the result go through the stream's pinned staging, so the
+  // copies are asynchronous.
+  const size_t amp_bytes = amp.size() * sizeof(float), q_bytes = nqp * sizeof(float),
+               l_bytes = nqp * sizeof(int);
+  void* h_in = nullptr;
+  void* h_res = nullptr;
+  CSX_HIP(call.take_pinned(0, amp_bytes + q_bytes + l_bytes, &h_in), "hipHostMalloc");
+  CSX_HIP(call.take_pinned(1, nqp * nterms * sizeof(double), &h_res), "hipHostMalloc");
+  char* in = static_cast<char*>(h_in);
+  std::memcpy(in, amp.data(), amp_bytes);
+  std::memcpy(in + amp_bytes, qv.data(), q_bytes);
+  std::memcpy(in + amp_bytes + q_bytes, qlmax.data(), l_bytes);
+  CSX_HIP(hipMemcpyAsync(d_amp.as<void>(), in, amp_bytes, hipMemcpyHostToDevice, st),
           "hipMemcpyAsync");
-  CSX_HIP(hipMemcpyAsync(d_l.as<void>(), qlmax.data(), nqp * sizeof(int), hipMemcpyHostToDevice, st),
+  CSX_HIP(hipMemcpyAsync(d_q.as<void>(), in + amp_bytes, q_bytes, hipMemcpyHostToDevice, st),
           "hipMemcpyAsync");
+  CSX_HIP(hipMemcpyAsync(d_l.as<void>(), in + amp_bytes + q_bytes, l_bytes, hipMemcpyHostToDevice,
+                         st), "hipMemcpyAsync");
   const int l_range = max_group_l_range(g->h_tile_l, multipole_tiles_per_group(g->ntiles));
   CSX_HIP(launch_multipole_coeff(g->radius.as<float>(), g->occ.as<float>(), g->chunks.as<int>(),
                                  nchunk, g->npad, g->Y.as<float>(), g->ntiles,
@@ -275,8 +287,7 @@ static csx_status multipole_run(csx_structure* s, const double* q, size_t nq, in
                                  d_partial.as<float>(), d_coef.as<double>(), d_out.as<double>(),
                                  st, call.timer()),
           "multipole_coeff");
-  std::vector<double> res(nqp * nterms);
-  CSX_HIP(hipMemcpyAsync(res.data(), d_out.as<void>(), res.size() * sizeof(double),
+  CSX_HIP(hipMemcpyAsync(h_res, d_out.as<void>(), nqp * nterms * sizeof(double),
                          hipMemcpyDeviceToHost, st), "hipMemcpyAsync");
   d_amp.reset();
   d_q.reset();
@@ -285,6 +296,7 @@ static csx_status multipole_run(csx_structure* s, const double* q, size_t nq, in
   d_coef.reset();
   d_out.reset();
   CSX_TRY(call.finish());
+  const double* res = static_cast<const double*>(h_res);
   const double scale = 4.0 * kPi;
   for (size_t k = 0; k < nq; ++k)
     for (int t = 0; t < nterms; ++t) {
