@@ -1,6 +1,7 @@
 """Build the HIP/C++ library libcrabsaxs_hip.so in-tree for gfx950.
 
-Usage: python -m crabsaxs_amd.build  (or build() from __graft_entry__).
+Usage: python -m crabsaxs_amd.build  (or build() from __graft_entry__);
+python -m crabsaxs_amd.build --host-asan DIR for the host-sanitized variant.
 Compiles every source under csrc/ with hipcc; host-only physics code is built
 with -ffp-contract=off so no multiply-add is fused, matching the FP64
 operation order of the reference.
@@ -96,6 +97,47 @@ def build_cpp_suite() -> Path:
     return out
 
 
+def build_host_asan(out_dir: Path) -> Path:
+    """The library and the C++ suite with the host code under AddressSanitizer
+    (device code is not instrumented), into out_dir: out_dir/lib and
+    out_dir/cpp/reference_suite, which finds the fixture files through
+    out_dir/golden.  Run it with ASAN_OPTIONS=detect_leaks=0 (the HIP runtime
+    keeps allocations until exit).  The sanitizer runtime's device-allocator
+    support can stop on an internal check (dev_runtime_unloaded_) while HIP's
+    threads are torn down at exit, after the suite has finished; that check
+    is not a memory-error report."""
+    out_dir = Path(out_dir).resolve()
+    (out_dir / "lib").mkdir(parents=True, exist_ok=True)
+    (out_dir / "cpp").mkdir(parents=True, exist_ok=True)
+    golden = out_dir / "golden"
+    if not golden.exists():
+        golden.symlink_to(PKG.parent / "tests" / "golden")
+    asan = ["-Xarch_host", "-fsanitize=address", "-Xarch_host", "-fno-omit-frame-pointer"]
+    lib = out_dir / "lib" / LIB.name
+    with tempfile.TemporaryDirectory(prefix="csx_asan_") as tmp:
+        objs = []
+        for name, kind in SOURCES:
+            src = CSRC / name
+            obj = Path(tmp) / (src.name + ".o")
+            if kind.startswith("device"):
+                extra = ["-ffp-contract=off"] if kind == "device-exact" else []
+                cmd = [HIPCC, "-x", "hip", *COMMON, *DEVICE, *extra, *asan, "-c", str(src), "-o", str(obj)]
+            else:
+                cmd = [HIPCC, "-x", "c++", *COMMON, *HOST_ONLY, *asan, "-c", str(src), "-o", str(obj)]
+            _run(cmd)
+            objs.append(str(obj))
+        _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *asan, *objs, "-o", str(lib)])
+    suite = out_dir / "cpp" / "reference_suite"
+    _run([HIPCC, "-x", "c++", "-std=c++17", "-O1", "-g", *asan, "-D__HIP_PLATFORM_AMD__",
+          f"-I{PKG.parent / 'include'}", str(PKG.parent / "tests" / "cpp" / "reference_suite.cpp"),
+          "-o", str(suite), f"-L{lib.parent}", "-lcrabsaxs_hip", "-Wl,-rpath,$ORIGIN/../lib",
+          f"-Wl,-rpath-link,{ROCM}/lib"])
+    return suite
+
+
 if __name__ == "__main__":
-    print(build(verbose="-v" in sys.argv))
-    print(build_cpp_suite())
+    if "--host-asan" in sys.argv:
+        print(build_host_asan(Path(sys.argv[sys.argv.index("--host-asan") + 1])))
+    else:
+        print(build(verbose="-v" in sys.argv))
+        print(build_cpp_suite())
