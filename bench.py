@@ -68,7 +68,8 @@ def parse():
     ap.add_argument("--q-tile", type=int, default=None)
     ap.add_argument("--j-chunk", type=int, default=None)
     ap.add_argument("--persist", type=int, default=None,
-                    help="persistent workgroups per CU for the pair kernel (0: grid launch)")
+                    help="persistent workgroups per CU for the pair kernel (0: grid launch, "
+                         "-1: the library's per-call choice)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="run the headline with the library's default launch shape instead "
                          "of the autotuned one")
@@ -405,7 +406,7 @@ def main():
         print(msg, file=sys.stderr, flush=True)
 
     ctx = csx.default_context()
-    tune = dict(q_tile=0, sin_mode=2, j_chunk=512, persist=0)
+    tune = dict(q_tile=0, sin_mode=2, j_chunk=512, persist=-1)
     explicit = {k: v for k, v in (("sin_mode", args.sin_mode), ("q_tile", args.q_tile),
                                   ("j_chunk", args.j_chunk), ("persist", args.persist))
                 if v is not None}

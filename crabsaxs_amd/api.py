@@ -357,7 +357,9 @@ class Context:
         _check(_lib.load().csx_ctx_device_info(self.ptr, None, buf, 256))
         return buf.value.decode()
 
-    def set_persistence(self, workgroups_per_cu: int = 0) -> None:
+    def set_persistence(self, workgroups_per_cu: int = -1) -> None:
+        """Pair-kernel launch form: a persistent grid of this many workgroups
+        per CU, 0 for one workgroup per task, -1 (default) to choose per call."""
         _check(_lib.load().csx_ctx_set_persistence(self.ptr, workgroups_per_cu))
 
     def close(self):

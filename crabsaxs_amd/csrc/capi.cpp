@@ -178,11 +178,18 @@ static const int kTilesAll[] = {8, 16, 32, 64};
 static const int kTilesHw[] = {8, 16, 24, 32, 40, 48, 50, 56, 63, 64};
 
 // q values per pair sweep: the tuned tile if it is instantiated for the sine
-// mode, else the tile with the least padded work ceil(nq / QT) (QT + c).  The
-// per-pair work (distance, correctly rounded sqrt, reciprocal, LDS read)
-// costs about c = 4.5 q evaluations (measured: 4.7 VALU cycles per pair.q at
-// QT = 64 against 4 for the q loop alone).
-static int choose_q_tile(int tuned, int sin_mode, size_t nq) {
+// mode, else the tile QT with the least modelled time
+//   (QT + c) (ceil(nq / QT) sum_j / cus + 2 max_j),
+// sum_j and max_j the sum and the largest of the work items' j counts: the
+// sweeps spread over the CUs, plus the tail of the longest item (one
+// workgroup sweeps an item's j atoms in turn, so a structure of a few
+// thousand atoms, with few items, is bound by that length and takes narrow
+// tiles).  The per-pair work (distance, correctly rounded sqrt, reciprocal,
+// LDS read) costs about c = 12 q evaluations (measured: 4.7 VALU cycles per
+// pair.q at QT = 64 against 4 for the q loop alone).  Fitted on MI355X over
+// globules of 100 to 10,000 atoms and 50 to 500 q: the chosen tile's time
+// is within 8 % of the fastest tile's.
+static int choose_q_tile(int tuned, int sin_mode, size_t nq, double sum_j, double max_j, int cus) {
   const int* tiles = sin_mode == 2 ? kTilesHw : kTilesAll;
   const int ntiles = sin_mode == 2 ? 10 : 4;
   for (int k = 0; k < ntiles; ++k)
@@ -190,8 +197,8 @@ static int choose_q_tile(int tuned, int sin_mode, size_t nq) {
   int best = tiles[0];
   double best_cost = 1e300;
   for (int k = 0; k < ntiles; ++k) {
-    const double chunks = static_cast<double>((nq + tiles[k] - 1) / tiles[k]);
-    const double cost = chunks * (tiles[k] + 4.5);
+    const double sweeps = static_cast<double>((nq + tiles[k] - 1) / tiles[k]);
+    const double cost = (tiles[k] + 12.0) * (sweeps * sum_j / std::max(cus, 1) + 2.0 * max_j);
     if (cost < best_cost) best = tiles[k], best_cost = cost;
   }
   return best;
@@ -208,13 +215,17 @@ static csx_status pair_tables_batch(Call& call, const std::vector<const PointSet
     std::lock_guard<std::mutex> lock(call.ctx()->mu);
     tune = call.ctx()->tuning;
   }
-  double rmax = 0.0;
+  double rmax = 0.0, sum_j = 0.0, max_j = 0.0;
   size_t npts = 0, nitems = 0, nbins = 0;
   for (const PointSet* ps : sets) {
     rmax = std::max(rmax, ps->max_radius);
     npts += static_cast<size_t>(ps->natoms + ps->nsites);
     nitems += static_cast<size_t>(ps->nitems);
     nbins += static_cast<size_t>(ps->nbins);
+    for (size_t k = 3; k < ps->h_items.size(); k += 4) {
+      sum_j += ps->h_items[k];
+      max_j = std::max(max_j, static_cast<double>(ps->h_items[k]));
+    }
   }
   if (npts > static_cast<size_t>(INT32_MAX / 2) || nitems > static_cast<size_t>(INT32_MAX / 2) ||
       nbins > static_cast<size_t>(INT32_MAX / 2))
@@ -225,7 +236,8 @@ static csx_status pair_tables_batch(Call& call, const std::vector<const PointSet
                               2.0 * rmax);
   // The unreduced hardware sine takes |q r / 2 pi| <= 256 revolutions.
   const int sin_mode = tune.sin_mode == 2 && !(umax <= 250.0) ? 1 : tune.sin_mode;
-  const int qt = choose_q_tile(tune.q_tile, sin_mode, nq);
+  const int cus = call.ctx()->cu_count;
+  const int qt = choose_q_tile(tune.q_tile, sin_mode, nq, sum_j, max_j, cus);
   const size_t nq_pad = (nq + qt - 1) / qt * qt;
   std::vector<float> qrev(nq_pad, 0.0f);
   for (size_t k = 0; k < nq; ++k) qrev[k] = static_cast<float>(q[k] / (2.0 * kPi));
@@ -287,7 +299,13 @@ static csx_status pair_tables_batch(Call& call, const std::vector<const PointSet
     items = d_items.as<int4>();
     bins = d_bins.as<int>();
   }
-  const int persist_wgs = tune.persist > 0 ? tune.persist * call.ctx()->cu_count : 0;
+  // Automatic launch form: the persistent grid pays (4-8 % on large calls)
+  // once every one of its workgroups takes at least two tasks; with fewer
+  // tasks one workgroup per task starts sooner.
+  const double tasks = static_cast<double>(nitems) * static_cast<double>(nq_pad / qt);
+  const int persist_wgs = tune.persist > 0 ? tune.persist * cus
+                          : tune.persist < 0 && tasks >= 2.0 * kAutoPersist * cus ? kAutoPersist * cus
+                                                                                   : 0;
   {
     std::lock_guard<std::mutex> lock(call.ctx()->mu);
     call.ctx()->last_launch[0] = qt;
@@ -867,8 +885,8 @@ csx_status csx_ctx_device_info(csx_ctx* ctx, int* cu_count, char* arch, size_t a
 
 csx_status csx_ctx_set_persistence(csx_ctx* ctx, int workgroups_per_cu) {
   if (!ctx) return fail(CSX_INVALID_INPUT, "context is null");
-  if (workgroups_per_cu < 0 || workgroups_per_cu > 64)
-    return fail(CSX_INVALID_INPUT, "workgroups_per_cu must be in [0, 64]");
+  if (workgroups_per_cu < -1 || workgroups_per_cu > 64)
+    return fail(CSX_INVALID_INPUT, "workgroups_per_cu must be -1 (automatic) or in [0, 64]");
   std::lock_guard<std::mutex> lock(ctx->mu);
   ctx->tuning.persist = workgroups_per_cu;
   return CSX_OK;

@@ -25,6 +25,9 @@ csx_status multipole_dispatch(csx_structure* s, const double* q, size_t nq, int 
                               bool implicit_h, const csx_solvent_params* p, int precision,
                               double* out);
 
+// Workgroups per CU of the automatic persistent launch.
+constexpr int kAutoPersist = 4;
+
 // Launch shape of debye_pairs_f32.  Defaults are the fastest shape measured
 // on MI355X for a 20,000-atom x 500-q profile (all shapes meet the same
 // accuracy); the autotuner can replace them per workload.
@@ -32,7 +35,8 @@ struct Tuning {
   int q_tile = 0;      // q values per sweep of a pair (0: automatic)
   int sin_mode = 2;    // 0 software polynomial, 1 reduced + V_SIN_F32, 2 V_SIN_F32
   int j_chunk = 512;   // j atoms per work item
-  int persist = 0;     // persistent launch with this many workgroups per CU (0: grid)
+  int persist = -1;    // persistent launch with this many workgroups per CU (0: grid;
+                       // -1: automatic, kAutoPersist per CU when tasks are plenty)
   size_t workspace_limit = size_t(2) << 30;  // bytes of adjoint partials per launch
 };
 

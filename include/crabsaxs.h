@@ -125,7 +125,9 @@ csx_status csx_ctx_reset_timing(csx_ctx *ctx);
 
 /* Launch shape of the FP32 pair kernel (no reference counterpart; chosen by
  * the autotuner): q_tile q values per pair sweep, 0 (automatic: the tile
- * with the least padded work for the call's q count), a multiple of 8 up
+ * with the least modelled time for the call's q count and work items, the
+ * work spread over the CUs plus twice the longest item, so small structures
+ * take narrow tiles and more workgroups), a multiple of 8 up
  * to 64, 50 or 63 (sin mode 2 has 8, 16, 24, 32, 40, 48, 50, 56, 63 and 64;
  * sin modes 0 and 1 have 8, 16, 32 and 64; another tile falls back to the
  * automatic choice); sin_mode 0 (software
@@ -149,7 +151,9 @@ csx_status csx_ctx_set_workspace_limit(csx_ctx *ctx, size_t bytes);
 
 /* Persistent launch of the FP32 pair kernel: a fixed grid of
  * workgroups_per_cu x CU workgroups drains a global task counter (0 = one
- * workgroup per task).  No reference counterpart. */
+ * workgroup per task; -1, the default = automatic: 4 per CU when the call
+ * has at least twice that many tasks, else one workgroup per task).  Every
+ * launch form gives bitwise the same sums.  No reference counterpart. */
 csx_status csx_ctx_set_persistence(csx_ctx *ctx, int workgroups_per_cu);
 
 /* Compute-unit count and architecture name (e.g. "gfx950:...") of the

@@ -31,11 +31,11 @@ def test_persistent_launch_is_bitwise_identical(gpu_ctx):
     q = np.linspace(0.0, 1.0, 130)
     base = csx.batch_debye_intensity(cs, q)
     try:
-        for wgs in (1, 4, 8):
+        for wgs in (0, 1, 4, 8):
             gpu_ctx.set_persistence(wgs)
             assert np.array_equal(csx.batch_debye_intensity(cs, q), base)
     finally:
-        gpu_ctx.set_persistence(0)
+        gpu_ctx.set_persistence()
 
 
 def test_autotuner_picks_a_measured_shape(gpu_ctx, tmp_path, monkeypatch):
@@ -54,7 +54,7 @@ def test_autotuner_picks_a_measured_shape(gpu_ctx, tmp_path, monkeypatch):
         assert again["cached"] and again["q_tile"] == best["q_tile"]
     finally:
         gpu_ctx.set_tuning()
-        gpu_ctx.set_persistence(0)
+        gpu_ctx.set_persistence()
 
 
 def test_ensemble_fitter_uses_batched_bases(gpu_ctx):
@@ -77,8 +77,11 @@ def test_ensemble_fitter_uses_batched_bases(gpu_ctx):
 def test_every_q_tile_gives_identical_sums(gpu_ctx):
     """Each q value's sum runs over the same pairs in the same order whatever
     the tile, so every instantiated tile (and the automatic choice) agrees
-    bitwise; 200 q makes the automatic choice a 50-wide tile (four sweeps,
-    no padding)."""
+    bitwise.  The 2,300-atom globule has 41 work items (four element types,
+    256-atom i blocks, 512-atom j chunks): its time is set by the longest
+    item, so the automatic choice is the narrowest tile, launched one
+    workgroup per task (1,025 tasks, fewer than two per persistent
+    workgroup)."""
     s = csx.generate_globule(2300, 15.0, 1.0, seed=12)
     q = np.linspace(0.005, 0.5, 200)
     try:
@@ -90,12 +93,27 @@ def test_every_q_tile_gives_identical_sums(gpu_ctx):
         assert gpu_ctx.last_launch()["q_tile"] == 63
         gpu_ctx.set_tuning(0, 2, 512)
         csx.debye_intensity(s, q)
-        assert gpu_ctx.last_launch() == {"q_tile": 50, "sin_mode": 2, "persistent_workgroups": 0}
+        assert gpu_ctx.last_launch() == {"q_tile": 8, "sin_mode": 2, "persistent_workgroups": 0}
         gpu_ctx.set_tuning(40, 1, 512)  # no 40-wide tile for the reduced sine: automatic
         csx.debye_intensity(s, q)
         assert gpu_ctx.last_launch()["q_tile"] in (8, 16, 32, 64)
     finally:
         gpu_ctx.set_tuning()
+
+
+def test_automatic_shape_for_a_large_structure(gpu_ctx):
+    """20,000 atoms (1,671 work items) are bound by total work: the automatic
+    choice is the tile with the least padded work (50 for 200 q, four sweeps
+    without padding; 63 for 500 q), launched persistent, 4 workgroups per
+    CU, as there are more than two tasks per workgroup."""
+    s = csx.generate_globule(20000, 35.0, 1.0, seed=1)
+    cus = gpu_ctx.cu_count()
+    gpu_ctx.set_tuning()
+    gpu_ctx.set_persistence()
+    for nq, tile in ((200, 50), (500, 63)):
+        csx.debye_intensity(s, np.linspace(0.0, 1.0, nq))
+        assert gpu_ctx.last_launch() == {"q_tile": tile, "sin_mode": 2,
+                                         "persistent_workgroups": 4 * cus}, nq
 
 
 def test_ensemble_nnls_recovers_weights(gpu_ctx):
@@ -129,7 +147,7 @@ def test_batch_autotuner_applies_a_measured_shape(gpu_ctx):
         assert np.array_equal(tuned, csx.batch_debye_intensity(cs, q))
     finally:
         gpu_ctx.set_tuning()
-        gpu_ctx.set_persistence(0)
+        gpu_ctx.set_persistence()
 
 
 @pytest.mark.parametrize("model", [csx.HydrationModel.AdaptiveGrid, csx.HydrationModel.Grid])
