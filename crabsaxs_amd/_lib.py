@@ -165,6 +165,7 @@ SIGNATURES = {
                                     C.POINTER(ComputeOptionsC), c_double_p,
                                     C.POINTER(C.c_int)]),
     "csx_ctx_set_persistence": (C.c_int, [C.c_void_p, C.c_int]),
+    "csx_ctx_set_vjp_tile": (C.c_int, [C.c_void_p, C.c_int]),
     "csx_ctx_device_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.c_char_p, C.c_size_t]),
     "csx_ctx_set_workspace_limit": (C.c_int, [C.c_void_p, C.c_size_t]),
     "csx_ctx_last_launch": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int),

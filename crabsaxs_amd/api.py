@@ -357,6 +357,10 @@ class Context:
         _check(_lib.load().csx_ctx_device_info(self.ptr, None, buf, 256))
         return buf.value.decode()
 
+    def set_vjp_tile(self, q_tile: int = 0) -> None:
+        """q values per sweep of the coordinate adjoint; 0 picks per call."""
+        _check(_lib.load().csx_ctx_set_vjp_tile(self.ptr, q_tile))
+
     def set_persistence(self, workgroups_per_cu: int = -1) -> None:
         """Pair-kernel launch form: a persistent grid of this many workgroups
         per CU, 0 for one workgroup per task, -1 (default) to choose per call."""

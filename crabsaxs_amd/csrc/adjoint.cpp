@@ -16,7 +16,8 @@ hipError_t launch_debye_vjp(const float4* pts, const int4* blocks, int nb, const
                             size_t items_per_launch, double* part, double* gacc, double* out,
                             hipStream_t stream, KernelTimer* timer);
 int jacobian_q_tile();
-int vjp_q_tile(size_t nq, int sin_mode);
+int vjp_q_tile(int tuned, size_t nq, int sin_mode, size_t nitems, int cus);
+int vjp_qsplit(size_t nqc, size_t nitems);
 
 namespace {
 
@@ -65,7 +66,12 @@ static csx_status vjp(Call& call, const csx_structure* s, PointSet* ps, const do
     umax = std::max(umax, std::fabs(static_cast<double>(static_cast<float>(q[k] / (2.0 * kPi)))) *
                               2.0 * ps->max_radius);
   const int sin_mode = umax <= 250.0 ? 2 : 1;
-  const int qt = vjp_q_tile(nq, sin_mode);
+  int tuned;
+  {
+    std::lock_guard<std::mutex> lock(call.ctx()->mu);
+    tuned = call.ctx()->tuning.vjp_q_tile;
+  }
+  const int qt = vjp_q_tile(tuned, nq, sin_mode, nitems, call.ctx()->cu_count);
   const size_t nq_pad = (nq + qt - 1) / qt * qt;
   const std::vector<double> A = type_amplitudes_for(s, p, implicit_h, q, nq, T);
   // AB[a][b][0][k] = qe_k B_ab(k), AB[a][b][1][k] = B_ab(k) = w_k 2 A_a A_b / qe_k, where
@@ -85,8 +91,7 @@ static csx_status vjp(Call& call, const csx_structure* s, PointSet* ps, const do
   }
   const int nqc = static_cast<int>(nq_pad / qt);
   // enough workgroups to fill the device: split the q chunks of small problems
-  const int qsplit = static_cast<int>(std::min<size_t>(
-      static_cast<size_t>(nqc), std::max<size_t>(1, (16384 + nitems - 1) / nitems)));
+  const int qsplit = vjp_qsplit(static_cast<size_t>(nqc), nitems);
   const size_t per_item = static_cast<size_t>(qsplit) * 2 * 3 * kBlock * sizeof(double);
   // partial sums of one launch stay within the context's workspace limit
   size_t budget;

@@ -264,22 +264,42 @@ constexpr int kJacQT = 8;
 
 int jacobian_q_tile() { return kJacQT; }
 
-// 16 or 32 q per pair sweep, whichever needs fewer cycles: about 7 per pair.q
-// (one multiply, V_SIN and V_COS at two cycles each, two FMAs) plus about 45
-// per pair (distance, correctly rounded sqrt, weights, the three sums).
-// The tile with the least modelled work: ceil(nq / QT) sweeps of QT q at 7
-// VALU cycles each plus 45 per pair, times 1.055 for the tiles of 48 and up
-// (two waves per SIMD instead of three; measured at 20k atoms x 300 q:
-// QT 32 / 48 / 60 / 64 = 13.3 / 14.0 / 12.1 / 12.9 ms).  Sine mode 1 has the
-// tiles 16 and 32 only.
-int vjp_q_tile(size_t nq, int sin_mode) {
+// Workgroups per block pair: a block pair's q chunks are split over up to
+// this many workgroups, so that small problems still launch about 16384.
+int vjp_qsplit(size_t nqc, size_t nitems) {
+  nitems = std::max<size_t>(nitems, 1);
+  return static_cast<int>(std::min<size_t>(nqc, std::max<size_t>(1, (16384 + nitems - 1) / nitems)));
+}
+
+// q values per pair sweep.  A sweep of a block pair costs about 7 VALU
+// cycles per pair.q (one multiply, V_SIN and V_COS at two cycles each, two
+// FMAs) plus about 45 per pair (distance, correctly rounded sqrt, weights,
+// the three sums), times 1.055 for the tiles of 48 and up (two waves per
+// SIMD instead of three; measured at 20k atoms x 300 q: QT 32 / 48 / 60 / 64
+// = 13.3 / 14.0 / 12.1 / 12.9 ms).  The modelled time is that cost times
+// the sweeps of one workgroup times the rounds of one workgroup per CU:
+//   ceil(nqc / qsplit) ceil(nitems qsplit / cus),  nqc = ceil(nq / QT).
+// Large structures reduce to the least padded work; small ones, with few
+// block pairs, to one sweep per workgroup, which narrow tiles shorten.
+// Measured on MI355X over globules of 300 to 20,000 atoms and 50 to 300 q:
+// the chosen tile is within 3 % of the fastest.  Sine mode 1 has the tiles
+// 16 and 32 only; a tuned tile instantiated for the sine mode is taken as it
+// is.
+int vjp_q_tile(int tuned, size_t nq, int sin_mode, size_t nitems, int cus) {
   static const int kTiles[] = {16, 32, 48, 60, 64};
   const int ntiles = sin_mode == 2 ? 5 : 2;
+  for (int k = 0; k < ntiles; ++k)
+    if (kTiles[k] == tuned) return tuned;
   int best = 16;
   double best_cost = 1e300;
   for (int k = 0; k < ntiles; ++k) {
     const int qt = kTiles[k];
-    const double cost = double((nq + qt - 1) / qt) * (7.0 * qt + 45.0) * (qt >= 48 ? 1.055 : 1.0);
+    const size_t nqc = (nq + qt - 1) / qt;
+    const size_t qsplit = static_cast<size_t>(vjp_qsplit(nqc, nitems));
+    const size_t c = static_cast<size_t>(std::max(cus, 1));
+    const double sweeps = double((nqc + qsplit - 1) / qsplit);
+    const double rounds = double((nitems * qsplit + c - 1) / c);
+    const double cost = (7.0 * qt + 45.0) * (qt >= 48 ? 1.055 : 1.0) * sweeps * rounds;
     if (cost < best_cost) best = qt, best_cost = cost;
   }
   return best;

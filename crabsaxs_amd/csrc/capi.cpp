@@ -614,6 +614,15 @@ csx_status csx_ctx_set_workspace_limit(csx_ctx* ctx, size_t bytes) {
   return CSX_OK;
 }
 
+csx_status csx_ctx_set_vjp_tile(csx_ctx* ctx, int q_tile) {
+  if (!ctx) return fail(CSX_INVALID_INPUT, "context is null");
+  if (q_tile != 0 && q_tile != 16 && q_tile != 32 && q_tile != 48 && q_tile != 60 && q_tile != 64)
+    return fail(CSX_INVALID_INPUT, "vjp q_tile must be 0 (automatic), 16, 32, 48, 60 or 64");
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  ctx->tuning.vjp_q_tile = q_tile;
+  return CSX_OK;
+}
+
 csx_status csx_ctx_last_launch(csx_ctx* ctx, int* q_tile, int* sin_mode, int* persistent_wgs) {
   if (!ctx) return fail(CSX_INVALID_INPUT, "context is null");
   std::lock_guard<std::mutex> lock(ctx->mu);

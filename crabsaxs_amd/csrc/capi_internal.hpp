@@ -38,6 +38,7 @@ struct Tuning {
   int persist = -1;    // persistent launch with this many workgroups per CU (0: grid;
                        // -1: automatic, kAutoPersist per CU when tasks are plenty)
   size_t workspace_limit = size_t(2) << 30;  // bytes of adjoint partials per launch
+  int vjp_q_tile = 0;  // q values per sweep of the coordinate adjoint (0: automatic)
 };
 
 }  // namespace csx

@@ -149,6 +149,14 @@ csx_status csx_ctx_last_launch(csx_ctx *ctx, int *q_tile, int *sin_mode,
  * several launches.  No reference counterpart. */
 csx_status csx_ctx_set_workspace_limit(csx_ctx *ctx, size_t bytes);
 
+/* q values per pair sweep of the coordinate adjoint (csx_debye_grad_x and
+ * the gradient fitter): 0 (automatic, the default: the tile with the least
+ * modelled time for the call's q count and block pairs, narrow for small
+ * structures), 16, 32, 48, 60 or 64 (48 and up need the unreduced sine,
+ * |q| r_max / pi <= 250; otherwise the automatic choice among 16 and 32).
+ * No reference counterpart. */
+csx_status csx_ctx_set_vjp_tile(csx_ctx *ctx, int q_tile);
+
 /* Persistent launch of the FP32 pair kernel: a fixed grid of
  * workgroups_per_cu x CU workgroups drains a global task counter (0 = one
  * workgroup per task; -1, the default = automatic: 4 per CU when the call

@@ -151,6 +151,7 @@ class Context {
   void set_tuning(int q_tile = 0, int sin_mode = 2, int j_chunk = 512) {
     detail::check(csx_ctx_set_tuning(ptr_, q_tile, sin_mode, j_chunk));
   }
+  void set_vjp_tile(int q_tile = 0) { detail::check(csx_ctx_set_vjp_tile(ptr_, q_tile)); }
   void set_persistence(int workgroups_per_cu = -1) {
     detail::check(csx_ctx_set_persistence(ptr_, workgroups_per_cu));
   }

@@ -33,19 +33,32 @@ def test_jacobian_and_vjp_match_oracle(gpu_ctx, solvent):
     assert e < 1e-4
 
 
-@pytest.mark.parametrize("nq", [24, 128])
-def test_vjp_tiles_match_oracle(gpu_ctx, nq):
-    """The VJP's q tile follows the q count (24 q: 32-wide, 128 q: 64-wide;
-    40 and 300 q, above and below, take 48 and 60)."""
-    s = csx.generate_globule(1200, 15.0, 1.0, seed=52)
-    q = np.linspace(0.01, 0.9, nq)
-    solvent = csx.SolventParams(c1=0.98, c2=1.5)
-    w = np.random.default_rng(4).normal(size=q.size)
-    want = O.debye_grad_x(oatoms(s), q, w, solvent=(solvent.c1, solvent.c2, 3.0, 0.334))
-    got = csx.debye_grad_x(s, q, w, solvent)
-    e = np.abs(got - want).max() / np.abs(want).max()
-    print("vjp", nq, "q: max err", e, flush=True)
+_TILE_CASE = {}
+
+
+@pytest.mark.parametrize("tile", [0, 16, 32, 48, 60, 64])
+def test_vjp_tiles_match_oracle(gpu_ctx, tile):
+    """Every VJP q tile (0: the automatic choice, 16-wide for a structure
+    this small) against the oracle gradient; 100 q leave a partial last
+    sweep for every tile."""
+    if not _TILE_CASE:
+        s = csx.generate_globule(1200, 15.0, 1.0, seed=52)
+        q = np.linspace(0.01, 0.9, 100)
+        solvent = csx.SolventParams(c1=0.98, c2=1.5)
+        w = np.random.default_rng(4).normal(size=q.size)
+        want = O.debye_grad_x(oatoms(s), q, w, solvent=(solvent.c1, solvent.c2, 3.0, 0.334))
+        _TILE_CASE.update(s=s, q=q, solvent=solvent, w=w, want=want)
+    c = _TILE_CASE
+    try:
+        gpu_ctx.set_vjp_tile(tile)
+        got = csx.debye_grad_x(c["s"], c["q"], c["w"], c["solvent"])
+    finally:
+        gpu_ctx.set_vjp_tile()
+    e = np.abs(got - c["want"]).max() / np.abs(c["want"]).max()
+    print("vjp tile", tile, "max err", e, flush=True)
     assert e < 1e-4
+    with pytest.raises(csx.InvalidInput):
+        gpu_ctx.set_vjp_tile(40)
 
 
 def test_directional_derivative_full_size(gpu_ctx):
