@@ -32,6 +32,9 @@ the metric:
   structure   the hydration-grid builders (exterior at 1 A, adaptive at 1 A
               and 4 A) on the 20k and 100k structures, and the coordinate
               P(r) pair histogram of the 20k structure
+  small       globules of 300 to 5,000 atoms at 100 and 200 q with the
+              library's default launch shape: Debye and adjoint kernel and
+              call times, the automatic shape, error against FP64 GPU Debye
 """
 from __future__ import annotations
 
@@ -52,7 +55,7 @@ VALU_OPS_PER_PAIRQ = 4          # x = q r, sin x, * 1/(q r), accumulate
 PEAK_VALU_OPS = 256 * 64 * 2.4e9  # CUs x lanes x clock (unpacked FP32 issue)
 PEAK_FP32_MFMA = 157.3e12
 TRAFFIC_FILE = ROOT / "profiles" / "hbm_traffic.json"
-LEGS = ("fp64", "cpu", "ensemble", "multipole", "adjoint", "structure")
+LEGS = ("fp64", "cpu", "ensemble", "multipole", "adjoint", "structure", "small")
 
 
 def parse():
@@ -343,6 +346,40 @@ def structure_leg(csx, ctx, log):
                     "pairs_per_s": pairs / (kms / 1e3), "dmax": f.dmax, "rg": f.rg}}
 
 
+def small_leg(csx, ctx, log):
+    """Debye and its coordinate adjoint on small globules (300 to 5,000
+    atoms) with the library's default launch shape: the automatic q tiles and
+    launch form, which these sizes take narrow (their time is set by the
+    longest work item, not by the total work)."""
+    ctx.set_tuning()
+    ctx.set_persistence()
+    ctx.set_vjp_tile()
+    out = []
+    for n in (300, 1200, 2300, 5000):
+        s = csx.generate_globule(n, 35.0 * (n / 20000) ** (1 / 3), 1.0, seed=5)
+        for nq in (100, 200):
+            q = np.linspace(0.0, 1.0, nq)
+            csx.debye_intensity(s, q, ctx=ctx)  # point set, warm-up
+            got, dt, kms, _ = timed(ctx, "debye_pairs_f32",
+                                    lambda: csx.debye_intensity(s, q, ctx=ctx), 20)
+            launch = ctx.last_launch()
+            ref = csx.debye_intensity(s, q, precision=csx.Precision.FP64_PLAIN, ctx=ctx)
+            w = np.ones(nq)
+            qa = np.linspace(0.01, 1.0, nq)
+            csx.debye_grad_x(s, qa, w, ctx=ctx)
+            _, dta, kmsa, _ = timed(ctx, "debye_vjp_f32",
+                                    lambda: csx.debye_grad_x(s, qa, w, ctx=ctx), 10)
+            row = {"atoms": n, "nq": nq, "debye_kernel_ms": kms, "debye_wall_ms": dt * 1e3,
+                   "launch": launch, "max_rel_err_vs_fp64": rel_err(got, ref)[0],
+                   "vjp_kernel_ms": kmsa, "vjp_wall_ms": dta * 1e3}
+            out.append(row)
+            log(f"small: {n} atoms x {nq} q: Debye kernel {kms * 1e3:.1f} us "
+                f"(wall {dt * 1e3:.3f} ms, {launch}), VJP kernel {kmsa * 1e3:.1f} us")
+    return {"workload": "globules of 300-5,000 atoms, q = linspace(0, 1, nq) (VJP: "
+                        "linspace(0.01, 1, nq), unit weights), library default launch shape",
+            "cases": out}
+
+
 def adjoint_leg(csx, ctx, log):
     from crabsaxs_amd import workloads
     F64 = csx.Precision.FP64_PLAIN
@@ -497,6 +534,9 @@ def main():
         result["adjoint"] = adjoint_leg(csx, ctx, log)
     if "structure" in args.legs:
         result["structure"] = structure_leg(csx, ctx, log)
+    if "small" in args.legs:
+        result["small"] = small_leg(csx, ctx, log)
+        apply_tune()
     print(json.dumps(result), flush=True)
 
 
