@@ -13,8 +13,8 @@ hipError_t launch_debye_jacobian(const float4* pts, const int* orig, const int* 
 hipError_t launch_debye_vjp(const float4* pts, const int4* blocks, int nb, const int2* items,
                             size_t nitems, const int* orig, int ntypes, const float* qrev,
                             const float* AB, int nq_pad, int qt, int sin_mode, int qsplit,
-                            size_t items_per_launch, double* part, double* gacc, double* out,
-                            hipStream_t stream, KernelTimer* timer);
+                            int tsplit, size_t items_per_launch, double* part, double* gacc,
+                            double* out, hipStream_t stream, KernelTimer* timer);
 int jacobian_q_tile();
 int vjp_q_tile(int tuned, size_t nq, int sin_mode, size_t nitems, int cus);
 int vjp_qsplit(size_t nqc, size_t nitems);
@@ -92,7 +92,15 @@ static csx_status vjp(Call& call, const csx_structure* s, PointSet* ps, const do
   const int nqc = static_cast<int>(nq_pad / qt);
   // enough workgroups to fill the device: split the q chunks of small problems
   const int qsplit = vjp_qsplit(static_cast<size_t>(nqc), nitems);
-  const size_t per_item = static_cast<size_t>(qsplit) * 2 * 3 * kBlock * sizeof(double);
+  // and, where the q chunks alone leave most CUs idle, the 256 steps of each
+  // sweep over 2, 4 or 8 workgroups (a structure of a few thousand atoms has
+  // tens of block pairs; its time is otherwise one whole sweep)
+  const size_t wgs = nitems * static_cast<size_t>(qsplit);
+  const size_t fill = 4 * static_cast<size_t>(std::max(call.ctx()->cu_count, 1));
+  int tsplit = 1;
+  while (tsplit < 8 && 2 * wgs * static_cast<size_t>(tsplit) <= fill) tsplit *= 2;
+  const size_t per_item =
+      static_cast<size_t>(qsplit) * static_cast<size_t>(tsplit) * 2 * 3 * kBlock * sizeof(double);
   // partial sums of one launch stay within the context's workspace limit
   size_t budget;
   {
@@ -120,7 +128,8 @@ static csx_status vjp(Call& call, const csx_structure* s, PointSet* ps, const do
                          hipMemcpyHostToDevice, st), "hipMemcpyAsync");
   CSX_HIP(launch_debye_vjp(ps->pts32.as<float4>(), d_blocks.as<int4>(), nb, d_items.as<int2>(),
                            nitems, ps->orig32.as<int>(), T, d_q.as<float>(), d_ab.as<float>(),
-                           static_cast<int>(nq_pad), qt, sin_mode, qsplit, ipl, d_part.as<double>(),
+                           static_cast<int>(nq_pad), qt, sin_mode, qsplit, tsplit, ipl,
+                           d_part.as<double>(),
                            d_acc.as<double>(), d_out.as<double>(), st, call.timer()),
           "debye_vjp_f32");
   CSX_HIP(hipMemcpyAsync(out, d_out.as<void>(), n * 3 * sizeof(double), hipMemcpyDeviceToHost, st),

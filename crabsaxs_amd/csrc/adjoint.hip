@@ -47,7 +47,8 @@ __global__ __launch_bounds__(kAdjBlock) void debye_vjp_f32(
     const float4* __restrict__ pts, const int4* __restrict__ blocks /* start, len, type, 0 */,
     const int2* __restrict__ items /* (I, J), I <= J */, int ntypes,
     const float* __restrict__ qrev, const float* __restrict__ AB /* [T][T][2][nq_pad] */,
-    int nq_pad, int qsplit, double* __restrict__ part /* [items][qsplit][2][3][256] */) {
+    int nq_pad, int qsplit, int tsplit,
+    double* __restrict__ part /* [items][qsplit * tsplit][2][3][256] */) {
   __shared__ float4 tile[kAdjBlock];
   __shared__ float accj[4][3][kAdjBlock];
   __shared__ float qab[3][QT];
@@ -65,9 +66,12 @@ __global__ __launch_bounds__(kAdjBlock) void debye_vjp_f32(
   const float* A = AB + (size_t(bi.z) * ntypes + bj.z) * 2 * nq_pad;
   const float* B = A + nq_pad;
   double gi64[3] = {0.0, 0.0, 0.0}, gj64[3] = {0.0, 0.0, 0.0};
-  // this workgroup's share of the q chunks
+  // this workgroup's share of the q chunks and of the 256 steps of a sweep
+  // (blockIdx.y = q part * tsplit + step part; tsplit divides 256)
   const int nqc = nq_pad / QT;
-  const int c0 = int(blockIdx.y) * nqc / qsplit, c1 = (int(blockIdx.y) + 1) * nqc / qsplit;
+  const int qpart = int(blockIdx.y) / tsplit, tpart = int(blockIdx.y) % tsplit;
+  const int c0 = qpart * nqc / qsplit, c1 = (qpart + 1) * nqc / qsplit;
+  const int ta = tpart * (kAdjBlock / tsplit), tb = ta + kAdjBlock / tsplit;
   __syncthreads();
   for (int qc = c0 * QT; qc < c1 * QT; qc += QT) {
     // The chunk's q, A and B pass through LDS so that they sit in VGPRs:
@@ -86,8 +90,8 @@ __global__ __launch_bounds__(kAdjBlock) void debye_vjp_f32(
       bk[k] = qab[2][k];
     }
     float gi[3] = {0.f, 0.f, 0.f};
-    float4 pnext = tile[tid];  // the next step's j, read one step ahead
-    for (int t = 0; t < kAdjBlock; ++t) {
+    float4 pnext = tile[(ta + tid) & (kAdjBlock - 1)];  // the next step's j, read one step ahead
+    for (int t = ta; t < tb; ++t) {
       const int jj = (t + tid) & (kAdjBlock - 1);
       const float4 pj = pnext;
       pnext = tile[(t + 1 + tid) & (kAdjBlock - 1)];
@@ -131,7 +135,7 @@ __global__ __launch_bounds__(kAdjBlock) void debye_vjp_f32(
     }
     __syncthreads();
   }
-  double* out = part + (size_t(blockIdx.x) * qsplit + blockIdx.y) * 2 * 3 * kAdjBlock;
+  double* out = part + (size_t(blockIdx.x) * gridDim.y + blockIdx.y) * 2 * 3 * kAdjBlock;
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
     out[c * kAdjBlock + tid] = gi64[c];
@@ -321,23 +325,24 @@ hipError_t launch_debye_jacobian(const float4* pts, const int* orig, const int* 
 hipError_t launch_debye_vjp(const float4* pts, const int4* blocks, int nb, const int2* items,
                             size_t nitems, const int* orig, int ntypes, const float* qrev,
                             const float* AB, int nq_pad, int qt, int sin_mode, int qsplit,
-                            size_t items_per_launch, double* part, double* gacc, double* out,
-                            hipStream_t stream, KernelTimer* timer) {
+                            int tsplit, size_t items_per_launch, double* part, double* gacc,
+                            double* out, hipStream_t stream, KernelTimer* timer) {
   if (nb <= 0) return hipSuccess;
   const bool hw_only = qt == 48 || qt == 60 || qt == 64;  // sine mode 2 only
   if ((qt != 16 && qt != 32 && !(hw_only && sin_mode == 2)) || nq_pad % qt != 0 || qsplit < 1 ||
-      qsplit > nq_pad / qt || items_per_launch == 0)
+      qsplit > nq_pad / qt || items_per_launch == 0 ||
+      (tsplit != 1 && tsplit != 2 && tsplit != 4 && tsplit != 8))
     return hipErrorInvalidValue;
   const size_t nacc = size_t(nb) * 3 * kAdjBlock;
   zero_f64<<<(nacc + 255) / 256, 256, 0, stream>>>(gacc, nacc);
   for (size_t k0 = 0; k0 < nitems; k0 += items_per_launch) {
     const size_t k1 = std::min(nitems, k0 + items_per_launch);
-    dim3 grid(static_cast<unsigned>(k1 - k0), qsplit);
+    dim3 grid(static_cast<unsigned>(k1 - k0), qsplit * tsplit);
     {
       TimerScope scope(timer, "debye_vjp_f32", stream);
 #define CSX_VJP(QT_, SM_)                                                               \
   debye_vjp_f32<QT_, SM_><<<grid, kAdjBlock, 0, stream>>>(pts, blocks, items + k0, ntypes, qrev, \
-                                                          AB, nq_pad, qsplit, part)
+                                                          AB, nq_pad, qsplit, tsplit, part)
       if (qt == 64)
         CSX_VJP(64, 2);
       else if (qt == 60)
@@ -354,7 +359,7 @@ hipError_t launch_debye_vjp(const float4* pts, const int4* blocks, int nb, const
         CSX_VJP(16, 1);
 #undef CSX_VJP
     }
-    reduce_vjp<<<nb, kAdjBlock, 0, stream>>>(part, k0, k1, qsplit, nb, gacc);
+    reduce_vjp<<<nb, kAdjBlock, 0, stream>>>(part, k0, k1, qsplit * tsplit, nb, gacc);
   }
   scatter_vjp<<<nb, kAdjBlock, 0, stream>>>(gacc, blocks, orig, out);
   return hipGetLastError();
