@@ -54,6 +54,7 @@ N_ATOMS, RADIUS, N_Q, SEED = 20000, 35.0, 500, 1
 VALU_OPS_PER_PAIRQ = 4          # x = q r, sin x, * 1/(q r), accumulate
 PEAK_VALU_OPS = 256 * 64 * 2.4e9  # CUs x lanes x clock (unpacked FP32 issue)
 PEAK_FP32_MFMA = 157.3e12
+PEAK_FP32_VECTOR = 157.3e12     # published MI355X FP32 vector peak
 TRAFFIC_FILE = ROOT / "profiles" / "hbm_traffic.json"
 LEGS = ("fp64", "cpu", "ensemble", "multipole", "adjoint", "structure", "small")
 
@@ -141,22 +142,144 @@ def traffic_per_launch(kernel: str):
         return None
 
 
-def cpu_baseline(structure, q, threads):
-    """FP64 oracle (C restatement of scattering.rs:86-110) on a q subset."""
+BOX_CPU_SHARE = 16  # host CPUs a one-GPU job may keep busy on the MI355X machines
+
+
+def host_cores():
+    """The host's CPU inventory: what the OS reports, what this process may
+    run on (affinity mask, cgroup quota) and the CPU model (lscpu)."""
+    info = {"nproc": os.cpu_count()}
+    try:
+        info["affinity"] = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        info["affinity"] = os.cpu_count()
+    quota = None
+    try:  # cgroup v2
+        q_us, period = Path("/sys/fs/cgroup/cpu.max").read_text().split()[:2]
+        if q_us != "max":
+            quota = int(q_us) / int(period)
+    except (OSError, ValueError):
+        try:  # cgroup v1
+            q_us = int(Path("/sys/fs/cgroup/cpu/cpu.cfs_quota_us").read_text())
+            period = int(Path("/sys/fs/cgroup/cpu/cpu.cfs_period_us").read_text())
+            if q_us > 0:
+                quota = q_us / period
+        except (OSError, ValueError):
+            pass
+    info["cgroup_cpu_quota"] = quota
+    try:
+        import subprocess
+        text = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        keep = ("Model name", "Socket(s)", "Core(s) per socket", "Thread(s) per core",
+                "CPU max MHz", "Flags")
+        for line in text.splitlines():
+            key, _, val = line.partition(":")
+            if key.strip() in keep:
+                val = val.strip()
+                if key.strip() == "Flags":
+                    val = " ".join(f for f in val.split() if f.startswith("avx"))
+                info["lscpu " + key.strip()] = val
+    except (OSError, ValueError):
+        pass
+    usable = info["affinity"] or 1
+    if quota:
+        usable = min(usable, max(int(quota), 1))
+    info["usable"] = usable
+    info["threads_used"] = min(usable, BOX_CPU_SHARE)
+    return info
+
+
+def cpu_leg(structure, q, params, out32, out64):
+    """The FP64 CPU oracle (a C restatement of solvent.rs:490-584, the
+    reference's loop order: parallel over q, r_ij and sin per pair and q) on a
+    bounded q sample of the headline workload, computing the same quantity as
+    the headline (the solvent-corrected intensity): its throughput is the CPU
+    baseline, and its values give the FP32 path's relative error per q against
+    the FP64 CPU oracle."""
     sys.path.insert(0, str(ROOT / "tests"))
     import oracle_py as O
+    cores = host_cores()
+    threads = cores["threads_used"]
     atoms = O.Atoms(structure.xyz, structure.element, structure.group, structure.occupancy)
-    sample = q[np.linspace(1, len(q) - 1, threads).astype(int)]
-    O.debye_intensity(atoms, sample[:1], threads=1)  # warm-up of the loader
+    # two q per thread, spread over the grid from q = 0 to q = 1 (both included)
+    idx = np.unique(np.linspace(0, len(q) - 1, 2 * threads).round().astype(int))
+    sample = q[idx]
+    O.lib()  # load the library outside the timed region
     t0 = time.perf_counter()
-    O.debye_intensity(atoms, sample, threads=threads)
+    basis = O.debye_solvent_basis(atoms, sample, params.c1, params.c2,
+                                  params.shell_thickness_angstrom, params.bulk_density_e_per_a3,
+                                  threads=threads)
     dt = time.perf_counter() - t0
+    ref = O.combine_solvent_basis(basis, sample, params.c1, params.c2)
     n = len(structure)
     pairq = n * (n - 1) / 2 * len(sample)
-    return {"value": pairq / dt, "unit": "pair.q evals/s", "cores": threads, "kind": "port",
-            "sample": f"FP64 oracle debye_intensity on the same 20,000-atom structure at "
-                      f"{len(sample)} of the 500 q values ({pairq:.3g} pair.q, {dt:.1f} s); "
-                      f"one q per thread"}
+    e32 = np.abs(out32[idx] - ref) / np.abs(ref)
+    e64 = np.abs(out64[idx] - ref) / np.abs(ref) if out64 is not None else None
+    baseline = {
+        "value": pairq / dt, "unit": "pair.q evals/s", "cores": threads, "kind": "port",
+        "sample": f"FP64 oracle solvent_corrected_intensity (default solvent) on the same "
+                  f"20,000-atom structure at {len(sample)} of the 500 q values, q = 0 to 1 "
+                  f"({pairq:.3g} pair.q, {dt:.1f} s), q split over {threads} threads",
+        "host": cores}
+    parity = {"q": sample.round(6).tolist(),
+              "fp32_rel_err": [float(f"{v:.3g}") for v in e32],
+              "fp32_max_rel_err": float(e32.max()), "fp32_mean_rel_err": float(e32.mean())}
+    if e64 is not None:
+        parity["fp64_gpu_rel_err"] = [float(f"{v:.3g}") for v in e64]
+        parity["fp64_gpu_max_rel_err"] = float(e64.max())
+    if sample[0] == 0.0:
+        # I(0) = (sum_i A_i)^2 exactly, A_i = f_i(0) - G(0, c1) e_i(0) + c2 h_i(0):
+        # the oracle's sequential FP64 sum of 2e8 same-sign terms is off by
+        # ~N eps there, so the exact value shows which side the error is on
+        import math
+        L = O.lib()
+        g0 = L.or_excluded_volume_scale(0.0, params.c1)
+        amp = {}
+        for e in set(structure.element.tolist()):
+            v = L.or_form_factor(e, -1, 1.0, 1, 0.0)
+            vol = L.or_excluded_volume(e, -1, 1)
+            r = (3.0 * vol / (4.0 * math.pi)) ** (1.0 / 3.0)
+            shell = 4.0 * math.pi * (r + params.shell_thickness_angstrom / 2) ** 2 \
+                * params.shell_thickness_angstrom
+            amp[e] = (v - g0 * params.bulk_density_e_per_a3 * vol
+                      + params.c2 * params.bulk_density_e_per_a3 * 0.1 * shell)
+        exact = math.fsum(amp[e] * o for e, o in zip(structure.element.tolist(),
+                                                       structure.occupancy.tolist())) ** 2
+        parity["q0_exact"] = {"value": exact,
+                              "fp32_rel_err": abs(out32[0] - exact) / exact,
+                              "cpu_oracle_rel_err": abs(ref[0] - exact) / exact}
+    return baseline, parity
+
+
+def per_q_report(q, err):
+    """Relative error at every q (3 significant digits), with its maximum and
+    mean over the grid and the maximum in five q bands."""
+    edges = np.linspace(q.min(), q.max(), 6)
+    bands = []
+    for lo, hi in zip(edges[:-1], edges[1:]):
+        sel = (q >= lo) & (q <= hi)
+        bands.append({"q": [round(float(lo), 4), round(float(hi), 4)],
+                      "max": float(err[sel].max()), "mean": float(err[sel].mean())})
+    return {"q": f"linspace({q[0]:g}, {q[-1]:g}, {q.size})",
+            "rel_err": [float(f"{v:.3g}") for v in err],
+            "max": float(err.max()), "mean": float(err.mean()), "bands": bands}
+
+
+def pair_items(structure, j_chunk):
+    """Work items of the pair kernel for a structure (type-pure blocks of 256
+    i points x chunks of j_chunk j points over the upper triangle, in type
+    order) and its type count."""
+    keys = sorted(set(zip(structure.element.tolist(), structure.group.tolist())))
+    counts = [int(np.sum((structure.element == e) & (structure.group == g))) for e, g in keys]
+    start = np.concatenate([[0], np.cumsum(counts)])
+    items = 0
+    for a in range(len(keys)):
+        for b in range(a, len(keys)):
+            for i0 in range(start[a], start[a + 1], 256):
+                lo = max(start[b], i0)
+                if start[b + 1] > lo:
+                    items += -(-(start[b + 1] - lo) // j_chunk)
+    return int(items), len(keys)
 
 
 def timed(ctx, kernel, fn, repeats=1):
@@ -488,6 +611,14 @@ def main():
     value = pairq * args.steps / dt
     avg_kernel_s = kernel_ms / max(launches, 1) / 1e3
     achieved = pairq / avg_kernel_s * VALU_OPS_PER_PAIRQ
+    # HBM bytes a launch needs: the FP32 points once and the FP64 tables out;
+    # the design adds FP64 partials per (work item, q) and the points of
+    # every item's blocks (most of them from L2)
+    nitems, ntypes = pair_items(s, tune["j_chunk"])
+    nq_pad = -(-N_Q // launch["q_tile"]) * launch["q_tile"]
+    algorithmic_bytes = {"minimum": N_ATOMS * 16 + ntypes * (ntypes + 1) // 2 * N_Q * 8,
+                         "design": nitems * nq_pad * 8 + N_ATOMS * 16,
+                         "work_items": nitems}
 
     result = {
         "metric": "pair.q evals/s (FP32-compensated Debye, <=1e-5 max rel err vs FP64 GPU Debye)",
@@ -506,25 +637,41 @@ def main():
                                "q=linspace(0,1,500), default solvent, per-type S_ab tables",
                    "atoms": N_ATOMS, "q_points": N_Q, "pairs": N_ATOMS * (N_ATOMS - 1) // 2,
                    "tuning": tune, "autotuned": tuned is not None, "launch": launch},
-        "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": PEAK_VALU_OPS / 1e12,
-                     "unit": "T VALU-op/s", "frac": achieved / PEAK_VALU_OPS,
+        # The kernel issues VALU work only (no MFMA; HBM traffic is O(N) per
+        # tile row).  Against the published FP32 vector peak, a pair.q counts
+        # the 4 operations x = q r, sin x, * 1/(q r), accumulate; the issue
+        # ceiling is what those 4 operations cost on one lane each cycle.
+        "roofline": {"bound": "valu", "achieved": achieved / 1e12,
+                     "peak": PEAK_FP32_VECTOR / 1e12, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP32_VECTOR,
                      "traffic": traffic_per_launch("debye_pairs_f32"),
                      "kernel": "debye_pairs_f32", "kernel_ms": avg_kernel_s * 1e3,
-                     "ops_per_pairq": VALU_OPS_PER_PAIRQ},
+                     "ops_per_pairq": VALU_OPS_PER_PAIRQ,
+                     "algorithmic_bytes": algorithmic_bytes,
+                     "issue_ceiling": {
+                         "peak": PEAK_VALU_OPS / 1e12, "unit": "T lane-cycles/s",
+                         "frac": achieved / PEAK_VALU_OPS,
+                         "note": "256 CU x 64 lanes x 2.4 GHz, one lane-cycle per operation "
+                                 "(V_SIN_F32 takes 2): packed FP32, which the published "
+                                 "157.3 TFLOP/s counts, is disabled in this build"}},
     }
+    out64 = None
     if "fp64" in args.legs:
-        ref, dt64, ms64, _ = timed(ctx, "debye_plain_f64",
-                                   lambda: csx.solvent_corrected_intensity(s, q, params,
-                                                                           precision=F64, ctx=ctx))
-        e_max, e_mean = rel_err(out, ref)
+        out64, dt64, ms64, _ = timed(ctx, "debye_plain_f64",
+                                     lambda: csx.solvent_corrected_intensity(
+                                         s, q, params, precision=F64, ctx=ctx))
+        e = np.abs(out - out64) / np.abs(out64)
         result["fp64_gpu"] = {"value": pairq / dt64, "unit": "pair.q evals/s",
                               "kernel_ms": ms64, "speedup": value / (pairq / dt64),
-                              "max_rel_err": e_max, "mean_rel_err": e_mean}
-        log(f"fp64: {dt64:.3f} s, max rel err {e_max:.2e}")
+                              "max_rel_err": float(e.max()), "mean_rel_err": float(e.mean())}
+        result["per_q_err_vs_fp64_gpu"] = per_q_report(q, e)
+        log(f"fp64: {dt64:.3f} s, max rel err {e.max():.2e}")
     if "cpu" in args.legs:
-        threads = min(os.cpu_count() or 1, 16)
-        result["cpu_baseline"] = cpu_baseline(s, q, threads)
-        log("cpu baseline done")
+        baseline, parity = cpu_leg(s, q, params, out, out64)
+        result["cpu_baseline"] = baseline
+        result["per_q_err_vs_cpu_oracle"] = parity
+        log(f"cpu baseline: {baseline['value']:.3g} pair.q/s on {baseline['cores']} threads, "
+            f"FP32 max rel err vs the CPU oracle {parity['fp32_max_rel_err']:.2e}")
     if "ensemble" in args.legs:
         result["ensemble"] = ensemble_leg(csx, ctx, args.conformers, log)
         apply_tune()  # the other legs run with the headline shape
@@ -537,7 +684,8 @@ def main():
     if "small" in args.legs:
         result["small"] = small_leg(csx, ctx, log)
         apply_tune()
-    print(json.dumps(result), flush=True)
+    print(json.dumps(result, default=lambda o: o.item() if hasattr(o, "item") else str(o)),
+          flush=True)
 
 
 if __name__ == "__main__":

@@ -6,6 +6,7 @@ oracle at oracle-feasible sizes and against the FP64 GPU kernel at the full
 20,000-atom, 500-q precision-study size.
 """
 import json
+import math
 from pathlib import Path
 
 import numpy as np
@@ -202,6 +203,42 @@ def test_precision_study_20k_against_fp64_gpu(gpu_ctx):
     err = rel(got, ref)
     print("20k solvent max", err.max(), "mean", err.mean(), flush=True)
     assert err.max() < 1e-5
+
+
+def test_precision_study_20k_against_cpu_oracle(gpu_ctx):
+    """The 20,000-atom precision-study output against the FP64 CPU oracle
+    (solvent.rs:490-584 restated) at 8 of its 500 q, q = 1 included, where
+    the pair sum's condition number is largest: FP32 per q within 1e-5.
+
+    The oracle sums 2e8 pair terms one after another in FP64, as the
+    reference does; where they share a sign (small q) its own error reaches
+    about N eps = 2e-8, so the FP64 GPU path is held to 5e-8 against it.  At
+    q = 0, where I(0) = (sum_i A_i)^2 is known exactly from the per-atom
+    amplitudes A_i = f_i(0) - G(0, c1) e_i(0), the FP32 path (whose q = 0
+    tables are exact occupancy products) is held to 1e-12 and the FP64 GPU
+    path (blocked FP64 sums of the 2e8 terms) to 1e-10."""
+    s = csx.generate_globule(20000, 35.0, 1.0, seed=1)
+    q = np.linspace(0.0, 1.0, 500)
+    idx = np.array([0, 1, 50, 125, 250, 375, 450, 499])
+    p = csx.SolventParams()
+    got = csx.solvent_corrected_intensity(s, q, p, precision=F32)[idx]
+    got64 = csx.solvent_corrected_intensity(s, q[idx], p, precision=F64)
+    basis = O.debye_solvent_basis(oracle_atoms(s), q[idx])
+    want = O.combine_solvent_basis(basis, q[idx], p.c1, p.c2)
+    err = rel(got, want)
+    print("20k vs CPU oracle per q:", dict(zip(q[idx].round(4).tolist(), err.tolist())), flush=True)
+    print("20k FP64 GPU vs CPU oracle per q:", rel(got64, want).tolist(), flush=True)
+    assert err.max() < 1e-5
+    assert rel(got64, want).max() < 5e-8
+    L = O.lib()
+    g0 = L.or_excluded_volume_scale(0.0, p.c1)
+    amp = {e: L.or_form_factor(e, -1, 1.0, 1, 0.0)
+           - g0 * p.bulk_density_e_per_a3 * L.or_excluded_volume(e, -1, 1)
+           for e in set(s.element.tolist())}
+    exact0 = math.fsum(amp[e] for e in s.element.tolist()) ** 2
+    print("I(0): exact", exact0, "FP32", got[0], "FP64 GPU", got64[0], "oracle", want[0], flush=True)
+    assert abs(got[0] - exact0) < 1e-12 * exact0
+    assert abs(got64[0] - exact0) < 1e-10 * exact0
 
 
 @pytest.mark.parametrize("n", [1200, 20000])
