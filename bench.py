@@ -22,9 +22,10 @@ the metric:
               sampled conformers, and the weights recovered from a
               5-conformer Dirichlet(1) mixture with 1% noise
   multipole   100,000-atom 2:1 prolate globule + hydration pseudo-atoms on a
-              3 A grid 3-6 A out, 200 q, l_max 15 / 25 / 40 (and 64, where
-              the expansion converges), against FP64 GPU Debye on the same
-              points; the six-term solvent basis at l_max 25
+              3 A grid 3-6 A out as hydration sites, 200 q, l_max 15 / 25 /
+              40 (and 64, where the expansion converges), against FP64 GPU
+              Debye on the same atoms and sites, with the q range where each
+              order has converged; the six-term solvent basis at l_max 25
   adjoint     20k structure, 300 q: dI/dx vector-Jacobian product throughput,
               checked against a central difference of FP64 GPU Debye; the
               Nelder-Mead and gradient fits of a noisy target (scale 2.3,
@@ -108,20 +109,22 @@ def copy_calibration(nbytes: int) -> None:
 
 
 def multipole_profile(L: int) -> None:
-    """The multipole workload's vacuum intensity at one l_max, 1 warm-up and 3
-    timed launches and nothing else, so that counters of a profiler run
-    belong to that order only."""
+    """The multipole workload's solvent intensity (atoms and hydration
+    sites) at one l_max, 1 warm-up and 3 timed launches and nothing else, so
+    that counters of a profiler run belong to that order only."""
     import crabsaxs_amd as csx
     from crabsaxs_amd import workloads
     ctx = csx.default_context()
+    ctx.set_cache_limit(0)  # every call runs the coefficient kernel
     s, q, _ = workloads.prolate_100k()
+    p = csx.SolventParams(c1=1.0, c2=1.0, hydration_model=csx.HydrationModel.ExplicitSites)
     s.handle(ctx)
     mp = csx.MultipoleParams(l_max=L)
-    csx.multipole_intensity(s, q, mp, ctx=ctx)
+    csx.multipole_solvent_intensity(s, q, mp, p, ctx=ctx)
     ctx.reset_timing()
     ctx.set_timing(True)
     for _ in range(3):
-        csx.multipole_intensity(s, q, mp, ctx=ctx)
+        csx.multipole_solvent_intensity(s, q, mp, p, ctx=ctx)
     ctx.set_timing(False)
     n, ms = ctx.kernel_time("multipole_coeff")
     print(json.dumps({"multipole_profile_l_max": L, "points": len(s), "q": int(q.size),
@@ -378,43 +381,71 @@ def ensemble_leg(csx, ctx, m, log):
 
 
 def multipole_leg(csx, ctx, log):
+    """The multipole workload with its shell pseudo-atoms as hydration sites
+    (amplitude rho / 10 * V_voxel * f_w(q), V_voxel = 27 A^3), solvent
+    intensity at c1 = 1, c2 = 1, against FP64 GPU Debye on the same atoms and
+    sites.  The expansion of order L is exact for q r_max <= L - 8 (the
+    adaptive rule L_q = ceil(q R) + 8 of spherical.rs:201-203): the line
+    gives, for each L, the q range where L_q <= L and the error inside and
+    outside it."""
     from crabsaxs_amd import workloads
     F64 = csx.Precision.FP64_PLAIN
     s, q, nsites = workloads.prolate_100k()
-    n = len(s)
-    s_rmax = float(np.max(np.linalg.norm(s.xyz - s.xyz.mean(axis=0), axis=1)))
+    p = csx.SolventParams(c1=1.0, c2=1.0, hydration_model=csx.HydrationModel.ExplicitSites)
+    n = len(s) + nsites
+    sites = s.sites[0]
+    centre = s.xyz.mean(axis=0)
+    rmax = float(max(np.linalg.norm(s.xyz - centre, axis=1).max(),
+                     np.linalg.norm(sites - centre, axis=1).max()))
     pairq = n * (n - 1) / 2 * q.size
     s.handle(ctx)
     ref, dt64, ms64, _ = timed(ctx, "debye_plain_f64",
-                               lambda: csx.debye_intensity(s, q, precision=F64, ctx=ctx))
-    log(f"multipole: FP64 Debye {dt64:.2f} s on {n} points")
-    csx.debye_intensity(s, q, ctx=ctx)
-    d32, dt32, ms32, _ = timed(ctx, "debye_pairs_f32", lambda: csx.debye_intensity(s, q, ctx=ctx), 2)
+                               lambda: csx.solvent_corrected_intensity(s, q, p, precision=F64,
+                                                                       ctx=ctx))
+    log(f"multipole: FP64 Debye {dt64:.2f} s on {len(s)} atoms + {nsites} sites")
+    csx.solvent_corrected_intensity(s, q, p, ctx=ctx)
+    d32, dt32, ms32, _ = timed(ctx, "debye_pairs_f32",
+                               lambda: csx.solvent_corrected_intensity(s, q, p, ctx=ctx), 2)
     out = {"workload": f"100,000-atom 2:1 prolate globule + {nsites} hydration pseudo-atoms "
-                       "(3 A grid, 3-6 A out), q=linspace(0.005,0.5,200), vacuum I(q)",
-           "points": n, "fp64_debye": {"pair_q_per_s": pairq / dt64, "kernel_ms": ms64},
+                       "(3 A grid, 3-6 A out) as hydration sites (V_voxel 27 A^3), "
+                       "q=linspace(0.005,0.5,200), solvent intensity at c1 = 1, c2 = 1",
+           "atoms": len(s), "sites": nsites, "r_max": rmax,
+           "fp64_debye": {"pair_q_per_s": pairq / dt64, "kernel_ms": ms64},
            "fp32_debye": {"pair_q_per_s": pairq / dt32, "kernel_ms": ms32,
                           "max_rel_err": rel_err(d32, ref)[0]},
            "l_max": {}}
     for L in (15, 25, 40, 64):  # the config's orders, then the largest supported
         mp = csx.MultipoleParams(l_max=L)
-        csx.multipole_intensity(s, q, mp, ctx=ctx)
+        csx.multipole_solvent_intensity(s, q, mp, p, ctx=ctx)
         got, dt, kms, _ = timed(ctx, "multipole_coeff",
-                                lambda: csx.multipole_intensity(s, q, mp, ctx=ctx), 3)
-        e_max, e_mean = rel_err(got, ref)
+                                lambda: csx.multipole_solvent_intensity(s, q, mp, p, ctx=ctx), 3)
+        err = np.abs(got - ref) / np.abs(ref)
         # 2 N sum_q (L_q + 1)^2 with the per-q order clamp(ceil(q R) + 8, min(8, L), L)
-        lq = np.clip(np.ceil(q * s_rmax) + 8, min(8, L), L)
+        lq_full = np.ceil(q * rmax) + 8
+        lq = np.clip(lq_full, min(8, L), L)
         flops = 2.0 * n * float(np.sum((lq + 1) ** 2))
-        out["l_max"][str(L)] = {"wall_ms": dt * 1e3, "kernel_ms": kms,
-                                "equiv_pair_q_per_s": pairq / dt,
-                                "speedup_vs_fp64_debye": dt64 / dt,
-                                "mfma_tflops_algorithmic": flops / (kms / 1e3) / 1e12,
-                                "mfma_frac": flops / (kms / 1e3) / PEAK_FP32_MFMA,
-                                "max_rel_err": e_max, "mean_rel_err": e_mean}
-        log(f"multipole: L={L} {dt * 1e3:.1f} ms, max rel err {e_max:.2e}")
-    # The six-term solvent basis (three amplitude channels) at L = 25, checked
-    # by recombining it at (c1, c2) against the one-channel solvent intensity.
-    sp = csx.SolventParams(c1=1.02, c2=2.0)
+        conv = lq_full <= L
+        row = {"wall_ms": dt * 1e3, "kernel_ms": kms,
+               "equiv_pair_q_per_s": pairq / dt,
+               "speedup_vs_fp64_debye": dt64 / dt,
+               "mfma_tflops_algorithmic": flops / (kms / 1e3) / 1e12,
+               "mfma_frac": flops / (kms / 1e3) / PEAK_FP32_MFMA,
+               "max_rel_err": float(err.max()), "mean_rel_err": float(err.mean()),
+               "converged_q": ([round(float(q[conv].min()), 4), round(float(q[conv].max()), 4)]
+                               if conv.any() else None),
+               "converged_points": int(conv.sum()),
+               "max_rel_err_converged": float(err[conv].max()) if conv.any() else None,
+               "max_rel_err_truncated": float(err[~conv].max()) if (~conv).any() else None,
+               "per_q_rel_err": [float(f"{v:.3g}") for v in err]}
+        row["gate_1e-5_holds_where_converged"] = (row["max_rel_err_converged"] is not None
+                                                  and row["max_rel_err_converged"] <= 1e-5)
+        out["l_max"][str(L)] = row
+        log(f"multipole: L={L} {dt * 1e3:.2f} ms (kernel {kms:.3f} ms), converged for q <= "
+            f"{row['converged_q']}, max rel err there {row['max_rel_err_converged']}, "
+            f"overall {err.max():.2e}")
+    # The six-term solvent basis (three amplitude channels) at L = 25,
+    # recombined at (c1, c2) against the one-channel solvent intensity.
+    sp = csx.SolventParams(c1=1.02, c2=2.0, hydration_model=csx.HydrationModel.ExplicitSites)
     mp = csx.MultipoleParams(l_max=25)
     csx.multipole_solvent_basis(s, q, mp, sp, ctx=ctx)
     basis, dtb, kmsb, _ = timed(ctx, "multipole_coeff",
@@ -424,6 +455,17 @@ def multipole_leg(csx, ctx, log):
     out["solvent_basis_l25"] = {"wall_ms": dtb * 1e3, "kernel_ms": kmsb,
                                 "recombined_vs_direct_max_rel": rel_err(recombined, direct)[0]}
     log(f"multipole: solvent basis L=25 {dtb * 1e3:.1f} ms (kernel {kmsb:.3f} ms)")
+    # Cached coefficient tables: a call that changes only (c1, c2) re-contracts
+    # the per-type coefficients instead of running the coefficient kernel.
+    with ctx.caching(ctx.DEFAULT_CACHE_LIMIT):
+        mp = csx.MultipoleParams(l_max=40)
+        csx.multipole_solvent_intensity(s, q, mp, p, ctx=ctx)
+        cands = [csx.SolventParams(c1=c1, c2=c2, hydration_model=p.hydration_model)
+                 for c1 in (0.95, 1.0, 1.05) for c2 in (0.0, 1.0, 2.0)]
+        t0 = time.perf_counter()
+        for c in cands:
+            csx.multipole_solvent_intensity(s, q, mp, c, ctx=ctx)
+        out["cached_recontraction_l40_ms"] = (time.perf_counter() - t0) / len(cands) * 1e3
     return out
 
 
@@ -444,7 +486,7 @@ def structure_leg(csx, ctx, log):
               ("adaptive_1A", lambda t: csx.build_adaptive_hydration_grid(t, 1.0, ctx=ctx)),
               ("adaptive_4A", lambda t: csx.build_adaptive_hydration_grid(t, 4.0, ctx=ctx)))
     grids = {}
-    for name, s in (("20k", s20), (f"100k+{nsites}", s100)):
+    for name, s in (("20k", s20), ("100k", s100)):
         for kind, build in builds:
             build(fresh(s))  # warm-up
             best, sites = float("inf"), 0
@@ -463,7 +505,7 @@ def structure_leg(csx, ctx, log):
     pairs = n * (n - 1) / 2
     log(f"structure: P(r) of {n} atoms {dt * 1e3:.2f} ms (histogram kernel {kms:.3f} ms)")
     return {"workload": "20,000-atom precision-study globule and the 100,000-atom prolate "
-                        "structure with its hydration pseudo-atoms",
+                        "globule",
             "grids_wall": grids,
             "p_r": {"atoms": n, "bins": 160, "wall_ms": dt * 1e3, "hist_kernel_ms": kms,
                     "pairs_per_s": pairs / (kms / 1e3), "dmax": f.dmax, "rg": f.rg}}
@@ -566,6 +608,10 @@ def main():
         print(msg, file=sys.stderr, flush=True)
 
     ctx = csx.default_context()
+    # Every timed call computes: the library's result cache (which would
+    # return repeated calls' tables) is off, and its effect is measured
+    # separately.
+    ctx.set_cache_limit(0)
     tune = dict(q_tile=0, sin_mode=2, j_chunk=512, persist=-1)
     explicit = {k: v for k, v in (("sin_mode", args.sin_mode), ("q_tile", args.q_tile),
                                   ("j_chunk", args.j_chunk), ("persist", args.persist))
@@ -655,6 +701,26 @@ def main():
                                  "(V_SIN_F32 takes 2): packed FP32, which the published "
                                  "157.3 TFLOP/s counts, is disabled in this build"}},
     }
+    # The result cache on the same workload: the 25 (c1, c2) candidates of an
+    # ensemble fit (ensemble.rs:243-261) as repeated calls, the first
+    # computing the pair tables and the rest re-contracting them.
+    with ctx.caching(ctx.DEFAULT_CACHE_LIMIT):
+        cands = [csx.SolventParams(c1=c1, c2=c2) for c1 in np.linspace(0.95, 1.05, 5)
+                 for c2 in np.linspace(-2.0, 4.0, 5)]
+        ctx.reset_timing()
+        ctx.set_timing(True)
+        t0 = time.perf_counter()
+        for c in cands:
+            csx.solvent_corrected_intensity(s, q, c, precision=F32, ctx=ctx)
+        dtc = time.perf_counter() - t0
+        ctx.set_timing(False)
+        result["cache"] = {"calls": len(cands),
+                           "pair_kernel_launches": ctx.kernel_time("debye_pairs_f32")[0],
+                           "ms_per_call": dtc / len(cands) * 1e3,
+                           "uncached_ms_per_call": ms_per_step,
+                           "note": "25 (c1, c2) candidates through solvent_corrected_intensity "
+                                   "with the library's result cache on; the timed steps above "
+                                   "run with it off"}
     out64 = None
     if "fp64" in args.legs:
         out64, dt64, ms64, _ = timed(ctx, "debye_plain_f64",

@@ -122,6 +122,8 @@ SIGNATURES = {
                                        C.POINTER(C.c_void_p)]),
     "csx_structure_destroy": (None, [C.c_void_p]),
     "csx_structure_atom_count": (C.c_size_t, [C.c_void_p]),
+    "csx_structure_set_hydration_sites": (C.c_int, [C.c_void_p, c_double_p, C.c_size_t,
+                                                    C.c_double]),
     "csx_residue_kind": (C.c_int, [C.c_char_p]),
     "csx_parse_options_default": (None, [C.c_void_p]),
     "csx_read_structure_file": (C.c_int, [C.c_char_p, C.c_void_p, C.POINTER(C.c_void_p)]),
@@ -168,6 +170,10 @@ SIGNATURES = {
     "csx_ctx_set_vjp_tile": (C.c_int, [C.c_void_p, C.c_int]),
     "csx_ctx_device_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.c_char_p, C.c_size_t]),
     "csx_ctx_set_workspace_limit": (C.c_int, [C.c_void_p, C.c_size_t]),
+    "csx_ctx_set_cache_limit": (C.c_int, [C.c_void_p, C.c_size_t]),
+    "csx_ctx_set_vjp_split": (C.c_int, [C.c_void_p, C.c_int]),
+    "csx_ctx_last_vjp_launch": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                          C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "csx_ctx_last_launch": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int),
                                       C.POINTER(C.c_int)]),
     "csx_batch_debye_intensity": (C.c_int, [C.POINTER(C.c_void_p), C.c_size_t, c_double_p,

@@ -8,6 +8,7 @@ libcrabsaxs_hip.so on the GPU.  Errors raise SaxsError subclasses mirroring
 """
 from __future__ import annotations
 
+import contextlib
 import ctypes as C
 import enum
 import threading
@@ -105,9 +106,13 @@ class ScatteringMethod(enum.IntEnum):
 
 
 class HydrationModel(enum.IntEnum):
+    """`HydrationModel` (src/solvent.rs:17-25), plus ExplicitSites (no
+    reference counterpart): the sites given with Structure.set_hydration_sites,
+    entering like the sites of a Grid model."""
     Analytic = 0
     Grid = 1
     AdaptiveGrid = 2
+    ExplicitSites = 3
 
 
 class Precision(enum.IntEnum):
@@ -206,6 +211,7 @@ class Structure:
         self.occupancy = f64([a.occupancy for a in atoms])
         self.atoms = atoms
         self._handles = {}
+        self.sites = None
 
     @classmethod
     def from_pdb_file(cls, path, options: "ParseOptions" = None) -> "Structure":
@@ -267,7 +273,21 @@ class Structure:
             np.full(n, -1, dtype=np.int8) if group is None else np.asarray(group, dtype=np.int8))
         s.atoms = None
         s._handles = {}
+        s.sites = None
         return s
+
+    def set_hydration_sites(self, sites, voxel_volume: float) -> None:
+        """Hydration sites (n x 3, A) of one voxel volume (A^3) used with
+        HydrationModel.ExplicitSites, in place of the sites a Grid or
+        AdaptiveGrid model builds (no reference counterpart)."""
+        xyz = f64(sites).reshape(-1, 3)
+        self.sites = (xyz, float(voxel_volume))
+        for _, h in list(self._handles.values()):
+            self._attach_sites(h)
+
+    def _attach_sites(self, h) -> None:
+        xyz, vol = self.sites
+        _check(_lib.load().csx_structure_set_hydration_sites(h, dptr(xyz), xyz.shape[0], vol))
 
     def __len__(self) -> int:
         return int(self.xyz.shape[0])
@@ -297,6 +317,8 @@ class Structure:
                 self.group.ctypes.data_as(C.POINTER(C.c_int8)), dptr(self.occupancy),
                 C.byref(h)))
             self._handles[id(ctx)] = (ctx, h)
+            if getattr(self, "sites", None) is not None:
+                self._attach_sites(h)
             return h
 
     def _free(self):
@@ -317,10 +339,13 @@ class Structure:
 class Context:
     """One HIP device; owns streams and kernel timing statistics."""
 
+    DEFAULT_CACHE_LIMIT = 256 << 20
+
     def __init__(self, device: int = 0):
         self.ptr = C.c_void_p()
         _check(_lib.load().csx_ctx_create(device, C.byref(self.ptr)))
         self.device = device
+        self.cache_limit = self.DEFAULT_CACHE_LIMIT
 
     def set_timing(self, enabled: bool) -> None:
         _check(_lib.load().csx_ctx_set_timing(self.ptr, int(enabled)))
@@ -336,6 +361,23 @@ class Context:
     def set_tuning(self, q_tile: int = 0, sin_mode: int = 2, j_chunk: int = 512) -> None:
         """Pair-kernel launch shape; q_tile 0 picks the tile per call."""
         _check(_lib.load().csx_ctx_set_tuning(self.ptr, q_tile, sin_mode, j_chunk))
+
+    def set_cache_limit(self, nbytes: int = DEFAULT_CACHE_LIMIT) -> None:
+        """Bytes of cached pair tables and multipole coefficients kept per
+        structure and solvent model (0: no caching)."""
+        _check(_lib.load().csx_ctx_set_cache_limit(self.ptr, int(nbytes)))
+        self.cache_limit = int(nbytes)
+
+    @contextlib.contextmanager
+    def caching(self, nbytes: int):
+        """Run a block with the given cache limit (0: every call computes),
+        then restore the previous one."""
+        prev = self.cache_limit
+        self.set_cache_limit(nbytes)
+        try:
+            yield self
+        finally:
+            self.set_cache_limit(prev)
 
     def set_workspace_limit(self, nbytes: int = 2 << 30) -> None:
         """Bytes of adjoint partial sums per launch (larger runs split)."""
@@ -360,6 +402,17 @@ class Context:
     def set_vjp_tile(self, q_tile: int = 0) -> None:
         """q values per sweep of the coordinate adjoint; 0 picks per call."""
         _check(_lib.load().csx_ctx_set_vjp_tile(self.ptr, q_tile))
+
+    def set_vjp_split(self, split: int = 0) -> None:
+        """Workgroups per sweep of the coordinate adjoint; 0 picks per call."""
+        _check(_lib.load().csx_ctx_set_vjp_split(self.ptr, split))
+
+    def last_vjp_launch(self) -> dict:
+        """Shape of the most recent coordinate-adjoint call on this context."""
+        v = [C.c_int() for _ in range(4)]
+        _check(_lib.load().csx_ctx_last_vjp_launch(self.ptr, *(C.byref(x) for x in v)))
+        return {"q_tile": v[0].value, "q_split": v[1].value, "step_split": v[2].value,
+                "launches": v[3].value}
 
     def set_persistence(self, workgroups_per_cu: int = -1) -> None:
         """Pair-kernel launch form: a persistent grid of this many workgroups

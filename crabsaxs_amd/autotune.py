@@ -62,6 +62,12 @@ def apply(ctx: Context, shape: dict) -> None:
 
 
 def _time_shapes(run, ctx: Context, cand: dict, repeats: int):
+    # every timed call runs the kernel: no cached tables
+    with ctx.caching(0):
+        return _time_shapes_uncached(run, ctx, cand, repeats)
+
+
+def _time_shapes_uncached(run, ctx: Context, cand: dict, repeats: int):
     results = []
     for qt, sm, jc, pe in itertools.product(cand["q_tile"], cand["sin_mode"], cand["j_chunk"],
                                             cand["persist"]):

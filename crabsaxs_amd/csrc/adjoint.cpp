@@ -66,10 +66,11 @@ static csx_status vjp(Call& call, const csx_structure* s, PointSet* ps, const do
     umax = std::max(umax, std::fabs(static_cast<double>(static_cast<float>(q[k] / (2.0 * kPi)))) *
                               2.0 * ps->max_radius);
   const int sin_mode = umax <= 250.0 ? 2 : 1;
-  int tuned;
+  int tuned, split;
   {
     std::lock_guard<std::mutex> lock(call.ctx()->mu);
     tuned = call.ctx()->tuning.vjp_q_tile;
+    split = call.ctx()->tuning.vjp_split;
   }
   const int qt = vjp_q_tile(tuned, nq, sin_mode, nitems, call.ctx()->cu_count);
   const size_t nq_pad = (nq + qt - 1) / qt * qt;
@@ -99,6 +100,7 @@ static csx_status vjp(Call& call, const csx_structure* s, PointSet* ps, const do
   const size_t fill = 4 * static_cast<size_t>(std::max(call.ctx()->cu_count, 1));
   int tsplit = 1;
   while (tsplit < 8 && 2 * wgs * static_cast<size_t>(tsplit) <= fill) tsplit *= 2;
+  if (split > 0) tsplit = split;
   const size_t per_item =
       static_cast<size_t>(qsplit) * static_cast<size_t>(tsplit) * 2 * 3 * kBlock * sizeof(double);
   // partial sums of one launch stay within the context's workspace limit
@@ -108,6 +110,14 @@ static csx_status vjp(Call& call, const csx_structure* s, PointSet* ps, const do
     budget = call.ctx()->tuning.workspace_limit;
   }
   const size_t ipl = std::max<size_t>(1, std::min(nitems, budget / per_item));
+  {
+    std::lock_guard<std::mutex> lock(call.ctx()->mu);
+    int* last = call.ctx()->last_vjp;
+    last[0] = qt;
+    last[1] = qsplit;
+    last[2] = tsplit;
+    last[3] = static_cast<int>((nitems + ipl - 1) / ipl);
+  }
   const hipStream_t st = call.stream();
   Scratch d_q(call), d_ab(call), d_blocks(call), d_items(call), d_part(call), d_acc(call),
       d_out(call);
@@ -200,10 +210,10 @@ static csx_status adjoint(csx_structure* s, const double* q, size_t nq, const do
   if (n == 0 || nq == 0) return CSX_OK;
   Call call(s->ctx);
   CSX_TRY(call.begin());
-  PointSet* ps = nullptr;
+  std::shared_ptr<PointSet> ps;
   CSX_TRY(get_point_set(s, nullptr, &ps));
-  return jacobian ? jacobian_run(call, s, ps, q, nq, p, implicit_h, out)
-                  : vjp(call, s, ps, q, nq, weights, p, implicit_h, out);
+  return jacobian ? jacobian_run(call, s, ps.get(), q, nq, p, implicit_h, out)
+                  : vjp(call, s, ps.get(), q, nq, weights, p, implicit_h, out);
 }
 
 }  // namespace csx

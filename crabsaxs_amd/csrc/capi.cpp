@@ -29,6 +29,7 @@ csx_status hip_fail(hipError_t e, const char* what) {
 
 static std::string set_key(const csx_solvent_params* p) {
   if (!p || p->hydration_model == CSX_HYDRATION_ANALYTIC) return "atoms";
+  if (p->hydration_model == CSX_HYDRATION_EXPLICIT_SITES) return "sites";
   std::ostringstream k;
   k.precision(17);
   if (p->hydration_model == CSX_HYDRATION_GRID)
@@ -41,7 +42,14 @@ static std::string set_key(const csx_solvent_params* p) {
 static csx_status build_point_set(csx_structure* s, const csx_solvent_params* p, PointSet& ps) {
   const int T = static_cast<int>(s->types.size());
   const size_t n = s->n;
-  if (p && p->hydration_model != CSX_HYDRATION_ANALYTIC) {
+  if (p && p->hydration_model == CSX_HYDRATION_EXPLICIT_SITES) {
+    if (!s->has_sites)
+      return fail(CSX_INVALID_INPUT,
+                  "the explicit-sites hydration model needs sites attached to the structure");
+    ps.site_xyz = s->sites;
+    ps.site_volume = s->site_volume;
+    ps.nsites = static_cast<int>(ps.site_xyz.size() / 3);
+  } else if (p && p->hydration_model != CSX_HYDRATION_ANALYTIC) {
     const Grid* g = nullptr;
     CSX_TRY(p->hydration_model == CSX_HYDRATION_GRID
                 ? structure_grid(s, kExteriorGrid, 1.0, 1.4, p->shell_thickness_angstrom, &g)
@@ -95,29 +103,7 @@ static csx_status build_point_set(csx_structure* s, const csx_solvent_params* p,
   }
   for (int t = 0; t < TT; ++t) seg_start[t + 1] += seg_start[t];
 
-  // Work items in bin-major order; bins are (a <= b) row-major.
-  const int kI = 256;
-  const int chunk = s->ctx->tuning.j_chunk;
-  std::vector<int> items;
-  std::vector<int> bin_first;
-  for (int a = 0; a < TT; ++a)
-    for (int b = a; b < TT; ++b) {
-      bin_first.push_back(static_cast<int>(items.size() / 4));
-      for (int i0 = seg_start[a]; i0 < seg_start[a + 1]; i0 += kI) {
-        const int ilen = std::min(kI, seg_start[a + 1] - i0);
-        const int jlo = std::max(seg_start[b], i0), jhi = seg_start[b + 1];
-        for (int j0 = jlo; j0 < jhi; j0 += chunk) {
-          items.insert(items.end(), {i0, ilen, j0, std::min(chunk, jhi - j0)});
-        }
-      }
-    }
-  bin_first.push_back(static_cast<int>(items.size() / 4));
-  ps.h_items = items;
-  ps.h_bin_first = bin_first;
-  ps.nitems = static_cast<int>(items.size() / 4);
   ps.nbins = TT * (TT + 1) / 2;
-  ps.j_chunk = chunk;
-
   // FP64 layout in original order.
   std::vector<double> pts64(4 * npts, 0.0), occ64(npts);
   std::vector<int> type64(npts);
@@ -145,27 +131,166 @@ static csx_status build_point_set(csx_structure* s, const csx_solvent_params* p,
   CSX_HIP(ps.seg32.upload(seg_start.data(), seg_start.size() * sizeof(int)), "upload segments");
   ps.h_seg = seg_start;
   CSX_HIP(ps.pts32.upload(pts32.data(), pts32.size() * sizeof(float)), "upload points");
-  CSX_HIP(ps.items.upload(items.data(), items.size() * sizeof(int)), "upload items");
-  CSX_HIP(ps.bin_first.upload(bin_first.data(), bin_first.size() * sizeof(int)), "upload bins");
   CSX_HIP(ps.pts64.upload(pts64.data(), pts64.size() * sizeof(double)), "upload points");
   CSX_HIP(ps.type64.upload(type64.data(), type64.size() * sizeof(int)), "upload types");
   CSX_HIP(ps.occ64.upload(occ64.data(), occ64.size() * sizeof(double)), "upload occupancy");
   return CSX_OK;
 }
 
-csx_status get_point_set(csx_structure* s, const csx_solvent_params* p, PointSet** out) {
+csx_status get_point_set(csx_structure* s, const csx_solvent_params* p,
+                         std::shared_ptr<PointSet>* out) {
   std::lock_guard<std::mutex> lock(s->mu);
   const std::string key = set_key(p);
   auto it = s->sets.find(key);
-  if (it != s->sets.end() && it->second->j_chunk == s->ctx->tuning.j_chunk) {
-    *out = it->second.get();
+  if (it != s->sets.end()) {
+    *out = it->second;
     return CSX_OK;
   }
-  std::unique_ptr<PointSet> ps(new PointSet);
+  auto ps = std::make_shared<PointSet>();
   CSX_TRY(build_point_set(s, p, *ps));
-  *out = ps.get();
-  s->sets[key] = std::move(ps);
+  s->sets[key] = ps;
+  *out = std::move(ps);
   return CSX_OK;
+}
+
+csx_status get_item_set(PointSet& ps, int chunk, std::shared_ptr<ItemSet>* out) {
+  std::lock_guard<std::mutex> lock(ps.items_mu);
+  auto it = ps.item_sets.find(chunk);
+  if (it != ps.item_sets.end()) {
+    *out = it->second;
+    return CSX_OK;
+  }
+  auto is = std::make_shared<ItemSet>();
+  const int kI = 256;
+  const int TT = ps.ntypes;
+  const std::vector<int>& seg = ps.h_seg;
+  std::vector<int>& items = is->h_items;
+  std::vector<int>& bin_first = is->h_bin_first;
+  for (int a = 0; a < TT; ++a)
+    for (int b = a; b < TT; ++b) {
+      bin_first.push_back(static_cast<int>(items.size() / 4));
+      for (int i0 = seg[a]; i0 < seg[a + 1]; i0 += kI) {
+        const int ilen = std::min(kI, seg[a + 1] - i0);
+        const int jlo = std::max(seg[b], i0), jhi = seg[b + 1];
+        for (int j0 = jlo; j0 < jhi; j0 += chunk)
+          items.insert(items.end(), {i0, ilen, j0, std::min(chunk, jhi - j0)});
+      }
+    }
+  bin_first.push_back(static_cast<int>(items.size() / 4));
+  is->nitems = static_cast<int>(items.size() / 4);
+  is->j_chunk = chunk;
+  CSX_HIP(is->items.upload(items.data(), items.size() * sizeof(int)), "upload items");
+  CSX_HIP(is->bin_first.upload(bin_first.data(), bin_first.size() * sizeof(int)), "upload bins");
+  ps.item_sets[chunk] = is;
+  *out = std::move(is);
+  return CSX_OK;
+}
+
+// ---------------------------------------------------------- result cache --
+
+std::shared_ptr<CacheEntry> cache_acquire(csx_ctx* ctx, PointSet& ps, const std::string& key,
+                                          std::unique_lock<std::mutex>* lock) {
+  size_t limit;
+  {
+    std::lock_guard<std::mutex> g(ctx->mu);
+    limit = ctx->tuning.cache_bytes;
+  }
+  std::shared_ptr<CacheEntry> e;
+  if (limit > 0) {
+    std::lock_guard<std::mutex> g(ps.cache_mu);
+    for (size_t k = 0; k < ps.cache.size(); ++k)
+      if (ps.cache[k]->key == key) {
+        e = ps.cache[k];
+        ps.cache.erase(ps.cache.begin() + k);  // to the front: most recently used
+        break;
+      }
+    if (!e) {
+      e = std::make_shared<CacheEntry>();
+      e->key = key;
+    }
+    ps.cache.insert(ps.cache.begin(), e);
+  } else {
+    e = std::make_shared<CacheEntry>();
+    e->key = key;
+  }
+  *lock = std::unique_lock<std::mutex>(e->mu);
+  return e;
+}
+
+void cache_commit(csx_ctx* ctx, PointSet& ps, const std::shared_ptr<CacheEntry>& e) {
+  size_t limit;
+  {
+    std::lock_guard<std::mutex> g(ctx->mu);
+    limit = ctx->tuning.cache_bytes;
+  }
+  if (limit == 0) return;
+  std::lock_guard<std::mutex> g(ps.cache_mu);
+  // The entry goes first (another call may have dropped it meanwhile); then
+  // the most recently used ready entries are kept within the limit.  Entries
+  // still being computed by other calls are kept; entries dropped here stay
+  // alive for the calls that hold them.
+  std::vector<std::shared_ptr<CacheEntry>> kept{e};
+  size_t total = e->bytes;
+  for (auto& c : ps.cache) {
+    if (c == e) continue;
+    if (!c->ready) {
+      kept.push_back(c);
+      continue;
+    }
+    if (total + c->bytes <= limit) {
+      total += c->bytes;
+      kept.push_back(c);
+    }
+  }
+  ps.cache.swap(kept);
+}
+
+std::shared_ptr<CacheEntry> cache_find(csx_ctx* ctx, PointSet& ps, const std::string& key) {
+  {
+    std::lock_guard<std::mutex> g(ctx->mu);
+    if (ctx->tuning.cache_bytes == 0) return nullptr;
+  }
+  std::lock_guard<std::mutex> g(ps.cache_mu);
+  for (size_t k = 0; k < ps.cache.size(); ++k)
+    if (ps.cache[k]->key == key && ps.cache[k]->ready) {
+      std::shared_ptr<CacheEntry> e = ps.cache[k];
+      ps.cache.erase(ps.cache.begin() + k);
+      ps.cache.insert(ps.cache.begin(), e);
+      return e;
+    }
+  return nullptr;
+}
+
+void cache_store(csx_ctx* ctx, PointSet& ps, const std::string& key,
+                 const std::vector<double>& tables) {
+  {
+    std::lock_guard<std::mutex> g(ctx->mu);
+    if (ctx->tuning.cache_bytes == 0) return;
+  }
+  {
+    std::lock_guard<std::mutex> g(ps.cache_mu);
+    for (auto& c : ps.cache)
+      if (c->key == key) return;  // present, or being computed by another call
+  }
+  auto e = std::make_shared<CacheEntry>();
+  e->key = key;
+  e->tables = tables;
+  e->bytes = tables.size() * sizeof(double) + key.size();
+  e->ready = true;
+  {
+    std::lock_guard<std::mutex> g(ps.cache_mu);
+    ps.cache.insert(ps.cache.begin(), e);
+  }
+  cache_commit(ctx, ps, e);
+}
+
+void cache_drop(PointSet& ps, const std::shared_ptr<CacheEntry>& e) {
+  std::lock_guard<std::mutex> g(ps.cache_mu);
+  for (size_t k = 0; k < ps.cache.size(); ++k)
+    if (ps.cache[k] == e) {
+      ps.cache.erase(ps.cache.begin() + k);
+      break;
+    }
 }
 
 // --------------------------------------------------------- pair tables --
@@ -204,9 +329,47 @@ static int choose_q_tile(int tuned, int sin_mode, size_t nq, double sum_j, doubl
   return best;
 }
 
+// Key of a point set's pair tables: they depend on the q grid, the sine mode
+// and the j chunk (which orders the FP64 sums); every q tile and launch form
+// gives the same bits.
+static std::string table_key(int sin_mode, int j_chunk, const double* q, size_t nq) {
+  std::string k = "pairs:" + std::to_string(sin_mode) + ":" + std::to_string(j_chunk) + ":";
+  k.append(reinterpret_cast<const char*>(q), nq * sizeof(double));
+  return k;
+}
+
+// Pair tables of one or more point sets.  Tables already cached on a set
+// for the same q grid and arithmetic are reused; the others come from one
+// launch of the pair kernel (a batch concatenates the sets' points, work
+// items and bins) and are cached:
+// P[c][b*nq + k] = sum_{i<j in bin b of set c} occ_i occ_j sinc(q_k r_ij).
+// One set waits for a concurrent call computing the same tables; in a batch,
+// tables that another call is still computing are computed again.
+// The pair kernel's sine mode for a q grid and the largest point radius:
+// the tuned mode, except that the unreduced hardware sine (mode 2) takes
+// |q r / 2 pi| <= 256 revolutions, so larger phases take mode 1.
+static int pair_sin_mode(int tuned, const double* q, size_t nq, double rmax) {
+  double umax = 0.0;
+  for (size_t k = 0; k < nq; ++k)
+    umax = std::max(umax, std::fabs(static_cast<double>(static_cast<float>(q[k] / (2.0 * kPi)))) *
+                              2.0 * rmax);
+  return tuned == 2 && !(umax <= 250.0) ? 1 : tuned;
+}
+
+// Automatic launch form: the persistent grid pays (4-8 % on large calls)
+// once every one of its workgroups takes at least two tasks; with fewer
+// tasks one workgroup per task starts sooner.
+static int pair_persist_wgs(int tuned, size_t nitems, size_t nq_pad, int qt, int cus) {
+  const double tasks = static_cast<double>(nitems) * static_cast<double>(nq_pad / qt);
+  return tuned > 0 ? tuned * cus
+         : tuned < 0 && tasks >= 2.0 * kAutoPersist * cus ? kAutoPersist * cus
+                                                          : 0;
+}
+
 // `overlap`, when given, runs on the host while the pair kernel runs (after
-// the launch, before the blocking download of the tables).
-static csx_status pair_tables_batch(Call& call, const std::vector<const PointSet*>& sets,
+// the launch, before the blocking download of the tables), or at once when
+// nothing is launched.
+static csx_status pair_tables_batch(Call& call, const std::vector<PointSet*>& sets,
                                     const double* q, size_t nq,
                                     std::vector<std::vector<double>>& P,
                                     const std::function<void()>& overlap = nullptr) {
@@ -215,27 +378,62 @@ static csx_status pair_tables_batch(Call& call, const std::vector<const PointSet
     std::lock_guard<std::mutex> lock(call.ctx()->mu);
     tune = call.ctx()->tuning;
   }
-  double rmax = 0.0, sum_j = 0.0, max_j = 0.0;
+  double rmax = 0.0;
+  for (const PointSet* ps : sets) rmax = std::max(rmax, ps->max_radius);
+  const int sin_mode = pair_sin_mode(tune.sin_mode, q, nq, rmax);
+  const int j_chunk = tune.j_chunk;
+  const std::string key = table_key(sin_mode, j_chunk, q, nq);
+
+  P.assign(sets.size(), std::vector<double>());
+  std::vector<size_t> todo;
+  std::shared_ptr<CacheEntry> entry;  // the one set's entry, locked while computed
+  std::unique_lock<std::mutex> entry_lock;
+  if (sets.size() == 1) {
+    entry = cache_acquire(call.ctx(), *sets[0], key, &entry_lock);
+    if (entry->ready) {
+      P[0] = entry->tables;
+      if (overlap) overlap();
+      return CSX_OK;
+    }
+    todo.push_back(0);
+  } else {
+    for (size_t c = 0; c < sets.size(); ++c) {
+      std::shared_ptr<CacheEntry> e = cache_find(call.ctx(), *sets[c], key);
+      if (e) P[c] = e->tables;
+      else todo.push_back(c);
+    }
+    if (todo.empty()) {
+      if (overlap) overlap();
+      return CSX_OK;
+    }
+  }
+  // A failed computation leaves no half-made entry behind.
+  struct Dropper {
+    PointSet* ps;
+    std::shared_ptr<CacheEntry>* e;
+    ~Dropper() {
+      if (*e && !(*e)->ready) cache_drop(*ps, *e);
+    }
+  } dropper{sets[0], &entry};
+
+  std::vector<std::shared_ptr<ItemSet>> isets(todo.size());
+  double sum_j = 0.0, max_j = 0.0;
   size_t npts = 0, nitems = 0, nbins = 0;
-  for (const PointSet* ps : sets) {
-    rmax = std::max(rmax, ps->max_radius);
+  for (size_t t = 0; t < todo.size(); ++t) {
+    PointSet* ps = sets[todo[t]];
+    CSX_TRY(get_item_set(*ps, j_chunk, &isets[t]));
     npts += static_cast<size_t>(ps->natoms + ps->nsites);
-    nitems += static_cast<size_t>(ps->nitems);
+    nitems += static_cast<size_t>(isets[t]->nitems);
     nbins += static_cast<size_t>(ps->nbins);
-    for (size_t k = 3; k < ps->h_items.size(); k += 4) {
-      sum_j += ps->h_items[k];
-      max_j = std::max(max_j, static_cast<double>(ps->h_items[k]));
+    const std::vector<int>& hi = isets[t]->h_items;
+    for (size_t k = 3; k < hi.size(); k += 4) {
+      sum_j += hi[k];
+      max_j = std::max(max_j, static_cast<double>(hi[k]));
     }
   }
   if (npts > static_cast<size_t>(INT32_MAX / 2) || nitems > static_cast<size_t>(INT32_MAX / 2) ||
       nbins > static_cast<size_t>(INT32_MAX / 2))
     return fail(CSX_INVALID_INPUT, "batch too large for one launch");
-  double umax = 0.0;
-  for (size_t k = 0; k < nq; ++k)
-    umax = std::max(umax, std::fabs(static_cast<double>(static_cast<float>(q[k] / (2.0 * kPi)))) *
-                              2.0 * rmax);
-  // The unreduced hardware sine takes |q r / 2 pi| <= 256 revolutions.
-  const int sin_mode = tune.sin_mode == 2 && !(umax <= 250.0) ? 1 : tune.sin_mode;
   const int cus = call.ctx()->cu_count;
   const int qt = choose_q_tile(tune.q_tile, sin_mode, nq, sum_j, max_j, cus);
   const size_t nq_pad = (nq + qt - 1) / qt * qt;
@@ -261,30 +459,32 @@ static csx_status pair_tables_batch(Call& call, const std::vector<const PointSet
   const int4* items = nullptr;
   const int* bins = nullptr;
   std::vector<int> h_items, h_bins;
-  if (sets.size() == 1) {
-    pts = sets[0]->pts32.as<float4>();
-    items = sets[0]->items.as<int4>();
-    bins = sets[0]->bin_first.as<int>();
+  if (todo.size() == 1) {
+    pts = sets[todo[0]]->pts32.as<float4>();
+    items = isets[0]->items.as<int4>();
+    bins = isets[0]->bin_first.as<int>();
   } else {
     CSX_HIP(d_pts.alloc(npts * sizeof(float4), st), "hipMallocAsync");
     h_items.reserve(4 * nitems);
     h_bins.reserve(nbins + 1);
     size_t off = 0, item_off = 0;
-    for (const PointSet* ps : sets) {
+    for (size_t t = 0; t < todo.size(); ++t) {
+      const PointSet* ps = sets[todo[t]];
+      const ItemSet& is = *isets[t];
       const size_t np = static_cast<size_t>(ps->natoms + ps->nsites);
       CSX_HIP(hipMemcpyAsync(d_pts.as<float4>() + off, ps->pts32.as<void>(), np * sizeof(float4),
                              hipMemcpyDeviceToDevice, st),
               "hipMemcpyAsync");
-      for (size_t k = 0; k < ps->h_items.size(); k += 4) {
-        h_items.push_back(ps->h_items[k] + static_cast<int>(off));
-        h_items.push_back(ps->h_items[k + 1]);
-        h_items.push_back(ps->h_items[k + 2] + static_cast<int>(off));
-        h_items.push_back(ps->h_items[k + 3]);
+      for (size_t k = 0; k < is.h_items.size(); k += 4) {
+        h_items.push_back(is.h_items[k] + static_cast<int>(off));
+        h_items.push_back(is.h_items[k + 1]);
+        h_items.push_back(is.h_items[k + 2] + static_cast<int>(off));
+        h_items.push_back(is.h_items[k + 3]);
       }
       for (int b = 0; b < ps->nbins; ++b)
-        h_bins.push_back(ps->h_bin_first[b] + static_cast<int>(item_off));
+        h_bins.push_back(is.h_bin_first[b] + static_cast<int>(item_off));
       off += np;
-      item_off += static_cast<size_t>(ps->nitems);
+      item_off += static_cast<size_t>(is.nitems);
     }
     h_bins.push_back(static_cast<int>(item_off));
     CSX_HIP(d_items.alloc(h_items.size() * sizeof(int), st), "hipMallocAsync");
@@ -299,13 +499,7 @@ static csx_status pair_tables_batch(Call& call, const std::vector<const PointSet
     items = d_items.as<int4>();
     bins = d_bins.as<int>();
   }
-  // Automatic launch form: the persistent grid pays (4-8 % on large calls)
-  // once every one of its workgroups takes at least two tasks; with fewer
-  // tasks one workgroup per task starts sooner.
-  const double tasks = static_cast<double>(nitems) * static_cast<double>(nq_pad / qt);
-  const int persist_wgs = tune.persist > 0 ? tune.persist * cus
-                          : tune.persist < 0 && tasks >= 2.0 * kAutoPersist * cus ? kAutoPersist * cus
-                                                                                   : 0;
+  const int persist_wgs = pair_persist_wgs(tune.persist, nitems, nq_pad, qt, cus);
   {
     std::lock_guard<std::mutex> lock(call.ctx()->mu);
     call.ctx()->last_launch[0] = qt;
@@ -330,31 +524,39 @@ static csx_status pair_tables_batch(Call& call, const std::vector<const PointSet
   d_bins.reset();
   CSX_TRY(call.finish());
   const double* raw = static_cast<const double*>(h_out);
-  P.assign(sets.size(), std::vector<double>());
   size_t bin_off = 0;
-  for (size_t c = 0; c < sets.size(); ++c) {
-    const PointSet& ps = *sets[c];
-    std::vector<double>& Pc = P[c];
+  for (size_t t = 0; t < todo.size(); ++t) {
+    const PointSet& ps = *sets[todo[t]];
+    std::vector<double>& Pc = P[todo[t]];
     Pc.assign(size_t(ps.nbins) * nq, 0.0);
     const int TT = ps.ntypes;
     int b = 0;
     for (int a = 0; a < TT; ++a)
-      for (int t = a; t < TT; ++t, ++b)
+      for (int u = a; u < TT; ++u, ++b)
         for (size_t k = 0; k < nq; ++k) {
           const double qe = 2.0 * kPi * static_cast<double>(qrev[k]);
           if (qe == 0.0) {
-            Pc[b * nq + k] = a == t ? 0.5 * (ps.wsum[a] * ps.wsum[a] - ps.self[a])
-                                    : ps.wsum[a] * ps.wsum[t];
+            Pc[b * nq + k] = a == u ? 0.5 * (ps.wsum[a] * ps.wsum[a] - ps.self[a])
+                                    : ps.wsum[a] * ps.wsum[u];
           } else {
             Pc[b * nq + k] = raw[(bin_off + b) * nq_pad + k] / qe;
           }
         }
     bin_off += static_cast<size_t>(ps.nbins);
   }
+  if (entry) {
+    entry->tables = P[0];
+    entry->bytes = P[0].size() * sizeof(double) + key.size();
+    entry->ready = true;
+    cache_commit(call.ctx(), *sets[0], entry);
+  } else {
+    for (size_t t = 0; t < todo.size(); ++t)
+      cache_store(call.ctx(), *sets[todo[t]], key, P[todo[t]]);
+  }
   return CSX_OK;
 }
 
-static csx_status pair_tables_f32(Call& call, const PointSet& ps, const double* q, size_t nq,
+static csx_status pair_tables_f32(Call& call, PointSet& ps, const double* q, size_t nq,
                                   std::vector<double>& P,
                                   const std::function<void()>& overlap = nullptr) {
   std::vector<std::vector<double>> all;
@@ -463,7 +665,7 @@ static csx_status debye_basis(csx_structure* s, const double* q, size_t nq,
                               double* out) {
   Call call(s->ctx);
   CSX_TRY(call.begin());
-  PointSet* ps = nullptr;
+  std::shared_ptr<PointSet> ps;
   CSX_TRY(get_point_set(s, p, &ps));
   std::vector<double> V, E, H;
   auto amplitudes = [&] { type_amplitudes(s, *ps, q, nq, implicit_h, p, V, E, H); };
@@ -606,6 +808,13 @@ csx_status csx_ctx_set_tuning(csx_ctx* ctx, int q_tile, int sin_mode, int j_chun
   return CSX_OK;
 }
 
+csx_status csx_ctx_set_cache_limit(csx_ctx* ctx, size_t bytes) {
+  if (!ctx) return fail(CSX_INVALID_INPUT, "context is null");
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  ctx->tuning.cache_bytes = bytes;
+  return CSX_OK;
+}
+
 csx_status csx_ctx_set_workspace_limit(csx_ctx* ctx, size_t bytes) {
   if (!ctx) return fail(CSX_INVALID_INPUT, "context is null");
   if (bytes < (size_t(1) << 20)) return fail(CSX_INVALID_INPUT, "workspace limit must be >= 1 MiB");
@@ -620,6 +829,26 @@ csx_status csx_ctx_set_vjp_tile(csx_ctx* ctx, int q_tile) {
     return fail(CSX_INVALID_INPUT, "vjp q_tile must be 0 (automatic), 16, 32, 48, 60 or 64");
   std::lock_guard<std::mutex> lock(ctx->mu);
   ctx->tuning.vjp_q_tile = q_tile;
+  return CSX_OK;
+}
+
+csx_status csx_ctx_set_vjp_split(csx_ctx* ctx, int split) {
+  if (!ctx) return fail(CSX_INVALID_INPUT, "context is null");
+  if (split != 0 && split != 1 && split != 2 && split != 4 && split != 8)
+    return fail(CSX_INVALID_INPUT, "vjp split must be 0 (automatic), 1, 2, 4 or 8");
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  ctx->tuning.vjp_split = split;
+  return CSX_OK;
+}
+
+csx_status csx_ctx_last_vjp_launch(csx_ctx* ctx, int* q_tile, int* q_split, int* step_split,
+                                   int* launches) {
+  if (!ctx) return fail(CSX_INVALID_INPUT, "context is null");
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  if (q_tile) *q_tile = ctx->last_vjp[0];
+  if (q_split) *q_split = ctx->last_vjp[1];
+  if (step_split) *step_split = ctx->last_vjp[2];
+  if (launches) *launches = ctx->last_vjp[3];
   return CSX_OK;
 }
 
@@ -707,7 +936,7 @@ csx_status csx_structure_create(csx_ctx* ctx, size_t n, const double* xyz, const
   }
   const hipError_t e = hipSetDevice(ctx->device);
   if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
-  PointSet* ps = nullptr;
+  std::shared_ptr<PointSet> ps;
   CSX_TRY(get_point_set(s.get(), nullptr, &ps));
   *out = s.release();
   return CSX_OK;
@@ -720,6 +949,24 @@ void csx_structure_destroy(csx_structure* s) {
 }
 
 size_t csx_structure_atom_count(const csx_structure* s) { return s ? s->n : 0; }
+
+csx_status csx_structure_set_hydration_sites(csx_structure* s, const double* xyz, size_t count,
+                                             double voxel_volume) {
+  if (!s) return fail(CSX_INVALID_INPUT, "structure handle is null");
+  if (count > 0 && !xyz) return fail(CSX_INVALID_INPUT, "site coordinates are required");
+  if (!std::isfinite(voxel_volume) || voxel_volume < 0.0)
+    return fail(CSX_INVALID_INPUT, "voxel volume must be finite and non-negative");
+  if (count > static_cast<size_t>(INT32_MAX / 4)) return fail(CSX_INVALID_INPUT, "too many sites");
+  for (size_t k = 0; k < 3 * count; ++k)
+    if (!std::isfinite(xyz[k])) return fail(CSX_INVALID_INPUT, "site coordinates must be finite");
+  std::lock_guard<std::mutex> lock(s->mu);
+  s->has_sites = true;
+  s->sites.assign(xyz, xyz + 3 * count);
+  s->site_volume = voxel_volume;
+  // calls already holding the previous point set finish with it
+  s->sets.erase("sites");
+  return CSX_OK;
+}
 
 static csx_status copy_grid(const Grid& g, double* sites, size_t capacity, size_t* count,
                             double* volume) {
@@ -761,7 +1008,7 @@ csx_status csx_debye_intensity(csx_structure* s, const double* q, size_t nq, int
   if (nq == 0) return CSX_OK;
   Call call(s->ctx);
   CSX_TRY(call.begin());
-  PointSet* ps = nullptr;
+  std::shared_ptr<PointSet> ps;
   CSX_TRY(get_point_set(s, nullptr, &ps));
   std::vector<double> V, E, H;
   auto amplitudes = [&] { type_amplitudes(s, *ps, q, nq, implicit_h != 0, nullptr, V, E, H); };
@@ -806,7 +1053,7 @@ csx_status csx_debye_pair_tables(csx_structure* s, const double* q, size_t nq,
                                  size_t* n_types) {
   if (!s) return fail(CSX_INVALID_INPUT, "structure handle is null");
   if (nq > 0 && !q) return fail(CSX_INVALID_INPUT, "q is required");
-  PointSet* ps = nullptr;
+  std::shared_ptr<PointSet> ps;
   {
     const hipError_t e = hipSetDevice(s->ctx->device);
     if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
@@ -839,11 +1086,11 @@ static csx_status batch_debye(csx_structure* const* structures, size_t m, const 
   if (nq == 0) return CSX_OK;
   Call call(ctx);
   CSX_TRY(call.begin());
-  std::vector<const PointSet*> sets(m);
+  std::vector<std::shared_ptr<PointSet>> held(m);
+  std::vector<PointSet*> sets(m);
   for (size_t c = 0; c < m; ++c) {
-    PointSet* ps = nullptr;
-    CSX_TRY(get_point_set(structures[c], p, &ps));
-    sets[c] = ps;
+    CSX_TRY(get_point_set(structures[c], p, &held[c]));
+    sets[c] = held[c].get();
   }
   std::vector<std::vector<double>> P, V(m), E(m), H(m);
   CSX_TRY(pair_tables_batch(call, sets, q, nq, P, [&] {
@@ -926,7 +1173,7 @@ csx_status csx_compute_curve(csx_structure* s, const double* q, size_t nq,
     if (!ok)
       return fail(CSX_INVALID_INPUT,
                   "solvent parameters must be finite with positive dimensions, density, and c1");
-    if (p->hydration_model < 0 || p->hydration_model > 2)
+    if (p->hydration_model < 0 || p->hydration_model > CSX_HYDRATION_EXPLICIT_SITES)
       return fail(CSX_INVALID_INPUT, "unknown hydration model");
   }
   if (o->l_max != 0 && (o->l_max < 8 || o->l_max > 64))

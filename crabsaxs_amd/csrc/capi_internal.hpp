@@ -39,6 +39,8 @@ struct Tuning {
                        // -1: automatic, kAutoPersist per CU when tasks are plenty)
   size_t workspace_limit = size_t(2) << 30;  // bytes of adjoint partials per launch
   int vjp_q_tile = 0;  // q values per sweep of the coordinate adjoint (0: automatic)
+  int vjp_split = 0;   // workgroups per sweep of the coordinate adjoint (0: automatic)
+  size_t cache_bytes = size_t(256) << 20;  // cached results kept per point set (0: off)
 };
 
 }  // namespace csx
@@ -53,6 +55,7 @@ struct csx_ctx {
   int cu_count = 256;
   size_t lds_per_workgroup = 64 * 1024;  // the device's per-workgroup LDS limit
   int last_launch[3] = {0, 0, 0};  // q tile, sin mode, persistent workgroups
+  int last_vjp[4] = {0, 0, 0, 0};  // q tile, q split, step split, launches
   // Grow-only device buffers kept per stream (see csx::Scratch), and
   // grow-only pinned host staging buffers per stream; freed with the context.
   std::map<hipStream_t, std::vector<std::pair<void*, size_t>>> scratch, pinned;
@@ -119,7 +122,34 @@ struct MultipoleGeometry {
   // Per padded point: radius, occupancy (0 for padding); chunks: (first
   // slab, end slab, type) of each one-type chunk of the coefficient kernel.
   DevArray radius, occ, chunks, Y, tile_l;
-  std::vector<int> h_tile_l;  // l of each column tile
+  std::vector<int> h_tile_l;      // l of each column tile
+  std::vector<int> h_type_chunk;  // per type t: its chunks are [h_type_chunk[t], [t + 1])
+  DevArray type_chunk;
+};
+
+// Work items of the FP32 pair kernel over one point set for one j chunk:
+// type-pure blocks of 256 i points x chunks of j points over the upper
+// triangle, bin-major over the T(T+1)/2 type pairs (a <= b).
+struct ItemSet {
+  DevArray items;       // int4[nitems]: i0, ilen, j0, jlen
+  DevArray bin_first;   // int[nbins + 1]
+  std::vector<int> h_items, h_bin_first;  // host copies (batch concatenation)
+  int nitems = 0, j_chunk = 0;
+};
+
+// One cached result of a point set: the pair tables of a q grid (host,
+// FP64 [nbins][nq]) or the per-type multipole coefficients of a q grid and
+// l_max (device, FP64 [ntypes][nqp][ncol], with the per-q orders).  `key`
+// holds the exact inputs the result depends on.  The entry's mutex is held
+// while it is computed, so concurrent callers of one key wait for the first.
+struct CacheEntry {
+  std::string key;
+  std::mutex mu;
+  std::atomic<bool> ready{false};
+  size_t bytes = 0;
+  std::vector<double> tables;
+  DevArray coef, qlmax;
+  int nqp = 0, ncol = 0;
 };
 
 // One set of scattering points (atoms, optionally followed by hydration
@@ -129,25 +159,28 @@ struct PointSet {
   std::map<int, std::unique_ptr<MultipoleGeometry>> geoms;  // by l_max
   int ntypes = 0;         // atom types (+1 when sites are present)
   int natoms = 0, nsites = 0;
+  int nbins = 0;          // ntypes (ntypes + 1) / 2
   double site_volume = 0.0;
   std::vector<double> self;   // per type: sum occ^2
   std::vector<double> wsum;   // per type: sum occ
   // FP32 pair-table layout: type-sorted, centred, w = occupancy.
   DevArray pts32;             // float4[natoms + nsites]
-  DevArray items;             // int4[nitems]: i0, ilen, j0, jlen
-  DevArray bin_first;         // int[nbins + 1]
-  int nitems = 0, nbins = 0, j_chunk = 0;
   double max_radius = 0.0;    // of the FP32 points about the centroid
   DevArray orig32;            // int[npts]: original index of each sorted point
   DevArray type32;            // int[npts]: type of each sorted point
   DevArray seg32;             // int[ntypes + 1]: type segments of the sorted order
-  std::vector<int> h_items, h_bin_first;  // host copies (batch concatenation)
-  std::vector<int> h_seg;                  // host copy of the type segments
+  std::vector<int> h_seg;     // host copy of the type segments
   // FP64 plain layout: original order, atoms then sites.
   DevArray pts64;             // double4[natoms + nsites]
   DevArray type64;            // int[natoms + nsites]
   DevArray occ64;             // double[natoms + nsites]
   std::vector<double> site_xyz;
+  // Work items by j chunk, built on first use.
+  std::mutex items_mu;
+  std::map<int, std::shared_ptr<ItemSet>> item_sets;
+  // Cached results, most recently used first.
+  std::mutex cache_mu;
+  std::vector<std::shared_ptr<CacheEntry>> cache;
 };
 
 }  // namespace csx
@@ -164,8 +197,14 @@ struct csx_structure {
   double centre[3] = {0, 0, 0};
   double max_radius = 0.0;          // from the centroid
   std::mutex mu;
-  std::map<std::string, std::unique_ptr<csx::PointSet>> sets;
+  // Point sets by solvent model; a caller holds its set for the call, so a
+  // set replaced meanwhile (new hydration sites) outlives the calls using it.
+  std::map<std::string, std::shared_ptr<csx::PointSet>> sets;
   std::map<std::string, csx::Grid> grids;  // hydration grids by kind and options
+  // Hydration sites given by the caller (CSX_HYDRATION_EXPLICIT_SITES).
+  bool has_sites = false;
+  std::vector<double> sites;
+  double site_volume = 0.0;
 };
 
 namespace csx {
@@ -328,7 +367,27 @@ class Scratch {
 
 // Point set of a structure for a solvent model (atoms only when p is NULL or
 // analytic), built and uploaded on first use.
-csx_status get_point_set(csx_structure* s, const csx_solvent_params* p, PointSet** out);
+csx_status get_point_set(csx_structure* s, const csx_solvent_params* p,
+                         std::shared_ptr<PointSet>* out);
+
+// Work items of a point set for a j chunk, built on first use.
+csx_status get_item_set(PointSet& ps, int j_chunk, std::shared_ptr<ItemSet>* out);
+
+// The cached result of `key` on a point set.  On return the entry's mutex
+// is locked by *lock; entry->ready tells whether it holds the result.  When
+// it does not, the caller computes it, sets `bytes` and `ready`, and calls
+// cache_commit (or cache_drop on failure) before releasing the lock.  With
+// caching off the entry is private to the call.
+std::shared_ptr<CacheEntry> cache_acquire(csx_ctx* ctx, PointSet& ps, const std::string& key,
+                                          std::unique_lock<std::mutex>* lock);
+void cache_commit(csx_ctx* ctx, PointSet& ps, const std::shared_ptr<CacheEntry>& e);
+void cache_drop(PointSet& ps, const std::shared_ptr<CacheEntry>& e);
+// The ready entry of `key` without waiting (null when absent, still being
+// computed, or with caching off), and the store of a result computed without
+// an entry (kept unless the key is already present).
+std::shared_ptr<CacheEntry> cache_find(csx_ctx* ctx, PointSet& ps, const std::string& key);
+void cache_store(csx_ctx* ctx, PointSet& ps, const std::string& key,
+                 const std::vector<double>& tables);
 
 // Hydration grid of a structure, built on its device on first use and kept
 // with the structure; the caller holds s->mu.  kAdaptiveGrid takes (spacing,

@@ -12,16 +12,20 @@ namespace csx {
 
 int multipole_chunk();
 int multipole_tiles_per_group(int ntiles);
-size_t multipole_coeff_lds_bytes(int l_range, int nch);
+size_t multipole_coeff_lds_bytes(int l_range);
+int multipole_max_types();
 hipError_t launch_multipole_geometry(const double* xyz, int npts, const double* centre, int l_max,
                                      const double* norms, const int* tile_first, int ntiles,
                                      int npad, float* radius, float* Y, hipStream_t stream);
 hipError_t launch_multipole_coeff(const float* radius, const float* occ, const int* chunks,
                                   int nchunk, int npad, const float* Y, int ntiles,
-                                  const int* tile_l, int max_l_range, const float* amp,
-                                  const float* qv, const int* qlmax, int nqp, int nch,
-                                  float* partial, double* coef, double* out, hipStream_t stream,
+                                  const int* tile_l, int max_l_range, const int* type_chunk,
+                                  int ntypes, const float* qv, const int* qlmax, int nqp,
+                                  float* partial, double* coef, hipStream_t stream,
                                   KernelTimer* timer);
+hipError_t launch_multipole_contract(const double* coef, const double* amp, int ntypes, int nch,
+                                     const int* tile_l, const int* qlmax, int nqp, int ncol,
+                                     double* out, hipStream_t stream);
 
 namespace {
 
@@ -127,8 +131,10 @@ static csx_status get_geometry(Call& call, csx_structure* s, PointSet& ps, int L
   std::vector<double> xyz;
   std::vector<float> occ;
   int slab = 0;
+  std::vector<int> type_chunk(T + 2, 0);
   for (int t = 0; t <= T; ++t) {
     const int count = seg[t + 1] - seg[t];
+    type_chunk[t] = static_cast<int>(chunks.size() / 4);
     if (count == 0) continue;
     const int slabs = (count + kSlabPoints - 1) / kSlabPoints;
     for (int k = 0; k < slabs * kSlabPoints; ++k) {
@@ -149,6 +155,11 @@ static csx_status get_geometry(Call& call, csx_structure* s, PointSet& ps, int L
   }
   g->npad = slab * kSlabPoints;
   g->nchunk = static_cast<int>(chunks.size() / 4);
+  type_chunk[T + 1] = g->nchunk;
+  // Types of the point set: the atom types, then the site type when the set
+  // has sites; the chunk table keeps the site chunks last.
+  if (ps.nsites == 0) type_chunk.pop_back();
+  g->h_type_chunk = type_chunk;
   for (int i = 0; i < npts; ++i) {
     const double* p = point(i);
     double r2 = 0.0;
@@ -178,10 +189,11 @@ static csx_status get_geometry(Call& call, csx_structure* s, PointSet& ps, int L
   CSX_HIP(g->occ.upload(occ.data(), occ.size() * sizeof(float)), "upload occupancy");
   CSX_HIP(g->chunks.upload(chunks.data(), chunks.size() * sizeof(int)), "upload chunks");
   CSX_HIP(g->tile_l.upload(tile_l.data(), tile_l.size() * sizeof(int)), "upload tiles");
-  for (int nch : {1, 3})
-    if (multipole_coeff_lds_bytes(max_group_l_range(tile_l, multipole_tiles_per_group(g->ntiles)),
-                                  nch) > s->ctx->lds_per_workgroup)
-      return fail(CSX_INVALID_INPUT, "multipole l_max too large for the device's LDS");
+  CSX_HIP(g->type_chunk.upload(type_chunk.data(), type_chunk.size() * sizeof(int)),
+          "upload type chunks");
+  if (multipole_coeff_lds_bytes(max_group_l_range(tile_l, multipole_tiles_per_group(g->ntiles))) >
+      s->ctx->lds_per_workgroup)
+    return fail(CSX_INVALID_INPUT, "multipole l_max too large for the device's LDS");
   const size_t ybytes = static_cast<size_t>(g->ntiles) * g->npad * 16 * sizeof(float);
   CSX_HIP(g->Y.alloc(ybytes), "hipMalloc harmonics");
   CSX_HIP(launch_multipole_geometry(d_xyz.as<double>(), g->npad, d_centre.as<double>(), L,
@@ -198,6 +210,70 @@ static csx_status get_geometry(Call& call, csx_structure* s, PointSet& ps, int L
   return CSX_OK;
 }
 
+// Per-type coefficient tables S_t[q][lm] of a point set for a q grid and
+// l_max, in the point set's cache: computed by the coefficient kernel on a
+// miss.  On return the entry is ready and *lock holds it.
+static csx_status type_coefficients(Call& call, PointSet& ps, const MultipoleGeometry& g,
+                                    const double* q, size_t nq, double rmax,
+                                    std::shared_ptr<CacheEntry>* out,
+                                    std::unique_lock<std::mutex>* lock) {
+  std::string key = "multipole:" + std::to_string(g.L) + ":";
+  key.append(reinterpret_cast<const char*>(q), nq * sizeof(double));
+  std::shared_ptr<CacheEntry> e = cache_acquire(call.ctx(), ps, key, lock);
+  *out = e;
+  if (e->ready) return CSX_OK;
+  struct Dropper {
+    PointSet& ps;
+    const std::shared_ptr<CacheEntry>& e;
+    ~Dropper() {
+      if (!e->ready) cache_drop(ps, e);
+    }
+  } dropper{ps, e};
+  const int TT = static_cast<int>(g.h_type_chunk.size()) - 1;
+  if (TT != ps.ntypes) return fail(CSX_DEVICE_ERROR, "multipole type table mismatch");
+  const size_t nqp = (nq + 15) / 16 * 16;
+  const int ncol = g.ntiles * 16;
+  std::vector<float> qv(nqp, 0.0f);
+  std::vector<int> qlmax(nqp, 0);
+  for (size_t k = 0; k < nq; ++k) {
+    qv[k] = static_cast<float>(q[k]);
+    qlmax[k] = adaptive_l_max(q[k], rmax, g.L);
+  }
+  const hipStream_t st = call.stream();
+  CSX_HIP(e->qlmax.alloc(nqp * sizeof(int)), "hipMalloc orders");
+  CSX_HIP(e->coef.alloc(size_t(TT) * nqp * ncol * sizeof(double)), "hipMalloc coefficients");
+  Scratch d_q(call), d_partial(call);
+  CSX_HIP(d_q.alloc(nqp * sizeof(float), st), "hipMallocAsync");
+  CSX_HIP(d_partial.alloc(static_cast<size_t>(g.nchunk) * nqp * ncol * sizeof(float), st),
+          "hipMallocAsync");
+  void* h_in = nullptr;
+  CSX_HIP(call.take_pinned(2, nqp * (sizeof(float) + sizeof(int)), &h_in), "hipHostMalloc");
+  char* in = static_cast<char*>(h_in);
+  std::memcpy(in, qv.data(), nqp * sizeof(float));
+  std::memcpy(in + nqp * sizeof(float), qlmax.data(), nqp * sizeof(int));
+  CSX_HIP(hipMemcpyAsync(d_q.as<void>(), in, nqp * sizeof(float), hipMemcpyHostToDevice, st),
+          "hipMemcpyAsync");
+  CSX_HIP(hipMemcpyAsync(e->qlmax.as<void>(), in + nqp * sizeof(float), nqp * sizeof(int),
+                         hipMemcpyHostToDevice, st),
+          "hipMemcpyAsync");
+  const int l_range = max_group_l_range(g.h_tile_l, multipole_tiles_per_group(g.ntiles));
+  CSX_HIP(launch_multipole_coeff(g.radius.as<float>(), g.occ.as<float>(), g.chunks.as<int>(),
+                                 g.nchunk, g.npad, g.Y.as<float>(), g.ntiles, g.tile_l.as<int>(),
+                                 l_range, g.type_chunk.as<int>(), TT, d_q.as<float>(),
+                                 e->qlmax.as<int>(), static_cast<int>(nqp), d_partial.as<float>(),
+                                 e->coef.as<double>(), st, call.timer()),
+          "multipole_coeff");
+  // The entry is complete once the stream has run the kernels; every later
+  // use of it is stream-ordered after a synchronisation of this call.
+  CSX_HIP(hipStreamSynchronize(st), "hipStreamSynchronize");
+  e->nqp = static_cast<int>(nqp);
+  e->ncol = ncol;
+  e->bytes = size_t(TT) * nqp * ncol * sizeof(double) + nqp * sizeof(int) + key.size();
+  e->ready = true;
+  cache_commit(call.ctx(), ps, e);
+  return CSX_OK;
+}
+
 // mode 0: vacuum intensity; 1: solvent intensity; 2: six-term basis.
 static csx_status multipole_run(csx_structure* s, const double* q, size_t nq, int L,
                                 bool implicit_h, const csx_solvent_params* p, int mode,
@@ -209,91 +285,72 @@ static csx_status multipole_run(csx_structure* s, const double* q, size_t nq, in
   }
   Call call(s->ctx);
   CSX_TRY(call.begin());
-  PointSet* ps = nullptr;
+  std::shared_ptr<PointSet> ps;
   CSX_TRY(get_point_set(s, mode == 0 ? nullptr : p, &ps));
+  if (ps->ntypes > multipole_max_types())
+    return fail(CSX_INVALID_INPUT, "too many form-factor types for the multipole path");
   MultipoleGeometry* g = nullptr;
   CSX_TRY(get_geometry(call, s, *ps, L, &g));
   const bool grid = mode != 0 && p->hydration_model != CSX_HYDRATION_ANALYTIC;
+  const double rmax = grid ? std::max(g->rmax_atoms, g->rmax_sites) : g->rmax_atoms;
+  std::shared_ptr<CacheEntry> coef;
+  std::unique_lock<std::mutex> coef_lock;
+  CSX_TRY(type_coefficients(call, *ps, *g, q, nq, rmax, &coef, &coef_lock));
+  // FP64 amplitudes per type and q: the vacuum form factor (mode 0), the
+  // combined solvent amplitude v - G(c1) e + c2 h (mode 1), or the three
+  // channels v, e, h (mode 2); the site type carries the water amplitude of
+  // spherical.rs:449-451.
   const int nch = mode == 2 ? 3 : 1;
-  const size_t nqp = (nq + 15) / 16 * 16;
+  const int nqp = coef->nqp;
   const int T = static_cast<int>(s->types.size());
   const int TT = ps->ntypes;
-  std::vector<float> amp(static_cast<size_t>(TT) * nqp * nch, 0.0f);
-  std::vector<float> qv(nqp, 0.0f);
-  std::vector<int> qlmax(nqp, 0);
-  const double rmax = grid ? std::max(g->rmax_atoms, g->rmax_sites) : g->rmax_atoms;
+  std::vector<double> amp(static_cast<size_t>(TT) * nqp * nch, 0.0);
   for (size_t k = 0; k < nq; ++k) {
-    qv[k] = static_cast<float>(q[k]);
-    qlmax[k] = adaptive_l_max(q[k], rmax, L);
     const double gscale = mode == 1 ? excluded_volume_scale(q[k], p->c1) : 0.0;
     for (int t = 0; t < T; ++t) {
-      float* a = &amp[(static_cast<size_t>(t) * nqp + k) * nch];
+      double* a = &amp[(static_cast<size_t>(t) * nqp + k) * nch];
       if (mode == 0) {
-        a[0] = static_cast<float>(type_form_factor(s->types[t], implicit_h, q[k]));
+        a[0] = type_form_factor(s->types[t], implicit_h, q[k]);
         continue;
       }
       const Amplitudes c = multipole_amplitudes(s->types[t], implicit_h, q[k],
                                                 p->shell_thickness_angstrom,
                                                 p->bulk_density_e_per_a3);
       if (mode == 1) {
-        a[0] = static_cast<float>(c.v - gscale * c.e + (grid ? 0.0 : p->c2 * c.h));
+        a[0] = c.v - gscale * c.e + (grid ? 0.0 : p->c2 * c.h);
       } else {
-        a[0] = static_cast<float>(c.v);
-        a[1] = static_cast<float>(c.e);
-        a[2] = static_cast<float>(grid ? 0.0 : c.h);
+        a[0] = c.v;
+        a[1] = c.e;
+        a[2] = grid ? 0.0 : c.h;
       }
     }
     if (grid) {
       const double site = p->bulk_density_e_per_a3 / 10.0 * ps->site_volume * water_form_factor(q[k]);
-      float* a = &amp[(static_cast<size_t>(T) * nqp + k) * nch];
-      if (mode == 1) a[0] = static_cast<float>(p->c2 * site);
-      else a[2] = static_cast<float>(site);
+      double* a = &amp[(static_cast<size_t>(T) * nqp + k) * nch];
+      if (mode == 1) a[0] = p->c2 * site;
+      else a[2] = site;
     }
   }
-  const int nchunk = g->nchunk;
   const hipStream_t st = call.stream();
-  Scratch d_amp(call), d_q(call), d_l(call), d_partial(call), d_coef(call), d_out(call);
-  CSX_HIP(d_amp.alloc(amp.size() * sizeof(float), st), "hipMallocAsync");
-  CSX_HIP(d_q.alloc(nqp * sizeof(float), st), "hipMallocAsync");
-  CSX_HIP(d_l.alloc(nqp * sizeof(int), st), "hipMallocAsync");
-  CSX_HIP(d_partial.alloc(static_cast<size_t>(nchunk) * nch * nqp * g->ntiles * 16 * sizeof(float), st),
-          "hipMallocAsync");
-  CSX_HIP(d_out.alloc(nqp * nterms * sizeof(double), st), "hipMallocAsync");
-  CSX_HIP(d_coef.alloc(static_cast<size_t>(nch) * nqp * g->ntiles * 16 * sizeof(double), st),
-          "hipMallocAsync");
-  // Inputs and the result go through the stream's pinned staging, so the
-  // copies are asynchronous.
-  const size_t amp_bytes = amp.size() * sizeof(float), q_bytes = nqp * sizeof(float),
-               l_bytes = nqp * sizeof(int);
+  Scratch d_amp(call), d_out(call);
+  CSX_HIP(d_amp.alloc(amp.size() * sizeof(double), st), "hipMallocAsync");
+  CSX_HIP(d_out.alloc(size_t(nqp) * nterms * sizeof(double), st), "hipMallocAsync");
   void* h_in = nullptr;
   void* h_res = nullptr;
-  CSX_HIP(call.take_pinned(0, amp_bytes + q_bytes + l_bytes, &h_in), "hipHostMalloc");
-  CSX_HIP(call.take_pinned(1, nqp * nterms * sizeof(double), &h_res), "hipHostMalloc");
-  char* in = static_cast<char*>(h_in);
-  std::memcpy(in, amp.data(), amp_bytes);
-  std::memcpy(in + amp_bytes, qv.data(), q_bytes);
-  std::memcpy(in + amp_bytes + q_bytes, qlmax.data(), l_bytes);
-  CSX_HIP(hipMemcpyAsync(d_amp.as<void>(), in, amp_bytes, hipMemcpyHostToDevice, st),
+  CSX_HIP(call.take_pinned(0, amp.size() * sizeof(double), &h_in), "hipHostMalloc");
+  CSX_HIP(call.take_pinned(1, size_t(nqp) * nterms * sizeof(double), &h_res), "hipHostMalloc");
+  std::memcpy(h_in, amp.data(), amp.size() * sizeof(double));
+  CSX_HIP(hipMemcpyAsync(d_amp.as<void>(), h_in, amp.size() * sizeof(double),
+                         hipMemcpyHostToDevice, st),
           "hipMemcpyAsync");
-  CSX_HIP(hipMemcpyAsync(d_q.as<void>(), in + amp_bytes, q_bytes, hipMemcpyHostToDevice, st),
+  CSX_HIP(launch_multipole_contract(coef->coef.as<double>(), d_amp.as<double>(), TT, nch,
+                                    g->tile_l.as<int>(), coef->qlmax.as<int>(), nqp, coef->ncol,
+                                    d_out.as<double>(), st),
+          "multipole_contract");
+  CSX_HIP(hipMemcpyAsync(h_res, d_out.as<void>(), size_t(nqp) * nterms * sizeof(double),
+                         hipMemcpyDeviceToHost, st),
           "hipMemcpyAsync");
-  CSX_HIP(hipMemcpyAsync(d_l.as<void>(), in + amp_bytes + q_bytes, l_bytes, hipMemcpyHostToDevice,
-                         st), "hipMemcpyAsync");
-  const int l_range = max_group_l_range(g->h_tile_l, multipole_tiles_per_group(g->ntiles));
-  CSX_HIP(launch_multipole_coeff(g->radius.as<float>(), g->occ.as<float>(), g->chunks.as<int>(),
-                                 nchunk, g->npad, g->Y.as<float>(), g->ntiles,
-                                 g->tile_l.as<int>(), l_range, d_amp.as<float>(),
-                                 d_q.as<float>(), d_l.as<int>(), static_cast<int>(nqp), nch,
-                                 d_partial.as<float>(), d_coef.as<double>(), d_out.as<double>(),
-                                 st, call.timer()),
-          "multipole_coeff");
-  CSX_HIP(hipMemcpyAsync(h_res, d_out.as<void>(), nqp * nterms * sizeof(double),
-                         hipMemcpyDeviceToHost, st), "hipMemcpyAsync");
   d_amp.reset();
-  d_q.reset();
-  d_l.reset();
-  d_partial.reset();
-  d_coef.reset();
   d_out.reset();
   CSX_TRY(call.finish());
   const double* res = static_cast<const double*>(h_res);

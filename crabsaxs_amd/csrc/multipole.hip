@@ -9,13 +9,13 @@
 //                      in slabs of 16 points, each slab 256 floats laid out so
 //                      that lane 16k + n of a wave reads its four k-step values
 //                      (points 4 ks + k, column n) with one 16-byte load.
-//  multipole_coeff     C_c[q][lm] = sum_i A_c,i(q) j_l(q r_i) Y_lm,i as FP32
-//                      MFMA (v_mfma_f32_16x16x4f32: 16 q rows x 16 columns,
-//                      k = 4 points).  A workgroup owns a chunk of up to 512
-//                      points of one type, a block of 16 q and a group of
-//                      column tiles (4 waves x MAXT accumulators); the type's
-//                      amplitudes A_c(q) factor out of the chunk's sum and
-//                      multiply it once at the end.  Per slab of 16 points,
+//  multipole_coeff     S_t[q][lm] = sum_{i of type t} occ_i j_l(q r_i) Y_lm,i
+//                      as FP32 MFMA (v_mfma_f32_16x16x4f32: 16 q rows x 16
+//                      columns, k = 4 points).  A workgroup owns a chunk of up
+//                      to 512 points of one type, a block of 16 q and a group
+//                      of column tiles (4 waves x MAXT accumulators).  The
+//                      type's amplitudes factor out of its sum, so they are
+//                      applied after it (multipole_contract).  Per slab of 16 points,
 //                      each thread generates occ_i j_l(q r_i) for one
 //                      (point, q) and every l of the group into LDS (FP32
 //                      sin/cos; FP32 upward recurrence for l <= q r; FP64
@@ -24,9 +24,12 @@
 //                      divides), then the waves run the MFMAs of their tiles.  Tiles above the
 //                      block's largest per-q order are skipped.  Partial
 //                      coefficients per chunk are written in FP32.
-//  multipole_reduce    FP64 sum of the chunk partials per (q, column) over the
-//                      columns with l <= that q's order, and the quadratic
-//                      forms sum_lm C_a C_b (1 or 6 per q).
+//  multipole_sum_types FP64 sum of the chunk partials of each type, in chunk
+//                      order: the per-type coefficient tables S_t[q][lm],
+//                      which depend on the points, q grid and l_max only.
+//  multipole_contract  per q, the channel coefficients C_c = sum_t A_c,t(q)
+//                      S_t (FP64 amplitudes) and the quadratic forms
+//                      sum_lm C_a C_b (1 or 6 per q) over l <= that q's order.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -255,17 +258,16 @@ __device__ inline void bessel_column(float xf, int lq, int l_lo, int l_hi, float
   for (int l = max(l_zero, l_lo); l <= l_hi; ++l) X[(l - l_lo) * kXL] = 0.f;
 }
 
-// partial[((chunk * NCH + c) * nqp + q) * ncol + col]: a chunk holds points
-// of one type, so the sum over its points of j_l occ Y is formed once and
-// multiplied by the type's NCH amplitudes at q on the way out.
+// partial[(chunk * nqp + q) * ncol + col]: the sum over the chunk's points
+// (all of one type) of occ j_l Y.
 // At least four waves per SIMD (128 registers).
-template <int NCH, int MAXT>
+template <int MAXT>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) void multipole_coeff(
     const float* __restrict__ radius, const float* __restrict__ occ,
     const int4* __restrict__ chunks /* (first slab, end slab, type) */, int nchunk, int nslab,
     const float* __restrict__ Y, int ntiles, const int* __restrict__ tile_l,
-    const float* __restrict__ amp /* [ntypes][nqp][NCH] */, const float* __restrict__ qv,
-    const int* __restrict__ qlmax, int nqp, int ngroups, float* __restrict__ partial) {
+    const float* __restrict__ qv, const int* __restrict__ qlmax, int nqp, int ngroups,
+    float* __restrict__ partial) {
   extern __shared__ float smem[];
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -290,13 +292,10 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
   const int l_hi = min(tile_l[g1 - 1], Lb);
   const int lr = l_hi - l_lo + 1;
   const int4 ck = chunks[chunk];
-  const int s0 = ck.x, s1 = ck.y, ty = ck.z;
+  const int s0 = ck.x, s1 = ck.y;
   float* X = smem;                   // [lr][16 q][4 slots][4 ks]
-  float* ATAB = X + lr * kXL;        // [16 q][NCH]: the chunk type's amplitudes
-  float* PR = ATAB + kQB * NCH;      // [kChunk] radius and occupancy of the
+  float* PR = X + lr * kXL;          // [kChunk] radius and occupancy of the
   float* PO = PR + kChunk;           //   chunk's points
-  for (int k = threadIdx.x; k < kQB * NCH; k += kThreads)
-    ATAB[k] = amp[(size_t(ty) * nqp + q0) * NCH + k];
   for (int k = threadIdx.x; k < (s1 - s0) * kSlab; k += kThreads) {
     PR[k] = radius[s0 * kSlab + k];
     PO[k] = occ[s0 * kSlab + k];
@@ -380,52 +379,54 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
     if (t >= nt) continue;
     const int tile = g0 + wave + 4 * t;
 #pragma unroll
-    for (int c = 0; c < NCH; ++c)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int q = q0 + 4 * mk + r;
-        partial[((size_t(chunk) * NCH + c) * nqp + q) * ncol + tile * 16 + mn] =
-            acc[t][r] * ATAB[(4 * mk + r) * NCH + c];
-      }
+    for (int r = 0; r < 4; ++r) {
+      const int q = q0 + 4 * mk + r;
+      partial[(size_t(chunk) * nqp + q) * ncol + tile * 16 + mn] = acc[t][r];
+    }
   }
 }
 
-// coef[(c * nqp + q) * ncol + col] = FP64 sum over chunks, in chunk order,
-// of the partial coefficients; 0 for the columns with l > qlmax[q] (not
-// written for this q's block).  One thread per (q, column).
-template <int NCH>
-__global__ __launch_bounds__(256) void multipole_sum_chunks(const float* __restrict__ partial,
-                                                            const int* __restrict__ tile_l,
-                                                            const int* __restrict__ qlmax,
-                                                            int nchunk, int nqp, int ncol,
-                                                            double* __restrict__ coef) {
+// coef[(t * nqp + q) * ncol + col] = FP64 sum, in chunk order, of the
+// partial coefficients of type t's chunks; 0 for the columns with
+// l > qlmax[q] (not written for this q's block).  One thread per (q, column).
+__global__ __launch_bounds__(256) void multipole_sum_types(const float* __restrict__ partial,
+                                                           const int* __restrict__ type_chunk,
+                                                           const int* __restrict__ tile_l,
+                                                           const int* __restrict__ qlmax, int nqp,
+                                                           int ncol, double* __restrict__ coef) {
   const int col = blockIdx.x * blockDim.x + threadIdx.x;
-  const int q = blockIdx.y;
+  const int q = blockIdx.y, t = blockIdx.z;
   if (col >= ncol) return;
-  const bool live = tile_l[col / 16] <= qlmax[q];
-#pragma unroll
-  for (int c = 0; c < NCH; ++c) {
-    double v = 0.0;
-    if (live)
-#pragma unroll 32
-      for (int ch = 0; ch < nchunk; ++ch)
-        v += double(partial[((size_t(ch) * NCH + c) * nqp + q) * ncol + col]);
-    coef[(size_t(c) * nqp + q) * ncol + col] = v;
+  double v = 0.0;
+  if (tile_l[col / 16] <= qlmax[q]) {
+    const int c1 = type_chunk[t + 1];
+#pragma unroll 16
+    for (int ch = type_chunk[t]; ch < c1; ++ch)
+      v += double(partial[(size_t(ch) * nqp + q) * ncol + col]);
   }
+  coef[(size_t(t) * nqp + q) * ncol + col] = v;
 }
 
-// out[q * NT + k]: NCH = 1 -> sum C^2; NCH = 3 -> vv, ee, hh, ve, vh, eh,
-// over the columns with l <= qlmax[q].
+// out[q * NT + k], NCH = 1 -> sum C^2; NCH = 3 -> vv, ee, hh, ve, vh, eh,
+// over the columns with l <= qlmax[q], where C_c = sum_t amp[(t nqp + q) NCH
+// + c] coef_t (the types in order).  One workgroup per q.
 template <int NCH>
-__global__ __launch_bounds__(256) void multipole_reduce(const double* __restrict__ coef,
-                                                        const int* __restrict__ tile_l,
-                                                        const int* __restrict__ qlmax,
-                                                        int nqp, int ncol,
-                                                        double* __restrict__ out) {
+__global__ __launch_bounds__(256) void multipole_contract(const double* __restrict__ coef,
+                                                          const double* __restrict__ amp,
+                                                          int ntypes,
+                                                          const int* __restrict__ tile_l,
+                                                          const int* __restrict__ qlmax,
+                                                          int nqp, int ncol,
+                                                          double* __restrict__ out) {
   constexpr int NT = NCH == 1 ? 1 : 6;
+  constexpr int kMaxTypes = 256;
+  __shared__ double A[kMaxTypes * NCH];
   __shared__ double scratch[4 * NT];
   const int q = blockIdx.x;
   const int lq = qlmax[q];
+  for (int k = threadIdx.x; k < ntypes * NCH; k += blockDim.x)
+    A[k] = amp[(size_t(k / NCH) * nqp + q) * NCH + k % NCH];
+  __syncthreads();
   double s[NT];
 #pragma unroll
   for (int k = 0; k < NT; ++k) s[k] = 0.0;
@@ -433,7 +434,12 @@ __global__ __launch_bounds__(256) void multipole_reduce(const double* __restrict
     if (tile_l[col / 16] > lq) continue;
     double cc[NCH];
 #pragma unroll
-    for (int c = 0; c < NCH; ++c) cc[c] = coef[(size_t(c) * nqp + q) * ncol + col];
+    for (int c = 0; c < NCH; ++c) cc[c] = 0.0;
+    for (int t = 0; t < ntypes; ++t) {
+      const double v = coef[(size_t(t) * nqp + q) * ncol + col];
+#pragma unroll
+      for (int c = 0; c < NCH; ++c) cc[c] += A[t * NCH + c] * v;
+    }
     if (NCH == 1) {
       s[0] += cc[0] * cc[0];
     } else {
@@ -480,9 +486,11 @@ int multipole_tiles_per_group(int ntiles) {
   return (ntiles + small - 1) / small == (ntiles + big - 1) / big ? small : big;
 }
 
-size_t multipole_coeff_lds_bytes(int l_range, int nch) {
-  return (size_t(l_range) * kXL + size_t(kQB) * nch + 2 * kChunk) * sizeof(float);
+size_t multipole_coeff_lds_bytes(int l_range) {
+  return (size_t(l_range) * kXL + 2 * kChunk) * sizeof(float);
 }
+
+int multipole_max_types() { return 256; }
 
 hipError_t launch_multipole_geometry(const double* xyz, int npts, const double* centre,
                                      int l_max, const double* norms, const int* tile_first,
@@ -499,45 +507,43 @@ hipError_t launch_multipole_geometry(const double* xyz, int npts, const double* 
 
 hipError_t launch_multipole_coeff(const float* radius, const float* occ, const int* chunks,
                                   int nchunk, int npad, const float* Y, int ntiles,
-                                  const int* tile_l, int max_l_range, const float* amp,
-                                  const float* qv, const int* qlmax, int nqp, int nch,
-                                  float* partial, double* coef, double* out, hipStream_t stream,
+                                  const int* tile_l, int max_l_range, const int* type_chunk,
+                                  int ntypes, const float* qv, const int* qlmax, int nqp,
+                                  float* partial, double* coef, hipStream_t stream,
                                   KernelTimer* timer) {
-  if (nqp % kQB != 0 || npad % kSlab != 0 || ntiles <= 0 || nchunk <= 0 || (nch != 1 && nch != 3))
+  if (nqp % kQB != 0 || npad % kSlab != 0 || ntiles <= 0 || nchunk <= 0 || ntypes <= 0)
     return hipErrorInvalidValue;
   const int tpg = multipole_tiles_per_group(ntiles);
   const int ngroups = (ntiles + tpg - 1) / tpg;
   const long long total = (long long)(nqp / kQB) * nchunk * ngroups;
   if (total > INT32_MAX) return hipErrorInvalidValue;
-  const size_t lds = multipole_coeff_lds_bytes(max_l_range, nch);
+  const size_t lds = multipole_coeff_lds_bytes(max_l_range);
   const int4* ck = reinterpret_cast<const int4*>(chunks);
   const int nslab = npad / kSlab;
   {
     TimerScope scope(timer, "multipole_coeff", stream);
-#define CSX_MP_LAUNCH(NCH, MAXT)                                                             \
-  multipole_coeff<NCH, MAXT><<<unsigned(total), kThreads, lds, stream>>>(                      \
-      radius, occ, ck, nchunk, nslab, Y, ntiles, tile_l, amp, qv, qlmax, nqp, ngroups, partial)
-    if (nch == 1 && tpg == 4 * kMaxT)
-      CSX_MP_LAUNCH(1, kMaxT);
-    else if (nch == 1)
-      CSX_MP_LAUNCH(1, kMaxT / 2);
-    else if (tpg == 4 * kMaxT)
-      CSX_MP_LAUNCH(3, kMaxT);
+    if (tpg == 4 * kMaxT)
+      multipole_coeff<kMaxT><<<unsigned(total), kThreads, lds, stream>>>(
+          radius, occ, ck, nchunk, nslab, Y, ntiles, tile_l, qv, qlmax, nqp, ngroups, partial);
     else
-      CSX_MP_LAUNCH(3, kMaxT / 2);
-#undef CSX_MP_LAUNCH
+      multipole_coeff<kMaxT / 2><<<unsigned(total), kThreads, lds, stream>>>(
+          radius, occ, ck, nchunk, nslab, Y, ntiles, tile_l, qv, qlmax, nqp, ngroups, partial);
   }
   const int ncol = ntiles * 16;
-  const dim3 sum_grid((ncol + 255) / 256, nqp);
-  if (nch == 1) {
-    multipole_sum_chunks<1><<<sum_grid, 256, 0, stream>>>(partial, tile_l, qlmax, nchunk, nqp,
-                                                          ncol, coef);
-    multipole_reduce<1><<<nqp, 256, 0, stream>>>(coef, tile_l, qlmax, nqp, ncol, out);
-  } else {
-    multipole_sum_chunks<3><<<sum_grid, 256, 0, stream>>>(partial, tile_l, qlmax, nchunk, nqp,
-                                                          ncol, coef);
-    multipole_reduce<3><<<nqp, 256, 0, stream>>>(coef, tile_l, qlmax, nqp, ncol, out);
-  }
+  multipole_sum_types<<<dim3((ncol + 255) / 256, nqp, ntypes), 256, 0, stream>>>(
+      partial, type_chunk, tile_l, qlmax, nqp, ncol, coef);
+  return hipGetLastError();
+}
+
+hipError_t launch_multipole_contract(const double* coef, const double* amp, int ntypes, int nch,
+                                     const int* tile_l, const int* qlmax, int nqp, int ncol,
+                                     double* out, hipStream_t stream) {
+  if (ntypes <= 0 || ntypes > multipole_max_types() || (nch != 1 && nch != 3))
+    return hipErrorInvalidValue;
+  if (nch == 1)
+    multipole_contract<1><<<nqp, 256, 0, stream>>>(coef, amp, ntypes, tile_l, qlmax, nqp, ncol, out);
+  else
+    multipole_contract<3><<<nqp, 256, 0, stream>>>(coef, amp, ntypes, tile_l, qlmax, nqp, ncol, out);
   return hipGetLastError();
 }
 

@@ -4,8 +4,10 @@
   precision_20k     20,000 atoms, R = 35 A, q = linspace(0, 1, 500)
   prolate_100k      100,000 atoms in a 2:1 prolate spheroid of the volume of a
                     60 A sphere (semi-axes 47.6, 47.6, 95.2 A) plus hydration
-                    pseudo-atoms on a 3 A grid 3-6 A from the nearest atom;
-                    q = linspace(0.005, 0.5, 200)
+                    pseudo-atoms on a 3 A grid 3-6 A from the nearest atom,
+                    attached as hydration sites (voxel volume 27 A^3, used
+                    with HydrationModel.ExplicitSites); q = linspace(0.005,
+                    0.5, 200)
   fit_target_20k    precision_20k structure, 300 q = linspace(0.01, 1, 300),
                     target with scale 2.3, bg 1e-3, c1 1.02, c2 2.0 and
                     Gaussian noise sigma = 1% I + 1e-4 I(0)
@@ -31,9 +33,28 @@ def precision_20k(seed: int = 1):
     return generate_globule(20000, 35.0, 1.0, seed=seed), PRECISION_Q.copy()
 
 
+SITE_SPACING = 3.0
+
+
+def prolate_with_sites(n: int, seed: int, radius: float):
+    """A 2:1 prolate globule of n atoms (equatorial semi-axis `radius`) with
+    its hydration pseudo-atoms (3 A grid, nearest atom 3-6 A away) attached
+    as hydration sites of voxel volume 27 A^3."""
+    s = generate_globule(n, radius, 2.0, seed=seed)
+    sites = generate_shell_sites(s, SITE_SPACING, 3.0, 6.0)
+    s.set_hydration_sites(sites, SITE_SPACING ** 3)
+    return s, sites
+
+
 def prolate_100k(seed: int = 2, n: int = 100000):
-    s = generate_globule(n, 60.0 / 2 ** (1 / 3), 2.0, seed=seed)
-    sites = generate_shell_sites(s, 3.0, 3.0, 6.0)
+    s, sites = prolate_with_sites(n, seed, 60.0 / 2 ** (1 / 3))
+    return s, np.linspace(0.005, 0.5, 200), len(sites)
+
+
+def prolate_points_100k(seed: int = 2, n: int = 100000):
+    """The prolate_100k points with the pseudo-atoms as oxygen atoms (one
+    vacuum point set of atoms and sites together)."""
+    s, sites = prolate_with_sites(n, seed, 60.0 / 2 ** (1 / 3))
     xyz = np.vstack([s.xyz, sites])
     el = np.concatenate([s.element, np.full(len(sites), int(Element.O), dtype=np.uint8)])
     return Structure.from_arrays(xyz, el), np.linspace(0.005, 0.5, 200), len(sites)

@@ -54,11 +54,16 @@ typedef enum {
     CSX_FP32_COMPENSATED = 1
 } csx_precision;
 
-/* HydrationModel, src/solvent.rs:17-25. */
+/* HydrationModel, src/solvent.rs:17-25, plus one value with no reference
+ * counterpart: the hydration sites attached to the structure with
+ * csx_structure_set_hydration_sites, entering every path exactly like the
+ * sites of a Grid model (uniform amplitude rho / 10 * V_voxel * f_w(q),
+ * solvent.rs:560-573 and spherical.rs:449-451). */
 typedef enum {
     CSX_HYDRATION_ANALYTIC = 0,
     CSX_HYDRATION_GRID = 1,
-    CSX_HYDRATION_ADAPTIVE_GRID = 2
+    CSX_HYDRATION_ADAPTIVE_GRID = 2,
+    CSX_HYDRATION_EXPLICIT_SITES = 3
 } csx_hydration_model;
 
 /* ScatteringMethod, src/scattering.rs:16-21. */
@@ -144,6 +149,23 @@ csx_status csx_ctx_set_tuning(csx_ctx *ctx, int q_tile, int sin_mode,
 csx_status csx_ctx_last_launch(csx_ctx *ctx, int *q_tile, int *sin_mode,
                                int *persistent_workgroups);
 
+/* Results cached per structure and solvent model (no reference
+ * counterpart): the FP32 pair tables S_ab(q) of a q grid, and the per-type
+ * multipole coefficients sum_{i of type t} occ_i j_l(q r_i) Y_lm,i of a q
+ * grid and l_max (device memory).  Every Debye and multipole entry point
+ * reuses them, so calls that change only (c1, c2), the shell thickness,
+ * the bulk density or the hydrogen mode re-contract the cached tables
+ * instead of re-running the pair or coefficient kernel, with bitwise the
+ * results of a first call.  The key is the exact q grid (and l_max), the
+ * sine mode and j chunk of the pair kernel; each structure keeps its most
+ * recently used results up to `bytes` per solvent model (default 256 MiB;
+ * 0 turns caching off and frees nothing already cached until it is
+ * replaced).  Replaces the 25 x M compute_curve calls of
+ * EnsembleFitter::fit_core (ensemble.rs:234-261) and the repeated calls of
+ * SaxsScorer::score (analysis.rs:156-177) by re-contractions, with the
+ * callers unchanged. */
+csx_status csx_ctx_set_cache_limit(csx_ctx *ctx, size_t bytes);
+
 /* Upper bound (bytes, >= 1 MiB; default 2 GiB) on the partial-sum
  * workspace of one coordinate-adjoint launch; larger problems run as
  * several launches.  No reference counterpart. */
@@ -156,6 +178,18 @@ csx_status csx_ctx_set_workspace_limit(csx_ctx *ctx, size_t bytes);
  * |q| r_max / pi <= 250; otherwise the automatic choice among 16 and 32).
  * No reference counterpart. */
 csx_status csx_ctx_set_vjp_tile(csx_ctx *ctx, int q_tile);
+
+/* Workgroups sharing each 256-step sweep of the coordinate adjoint: 0 (the
+ * default: automatic, 2, 4 or 8 where the block pairs and q chunks leave
+ * most CUs idle, else 1), or 1, 2, 4 or 8.  Each part writes its own
+ * partial sums, added in a fixed order.  No reference counterpart. */
+csx_status csx_ctx_set_vjp_split(csx_ctx *ctx, int split);
+
+/* Shape of the most recent coordinate-adjoint call on the context: q tile,
+ * q chunks per workgroup split, step split, and the number of launches its
+ * block pairs took under the workspace limit.  Any output may be NULL. */
+csx_status csx_ctx_last_vjp_launch(csx_ctx *ctx, int *q_tile, int *q_split,
+                                   int *step_split, int *launches);
 
 /* Persistent launch of the FP32 pair kernel: a fixed grid of
  * workgroups_per_cu x CU workgroups drains a global task counter (0 = one
@@ -202,6 +236,18 @@ csx_status csx_structure_create(csx_ctx *ctx, size_t n, const double *xyz,
                                 csx_structure **out);
 void csx_structure_destroy(csx_structure *s);
 size_t csx_structure_atom_count(const csx_structure *s);
+
+/* Attach hydration sites to a structure: `count` points (xyz: 3*count
+ * doubles, A) of one voxel volume (A^3), used by every call whose solvent
+ * model is CSX_HYDRATION_EXPLICIT_SITES.  They replace the sites a Grid or
+ * AdaptiveGrid model builds from the structure (solvent.rs:121-401) with a
+ * list the caller made (the proposed csx_sites_create of the seam: a site
+ * list from the host grid builders or a generator).  A later call replaces
+ * the list; calls running meanwhile finish with the previous one.  No
+ * reference counterpart. */
+csx_status csx_structure_set_hydration_sites(csx_structure *s,
+                                             const double *xyz, size_t count,
+                                             double voxel_volume);
 
 /* ---- structure and curve ingest (src/structure.rs, src/io.rs) ---------- */
 

@@ -130,12 +130,14 @@ class Context {
   Context& operator=(const Context&) = delete;
   csx_ctx* get() const { return ptr_; }
 
-  // The process-wide context on device 0, created on first use.
+  // The process-wide context on device 0, created on first use and never
+  // destroyed: a static destructor would run after the HIP runtime has been
+  // torn down at exit and release streams and memory into it.
   static Context& default_context() {
     static std::mutex mu;
-    static std::unique_ptr<Context> ctx;
+    static Context* ctx = nullptr;
     std::lock_guard<std::mutex> lock(mu);
-    if (!ctx) ctx.reset(new Context(0));
+    if (!ctx) ctx = new Context(0);
     return *ctx;
   }
 
@@ -158,6 +160,8 @@ class Context {
   void set_workspace_limit(size_t bytes) {
     detail::check(csx_ctx_set_workspace_limit(ptr_, bytes));
   }
+  // Cached pair tables and multipole coefficients per structure (0: off).
+  void set_cache_limit(size_t bytes) { detail::check(csx_ctx_set_cache_limit(ptr_, bytes)); }
   int cu_count() const {
     int n = 0;
     detail::check(csx_ctx_device_info(ptr_, &n, nullptr, 0));
@@ -179,9 +183,11 @@ class Structure {
 
   Structure() = default;
   explicit Structure(std::vector<Atom> a) : atoms(std::move(a)) {}
-  Structure(const Structure& o) : atoms(o.atoms) {}
+  Structure(const Structure& o) : atoms(o.atoms) { copy_sites(o); }
   Structure& operator=(const Structure& o) {
+    if (this == &o) return *this;
     atoms = o.atoms;
+    copy_sites(o);
     std::lock_guard<std::mutex> lock(state_->mu);
     state_->uploads.clear();
     return *this;
@@ -198,6 +204,17 @@ class Structure {
   }
 
   size_t atom_count() const { return atoms.size(); }
+
+  // Hydration sites used with HydrationModel::ExplicitSites (no reference
+  // counterpart): 3 coordinates per site and one voxel volume.
+  void set_hydration_sites(std::vector<double> xyz, double voxel_volume) {
+    if (xyz.size() % 3 != 0) throw InvalidInput("hydration sites need 3 coordinates each");
+    std::lock_guard<std::mutex> lock(state_->mu);
+    state_->sites = std::move(xyz);
+    state_->site_volume = voxel_volume;
+    state_->has_sites = true;
+    ++state_->sites_version;
+  }
 
   // PDB / mmCIF input (src/structure.rs:205-346); defined with ParseOptions.
   static Structure from_pdb_file(const std::string& path);
@@ -217,7 +234,10 @@ class Structure {
     std::unique_ptr<Upload>* slot = nullptr;
     for (auto& u : state_->uploads)
       if (u->ctx == ctx.get()) slot = &u;
-    if (slot && (*slot)->fingerprint == fp && (*slot)->n == atoms.size()) return (*slot)->h;
+    if (slot && (*slot)->fingerprint == fp && (*slot)->n == atoms.size()) {
+      attach_sites(**slot);
+      return (*slot)->h;
+    }
     const size_t n = atoms.size();
     std::vector<double> xyz(3 * n), occ(n);
     std::vector<uint8_t> el(n);
@@ -239,6 +259,7 @@ class Structure {
     fresh->fingerprint = fp;
     fresh->n = n;
     csx_structure* h = fresh->h;
+    attach_sites(*fresh);
     if (slot)
       *slot = std::move(fresh);
     else
@@ -251,6 +272,7 @@ class Structure {
     csx_ctx* ctx = nullptr;
     uint64_t fingerprint = 0;
     size_t n = 0;
+    uint64_t sites_version = 0;
     csx_structure* h = nullptr;
     ~Upload() {
       if (h) csx_structure_destroy(h);
@@ -259,7 +281,35 @@ class Structure {
   struct State {
     std::mutex mu;
     std::vector<std::unique_ptr<Upload>> uploads;  // one per context
+    bool has_sites = false;
+    std::vector<double> sites;
+    double site_volume = 0.0;
+    uint64_t sites_version = 0;
   };
+  void copy_sites(const Structure& o) {
+    bool has;
+    std::vector<double> xyz;
+    double vol;
+    {
+      std::lock_guard<std::mutex> lock(o.state_->mu);
+      has = o.state_->has_sites;
+      xyz = o.state_->sites;
+      vol = o.state_->site_volume;
+    }
+    std::lock_guard<std::mutex> lock(state_->mu);
+    state_->has_sites = has;
+    state_->sites = std::move(xyz);
+    state_->site_volume = vol;
+    ++state_->sites_version;
+  }
+  // Called with state_->mu held.
+  void attach_sites(Upload& u) const {
+    if (!state_->has_sites || u.sites_version == state_->sites_version) return;
+    detail::check(csx_structure_set_hydration_sites(u.h, state_->sites.data(),
+                                                    state_->sites.size() / 3,
+                                                    state_->site_volume));
+    u.sites_version = state_->sites_version;
+  }
   std::unique_ptr<State> state_{new State};
 
   static uint64_t mix(uint64_t h, uint64_t w) {
@@ -415,7 +465,9 @@ inline double excluded_volume_for_atom(const Atom& a, bool implicit_hydrogen) {
 // ------------------------------------------------------------------ solvent
 // HydrationModel, SolventParams (src/solvent.rs:17-50).
 
-enum class HydrationModel { Analytic = 0, Grid = 1, AdaptiveGrid = 2 };
+// ExplicitSites (no reference counterpart): the sites given with
+// Structure::set_hydration_sites, entering like the sites of a Grid model.
+enum class HydrationModel { Analytic = 0, Grid = 1, AdaptiveGrid = 2, ExplicitSites = 3 };
 
 struct SolventParams {
   double c1 = 1.0;

@@ -34,5 +34,11 @@ def _built_artifacts():
 
 @pytest.fixture(scope="session")
 def gpu_ctx():
+    """The default context with the result cache off, so that every call
+    runs its kernels and comparisons between calls (launch shapes, batch
+    against single, run against run) compare two computations.  The cache
+    has its own tests, which turn it on."""
     import crabsaxs_amd as csx
-    return csx.default_context()
+    ctx = csx.default_context()
+    ctx.set_cache_limit(0)
+    return ctx

@@ -107,3 +107,37 @@ def test_vjp_over_several_launches(gpu_ctx):
     finally:
         gpu_ctx.set_workspace_limit()
     assert np.abs(many - one).max() <= 1e-12 * np.abs(one).max()
+
+
+def test_vjp_step_split_and_several_launches(gpu_ctx):
+    """A 300-atom structure: the automatic shape splits each sweep over 8
+    workgroups; a 1 MiB workspace then takes its block pairs in several
+    launches.  Every split (1, 2, 4, 8), in one launch or several, matches
+    the oracle gradient."""
+    s = csx.generate_globule(300, 9.0, 1.0, seed=57)
+    q = np.linspace(0.01, 0.9, 120)
+    w = np.random.default_rng(8).normal(size=q.size)
+    solvent = csx.SolventParams(c1=1.01, c2=0.8)
+    want = O.debye_grad_x(oatoms(s), q, w, solvent=(1.01, 0.8, 3.0, 0.334))
+    results = {}
+    try:
+        csx.debye_grad_x(s, q, w, solvent)
+        assert gpu_ctx.last_vjp_launch()["step_split"] == 8
+        assert gpu_ctx.last_vjp_launch()["launches"] == 1
+        gpu_ctx.set_workspace_limit(1 << 20)
+        for split in (1, 2, 4, 8):
+            gpu_ctx.set_vjp_split(split)
+            got = csx.debye_grad_x(s, q, w, solvent)
+            shape = gpu_ctx.last_vjp_launch()
+            assert shape["step_split"] == split
+            if split == 8:
+                assert shape["launches"] > 1, shape
+            e = np.abs(got - want).max() / np.abs(want).max()
+            print("vjp split", split, shape, "max err", e, flush=True)
+            assert e < 1e-5
+            results[split] = got
+    finally:
+        gpu_ctx.set_vjp_split()
+        gpu_ctx.set_workspace_limit()
+    for split in (2, 4, 8):
+        assert np.abs(results[split] - results[1]).max() < 1e-5 * np.abs(want).max()

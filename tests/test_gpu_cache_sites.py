@@ -1,0 +1,268 @@
+"""The result cache inside the library and hydration sites given by the
+caller (HydrationModel.ExplicitSites), on the GPU through the C ABI.
+
+Cache bar: repeated calls that change only (c1, c2) or other amplitude
+parameters run the pair or coefficient kernel once and return bitwise what
+a call without the cache returns.  Sites bar: the FP64 GPU path within 1e-9
+and the FP32 paths within 1e-5 of the oracle given the same sites
+(solvent.rs:504-584, spherical.rs:408-565 restated), the multipole path at
+l_max 15, 25 and 40 against the oracle's exact-j_l multipole.
+"""
+import threading
+
+import numpy as np
+import pytest
+
+import crabsaxs_amd as csx
+from crabsaxs_amd import workloads
+import oracle_py as O
+
+pytestmark = pytest.mark.gpu
+F32 = csx.Precision.FP32_COMPENSATED
+F64 = csx.Precision.FP64_PLAIN
+CANDIDATES = [(c1, c2) for c1 in np.linspace(0.95, 1.05, 5) for c2 in np.linspace(-2.0, 4.0, 5)]
+
+
+def rel(a, b):
+    return np.abs(a - b) / np.maximum(np.abs(b), 1e-300)
+
+
+def oatoms(s):
+    return O.Atoms(s.xyz, s.element, s.group, s.occupancy)
+
+
+def launches(ctx, kernel):
+    return ctx.kernel_time(kernel)[0]
+
+
+@pytest.mark.parametrize("method", [csx.ScatteringMethod.Debye, csx.ScatteringMethod.Multipole])
+def test_compute_curve_25_candidates_one_launch(gpu_ctx, method):
+    """The 25 (c1, c2) candidates of EnsembleFitter::fit_core
+    (ensemble.rs:243-261) through an unchanged compute_curve: one pair (or
+    coefficient) kernel launch, and the same bits as without the cache."""
+    s = csx.generate_globule(3000, 20.0, 1.0, seed=31)
+    q = np.linspace(0.005, 0.5, 120)
+    kernel = "debye_pairs_f32" if method == csx.ScatteringMethod.Debye else "multipole_coeff"
+
+    def curve(c1, c2):
+        o = csx.ComputeOptions(method=method, solvent=csx.SolventParams(c1=c1, c2=c2))
+        return csx.compute_curve(s, q, o).intensity
+
+    uncached = [curve(c1, c2) for c1, c2 in CANDIDATES]  # the fixture's context caches nothing
+    with gpu_ctx.caching(256 << 20):
+        gpu_ctx.reset_timing()
+        gpu_ctx.set_timing(True)
+        try:
+            cached = [curve(c1, c2) for c1, c2 in CANDIDATES]
+        finally:
+            gpu_ctx.set_timing(False)
+        assert launches(gpu_ctx, kernel) == 1
+    for a, b in zip(cached, uncached):
+        assert np.array_equal(a, b)
+
+
+def test_cached_results_match_the_oracle(gpu_ctx):
+    """Cached tables re-contracted at other solvent parameters and in the
+    other hydrogen mode agree with the FP64 oracle at those parameters."""
+    s = csx.generate_globule(900, 14.0, 1.0, seed=32)
+    q = np.linspace(0.0, 0.6, 41)
+    with gpu_ctx.caching(256 << 20):
+        csx.solvent_corrected_intensity(s, q, csx.SolventParams(c1=1.0, c2=0.0))
+        for c1, c2, t, rho in ((1.03, 1.5, 3.0, 0.334), (0.97, -0.5, 2.5, 0.34)):
+            p = csx.SolventParams(c1=c1, c2=c2, shell_thickness_angstrom=t,
+                                  bulk_density_e_per_a3=rho)
+            got = csx.solvent_corrected_intensity(s, q, p)
+            basis = O.debye_solvent_basis(oatoms(s), q, c1, c2, t, rho)
+            want = O.combine_solvent_basis(basis, q, c1, c2)
+            assert rel(got, want).max() < 1e-5
+        mp = csx.MultipoleParams(l_max=30)
+        csx.multipole_intensity(s, q, mp)
+        got = csx.multipole_solvent_basis(s, q, mp, csx.SolventParams(c1=1.02, c2=0.6))
+        want = O.multipole_solvent(oatoms(s), q, 30, c1=1.02, c2=0.6, basis=True, exact=True)
+        scale = np.abs(want).max(axis=1, keepdims=True)
+        assert (np.abs(got - want) / scale).max() < 1e-5
+
+
+def test_concurrent_callers_wait_for_one_computation(gpu_ctx):
+    """25 threads asking for the same tables at once (fit_core runs its
+    candidates in parallel, ensemble.rs:243-256): one launch, equal bits."""
+    s = csx.generate_globule(4000, 22.0, 1.0, seed=33)
+    q = np.linspace(0.0, 0.8, 150)
+    s.handle(gpu_ctx)
+    want = {k: csx.solvent_corrected_intensity(s, q, csx.SolventParams(c1=c1, c2=c2))
+            for k, (c1, c2) in enumerate(CANDIDATES)}
+    got, errors = {}, []
+
+    def work(k, c1, c2):
+        try:
+            got[k] = csx.solvent_corrected_intensity(s, q, csx.SolventParams(c1=c1, c2=c2))
+        except Exception as e:  # reported below
+            errors.append(e)
+
+    with gpu_ctx.caching(256 << 20):
+        gpu_ctx.reset_timing()
+        gpu_ctx.set_timing(True)
+        threads = [threading.Thread(target=work, args=(k, c1, c2))
+                   for k, (c1, c2) in enumerate(CANDIDATES)]
+        try:
+            for t in threads:
+                t.start()
+            for t in threads:
+                t.join()
+        finally:
+            gpu_ctx.set_timing(False)
+        assert not errors, errors
+        assert launches(gpu_ctx, "debye_pairs_f32") == 1
+    for k in want:
+        assert np.array_equal(got[k], want[k])
+
+
+def test_batch_reuses_and_fills_the_cache(gpu_ctx):
+    cs = [csx.generate_globule(n, 10.0 + n / 400, 1.0, seed=70 + n) for n in (400, 1500, 2600)]
+    q = np.linspace(0.0, 0.7, 64)
+    p = csx.SolventParams(c1=1.01, c2=0.3)
+    want = [csx.debye_solvent_basis(c, q, p) for c in cs]
+    with gpu_ctx.caching(256 << 20):
+        gpu_ctx.reset_timing()
+        gpu_ctx.set_timing(True)
+        try:
+            csx.debye_solvent_basis(cs[1], q, p)           # one set cached
+            b1 = csx.batch_debye_solvent_basis(cs, q, p)    # the other two in one launch
+            b2 = csx.batch_debye_solvent_basis(cs, q, p)    # all cached
+            singles = [csx.debye_solvent_basis(c, q, p) for c in cs]
+        finally:
+            gpu_ctx.set_timing(False)
+        assert launches(gpu_ctx, "debye_pairs_f32") == 2
+    for k in range(3):
+        assert np.array_equal(b1[k], want[k]) and np.array_equal(b2[k], want[k])
+        assert np.array_equal(singles[k], want[k])
+
+
+def test_cache_keys_on_the_q_grid_and_arithmetic(gpu_ctx):
+    s = csx.generate_globule(1200, 16.0, 1.0, seed=34)
+    q1 = np.linspace(0.0, 0.5, 50)
+    q2 = q1.copy()
+    q2[-1] = np.nextafter(q2[-1], 1.0)  # one bit apart: another grid
+    with gpu_ctx.caching(256 << 20):
+        gpu_ctx.reset_timing()
+        gpu_ctx.set_timing(True)
+        try:
+            a = csx.debye_intensity(s, q1)
+            csx.debye_intensity(s, q2)
+            gpu_ctx.set_tuning(0, 1, 512)  # another sine: not the cached tables
+            b = csx.debye_intensity(s, q1)
+            gpu_ctx.set_tuning(0, 2, 512)
+            c = csx.debye_intensity(s, q1)  # cached again
+        finally:
+            gpu_ctx.set_timing(False)
+            gpu_ctx.set_tuning()
+        assert launches(gpu_ctx, "debye_pairs_f32") == 3
+    assert np.array_equal(a, c) and rel(a, b).max() < 1e-5
+
+
+def test_cache_limit_evicts(gpu_ctx):
+    s = csx.generate_globule(800, 13.0, 1.0, seed=35)
+    grids = [np.linspace(0.0, 0.5, 40 + k) for k in range(4)]
+    # each entry: 10 bins x ~42 q x 8 B plus its key, ~4 KB; a 6 KB limit keeps one
+    with gpu_ctx.caching(6 << 10):
+        gpu_ctx.reset_timing()
+        gpu_ctx.set_timing(True)
+        try:
+            for q in grids + grids:
+                csx.debye_intensity(s, q)
+        finally:
+            gpu_ctx.set_timing(False)
+        assert launches(gpu_ctx, "debye_pairs_f32") == 8
+    s = csx.Structure.from_arrays(s.xyz, s.element)  # a structure with nothing cached
+    with gpu_ctx.caching(1 << 20):
+        gpu_ctx.reset_timing()
+        gpu_ctx.set_timing(True)
+        try:
+            for q in grids + grids:
+                csx.debye_intensity(s, q)
+        finally:
+            gpu_ctx.set_timing(False)
+        assert launches(gpu_ctx, "debye_pairs_f32") == 4
+
+
+def prolate_small(seed=41):
+    return workloads.prolate_with_sites(3000, seed, 20.0 / 2 ** (1 / 3))
+
+
+def test_explicit_sites_debye_match_oracle(gpu_ctx):
+    s, sites = prolate_small()
+    assert len(sites) > 500
+    q = np.linspace(0.0, 0.5, 33)
+    p = csx.SolventParams(c1=1.02, c2=1.1, hydration_model=csx.HydrationModel.ExplicitSites)
+    want = O.debye_solvent_basis(oatoms(s), q, 1.02, 1.1, sites=sites, voxel_volume=27.0)
+    scale = np.abs(want).max(axis=1, keepdims=True)
+    for prec, tol in ((F64, 1e-9), (F32, 1e-5)):
+        got = csx.debye_solvent_basis(s, q, p, precision=prec)
+        err = (np.abs(got - want) / scale).max()
+        print("explicit sites Debye", prec.name, err, flush=True)
+        assert err < tol
+        inten = csx.solvent_corrected_intensity(s, q, p, precision=prec)
+        assert rel(inten, O.combine_solvent_basis(want, q, 1.02, 1.1)).max() < 10 * tol
+
+
+@pytest.mark.parametrize("l_max", [15, 25, 40])
+def test_explicit_sites_multipole_match_exact_oracle(gpu_ctx, l_max):
+    """The c2 workload's shape at an oracle-feasible size: a 2:1 prolate
+    globule with 3 A shell sites as hydration sites, at the config's orders."""
+    s, sites = prolate_small()
+    q = np.linspace(0.005, 0.5, 24)
+    p = csx.SolventParams(c1=1.0, c2=1.0, hydration_model=csx.HydrationModel.ExplicitSites)
+    mp = csx.MultipoleParams(l_max=l_max)
+    got_i = csx.multipole_solvent_intensity(s, q, mp, p)
+    got_b = csx.multipole_solvent_basis(s, q, mp, p)
+    want_i = O.multipole_solvent(oatoms(s), q, l_max, c1=1.0, c2=1.0, sites=sites,
+                                 voxel_volume=27.0, exact=True)
+    want_b = O.multipole_solvent(oatoms(s), q, l_max, c1=1.0, c2=1.0, sites=sites,
+                                 voxel_volume=27.0, basis=True, exact=True)
+    print("explicit sites multipole L", l_max, rel(got_i, want_i).max(), flush=True)
+    assert rel(got_i, want_i).max() < 1e-5
+    scale = np.abs(want_b).max(axis=1, keepdims=True)
+    assert (np.abs(got_b - want_b) / scale).max() < 1e-5
+
+
+def test_explicit_sites_need_sites_and_are_replaceable(gpu_ctx):
+    s = csx.generate_globule(500, 12.0, 1.0, seed=42)
+    q = np.linspace(0.01, 0.3, 10)
+    p = csx.SolventParams(c2=1.0, hydration_model=csx.HydrationModel.ExplicitSites)
+    with pytest.raises(csx.InvalidInput):
+        csx.solvent_corrected_intensity(s, q, p)
+    sites = csx.generate_shell_sites(s, 3.0, 3.0, 6.0)
+    s.set_hydration_sites(sites, 27.0)
+    a = csx.solvent_corrected_intensity(s, q, p, precision=F64)
+    s.set_hydration_sites(sites[: len(sites) // 2], 27.0)
+    b = csx.solvent_corrected_intensity(s, q, p, precision=F64)
+    want_b = O.combine_solvent_basis(
+        O.debye_solvent_basis(oatoms(s), q, 1.0, 1.0, sites=sites[: len(sites) // 2],
+                              voxel_volume=27.0), q, 1.0, 1.0)
+    assert rel(b, want_b).max() < 1e-9 and rel(a, b).max() > 1e-3
+    bad = csx.Structure.from_arrays(s.xyz, s.element)
+    bad.set_hydration_sites([[np.nan, 0.0, 0.0]], 27.0)
+    with pytest.raises(csx.InvalidInput):
+        csx.solvent_corrected_intensity(bad, q, p)
+
+
+def test_prolate_100k_with_sites(gpu_ctx):
+    """The full c2 workload (100,000 atoms + shell sites as hydration sites):
+    at l_max 64 the expansion has converged over the whole q range and the
+    multipole intensity meets 1e-5 against FP64 GPU Debye with the same
+    sites; at l_max 15 and 40 the GPU matches the exact-j_l oracle on a q
+    subset (the truncated expansion itself)."""
+    s, q, nsites = workloads.prolate_100k()
+    p = csx.SolventParams(c1=1.0, c2=1.0, hydration_model=csx.HydrationModel.ExplicitSites)
+    qs = q[::25]
+    ref = csx.solvent_corrected_intensity(s, qs, p, precision=F64)
+    got = csx.multipole_solvent_intensity(s, qs, csx.MultipoleParams(l_max=64), p)
+    print("100k + sites L=64 vs FP64 Debye", rel(got, ref).max(), flush=True)
+    assert rel(got, ref).max() < 1e-5
+    sites = s.sites[0]
+    for L in (15, 40):
+        want = O.multipole_solvent(oatoms(s), qs[::2], L, c1=1.0, c2=1.0, sites=sites,
+                                   voxel_volume=27.0, exact=True)
+        got = csx.multipole_solvent_intensity(s, qs[::2], csx.MultipoleParams(l_max=L), p)
+        print("100k + sites L", L, "vs exact oracle", rel(got, want).max(), flush=True)
+        assert rel(got, want).max() < 1e-5

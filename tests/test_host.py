@@ -1,6 +1,8 @@
 """CPU tests of the library's host side: ABI exports, typing, tables, grids,
 generator, and error behaviour without a device (this container has none)."""
 import ctypes
+import json
+import math
 import re
 from pathlib import Path
 
@@ -10,6 +12,8 @@ import pytest
 import crabsaxs_amd as csx
 from crabsaxs_amd import _lib
 import oracle_py as O
+
+GOLDEN = Path(__file__).resolve().parent / "golden"
 
 ROOT = Path(__file__).resolve().parents[1]
 
@@ -149,3 +153,52 @@ def test_combine_solvent_basis_matches_oracle():
         a = csx.combine_solvent_basis(basis, q, c1, c2)
         b = O.combine_solvent_basis(basis, q, c1, c2)
         assert np.array_equal(a, b)
+
+
+def _header_table(name):
+    """A numeric array of include/crabsaxs_tables.h (rows for 2-D arrays)."""
+    import re
+    text = (Path(__file__).resolve().parents[1] / "include" / "crabsaxs_tables.h").read_text()
+    m = re.search(name + r"\[[^\]]*\](?:\[[^\]]*\])?\s*=\s*\{(.*?)\};", text, re.S)
+    body = re.sub(r"/\*.*?\*/", "", m.group(1), flags=re.S)
+    rows = re.findall(r"\{([^{}]*)\}", body)
+    if rows:
+        return [[float(v) for v in r.split(",") if v.strip()] for r in rows]
+    return [float(v) for v in body.split(",") if v.strip()]
+
+
+def test_tables_equal_the_reference_sources():
+    """include/crabsaxs_tables.h against the numbers parsed from the
+    reference's own sources (tests/golden/make_tables_fixture.py): every
+    coefficient exactly, and the library's form factors, displaced volumes
+    and hydrogen counts evaluated from them."""
+    ref = json.loads((GOLDEN / "reference_tables.json").read_text())
+    assert ref["elements"] == [e.name for e in csx.Element]
+    cm = ref["cromer_mann"]
+    assert _header_table("CSX_CM_A") == [r["a"] for r in cm]
+    assert _header_table("CSX_CM_B") == [r["b"] for r in cm]
+    assert _header_table("CSX_CM_C") == [r["c"] for r in cm]
+    ua = ref["united_atom"]
+    assert _header_table("CSX_UA_A") == [r["a"] for r in ua]
+    assert _header_table("CSX_UA_B") == [r["b"] for r in ua]
+    assert _header_table("CSX_UA_C") == [r["c"] for r in ua]
+    assert _header_table("CSX_UA_HYDROGENS") == [r["hydrogens"] for r in ua]
+    assert _header_table("CSX_EXCLUDED_VOLUME") == ref["excluded_volume"]
+    assert _header_table("CSX_VDW_RADIUS") == ref["van_der_waals_radius"]
+    lib = _lib.load()
+    qs = [0.0, 0.05, 0.3, 0.7, 1.2]
+
+    def gauss(a, b, c, q):
+        s2 = (q / (4 * math.pi)) ** 2
+        return sum(ak * math.exp(-bk * s2) for ak, bk in zip(a, b)) + c
+    for e, r in enumerate(cm):
+        for q in qs:
+            want = gauss(r["a"], r["b"], r["c"], q)
+            assert abs(lib.csx_form_factor(e, -1, 1.0, 0, q) - want) <= 1e-14 * max(abs(want), 1)
+        assert lib.csx_excluded_volume(e, -1, 0) == ref["excluded_volume"][e]
+    for g, r in enumerate(ua):
+        for q in qs:
+            want = gauss(r["a"], r["b"], r["c"], q)
+            assert abs(lib.csx_form_factor(1, g, 1.0, 1, q) - want) <= 1e-14 * max(abs(want), 1)
+        assert lib.csx_excluded_volume(1, g, 1) == pytest.approx(
+            ref["excluded_volume"][1] + r["hydrogens"] * ref["excluded_volume"][0], abs=1e-12)
