@@ -174,6 +174,8 @@ SIGNATURES = {
     "csx_ctx_set_vjp_split": (C.c_int, [C.c_void_p, C.c_int]),
     "csx_ctx_last_vjp_launch": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int),
                                           C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "csx_debye_pair_tables_noalloc": (C.c_int, [C.c_void_p, c_double_p, C.c_size_t, C.c_void_p,
+                                                C.c_void_p]),
     "csx_ctx_last_launch": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int),
                                       C.POINTER(C.c_int)]),
     "csx_batch_debye_intensity": (C.c_int, [C.POINTER(C.c_void_p), C.c_size_t, c_double_p,

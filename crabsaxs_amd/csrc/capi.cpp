@@ -104,6 +104,12 @@ static csx_status build_point_set(csx_structure* s, const csx_solvent_params* p,
   for (int t = 0; t < TT; ++t) seg_start[t + 1] += seg_start[t];
 
   ps.nbins = TT * (TT + 1) / 2;
+  // The tables at q = 0 are the occupancy products sum_{i<j} occ_i occ_j.
+  std::vector<double> q0;
+  for (int a = 0; a < TT; ++a)
+    for (int b = a; b < TT; ++b)
+      q0.push_back(a == b ? 0.5 * (ps.wsum[a] * ps.wsum[a] - ps.self[a]) : ps.wsum[a] * ps.wsum[b]);
+
   // FP64 layout in original order.
   std::vector<double> pts64(4 * npts, 0.0), occ64(npts);
   std::vector<int> type64(npts);
@@ -131,6 +137,7 @@ static csx_status build_point_set(csx_structure* s, const csx_solvent_params* p,
   CSX_HIP(ps.seg32.upload(seg_start.data(), seg_start.size() * sizeof(int)), "upload segments");
   ps.h_seg = seg_start;
   CSX_HIP(ps.pts32.upload(pts32.data(), pts32.size() * sizeof(float)), "upload points");
+  CSX_HIP(ps.q0.upload(q0.data(), q0.size() * sizeof(double)), "upload q = 0 tables");
   CSX_HIP(ps.pts64.upload(pts64.data(), pts64.size() * sizeof(double)), "upload points");
   CSX_HIP(ps.type64.upload(type64.data(), type64.size() * sizeof(int)), "upload types");
   CSX_HIP(ps.occ64.upload(occ64.data(), occ64.size() * sizeof(double)), "upload occupancy");
@@ -754,6 +761,11 @@ csx_status csx_ctx_create(int device, csx_ctx** out) {
   if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess &&
       lds > 0)
     ctx->lds_per_workgroup = static_cast<size_t>(lds);
+  // The no-allocation entry point's device scratch, located once per device
+  // with the device current; a failure here disables that entry point only.
+  hipError_t e2 = hipSetDevice(device);
+  if (e2 == hipSuccess) e2 = noalloc_scratch_setup(device);
+  ctx->scratch_status = static_cast<int>(e2);
   *out = ctx;
   return CSX_OK;
 }
@@ -938,6 +950,13 @@ csx_status csx_structure_create(csx_ctx* ctx, size_t n, const double* xyz, const
   if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
   std::shared_ptr<PointSet> ps;
   CSX_TRY(get_point_set(s.get(), nullptr, &ps));
+  int j_chunk;
+  {
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    j_chunk = ctx->tuning.j_chunk;
+  }
+  std::shared_ptr<ItemSet> is;
+  CSX_TRY(get_item_set(*ps, j_chunk, &is));
   *out = s.release();
   return CSX_OK;
 }
@@ -1067,6 +1086,96 @@ csx_status csx_debye_pair_tables(csx_structure* s, const double* q, size_t nq,
   std::vector<double> P;
   CSX_TRY(pair_tables_f32(call, *ps, q, nq, P));
   std::copy(P.begin(), P.end(), out);
+  return CSX_OK;
+}
+
+csx_status csx_debye_pair_tables_noalloc(csx_structure* s, const double* q, size_t nq,
+                                         double* tables, void* stream) {
+  if (!s) return fail(CSX_INVALID_INPUT, "structure handle is null");
+  if (nq == 0) return CSX_OK;
+  if (!q || !tables) return fail(CSX_INVALID_INPUT, "q and the device table buffer are required");
+  if (nq > static_cast<size_t>(INT32_MAX / 64)) return fail(CSX_INVALID_INPUT, "too many q values");
+  csx_ctx* ctx = s->ctx;
+  CSX_HIP(hipSetDevice(ctx->device), "hipSetDevice");
+  Tuning tune;
+  int scratch_status;
+  {
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    tune = ctx->tuning;
+    scratch_status = ctx->scratch_status;
+  }
+  size_t cap = 0;
+  unsigned char* base = static_cast<unsigned char*>(noalloc_scratch(ctx->device, &cap));
+  if (!base)
+    return fail(CSX_DEVICE_ERROR,
+                std::string("the no-allocation scratch is not available on this device: ") +
+                    hipGetErrorString(static_cast<hipError_t>(scratch_status)));
+  // The atoms' point set and work items are made when the structure is
+  // created (and for another j chunk by the first ordinary call with it).
+  std::shared_ptr<PointSet> ps;
+  {
+    std::lock_guard<std::mutex> lock(s->mu);
+    auto it = s->sets.find("atoms");
+    if (it != s->sets.end()) ps = it->second;
+  }
+  if (!ps) return fail(CSX_INVALID_INPUT, "the structure has no device point set");
+  std::shared_ptr<ItemSet> is;
+  {
+    std::lock_guard<std::mutex> lock(ps->items_mu);
+    auto it = ps->item_sets.find(tune.j_chunk);
+    if (it != ps->item_sets.end()) is = it->second;
+  }
+  if (!is)
+    return fail(CSX_INVALID_INPUT,
+                "no work items for the context's j chunk on this structure: call "
+                "csx_debye_pair_tables on it once after changing the j chunk");
+  const int sin_mode = pair_sin_mode(tune.sin_mode, q, nq, ps->max_radius);
+  double sum_j = 0.0, max_j = 0.0;
+  for (size_t k = 3; k < is->h_items.size(); k += 4) {
+    sum_j += is->h_items[k];
+    max_j = std::max(max_j, static_cast<double>(is->h_items[k]));
+  }
+  const int cus = ctx->cu_count;
+  const int qt = choose_q_tile(tune.q_tile, sin_mode, nq, sum_j, max_j, cus);
+  const size_t nq_pad = (nq + qt - 1) / qt * qt;
+  const size_t nitems = static_cast<size_t>(is->nitems), nbins = static_cast<size_t>(ps->nbins);
+  const int persist_wgs = pair_persist_wgs(tune.persist, nitems, nq_pad, qt, cus);
+  // scratch: task counter | q (revolutions) | item partials | raw tables
+  auto round256 = [](size_t b) { return (b + 255) / 256 * 256; };
+  const size_t off_q = 256;
+  const size_t off_partial = off_q + round256(nq_pad * sizeof(float));
+  const size_t off_raw = off_partial + round256(nitems * nq_pad * sizeof(double));
+  const size_t need = off_raw + round256(nbins * nq_pad * sizeof(double));
+  if (need > cap) {
+    char msg[256];
+    std::snprintf(msg, sizeof msg,
+                  "this call needs %zu bytes of device scratch; the no-allocation entry point "
+                  "has %zu (use csx_debye_pair_tables for larger problems)",
+                  need, cap);
+    return fail(CSX_INVALID_INPUT, msg);
+  }
+  int* counter = reinterpret_cast<int*>(base);
+  float* qrev = reinterpret_cast<float*>(base + off_q);
+  double* partial = reinterpret_cast<double*>(base + off_partial);
+  double* raw = reinterpret_cast<double*>(base + off_raw);
+  const hipStream_t st = static_cast<hipStream_t>(stream);
+  {
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    ctx->last_launch[0] = qt;
+    ctx->last_launch[1] = sin_mode;
+    ctx->last_launch[2] = persist_wgs;
+  }
+  CSX_HIP(launch_write_floats(qrev, q, static_cast<int>(nq), static_cast<int>(nq_pad), 2.0 * kPi,
+                              st),
+          "write_floats");
+  CSX_HIP(launch_debye_pairs(ps->pts32.as<float4>(), is->items.as<int4>(), static_cast<int>(nitems),
+                             is->bin_first.as<int>(), static_cast<int>(nbins), qrev,
+                             static_cast<int>(nq_pad), qt, sin_mode, persist_wgs, counter, partial,
+                             raw, st, nullptr),
+          "debye_pairs_f32");
+  CSX_HIP(launch_finish_tables(raw, qrev, ps->q0.as<double>(), static_cast<int>(nbins),
+                               static_cast<int>(nq), static_cast<int>(nq_pad), tables, st),
+          "finish_tables");
   return CSX_OK;
 }
 

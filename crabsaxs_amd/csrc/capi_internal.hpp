@@ -55,6 +55,7 @@ struct csx_ctx {
   int cu_count = 256;
   size_t lds_per_workgroup = 64 * 1024;  // the device's per-workgroup LDS limit
   int last_launch[3] = {0, 0, 0};  // q tile, sin mode, persistent workgroups
+  int scratch_status = 0;          // hipError_t of the no-allocation scratch set-up
   int last_vjp[4] = {0, 0, 0, 0};  // q tile, q split, step split, launches
   // Grow-only device buffers kept per stream (see csx::Scratch), and
   // grow-only pinned host staging buffers per stream; freed with the context.
@@ -163,6 +164,7 @@ struct PointSet {
   double site_volume = 0.0;
   std::vector<double> self;   // per type: sum occ^2
   std::vector<double> wsum;   // per type: sum occ
+  DevArray q0;                // double[nbins]: the pair tables at q = 0
   // FP32 pair-table layout: type-sorted, centred, w = occupancy.
   DevArray pts32;             // float4[natoms + nsites]
   double max_radius = 0.0;    // of the FP32 points about the centroid

@@ -18,6 +18,8 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <map>
+#include <mutex>
 
 #include "kernels.hpp"
 
@@ -25,6 +27,7 @@ namespace csx {
 namespace {
 
 constexpr int kBlock = 256;
+constexpr int kFloatsPerLaunch = 512;
 
 // Fixed-order FP64 sum over a 256-thread workgroup; valid in thread 0.
 template <int NV>
@@ -292,6 +295,29 @@ __global__ __launch_bounds__(kBlock) void debye_pairs_f32_persistent(
 
 __global__ void reset_counter(int* counter) { *counter = 0; }
 
+// dst[0 .. n) = v[0 .. n): values handed over as a kernel argument, so that
+// host data reaches the device without a staging allocation.
+struct FloatBlock {
+  float v[kFloatsPerLaunch];
+};
+__global__ void write_floats(float* __restrict__ dst, int n, FloatBlock b) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) dst[k] = b.v[k];
+}
+
+// tables[b * nq + k] = raw[b * nq_pad + k] / (2 pi qrev[k]), or q0[b] where
+// that is 0 (the occupancy products of the q = 0 row): the host's final step
+// of the pair tables, in the same operations.
+__global__ void finish_tables(const double* __restrict__ raw, const float* __restrict__ qrev,
+                              const double* __restrict__ q0, int nbins, int nq, int nq_pad,
+                              double* __restrict__ tables) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const int b = blockIdx.y;
+  if (k >= nq || b >= nbins) return;
+  const double qe = 2.0 * 3.14159265358979323846 * double(qrev[k]);
+  tables[size_t(b) * nq + k] = qe == 0.0 ? q0[b] : raw[size_t(b) * nq_pad + k] / qe;
+}
+
 // out[b][q] = sum_{items of bin b, in order} partial[item][q].
 __global__ void reduce_bins(const double* __restrict__ partial, const int* __restrict__ bin_first,
                             int nq_pad, double* __restrict__ out) {
@@ -308,6 +334,59 @@ __global__ void reduce_bins(const double* __restrict__ partial, const int* __res
 }  // namespace
 
 // ------------------------------------------------------------- launchers --
+
+// ------------------------------------------------ no-allocation scratch --
+
+// Device memory of the no-allocation pair-table entry point: the launch's
+// task counter, q in revolutions, item partials and raw tables.  Every GPU
+// that loads the library holds one copy until the process ends; calls on
+// one GPU share it, so calls that overlap on one GPU race.
+constexpr size_t kNoAllocScratchBytes = size_t(64) << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+static std::mutex g_scratch_mu;
+static std::map<int, void*> g_scratch_ptr;  // device -> address of `scratch`
+
+hipError_t noalloc_scratch_setup(int device) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr.count(device)) return hipSuccess;
+  int current = -1;
+  hipError_t e = hipGetDevice(&current);
+  if (e != hipSuccess) return e;
+  if (current != device) return hipErrorInvalidDevice;  // the caller makes it current
+  void* ptr = nullptr;
+  e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess) return e;
+  g_scratch_ptr[device] = ptr;
+  return hipSuccess;
+}
+
+void* noalloc_scratch(int device, size_t* bytes) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  auto it = g_scratch_ptr.find(device);
+  if (bytes) *bytes = kNoAllocScratchBytes;
+  return it == g_scratch_ptr.end() ? nullptr : it->second;
+}
+
+hipError_t launch_write_floats(float* dst, const double* q, int n, int n_total, double divisor,
+                               hipStream_t stream) {
+  for (int k0 = 0; k0 < n_total; k0 += kFloatsPerLaunch) {
+    FloatBlock b;
+    const int m = n_total - k0 < kFloatsPerLaunch ? n_total - k0 : kFloatsPerLaunch;
+    for (int k = 0; k < kFloatsPerLaunch; ++k)
+      b.v[k] = k < m && k0 + k < n ? static_cast<float>(q[k0 + k] / divisor) : 0.f;
+    write_floats<<<(m + 255) / 256, 256, 0, stream>>>(dst + k0, m, b);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_finish_tables(const double* raw, const float* qrev, const double* q0, int nbins,
+                                int nq, int nq_pad, double* tables, hipStream_t stream) {
+  if (nbins <= 0 || nq <= 0) return hipSuccess;
+  finish_tables<<<dim3((nq + 255) / 256, nbins), 256, 0, stream>>>(raw, qrev, q0, nbins, nq,
+                                                                  nq_pad, tables);
+  return hipGetLastError();
+}
 
 hipError_t launch_debye_plain(const double4* pts, const double* amp, const double* q, int npts,
                               int nq, int nch, int nterms, double* partial, double* out,

@@ -102,4 +102,16 @@ hipError_t launch_debye_pairs(const float4* pts, const int4* items, int nitems, 
                               int persist_wgs, int* counter, double* partial, double* out,
                               hipStream_t stream, KernelTimer* timer);
 
+// The no-allocation entry point's device scratch (debye.hip): its address on
+// a device (hipGetSymbolAddress, once per device with that device current),
+// then the address and size for use by host code.
+hipError_t noalloc_scratch_setup(int device);
+void* noalloc_scratch(int device, size_t* bytes);
+// dst[k] = float(q[k] / divisor) for k < n and 0 for n <= k < n_total, the
+// values passed as kernel arguments (no staging allocation).
+hipError_t launch_write_floats(float* dst, const double* q, int n, int n_total, double divisor,
+                               hipStream_t stream);
+hipError_t launch_finish_tables(const double* raw, const float* qrev, const double* q0, int nbins,
+                                int nq, int nq_pad, double* tables, hipStream_t stream);
+
 }  // namespace csx

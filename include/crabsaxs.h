@@ -386,6 +386,29 @@ csx_status csx_debye_pair_tables(csx_structure *s, const double *q,
                                  double *out, double *self_terms,
                                  size_t *n_types);
 
+/* The pair tables of csx_debye_pair_tables (atoms only, p = NULL) written
+ * to device memory on the caller's stream, with nothing allocated: no
+ * hipMalloc, hipFree, pinned staging or stream creation.  No reference
+ * counterpart; for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops).  q: nq values in host memory (handed to the
+ * device as kernel arguments); tables: device memory for T(T+1)/2 * nq
+ * doubles, the bins of csx_debye_pair_tables (query T with it, out = NULL);
+ * stream: a hipStream_t (NULL: the null stream).  The call is asynchronous:
+ * the tables are complete once the stream has run its kernels.  Its
+ * temporaries (task counter, q, per-work-item partials, raw tables) live in
+ * one 64 MiB device array per GPU, held until the process ends, which limits
+ * it to  256 + 4 nq' + 8 (items + bins) nq'  bytes (nq' the q count padded to
+ * the q tile, items the pair kernel's work items: about n^2 / (2 * 256 *
+ * j_chunk)); larger calls fail with CSX_INVALID_INPUT and a message.  Every
+ * call on a GPU shares that array, so calls that overlap on one GPU (two
+ * streams or two threads at once) race: serialise them.  The structure's
+ * work items for the context's j chunk must exist: csx_structure_create
+ * makes them for the j chunk of that moment.  Same sums, bit for bit, as
+ * csx_debye_pair_tables with the same tuning. */
+csx_status csx_debye_pair_tables_noalloc(csx_structure *s, const double *q,
+                                         size_t nq, double *tables,
+                                         void *stream);
+
 /* Batched conformer profiles (no reference counterpart; replaces the per-
  * conformer compute_curve calls of EnsembleFitter::fit_core,
  * ensemble.rs:244-261): the work items of all m structures (any sizes, one
