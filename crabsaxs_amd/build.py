@@ -101,11 +101,14 @@ def build_host_asan(out_dir: Path) -> Path:
     """The library and the C++ suite with the host code under AddressSanitizer
     (device code is not instrumented), into out_dir: out_dir/lib and
     out_dir/cpp/reference_suite, which finds the fixture files through
-    out_dir/golden.  Run it with ASAN_OPTIONS=detect_leaks=0 (the HIP runtime
-    keeps allocations until exit).  The sanitizer runtime's device-allocator
-    support can stop on an internal check (dev_runtime_unloaded_) while HIP's
-    threads are torn down at exit, after the suite has finished; that check
-    is not a memory-error report."""
+    out_dir/golden.  It is meant for the host-only paths, in a container
+    without a GPU: `reference_suite --cpu` (typing, form factors, fitting,
+    ingest, RNG, the no-device context path), which runs clean with
+    ASAN_OPTIONS=detect_leaks=1.  With a GPU, the sanitizer runtime also
+    tracks the HIP runtime's device allocations, and at process exit it can
+    recycle them after the HIP runtime has unloaded (its internal check
+    dev_runtime_unloaded_); the C++ API's default context is therefore
+    never destroyed at exit, and the sanitized build is not run on a GPU."""
     out_dir = Path(out_dir).resolve()
     (out_dir / "lib").mkdir(parents=True, exist_ok=True)
     (out_dir / "cpp").mkdir(parents=True, exist_ok=True)

@@ -47,18 +47,25 @@ void chacha_block(const uint32_t k[8], uint64_t counter, uint64_t stream, int ro
   for (int i = 0; i < 16; ++i) out[i] = x[i] + s[i];
 }
 
+// PCG32 (O'Neill 2014): the 64-bit LCG step state = state * 6364136223846793005
+// + increment and the XSH-RR output, here of the state before the step (the
+// reference generator's convention) or after it (seed_from_u64's).
+inline uint32_t pcg32_next(uint64_t& state, uint64_t increment, bool output_new) {
+  const uint64_t old = state;
+  state = old * 6364136223846793005ull + increment;
+  const uint64_t x = output_new ? state : old;
+  const uint32_t xorshifted = static_cast<uint32_t>(((x >> 18) ^ x) >> 27);
+  const uint32_t rot = static_cast<uint32_t>(x >> 59);
+  return (xorshifted >> rot) | (xorshifted << ((32 - rot) & 31));
+}
+
 // ChaCha8Rng seeded like SeedableRng::seed_from_u64: the 32-byte key is eight
-// successive PCG32 (XSH-RR) outputs of the advanced state, little-endian.
+// successive PCG32 (XSH-RR) outputs of the advanced state, little-endian,
+// with the increment 11634580027462260723.
 class ChaCha8 {
  public:
   explicit ChaCha8(uint64_t state) {
-    const uint64_t mul = 6364136223846793005ull, inc = 11634580027462260723ull;
-    for (int i = 0; i < 8; ++i) {
-      state = state * mul + inc;
-      const uint32_t xorshifted = static_cast<uint32_t>(((state >> 18) ^ state) >> 27);
-      const uint32_t rot = static_cast<uint32_t>(state >> 59);
-      key_[i] = (xorshifted >> rot) | (xorshifted << ((32 - rot) & 31));
-    }
+    for (int i = 0; i < 8; ++i) key_[i] = pcg32_next(state, 11634580027462260723ull, true);
   }
   // next_u64: two consecutive keystream words, the first as the low half.
   uint64_t next_u64() {
@@ -103,6 +110,13 @@ csx_status csx_chacha_block(const uint32_t* key, uint64_t counter, uint64_t stre
   if (!key || !out || rounds <= 0 || rounds % 2 != 0)
     return fail(CSX_INVALID_INPUT, "key, output and an even positive round count are required");
   chacha_block(key, counter, stream, rounds, out);
+  return CSX_OK;
+}
+
+csx_status csx_pcg32(uint64_t state, uint64_t increment, size_t n, int output_new_state,
+                     uint32_t* out) {
+  if (n > 0 && !out) return fail(CSX_INVALID_INPUT, "an output buffer is required");
+  for (size_t k = 0; k < n; ++k) out[k] = pcg32_next(state, increment, output_new_state != 0);
   return CSX_OK;
 }
 

@@ -654,6 +654,15 @@ csx_status csx_bootstrap_indices(uint64_t seed, size_t n, size_t replicates,
 csx_status csx_chacha_block(const uint32_t *key, uint64_t counter,
                             uint64_t stream, int rounds, uint32_t *out);
 
+/* The PCG32 generator under seed_from_u64 (O'Neill 2014): from `state`, n
+ * steps state = state * 6364136223846793005 + increment, each giving the
+ * XSH-RR output of the state before the step (output_new_state = 0, the
+ * reference generator's convention) or after it (1, the seed expansion of
+ * SeedableRng::seed_from_u64, which fills the ChaCha8 key with 8 outputs
+ * for increment 11634580027462260723).  out: n words. */
+csx_status csx_pcg32(uint64_t state, uint64_t increment, size_t n,
+                     int output_new_state, uint32_t *out);
+
 /* csx_ensemble_fit_core on the GPU of `ctx`: the 25 candidate profile sets
  * and their projected-gradient weight loops run on the device (one
  * workgroup per candidate), with every order-dependent sum in the host

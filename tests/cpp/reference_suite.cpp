@@ -696,6 +696,30 @@ CSX_TEST(ensemble_fitter_recovers_a_mixture, true) {
   CHECK(r.reduced_chi2 < 1e-6 && r.bootstrap_reduced_chi2.size() == 3);
 }
 
+// Without a HIP device every compute call reports NoDevice (SaxsError) and
+// nothing else fails or leaks; host-only calls keep working.  With a device
+// the same calls succeed.  (No reference counterpart: the reference has no
+// device.)
+CSX_TEST(no_device_paths_report_cleanly, false) {
+  Structure s({atom(0, 0, 0, Element::C, "C1"), atom(2, 0, 0, Element::C, "C2")});
+  const std::vector<double> q = {0.0, 0.1, 0.2};
+  if (csx_device_count() == 0) {
+    CHECK(throws<NoDevice>([] { Context ctx(0); }));
+    csx_ctx* raw = reinterpret_cast<csx_ctx*>(0x1);
+    CHECK(csx_ctx_create(0, &raw) == CSX_NO_DEVICE && raw == nullptr);
+    CHECK(std::string(csx_last_error()).find("no HIP device") != std::string::npos);
+  } else {
+    Context ctx(0);
+    CHECK(debye_intensity(s, q, Precision::FP32Compensated, ctx).size() == q.size());
+  }
+  // host-only paths
+  CHECK(std::fabs(csx_water_form_factor(0.0) - 10.0) < 1e-3);
+  std::vector<double> basis(6 * q.size(), 1.0), out(q.size());
+  CHECK(csx_combine_solvent_basis(basis.data(), q.data(), q.size(), 1.0, 0.0, out.data()) == CSX_OK);
+  uint32_t words[4];
+  CHECK(csx_pcg32(1, 3, 4, 1, words) == CSX_OK);
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {

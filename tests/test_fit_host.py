@@ -258,3 +258,23 @@ def test_bootstrap_indices_stream():
     assert chi < 220, chi
     # n = 1 always draws 0
     assert np.all(csx.bootstrap_indices(7, 1, 5) == 0)
+
+
+def test_pcg32_matches_published_vectors():
+    """The PCG32 generator of the bootstrap's seed expansion against the
+    reference PCG32 demo (pcg32_srandom(42, 54): increment 2*54+1, state
+    advanced by the published seeding steps) and its first six outputs."""
+    import ctypes as C
+    from crabsaxs_amd import _lib
+    mask = (1 << 64) - 1
+    mul = 6364136223846793005
+    inc = (54 << 1) | 1
+    state = (((inc + 42) & mask) * mul + inc) & mask  # step, add 42, step
+    out = (C.c_uint32 * 6)()
+    assert _lib.load().csx_pcg32(state, inc, 6, 0, out) == 0
+    assert [int(v) for v in out] == [0xa15c02b7, 0x7b47f409, 0xba1d3330, 0x83d2f293,
+                                     0xbfa4784b, 0xcbed606e]
+    # the seed expansion's convention: outputs of the state after each step
+    new = (C.c_uint32 * 5)()
+    assert _lib.load().csx_pcg32((state - inc) * pow(mul, -1, 1 << 64) & mask, inc, 5, 1, new) == 0
+    assert [int(v) for v in new] == [int(v) for v in out[:5]]
