@@ -84,6 +84,8 @@ def parse():
     ap.add_argument("--multipole-profile", type=int, default=0, metavar="L",
                     help="only run the multipole workload at l_max L (1 + 3 launches), "
                          "for counter collection")
+    ap.add_argument("--mp-group", type=int, default=0,
+                    help="column tiles per multipole tile group (32, 64, 128; 0: automatic)")
     a = ap.parse_args()
     a.legs = set() if a.legs in ("", "none") else set(a.legs.split(","))
     if a.no_cpu:
@@ -108,7 +110,7 @@ def copy_calibration(nbytes: int) -> None:
     print(json.dumps({"copy_calibration_bytes": nbytes, "read": nbytes, "written": nbytes}))
 
 
-def multipole_profile(L: int) -> None:
+def multipole_profile(L: int, group: int = 0) -> None:
     """The multipole workload's solvent intensity (atoms and hydration
     sites) at one l_max, 1 warm-up and 3 timed launches and nothing else, so
     that counters of a profiler run belong to that order only."""
@@ -116,6 +118,7 @@ def multipole_profile(L: int) -> None:
     from crabsaxs_amd import workloads
     ctx = csx.default_context()
     ctx.set_cache_limit(0)  # every call runs the coefficient kernel
+    ctx.set_multipole_group(group)
     s, q, _ = workloads.prolate_100k()
     p = csx.SolventParams(c1=1.0, c2=1.0, hydration_model=csx.HydrationModel.ExplicitSites)
     s.handle(ctx)
@@ -127,8 +130,8 @@ def multipole_profile(L: int) -> None:
         csx.multipole_solvent_intensity(s, q, mp, p, ctx=ctx)
     ctx.set_timing(False)
     n, ms = ctx.kernel_time("multipole_coeff")
-    print(json.dumps({"multipole_profile_l_max": L, "points": len(s), "q": int(q.size),
-                      "multipole_coeff_ms": ms / n}))
+    print(json.dumps({"multipole_profile_l_max": L, "group_tiles": group, "points": len(s),
+                      "q": int(q.size), "multipole_coeff_ms": ms / n}))
 
 
 def traffic_per_launch(kernel: str):
@@ -338,35 +341,69 @@ def ensemble_leg(csx, ctx, m, log):
         e, _ = rel_err(got, want)
         worst = max(worst, e)
         checked.append(int(sizes[k]))
-    # weights: Dirichlet(1) mixture of 5 conformers, 1% Gaussian noise
+    def mixture(pick, w_true, seed):
+        """Weights recovered from a 5-conformer mixture with 1% Gaussian noise
+        by both weight solvers over all m conformers."""
+        rng = np.random.default_rng(seed)
+        # (c1, c2) = (1, 1) is one of the 5 x 5 candidates the ensemble fit tries
+        prof = np.stack([csx.combine_solvent_basis(bases[k], q, 1.0, 1.0) for k in pick])
+        mix = w_true @ prof
+        sigma = 0.01 * mix
+        y = mix + sigma * rng.normal(size=q.size)
+        exp = csx.ExperimentalCurve(q, y, sigma)
+        # the noise floor: chi2 of the true mixture with its scale and background fitted
+        out = {"conformers": [int(k) for k in pick], "atoms": [int(sizes[k]) for k in pick],
+               "true": np.round(w_true, 4).tolist(),
+               "true_weights_reduced_chi2": csx.solve_scale_background(mix, exp)[2]}
+        for solver in (csx.WeightSolver.ProjectedGradient, csx.WeightSolver.NNLS):
+            opts = csx.EnsembleOptions(fit=csx.FitOptions(method=csx.ScatteringMethod.Debye),
+                                       weight_solver=solver)
+            fitter = csx.EnsembleFitter(conformers, [str(k) for k in range(m)], exp, opts)
+            fitter._bases = bases  # the batch above is exactly what bases() computes
+            t1 = time.perf_counter()
+            r = fitter.fit(ctx)
+            fit_s = time.perf_counter() - t1
+            w_all = np.zeros(m)
+            w_all[[int(n) for n in r.conformers]] = r.weights
+            out[solver.name] = {"recovered": np.round(w_all[pick], 4).tolist(),
+                                "weight_elsewhere": float(w_all.sum() - w_all[pick].sum()),
+                                "max_abs_weight_error": float(np.abs(w_all[pick] - w_true).max()),
+                                "reduced_chi2": r.reduced_chi2, "c1": r.params.c1,
+                                "c2": r.params.c2, "fit_s": fit_s}
+            log(f"ensemble: {solver.name} weights {out[solver.name]}")
+        return out
+
+    # (a) 5 conformers drawn at random: globules of similar sizes and shapes,
+    # which 1% noise does not tell apart from the other 507
     rng = np.random.default_rng(2024)
     pick = np.sort(rng.choice(m, size=5, replace=False))
     w_true = rng.dirichlet(np.ones(5))
-    # (c1, c2) = (1, 1) is one of the 5 x 5 candidates the ensemble fit tries
-    prof = np.stack([csx.combine_solvent_basis(bases[k], q, 1.0, 1.0) for k in pick])
-    mix = w_true @ prof
-    sigma = 0.01 * mix
-    y = mix + sigma * rng.normal(size=q.size)
-    exp = csx.ExperimentalCurve(q, y, sigma)
-    # the noise floor: chi2 of the true mixture with its scale and background fitted
-    truth_chi2 = csx.solve_scale_background(mix, exp)[2]
-    weights = {"conformers": pick.tolist(), "true": w_true.round(4).tolist(),
-               "true_weights_reduced_chi2": truth_chi2}
-    for solver in (csx.WeightSolver.ProjectedGradient, csx.WeightSolver.NNLS):
-        opts = csx.EnsembleOptions(fit=csx.FitOptions(method=csx.ScatteringMethod.Debye),
-                                   weight_solver=solver)
-        fitter = csx.EnsembleFitter(conformers, [str(k) for k in range(m)], exp, opts)
-        fitter._bases = bases  # the batch above is exactly what bases() computes
-        t1 = time.perf_counter()
-        r = fitter.fit(ctx)
-        fit_s = time.perf_counter() - t1
-        w_all = np.zeros(m)
-        w_all[[int(n) for n in r.conformers]] = r.weights
-        weights[solver.name] = {"recovered": w_all[pick].round(4).tolist(),
-                                "weight_elsewhere": float(w_all.sum() - w_all[pick].sum()),
-                                "reduced_chi2": r.reduced_chi2, "c1": r.params.c1,
-                                "c2": r.params.c2, "fit_s": fit_s}
-        log(f"ensemble: {solver.name} weights {weights[solver.name]}")
+    weights = {"random_five": mixture(pick, w_true, 2025)}
+    # (b) 5 conformers of distinct shape and size: a small and a large globule
+    # and a 2-, 3- and 4-domain chain of different lengths (the workload
+    # alternates globules at even and chains of 2 + k % 3 domains at odd k)
+    targets = [(0, None, 3000), (0, None, 28000), (1, 2, 8000), (1, 3, 16000), (1, 4, 24000)]
+    distinct = []
+    for parity, domains, n_atoms in targets:
+        cand = [k for k in range(m) if k % 2 == parity
+                and (domains is None or 2 + k % 3 == domains)]
+        distinct.append(min(cand, key=lambda k: abs(int(sizes[k]) - n_atoms)))
+    rng = np.random.default_rng(2026)
+    weights["distinct_five"] = mixture(np.array(sorted(distinct)), rng.dirichlet(np.ones(5)), 2027)
+    for case in ("random_five", "distinct_five"):
+        w = weights[case]
+        nn, pg = w["NNLS"], w["ProjectedGradient"]
+        if nn["max_abs_weight_error"] < 0.05:
+            w["reading"] = "recovered: every true weight within 0.05 (NNLS)"
+        elif nn["reduced_chi2"] <= w["true_weights_reduced_chi2"] * 1.02:
+            w["reading"] = ("not identified by the data: NNLS reaches a chi2 at or below the "
+                            "true mixture's with weight on other conformers (a degeneracy, not "
+                            "a solver miss)")
+        else:
+            w["reading"] = "NNLS stops above the true mixture's chi2 (a solver miss)"
+        if pg["reduced_chi2"] > 2 * w["true_weights_reduced_chi2"]:
+            w["reading"] += ("; the reference's projected gradient (250 fixed steps, "
+                             "ensemble.rs:317-381) does not converge at this size")
     return {
         "workload": f"{m} conformers, atoms U[2000, 30000] (mean {sizes.mean():.0f}), "
                     "globules and 2-4 domain chains, q=linspace(0.005,0.5,200), one launch",
@@ -572,6 +609,7 @@ def adjoint_leg(csx, ctx, log):
     log(f"adjoint: {kms:.1f} ms, directional {ad:.6e} vs FD {fd:.6e}")
     e = csx.ExperimentalCurve(q, y, sigma)
     fits = {}
+    bounds = csx.FitOptions()
     for opt in (csx.Optimizer.NelderMead, csx.Optimizer.Gradient):
         o = csx.FitOptions(method=csx.ScatteringMethod.Debye, optimizer=opt, q_sampling_stride=1,
                            max_iterations=200)
@@ -580,14 +618,42 @@ def adjoint_leg(csx, ctx, log):
         r = csx.fit_to_experimental_with_options(s, e, csx.FitParams(), o, ctx)
         fits[opt.name] = {"s": time.perf_counter() - t0, "reduced_chi2": r.chi2,
                           "iterations": r.iterations,
-                          "params": [r.params.scale, r.params.background, r.params.c1, r.params.c2]}
+                          "params": [r.params.scale, r.params.background, r.params.c1, r.params.c2],
+                          "at_bounds": {"c1": r.params.c1 in bounds.c1_bounds,
+                                        "c2": r.params.c2 in bounds.c2_bounds}}
+    # Why the fitted parameters are not the truth: reduced chi^2 at the true
+    # parameters, and the brute-force minimum over a (c1, c2) grid (scale and
+    # background solved by least squares at each point) from the FP64 basis.
+    basis = csx.debye_solvent_basis(s, q, csx.SolventParams(), precision=F64, ctx=ctx)
+    chi_truth = csx.chi2_grad(basis, e, csx.FitParams(scale=2.3, background=1e-3, c1=1.02,
+                                                      c2=2.0))[0]
+    best = (np.inf, None, None)
+    for c1 in np.linspace(bounds.c1_bounds[0], bounds.c1_bounds[1], 41):
+        for c2 in np.linspace(bounds.c2_bounds[0], bounds.c2_bounds[1], 121):
+            chi = csx.solve_scale_background(csx.combine_solvent_basis(basis, q, c1, c2), e)[2]
+            if chi < best[0]:
+                best = (chi, float(c1), float(c2))
+    i0 = csx.solvent_corrected_intensity(s, q, truth, ctx=ctx)
+    fit_explained = {
+        "reduced_chi2_at_truth": chi_truth,
+        "grid_minimum": {"reduced_chi2": best[0], "c1": best[1], "c2": best[2],
+                         "grid": "41 x 121 over the default bounds, LSQ scale and background"},
+        "sigma_floor_over_I_at_qmax": float(1e-4 * (2.3 * i0[0] + 1e-3) / (2.3 * i0[-1])),
+        "reading": ("the fits reach the grid's chi2 minimum, below chi2 at the truth: the "
+                    "sigma floor 1e-4 I(0) dwarfs I(q) at high q, so the noisy target does not "
+                    "identify c1, and the fitted c1 sits on the bound where the data put the "
+                    "minimum; a data degeneracy, not a solver miss"
+                    if min(f["reduced_chi2"] for f in fits.values()) <= best[0] * 1.001
+                    and best[0] <= chi_truth else "see the figures"),
+    }
+    log(f"adjoint: chi2 at truth {chi_truth:.4f}, grid minimum {best}")
     return {"workload": "20,000-atom globule, q=linspace(0.01,1,300), target scale 2.3, bg 1e-3, "
                         "c1 1.02, c2 2.0, noise 1% I + 1e-4 I(0)",
             "vjp_kernel_ms": kms, "vjp_wall_ms": dt * 1e3,
             "vjp_pair_q_per_s": pairq / (kms / 1e3),
             "directional_derivative": {"adjoint": ad, "central_difference_fp64": fd,
                                        "h_angstrom": h, "rel_diff": abs(ad - fd) / abs(fd)},
-            "fits": fits}
+            "fits": fits, "fit_explained": fit_explained}
 
 
 def main():
@@ -596,7 +662,7 @@ def main():
         copy_calibration(args.copy_calibration)
         return
     if args.multipole_profile:
-        multipole_profile(args.multipole_profile)
+        multipole_profile(args.multipole_profile, args.mp_group)
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not "

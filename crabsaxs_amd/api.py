@@ -403,6 +403,10 @@ class Context:
         """q values per sweep of the coordinate adjoint; 0 picks per call."""
         _check(_lib.load().csx_ctx_set_vjp_tile(self.ptr, q_tile))
 
+    def set_multipole_group(self, tiles: int = 0) -> None:
+        """Column tiles per multipole tile group (32, 64, 128); 0 picks per call."""
+        _check(_lib.load().csx_ctx_set_multipole_group(self.ptr, tiles))
+
     def set_vjp_split(self, split: int = 0) -> None:
         """Workgroups per sweep of the coordinate adjoint; 0 picks per call."""
         _check(_lib.load().csx_ctx_set_vjp_split(self.ptr, split))

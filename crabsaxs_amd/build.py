@@ -28,6 +28,8 @@ DEVICE = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang",
           "-packed-fp32-ops", "-munsafe-fp-atomics"]
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HOST_ONLY = ["-ffp-contract=off", "-D__HIP_PLATFORM_AMD__", f"-I{ROCM}/include"]
+# roctx ranges around the C-ABI calls
+LINK = [f"-L{ROCM}/lib", "-lrocprofiler-sdk-roctx", f"-Wl,-rpath,{ROCM}/lib"]
 
 SOURCES = [
     ("physics.cpp", "host"),
@@ -76,7 +78,7 @@ def build(verbose: bool = False) -> Path:
             _run(cmd)
             objs.append(str(obj))
         tmp_lib = Path(tmp) / LIB.name
-        _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", str(tmp_lib)])
+        _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, *LINK, "-o", str(tmp_lib)])
         shutil.copyfile(tmp_lib, str(LIB) + ".tmp")
         os.replace(str(LIB) + ".tmp", LIB)
     return LIB
@@ -129,7 +131,7 @@ def build_host_asan(out_dir: Path) -> Path:
                 cmd = [HIPCC, "-x", "c++", *COMMON, *HOST_ONLY, *asan, "-c", str(src), "-o", str(obj)]
             _run(cmd)
             objs.append(str(obj))
-        _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *asan, *objs, "-o", str(lib)])
+        _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *asan, *objs, *LINK, "-o", str(lib)])
     suite = out_dir / "cpp" / "reference_suite"
     _run([HIPCC, "-x", "c++", "-std=c++17", "-O1", "-g", *asan, "-D__HIP_PLATFORM_AMD__",
           f"-I{PKG.parent / 'include'}", str(PKG.parent / "tests" / "cpp" / "reference_suite.cpp"),

@@ -224,11 +224,13 @@ extern "C" {
 
 csx_status csx_debye_grad_x(csx_structure* s, const double* q, size_t nq, const double* weights,
                             const csx_solvent_params* p, int implicit_h, double* grad) {
+  CSX_API_RANGE();
   return adjoint(s, q, nq, weights, p, implicit_h != 0, false, grad);
 }
 
 csx_status csx_debye_jacobian(csx_structure* s, const double* q, size_t nq,
                               const csx_solvent_params* p, int implicit_h, double* jac) {
+  CSX_API_RANGE();
   return adjoint(s, q, nq, nullptr, p, implicit_h != 0, true, jac);
 }
 

@@ -1,5 +1,7 @@
 // C-ABI entry points: contexts, structures, the Debye path and compute_curve.
 #include <algorithm>
+#include <atomic>
+#include <cstdlib>
 #include <functional>
 #include <cmath>
 #include <cstring>
@@ -23,6 +25,58 @@ csx_status hip_fail(hipError_t e, const char* what) {
                         : e == hipErrorNoDevice || e == hipErrorInvalidDevice   ? CSX_NO_DEVICE
                                                                                  : CSX_DEVICE_ERROR;
   return fail(st, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// ------------------------------------------------------ device memory --
+
+static std::atomic<size_t> g_device_bytes{0};
+
+static size_t device_limit() {
+  static const size_t limit = [] {
+    const char* v = std::getenv("CSX_DEVICE_MEMORY_LIMIT");
+    if (!v || !*v) return SIZE_MAX;
+    char* end = nullptr;
+    const unsigned long long n = std::strtoull(v, &end, 10);
+    return end && *end == '\0' ? static_cast<size_t>(n) : SIZE_MAX;
+  }();
+  return limit;
+}
+
+static bool reserve(size_t bytes) {
+  const size_t limit = device_limit();
+  size_t held = g_device_bytes.load();
+  do {
+    if (bytes > limit || held > limit - bytes) return false;
+  } while (!g_device_bytes.compare_exchange_weak(held, held + bytes));
+  return true;
+}
+
+hipError_t device_alloc(void** p, size_t bytes) {
+  *p = nullptr;
+  if (!reserve(bytes)) return hipErrorOutOfMemory;
+  const hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) g_device_bytes -= bytes;
+  return e;
+}
+
+hipError_t device_alloc_async(void** p, size_t bytes, hipStream_t s) {
+  *p = nullptr;
+  if (!reserve(bytes)) return hipErrorOutOfMemory;
+  const hipError_t e = hipMallocAsync(p, bytes, s);
+  if (e != hipSuccess) g_device_bytes -= bytes;
+  return e;
+}
+
+void device_free(void* p, size_t bytes) {
+  if (!p) return;
+  (void)hipFree(p);
+  g_device_bytes -= bytes;
+}
+
+void device_free_async(void* p, size_t bytes, hipStream_t s) {
+  if (!p) return;
+  (void)hipFreeAsync(p, s);
+  g_device_bytes -= bytes;
 }
 
 // ------------------------------------------------------------ point sets --
@@ -744,6 +798,7 @@ int csx_device_count(void) {
 }
 
 csx_status csx_ctx_create(int device, csx_ctx** out) {
+  CSX_API_RANGE();
   if (!out) return fail(CSX_INVALID_INPUT, "output pointer is null");
   *out = nullptr;
   int n = 0;
@@ -776,7 +831,7 @@ void csx_ctx_destroy(csx_ctx* ctx) {
   for (hipStream_t s : ctx->free_streams) (void)hipStreamDestroy(s);
   for (auto& entry : ctx->scratch)
     for (auto& slot : entry.second)
-      if (slot.first) (void)hipFree(slot.first);
+      if (slot.first) device_free(slot.first, slot.second);
   for (auto& entry : ctx->pinned)
     for (auto& slot : entry.second)
       if (slot.first) (void)hipHostFree(slot.first);
@@ -844,6 +899,15 @@ csx_status csx_ctx_set_vjp_tile(csx_ctx* ctx, int q_tile) {
   return CSX_OK;
 }
 
+csx_status csx_ctx_set_multipole_group(csx_ctx* ctx, int tiles) {
+  if (!ctx) return fail(CSX_INVALID_INPUT, "context is null");
+  if (tiles != 0 && tiles != 32 && tiles != 64 && tiles != 128)
+    return fail(CSX_INVALID_INPUT, "multipole tile group must be 0 (automatic), 32, 64 or 128");
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  ctx->tuning.mp_group_tiles = tiles;
+  return CSX_OK;
+}
+
 csx_status csx_ctx_set_vjp_split(csx_ctx* ctx, int split) {
   if (!ctx) return fail(CSX_INVALID_INPUT, "context is null");
   if (split != 0 && split != 1 && split != 2 && split != 4 && split != 8)
@@ -903,6 +967,7 @@ double csx_excluded_volume_scale(double q, double radius_scale) {
 csx_status csx_structure_create(csx_ctx* ctx, size_t n, const double* xyz, const uint8_t* element,
                                 const int8_t* group, const double* occupancy,
                                 csx_structure** out) {
+  CSX_API_RANGE();
   if (!ctx || !out) return fail(CSX_INVALID_INPUT, "context and output pointer are required");
   *out = nullptr;
   if (n > 0 && (!xyz || !element || !occupancy))
@@ -971,6 +1036,7 @@ size_t csx_structure_atom_count(const csx_structure* s) { return s ? s->n : 0; }
 
 csx_status csx_structure_set_hydration_sites(csx_structure* s, const double* xyz, size_t count,
                                              double voxel_volume) {
+  CSX_API_RANGE();
   if (!s) return fail(CSX_INVALID_INPUT, "structure handle is null");
   if (count > 0 && !xyz) return fail(CSX_INVALID_INPUT, "site coordinates are required");
   if (!std::isfinite(voxel_volume) || voxel_volume < 0.0)
@@ -999,12 +1065,14 @@ static csx_status copy_grid(const Grid& g, double* sites, size_t capacity, size_
 
 csx_status csx_generate_shell_sites(const double* xyz, size_t n, double spacing, double inner,
                                     double outer, double* sites, size_t capacity, size_t* count) {
+  CSX_API_RANGE();
   if (n > 0 && !xyz) return fail(CSX_INVALID_INPUT, "xyz is required");
   return copy_grid(shell_sites(xyz, n, spacing, inner, outer), sites, capacity, count, nullptr);
 }
 
 csx_status csx_generate_globule(size_t n, double radius, double elongation, uint64_t seed,
                                 double* xyz, uint8_t* element) {
+  CSX_API_RANGE();
   if (n > 0 && (!xyz || !element)) return fail(CSX_INVALID_INPUT, "output arrays are required");
   if (!(radius > 0.0) || !(elongation > 0.0) || !std::isfinite(radius) || !std::isfinite(elongation))
     return fail(CSX_INVALID_INPUT, "radius and elongation must be finite and positive");
@@ -1014,6 +1082,7 @@ csx_status csx_generate_globule(size_t n, double radius, double elongation, uint
 
 csx_status csx_generate_domain_chain(size_t n, int domains, uint64_t seed, double* xyz,
                                      uint8_t* element) {
+  CSX_API_RANGE();
   if (n > 0 && (!xyz || !element)) return fail(CSX_INVALID_INPUT, "output arrays are required");
   if (domains < 1 || domains > 16) return fail(CSX_INVALID_INPUT, "domains must be in [1, 16]");
   generate_domain_chain(n, domains, seed, xyz, element);
@@ -1022,6 +1091,7 @@ csx_status csx_generate_domain_chain(size_t n, int domains, uint64_t seed, doubl
 
 csx_status csx_debye_intensity(csx_structure* s, const double* q, size_t nq, int implicit_h,
                                int precision, double* out) {
+  CSX_API_RANGE();
   CSX_TRY(check_structure(s, q, nq, out));
   if (!precision_ok(precision)) return fail(CSX_INVALID_INPUT, "unknown precision");
   if (nq == 0) return CSX_OK;
@@ -1047,6 +1117,7 @@ csx_status csx_debye_intensity(csx_structure* s, const double* q, size_t nq, int
 csx_status csx_debye_solvent_basis(csx_structure* s, const double* q, size_t nq,
                                    const csx_solvent_params* p, int implicit_h, int precision,
                                    double* out) {
+  CSX_API_RANGE();
   CSX_TRY(check_structure(s, q, nq, out));
   if (!p) return fail(CSX_INVALID_INPUT, "solvent parameters are required");
   if (!precision_ok(precision)) return fail(CSX_INVALID_INPUT, "unknown precision");
@@ -1057,6 +1128,7 @@ csx_status csx_debye_solvent_basis(csx_structure* s, const double* q, size_t nq,
 csx_status csx_solvent_corrected_intensity(csx_structure* s, const double* q, size_t nq,
                                            const csx_solvent_params* p, int implicit_h,
                                            int precision, double* out) {
+  CSX_API_RANGE();
   CSX_TRY(check_structure(s, q, nq, out));
   if (!p) return fail(CSX_INVALID_INPUT, "solvent parameters are required");
   if (!precision_ok(precision)) return fail(CSX_INVALID_INPUT, "unknown precision");
@@ -1070,6 +1142,7 @@ csx_status csx_solvent_corrected_intensity(csx_structure* s, const double* q, si
 csx_status csx_debye_pair_tables(csx_structure* s, const double* q, size_t nq,
                                  const csx_solvent_params* p, double* out, double* self_terms,
                                  size_t* n_types) {
+  CSX_API_RANGE();
   if (!s) return fail(CSX_INVALID_INPUT, "structure handle is null");
   if (nq > 0 && !q) return fail(CSX_INVALID_INPUT, "q is required");
   std::shared_ptr<PointSet> ps;
@@ -1226,12 +1299,14 @@ static csx_status batch_debye(csx_structure* const* structures, size_t m, const 
 
 csx_status csx_batch_debye_intensity(csx_structure* const* structures, size_t m, const double* q,
                                      size_t nq, int implicit_h, double* out) {
+  CSX_API_RANGE();
   return batch_debye(structures, m, q, nq, nullptr, implicit_h != 0, out);
 }
 
 csx_status csx_batch_debye_solvent_basis(csx_structure* const* structures, size_t m,
                                          const double* q, size_t nq, const csx_solvent_params* p,
                                          int implicit_h, double* out) {
+  CSX_API_RANGE();
   if (!p) return fail(CSX_INVALID_INPUT, "solvent parameters are required");
   return batch_debye(structures, m, q, nq, p, implicit_h != 0, out);
 }
@@ -1266,6 +1341,7 @@ csx_status csx_combine_solvent_basis(const double* basis, const double* q, size_
 
 csx_status csx_compute_curve(csx_structure* s, const double* q, size_t nq,
                              const csx_compute_options* o, double* out, int* method_used) {
+  CSX_API_RANGE();
   CSX_TRY(check_structure(s, q, nq, out));
   if (!o) return fail(CSX_INVALID_INPUT, "options are required");
   for (size_t k = 0; k < nq; ++k)

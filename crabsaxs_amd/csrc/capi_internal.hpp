@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "../../include/crabsaxs.h"
+#include "api_range.hpp"
 #include "kernels.hpp"
 #include "physics.hpp"
 
@@ -40,6 +41,7 @@ struct Tuning {
   size_t workspace_limit = size_t(2) << 30;  // bytes of adjoint partials per launch
   int vjp_q_tile = 0;  // q values per sweep of the coordinate adjoint (0: automatic)
   int vjp_split = 0;   // workgroups per sweep of the coordinate adjoint (0: automatic)
+  int mp_group_tiles = 0;  // column tiles per multipole tile group (0: automatic)
   size_t cache_bytes = size_t(256) << 20;  // cached results kept per point set (0: off)
 };
 
@@ -64,6 +66,17 @@ struct csx_ctx {
 
 namespace csx {
 
+// Every device allocation of the library goes through these, which count
+// the bytes it holds.  With CSX_DEVICE_MEMORY_LIMIT=<bytes> in the
+// environment (read once), an allocation that would take that count above
+// the limit fails with hipErrorOutOfMemory (CSX_OUT_OF_MEMORY to the
+// caller) before reaching HIP: out-of-memory paths can then be tested on a
+// device with plenty of memory.
+hipError_t device_alloc(void** p, size_t bytes);
+hipError_t device_alloc_async(void** p, size_t bytes, hipStream_t s);
+void device_free(void* p, size_t bytes);
+void device_free_async(void* p, size_t bytes, hipStream_t s);
+
 // Device buffer freed on its stream (stream-ordered allocator).
 class DevBuf {
  public:
@@ -74,10 +87,13 @@ class DevBuf {
   hipError_t alloc(size_t bytes, hipStream_t s) {
     reset();
     stream_ = s;
-    return hipMallocAsync(&p_, bytes == 0 ? 16 : bytes, s);
+    bytes_ = bytes == 0 ? 16 : bytes;
+    const hipError_t e = device_alloc_async(&p_, bytes_, s);
+    if (e != hipSuccess) p_ = nullptr;
+    return e;
   }
   void reset() {
-    if (p_) (void)hipFreeAsync(p_, stream_);
+    if (p_) device_free_async(p_, bytes_, stream_);
     p_ = nullptr;
   }
   template <class T>
@@ -85,6 +101,7 @@ class DevBuf {
 
  private:
   void* p_ = nullptr;
+  size_t bytes_ = 0;
   hipStream_t stream_ = nullptr;
 };
 
@@ -95,12 +112,15 @@ class DevArray {
   DevArray(const DevArray&) = delete;
   DevArray& operator=(const DevArray&) = delete;
   ~DevArray() {
-    if (p_) (void)hipFree(p_);
+    if (p_) device_free(p_, bytes_);
   }
   hipError_t alloc(size_t bytes) {
-    if (p_) (void)hipFree(p_);
+    if (p_) device_free(p_, bytes_);
     p_ = nullptr;
-    return hipMalloc(&p_, bytes == 0 ? 16 : bytes);
+    bytes_ = bytes == 0 ? 16 : bytes;
+    const hipError_t e = device_alloc(&p_, bytes_);
+    if (e != hipSuccess) p_ = nullptr;
+    return e;
   }
   hipError_t upload(const void* host, size_t bytes) {
     hipError_t e = alloc(bytes);
@@ -112,6 +132,7 @@ class DevArray {
 
  private:
   void* p_ = nullptr;
+  size_t bytes_ = 0;
 };
 
 // Multipole geometry of one point set at one l_max: the points in segments
@@ -119,6 +140,7 @@ class DevArray {
 // slabs, and the FP32 real spherical harmonics in MFMA B-operand tiles.
 struct MultipoleGeometry {
   int L = -1, ntiles = 0, npad = 0, npts = 0, nchunk = 0;
+  int aexp = 14;  // exponent of the j_l occ operand scale of the coefficient kernel
   double rmax_atoms = 0.0, rmax_sites = 0.0;
   // Per padded point: radius, occupancy (0 for padding); chunks: (first
   // slab, end slab, type) of each one-type chunk of the coefficient kernel.
@@ -277,12 +299,11 @@ class Call {
     if (slot.second < bytes) {
       if (slot.first) {
         // hipFree waits for the device, so no queued work still uses it
-        const hipError_t e = hipFree(slot.first);
+        device_free(slot.first, slot.second);
         slot = {nullptr, 0};
-        if (e != hipSuccess) return e;
       }
       const size_t rounded = (bytes + (size_t(1) << 20) - 1) >> 20 << 20;
-      const hipError_t e = hipMalloc(&slot.first, rounded);
+      const hipError_t e = device_alloc(&slot.first, rounded);
       if (e != hipSuccess) {
         slot.first = nullptr;
         return e;

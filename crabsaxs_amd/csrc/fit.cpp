@@ -761,6 +761,7 @@ csx_status csx_cubic_hermite_interpolate(const double* x, const double* y, size_
 csx_status csx_chi2_grad(const double* basis, const double* q, const double* intensity,
                          const double* sigma, size_t n, const csx_fit_params* p, double* chi2,
                          double* grad4) {
+  CSX_API_RANGE();
   if (!basis || !q || !intensity || !sigma || !p || n == 0)
     return fail(CSX_INVALID_INPUT, "buffers are required");
   std::vector<double> I(n), d1(n), d2(n);
@@ -788,6 +789,7 @@ csx_status csx_chi2_grad(const double* basis, const double* q, const double* int
 
 csx_status csx_fit_basis(csx_structure* s, const double* q, size_t n, const csx_fit_options* o,
                          double* basis_out) {
+  CSX_API_RANGE();
   if (!s || !o || !basis_out || !q || n == 0) return fail(CSX_INVALID_INPUT, "arguments are required");
   if (!fit_options_valid(o))
     return fail(CSX_INVALID_INPUT, "fit bounds, iteration limit, and q sampling stride are invalid");
@@ -801,6 +803,7 @@ csx_status csx_fit_to_experimental(csx_structure* s, const double* q, const doub
                                    const double* sigma, size_t n, const csx_fit_params* initial,
                                    const csx_fit_options* o, csx_fit_result* result,
                                    double* fitted) {
+  CSX_API_RANGE();
   if (!s || !o || !result || !initial) return fail(CSX_INVALID_INPUT, "arguments are required");
   if (!curve_valid(q, intensity, sigma, n) || s->n == 0)
     return fail(CSX_INVALID_INPUT,
@@ -816,6 +819,7 @@ csx_status csx_fit_to_experimental(csx_structure* s, const double* q, const doub
 csx_status csx_fit_from_basis(const double* basis, const double* q, const double* intensity,
                               const double* sigma, size_t n, const csx_fit_params* initial,
                               const csx_fit_options* o, csx_fit_result* result, double* fitted) {
+  CSX_API_RANGE();
   if (!basis || !o || !result || !initial) return fail(CSX_INVALID_INPUT, "arguments are required");
   if (!curve_valid(q, intensity, sigma, n))
     return fail(CSX_INVALID_INPUT, "fit requires finite q, intensity and positive sigma arrays");
@@ -836,6 +840,7 @@ csx_status csx_optimize_weights(const double* curves, size_t m, const double* in
                                 const double* sigma, size_t n, const csx_fit_options* o,
                                 int uniform, double regularization, double* weights,
                                 double* out3, double* fitted) {
+  CSX_API_RANGE();
   if (!curves || !intensity || !sigma || !o || !weights || !out3 || m == 0 || n == 0)
     return fail(CSX_INVALID_INPUT, "buffers are required");
   std::vector<const double*> cv(m);
@@ -852,6 +857,7 @@ csx_status csx_optimize_weights(const double* curves, size_t m, const double* in
 csx_status csx_nnls_weights(const double* curves, size_t m, const double* intensity,
                             const double* sigma, size_t n, const csx_fit_options* o,
                             double* weights, double* out3, double* fitted) {
+  CSX_API_RANGE();
   if (!curves || !intensity || !sigma || !o || !weights || !out3 || m == 0 || n == 0)
     return fail(CSX_INVALID_INPUT, "buffers are required");
   std::vector<const double*> cv(m);
@@ -874,6 +880,7 @@ csx_status csx_ensemble_fit_core(const double* bases, size_t m, const double* q,
                                  const csx_fit_options* o, int uniform, double regularization,
                                  int weight_solver, double* weights, csx_fit_params* params,
                                  double* chi2, double* fitted) {
+  CSX_API_RANGE();
   if (!bases || !q || !intensity || !sigma || !o || !weights || !params || !chi2 || m == 0 || n == 0)
     return fail(CSX_INVALID_INPUT, "buffers are required");
   if (weight_solver != CSX_WEIGHTS_PROJECTED_GRADIENT && weight_solver != CSX_WEIGHTS_NNLS)
@@ -917,6 +924,7 @@ csx_status csx_ensemble_fit_core_device(csx_ctx* ctx, const double* bases, size_
                                         int uniform, double regularization, int weight_solver,
                                         double* weights, csx_fit_params* params, double* chi2,
                                         double* fitted) {
+  CSX_API_RANGE();
   if (!ctx) return fail(CSX_INVALID_INPUT, "context is null");
   if (weight_solver == CSX_WEIGHTS_NNLS || (weight_solver == CSX_WEIGHTS_PROJECTED_GRADIENT &&
                                             (!bases || !q || !intensity || !sigma || !o ||

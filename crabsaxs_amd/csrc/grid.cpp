@@ -222,6 +222,7 @@ extern "C" {
 csx_status csx_build_adaptive_hydration_grid(csx_structure* s, double spacing, double clearance,
                                              double padding, double* sites, size_t capacity,
                                              size_t* count, double* voxel_volume) {
+  CSX_API_RANGE();
   return grid_call(s, kAdaptiveGrid, spacing, clearance, padding, sites, capacity, count,
                    voxel_volume);
 }
@@ -229,6 +230,7 @@ csx_status csx_build_adaptive_hydration_grid(csx_structure* s, double spacing, d
 csx_status csx_build_hydration_grid(csx_structure* s, double spacing, double probe_radius,
                                     double shell_thickness, double* sites, size_t capacity,
                                     size_t* count, double* voxel_volume) {
+  CSX_API_RANGE();
   return grid_call(s, kExteriorGrid, spacing, probe_radius, shell_thickness, sites, capacity,
                    count, voxel_volume);
 }

@@ -27,6 +27,7 @@
 
 #include "../../include/crabsaxs.h"
 #include "../../include/crabsaxs_tables.h"
+#include "api_range.hpp"
 
 namespace csx {
 csx_status fail(csx_status st, const std::string& msg);
@@ -584,12 +585,14 @@ int csx_residue_kind(const char* residue_name) {
 
 csx_status csx_parse_structure(const char* text, size_t len, int format,
                                const csx_parse_options* options, csx_atoms** out) {
+  CSX_API_RANGE();
   if (!text && len) return fail(CSX_INVALID_INPUT, "text is required");
   return parse_structure(std::string(text ? text : "", len), format, options, out);
 }
 
 csx_status csx_read_structure_file(const char* path, const csx_parse_options* options,
                                    csx_atoms** out) {
+  CSX_API_RANGE();
   if (!path) return fail(CSX_INVALID_INPUT, "path is required");
   std::string text;
   if (!read_file(path, text)) return fail(CSX_IO_ERROR, std::string("cannot read ") + path);
@@ -630,6 +633,7 @@ void csx_atoms_destroy(csx_atoms* a) { delete a; }
 
 csx_status csx_parse_dat(const char* text, size_t len, double* q, double* intensity, double* sigma,
                          size_t capacity, size_t* count, int* has_errors) {
+  CSX_API_RANGE();
   if (!text && len) return fail(CSX_INVALID_INPUT, "text is required");
   std::vector<double> qq, ii, ss;
   int has = 1;
@@ -639,6 +643,7 @@ csx_status csx_parse_dat(const char* text, size_t len, double* q, double* intens
 
 csx_status csx_read_dat_file(const char* path, double* q, double* intensity, double* sigma,
                              size_t capacity, size_t* count, int* has_errors) {
+  CSX_API_RANGE();
   if (!path) return fail(CSX_INVALID_INPUT, "path is required");
   std::string text;
   if (!read_file(path, text)) return fail(CSX_IO_ERROR, std::string("cannot read ") + path);

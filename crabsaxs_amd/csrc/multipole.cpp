@@ -11,25 +11,24 @@
 namespace csx {
 
 int multipole_chunk();
-int multipole_tiles_per_group(int ntiles);
+int multipole_slab();
+int multipole_tiles_per_group(int ntiles, int tuned);
 size_t multipole_coeff_lds_bytes(int l_range);
 int multipole_max_types();
 hipError_t launch_multipole_geometry(const double* xyz, int npts, const double* centre, int l_max,
                                      const double* norms, const int* tile_first, int ntiles,
-                                     int npad, float* radius, float* Y, hipStream_t stream);
+                                     int npad, float* radius, void* Y, hipStream_t stream);
 hipError_t launch_multipole_coeff(const float* radius, const float* occ, const int* chunks,
-                                  int nchunk, int npad, const float* Y, int ntiles,
-                                  const int* tile_l, int max_l_range, const int* type_chunk,
-                                  int ntypes, const float* qv, const int* qlmax, int nqp,
-                                  float* partial, double* coef, hipStream_t stream,
-                                  KernelTimer* timer);
+                                  int nchunk, int npad, const void* Y, int ntiles,
+                                  const int* tile_l, int tpg, int max_l_range, int aexp,
+                                  const int* type_chunk, int ntypes, const float* qv,
+                                  const int* qlmax, int nqp, float* partial, double* coef,
+                                  hipStream_t stream, KernelTimer* timer);
 hipError_t launch_multipole_contract(const double* coef, const double* amp, int ntypes, int nch,
                                      const int* tile_l, const int* qlmax, int nqp, int ncol,
                                      double* out, hipStream_t stream);
 
 namespace {
-
-constexpr int kSlabPoints = 16;  // points per harmonic-tile slab
 
 // Largest l range of a column-tile group (the LDS rows one workgroup needs).
 int max_group_l_range(const std::vector<int>& tile_l, int tiles_per_group) {
@@ -84,8 +83,8 @@ static csx_status get_geometry(Call& call, csx_structure* s, PointSet& ps, int L
   g->ntiles = static_cast<int>(tile_l.size());
   g->h_tile_l = tile_l;
   // Points (atoms, then hydration sites as type T) in segments by type, each
-  // segment ordered by distance from the centre and padded to whole slabs of
-  // 16 with zero-occupancy points at the centre.  Every chunk of the
+  // segment ordered by distance from the centre and padded to whole slabs
+  // with zero-occupancy copies of its last point.  Every chunk of the
   // coefficient kernel then holds one type, whose amplitudes factor out of
   // its sum, and the lanes generating j_l(q r) for one slab see nearly equal
   // radii (nearly equal recurrence trip counts).  The sums over points do not
@@ -132,28 +131,32 @@ static csx_status get_geometry(Call& call, csx_structure* s, PointSet& ps, int L
   std::vector<float> occ;
   int slab = 0;
   std::vector<int> type_chunk(T + 2, 0);
+  const int slab_points = multipole_slab();
+  float occ_max = 0.0f;
   for (int t = 0; t <= T; ++t) {
     const int count = seg[t + 1] - seg[t];
     type_chunk[t] = static_cast<int>(chunks.size() / 4);
     if (count == 0) continue;
-    const int slabs = (count + kSlabPoints - 1) / kSlabPoints;
-    for (int k = 0; k < slabs * kSlabPoints; ++k) {
-      if (k < count) {
-        const int idx = static_cast<int>(key[seg[t] + k] & 0xffffffffu);
-        const double* p = point(idx);
-        xyz.insert(xyz.end(), p, p + 3);
-        occ.push_back(idx < n ? static_cast<float>(s->occ[idx]) : 1.0f);
-      } else {
-        xyz.insert(xyz.end(), s->centre, s->centre + 3);
-        occ.push_back(0.0f);
-      }
+    const int slabs = (count + slab_points - 1) / slab_points;
+    for (int k = 0; k < slabs * slab_points; ++k) {
+      const int idx = static_cast<int>(key[seg[t] + std::min(k, count - 1)] & 0xffffffffu);
+      const double* p = point(idx);
+      xyz.insert(xyz.end(), p, p + 3);
+      const float o = k >= count ? 0.0f : idx < n ? static_cast<float>(s->occ[idx]) : 1.0f;
+      occ.push_back(o);
+      occ_max = std::max(occ_max, std::fabs(o));
     }
-    const int per = multipole_chunk() / kSlabPoints;
+    const int per = multipole_chunk() / slab_points;
     for (int c = 0; c < slabs; c += per)
       chunks.insert(chunks.end(), {slab + c, slab + std::min(slabs, c + per), t, 0});
     slab += slabs;
   }
-  g->npad = slab * kSlabPoints;
+  g->npad = slab * slab_points;
+  // j_l occ enters the FP16 contraction times 2^aexp: the largest occupancy
+  // maps below 2^14 (occupancy 1 -> 2^14, j_l <= 1).
+  g->aexp = 14;
+  if (occ_max > 1.0f) g->aexp -= static_cast<int>(std::ceil(std::log2(static_cast<double>(occ_max))));
+  if (!(occ_max < 1e30f)) return fail(CSX_INVALID_INPUT, "occupancy not finite");
   g->nchunk = static_cast<int>(chunks.size() / 4);
   type_chunk[T + 1] = g->nchunk;
   // Types of the point set: the atom types, then the site type when the set
@@ -191,14 +194,11 @@ static csx_status get_geometry(Call& call, csx_structure* s, PointSet& ps, int L
   CSX_HIP(g->tile_l.upload(tile_l.data(), tile_l.size() * sizeof(int)), "upload tiles");
   CSX_HIP(g->type_chunk.upload(type_chunk.data(), type_chunk.size() * sizeof(int)),
           "upload type chunks");
-  if (multipole_coeff_lds_bytes(max_group_l_range(tile_l, multipole_tiles_per_group(g->ntiles))) >
-      s->ctx->lds_per_workgroup)
-    return fail(CSX_INVALID_INPUT, "multipole l_max too large for the device's LDS");
   const size_t ybytes = static_cast<size_t>(g->ntiles) * g->npad * 16 * sizeof(float);
   CSX_HIP(g->Y.alloc(ybytes), "hipMalloc harmonics");
   CSX_HIP(launch_multipole_geometry(d_xyz.as<double>(), g->npad, d_centre.as<double>(), L,
                                     d_norms.as<double>(), d_first.as<int>(), g->ntiles, g->npad,
-                                    g->radius.as<float>(), g->Y.as<float>(), st),
+                                    g->radius.as<float>(), g->Y.as<void>(), st),
           "multipole_geometry");
   d_xyz.reset();
   d_centre.reset();
@@ -256,10 +256,18 @@ static csx_status type_coefficients(Call& call, PointSet& ps, const MultipoleGeo
   CSX_HIP(hipMemcpyAsync(e->qlmax.as<void>(), in + nqp * sizeof(float), nqp * sizeof(int),
                          hipMemcpyHostToDevice, st),
           "hipMemcpyAsync");
-  const int l_range = max_group_l_range(g.h_tile_l, multipole_tiles_per_group(g.ntiles));
+  int tuned;
+  {
+    std::lock_guard<std::mutex> lock(call.ctx()->mu);
+    tuned = call.ctx()->tuning.mp_group_tiles;
+  }
+  const int tpg = multipole_tiles_per_group(g.ntiles, tuned);
+  const int l_range = max_group_l_range(g.h_tile_l, tpg);
+  if (multipole_coeff_lds_bytes(l_range) > call.ctx()->lds_per_workgroup)
+    return fail(CSX_INVALID_INPUT, "multipole l_max too large for the device's LDS");
   CSX_HIP(launch_multipole_coeff(g.radius.as<float>(), g.occ.as<float>(), g.chunks.as<int>(),
-                                 g.nchunk, g.npad, g.Y.as<float>(), g.ntiles, g.tile_l.as<int>(),
-                                 l_range, g.type_chunk.as<int>(), TT, d_q.as<float>(),
+                                 g.nchunk, g.npad, g.Y.as<void>(), g.ntiles, g.tile_l.as<int>(),
+                                 tpg, l_range, g.aexp, g.type_chunk.as<int>(), TT, d_q.as<float>(),
                                  e->qlmax.as<int>(), static_cast<int>(nqp), d_partial.as<float>(),
                                  e->coef.as<double>(), st, call.timer()),
           "multipole_coeff");
@@ -388,6 +396,7 @@ extern "C" {
 
 csx_status csx_multipole_intensity(csx_structure* s, const double* q, size_t nq, int l_max,
                                    int implicit_h, double* out) {
+  CSX_API_RANGE();
   CSX_TRY(check(s, q, nq, l_max, out));
   if (nq == 0) return CSX_OK;
   return multipole_run(s, q, nq, l_max, implicit_h != 0, nullptr, 0, out);
@@ -396,6 +405,7 @@ csx_status csx_multipole_intensity(csx_structure* s, const double* q, size_t nq,
 csx_status csx_multipole_solvent_intensity(csx_structure* s, const double* q, size_t nq,
                                            int l_max, int implicit_h,
                                            const csx_solvent_params* p, double* out) {
+  CSX_API_RANGE();
   CSX_TRY(check(s, q, nq, l_max, out));
   if (!p) return fail(CSX_INVALID_INPUT, "solvent parameters are required");
   if (nq == 0) return CSX_OK;
@@ -404,6 +414,7 @@ csx_status csx_multipole_solvent_intensity(csx_structure* s, const double* q, si
 
 csx_status csx_multipole_solvent_basis(csx_structure* s, const double* q, size_t nq, int l_max,
                                        int implicit_h, const csx_solvent_params* p, double* out) {
+  CSX_API_RANGE();
   CSX_TRY(check(s, q, nq, l_max, out));
   if (!p) return fail(CSX_INVALID_INPUT, "solvent parameters are required");
   if (nq == 0) return CSX_OK;

@@ -3,25 +3,25 @@
 //
 //  multipole_geometry  once per (point set, l_max): FP64 real spherical
 //                      harmonics by the associated-Legendre recurrence of
-//                      spherical.rs:54-93, stored FP32 as MFMA B operands.
-//                      Each l owns ceil((2l+1)/16) column tiles of 16 (column
-//                      c of l is m = c - l; padding is 0).  A tile is stored
-//                      in slabs of 16 points, each slab 256 floats laid out so
-//                      that lane 16k + n of a wave reads its four k-step values
-//                      (points 4 ks + k, column n) with one 16-byte load.
+//                      spherical.rs:54-93, stored as FP16 (hi, lo) pairs in
+//                      MFMA B-operand fragments.  Each l owns ceil((2l+1)/16)
+//                      column tiles of 16 (column c of l is m = c - l; padding
+//                      is 0).  A tile is stored in slabs of 32 points.
 //  multipole_coeff     S_t[q][lm] = sum_{i of type t} occ_i j_l(q r_i) Y_lm,i
-//                      as FP32 MFMA (v_mfma_f32_16x16x4f32: 16 q rows x 16
-//                      columns, k = 4 points).  A workgroup owns a chunk of up
-//                      to 512 points of one type, a block of 16 q and a group
-//                      of column tiles (4 waves x MAXT accumulators).  The
-//                      type's amplitudes factor out of its sum, so they are
-//                      applied after it (multipole_contract).  Per slab of 16 points,
-//                      each thread generates occ_i j_l(q r_i) for one
-//                      (point, q) and every l of the group into LDS (FP32
-//                      sin/cos; FP32 upward recurrence for l <= q r; FP64
-//                      Miller downward recurrence from l + 14 above it,
-//                      normalised on the upward value, two passes so no step
-//                      divides), then the waves run the MFMAs of their tiles.  Tiles above the
+//                      on v_mfma_f32_16x16x32_f16 (16 q rows x 16 columns,
+//                      k = 32 points), each FP32 product formed from FP16
+//                      halves as hi*hi + hi*lo + lo*hi.  A workgroup owns a
+//                      chunk of up to 512 points of one type, a block of 16 q
+//                      and a group of column tiles (4 waves x MAXT
+//                      accumulators).  The type's amplitudes factor out of its
+//                      sum, so they are applied after it (multipole_contract).
+//                      Per slab of 32 points, each thread generates occ_i
+//                      j_l(q r_i) for two neighbouring points of one q and
+//                      every l of the group into LDS (FP32 sin/cos; FP32
+//                      upward recurrence for l <= q r; FP64 Miller downward
+//                      recurrence from l + 14 above it, normalised on the
+//                      upward value, two passes so no step divides), then the
+//                      waves run the MFMAs of their tiles.  Tiles above the
 //                      block's largest per-q order are skipped.  Partial
 //                      coefficients per chunk are written in FP32.
 //  multipole_sum_types FP64 sum of the chunk partials of each type, in chunk
@@ -41,30 +41,47 @@ namespace csx {
 namespace {
 
 typedef float v4f __attribute__((ext_vector_type(4)));
+typedef _Float16 v8h __attribute__((ext_vector_type(8)));
 
-constexpr int kSlab = 16;    // points per slab (4 MFMA k-steps of 4)
+constexpr int kSlab = 32;    // points per slab: the k = 32 of one MFMA
 constexpr int kQB = 16;      // q values per MFMA row block
 constexpr int kThreads = 256;
-constexpr int kChunk = 512;  // points per workgroup
+constexpr int kChunk = 512;  // points per workgroup (16 slabs)
+constexpr int kFrag = 512;   // FP16 values in one operand fragment (64 lanes x 8)
 
-// Offset of (point p of a slab, column n) inside a 256-float tile slab: lane
-// 16 k + n holds the values of points 4 ks + k, ks = 0..3, contiguously.
-__device__ inline int slab_offset(int p, int n) { return ((p & 3) * 16 + n) * 4 + (p >> 2); }
+// FP32 operands as FP16 pairs x = hi + lo (22 significant bits together).
+// The harmonics are stored times 2^12 (|Y_lm| <= sqrt((2l + 1) / 4 pi) stays
+// below the FP16 maximum for l < 1600) and the j_l occ operands times
+// 2^kAExp, chosen per launch from the largest occupancy, so that all but
+// negligible values are normal FP16 numbers.  The dropped lo*lo term is
+// ~2^-22 of a product; the sums are scaled back exactly when written.
+constexpr int kYExp = 12;
 
-// j_l operands of one slab in LDS: 256 floats per l, q-major.  The value for
-// (q, point 4 ks + k) sits at q * 16 + 4 xslot(q, k) + ks, so the MFMA lane
-// (k, n = q) reads its four k-step values with one 16-byte load.  The slot
-// is k XOR two bits of q: the 16-byte reads of 16 consecutive lanes (64
-// banks) or of 8 (32 banks) then fall on distinct banks, and a generating
-// wave (4 q x 16 points) stores a permutation of 64 consecutive words.
-constexpr int kXL = 256;  // floats per l
-__device__ inline int xslot(int q, int k) { return k ^ (((q >> 1) ^ (q >> 3)) & 3); }
+// Operand fragment of one slab: lane 16 g + c holds row (A: q) or column
+// (B: l, m) c at the points 8 g .. 8 g + 7 of the slab, the MFMA's k =
+// 8 g + j.  In memory, point p of row/column c is FP16 number frag_half.
+__device__ inline int frag_half(int p, int c) { return (16 * (p >> 3) + c) * 8 + (p & 7); }
+
+// In LDS the 16-byte fragment of lane 16 g + c sits at slot 16 g + (c ^ 4g):
+// the MFMA reads of 16 lanes (one g) cover 16 consecutive slots, and a
+// generating wave (4 rows x 16 point pairs) writes 64 distinct banks.
+__device__ inline int lds_slot(int g, int c) { return 16 * g + (c ^ (4 * g)); }
+
+// Two FP32 values (points 2k and 2k + 1 of one row) as packed FP16 pairs:
+// hi keeps the leading 11 significant bits (exact in FP16 from 2^-14 up), lo
+// is the FP16 (toward zero) of the remainder.
+__device__ inline void split_pair(float v0, float v1, uint32_t& hi, uint32_t& lo) {
+  const float h0 = __uint_as_float(__float_as_uint(v0) & 0xffffe000u);
+  const float h1 = __uint_as_float(__float_as_uint(v1) & 0xffffe000u);
+  hi = __builtin_bit_cast(uint32_t, __builtin_amdgcn_cvt_pkrtz(h0, h1));
+  lo = __builtin_bit_cast(uint32_t, __builtin_amdgcn_cvt_pkrtz(v0 - h0, v1 - h1));
+}
 
 // ------------------------------------------------------------ geometry --
 __global__ void multipole_geometry(const double* __restrict__ xyz, int npts,
                                    const double* __restrict__ centre3, int l_max,
                                    const double* __restrict__ norms, const int* __restrict__ tile_first,
-                                   int npad, float* __restrict__ radius, float* __restrict__ Y) {
+                                   int npad, float* __restrict__ radius, _Float16* __restrict__ Y) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= npts) return;
   const double px = xyz[3 * i] - centre3[0], py = xyz[3 * i + 1] - centre3[1],
@@ -84,7 +101,12 @@ __global__ void multipole_geometry(const double* __restrict__ xyz, int npts,
   const int nslab = npad / kSlab;
   auto store = [&](int l, int col, double v) {
     const int t = tile_first[l] + col / 16;
-    Y[(size_t(t) * nslab + i / kSlab) * 256 + slab_offset(i % kSlab, col & 15)] = float(v);
+    const double y = v * double(1 << kYExp);
+    const _Float16 hi = static_cast<_Float16>(static_cast<float>(y));
+    const _Float16 lo = static_cast<_Float16>(static_cast<float>(y - double(hi)));
+    const size_t h = (size_t(t) * nslab + i / kSlab) * (2 * kFrag) + frag_half(i % kSlab, col & 15);
+    Y[h] = hi;
+    Y[h + kFrag] = lo;
   };
   for (int m = 0; m <= l_max; ++m) {
     if (m > 0) {
@@ -118,9 +140,9 @@ __global__ void multipole_geometry(const double* __restrict__ xyz, int npts,
 }
 
 
-__global__ void fill_zero(float* __restrict__ p, size_t n) {
+__global__ void fill_zero(_Float16* __restrict__ p, size_t n) {
   const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = 0.f;
+  if (i < n) p[i] = static_cast<_Float16>(0.f);
 }
 
 // 1 / a for a positive normal double whose reciprocal is normal: the
@@ -132,26 +154,35 @@ __device__ inline double recip(double a) {
   return fma(r, fma(-a, r, 1.0), r);
 }
 
-// One (point, q) column of spherical Bessel values: X[(l - l_lo) * kXL] =
-// j_l(x) for l in [l_lo, min(lq, l_hi)] and 0 for the rest of [l_lo, l_hi].
+// Miller start above the highest stored order.
+constexpr int kMillerExtra = 14;
+
+// Whether the Miller recurrence from 1e-250 at S down to lu stays inside the
+// double range without tests: its growth bound ((2S+1)/x)^(S - lu) is below
+// 2^1800.
+__device__ inline bool miller_safe(int S, int lu, float x) {
+  return float(S - lu) * (__log2f(float(2 * S + 1)) - __log2f(x)) < 1800.f;
+}
+
+// One (point, q) column of spherical Bessel values: put(l, j_l(x) scale)
+// for l in [l_lo, min(lq, l_hi)] and put(l, 0) for the rest of [l_lo, l_hi].
 // l <= floor(x): FP32 upward recurrence from j_0, j_1 (formed in FP64 from
 // the FP32 sine and cosine; the recurrence is stable there).  Above:
 // Miller's downward recurrence f_{l-1} = (2l+1)/x f_l - f_{l+1} started at
 // S = top + 14 (f = 1e-250, 0) and normalised by f at floor(x) against the
 // upward value.  Pass 1 runs it down to floor(x) for the normalisation and
 // keeps the state at `top`; pass 2 resumes from that state and stores.
-// Where the growth bound ((2S+1)/x)^(S - floor(x)) stays below 2^1800 the
-// recurrence cannot leave the double range from 1e-250 and runs without
-// overflow tests.  Otherwise f is rescaled by 1e-250 on overflow and values
-// computed before the last rescale (below 1e-250 of the normalising value)
-// are stored as 0.  Every stored value is the FP32 j_l times `scale` (one
-// FP32 multiply; 1 leaves it unchanged).
-__device__ inline void bessel_column(float xf, int lq, int l_lo, int l_hi, float scale, float* X) {
+// Where miller_safe holds the recurrence runs without overflow tests.
+// Otherwise f is rescaled by 1e-250 on overflow and values computed before
+// the last rescale (below 1e-250 of the normalising value) are stored as 0.
+// Every stored value is the FP32 j_l times `scale` (one FP32 multiply).
+template <class Put>
+__device__ inline void bessel_column(float xf, int lq, int l_lo, int l_hi, float scale, Put put) {
   const int top = min(lq, l_hi);
   int l_zero = l_lo;  // first l to clear
   if (top >= l_lo) {
     if (!(xf >= 1e-8f)) {
-      if (l_lo == 0) X[0] = scale;
+      if (l_lo == 0) put(0, scale);
       l_zero = max(l_lo, 1);
     } else {
       const double x = double(xf);
@@ -161,10 +192,10 @@ __device__ inline void bessel_column(float xf, int lq, int l_lo, int l_hi, float
       const double j0 = double(sf) * ix;
       const double j1 = (double(sf) * ix - double(cf)) * ix;
       const int lu = min(top, int(floorf(xf)));
-      if (l_lo == 0) X[0] = float(j0) * scale;
+      if (l_lo == 0) put(0, float(j0) * scale);
       double jlu = j0;
       if (lu >= 1) {
-        if (l_lo <= 1) X[(1 - l_lo) * kXL] = float(j1) * scale;
+        if (l_lo <= 1) put(1, float(j1) * scale);
         // FP32 upward recurrence; (2l + 1) / x is formed from the exact odd
         // integer every step, so the coefficient carries one rounding.
         const float ixf = float(ix);
@@ -174,22 +205,21 @@ __device__ inline void bessel_column(float xf, int lq, int l_lo, int l_hi, float
           jm = jc;
           jc = jn;
           odd += 2.f;
-          if (l + 1 >= l_lo) X[(l + 1 - l_lo) * kXL] = jn * scale;
+          if (l + 1 >= l_lo) put(l + 1, jn * scale);
         }
         jlu = double(jc);
       }
       if (top > lu) {
-        const int S = top + 14;
+        const int S = top + kMillerExtra;
         const int lo = max(lu + 1, l_lo);
-        const bool safe = float(S - lu) * (__log2f(float(2 * S + 1)) - __log2f(xf)) < 1800.f;
-        if (safe) {
+        if (miller_safe(S, lu, xf)) {
           // state at index l: f = f_l, fp = f_{l+1}, c = (2l + 1) / x.  The
           // loops take two steps per trip, f and fp trading roles, so the
           // state needs no register moves.
           const double d = 2.0 * ix;
           double fp = 0.0, f = 1e-250, c = double(2 * S + 1) * ix;
 #pragma unroll
-          for (int k = 0; k < 14; k += 2) {  // S down to top
+          for (int k = 0; k < kMillerExtra; k += 2) {  // S down to top
             fp = c * f - fp;
             c -= d;
             f = c * fp - f;
@@ -211,14 +241,14 @@ __device__ inline void bessel_column(float xf, int lq, int l_lo, int l_hi, float
           fp = fp_top;
           c = c_top;
           for (l = top; l - 1 >= lo; l -= 2) {
-            X[(l - l_lo) * kXL] = float(ns * f);
+            put(l, float(ns * f));
             fp = c * f - fp;
             c -= d;
-            X[(l - 1 - l_lo) * kXL] = float(ns * fp);
+            put(l - 1, float(ns * fp));
             f = c * fp - f;
             c -= d;
           }
-          if (l >= lo) X[(l - l_lo) * kXL] = float(ns * f);
+          if (l >= lo) put(l, float(ns * f));
         } else {
           // pass 1: normalisation and the last rescale
           double fp = 0.0, f = 1e-250, c = double(2 * S + 1) * ix;
@@ -240,7 +270,7 @@ __device__ inline void bessel_column(float xf, int lq, int l_lo, int l_hi, float
           f = 1e-250;
           c = double(2 * S + 1) * ix;
           for (int l = S; l >= lo; --l) {
-            if (l <= top) X[(l - l_lo) * kXL] = l <= last ? float(norm * f) * scale : 0.f;
+            if (l <= top) put(l, l <= last ? float(norm * f) * scale : 0.f);
             const double fn = c * f - fp;
             fp = f;
             f = fn;
@@ -255,19 +285,135 @@ __device__ inline void bessel_column(float xf, int lq, int l_lo, int l_hi, float
       l_zero = top + 1;
     }
   }
-  for (int l = max(l_zero, l_lo); l <= l_hi; ++l) X[(l - l_lo) * kXL] = 0.f;
+  for (int l = max(l_zero, l_lo); l <= l_hi; ++l) put(l, 0.f);
+}
+
+// The columns of two points of one q (x0, x1) in lock step: put2(l, v0, v1)
+// with the values bessel_column gives each of them.  Both use the same
+// split order lu = floor(min(x0, x1)): for the larger x, lu is floor(x) or
+// one less, where j_lu(x) is still positive (the first zero of j_l lies
+// beyond l + 2) and the upward recurrence still stable.  Returns false,
+// having stored nothing, where that does not hold (floor(x) differing by
+// more than one, x below 1e-8) or the Miller recurrence would need overflow
+// tests; the caller then takes the two columns one at a time.
+template <class Put2>
+__device__ inline bool bessel_pair(float x0f, float x1f, int lq, int l_lo, int l_hi, float s0,
+                                   float s1, Put2 put2) {
+  const int top = min(lq, l_hi);
+  if (top < l_lo) {
+    for (int l = l_lo; l <= l_hi; ++l) put2(l, 0.f, 0.f);
+    return true;
+  }
+  const float xmin = fminf(x0f, x1f);
+  if (!(xmin >= 1e-8f)) return false;
+  const int fl = int(floorf(xmin));
+  if (int(floorf(fmaxf(x0f, x1f))) > fl + 1) return false;
+  const int lu = min(top, fl);
+  const int S = top + kMillerExtra;
+  if (top > lu && !miller_safe(S, lu, xmin)) return false;
+  float sf0, cf0, sf1, cf1;
+  sincosf(x0f, &sf0, &cf0);
+  sincosf(x1f, &sf1, &cf1);
+  const double ix0 = recip(double(x0f)), ix1 = recip(double(x1f));
+  const double a0 = double(sf0) * ix0, a1 = double(sf1) * ix1;
+  const double b0 = (a0 - double(cf0)) * ix0, b1 = (a1 - double(cf1)) * ix1;
+  if (l_lo == 0) put2(0, float(a0) * s0, float(a1) * s1);
+  double u0 = a0, u1 = a1;  // j_lu
+  if (lu >= 1) {
+    if (l_lo <= 1) put2(1, float(b0) * s0, float(b1) * s1);
+    const float ixf0 = float(ix0), ixf1 = float(ix1);
+    float jm0 = float(a0), jc0 = float(b0), jm1 = float(a1), jc1 = float(b1), odd = 3.f;
+    int l = 1;
+    for (; l < lu && l + 1 < l_lo; ++l) {  // orders below the group: not stored
+      const float jn0 = (odd * ixf0) * jc0 - jm0;
+      const float jn1 = (odd * ixf1) * jc1 - jm1;
+      jm0 = jc0;
+      jc0 = jn0;
+      jm1 = jc1;
+      jc1 = jn1;
+      odd += 2.f;
+    }
+    for (; l < lu; ++l) {
+      const float jn0 = (odd * ixf0) * jc0 - jm0;
+      const float jn1 = (odd * ixf1) * jc1 - jm1;
+      jm0 = jc0;
+      jc0 = jn0;
+      jm1 = jc1;
+      jc1 = jn1;
+      odd += 2.f;
+      put2(l + 1, jn0 * s0, jn1 * s1);
+    }
+    u0 = double(jc0);
+    u1 = double(jc1);
+  }
+  if (top > lu) {
+    const int lo = max(lu + 1, l_lo);
+    const double d0 = 2.0 * ix0, d1 = 2.0 * ix1;
+    double p0 = 0.0, f0 = 1e-250, c0 = double(2 * S + 1) * ix0;
+    double p1 = 0.0, f1 = 1e-250, c1 = double(2 * S + 1) * ix1;
+#pragma unroll
+    for (int k = 0; k < kMillerExtra; k += 2) {
+      p0 = c0 * f0 - p0;
+      p1 = c1 * f1 - p1;
+      c0 -= d0;
+      c1 -= d1;
+      f0 = c0 * p0 - f0;
+      f1 = c1 * p1 - f1;
+      c0 -= d0;
+      c1 -= d1;
+    }
+    const double F0 = f0, P0 = p0, C0 = c0, F1 = f1, P1 = p1, C1 = c1;
+    int l = top;
+    for (; l - 1 > lu; l -= 2) {
+      p0 = c0 * f0 - p0;
+      p1 = c1 * f1 - p1;
+      c0 -= d0;
+      c1 -= d1;
+      f0 = c0 * p0 - f0;
+      f1 = c1 * p1 - f1;
+      c0 -= d0;
+      c1 -= d1;
+    }
+    if (l > lu) {
+      f0 = c0 * f0 - p0;
+      f1 = c1 * f1 - p1;
+    }
+    const double n0 = u0 * recip(f0) * double(s0), n1 = u1 * recip(f1) * double(s1);
+    f0 = F0;
+    p0 = P0;
+    c0 = C0;
+    f1 = F1;
+    p1 = P1;
+    c1 = C1;
+    for (l = top; l - 1 >= lo; l -= 2) {
+      put2(l, float(n0 * f0), float(n1 * f1));
+      p0 = c0 * f0 - p0;
+      p1 = c1 * f1 - p1;
+      c0 -= d0;
+      c1 -= d1;
+      put2(l - 1, float(n0 * p0), float(n1 * p1));
+      f0 = c0 * p0 - f0;
+      f1 = c1 * p1 - f1;
+      c0 -= d0;
+      c1 -= d1;
+    }
+    if (l >= lo) put2(l, float(n0 * f0), float(n1 * f1));
+  }
+  for (int l = max(top + 1, l_lo); l <= l_hi; ++l) put2(l, 0.f, 0.f);
+  return true;
 }
 
 // partial[(chunk * nqp + q) * ncol + col]: the sum over the chunk's points
-// (all of one type) of occ j_l Y.
-// At least four waves per SIMD (128 registers).
-template <int MAXT>
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) void multipole_coeff(
+// (all of one type) of occ j_l Y, times 2^-(kYExp + aexp) where ascale =
+// 2^aexp scales the j_l occ operands.
+// At least WPE waves per SIMD: 2 (256 registers) or 3 (168).
+template <int MAXT, int WPE>
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(WPE))) void multipole_coeff(
     const float* __restrict__ radius, const float* __restrict__ occ,
     const int4* __restrict__ chunks /* (first slab, end slab, type) */, int nchunk, int nslab,
-    const float* __restrict__ Y, int ntiles, const int* __restrict__ tile_l,
+    const _Float16* __restrict__ Y, int ntiles, const int* __restrict__ tile_l,
     const float* __restrict__ qv, const int* __restrict__ qlmax, int nqp, int ngroups,
-    float* __restrict__ partial) {
+    float ascale, float unscale, float* __restrict__ partial) {
   extern __shared__ float smem[];
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -290,12 +436,11 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
   const int l_lo = tile_l[g0];
   if (l_lo > Lb) return;  // every tile of the group is above this block's orders
   const int l_hi = min(tile_l[g1 - 1], Lb);
-  const int lr = l_hi - l_lo + 1;
   const int4 ck = chunks[chunk];
   const int s0 = ck.x, s1 = ck.y;
-  float* X = smem;                   // [lr][16 q][4 slots][4 ks]
-  float* PR = X + lr * kXL;          // [kChunk] radius and occupancy of the
-  float* PO = PR + kChunk;           //   chunk's points
+  float* PR = smem;        // [kChunk] radius and occupancy of the chunk's points
+  float* PO = PR + kChunk;
+  _Float16* X = reinterpret_cast<_Float16*>(PO + kChunk);  // [l][hi, lo fragment]
   for (int k = threadIdx.x; k < (s1 - s0) * kSlab; k += kThreads) {
     PR[k] = radius[s0 * kSlab + k];
     PO[k] = occ[s0 * kSlab + k];
@@ -305,14 +450,20 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
 #pragma unroll
   for (int t = 0; t < MAXT; ++t) acc[t] = v4f{0.f, 0.f, 0.f, 0.f};
 
-  // generation role: wave w takes q = 4 w + lane / 16 and point lane % 16 of
-  // the slab.  Points are sorted by radius, so the lanes of a wave see
-  // nearly equal q r and run the recurrences with little divergence.
-  const int gq = 4 * wave + (lane >> 4), gp = lane & 15;
+  // generation role: wave w takes rows q = 4 w + lane / 16 and the point
+  // pair 2 (lane % 16), + 1 of the slab.  Points are sorted by radius, so the
+  // lanes of a wave see nearly equal q r and run the recurrences with little
+  // divergence.
+  const int gq = 4 * wave + (lane >> 4), gpp = lane & 15;
   const float qg = qv[q0 + gq];
   const int lg = qlmax[q0 + gq];
-  // MFMA role: lane = 16 k + n
+  // this thread's dword in the hi fragment of order l: Xw[(l - l_lo) * kFrag]
+  // (lo fragment kFrag / 2 dwords further)
+  uint32_t* const Xw = reinterpret_cast<uint32_t*>(X) + 4 * lds_slot(gpp >> 2, gq) + (gpp & 3);
+  _Float16* const Xh = X + 8 * lds_slot(gpp >> 2, gq) + 2 * (gpp & 3);
+  // MFMA role: lane = 16 g + c, rows/columns c, points 8 g .. 8 g + 7
   const int mk = lane >> 4, mn = lane & 15;
+  const _Float16* const Xa = X + 8 * lds_slot(mk, mn);
 
   // This wave's tiles are g0 + wave + 4 t.  tile_l is non-decreasing, so the
   // live ones (inside the group and at most l_hi) are t < nt; xoff[t] is the
@@ -324,52 +475,82 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
     const int tile = g0 + wave + 4 * t;
     const int l = tile < g1 ? tile_l[tile] : l_hi + 1;
     if (l <= l_hi) nt = t + 1;
-    xoff[t] = (min(l, l_hi) - l_lo) * kXL + mn * 16 + 4 * xslot(mn, mk);
+    xoff[t] = (min(l, l_hi) - l_lo) * (2 * kFrag);
   }
   // Harmonic tile rows read by this wave; tiles past the live ones read the
   // last live (or, with none, the last existing) tile, so every load in the
-  // loop is unconditional and in bounds.
+  // loop is in bounds.
   size_t yrow[MAXT];
 #pragma unroll
   for (int t = 0; t < MAXT; ++t)
-    yrow[t] = size_t(min(g0 + wave + 4 * max(min(t, nt - 1), 0), ntiles - 1)) * nslab * 256;
+    yrow[t] = size_t(min(g0 + wave + 4 * max(min(t, nt - 1), 0), ntiles - 1)) * nslab * (2 * kFrag);
 
-  // Harmonic tiles are read in batches of kB: the loads of batch b + 1 are in
-  // flight while batch b runs its MFMAs.
+  // Harmonic tiles are read in batches of kB: the first batch of a slab is
+  // requested before its j_l are generated, and batch b + 1 while batch b
+  // runs its MFMAs.
   constexpr int kB = 2, kNB = MAXT / kB;
   static_assert(MAXT % kB == 0, "whole batches of tiles");
   for (int s = s0; s < s1; ++s) {
-    __syncthreads();  // the previous slab's operands have been read
-    // j_l times the point's occupancy (padding points have occupancy 0)
-    const int k = (s - s0) * kSlab + gp;
-    bessel_column(qg * PR[k], lg, l_lo, l_hi, PO[k],
-                  X + gq * 16 + 4 * xslot(gq, gp & 3) + (gp >> 2));
-    __syncthreads();
-    const float* Ys = Y + size_t(s) * 256 + lane * 4;
-    v4f yb[2][kB];
+    const _Float16* Ys = Y + size_t(s) * (2 * kFrag) + lane * 8;
+    v8h yh[2][kB], yl[2][kB];
 #pragma unroll
-    for (int u = 0; u < kB; ++u) yb[0][u] = *reinterpret_cast<const v4f*>(Ys + yrow[u]);
+    for (int u = 0; u < kB; ++u) {
+      yh[0][u] = *reinterpret_cast<const v8h*>(Ys + yrow[u]);
+      yl[0][u] = *reinterpret_cast<const v8h*>(Ys + yrow[u] + kFrag);
+    }
+    __syncthreads();  // the previous slab's operands have been read
+    // j_l times the points' occupancy (padding points have occupancy 0)
+    {
+      const int k = (s - s0) * kSlab + 2 * gpp;
+      const float x0 = qg * PR[k], x1 = qg * PR[k + 1];
+      const float o0 = PO[k] * ascale, o1 = PO[k + 1] * ascale;
+      const bool paired = bessel_pair(x0, x1, lg, l_lo, l_hi, o0, o1, [&](int l, float v0, float v1) {
+        uint32_t hi, lo;
+        split_pair(v0, v1, hi, lo);
+        uint32_t* p = Xw + (l - l_lo) * kFrag;
+        p[0] = hi;
+        p[kFrag / 2] = lo;
+      });
+      if (!paired) {
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+          bessel_column(j ? x1 : x0, lg, l_lo, l_hi, j ? o1 : o0, [&](int l, float v) {
+            const _Float16 hi = static_cast<_Float16>(v);
+            const _Float16 lo = static_cast<_Float16>(v - static_cast<float>(hi));
+            _Float16* p = Xh + (l - l_lo) * (2 * kFrag) + j;
+            p[0] = hi;
+            p[kFrag] = lo;
+          });
+      }
+    }
+    __syncthreads();
 #pragma unroll
     for (int b = 0; b < kNB; ++b) {
-      if (b + 1 < kNB) {
+      if (b * kB >= nt) break;
+      if (b + 1 < kNB && (b + 1) * kB < nt) {
 #pragma unroll
-        for (int u = 0; u < kB; ++u)
-          yb[(b + 1) & 1][u] = *reinterpret_cast<const v4f*>(Ys + yrow[(b + 1) * kB + u]);
+        for (int u = 0; u < kB; ++u) {
+          const size_t r = yrow[(b + 1) * kB + u];
+          yh[(b + 1) & 1][u] = *reinterpret_cast<const v8h*>(Ys + r);
+          yl[(b + 1) & 1][u] = *reinterpret_cast<const v8h*>(Ys + r + kFrag);
+        }
       }
       // both tiles' A operands are read before either tile's MFMAs (offsets
       // clamped to the group's orders, so the reads are in bounds)
-      v4f xa[kB];
+      v8h ah[kB], al[kB];
 #pragma unroll
-      for (int u = 0; u < kB; ++u) xa[u] = *reinterpret_cast<const v4f*>(X + xoff[b * kB + u]);
+      for (int u = 0; u < kB; ++u) {
+        ah[u] = *reinterpret_cast<const v8h*>(Xa + xoff[b * kB + u]);
+        al[u] = *reinterpret_cast<const v8h*>(Xa + xoff[b * kB + u] + kFrag);
+      }
 #pragma unroll
       for (int u = 0; u < kB; ++u) {
         const int t = b * kB + u;
         if (t >= nt) continue;
-        const v4f a = xa[u];
-        const v4f y = yb[b & 1][u];
-#pragma unroll
-        for (int ks = 0; ks < 4; ++ks)
-          acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[ks], y[ks], acc[t], 0, 0, 0);
+        const v8h bh = yh[b & 1][u], bl = yl[b & 1][u];
+        acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[u], bh, acc[t], 0, 0, 0);
+        acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[u], bl, acc[t], 0, 0, 0);
+        acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al[u], bh, acc[t], 0, 0, 0);
       }
     }
   }
@@ -381,7 +562,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const int q = q0 + 4 * mk + r;
-      partial[(size_t(chunk) * nqp + q) * ncol + tile * 16 + mn] = acc[t][r];
+      partial[(size_t(chunk) * nqp + q) * ncol + tile * 16 + mn] = acc[t][r] * unscale;
     }
   }
 }
@@ -470,35 +651,42 @@ __global__ __launch_bounds__(256) void multipole_contract(const double* __restri
           ((scratch[k] + scratch[NT + k]) + scratch[2 * NT + k]) + scratch[3 * NT + k];
 }
 
-// Accumulator tiles per wave (4 x MAXT tiles per tile group): 16, or 8
-// where 8 still covers the tiles in as many groups.  Fewer accumulators
-// raise occupancy; more groups generate j_l once more each.  At 108k points
-// x 200 q, MAXT 8 vs 16: L = 15 (one group either way) 0.46 vs 0.52 ms;
-// L = 25 (two groups vs one) 0.99 vs 0.81 ms.
-constexpr int kMaxT = 16;
+// Accumulator tiles per wave (4 x MAXT tiles per tile group): 8, 16 or 32.
+// Each tile group generates j_l once more, more accumulators lower the
+// occupancy, and a group's l range sets its LDS (2 KB per order).  The
+// automatic choice is one group of 64 where 33 to 64 tiles cover the
+// expansion and groups of 32 otherwise: on the 108k-point prolate with
+// sites, 200 q, groups of 32 / 64 / 128 took 0.25 / 0.31 / 0.32 ms at
+// l_max 15, 0.57 / 0.54 / 0.55 ms at 25, 1.06 / 1.08 / 1.38 ms at 40 and
+// 2.40 / 2.54 / 3.89 ms at 64.
+constexpr int kGroupTiles[] = {32, 64, 128};
 
 }  // namespace
 
 int multipole_chunk() { return kChunk; }
 
-int multipole_tiles_per_group(int ntiles) {
-  const int big = 4 * kMaxT, small = big / 2;
-  return (ntiles + small - 1) / small == (ntiles + big - 1) / big ? small : big;
+int multipole_slab() { return kSlab; }
+
+int multipole_tiles_per_group(int ntiles, int tuned) {
+  for (int t : kGroupTiles)
+    if (t == tuned) return t;
+  return ntiles > 32 && ntiles <= 64 ? 64 : 32;
 }
 
 size_t multipole_coeff_lds_bytes(int l_range) {
-  return (size_t(l_range) * kXL + 2 * kChunk) * sizeof(float);
+  return size_t(l_range) * 2 * kFrag * sizeof(_Float16) + 2 * kChunk * sizeof(float);
 }
 
 int multipole_max_types() { return 256; }
 
 hipError_t launch_multipole_geometry(const double* xyz, int npts, const double* centre,
                                      int l_max, const double* norms, const int* tile_first,
-                                     int ntiles, int npad, float* radius, float* Y,
+                                     int ntiles, int npad, float* radius, void* Yraw,
                                      hipStream_t stream) {
+  _Float16* Y = static_cast<_Float16*>(Yraw);
   if (npts <= 0) return hipSuccess;
   if (npad % kSlab != 0) return hipErrorInvalidValue;
-  const size_t ny = size_t(ntiles) * npad * 16;
+  const size_t ny = size_t(ntiles) * npad * 32;  // 16 columns x (hi, lo) per point
   fill_zero<<<(ny + 255) / 256, 256, 0, stream>>>(Y, ny);
   multipole_geometry<<<(npts + 127) / 128, 128, 0, stream>>>(xyz, npts, centre, l_max, norms,
                                                              tile_first, npad, radius, Y);
@@ -506,28 +694,36 @@ hipError_t launch_multipole_geometry(const double* xyz, int npts, const double* 
 }
 
 hipError_t launch_multipole_coeff(const float* radius, const float* occ, const int* chunks,
-                                  int nchunk, int npad, const float* Y, int ntiles,
-                                  const int* tile_l, int max_l_range, const int* type_chunk,
-                                  int ntypes, const float* qv, const int* qlmax, int nqp,
-                                  float* partial, double* coef, hipStream_t stream,
-                                  KernelTimer* timer) {
-  if (nqp % kQB != 0 || npad % kSlab != 0 || ntiles <= 0 || nchunk <= 0 || ntypes <= 0)
+                                  int nchunk, int npad, const void* Yraw, int ntiles,
+                                  const int* tile_l, int tpg, int max_l_range, int aexp,
+                                  const int* type_chunk, int ntypes, const float* qv,
+                                  const int* qlmax, int nqp, float* partial, double* coef,
+                                  hipStream_t stream, KernelTimer* timer) {
+  if (nqp % kQB != 0 || npad % kSlab != 0 || ntiles <= 0 || nchunk <= 0 || ntypes <= 0 ||
+      (tpg != 32 && tpg != 64 && tpg != 128) || aexp < -60 || aexp > 60)
     return hipErrorInvalidValue;
-  const int tpg = multipole_tiles_per_group(ntiles);
   const int ngroups = (ntiles + tpg - 1) / tpg;
   const long long total = (long long)(nqp / kQB) * nchunk * ngroups;
   if (total > INT32_MAX) return hipErrorInvalidValue;
   const size_t lds = multipole_coeff_lds_bytes(max_l_range);
   const int4* ck = reinterpret_cast<const int4*>(chunks);
   const int nslab = npad / kSlab;
+  const _Float16* Y = static_cast<const _Float16*>(Yraw);
+  const float ascale = ldexpf(1.f, aexp), unscale = ldexpf(1.f, -(kYExp + aexp));
   {
     TimerScope scope(timer, "multipole_coeff", stream);
-    if (tpg == 4 * kMaxT)
-      multipole_coeff<kMaxT><<<unsigned(total), kThreads, lds, stream>>>(
-          radius, occ, ck, nchunk, nslab, Y, ntiles, tile_l, qv, qlmax, nqp, ngroups, partial);
+    if (tpg == 128)
+      multipole_coeff<32, 2><<<unsigned(total), kThreads, lds, stream>>>(
+          radius, occ, ck, nchunk, nslab, Y, ntiles, tile_l, qv, qlmax, nqp, ngroups, ascale,
+          unscale, partial);
+    else if (tpg == 64)
+      multipole_coeff<16, 2><<<unsigned(total), kThreads, lds, stream>>>(
+          radius, occ, ck, nchunk, nslab, Y, ntiles, tile_l, qv, qlmax, nqp, ngroups, ascale,
+          unscale, partial);
     else
-      multipole_coeff<kMaxT / 2><<<unsigned(total), kThreads, lds, stream>>>(
-          radius, occ, ck, nchunk, nslab, Y, ntiles, tile_l, qv, qlmax, nqp, ngroups, partial);
+      multipole_coeff<8, 3><<<unsigned(total), kThreads, lds, stream>>>(
+          radius, occ, ck, nchunk, nslab, Y, ntiles, tile_l, qv, qlmax, nqp, ngroups, ascale,
+          unscale, partial);
   }
   const int ncol = ntiles * 16;
   multipole_sum_types<<<dim3((ncol + 255) / 256, nqp, ntypes), 256, 0, stream>>>(

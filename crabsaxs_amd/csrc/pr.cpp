@@ -66,6 +66,7 @@ extern "C" {
 csx_status csx_coordinate_features(csx_structure* s, size_t bins_in, double* rg,
                                    double* dmax_out, double* r_out, double* p_out,
                                    size_t* n_out, double* pr_rg, double* pr_dmax, int* pr_flags) {
+  CSX_API_RANGE();
   if (!s || !rg || !dmax_out || !r_out || !p_out || !n_out || !pr_rg || !pr_dmax || !pr_flags)
     return fail(CSX_INVALID_INPUT, "structure and output buffers are required");
   const size_t n = s->n;

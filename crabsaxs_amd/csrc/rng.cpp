@@ -121,6 +121,7 @@ csx_status csx_pcg32(uint64_t state, uint64_t increment, size_t n, int output_ne
 }
 
 csx_status csx_bootstrap_indices(uint64_t seed, size_t n, size_t replicates, uint64_t* out) {
+  CSX_API_RANGE();
   if (n == 0 || (replicates > 0 && !out))
     return fail(CSX_INVALID_INPUT, "a non-empty curve and an output buffer are required");
   ChaCha8 rng(seed);

@@ -20,6 +20,11 @@
  *   - Functions are reentrant: handles may be shared between host threads
  *     (the reference calls its seam from rayon workers, ensemble.rs:244-256).
  *   - Coordinates are Angstrom, q is 1/Angstrom.
+ *   - Every call that does work is wrapped in a roctx range named after the
+ *     entry point (rocprofv3 --marker-trace shows them).
+ *   - CSX_DEVICE_MEMORY_LIMIT=<bytes> in the environment (read once per
+ *     process) caps the device memory the library holds: an allocation
+ *     beyond it fails with CSX_OUT_OF_MEMORY, as a real one would.
  */
 #ifndef CRABSAXS_H
 #define CRABSAXS_H
@@ -178,6 +183,14 @@ csx_status csx_ctx_set_workspace_limit(csx_ctx *ctx, size_t bytes);
  * |q| r_max / pi <= 250; otherwise the automatic choice among 16 and 32).
  * No reference counterpart. */
 csx_status csx_ctx_set_vjp_tile(csx_ctx *ctx, int q_tile);
+
+/* Column tiles (16 (l, m) columns each) per workgroup of the multipole
+ * coefficient kernel: 32, 64 or 128 (8, 16 or 32 accumulator tiles per
+ * wave), or 0 (the default: the fewest tile groups, each generating j_l
+ * once, with the fewest accumulators that give them).  The sums are the
+ * same for every choice up to FP32 accumulation order.  No reference
+ * counterpart. */
+csx_status csx_ctx_set_multipole_group(csx_ctx *ctx, int tiles);
 
 /* Workgroups sharing each 256-step sweep of the coordinate adjoint: 0 (the
  * default: automatic, 2, 4 or 8 where the block pairs and q chunks leave
