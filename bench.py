@@ -565,6 +565,7 @@ def small_leg(csx, ctx, log):
             got, dt, kms, _ = timed(ctx, "debye_pairs_f32",
                                     lambda: csx.debye_intensity(s, q, ctx=ctx), 20)
             launch = ctx.last_launch()
+            launch.update(ctx.last_pair_items())
             ref = csx.debye_intensity(s, q, precision=csx.Precision.FP64_PLAIN, ctx=ctx)
             w = np.ones(nq)
             qa = np.linspace(0.01, 1.0, nq)
@@ -678,7 +679,7 @@ def main():
     # return repeated calls' tables) is off, and its effect is measured
     # separately.
     ctx.set_cache_limit(0)
-    tune = dict(q_tile=0, sin_mode=2, j_chunk=512, persist=-1)
+    tune = dict(q_tile=0, sin_mode=2, j_chunk=0, persist=-1)
     explicit = {k: v for k, v in (("sin_mode", args.sin_mode), ("q_tile", args.q_tile),
                                   ("j_chunk", args.j_chunk), ("persist", args.persist))
                 if v is not None}
@@ -726,7 +727,9 @@ def main():
     # HBM bytes a launch needs: the FP32 points once and the FP64 tables out;
     # the design adds FP64 partials per (work item, q) and the points of
     # every item's blocks (most of them from L2)
-    nitems, ntypes = pair_items(s, tune["j_chunk"])
+    j_chunk = ctx.last_pair_items()["j_chunk"]
+    nitems, ntypes = pair_items(s, j_chunk)
+    assert nitems == ctx.last_pair_items()["items"]
     nq_pad = -(-N_Q // launch["q_tile"]) * launch["q_tile"]
     algorithmic_bytes = {"minimum": N_ATOMS * 16 + ntypes * (ntypes + 1) // 2 * N_Q * 8,
                          "design": nitems * nq_pad * 8 + N_ATOMS * 16,

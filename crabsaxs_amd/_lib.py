@@ -180,6 +180,7 @@ SIGNATURES = {
                                                 C.c_void_p]),
     "csx_ctx_last_launch": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int),
                                       C.POINTER(C.c_int)]),
+    "csx_ctx_last_pair_items": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "csx_batch_debye_intensity": (C.c_int, [C.POINTER(C.c_void_p), C.c_size_t, c_double_p,
                                             C.c_size_t, C.c_int, c_double_p]),
     "csx_batch_debye_solvent_basis": (C.c_int, [C.POINTER(C.c_void_p), C.c_size_t, c_double_p,

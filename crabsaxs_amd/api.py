@@ -358,8 +358,9 @@ class Context:
         _check(_lib.load().csx_ctx_kernel_time(self.ptr, name.encode(), C.byref(n), C.byref(ms)))
         return int(n.value), float(ms.value)
 
-    def set_tuning(self, q_tile: int = 0, sin_mode: int = 2, j_chunk: int = 512) -> None:
-        """Pair-kernel launch shape; q_tile 0 picks the tile per call."""
+    def set_tuning(self, q_tile: int = 0, sin_mode: int = 2, j_chunk: int = 0) -> None:
+        """Pair-kernel launch shape; q_tile 0 picks the tile per call and
+        j_chunk 0 the j chunk per structure."""
         _check(_lib.load().csx_ctx_set_tuning(self.ptr, q_tile, sin_mode, j_chunk))
 
     def set_cache_limit(self, nbytes: int = DEFAULT_CACHE_LIMIT) -> None:
@@ -388,6 +389,12 @@ class Context:
         v = [C.c_int() for _ in range(3)]
         _check(_lib.load().csx_ctx_last_launch(self.ptr, *(C.byref(x) for x in v)))
         return {"q_tile": v[0].value, "sin_mode": v[1].value, "persistent_workgroups": v[2].value}
+
+    def last_pair_items(self) -> dict:
+        """j chunk and work items of the most recent pair-kernel launch."""
+        v = [C.c_int() for _ in range(2)]
+        _check(_lib.load().csx_ctx_last_pair_items(self.ptr, *(C.byref(x) for x in v)))
+        return {"j_chunk": v[0].value, "items": v[1].value}
 
     def cu_count(self) -> int:
         n = C.c_int()

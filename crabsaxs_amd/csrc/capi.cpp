@@ -214,6 +214,20 @@ csx_status get_point_set(csx_structure* s, const csx_solvent_params* p,
   return CSX_OK;
 }
 
+// j points per work item: the tuned value, or automatically the largest of
+// 512, 256, 128 and 64 that still gives at least one full 256 x chunk item
+// per CU (n^2 / 2 pairs over 256 x chunk per item); small sets take short j
+// sweeps so that their work spreads over the device (1,200 points on 256 CUs:
+// 64).  The chunk orders the FP64 sums, so a set's tables depend on it; the
+// automatic value depends only on the set's point count and the CU count.
+int pair_j_chunk(int tuned, const PointSet& ps, int cus) {
+  if (tuned > 0) return tuned;
+  const double n = static_cast<double>(ps.natoms + ps.nsites);
+  for (int c : {512, 256, 128})
+    if (n * n / (2.0 * 256.0 * c) >= std::max(cus, 1)) return c;
+  return 64;
+}
+
 csx_status get_item_set(PointSet& ps, int chunk, std::shared_ptr<ItemSet>* out) {
   std::lock_guard<std::mutex> lock(ps.items_mu);
   auto it = ps.item_sets.find(chunk);
@@ -442,15 +456,20 @@ static csx_status pair_tables_batch(Call& call, const std::vector<PointSet*>& se
   double rmax = 0.0;
   for (const PointSet* ps : sets) rmax = std::max(rmax, ps->max_radius);
   const int sin_mode = pair_sin_mode(tune.sin_mode, q, nq, rmax);
-  const int j_chunk = tune.j_chunk;
-  const std::string key = table_key(sin_mode, j_chunk, q, nq);
+  const int cus = call.ctx()->cu_count;
+  std::vector<int> chunk(sets.size());
+  std::vector<std::string> keys(sets.size());
+  for (size_t c = 0; c < sets.size(); ++c) {
+    chunk[c] = pair_j_chunk(tune.j_chunk, *sets[c], cus);
+    keys[c] = table_key(sin_mode, chunk[c], q, nq);
+  }
 
   P.assign(sets.size(), std::vector<double>());
   std::vector<size_t> todo;
   std::shared_ptr<CacheEntry> entry;  // the one set's entry, locked while computed
   std::unique_lock<std::mutex> entry_lock;
   if (sets.size() == 1) {
-    entry = cache_acquire(call.ctx(), *sets[0], key, &entry_lock);
+    entry = cache_acquire(call.ctx(), *sets[0], keys[0], &entry_lock);
     if (entry->ready) {
       P[0] = entry->tables;
       if (overlap) overlap();
@@ -459,7 +478,7 @@ static csx_status pair_tables_batch(Call& call, const std::vector<PointSet*>& se
     todo.push_back(0);
   } else {
     for (size_t c = 0; c < sets.size(); ++c) {
-      std::shared_ptr<CacheEntry> e = cache_find(call.ctx(), *sets[c], key);
+      std::shared_ptr<CacheEntry> e = cache_find(call.ctx(), *sets[c], keys[c]);
       if (e) P[c] = e->tables;
       else todo.push_back(c);
     }
@@ -482,7 +501,7 @@ static csx_status pair_tables_batch(Call& call, const std::vector<PointSet*>& se
   size_t npts = 0, nitems = 0, nbins = 0;
   for (size_t t = 0; t < todo.size(); ++t) {
     PointSet* ps = sets[todo[t]];
-    CSX_TRY(get_item_set(*ps, j_chunk, &isets[t]));
+    CSX_TRY(get_item_set(*ps, chunk[todo[t]], &isets[t]));
     npts += static_cast<size_t>(ps->natoms + ps->nsites);
     nitems += static_cast<size_t>(isets[t]->nitems);
     nbins += static_cast<size_t>(ps->nbins);
@@ -495,7 +514,6 @@ static csx_status pair_tables_batch(Call& call, const std::vector<PointSet*>& se
   if (npts > static_cast<size_t>(INT32_MAX / 2) || nitems > static_cast<size_t>(INT32_MAX / 2) ||
       nbins > static_cast<size_t>(INT32_MAX / 2))
     return fail(CSX_INVALID_INPUT, "batch too large for one launch");
-  const int cus = call.ctx()->cu_count;
   const int qt = choose_q_tile(tune.q_tile, sin_mode, nq, sum_j, max_j, cus);
   const size_t nq_pad = (nq + qt - 1) / qt * qt;
   std::vector<float> qrev(nq_pad, 0.0f);
@@ -566,6 +584,8 @@ static csx_status pair_tables_batch(Call& call, const std::vector<PointSet*>& se
     call.ctx()->last_launch[0] = qt;
     call.ctx()->last_launch[1] = sin_mode;
     call.ctx()->last_launch[2] = persist_wgs;
+    call.ctx()->last_items[0] = chunk[todo[0]];
+    call.ctx()->last_items[1] = static_cast<int>(nitems);
   }
   CSX_HIP(launch_debye_pairs(pts, items, static_cast<int>(nitems), bins, static_cast<int>(nbins),
                              d_q.as<float>(), static_cast<int>(nq_pad), qt, sin_mode, persist_wgs,
@@ -607,12 +627,12 @@ static csx_status pair_tables_batch(Call& call, const std::vector<PointSet*>& se
   }
   if (entry) {
     entry->tables = P[0];
-    entry->bytes = P[0].size() * sizeof(double) + key.size();
+    entry->bytes = P[0].size() * sizeof(double) + keys[0].size();
     entry->ready = true;
     cache_commit(call.ctx(), *sets[0], entry);
   } else {
     for (size_t t = 0; t < todo.size(); ++t)
-      cache_store(call.ctx(), *sets[todo[t]], key, P[todo[t]]);
+      cache_store(call.ctx(), *sets[todo[t]], keys[todo[t]], P[todo[t]]);
   }
   return CSX_OK;
 }
@@ -866,8 +886,10 @@ csx_status csx_ctx_set_tuning(csx_ctx* ctx, int q_tile, int sin_mode, int j_chun
     return fail(CSX_INVALID_INPUT,
                 "q_tile must be 0 (automatic), a multiple of 8 up to 64, 50 or 63");
   if (sin_mode < 0 || sin_mode > 2) return fail(CSX_INVALID_INPUT, "sin_mode must be 0, 1 or 2");
-  if (j_chunk < 256 || j_chunk % 256 != 0 || j_chunk > (1 << 20))
-    return fail(CSX_INVALID_INPUT, "j_chunk must be a multiple of 256 in [256, 2^20]");
+  if (!(j_chunk == 0 || j_chunk == 32 || j_chunk == 64 || j_chunk == 128 ||
+        (j_chunk >= 256 && j_chunk % 256 == 0 && j_chunk <= (1 << 20))))
+    return fail(CSX_INVALID_INPUT,
+                "j_chunk must be 0 (automatic), 32, 64, 128 or a multiple of 256 in [256, 2^20]");
   std::lock_guard<std::mutex> lock(ctx->mu);
   ctx->tuning.q_tile = q_tile;
   ctx->tuning.sin_mode = sin_mode;
@@ -925,6 +947,14 @@ csx_status csx_ctx_last_vjp_launch(csx_ctx* ctx, int* q_tile, int* q_split, int*
   if (q_split) *q_split = ctx->last_vjp[1];
   if (step_split) *step_split = ctx->last_vjp[2];
   if (launches) *launches = ctx->last_vjp[3];
+  return CSX_OK;
+}
+
+csx_status csx_ctx_last_pair_items(csx_ctx* ctx, int* j_chunk, int* items) {
+  if (!ctx) return fail(CSX_INVALID_INPUT, "context is null");
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  if (j_chunk) *j_chunk = ctx->last_items[0];
+  if (items) *items = ctx->last_items[1];
   return CSX_OK;
 }
 
@@ -1021,7 +1051,7 @@ csx_status csx_structure_create(csx_ctx* ctx, size_t n, const double* xyz, const
     j_chunk = ctx->tuning.j_chunk;
   }
   std::shared_ptr<ItemSet> is;
-  CSX_TRY(get_item_set(*ps, j_chunk, &is));
+  CSX_TRY(get_item_set(*ps, pair_j_chunk(j_chunk, *ps, ctx->cu_count), &is));
   *out = s.release();
   return CSX_OK;
 }
@@ -1195,7 +1225,7 @@ csx_status csx_debye_pair_tables_noalloc(csx_structure* s, const double* q, size
   std::shared_ptr<ItemSet> is;
   {
     std::lock_guard<std::mutex> lock(ps->items_mu);
-    auto it = ps->item_sets.find(tune.j_chunk);
+    auto it = ps->item_sets.find(pair_j_chunk(tune.j_chunk, *ps, ctx->cu_count));
     if (it != ps->item_sets.end()) is = it->second;
   }
   if (!is)

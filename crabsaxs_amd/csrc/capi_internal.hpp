@@ -35,7 +35,7 @@ constexpr int kAutoPersist = 4;
 struct Tuning {
   int q_tile = 0;      // q values per sweep of a pair (0: automatic)
   int sin_mode = 2;    // 0 software polynomial, 1 reduced + V_SIN_F32, 2 V_SIN_F32
-  int j_chunk = 512;   // j atoms per work item
+  int j_chunk = 0;     // j atoms per work item (0: automatic, from the point count)
   int persist = -1;    // persistent launch with this many workgroups per CU (0: grid;
                        // -1: automatic, kAutoPersist per CU when tasks are plenty)
   size_t workspace_limit = size_t(2) << 30;  // bytes of adjoint partials per launch
@@ -57,6 +57,7 @@ struct csx_ctx {
   int cu_count = 256;
   size_t lds_per_workgroup = 64 * 1024;  // the device's per-workgroup LDS limit
   int last_launch[3] = {0, 0, 0};  // q tile, sin mode, persistent workgroups
+  int last_items[2] = {0, 0};      // j chunk and work items of the last pair-table launch
   int scratch_status = 0;          // hipError_t of the no-allocation scratch set-up
   int last_vjp[4] = {0, 0, 0, 0};  // q tile, q split, step split, launches
   // Grow-only device buffers kept per stream (see csx::Scratch), and
@@ -395,6 +396,7 @@ csx_status get_point_set(csx_structure* s, const csx_solvent_params* p,
 
 // Work items of a point set for a j chunk, built on first use.
 csx_status get_item_set(PointSet& ps, int j_chunk, std::shared_ptr<ItemSet>* out);
+int pair_j_chunk(int tuned, const PointSet& ps, int cus);
 
 // The cached result of `key` on a point set.  On return the entry's mutex
 // is locked by *lock; entry->ready tells whether it holds the result.  When

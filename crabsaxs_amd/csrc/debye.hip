@@ -232,6 +232,11 @@ __device__ __attribute__((always_inline)) inline void pair_item(const float4* __
   double run = 0.0;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int jend = item.z + item.w;
+  // This wave's first i; a wave with no atoms of the block, or whose atoms
+  // all lie at or past a diagonal tile's last j (every pair masked), skips
+  // the tile's sweep: its pairs would add exact zeros.
+  const int wave_i0 = item.x + 64 * wave;
+  const bool wave_live = 64 * wave < item.y;
   for (int t0 = item.z; t0 < jend; t0 += kBlock) {
     __syncthreads();
     const int j = t0 + threadIdx.x;
@@ -239,10 +244,12 @@ __device__ __attribute__((always_inline)) inline void pair_item(const float4* __
     __syncthreads();
     const int jn = min(kBlock, jend - t0);
     // only a tile overlapping the i block needs the j > i mask
-    if (t0 < item.x + item.y && t0 + kBlock > item.x)
-      tile_sweep<QT, SINMODE, true>(tile, jn, pi, t0 - i, qk, acc);
-    else
+    if (t0 < item.x + item.y && t0 + kBlock > item.x) {
+      if (wave_live && wave_i0 < t0 + jn - 1)
+        tile_sweep<QT, SINMODE, true>(tile, jn, pi, t0 - i, qk, acc);
+    } else if (wave_live) {
       tile_sweep<QT, SINMODE, false>(tile, jn, pi, t0 - i, qk, acc);
+    }
 #pragma unroll
     for (int k = 0; k < QT; ++k) {
       float v = acc[k] * pi.w;

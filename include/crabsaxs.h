@@ -144,9 +144,18 @@ csx_status csx_ctx_reset_timing(csx_ctx *ctx);
  * polynomial after exact reduction in revolutions), 1 (hardware V_SIN_F32
  * after the same reduction) or 2 (V_SIN_F32 on the unreduced argument; used
  * only when |q| r_max / pi <= 250 and otherwise replaced by 1); j_chunk
- * atoms per work item (multiple of 256). */
+ * atoms per work item: 0 (automatic: the largest of 512, 256, 128 and 64
+ * giving at least one full 256 x j_chunk item per CU, so small structures
+ * spread over the device), 64, 128 or a multiple of 256.  The chunk orders
+ * the FP64 sums: a structure's tables are bitwise reproducible for a given
+ * chunk (and, with the automatic chunk, a given CU count). */
 csx_status csx_ctx_set_tuning(csx_ctx *ctx, int q_tile, int sin_mode,
                               int j_chunk);
+
+/* j chunk and work-item count of the most recent FP32 pair-kernel launch on
+ * the context (no reference counterpart; for reports and tests); 0s before
+ * the first launch.  Any output may be NULL. */
+csx_status csx_ctx_last_pair_items(csx_ctx *ctx, int *j_chunk, int *items);
 
 /* Shape of the most recent FP32 pair-kernel launch on the context (after
  * the automatic tile choice and the sine-mode fallback); 0s before the

@@ -171,8 +171,43 @@ def test_deterministic(gpu_ctx):
     assert np.array_equal(a, b)
 
 
+def expected_auto_chunk(n, cus):
+    """The automatic j chunk: the largest of 512, 256, 128 that still gives
+    n^2 / (2 * 256 * chunk) >= one item per CU, else 64."""
+    for c in (512, 256, 128):
+        if n * n / (2.0 * 256 * c) >= cus:
+            return c
+    return 64
+
+
+@pytest.mark.parametrize("n", [1200, 5000, 20000])
+def test_automatic_j_chunk(gpu_ctx, n):
+    """Small structures take short j sweeps so that their work items spread
+    over the CUs; the chunk is a function of the point count and the CU
+    count, the results are bitwise reproducible (also on a fresh handle and
+    with the same chunk set explicitly) and within 1e-5 of FP64."""
+    s = csx.generate_globule(n, 35.0 * (n / 20000) ** (1 / 3), 1.0, seed=9)
+    q = np.linspace(0.0, 1.0, 100)
+    a = csx.debye_intensity(s, q)
+    info = gpu_ctx.last_pair_items()
+    assert info["j_chunk"] == expected_auto_chunk(n, gpu_ctx.cu_count()), info
+    b = csx.debye_intensity(csx.Structure.from_arrays(s.xyz, s.element), q)
+    assert np.array_equal(a, b)
+    gpu_ctx.set_tuning(0, 2, info["j_chunk"])
+    try:
+        c = csx.debye_intensity(csx.Structure.from_arrays(s.xyz, s.element), q)
+    finally:
+        gpu_ctx.set_tuning()
+    assert np.array_equal(a, c)
+    if n <= 5000:
+        want = csx.debye_intensity(s, q, precision=F64)
+        print("auto chunk", n, info, rel(a, want).max(), flush=True)
+        assert rel(a, want).max() < 1e-5
+
+
 @pytest.mark.parametrize("q_tile,sin_mode,j_chunk", [(8, 0, 256), (32, 0, 4096), (16, 1, 2048),
-                                                     (64, 2, 512), (64, 0, 512)])
+                                                     (64, 2, 512), (64, 0, 512), (8, 2, 64),
+                                                     (24, 2, 128)])
 def test_tuning_variants_agree(gpu_ctx, q_tile, sin_mode, j_chunk):
     s = csx.generate_globule(1500, 18.0, 1.0, seed=6)
     q = np.linspace(0, 1.0, 77)
