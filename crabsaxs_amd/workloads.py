@@ -14,6 +14,9 @@
   ensemble_512      512 conformers, atom counts uniform in [2000, 30000],
                     alternately a globule (density of the 20k globule) or a
                     2-4 domain chain; q = linspace(0.005, 0.5, 200)
+  distinct_shapes   five bodies unlike any globule or domain chain, each of
+                    about 6,000 atoms at the 20k globule's density: a rod,
+                    a hollow shell, a disk, a dumbbell and a 3:1 prolate
 """
 from __future__ import annotations
 
@@ -82,3 +85,41 @@ def ensemble_512(m: int = 512, seed: int = 4, lo: int = 2000, hi: int = 30000):
         else:
             conformers.append(generate_domain_chain(int(n), int(2 + k % 3), seed=seed * 100003 + k))
     return conformers, np.linspace(0.005, 0.5, 200)
+
+
+# atoms per cubic angstrom of the 20,000-atom, 35 A globule
+GLOBULE_DENSITY = 20000 / (4.0 / 3.0 * np.pi * 35.0 ** 3)
+
+
+def distinct_shapes(seed: int = 9):
+    """Rod (r 8 A, 270 A), hollow shell (30 / 24 A), disk (r 35 A, 12 A
+    thick), dumbbell (two 19 A spheres 80 A apart) and prolate (semi-axes
+    17, 17, 51 A): points uniform in each body at the globule density, with
+    the element mix of a generated globule."""
+    rng = np.random.default_rng(seed)
+
+    def fill(inside, box, volume):
+        n = int(round(volume * GLOBULE_DENSITY))
+        pts = np.empty((0, 3))
+        lo, hi = np.asarray(box[0], float), np.asarray(box[1], float)
+        while len(pts) < n:
+            c = lo + (hi - lo) * rng.random((2 * n, 3))
+            pts = np.concatenate([pts, c[inside(c)]])
+        pts = pts[:n]
+        el = generate_globule(n, 35.0 * (n / 20000.0) ** (1 / 3), 1.0,
+                              seed=int(rng.integers(1 << 30))).element
+        return Structure.from_arrays(pts, el)
+
+    r2 = lambda c: (c ** 2).sum(axis=1)
+    rod = fill(lambda c: c[:, 0] ** 2 + c[:, 1] ** 2 <= 64.0, ([-8, -8, -135], [8, 8, 135]),
+               np.pi * 64.0 * 270.0)
+    shell = fill(lambda c: (r2(c) <= 900.0) & (r2(c) >= 576.0), ([-30] * 3, [30] * 3),
+                 4.0 / 3.0 * np.pi * (30.0 ** 3 - 24.0 ** 3))
+    disk = fill(lambda c: c[:, 0] ** 2 + c[:, 1] ** 2 <= 35.0 ** 2, ([-35, -35, -6], [35, 35, 6]),
+                np.pi * 35.0 ** 2 * 12.0)
+    off = np.array([0.0, 0.0, 40.0])
+    bell = fill(lambda c: (r2(c - off) <= 361.0) | (r2(c + off) <= 361.0),
+                ([-19, -19, -59], [19, 19, 59]), 2 * 4.0 / 3.0 * np.pi * 19.0 ** 3)
+    prolate = fill(lambda c: (c[:, 0] ** 2 + c[:, 1] ** 2) / 289.0 + c[:, 2] ** 2 / 2601.0 <= 1.0,
+                   ([-17, -17, -51], [17, 17, 51]), 4.0 / 3.0 * np.pi * 17.0 * 17.0 * 51.0)
+    return [rod, shell, disk, bell, prolate]
