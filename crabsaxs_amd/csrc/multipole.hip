@@ -289,40 +289,47 @@ __device__ inline void bessel_column(float xf, int lq, int l_lo, int l_hi, float
 }
 
 // The columns of two points of one q (x0, x1) in lock step: put2(l, v0, v1)
-// with the values bessel_column gives each of them.  Both use the same
-// split order lu = floor(min(x0, x1)): for the larger x, lu is floor(x) or
-// one less, where j_lu(x) is still positive (the first zero of j_l lies
-// beyond l + 2) and the upward recurrence still stable.  Returns false,
-// having stored nothing, where that does not hold (floor(x) differing by
-// more than one, x below 1e-8) or the Miller recurrence would need overflow
-// tests; the caller then takes the two columns one at a time.
+// with j_l(x0) s0 and j_l(x1) s1 for l in [l_lo, min(lq, l_hi)].  Orders
+// above lq are left unwritten: the kernel's sums for them are never read
+// (multipole_sum_types takes l <= lq only, and an MFMA row is one q).  Both
+// columns use the split order lu = floor(min(x0, x1)): for the larger x, lu
+// is floor(x) or one less, where j_lu(x) is still positive (the first zero
+// of j_l lies beyond l + 2) and the upward recurrence still stable.  The
+// recurrences are linear, so the scales ride along from the start values,
+// and the Miller storing pass runs on the normalised FP64 state (values
+// j_l s, so it stores conversions only); that needs the Miller growth bound
+// below 2^900 so that the normalised top value cannot leave the double
+// range.  Returns false, having stored nothing, where floor(x) differs by
+// more than one, x is below 1e-8, or the growth bound is larger; the caller
+// then takes the two columns one at a time (bessel_column).
 template <class Put2>
 __device__ inline bool bessel_pair(float x0f, float x1f, int lq, int l_lo, int l_hi, float s0,
                                    float s1, Put2 put2) {
   const int top = min(lq, l_hi);
-  if (top < l_lo) {
-    for (int l = l_lo; l <= l_hi; ++l) put2(l, 0.f, 0.f);
-    return true;
-  }
+  if (top < l_lo) return true;
   const float xmin = fminf(x0f, x1f);
   if (!(xmin >= 1e-8f)) return false;
   const int fl = int(floorf(xmin));
   if (int(floorf(fmaxf(x0f, x1f))) > fl + 1) return false;
   const int lu = min(top, fl);
   const int S = top + kMillerExtra;
-  if (top > lu && !miller_safe(S, lu, xmin)) return false;
+  if (top > lu &&
+      !(float(S - lu) * (__log2f(float(2 * S + 1)) - __log2f(xmin)) < 900.f))
+    return false;
   float sf0, cf0, sf1, cf1;
   sincosf(x0f, &sf0, &cf0);
   sincosf(x1f, &sf1, &cf1);
   const double ix0 = recip(double(x0f)), ix1 = recip(double(x1f));
   const double a0 = double(sf0) * ix0, a1 = double(sf1) * ix1;
   const double b0 = (a0 - double(cf0)) * ix0, b1 = (a1 - double(cf1)) * ix1;
-  if (l_lo == 0) put2(0, float(a0) * s0, float(a1) * s1);
-  double u0 = a0, u1 = a1;  // j_lu
+  // scaled j_{l-1}, j_l of the upward recurrence at l = 1
+  float jm0 = float(a0) * s0, jc0 = float(b0) * s0, jm1 = float(a1) * s1, jc1 = float(b1) * s1;
+  if (l_lo == 0) put2(0, jm0, jm1);
+  float u0 = jm0, u1 = jm1;  // scaled j_lu
   if (lu >= 1) {
-    if (l_lo <= 1) put2(1, float(b0) * s0, float(b1) * s1);
+    if (l_lo <= 1) put2(1, jc0, jc1);
     const float ixf0 = float(ix0), ixf1 = float(ix1);
-    float jm0 = float(a0), jc0 = float(b0), jm1 = float(a1), jc1 = float(b1), odd = 3.f;
+    float odd = 3.f;
     int l = 1;
     for (; l < lu && l + 1 < l_lo; ++l) {  // orders below the group: not stored
       const float jn0 = (odd * ixf0) * jc0 - jm0;
@@ -333,18 +340,26 @@ __device__ inline bool bessel_pair(float x0f, float x1f, int lq, int l_lo, int l
       jc1 = jn1;
       odd += 2.f;
     }
-    for (; l < lu; ++l) {
+    // two orders per trip, j_{l-1} and j_l trading registers
+    for (; l + 1 < lu; l += 2) {
+      jm0 = (odd * ixf0) * jc0 - jm0;
+      jm1 = (odd * ixf1) * jc1 - jm1;
+      put2(l + 1, jm0, jm1);
+      odd += 2.f;
+      jc0 = (odd * ixf0) * jm0 - jc0;
+      jc1 = (odd * ixf1) * jm1 - jc1;
+      put2(l + 2, jc0, jc1);
+      odd += 2.f;
+    }
+    if (l < lu) {
       const float jn0 = (odd * ixf0) * jc0 - jm0;
       const float jn1 = (odd * ixf1) * jc1 - jm1;
-      jm0 = jc0;
+      put2(l + 1, jn0, jn1);
       jc0 = jn0;
-      jm1 = jc1;
       jc1 = jn1;
-      odd += 2.f;
-      put2(l + 1, jn0 * s0, jn1 * s1);
     }
-    u0 = double(jc0);
-    u1 = double(jc1);
+    u0 = jc0;
+    u1 = jc1;
   }
   if (top > lu) {
     const int lo = max(lu + 1, l_lo);
@@ -378,28 +393,30 @@ __device__ inline bool bessel_pair(float x0f, float x1f, int lq, int l_lo, int l
       f0 = c0 * f0 - p0;
       f1 = c1 * f1 - p1;
     }
-    const double n0 = u0 * recip(f0) * double(s0), n1 = u1 * recip(f1) * double(s1);
-    f0 = F0;
-    p0 = P0;
+    // f_lu > 0 (j_l(x) > 0 for l >= floor(x)), and on this path it is at
+    // most 2^900 times 1e-250, so its reciprocal is a normal double and the
+    // normalised state at top (>= 2^-900 j_lu s) stays normal too.
+    const double n0 = double(u0) * recip(f0), n1 = double(u1) * recip(f1);
+    f0 = F0 * n0;
+    p0 = P0 * n0;
     c0 = C0;
-    f1 = F1;
-    p1 = P1;
+    f1 = F1 * n1;
+    p1 = P1 * n1;
     c1 = C1;
     for (l = top; l - 1 >= lo; l -= 2) {
-      put2(l, float(n0 * f0), float(n1 * f1));
+      put2(l, float(f0), float(f1));
       p0 = c0 * f0 - p0;
       p1 = c1 * f1 - p1;
       c0 -= d0;
       c1 -= d1;
-      put2(l - 1, float(n0 * p0), float(n1 * p1));
+      put2(l - 1, float(p0), float(p1));
       f0 = c0 * p0 - f0;
       f1 = c1 * p1 - f1;
       c0 -= d0;
       c1 -= d1;
     }
-    if (l >= lo) put2(l, float(n0 * f0), float(n1 * f1));
+    if (l >= lo) put2(l, float(f0), float(f1));
   }
-  for (int l = max(top + 1, l_lo); l <= l_hi; ++l) put2(l, 0.f, 0.f);
   return true;
 }
 
@@ -559,6 +576,180 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(WPE)))
   for (int t = 0; t < MAXT; ++t) {
     if (t >= nt) continue;
     const int tile = g0 + wave + 4 * t;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int q = q0 + 4 * mk + r;
+      partial[(size_t(chunk) * nqp + q) * ncol + tile * 16 + mn] = acc[t][r] * unscale;
+    }
+  }
+}
+
+// Workgroup barrier for LDS hand-over only: this wave's LDS accesses are
+// complete (lgkmcnt 0), then s_barrier.  Unlike __syncthreads it does not
+// wait for outstanding global loads, so harmonic tiles requested before it
+// stay in flight.  The empty asm statements keep the compiler from moving
+// memory accesses across it.
+__device__ inline void lds_barrier() {
+  asm volatile("" ::: "memory");
+  __builtin_amdgcn_s_waitcnt(0xc07f);  // vmcnt 63, expcnt 7, lgkmcnt 0
+  __builtin_amdgcn_s_barrier();
+  asm volatile("" ::: "memory");
+}
+
+// The same sums as multipole_coeff with producer and consumer waves: a
+// workgroup of 8 waves, waves 0-3 generating the j_l operands of slab s + 1
+// into one LDS buffer while waves 4-7 run the MFMAs of slab s from the
+// other, one barrier per slab.  FP16 MFMA and VALU work of different waves
+// overlap on gfx950, so generation hides the contraction and the harmonic
+// loads.  Consumer waves request the first R harmonic tiles of their next
+// slab before the barrier and the rest through a ring of R registers.
+template <int MAXT, int WPE, int R>
+__global__ __launch_bounds__(2 * kThreads) __attribute__((amdgpu_waves_per_eu(WPE))) void multipole_coeff_pc(
+    const float* __restrict__ radius, const float* __restrict__ occ,
+    const int4* __restrict__ chunks /* (first slab, end slab, type) */, int nchunk, int nslab,
+    const _Float16* __restrict__ Y, int ntiles, const int* __restrict__ tile_l,
+    const float* __restrict__ qv, const int* __restrict__ qlmax, int nqp, int ngroups,
+    float ascale, float unscale, int xl /* halves per LDS buffer */, float* __restrict__ partial) {
+  static_assert(R >= 1 && R <= MAXT, "ring of harmonic tiles");
+  extern __shared__ float smem[];
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const bool producer = wave < 4;
+  const int w4 = wave & 3;
+  const int nqb = nqp / kQB;
+  const int total = nqb * nchunk * ngroups;
+  const int b = blockIdx.x;
+  const int per = total / 8, extra = total % 8, xcd = b % 8;
+  const int logical = xcd * per + min(xcd, extra) + b / 8;
+  const int qb = logical % nqb;
+  const int chunk = (logical / nqb) % nchunk;
+  const int grp = logical / (nqb * nchunk);
+  const int q0 = qb * kQB;
+  int Lb = 0;
+  for (int k = 0; k < kQB; ++k) Lb = max(Lb, qlmax[q0 + k]);
+  const int g0 = grp * 4 * MAXT;
+  const int g1 = min(g0 + 4 * MAXT, ntiles);
+  const int l_lo = tile_l[g0];
+  if (l_lo > Lb) return;  // every tile of the group is above this block's orders
+  const int l_hi = min(tile_l[g1 - 1], Lb);
+  const int4 ck = chunks[chunk];
+  const int s0 = ck.x, s1 = ck.y;
+  float* PR = smem;
+  float* PO = PR + kChunk;
+  _Float16* X = reinterpret_cast<_Float16*>(PO + kChunk);  // [2][l][hi, lo fragment]
+  for (int k = threadIdx.x; k < (s1 - s0) * kSlab; k += 2 * kThreads) {
+    PR[k] = radius[s0 * kSlab + k];
+    PO[k] = occ[s0 * kSlab + k];
+  }
+
+  const int nsl = s1 - s0;
+  // The two roles run separate loops with the same barriers (one after the
+  // point data, one per slab and one more), so neither keeps the other's
+  // registers live.
+  if (producer) {
+    const int gq = 4 * w4 + (lane >> 4), gpp = lane & 15;
+    const float qg = qv[q0 + gq];
+    const int lg = qlmax[q0 + gq];
+    const int wofs = 4 * lds_slot(gpp >> 2, gq) + (gpp & 3);  // dword in a hi fragment
+    const int hofs = 8 * lds_slot(gpp >> 2, gq) + 2 * (gpp & 3);
+    lds_barrier();  // point data staged
+    for (int it = 0; it <= nsl; ++it) {
+      if (it < nsl) {
+        _Float16* Xb = X + (it & 1) * xl;
+        uint32_t* const Xw = reinterpret_cast<uint32_t*>(Xb) + wofs;
+        _Float16* const Xh = Xb + hofs;
+        const int k = it * kSlab + 2 * gpp;
+        const float x0 = qg * PR[k], x1 = qg * PR[k + 1];
+        const float o0 = PO[k] * ascale, o1 = PO[k + 1] * ascale;
+        const bool paired = bessel_pair(x0, x1, lg, l_lo, l_hi, o0, o1, [&](int l, float v0, float v1) {
+          uint32_t hi, lo;
+          split_pair(v0, v1, hi, lo);
+          uint32_t* p = Xw + (l - l_lo) * kFrag;
+          p[0] = hi;
+          p[kFrag / 2] = lo;
+        });
+        if (!paired) {
+#pragma unroll
+          for (int j = 0; j < 2; ++j)
+            bessel_column(j ? x1 : x0, lg, l_lo, l_hi, j ? o1 : o0, [&](int l, float v) {
+              const _Float16 hi = static_cast<_Float16>(v);
+              const _Float16 lo = static_cast<_Float16>(v - static_cast<float>(hi));
+              _Float16* p = Xh + (l - l_lo) * (2 * kFrag) + j;
+              p[0] = hi;
+              p[kFrag] = lo;
+            });
+        }
+      }
+      lds_barrier();
+    }
+    return;
+  }
+  // consumer role: tiles g0 + w4 + 4 t
+  const int mk = lane >> 4, mn = lane & 15;
+  const int aofs = 8 * lds_slot(mk, mn);
+  int nt = 0;
+  int xoff[MAXT];
+#pragma unroll
+  for (int t = 0; t < MAXT; ++t) {
+    const int tile = g0 + w4 + 4 * t;
+    const int l = tile < g1 ? tile_l[tile] : l_hi + 1;
+    if (l <= l_hi) nt = t + 1;
+    xoff[t] = (min(l, l_hi) - l_lo) * (2 * kFrag);
+  }
+  size_t yrow[MAXT];
+#pragma unroll
+  for (int t = 0; t < MAXT; ++t)
+    yrow[t] = size_t(min(g0 + w4 + 4 * max(min(t, nt - 1), 0), ntiles - 1)) * nslab * (2 * kFrag);
+  v4f acc[MAXT];
+#pragma unroll
+  for (int t = 0; t < MAXT; ++t) acc[t] = v4f{0.f, 0.f, 0.f, 0.f};
+  v8h rh[R], rl[R];
+  {  // the first slab's first tiles
+    const _Float16* Ys = Y + size_t(s0) * (2 * kFrag) + lane * 8;
+#pragma unroll
+    for (int t = 0; t < R; ++t)
+      if (t < nt) {
+        rh[t] = *reinterpret_cast<const v8h*>(Ys + yrow[t]);
+        rl[t] = *reinterpret_cast<const v8h*>(Ys + yrow[t] + kFrag);
+      }
+  }
+  lds_barrier();  // point data staged
+  for (int it = 0; it <= nsl; ++it) {
+    if (it >= 1) {
+      const int s = s0 + it - 1;
+      const _Float16* Xa = X + ((it - 1) & 1) * xl + aofs;
+      const _Float16* Ys = Y + size_t(s) * (2 * kFrag) + lane * 8;
+#pragma unroll
+      for (int t = 0; t < MAXT; ++t) {
+        if (t < nt) {
+          const v8h ah = *reinterpret_cast<const v8h*>(Xa + xoff[t]);
+          const v8h al = *reinterpret_cast<const v8h*>(Xa + xoff[t] + kFrag);
+          acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, rh[t % R], acc[t], 0, 0, 0);
+          acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, rl[t % R], acc[t], 0, 0, 0);
+          acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, rh[t % R], acc[t], 0, 0, 0);
+          if (t + R < MAXT && t + R < nt) {
+            rh[t % R] = *reinterpret_cast<const v8h*>(Ys + yrow[t + R]);
+            rl[t % R] = *reinterpret_cast<const v8h*>(Ys + yrow[t + R] + kFrag);
+          }
+        }
+      }
+      if (s + 1 < s1) {  // the next slab's first tiles, in flight over the barrier
+        const _Float16* Yn = Ys + 2 * kFrag;
+#pragma unroll
+        for (int t = 0; t < R; ++t)
+          if (t < nt) {
+            rh[t] = *reinterpret_cast<const v8h*>(Yn + yrow[t]);
+            rl[t] = *reinterpret_cast<const v8h*>(Yn + yrow[t] + kFrag);
+          }
+      }
+    }
+    lds_barrier();
+  }
+  const int ncol = ntiles * 16;
+#pragma unroll
+  for (int t = 0; t < MAXT; ++t) {
+    if (t >= nt) continue;
+    const int tile = g0 + w4 + 4 * t;
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const int q = q0 + 4 * mk + r;
