@@ -923,8 +923,9 @@ csx_status csx_ctx_set_vjp_tile(csx_ctx* ctx, int q_tile) {
 
 csx_status csx_ctx_set_multipole_group(csx_ctx* ctx, int tiles) {
   if (!ctx) return fail(CSX_INVALID_INPUT, "context is null");
-  if (tiles != 0 && tiles != 32 && tiles != 64 && tiles != 128)
-    return fail(CSX_INVALID_INPUT, "multipole tile group must be 0 (automatic), 32, 64 or 128");
+  if (tiles != 0 && tiles != 32 && tiles != 48 && tiles != 64 && tiles != 128)
+    return fail(CSX_INVALID_INPUT,
+                "multipole tile group must be 0 (automatic), 32, 48, 64 or 128");
   std::lock_guard<std::mutex> lock(ctx->mu);
   ctx->tuning.mp_group_tiles = tiles;
   return CSX_OK;

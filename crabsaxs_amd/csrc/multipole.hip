@@ -423,7 +423,7 @@ __device__ inline bool bessel_pair(float x0f, float x1f, int lq, int l_lo, int l
 // partial[(chunk * nqp + q) * ncol + col]: the sum over the chunk's points
 // (all of one type) of occ j_l Y, times 2^-(kYExp + aexp) where ascale =
 // 2^aexp scales the j_l occ operands.
-// At least WPE waves per SIMD: 2 (256 registers) or 3 (168).
+// At least WPE waves per SIMD: 2 (256 registers), 3 (168) or 4 (128).
 template <int MAXT, int WPE>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(WPE))) void multipole_coeff(
     const float* __restrict__ radius, const float* __restrict__ occ,
@@ -455,13 +455,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(WPE)))
   const int l_hi = min(tile_l[g1 - 1], Lb);
   const int4 ck = chunks[chunk];
   const int s0 = ck.x, s1 = ck.y;
-  float* PR = smem;        // [kChunk] radius and occupancy of the chunk's points
-  float* PO = PR + kChunk;
-  _Float16* X = reinterpret_cast<_Float16*>(PO + kChunk);  // [l][hi, lo fragment]
-  for (int k = threadIdx.x; k < (s1 - s0) * kSlab; k += kThreads) {
-    PR[k] = radius[s0 * kSlab + k];
-    PO[k] = occ[s0 * kSlab + k];
-  }
+  _Float16* X = reinterpret_cast<_Float16*>(smem);  // [l][hi, lo fragment]
 
   v4f acc[MAXT];
 #pragma unroll
@@ -502,25 +496,21 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(WPE)))
   for (int t = 0; t < MAXT; ++t)
     yrow[t] = size_t(min(g0 + wave + 4 * max(min(t, nt - 1), 0), ntiles - 1)) * nslab * (2 * kFrag);
 
-  // Harmonic tiles are read in batches of kB: the first batch of a slab is
-  // requested before its j_l are generated, and batch b + 1 while batch b
-  // runs its MFMAs.
+  // Harmonic tiles are read in batches of kB, batch b + 1 while batch b runs
+  // its MFMAs; nothing is held in registers across the generation.
   constexpr int kB = 2, kNB = MAXT / kB;
   static_assert(MAXT % kB == 0, "whole batches of tiles");
   for (int s = s0; s < s1; ++s) {
     const _Float16* Ys = Y + size_t(s) * (2 * kFrag) + lane * 8;
-    v8h yh[2][kB], yl[2][kB];
-#pragma unroll
-    for (int u = 0; u < kB; ++u) {
-      yh[0][u] = *reinterpret_cast<const v8h*>(Ys + yrow[u]);
-      yl[0][u] = *reinterpret_cast<const v8h*>(Ys + yrow[u] + kFrag);
-    }
+    // radii and occupancies of this thread's two points (read while the
+    // barrier waits)
+    const float2 pr = *reinterpret_cast<const float2*>(radius + s * kSlab + 2 * gpp);
+    const float2 po = *reinterpret_cast<const float2*>(occ + s * kSlab + 2 * gpp);
     __syncthreads();  // the previous slab's operands have been read
     // j_l times the points' occupancy (padding points have occupancy 0)
     {
-      const int k = (s - s0) * kSlab + 2 * gpp;
-      const float x0 = qg * PR[k], x1 = qg * PR[k + 1];
-      const float o0 = PO[k] * ascale, o1 = PO[k + 1] * ascale;
+      const float x0 = qg * pr.x, x1 = qg * pr.y;
+      const float o0 = po.x * ascale, o1 = po.y * ascale;
       const bool paired = bessel_pair(x0, x1, lg, l_lo, l_hi, o0, o1, [&](int l, float v0, float v1) {
         uint32_t hi, lo;
         split_pair(v0, v1, hi, lo);
@@ -541,6 +531,12 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(WPE)))
       }
     }
     __syncthreads();
+    v8h yh[2][kB], yl[2][kB];
+#pragma unroll
+    for (int u = 0; u < kB; ++u) {
+      yh[0][u] = *reinterpret_cast<const v8h*>(Ys + yrow[u]);
+      yl[0][u] = *reinterpret_cast<const v8h*>(Ys + yrow[u] + kFrag);
+    }
 #pragma unroll
     for (int b = 0; b < kNB; ++b) {
       if (b * kB >= nt) break;
@@ -842,15 +838,15 @@ __global__ __launch_bounds__(256) void multipole_contract(const double* __restri
           ((scratch[k] + scratch[NT + k]) + scratch[2 * NT + k]) + scratch[3 * NT + k];
 }
 
-// Accumulator tiles per wave (4 x MAXT tiles per tile group): 8, 16 or 32.
-// Each tile group generates j_l once more, more accumulators lower the
+// Accumulator tiles per wave (4 x MAXT tiles per tile group): 8, 12, 16 or
+// 32.  Each tile group generates j_l once more, more accumulators lower the
 // occupancy, and a group's l range sets its LDS (2 KB per order).  The
 // automatic choice is one group of 64 where 33 to 64 tiles cover the
 // expansion and groups of 32 otherwise: on the 108k-point prolate with
-// sites, 200 q, groups of 32 / 64 / 128 took 0.25 / 0.31 / 0.32 ms at
-// l_max 15, 0.57 / 0.54 / 0.55 ms at 25, 1.06 / 1.08 / 1.38 ms at 40 and
-// 2.40 / 2.54 / 3.89 ms at 64.
-constexpr int kGroupTiles[] = {32, 64, 128};
+// sites, 200 q, groups of 32 / 48 / 64 took 0.22 / 0.25 / 0.25 ms at
+// l_max 15, 0.50 / 0.72 / 0.42 ms at 25, 0.95 / 1.00 / 1.06 ms at 40 and
+// 2.09 / 2.22 / 2.48 ms at 64.
+constexpr int kGroupTiles[] = {32, 48, 64, 128};
 
 }  // namespace
 
@@ -865,7 +861,7 @@ int multipole_tiles_per_group(int ntiles, int tuned) {
 }
 
 size_t multipole_coeff_lds_bytes(int l_range) {
-  return size_t(l_range) * 2 * kFrag * sizeof(_Float16) + 2 * kChunk * sizeof(float);
+  return size_t(l_range) * 2 * kFrag * sizeof(_Float16);
 }
 
 int multipole_max_types() { return 256; }
@@ -891,7 +887,7 @@ hipError_t launch_multipole_coeff(const float* radius, const float* occ, const i
                                   const int* qlmax, int nqp, float* partial, double* coef,
                                   hipStream_t stream, KernelTimer* timer) {
   if (nqp % kQB != 0 || npad % kSlab != 0 || ntiles <= 0 || nchunk <= 0 || ntypes <= 0 ||
-      (tpg != 32 && tpg != 64 && tpg != 128) || aexp < -60 || aexp > 60)
+      (tpg != 32 && tpg != 48 && tpg != 64 && tpg != 128) || aexp < -60 || aexp > 60)
     return hipErrorInvalidValue;
   const int ngroups = (ntiles + tpg - 1) / tpg;
   const long long total = (long long)(nqp / kQB) * nchunk * ngroups;
@@ -907,12 +903,16 @@ hipError_t launch_multipole_coeff(const float* radius, const float* occ, const i
       multipole_coeff<32, 2><<<unsigned(total), kThreads, lds, stream>>>(
           radius, occ, ck, nchunk, nslab, Y, ntiles, tile_l, qv, qlmax, nqp, ngroups, ascale,
           unscale, partial);
+    else if (tpg == 48)
+      multipole_coeff<12, 3><<<unsigned(total), kThreads, lds, stream>>>(
+          radius, occ, ck, nchunk, nslab, Y, ntiles, tile_l, qv, qlmax, nqp, ngroups, ascale,
+          unscale, partial);
     else if (tpg == 64)
-      multipole_coeff<16, 2><<<unsigned(total), kThreads, lds, stream>>>(
+      multipole_coeff<16, 3><<<unsigned(total), kThreads, lds, stream>>>(
           radius, occ, ck, nchunk, nslab, Y, ntiles, tile_l, qv, qlmax, nqp, ngroups, ascale,
           unscale, partial);
     else
-      multipole_coeff<8, 3><<<unsigned(total), kThreads, lds, stream>>>(
+      multipole_coeff<8, 4><<<unsigned(total), kThreads, lds, stream>>>(
           radius, occ, ck, nchunk, nslab, Y, ntiles, tile_l, qv, qlmax, nqp, ngroups, ascale,
           unscale, partial);
   }

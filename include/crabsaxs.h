@@ -194,9 +194,9 @@ csx_status csx_ctx_set_workspace_limit(csx_ctx *ctx, size_t bytes);
 csx_status csx_ctx_set_vjp_tile(csx_ctx *ctx, int q_tile);
 
 /* Column tiles (16 (l, m) columns each) per workgroup of the multipole
- * coefficient kernel: 32, 64 or 128 (8, 16 or 32 accumulator tiles per
- * wave), or 0 (the default: the fewest tile groups, each generating j_l
- * once, with the fewest accumulators that give them).  The sums are the
+ * coefficient kernel: 32, 48, 64 or 128 (8, 12, 16 or 32 accumulator tiles
+ * per wave), or 0 (the default: one group of 64 where 33 to 64 tiles cover
+ * the expansion, groups of 32 otherwise).  The sums are the
  * same for every choice up to FP32 accumulation order.  No reference
  * counterpart. */
 csx_status csx_ctx_set_multipole_group(csx_ctx *ctx, int tiles);
