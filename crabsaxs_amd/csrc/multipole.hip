@@ -580,180 +580,6 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(WPE)))
   }
 }
 
-// Workgroup barrier for LDS hand-over only: this wave's LDS accesses are
-// complete (lgkmcnt 0), then s_barrier.  Unlike __syncthreads it does not
-// wait for outstanding global loads, so harmonic tiles requested before it
-// stay in flight.  The empty asm statements keep the compiler from moving
-// memory accesses across it.
-__device__ inline void lds_barrier() {
-  asm volatile("" ::: "memory");
-  __builtin_amdgcn_s_waitcnt(0xc07f);  // vmcnt 63, expcnt 7, lgkmcnt 0
-  __builtin_amdgcn_s_barrier();
-  asm volatile("" ::: "memory");
-}
-
-// The same sums as multipole_coeff with producer and consumer waves: a
-// workgroup of 8 waves, waves 0-3 generating the j_l operands of slab s + 1
-// into one LDS buffer while waves 4-7 run the MFMAs of slab s from the
-// other, one barrier per slab.  FP16 MFMA and VALU work of different waves
-// overlap on gfx950, so generation hides the contraction and the harmonic
-// loads.  Consumer waves request the first R harmonic tiles of their next
-// slab before the barrier and the rest through a ring of R registers.
-template <int MAXT, int WPE, int R>
-__global__ __launch_bounds__(2 * kThreads) __attribute__((amdgpu_waves_per_eu(WPE))) void multipole_coeff_pc(
-    const float* __restrict__ radius, const float* __restrict__ occ,
-    const int4* __restrict__ chunks /* (first slab, end slab, type) */, int nchunk, int nslab,
-    const _Float16* __restrict__ Y, int ntiles, const int* __restrict__ tile_l,
-    const float* __restrict__ qv, const int* __restrict__ qlmax, int nqp, int ngroups,
-    float ascale, float unscale, int xl /* halves per LDS buffer */, float* __restrict__ partial) {
-  static_assert(R >= 1 && R <= MAXT, "ring of harmonic tiles");
-  extern __shared__ float smem[];
-  const int lane = threadIdx.x & 63;
-  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const bool producer = wave < 4;
-  const int w4 = wave & 3;
-  const int nqb = nqp / kQB;
-  const int total = nqb * nchunk * ngroups;
-  const int b = blockIdx.x;
-  const int per = total / 8, extra = total % 8, xcd = b % 8;
-  const int logical = xcd * per + min(xcd, extra) + b / 8;
-  const int qb = logical % nqb;
-  const int chunk = (logical / nqb) % nchunk;
-  const int grp = logical / (nqb * nchunk);
-  const int q0 = qb * kQB;
-  int Lb = 0;
-  for (int k = 0; k < kQB; ++k) Lb = max(Lb, qlmax[q0 + k]);
-  const int g0 = grp * 4 * MAXT;
-  const int g1 = min(g0 + 4 * MAXT, ntiles);
-  const int l_lo = tile_l[g0];
-  if (l_lo > Lb) return;  // every tile of the group is above this block's orders
-  const int l_hi = min(tile_l[g1 - 1], Lb);
-  const int4 ck = chunks[chunk];
-  const int s0 = ck.x, s1 = ck.y;
-  float* PR = smem;
-  float* PO = PR + kChunk;
-  _Float16* X = reinterpret_cast<_Float16*>(PO + kChunk);  // [2][l][hi, lo fragment]
-  for (int k = threadIdx.x; k < (s1 - s0) * kSlab; k += 2 * kThreads) {
-    PR[k] = radius[s0 * kSlab + k];
-    PO[k] = occ[s0 * kSlab + k];
-  }
-
-  const int nsl = s1 - s0;
-  // The two roles run separate loops with the same barriers (one after the
-  // point data, one per slab and one more), so neither keeps the other's
-  // registers live.
-  if (producer) {
-    const int gq = 4 * w4 + (lane >> 4), gpp = lane & 15;
-    const float qg = qv[q0 + gq];
-    const int lg = qlmax[q0 + gq];
-    const int wofs = 4 * lds_slot(gpp >> 2, gq) + (gpp & 3);  // dword in a hi fragment
-    const int hofs = 8 * lds_slot(gpp >> 2, gq) + 2 * (gpp & 3);
-    lds_barrier();  // point data staged
-    for (int it = 0; it <= nsl; ++it) {
-      if (it < nsl) {
-        _Float16* Xb = X + (it & 1) * xl;
-        uint32_t* const Xw = reinterpret_cast<uint32_t*>(Xb) + wofs;
-        _Float16* const Xh = Xb + hofs;
-        const int k = it * kSlab + 2 * gpp;
-        const float x0 = qg * PR[k], x1 = qg * PR[k + 1];
-        const float o0 = PO[k] * ascale, o1 = PO[k + 1] * ascale;
-        const bool paired = bessel_pair(x0, x1, lg, l_lo, l_hi, o0, o1, [&](int l, float v0, float v1) {
-          uint32_t hi, lo;
-          split_pair(v0, v1, hi, lo);
-          uint32_t* p = Xw + (l - l_lo) * kFrag;
-          p[0] = hi;
-          p[kFrag / 2] = lo;
-        });
-        if (!paired) {
-#pragma unroll
-          for (int j = 0; j < 2; ++j)
-            bessel_column(j ? x1 : x0, lg, l_lo, l_hi, j ? o1 : o0, [&](int l, float v) {
-              const _Float16 hi = static_cast<_Float16>(v);
-              const _Float16 lo = static_cast<_Float16>(v - static_cast<float>(hi));
-              _Float16* p = Xh + (l - l_lo) * (2 * kFrag) + j;
-              p[0] = hi;
-              p[kFrag] = lo;
-            });
-        }
-      }
-      lds_barrier();
-    }
-    return;
-  }
-  // consumer role: tiles g0 + w4 + 4 t
-  const int mk = lane >> 4, mn = lane & 15;
-  const int aofs = 8 * lds_slot(mk, mn);
-  int nt = 0;
-  int xoff[MAXT];
-#pragma unroll
-  for (int t = 0; t < MAXT; ++t) {
-    const int tile = g0 + w4 + 4 * t;
-    const int l = tile < g1 ? tile_l[tile] : l_hi + 1;
-    if (l <= l_hi) nt = t + 1;
-    xoff[t] = (min(l, l_hi) - l_lo) * (2 * kFrag);
-  }
-  size_t yrow[MAXT];
-#pragma unroll
-  for (int t = 0; t < MAXT; ++t)
-    yrow[t] = size_t(min(g0 + w4 + 4 * max(min(t, nt - 1), 0), ntiles - 1)) * nslab * (2 * kFrag);
-  v4f acc[MAXT];
-#pragma unroll
-  for (int t = 0; t < MAXT; ++t) acc[t] = v4f{0.f, 0.f, 0.f, 0.f};
-  v8h rh[R], rl[R];
-  {  // the first slab's first tiles
-    const _Float16* Ys = Y + size_t(s0) * (2 * kFrag) + lane * 8;
-#pragma unroll
-    for (int t = 0; t < R; ++t)
-      if (t < nt) {
-        rh[t] = *reinterpret_cast<const v8h*>(Ys + yrow[t]);
-        rl[t] = *reinterpret_cast<const v8h*>(Ys + yrow[t] + kFrag);
-      }
-  }
-  lds_barrier();  // point data staged
-  for (int it = 0; it <= nsl; ++it) {
-    if (it >= 1) {
-      const int s = s0 + it - 1;
-      const _Float16* Xa = X + ((it - 1) & 1) * xl + aofs;
-      const _Float16* Ys = Y + size_t(s) * (2 * kFrag) + lane * 8;
-#pragma unroll
-      for (int t = 0; t < MAXT; ++t) {
-        if (t < nt) {
-          const v8h ah = *reinterpret_cast<const v8h*>(Xa + xoff[t]);
-          const v8h al = *reinterpret_cast<const v8h*>(Xa + xoff[t] + kFrag);
-          acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, rh[t % R], acc[t], 0, 0, 0);
-          acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, rl[t % R], acc[t], 0, 0, 0);
-          acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, rh[t % R], acc[t], 0, 0, 0);
-          if (t + R < MAXT && t + R < nt) {
-            rh[t % R] = *reinterpret_cast<const v8h*>(Ys + yrow[t + R]);
-            rl[t % R] = *reinterpret_cast<const v8h*>(Ys + yrow[t + R] + kFrag);
-          }
-        }
-      }
-      if (s + 1 < s1) {  // the next slab's first tiles, in flight over the barrier
-        const _Float16* Yn = Ys + 2 * kFrag;
-#pragma unroll
-        for (int t = 0; t < R; ++t)
-          if (t < nt) {
-            rh[t] = *reinterpret_cast<const v8h*>(Yn + yrow[t]);
-            rl[t] = *reinterpret_cast<const v8h*>(Yn + yrow[t] + kFrag);
-          }
-      }
-    }
-    lds_barrier();
-  }
-  const int ncol = ntiles * 16;
-#pragma unroll
-  for (int t = 0; t < MAXT; ++t) {
-    if (t >= nt) continue;
-    const int tile = g0 + w4 + 4 * t;
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int q = q0 + 4 * mk + r;
-      partial[(size_t(chunk) * nqp + q) * ncol + tile * 16 + mn] = acc[t][r] * unscale;
-    }
-  }
-}
-
 // coef[(t * nqp + q) * ncol + col] = FP64 sum, in chunk order, of the
 // partial coefficients of type t's chunks; 0 for the columns with
 // l > qlmax[q] (not written for this q's block).  One thread per (q, column).
