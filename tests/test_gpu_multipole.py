@@ -1,4 +1,4 @@
-"""GPU parity of the multipole path (FP32 MFMA contraction, exact j_l)."""
+"""GPU parity of the multipole path (FP16-split MFMA contraction, exact j_l)."""
 import json
 import math
 from pathlib import Path
@@ -81,6 +81,40 @@ def test_globule_matches_exact_oracle(gpu_ctx, l_max):
     want = O.multipole_intensity(oatoms(s), q, l_max, exact=True)
     print("globule L", l_max, rel(got, want).max(), flush=True)
     assert rel(got, want).max() < 1e-5
+
+
+def test_occupancies_above_one(gpu_ctx):
+    """Occupancies up to 6 scale the FP16 j_l operands by 2^11 instead of
+    2^14 (the largest must stay below the FP16 range); the sums match the
+    exact oracle as before."""
+    s = csx.generate_globule(600, 12.0, 1.0, seed=31)
+    occ = np.random.default_rng(31).uniform(0.05, 6.0, len(s))
+    s = csx.Structure.from_arrays(s.xyz, s.element, occupancy=occ)
+    q = np.linspace(0.005, 0.5, 40)
+    for l_max in (15, 25):
+        got = csx.multipole_intensity(s, q, csx.MultipoleParams(l_max=l_max))
+        want = O.multipole_intensity(oatoms(s), q, l_max, exact=True)
+        print("occupancy up to 6, L", l_max, rel(got, want).max(), flush=True)
+        assert rel(got, want).max() < 1e-5
+
+
+def test_centre_point_and_distant_neighbours(gpu_ctx):
+    """A point exactly at the centre (q r = 0: j_0 = 1, j_l>0 = 0) and
+    neighbours in the radius order that lie far apart (radii 0, 5.2 and 20 A
+    in one slab) take the generator's one-column path; the sums still match
+    the exact oracle, and Debye where the expansion has converged."""
+    cube = [[x, y, z] for x in (-3, 3) for y in (-3, 3) for z in (-3, 3)]
+    octa = [[20, 0, 0], [-20, 0, 0], [0, 20, 0], [0, -20, 0], [0, 0, 20], [0, 0, -20]]
+    xyz = np.array([[0.0, 0.0, 0.0]] + cube + octa)
+    s = csx.Structure.from_arrays(xyz, np.full(len(xyz), 1, dtype=np.uint8))
+    q = np.linspace(0.0, 0.5, 41)
+    for l_max in (15, 40):
+        got = csx.multipole_intensity(s, q, csx.MultipoleParams(l_max=l_max))
+        want = O.multipole_intensity(oatoms(s), q, l_max, exact=True)
+        print("centre point, L", l_max, rel(got, want).max(), flush=True)
+        assert rel(got, want).max() < 1e-5
+    d = csx.debye_intensity(s, q, precision=csx.Precision.FP64_PLAIN)
+    assert rel(got, d).max() < 1e-5  # L = 40 has converged for q R <= 10
 
 
 @pytest.mark.parametrize("model", [csx.HydrationModel.Analytic, csx.HydrationModel.AdaptiveGrid])
