@@ -215,17 +215,18 @@ csx_status get_point_set(csx_structure* s, const csx_solvent_params* p,
 }
 
 // j points per work item: the tuned value, or automatically the largest of
-// 512, 256, 128 and 64 that still gives at least one full 256 x chunk item
-// per CU (n^2 / 2 pairs over 256 x chunk per item); small sets take short j
-// sweeps so that their work spreads over the device (1,200 points on 256 CUs:
-// 64).  The chunk orders the FP64 sums, so a set's tables depend on it; the
-// automatic value depends only on the set's point count and the CU count.
+// 512, 256, 128, 64 and 32 that still gives at least one full 256 x chunk
+// item per CU (n^2 / 2 pairs over 256 x chunk per item); small sets take
+// short j sweeps so that their work spreads over the device (1,200 points on
+// 256 CUs: 32).  The chunk orders the FP64 sums, so a set's tables depend on
+// it; the automatic value depends only on the set's point count and the CU
+// count.
 int pair_j_chunk(int tuned, const PointSet& ps, int cus) {
   if (tuned > 0) return tuned;
   const double n = static_cast<double>(ps.natoms + ps.nsites);
-  for (int c : {512, 256, 128})
+  for (int c : {512, 256, 128, 64})
     if (n * n / (2.0 * 256.0 * c) >= std::max(cus, 1)) return c;
-  return 64;
+  return 32;
 }
 
 csx_status get_item_set(PointSet& ps, int chunk, std::shared_ptr<ItemSet>* out) {
@@ -375,30 +376,34 @@ void cache_drop(PointSet& ps, const std::shared_ptr<CacheEntry>& e) {
 // P[c][b*nq + k] = sum_{i<j in bin b of set c} occ_i occ_j sinc(q_k r_ij).
 // Tiles instantiated for every sine mode, and those of sine mode 2 only.
 static const int kTilesAll[] = {8, 16, 32, 64};
-static const int kTilesHw[] = {8, 16, 24, 32, 40, 48, 50, 56, 63, 64};
+static const int kTilesHw[] = {8, 10, 16, 20, 24, 25, 32, 40, 48, 50, 56, 63, 64};
 
 // q values per pair sweep: the tuned tile if it is instantiated for the sine
 // mode, else the tile QT with the least modelled time
-//   (QT + c) (ceil(nq / QT) sum_j / cus + 2 max_j),
+//   (QT + c) (ceil(items ceil(nq / QT) / cus) sum_j / items + 2 max_j),
 // sum_j and max_j the sum and the largest of the work items' j counts: the
-// sweeps spread over the CUs, plus the tail of the longest item (one
-// workgroup sweeps an item's j atoms in turn, so a structure of a few
-// thousand atoms, with few items, is bound by that length and takes narrow
-// tiles).  The per-pair work (distance, correctly rounded sqrt, reciprocal,
-// LDS read) costs about c = 12 q evaluations (measured: 4.7 VALU cycles per
-// pair.q at QT = 64 against 4 for the q loop alone).  Fitted on MI355X over
-// globules of 100 to 10,000 atoms and 50 to 500 q: the chosen tile's time
-// is within 8 % of the fastest tile's.
-static int choose_q_tile(int tuned, int sin_mode, size_t nq, double sum_j, double max_j, int cus) {
+// tasks (item, q chunk) of the busiest CU, each an average item long, plus
+// the tail of the longest item (one workgroup sweeps an item's j atoms in
+// turn, so a structure of a few thousand atoms, with few items, is bound by
+// that length and takes narrow tiles).  Counting whole tasks per CU matters
+// when there are few of them: 1,200 atoms x 100 q (122 items) take 2 tasks
+// per CU with 25-wide tiles and 3 with 20-wide ones, 20.0 against 22.3 us.
+// The per-pair work (distance, refined square root, LDS read) costs about
+// c = 12 q evaluations (measured: 4.7 VALU cycles per pair.q at QT = 64
+// against 4 for the q loop alone).
+static int choose_q_tile(int tuned, int sin_mode, size_t nq, size_t nitems, double sum_j,
+                         double max_j, int cus) {
   const int* tiles = sin_mode == 2 ? kTilesHw : kTilesAll;
-  const int ntiles = sin_mode == 2 ? 10 : 4;
+  const int ntiles = sin_mode == 2 ? 13 : 4;
   for (int k = 0; k < ntiles; ++k)
     if (tiles[k] == tuned) return tuned;
   int best = tiles[0];
   double best_cost = 1e300;
+  const double items = static_cast<double>(std::max<size_t>(nitems, 1));
   for (int k = 0; k < ntiles; ++k) {
     const double sweeps = static_cast<double>((nq + tiles[k] - 1) / tiles[k]);
-    const double cost = (tiles[k] + 12.0) * (sweeps * sum_j / std::max(cus, 1) + 2.0 * max_j);
+    const double per_cu = std::ceil(items * sweeps / std::max(cus, 1));
+    const double cost = (tiles[k] + 12.0) * (per_cu * sum_j / items + 2.0 * max_j);
     if (cost < best_cost) best = tiles[k], best_cost = cost;
   }
   return best;
@@ -514,7 +519,7 @@ static csx_status pair_tables_batch(Call& call, const std::vector<PointSet*>& se
   if (npts > static_cast<size_t>(INT32_MAX / 2) || nitems > static_cast<size_t>(INT32_MAX / 2) ||
       nbins > static_cast<size_t>(INT32_MAX / 2))
     return fail(CSX_INVALID_INPUT, "batch too large for one launch");
-  const int qt = choose_q_tile(tune.q_tile, sin_mode, nq, sum_j, max_j, cus);
+  const int qt = choose_q_tile(tune.q_tile, sin_mode, nq, nitems, sum_j, max_j, cus);
   const size_t nq_pad = (nq + qt - 1) / qt * qt;
   std::vector<float> qrev(nq_pad, 0.0f);
   for (size_t k = 0; k < nq; ++k) qrev[k] = static_cast<float>(q[k] / (2.0 * kPi));
@@ -882,9 +887,11 @@ csx_status csx_ctx_reset_timing(csx_ctx* ctx) {
 
 csx_status csx_ctx_set_tuning(csx_ctx* ctx, int q_tile, int sin_mode, int j_chunk) {
   if (!ctx) return fail(CSX_INVALID_INPUT, "context is null");
-  if (q_tile < 0 || q_tile > 64 || (q_tile % 8 != 0 && q_tile != 50 && q_tile != 63))
+  if (q_tile < 0 || q_tile > 64 ||
+      (q_tile % 8 != 0 && q_tile != 10 && q_tile != 20 && q_tile != 25 && q_tile != 50 &&
+       q_tile != 63))
     return fail(CSX_INVALID_INPUT,
-                "q_tile must be 0 (automatic), a multiple of 8 up to 64, 50 or 63");
+                "q_tile must be 0 (automatic), a multiple of 8 up to 64, 10, 20, 25, 50 or 63");
   if (sin_mode < 0 || sin_mode > 2) return fail(CSX_INVALID_INPUT, "sin_mode must be 0, 1 or 2");
   if (!(j_chunk == 0 || j_chunk == 32 || j_chunk == 64 || j_chunk == 128 ||
         (j_chunk >= 256 && j_chunk % 256 == 0 && j_chunk <= (1 << 20))))
@@ -1240,7 +1247,8 @@ csx_status csx_debye_pair_tables_noalloc(csx_structure* s, const double* q, size
     max_j = std::max(max_j, static_cast<double>(is->h_items[k]));
   }
   const int cus = ctx->cu_count;
-  const int qt = choose_q_tile(tune.q_tile, sin_mode, nq, sum_j, max_j, cus);
+  const int qt = choose_q_tile(tune.q_tile, sin_mode, nq, static_cast<size_t>(is->nitems), sum_j,
+                               max_j, cus);
   const size_t nq_pad = (nq + qt - 1) / qt * qt;
   const size_t nitems = static_cast<size_t>(is->nitems), nbins = static_cast<size_t>(ps->nbins);
   const int persist_wgs = pair_persist_wgs(tune.persist, nitems, nq_pad, qt, cus);

@@ -460,6 +460,7 @@ hipError_t launch_debye_pairs(const float4* pts, const int4* items, int nitems, 
       CSX_PAIRS_CASE(64, 0) CSX_PAIRS_CASE(64, 1) CSX_PAIRS_CASE(64, 2)
       CSX_PAIRS_CASE(24, 2) CSX_PAIRS_CASE(40, 2) CSX_PAIRS_CASE(48, 2) CSX_PAIRS_CASE(50, 2)
       CSX_PAIRS_CASE(56, 2) CSX_PAIRS_CASE(63, 2)
+      CSX_PAIRS_CASE(10, 2) CSX_PAIRS_CASE(20, 2) CSX_PAIRS_CASE(25, 2)
       else return hipErrorInvalidValue;
 #undef CSX_PAIRS_CASE
     }

@@ -79,21 +79,21 @@ def test_every_q_tile_gives_identical_sums(gpu_ctx):
     the tile, so every instantiated tile (and the automatic choice) agrees
     bitwise.  The 2,300-atom globule has 41 work items (four element types,
     256-atom i blocks, 512-atom j chunks): its time is set by the longest
-    item, so the automatic choice is the narrowest tile, launched one
-    workgroup per task (1,025 tasks, fewer than two per persistent
-    workgroup)."""
+    item, so the automatic choice is a narrow tile (10 wide: 200 q in 20
+    sweeps, the least modelled time), launched one workgroup per task (820
+    tasks, fewer than two per persistent workgroup)."""
     s = csx.generate_globule(2300, 15.0, 1.0, seed=12)
     q = np.linspace(0.005, 0.5, 200)
     try:
         gpu_ctx.set_tuning(64, 2, 512)
         base = csx.debye_intensity(s, q)
-        for qt in (0, 8, 16, 24, 32, 40, 48, 50, 56, 63):
+        for qt in (0, 8, 10, 16, 20, 24, 25, 32, 40, 48, 50, 56, 63):
             gpu_ctx.set_tuning(qt, 2, 512)
             assert np.array_equal(csx.debye_intensity(s, q), base), qt
         assert gpu_ctx.last_launch()["q_tile"] == 63
         gpu_ctx.set_tuning(0, 2, 512)
         csx.debye_intensity(s, q)
-        assert gpu_ctx.last_launch() == {"q_tile": 8, "sin_mode": 2, "persistent_workgroups": 0}
+        assert gpu_ctx.last_launch() == {"q_tile": 10, "sin_mode": 2, "persistent_workgroups": 0}
         gpu_ctx.set_tuning(40, 1, 512)  # no 40-wide tile for the reduced sine: automatic
         csx.debye_intensity(s, q)
         assert gpu_ctx.last_launch()["q_tile"] in (8, 16, 32, 64)

@@ -172,15 +172,15 @@ def test_deterministic(gpu_ctx):
 
 
 def expected_auto_chunk(n, cus):
-    """The automatic j chunk: the largest of 512, 256, 128 that still gives
-    n^2 / (2 * 256 * chunk) >= one item per CU, else 64."""
-    for c in (512, 256, 128):
+    """The automatic j chunk: the largest of 512, 256, 128, 64 that still
+    gives n^2 / (2 * 256 * chunk) >= one item per CU, else 32."""
+    for c in (512, 256, 128, 64):
         if n * n / (2.0 * 256 * c) >= cus:
             return c
-    return 64
+    return 32
 
 
-@pytest.mark.parametrize("n", [1200, 5000, 20000])
+@pytest.mark.parametrize("n", [1200, 2300, 5000, 20000])
 def test_automatic_j_chunk(gpu_ctx, n):
     """Small structures take short j sweeps so that their work items spread
     over the CUs; the chunk is a function of the point count and the CU
@@ -207,7 +207,8 @@ def test_automatic_j_chunk(gpu_ctx, n):
 
 @pytest.mark.parametrize("q_tile,sin_mode,j_chunk", [(8, 0, 256), (32, 0, 4096), (16, 1, 2048),
                                                      (64, 2, 512), (64, 0, 512), (8, 2, 64),
-                                                     (24, 2, 128)])
+                                                     (24, 2, 128), (10, 2, 32), (20, 2, 64),
+                                                     (25, 2, 256)])
 def test_tuning_variants_agree(gpu_ctx, q_tile, sin_mode, j_chunk):
     s = csx.generate_globule(1500, 18.0, 1.0, seed=6)
     q = np.linspace(0, 1.0, 77)
