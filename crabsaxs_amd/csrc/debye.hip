@@ -224,8 +224,12 @@ __device__ __attribute__((always_inline)) inline void pair_item(const float4* __
 #pragma unroll
   for (int k = 0; k < QT; ++k) qk[k] = qrev[qc + k];
   // Per lane: FP32 tile sums acc[k] (<= 256 pairs).  After each tile they are
-  // weighted by occ_i, summed over the wave (fixed butterfly) and lane k adds
-  // the wave's tile sum for q_k to its FP64 running sum `run`.
+  // weighted by occ_i, summed over the wave (fixed butterfly) and lane
+  // kStride * k adds the wave's tile sum for q_k to its FP64 running sum
+  // `run` (kPow: QT rounded up to a power of two; kStride = 64 / kPow).
+  constexpr int kPow = QT <= 8 ? 8 : QT <= 16 ? 16 : QT <= 32 ? 32 : 64;
+  constexpr int kStride = 64 / kPow;
+  float red[kPow];
   float acc[QT];
 #pragma unroll
   for (int k = 0; k < QT; ++k) acc[k] = 0.f;
@@ -250,16 +254,33 @@ __device__ __attribute__((always_inline)) inline void pair_item(const float4* __
     } else if (wave_live) {
       tile_sweep<QT, SINMODE, false>(tile, jn, pi, t0 - i, qk, acc);
     }
+    // Wave sums of the QT tile sums: a transposing butterfly.  Each step
+    // exchanges half of a lane's values with its partner at xor offset o and
+    // keeps the half that the lane's bit o selects, so every value meets the
+    // same partners, in the same order (offsets 32 .. 1), as a full xor
+    // butterfly per q, at about one shuffle per q instead of six; lane
+    // kStride * k ends with q_k's sum.
 #pragma unroll
-    for (int k = 0; k < QT; ++k) {
-      float v = acc[k] * pi.w;
-#pragma unroll
-      for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-      if (lane == k) run += double(v);
-      acc[k] = 0.f;
+    for (int k = 0; k < kPow; ++k) {
+      red[k] = k < QT ? acc[k] * pi.w : 0.f;
+      if (k < QT) acc[k] = 0.f;
     }
+#pragma unroll
+    for (int o = 32, h = kPow / 2; o >= kStride; o >>= 1, h >>= 1) {
+      const bool up = (lane & o) != 0;
+#pragma unroll
+      for (int k = 0; k < h; ++k) {
+        const float keep = up ? red[h + k] : red[k];
+        const float send = up ? red[k] : red[h + k];
+        red[k] = keep + __shfl_xor(send, o, 64);
+      }
+    }
+    float v = red[0];
+#pragma unroll
+    for (int o = kStride / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if ((lane & (kStride - 1)) == 0 && lane / kStride < QT) run += double(v);
   }
-  if (lane < QT) scratch[wave * QT + lane] = run;
+  if ((lane & (kStride - 1)) == 0 && lane / kStride < QT) scratch[wave * QT + lane / kStride] = run;
   __syncthreads();
   if (threadIdx.x < QT) {
     const int k = threadIdx.x;
