@@ -358,9 +358,11 @@ class Context:
         _check(_lib.load().csx_ctx_kernel_time(self.ptr, name.encode(), C.byref(n), C.byref(ms)))
         return int(n.value), float(ms.value)
 
-    def set_tuning(self, q_tile: int = 0, sin_mode: int = 2, j_chunk: int = 0) -> None:
+    def set_tuning(self, q_tile: int = 0, sin_mode: int = 3, j_chunk: int = 0) -> None:
         """Pair-kernel launch shape; q_tile 0 picks the tile per call and
-        j_chunk 0 the j chunk per structure."""
+        j_chunk 0 the j chunk per structure; sin_mode 3 (the default) runs the
+        sine recurrence on uniform q grids where it models faster, 2 always
+        evaluates the hardware sine."""
         _check(_lib.load().csx_ctx_set_tuning(self.ptr, q_tile, sin_mode, j_chunk))
 
     def set_cache_limit(self, nbytes: int = DEFAULT_CACHE_LIMIT) -> None:

@@ -24,8 +24,8 @@ CACHE = Path(os.environ.get("CRABSAXS_TUNING_CACHE",
                             Path.home() / ".cache" / "crabsaxs_amd" / "tuning.json"))
 
 DEFAULT_CANDIDATES = {
-    "q_tile": (0, 32, 64),
-    "sin_mode": (2,),
+    "q_tile": (0, 64, 128),
+    "sin_mode": (3,),
     "j_chunk": (256, 512, 1024),
     "persist": (0, 4, 8),
 }

@@ -13,6 +13,7 @@ import shutil
 import subprocess
 import sys
 import tempfile
+from concurrent.futures import ThreadPoolExecutor
 from pathlib import Path
 
 PKG = Path(__file__).resolve().parent
@@ -60,9 +61,12 @@ def _run(cmd: list[str]) -> None:
         print(out, file=sys.stderr)
 
 
-def build(verbose: bool = False) -> Path:
+def build(verbose: bool = False, jobs: int = 0) -> Path:
+    """Compile every source (in parallel, `jobs` at a time; 0: one per CPU,
+    at most 8) and link the library."""
+    jobs = jobs or max(1, min(8, os.cpu_count() or 1))
     with tempfile.TemporaryDirectory(prefix="csx_build_") as tmp:
-        objs = []
+        objs, cmds = [], []
         for name, kind in SOURCES:
             src = CSRC / name
             obj = Path(tmp) / (src.name + ".o")
@@ -75,8 +79,11 @@ def build(verbose: bool = False) -> Path:
                 cmd = [HIPCC, "-x", "c++", *COMMON, *HOST_ONLY, "-c", str(src), "-o", str(obj)]
             if verbose:
                 print(" ".join(cmd))
-            _run(cmd)
+            cmds.append(cmd)
             objs.append(str(obj))
+        with ThreadPoolExecutor(max_workers=jobs) as pool:
+            for f in [pool.submit(_run, c) for c in cmds]:
+                f.result()
         tmp_lib = Path(tmp) / LIB.name
         _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, *LINK, "-o", str(tmp_lib)])
         shutil.copyfile(tmp_lib, str(LIB) + ".tmp")

@@ -371,69 +371,144 @@ void cache_drop(PointSet& ps, const std::shared_ptr<CacheEntry>& e) {
 
 // --------------------------------------------------------- pair tables --
 
-// Pair tables of one or more point sets from one launch of the pair kernel
-// (a batch concatenates the sets' points, work items and bins):
-// P[c][b*nq + k] = sum_{i<j in bin b of set c} occ_i occ_j sinc(q_k r_ij).
-// Tiles instantiated for every sine mode, and those of sine mode 2 only.
+// Tiles instantiated for every sine mode, those of sine mode 2 only, and
+// those of the recurrence (mode 3).
 static const int kTilesAll[] = {8, 16, 32, 64};
 static const int kTilesHw[] = {8, 10, 16, 20, 24, 25, 32, 40, 48, 50, 56, 63, 64};
+static const int kTilesRec[] = {16, 25, 32, 50, 64, 100, 125, 128};
 
-// q values per pair sweep: the tuned tile if it is instantiated for the sine
-// mode, else the tile QT with the least modelled time
-//   (QT + c) (ceil(items ceil(nq / QT) / cus) sum_j / items + 2 max_j),
+static bool tile_listed(const int* tiles, int n, int qt) {
+  for (int k = 0; k < n; ++k)
+    if (tiles[k] == qt) return true;
+  return false;
+}
+
+// Modelled cost of a pair sweep over QT q values: per pair, the q loop plus
+// the per-pair work (distance, refined square root, LDS read; for the
+// recurrence also its start: two hardware sines and the step constant's
+// series), in units of one q step of the mode.  Fitted on MI355X over
+// globules of 300-20,000 atoms at 100, 200 and 500 q (kernel times of every
+// tile): the recurrence's per-pair work is about 8 steps and its longest
+// item weighs half a CU's share; the direct sweep's 12 evaluations and two
+// CU shares.
+static double sweep_cost(int sin_mode, int qt) { return sin_mode == 3 ? qt + 8.0 : qt + 12.0; }
+static double tail_weight(int sin_mode) { return sin_mode == 3 ? 0.5 : 2.0; }
+
+// The tile of a sweep mode with the least modelled time
+//   sweep_cost(QT) (ceil(items ceil(nq / QT) / cus) sum_j / items + t max_j),
 // sum_j and max_j the sum and the largest of the work items' j counts: the
 // tasks (item, q chunk) of the busiest CU, each an average item long, plus
 // the tail of the longest item (one workgroup sweeps an item's j atoms in
 // turn, so a structure of a few thousand atoms, with few items, is bound by
-// that length and takes narrow tiles).  Counting whole tasks per CU matters
-// when there are few of them: 1,200 atoms x 100 q (122 items) take 2 tasks
-// per CU with 25-wide tiles and 3 with 20-wide ones, 20.0 against 22.3 us.
-// The per-pair work (distance, refined square root, LDS read) costs about
-// c = 12 q evaluations (measured: 4.7 VALU cycles per pair.q at QT = 64
-// against 4 for the q loop alone).
+// that length and takes narrow tiles; t = tail_weight).  Counting whole
+// tasks per CU matters when there are few of them: with the direct sweep,
+// 1,200 atoms x 100 q (122 items) take 2 tasks per CU with 25-wide tiles
+// and 3 with 20-wide ones, 20.0 against 22.3 us.  Over the fitted sizes the
+// recurrence's pick is within 2.4 % of its best tile on average (10 % at
+// most).
+// The tuned tile wins where it is instantiated for the mode.
 static int choose_q_tile(int tuned, int sin_mode, size_t nq, size_t nitems, double sum_j,
                          double max_j, int cus) {
-  const int* tiles = sin_mode == 2 ? kTilesHw : kTilesAll;
-  const int ntiles = sin_mode == 2 ? 13 : 4;
-  for (int k = 0; k < ntiles; ++k)
-    if (tiles[k] == tuned) return tuned;
+  const int* tiles = sin_mode == 3 ? kTilesRec : sin_mode == 2 ? kTilesHw : kTilesAll;
+  const int ntiles = sin_mode == 3 ? 8 : sin_mode == 2 ? 13 : 4;
   int best = tiles[0];
   double best_cost = 1e300;
   const double items = static_cast<double>(std::max<size_t>(nitems, 1));
   for (int k = 0; k < ntiles; ++k) {
     const double sweeps = static_cast<double>((nq + tiles[k] - 1) / tiles[k]);
     const double per_cu = std::ceil(items * sweeps / std::max(cus, 1));
-    const double cost = (tiles[k] + 12.0) * (per_cu * sum_j / items + 2.0 * max_j);
+    const double cost = sweep_cost(sin_mode, tiles[k]) *
+                        (per_cu * sum_j / items + tail_weight(sin_mode) * max_j);
+    if (tiles[k] == tuned) return tuned;
     if (cost < best_cost) best = tiles[k], best_cost = cost;
   }
   return best;
 }
 
-// Key of a point set's pair tables: they depend on the q grid, the sine mode
-// and the j chunk (which orders the FP64 sums); every q tile and launch form
-// gives the same bits.
-static std::string table_key(int sin_mode, int j_chunk, const double* q, size_t nq) {
+// The step of a uniform q grid in revolutions (q_k = q_0 + k step within 1e-9
+// of the grid's largest |q|), as FP32, or 0 where the grid is not uniform,
+// has fewer than 3 points, or a step whose phase advance over the largest
+// pair distance (2 r_max) exceeds 1/4 revolution (the recurrence's step
+// constant comes from a series valid up to there).  The recurrence then
+// evaluates q_k = fp32(q_t / 2 pi) + (k - t) step per tile starting at t,
+// within about 1e-7 of q_k relative: as close as the FP32 rounding of a
+// direct sweep's q.
+static float uniform_q_step(const double* q, size_t nq, double rmax) {
+  if (nq < 3) return 0.f;
+  const double step = (q[nq - 1] - q[0]) / static_cast<double>(nq - 1);
+  double qmax = 0.0;
+  for (size_t k = 0; k < nq; ++k) qmax = std::max(qmax, std::fabs(q[k]));
+  if (!(step != 0.0) || !std::isfinite(step)) return 0.f;
+  for (size_t k = 0; k < nq; ++k)
+    if (!(std::fabs(q[k] - (q[0] + static_cast<double>(k) * step)) <= 1e-9 * qmax)) return 0.f;
+  const float qstep = static_cast<float>(step / (2.0 * kPi));
+  if (!(std::fabs(static_cast<double>(qstep)) * 2.0 * rmax <= 0.25)) return 0.f;
+  return qstep;
+}
+
+// Key of a point set's pair tables: they depend on the q grid, the sweep
+// mode, the j chunk (which orders the FP64 sums) and, for the recurrence,
+// the q tile (where it restarts); a direct sweep gives the same bits for
+// every q tile and both launch forms.
+static std::string table_key(int sin_mode, int qt, int j_chunk, const double* q, size_t nq) {
   std::string k = "pairs:" + std::to_string(sin_mode) + ":" + std::to_string(j_chunk) + ":";
+  if (sin_mode == 3) k += "qt" + std::to_string(qt) + ":";
   k.append(reinterpret_cast<const char*>(q), nq * sizeof(double));
   return k;
 }
 
-// Pair tables of one or more point sets.  Tables already cached on a set
-// for the same q grid and arithmetic are reused; the others come from one
-// launch of the pair kernel (a batch concatenates the sets' points, work
-// items and bins) and are cached:
-// P[c][b*nq + k] = sum_{i<j in bin b of set c} occ_i occ_j sinc(q_k r_ij).
-// One set waits for a concurrent call computing the same tables; in a batch,
-// tables that another call is still computing are computed again.
-// The pair kernel's sine mode for a q grid and the largest point radius:
-// the tuned mode, except that the unreduced hardware sine (mode 2) takes
-// |q r / 2 pi| <= 256 revolutions, so larger phases take mode 1.
+// The sweep's sine mode for a q grid and the largest point radius, before
+// the recurrence is considered: the tuned mode, except that the unreduced
+// hardware sine (modes 2 and 3) takes |q r / 2 pi| <= 256 revolutions, so
+// larger phases take mode 1.
 static int pair_sin_mode(int tuned, const double* q, size_t nq, double rmax) {
   double umax = 0.0;
   for (size_t k = 0; k < nq; ++k)
     umax = std::max(umax, std::fabs(static_cast<double>(static_cast<float>(q[k] / (2.0 * kPi)))) *
                               2.0 * rmax);
-  return tuned == 2 && !(umax <= 250.0) ? 1 : tuned;
+  return tuned >= 2 && !(umax <= 250.0) ? 1 : tuned;
+}
+
+// Sweep mode, q tile and recurrence step of a pair-table launch for one
+// point set.  Mode 3 (the default) takes the recurrence wherever the grid
+// allows it (uniform_q_step), and the direct hardware sine (mode 2)
+// otherwise or when the tuned tile is one that only the direct sweep has
+// (on MI355X the recurrence's best tile beats the direct sweep's from 1,200
+// atoms up by 1.2-1.9x; at 300 atoms the direct sweep is 2 us faster).  The
+// plan follows from the q grid, the tuning and the set's own work items, so
+// a set gets the same plan, and the same bits, alone and in a batch.
+struct PairPlan {
+  int sin_mode = 2;
+  int qt = 64;
+  float qstep = 0.f;
+};
+
+static PairPlan pair_plan(const Tuning& tune, const double* q, size_t nq, double rmax,
+                          size_t nitems, double sum_j, double max_j, int cus) {
+  PairPlan plan;
+  plan.sin_mode = pair_sin_mode(tune.sin_mode, q, nq, rmax);
+  if (plan.sin_mode == 3) {
+    const float step = uniform_q_step(q, nq, rmax);
+    const bool direct_only = tune.q_tile != 0 && !tile_listed(kTilesRec, 8, tune.q_tile) &&
+                             tile_listed(kTilesHw, 13, tune.q_tile);
+    if (step != 0.f && !direct_only) {
+      plan.qt = choose_q_tile(tune.q_tile, 3, nq, nitems, sum_j, max_j, cus);
+      plan.qstep = step;
+      return plan;
+    }
+    plan.sin_mode = 2;
+  }
+  plan.qt = choose_q_tile(tune.q_tile, plan.sin_mode, nq, nitems, sum_j, max_j, cus);
+  return plan;
+}
+
+// The q (revolutions) of table column k that the sweep evaluated (the
+// device's finish_tables computes the same).
+static double plan_q_rev(const PairPlan& plan, const std::vector<float>& qrev, size_t k) {
+  if (plan.qstep == 0.f) return static_cast<double>(qrev[k]);
+  const size_t kt = k % static_cast<size_t>(plan.qt);
+  return static_cast<double>(qrev[k - kt]) +
+         static_cast<double>(kt) * static_cast<double>(plan.qstep);
 }
 
 // Automatic launch form: the persistent grid pays (4-8 % on large calls)
@@ -446,80 +521,53 @@ static int pair_persist_wgs(int tuned, size_t nitems, size_t nq_pad, int qt, int
                                                           : 0;
 }
 
-// `overlap`, when given, runs on the host while the pair kernel runs (after
-// the launch, before the blocking download of the tables), or at once when
-// nothing is launched.
-static csx_status pair_tables_batch(Call& call, const std::vector<PointSet*>& sets,
-                                    const double* q, size_t nq,
-                                    std::vector<std::vector<double>>& P,
-                                    const std::function<void()>& overlap = nullptr) {
-  Tuning tune;
-  {
-    std::lock_guard<std::mutex> lock(call.ctx()->mu);
-    tune = call.ctx()->tuning;
+// Pair tables of one or more point sets.  Tables already cached on a set
+// for the same q grid and arithmetic are reused; the others come from one
+// launch of the pair kernel (a batch concatenates the sets' points, work
+// items and bins) and are cached:
+// P[c][b*nq + k] = sum_{i<j in bin b of set c} occ_i occ_j sinc(q_k r_ij).
+// One set waits for a concurrent call computing the same tables; in a batch,
+// tables that another call is still computing are computed again.
+// Sum and largest of the work items' j counts of some item sets.
+static void item_stats(const std::vector<std::shared_ptr<ItemSet>>& isets, size_t* nitems,
+                       double* sum_j, double* max_j) {
+  *nitems = 0;
+  *sum_j = 0.0;
+  *max_j = 0.0;
+  for (const auto& is : isets) {
+    *nitems += static_cast<size_t>(is->nitems);
+    for (size_t k = 3; k < is->h_items.size(); k += 4) {
+      *sum_j += is->h_items[k];
+      *max_j = std::max(*max_j, static_cast<double>(is->h_items[k]));
+    }
   }
-  double rmax = 0.0;
-  for (const PointSet* ps : sets) rmax = std::max(rmax, ps->max_radius);
-  const int sin_mode = pair_sin_mode(tune.sin_mode, q, nq, rmax);
+}
+
+// One launch of the pair kernel over the sets `idx` (their points, work
+// items and bins concatenated) with one plan; fills P[c] for c in idx.
+// `overlap`, when given, runs on the host while the kernel runs.
+static csx_status pair_tables_launch(Call& call, const std::vector<PointSet*>& sets,
+                                     const std::vector<size_t>& idx,
+                                     const std::vector<std::shared_ptr<ItemSet>>& all_items,
+                                     const std::vector<int>& chunk, const PairPlan& plan,
+                                     const Tuning& tune, const double* q, size_t nq,
+                                     std::vector<std::vector<double>>& P,
+                                     const std::function<void()>& overlap) {
   const int cus = call.ctx()->cu_count;
-  std::vector<int> chunk(sets.size());
-  std::vector<std::string> keys(sets.size());
-  for (size_t c = 0; c < sets.size(); ++c) {
-    chunk[c] = pair_j_chunk(tune.j_chunk, *sets[c], cus);
-    keys[c] = table_key(sin_mode, chunk[c], q, nq);
-  }
-
-  P.assign(sets.size(), std::vector<double>());
-  std::vector<size_t> todo;
-  std::shared_ptr<CacheEntry> entry;  // the one set's entry, locked while computed
-  std::unique_lock<std::mutex> entry_lock;
-  if (sets.size() == 1) {
-    entry = cache_acquire(call.ctx(), *sets[0], keys[0], &entry_lock);
-    if (entry->ready) {
-      P[0] = entry->tables;
-      if (overlap) overlap();
-      return CSX_OK;
-    }
-    todo.push_back(0);
-  } else {
-    for (size_t c = 0; c < sets.size(); ++c) {
-      std::shared_ptr<CacheEntry> e = cache_find(call.ctx(), *sets[c], keys[c]);
-      if (e) P[c] = e->tables;
-      else todo.push_back(c);
-    }
-    if (todo.empty()) {
-      if (overlap) overlap();
-      return CSX_OK;
-    }
-  }
-  // A failed computation leaves no half-made entry behind.
-  struct Dropper {
-    PointSet* ps;
-    std::shared_ptr<CacheEntry>* e;
-    ~Dropper() {
-      if (*e && !(*e)->ready) cache_drop(*ps, *e);
-    }
-  } dropper{sets[0], &entry};
-
+  const int sin_mode = plan.sin_mode, qt = plan.qt;
+  std::vector<size_t> todo = idx;
   std::vector<std::shared_ptr<ItemSet>> isets(todo.size());
-  double sum_j = 0.0, max_j = 0.0;
   size_t npts = 0, nitems = 0, nbins = 0;
   for (size_t t = 0; t < todo.size(); ++t) {
     PointSet* ps = sets[todo[t]];
-    CSX_TRY(get_item_set(*ps, chunk[todo[t]], &isets[t]));
+    isets[t] = all_items[todo[t]];
     npts += static_cast<size_t>(ps->natoms + ps->nsites);
     nitems += static_cast<size_t>(isets[t]->nitems);
     nbins += static_cast<size_t>(ps->nbins);
-    const std::vector<int>& hi = isets[t]->h_items;
-    for (size_t k = 3; k < hi.size(); k += 4) {
-      sum_j += hi[k];
-      max_j = std::max(max_j, static_cast<double>(hi[k]));
-    }
   }
   if (npts > static_cast<size_t>(INT32_MAX / 2) || nitems > static_cast<size_t>(INT32_MAX / 2) ||
       nbins > static_cast<size_t>(INT32_MAX / 2))
     return fail(CSX_INVALID_INPUT, "batch too large for one launch");
-  const int qt = choose_q_tile(tune.q_tile, sin_mode, nq, nitems, sum_j, max_j, cus);
   const size_t nq_pad = (nq + qt - 1) / qt * qt;
   std::vector<float> qrev(nq_pad, 0.0f);
   for (size_t k = 0; k < nq; ++k) qrev[k] = static_cast<float>(q[k] / (2.0 * kPi));
@@ -593,9 +641,9 @@ static csx_status pair_tables_batch(Call& call, const std::vector<PointSet*>& se
     call.ctx()->last_items[1] = static_cast<int>(nitems);
   }
   CSX_HIP(launch_debye_pairs(pts, items, static_cast<int>(nitems), bins, static_cast<int>(nbins),
-                             d_q.as<float>(), static_cast<int>(nq_pad), qt, sin_mode, persist_wgs,
-                             d_counter.as<int>(), d_partial.as<double>(), d_out.as<double>(), st,
-                             call.timer()),
+                             d_q.as<float>(), plan.qstep, static_cast<int>(nq_pad), qt, sin_mode,
+                             persist_wgs, d_counter.as<int>(), d_partial.as<double>(),
+                             d_out.as<double>(), st, call.timer()),
           "debye_pairs_f32");
   CSX_HIP(hipMemcpyAsync(h_out, d_out.as<void>(), nbins * nq_pad * sizeof(double),
                          hipMemcpyDeviceToHost, st),
@@ -620,7 +668,7 @@ static csx_status pair_tables_batch(Call& call, const std::vector<PointSet*>& se
     for (int a = 0; a < TT; ++a)
       for (int u = a; u < TT; ++u, ++b)
         for (size_t k = 0; k < nq; ++k) {
-          const double qe = 2.0 * kPi * static_cast<double>(qrev[k]);
+          const double qe = 2.0 * kPi * plan_q_rev(plan, qrev, k);
           if (qe == 0.0) {
             Pc[b * nq + k] = a == u ? 0.5 * (ps.wsum[a] * ps.wsum[a] - ps.self[a])
                                     : ps.wsum[a] * ps.wsum[u];
@@ -629,6 +677,109 @@ static csx_status pair_tables_batch(Call& call, const std::vector<PointSet*>& se
           }
         }
     bin_off += static_cast<size_t>(ps.nbins);
+  }
+  return CSX_OK;
+}
+
+// Pair tables of one or more point sets.  Each set has its own launch plan
+// (sweep mode and q tile, from its own work items), so its tables are the
+// same whether it is computed alone or in a batch; tables already cached on
+// a set for the same q grid and plan are reused.  The others come from one
+// launch of the pair kernel per plan among them (a launch concatenates the
+// sets' points, work items and bins; sets on the direct sweep, whose bits
+// do not depend on the tile, share one launch) and are cached:
+// P[c][b*nq + k] = sum_{i<j in bin b of set c} occ_i occ_j sinc(q_k r_ij).
+// One set waits for a concurrent call computing the same tables; in a batch,
+// tables that another call is still computing are computed again.
+// `overlap`, when given, runs on the host while the (first) pair kernel
+// runs, or at once when nothing is launched.
+static csx_status pair_tables_batch(Call& call, const std::vector<PointSet*>& sets,
+                                    const double* q, size_t nq,
+                                    std::vector<std::vector<double>>& P,
+                                    const std::function<void()>& overlap = nullptr) {
+  Tuning tune;
+  {
+    std::lock_guard<std::mutex> lock(call.ctx()->mu);
+    tune = call.ctx()->tuning;
+  }
+  const int cus = call.ctx()->cu_count;
+  // Work items of every set (made when the structure is created, or once
+  // per j chunk) and the set's plan.
+  std::vector<int> chunk(sets.size());
+  std::vector<std::shared_ptr<ItemSet>> all_items(sets.size());
+  std::vector<PairPlan> plans(sets.size());
+  std::vector<std::string> keys(sets.size());
+  for (size_t c = 0; c < sets.size(); ++c) {
+    chunk[c] = pair_j_chunk(tune.j_chunk, *sets[c], cus);
+    CSX_TRY(get_item_set(*sets[c], chunk[c], &all_items[c]));
+    size_t ni;
+    double sum_j, max_j;
+    item_stats({all_items[c]}, &ni, &sum_j, &max_j);
+    plans[c] = pair_plan(tune, q, nq, sets[c]->max_radius, ni, sum_j, max_j, cus);
+    keys[c] = table_key(plans[c].sin_mode, plans[c].qt, chunk[c], q, nq);
+  }
+
+  P.assign(sets.size(), std::vector<double>());
+  std::vector<size_t> todo;
+  std::shared_ptr<CacheEntry> entry;  // the one set's entry, locked while computed
+  std::unique_lock<std::mutex> entry_lock;
+  if (sets.size() == 1) {
+    entry = cache_acquire(call.ctx(), *sets[0], keys[0], &entry_lock);
+    if (entry->ready) {
+      P[0] = entry->tables;
+      if (overlap) overlap();
+      return CSX_OK;
+    }
+    todo.push_back(0);
+  } else {
+    for (size_t c = 0; c < sets.size(); ++c) {
+      std::shared_ptr<CacheEntry> e = cache_find(call.ctx(), *sets[c], keys[c]);
+      if (e) P[c] = e->tables;
+      else todo.push_back(c);
+    }
+    if (todo.empty()) {
+      if (overlap) overlap();
+      return CSX_OK;
+    }
+  }
+  // A failed computation leaves no half-made entry behind.
+  struct Dropper {
+    PointSet* ps;
+    std::shared_ptr<CacheEntry>* e;
+    ~Dropper() {
+      if (*e && !(*e)->ready) cache_drop(*ps, *e);
+    }
+  } dropper{sets[0], &entry};
+
+  // Groups of sets with one plan; direct sweeps of one sine mode share a
+  // launch, whose tile follows the group's work items.
+  std::vector<std::vector<size_t>> groups;
+  std::vector<PairPlan> gplans;
+  for (size_t c : todo) {
+    size_t g = 0;
+    for (; g < groups.size(); ++g) {
+      const PairPlan& a = gplans[g];
+      const PairPlan& b = plans[c];
+      if (a.sin_mode == b.sin_mode && (b.sin_mode != 3 || (a.qt == b.qt && a.qstep == b.qstep)))
+        break;
+    }
+    if (g == groups.size()) {
+      groups.emplace_back();
+      gplans.push_back(plans[c]);
+    }
+    groups[g].push_back(c);
+  }
+  for (size_t g = 0; g < groups.size(); ++g) {
+    if (gplans[g].sin_mode != 3) {
+      std::vector<std::shared_ptr<ItemSet>> gi;
+      for (size_t c : groups[g]) gi.push_back(all_items[c]);
+      size_t ni;
+      double sum_j, max_j;
+      item_stats(gi, &ni, &sum_j, &max_j);
+      gplans[g].qt = choose_q_tile(tune.q_tile, gplans[g].sin_mode, nq, ni, sum_j, max_j, cus);
+    }
+    CSX_TRY(pair_tables_launch(call, sets, groups[g], all_items, chunk, gplans[g], tune, q, nq, P,
+                               g == 0 ? overlap : nullptr));
   }
   if (entry) {
     entry->tables = P[0];
@@ -887,12 +1038,16 @@ csx_status csx_ctx_reset_timing(csx_ctx* ctx) {
 
 csx_status csx_ctx_set_tuning(csx_ctx* ctx, int q_tile, int sin_mode, int j_chunk) {
   if (!ctx) return fail(CSX_INVALID_INPUT, "context is null");
-  if (q_tile < 0 || q_tile > 64 ||
-      (q_tile % 8 != 0 && q_tile != 10 && q_tile != 20 && q_tile != 25 && q_tile != 50 &&
-       q_tile != 63))
+  const bool tile_ok = q_tile == 100 || q_tile == 125 || q_tile == 128 ||
+                       (q_tile >= 0 && q_tile <= 64 &&
+                        (q_tile % 8 == 0 || q_tile == 10 || q_tile == 20 || q_tile == 25 ||
+                         q_tile == 50 || q_tile == 63));
+  if (!tile_ok)
     return fail(CSX_INVALID_INPUT,
-                "q_tile must be 0 (automatic), a multiple of 8 up to 64, 10, 20, 25, 50 or 63");
-  if (sin_mode < 0 || sin_mode > 2) return fail(CSX_INVALID_INPUT, "sin_mode must be 0, 1 or 2");
+                "q_tile must be 0 (automatic), a multiple of 8 up to 64, 10, 20, 25, 50, 63, "
+                "100, 125 or 128");
+  if (sin_mode < 0 || sin_mode > 3)
+    return fail(CSX_INVALID_INPUT, "sin_mode must be 0, 1, 2 or 3");
   if (!(j_chunk == 0 || j_chunk == 32 || j_chunk == 64 || j_chunk == 128 ||
         (j_chunk >= 256 && j_chunk % 256 == 0 && j_chunk <= (1 << 20))))
     return fail(CSX_INVALID_INPUT,
@@ -1240,15 +1395,15 @@ csx_status csx_debye_pair_tables_noalloc(csx_structure* s, const double* q, size
     return fail(CSX_INVALID_INPUT,
                 "no work items for the context's j chunk on this structure: call "
                 "csx_debye_pair_tables on it once after changing the j chunk");
-  const int sin_mode = pair_sin_mode(tune.sin_mode, q, nq, ps->max_radius);
   double sum_j = 0.0, max_j = 0.0;
   for (size_t k = 3; k < is->h_items.size(); k += 4) {
     sum_j += is->h_items[k];
     max_j = std::max(max_j, static_cast<double>(is->h_items[k]));
   }
   const int cus = ctx->cu_count;
-  const int qt = choose_q_tile(tune.q_tile, sin_mode, nq, static_cast<size_t>(is->nitems), sum_j,
-                               max_j, cus);
+  const PairPlan plan = pair_plan(tune, q, nq, ps->max_radius, static_cast<size_t>(is->nitems),
+                                  sum_j, max_j, cus);
+  const int sin_mode = plan.sin_mode, qt = plan.qt;
   const size_t nq_pad = (nq + qt - 1) / qt * qt;
   const size_t nitems = static_cast<size_t>(is->nitems), nbins = static_cast<size_t>(ps->nbins);
   const int persist_wgs = pair_persist_wgs(tune.persist, nitems, nq_pad, qt, cus);
@@ -1281,11 +1436,12 @@ csx_status csx_debye_pair_tables_noalloc(csx_structure* s, const double* q, size
                               st),
           "write_floats");
   CSX_HIP(launch_debye_pairs(ps->pts32.as<float4>(), is->items.as<int4>(), static_cast<int>(nitems),
-                             is->bin_first.as<int>(), static_cast<int>(nbins), qrev,
+                             is->bin_first.as<int>(), static_cast<int>(nbins), qrev, plan.qstep,
                              static_cast<int>(nq_pad), qt, sin_mode, persist_wgs, counter, partial,
                              raw, st, nullptr),
           "debye_pairs_f32");
-  CSX_HIP(launch_finish_tables(raw, qrev, ps->q0.as<double>(), static_cast<int>(nbins),
+  CSX_HIP(launch_finish_tables(raw, qrev, qt, plan.qstep, ps->q0.as<double>(),
+                               static_cast<int>(nbins),
                                static_cast<int>(nq), static_cast<int>(nq_pad), tables, st),
           "finish_tables");
   return CSX_OK;

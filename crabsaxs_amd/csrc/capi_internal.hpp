@@ -34,7 +34,8 @@ constexpr int kAutoPersist = 4;
 // accuracy); the autotuner can replace them per workload.
 struct Tuning {
   int q_tile = 0;      // q values per sweep of a pair (0: automatic)
-  int sin_mode = 2;    // 0 software polynomial, 1 reduced + V_SIN_F32, 2 V_SIN_F32
+  int sin_mode = 3;    // 0 software polynomial, 1 reduced + V_SIN_F32, 2 V_SIN_F32,
+                       // 3 sine recurrence on uniform q grids where it models faster, else 2
   int j_chunk = 0;     // j atoms per work item (0: automatic, from the point count)
   int persist = -1;    // persistent launch with this many workgroups per CU (0: grid;
                        // -1: automatic, kAutoPersist per CU when tasks are plenty)

@@ -9,12 +9,17 @@
 //  debye_pairs_f32   Per-type pair tables.  A work item pairs a type-pure block
 //                    of 256 "i" atoms (one per lane) with a chunk of "j" atoms
 //                    of one type; r_ij is computed once per pair and swept over
-//                    QT q values held in scalar registers.  The term
-//                    occ_j sin(q r)/r is accumulated in FP32 per LDS tile
-//                    (<= 256 pairs per lane), folded into FP64 per tile, and
-//                    the workgroup writes one FP64 partial per (item, q); a
-//                    second kernel sums partials per type bin in a fixed order,
-//                    so results are bitwise reproducible run to run.
+//                    QT q values.  The term occ_j sin(q r)/r is accumulated in
+//                    FP32 per LDS tile (<= 256 pairs per lane), folded into
+//                    FP64 per tile, and the workgroup writes one FP64 partial
+//                    per (item, q); a second kernel sums partials per type bin
+//                    in a fixed order, so results are bitwise reproducible run
+//                    to run.  Sweep modes 0-2 evaluate a sine per pair and q
+//                    (q in scalar registers); mode 3, for a uniform q grid,
+//                    runs Reinsch's form of the sine recurrence along the
+//                    tile's q values (three FMA-class operations per pair and
+//                    q instead of a multiply, a two-cycle V_SIN_F32 and an
+//                    FMA), restarted from hardware sines at every tile.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -182,6 +187,19 @@ __device__ inline float sin2pi(float u) {
   return sin2pi_hw_direct(u);
 }
 
+// 2 sin(pi t) for |t| <= 1/4 (t in revolutions): the Taylor series to t^9,
+// truncation below 3.4e-9 of the value, relative accuracy kept for tiny t
+// (the recurrence's step constants need it for short pairs).
+__device__ inline float two_sin_pi_small(float t) {
+  const float t2 = t * t;
+  float p = 0.16429176926612854f;
+  p = __builtin_fmaf(p, t2, -1.1985290050506592f);
+  p = __builtin_fmaf(p, t2, 5.100327968597412f);
+  p = __builtin_fmaf(p, t2, -10.33542537689209f);
+  p = __builtin_fmaf(p, t2, 6.2831854820251465f);
+  return t * p;
+}
+
 // The pairs of lane atom pi with tile[0 .. jn): acc[k] += occ_j / r *
 // sin(2 pi qk[k] r).  r = d2 * rsq(d2) refined by one Newton step (the raw
 // square-root approximations are biased, which shifts every phase q r the
@@ -210,30 +228,146 @@ __device__ __attribute__((always_inline)) inline void tile_sweep(const float4* t
   }
 }
 
+// The same pairs over the uniform q grid q0 + k qstep (revolutions), k < QT:
+// s_k = sin(2 pi (q0 + k qstep) r) by Reinsch's recurrence
+//   d_{k+1} = d_k + lam s_k,  s_{k+1} = s_k + d_{k+1},
+//   lam = 2 cos(2 pi th) - 2 = -(2 sin(pi th))^2,  th = qstep r,
+// which keeps the step difference d_k = s_k - s_{k-1} as its own variable:
+// a rounding of s moves the sinusoid's amplitude by that rounding, and one
+// of d by d's relative rounding, so the error does not grow as 1/th for
+// short pairs the way the three-term form s_{k+1} = 2 cos(th) s_k - s_{k-1}
+// does.  Start: s_0 = sin(2 pi q0 r) and d_0 = s_0 - s_{-1} =
+// 2 sin(pi th) cos(2 pi (q0 r - th / 2)), from V_SIN_F32 / V_COS_F32 on the
+// unreduced arguments (the host keeps |q0 r| <= 250 revolutions) and the
+// step constant from its own series (the host keeps |th| <= 1/4).
+// The same pairs over the uniform q grid q0 + k qstep (revolutions), k < QT.
+// Each pair runs Reinsch's form of the sine recurrence on its weighted sine
+// s_k = w sin(2 pi (q0 + k qstep) r), w = occ_j / r:
+//   d_{k+1} = d_k + lam s_k,  s_{k+1} = s_k + d_{k+1},
+//   lam = 2 cos(2 pi th) - 2 = -(2 sin(pi th))^2,  th = qstep r,
+// so a q costs one FMA and two adds (acc[k] += s_k), against a multiply, a
+// V_SIN_F32 (a quarter-rate transcendental) and an FMA for a direct sweep.
+// The form keeps the step difference d_k = s_k - s_{k-1} as its own
+// variable: a rounding of s moves the sinusoid's amplitude by that rounding,
+// and one of d by d's relative rounding, so the error does not grow as 1/th
+// for short pairs the way the three-term form s_{k+1} = 2 cos(th) s_k -
+// s_{k-1} does.  Start: s_0 = w sin(2 pi q0 r) and d_0 = s_0 - s_{-1} =
+// w 2 sin(pi th) cos(2 pi (q0 r - th / 2)), from V_SIN_F32 / V_COS_F32 on
+// the unreduced arguments (the host keeps |q0 r| <= 250 revolutions) and
+// the step constant from its own series (the host keeps |th| <= 1/4).
+// Masked pairs (j <= i) get w = 0 and add exact zeros.
+template <int QT, bool MASK>
+__device__ __attribute__((always_inline)) inline void tile_sweep_rec(const float4* tile, int jn,
+                                                                     const float4 pi,
+                                                                     int jt_minus_i, float q0,
+                                                                     float qstep, float* acc) {
+  for (int t = 0; t < jn; ++t) {
+    const float4 pj = tile[t];
+    const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+    const float d2 = fmaxf(__builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz)), 1e-30f);
+    const float y = __builtin_amdgcn_rsqf(d2);
+    const float r0 = d2 * y;
+    const float r = __builtin_fmaf(__builtin_fmaf(-r0, r0, d2), 0.5f * y, r0);
+    const float w = (!MASK || t + jt_minus_i > 0) ? pj.w * y : 0.f;
+    const float th = qstep * r;
+    const float h2 = two_sin_pi_small(th);
+    const float lam = -(h2 * h2);
+    const float u0 = q0 * r;
+    float s = w * __builtin_amdgcn_sinf(u0);
+    float d = (w * h2) * __builtin_amdgcn_cosf(__builtin_fmaf(-0.5f, th, u0));
+#pragma unroll
+    for (int k = 0; k < QT; ++k) {
+      acc[k] += s;
+      if (k + 1 < QT) {
+        d = __builtin_fmaf(lam, s, d);
+        s += d;
+      }
+    }
+  }
+}
+
+template <int QT, int SINMODE, bool MASK>
+__device__ __attribute__((always_inline)) inline void sweep(const float4* tile, int jn,
+                                                            const float4 pi, int jt_minus_i,
+                                                            const float* qk, float qstep,
+                                                            float* acc) {
+  if constexpr (SINMODE == 3)
+    tile_sweep_rec<QT, MASK>(tile, jn, pi, jt_minus_i, qk[0], qstep, acc);
+  else
+    tile_sweep<QT, SINMODE, MASK>(tile, jn, pi, jt_minus_i, qk, acc);
+}
+
+constexpr int pow2_tile(int n) { return n <= 8 ? 8 : n <= 16 ? 16 : n <= 32 ? 32 : 64; }
+
+// Wave sums of N <= 64 lane values acc[0 .. N), each weighted by wi first,
+// and acc cleared: a transposing butterfly.  Each step exchanges half of a
+// lane's values with its partner at xor offset o and keeps the half that
+// the lane's bit o selects, so every value meets the same partners, in the
+// same order (offsets 32 .. 1), as a full xor butterfly per value, at about
+// one shuffle per value instead of six.  Lane (64 / pow2_tile(N)) * k ends
+// with value k's sum.
+template <int N>
+__device__ __attribute__((always_inline)) inline float wave_tile_sums(float* acc, float wi,
+                                                                      int lane) {
+  constexpr int kPow = pow2_tile(N);
+  constexpr int kStride = 64 / kPow;
+  float red[kPow];
+#pragma unroll
+  for (int k = 0; k < kPow; ++k) {
+    red[k] = k < N ? acc[k] * wi : 0.f;
+    if (k < N) acc[k] = 0.f;
+  }
+#pragma unroll
+  for (int o = 32, h = kPow / 2; o >= kStride; o >>= 1, h >>= 1) {
+    const bool up = (lane & o) != 0;
+#pragma unroll
+    for (int k = 0; k < h; ++k) {
+      // The empty asm hands the selects register values, not array reads:
+      // otherwise the compiler folds them into one read at a lane-dependent
+      // index, and the array leaves registers for compare-select chains.
+      float lo = red[k], hi = red[h + k];
+      asm("" : "+v"(lo), "+v"(hi));
+      const float keep = up ? hi : lo;
+      const float send = up ? lo : hi;
+      red[k] = keep + __shfl_xor(send, o, 64);
+    }
+  }
+  float v = red[0];
+#pragma unroll
+  for (int o = kStride / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
 // One work item (type-pure block of 256 i atoms x chunk of j atoms) for the
-// QT q values starting at qc; writes partial_row[qc .. qc + QT).
+// QT q values starting at qc; writes partial_row[qc .. qc + QT).  QT <= 128:
+// the tile sums go through the wave reduction in one or two groups of <= 64.
 template <int QT, int SINMODE>
 __device__ __attribute__((always_inline)) inline void pair_item(const float4* __restrict__ pts, const int4 item, const int qc,
-                                 const float* __restrict__ qrev, double* __restrict__ partial_row) {
+                                 const float* __restrict__ qrev, float qstep,
+                                 double* __restrict__ partial_row) {
+  static_assert(QT >= 1 && QT <= 128, "q tile");
+  static_assert(SINMODE == 3 || QT <= 64, "direct sweeps hold q in registers");
+  constexpr int kN0 = QT < 64 ? QT : 64;
+  constexpr int kN1 = QT - kN0;
+  constexpr int kStride0 = 64 / pow2_tile(kN0);
+  constexpr int kStride1 = kN1 > 0 ? 64 / pow2_tile(kN1) : 1;
+  constexpr int kQk = SINMODE == 3 ? 1 : QT;
   __shared__ float4 tile[kBlock];
   __shared__ double scratch[4 * QT];
   const int i = item.x + threadIdx.x;
   const bool vi = threadIdx.x < item.y;
   const float4 pi = vi ? pts[i] : make_float4(0.f, 0.f, 0.f, 0.f);
-  float qk[QT];
+  float qk[kQk];
 #pragma unroll
-  for (int k = 0; k < QT; ++k) qk[k] = qrev[qc + k];
+  for (int k = 0; k < kQk; ++k) qk[k] = qrev[qc + k];
   // Per lane: FP32 tile sums acc[k] (<= 256 pairs).  After each tile they are
-  // weighted by occ_i, summed over the wave (fixed butterfly) and lane
-  // kStride * k adds the wave's tile sum for q_k to its FP64 running sum
-  // `run` (kPow: QT rounded up to a power of two; kStride = 64 / kPow).
-  constexpr int kPow = QT <= 8 ? 8 : QT <= 16 ? 16 : QT <= 32 ? 32 : 64;
-  constexpr int kStride = 64 / kPow;
-  float red[kPow];
+  // weighted by occ_i, summed over the wave, and the lane holding q_k's wave
+  // sum adds it to its FP64 running sum (run0 for the first 64 q, run1 for
+  // the rest).
   float acc[QT];
 #pragma unroll
   for (int k = 0; k < QT; ++k) acc[k] = 0.f;
-  double run = 0.0;
+  double run0 = 0.0, run1 = 0.0;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int jend = item.z + item.w;
   // This wave's first i; a wave with no atoms of the block, or whose atoms
@@ -250,53 +384,45 @@ __device__ __attribute__((always_inline)) inline void pair_item(const float4* __
     // only a tile overlapping the i block needs the j > i mask
     if (t0 < item.x + item.y && t0 + kBlock > item.x) {
       if (wave_live && wave_i0 < t0 + jn - 1)
-        tile_sweep<QT, SINMODE, true>(tile, jn, pi, t0 - i, qk, acc);
+        sweep<QT, SINMODE, true>(tile, jn, pi, t0 - i, qk, qstep, acc);
     } else if (wave_live) {
-      tile_sweep<QT, SINMODE, false>(tile, jn, pi, t0 - i, qk, acc);
+      sweep<QT, SINMODE, false>(tile, jn, pi, t0 - i, qk, qstep, acc);
     }
-    // Wave sums of the QT tile sums: a transposing butterfly.  Each step
-    // exchanges half of a lane's values with its partner at xor offset o and
-    // keeps the half that the lane's bit o selects, so every value meets the
-    // same partners, in the same order (offsets 32 .. 1), as a full xor
-    // butterfly per q, at about one shuffle per q instead of six; lane
-    // kStride * k ends with q_k's sum.
-#pragma unroll
-    for (int k = 0; k < kPow; ++k) {
-      red[k] = k < QT ? acc[k] * pi.w : 0.f;
-      if (k < QT) acc[k] = 0.f;
+    const float v0 = wave_tile_sums<kN0>(acc, pi.w, lane);
+    if ((lane & (kStride0 - 1)) == 0 && lane / kStride0 < kN0) run0 += double(v0);
+    if constexpr (kN1 > 0) {
+      const float v1 = wave_tile_sums<kN1>(acc + kN0, pi.w, lane);
+      if ((lane & (kStride1 - 1)) == 0 && lane / kStride1 < kN1) run1 += double(v1);
     }
-#pragma unroll
-    for (int o = 32, h = kPow / 2; o >= kStride; o >>= 1, h >>= 1) {
-      const bool up = (lane & o) != 0;
-#pragma unroll
-      for (int k = 0; k < h; ++k) {
-        const float keep = up ? red[h + k] : red[k];
-        const float send = up ? red[k] : red[h + k];
-        red[k] = keep + __shfl_xor(send, o, 64);
-      }
-    }
-    float v = red[0];
-#pragma unroll
-    for (int o = kStride / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    if ((lane & (kStride - 1)) == 0 && lane / kStride < QT) run += double(v);
   }
-  if ((lane & (kStride - 1)) == 0 && lane / kStride < QT) scratch[wave * QT + lane / kStride] = run;
+  if ((lane & (kStride0 - 1)) == 0 && lane / kStride0 < kN0)
+    scratch[wave * QT + lane / kStride0] = run0;
+  if constexpr (kN1 > 0) {
+    if ((lane & (kStride1 - 1)) == 0 && lane / kStride1 < kN1)
+      scratch[wave * QT + kN0 + lane / kStride1] = run1;
+  }
   __syncthreads();
-  if (threadIdx.x < QT) {
-    const int k = threadIdx.x;
+  for (int k = threadIdx.x; k < QT; k += kBlock)
     partial_row[qc + k] =
         ((scratch[k] + scratch[QT + k]) + scratch[2 * QT + k]) + scratch[3 * QT + k];
-  }
+}
+
+// Waves per SIMD the kernel is compiled for: the recurrence's wide tiles
+// (up to 128 accumulators) are held to three (168 VGPRs), so that other
+// waves cover each wave's dependent FMA-add steps.
+template <int SINMODE>
+constexpr int pair_min_waves() {
+  return SINMODE == 3 ? 3 : 1;
 }
 
 // One workgroup per (work item, q chunk).
 template <int QT, int SINMODE>
-__global__ __launch_bounds__(kBlock) void debye_pairs_f32(const float4* __restrict__ pts,
+__global__ __launch_bounds__(kBlock, pair_min_waves<SINMODE>()) void debye_pairs_f32(const float4* __restrict__ pts,
                                                           const int4* __restrict__ items,
                                                           const float* __restrict__ qrev,
-                                                          int nq_pad,
+                                                          float qstep, int nq_pad,
                                                           double* __restrict__ partial) {
-  pair_item<QT, SINMODE>(pts, items[blockIdx.x], blockIdx.y * QT, qrev,
+  pair_item<QT, SINMODE>(pts, items[blockIdx.x], blockIdx.y * QT, qrev, qstep,
                          partial + size_t(blockIdx.x) * nq_pad);
 }
 
@@ -304,9 +430,9 @@ __global__ __launch_bounds__(kBlock) void debye_pairs_f32(const float4* __restri
 // tasks from a global counter until all `total` tasks are done; every
 // workgroup exits once the counter passes `total`.
 template <int QT, int SINMODE>
-__global__ __launch_bounds__(kBlock) void debye_pairs_f32_persistent(
+__global__ __launch_bounds__(kBlock, pair_min_waves<SINMODE>()) void debye_pairs_f32_persistent(
     const float4* __restrict__ pts, const int4* __restrict__ items, const float* __restrict__ qrev,
-    int nq_pad, int total, int* __restrict__ counter, double* __restrict__ partial) {
+    float qstep, int nq_pad, int total, int* __restrict__ counter, double* __restrict__ partial) {
   __shared__ int task;
   const int nqc = nq_pad / QT;
   while (true) {
@@ -316,7 +442,7 @@ __global__ __launch_bounds__(kBlock) void debye_pairs_f32_persistent(
     const int w = task;
     if (w >= total) break;
     const int it = w / nqc;
-    pair_item<QT, SINMODE>(pts, items[it], (w - it * nqc) * QT, qrev,
+    pair_item<QT, SINMODE>(pts, items[it], (w - it * nqc) * QT, qrev, qstep,
                            partial + size_t(it) * nq_pad);
   }
 }
@@ -333,16 +459,21 @@ __global__ void write_floats(float* __restrict__ dst, int n, FloatBlock b) {
   if (k < n) dst[k] = b.v[k];
 }
 
-// tables[b * nq + k] = raw[b * nq_pad + k] / (2 pi qrev[k]), or q0[b] where
-// that is 0 (the occupancy products of the q = 0 row): the host's final step
-// of the pair tables, in the same operations.
+// tables[b * nq + k] = raw[b * nq_pad + k] / (2 pi q_k), or q0[b] where
+// that is 0 (the occupancy products of the q = 0 row), q_k the q (in
+// revolutions) the sweep evaluated: qrev[k] for a direct sweep, and for the
+// recurrence (qstep != 0) its tile's first value plus (k mod qt) steps.  The
+// host's final step of the pair tables, in the same operations.
 __global__ void finish_tables(const double* __restrict__ raw, const float* __restrict__ qrev,
-                              const double* __restrict__ q0, int nbins, int nq, int nq_pad,
-                              double* __restrict__ tables) {
+                              int qt, float qstep, const double* __restrict__ q0, int nbins,
+                              int nq, int nq_pad, double* __restrict__ tables) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   const int b = blockIdx.y;
   if (k >= nq || b >= nbins) return;
-  const double qe = 2.0 * 3.14159265358979323846 * double(qrev[k]);
+  const int kt = k % qt;
+  const double qk =
+      qstep != 0.f ? double(qrev[k - kt]) + double(kt) * double(qstep) : double(qrev[k]);
+  const double qe = 2.0 * 3.14159265358979323846 * qk;
   tables[size_t(b) * nq + k] = qe == 0.0 ? q0[b] : raw[size_t(b) * nq_pad + k] / qe;
 }
 
@@ -408,11 +539,13 @@ hipError_t launch_write_floats(float* dst, const double* q, int n, int n_total, 
   return hipGetLastError();
 }
 
-hipError_t launch_finish_tables(const double* raw, const float* qrev, const double* q0, int nbins,
-                                int nq, int nq_pad, double* tables, hipStream_t stream) {
+hipError_t launch_finish_tables(const double* raw, const float* qrev, int qt, float qstep,
+                                const double* q0, int nbins, int nq, int nq_pad, double* tables,
+                                hipStream_t stream) {
   if (nbins <= 0 || nq <= 0) return hipSuccess;
-  finish_tables<<<dim3((nq + 255) / 256, nbins), 256, 0, stream>>>(raw, qrev, q0, nbins, nq,
-                                                                  nq_pad, tables);
+  if (qt <= 0) return hipErrorInvalidValue;
+  finish_tables<<<dim3((nq + 255) / 256, nbins), 256, 0, stream>>>(raw, qrev, qt, qstep, q0,
+                                                                  nbins, nq, nq_pad, tables);
   return hipGetLastError();
 }
 
@@ -445,25 +578,26 @@ hipError_t launch_expand_amplitudes(const double* table, const int* type, const 
 
 template <int QT, int SM>
 static void launch_pairs_t(int nitems, int persist_wgs, int* counter, const float4* pts,
-                           const int4* items, const float* qrev, int nq_pad, double* partial,
-                           hipStream_t stream) {
+                           const int4* items, const float* qrev, float qstep, int nq_pad,
+                           double* partial, hipStream_t stream) {
   const int nqc = nq_pad / QT;
   if (persist_wgs > 0) {
     const int total = nitems * nqc;
     reset_counter<<<1, 1, 0, stream>>>(counter);
     debye_pairs_f32_persistent<QT, SM><<<std::min(persist_wgs, total), kBlock, 0, stream>>>(
-        pts, items, qrev, nq_pad, total, counter, partial);
+        pts, items, qrev, qstep, nq_pad, total, counter, partial);
   } else {
-    debye_pairs_f32<QT, SM><<<dim3(nitems, nqc), kBlock, 0, stream>>>(pts, items, qrev, nq_pad,
-                                                                       partial);
+    debye_pairs_f32<QT, SM><<<dim3(nitems, nqc), kBlock, 0, stream>>>(pts, items, qrev, qstep,
+                                                                       nq_pad, partial);
   }
 }
 
 hipError_t launch_debye_pairs(const float4* pts, const int4* items, int nitems, const int* bin_first,
-                              int nbins, const float* qrev, int nq_pad, int qt, int sin_mode,
-                              int persist_wgs, int* counter, double* partial, double* out,
-                              hipStream_t stream, KernelTimer* timer) {
-  if (nq_pad % qt != 0) return hipErrorInvalidValue;
+                              int nbins, const float* qrev, float qstep, int nq_pad, int qt,
+                              int sin_mode, int persist_wgs, int* counter, double* partial,
+                              double* out, hipStream_t stream, KernelTimer* timer) {
+  if (qt <= 0 || nq_pad % qt != 0) return hipErrorInvalidValue;
+  if ((sin_mode == 3) != (qstep != 0.f)) return hipErrorInvalidValue;
   if (persist_wgs > 0 && (!counter || static_cast<long long>(nitems) * (nq_pad / qt) > INT32_MAX))
     persist_wgs = 0;
   {
@@ -471,8 +605,8 @@ hipError_t launch_debye_pairs(const float4* pts, const int4* items, int nitems, 
     if (nitems > 0) {
 #define CSX_PAIRS_CASE(QT_, SM_)                                                              \
   else if (qt == QT_ && sin_mode == SM_) launch_pairs_t<QT_, SM_>(nitems, persist_wgs, counter, \
-                                                                  pts, items, qrev, nq_pad,    \
-                                                                  partial, stream);
+                                                                  pts, items, qrev, qstep,     \
+                                                                  nq_pad, partial, stream);
       if (false) {
       }
       CSX_PAIRS_CASE(8, 0) CSX_PAIRS_CASE(8, 1) CSX_PAIRS_CASE(8, 2)
@@ -482,6 +616,8 @@ hipError_t launch_debye_pairs(const float4* pts, const int4* items, int nitems, 
       CSX_PAIRS_CASE(24, 2) CSX_PAIRS_CASE(40, 2) CSX_PAIRS_CASE(48, 2) CSX_PAIRS_CASE(50, 2)
       CSX_PAIRS_CASE(56, 2) CSX_PAIRS_CASE(63, 2)
       CSX_PAIRS_CASE(10, 2) CSX_PAIRS_CASE(20, 2) CSX_PAIRS_CASE(25, 2)
+      CSX_PAIRS_CASE(16, 3) CSX_PAIRS_CASE(25, 3) CSX_PAIRS_CASE(32, 3) CSX_PAIRS_CASE(50, 3)
+      CSX_PAIRS_CASE(64, 3) CSX_PAIRS_CASE(100, 3) CSX_PAIRS_CASE(125, 3) CSX_PAIRS_CASE(128, 3)
       else return hipErrorInvalidValue;
 #undef CSX_PAIRS_CASE
     }

@@ -97,10 +97,12 @@ hipError_t launch_expand_amplitudes(const double* table, const int* type, const 
                                     hipStream_t stream);
 // persist_wgs > 0: persistent launch of that many workgroups over a task
 // counter (device int, reset by the launcher); 0: one workgroup per task.
+// sin_mode 0-2 evaluate a sine per pair and q (qstep 0); 3 runs the sine
+// recurrence over the uniform grid qrev[tile start] + k qstep (revolutions).
 hipError_t launch_debye_pairs(const float4* pts, const int4* items, int nitems, const int* bin_first,
-                              int nbins, const float* qrev, int nq_pad, int qt, int sin_mode,
-                              int persist_wgs, int* counter, double* partial, double* out,
-                              hipStream_t stream, KernelTimer* timer);
+                              int nbins, const float* qrev, float qstep, int nq_pad, int qt,
+                              int sin_mode, int persist_wgs, int* counter, double* partial,
+                              double* out, hipStream_t stream, KernelTimer* timer);
 
 // The no-allocation entry point's device scratch (debye.hip): its address on
 // a device (hipGetSymbolAddress, once per device with that device current),
@@ -111,7 +113,9 @@ void* noalloc_scratch(int device, size_t* bytes);
 // values passed as kernel arguments (no staging allocation).
 hipError_t launch_write_floats(float* dst, const double* q, int n, int n_total, double divisor,
                                hipStream_t stream);
-hipError_t launch_finish_tables(const double* raw, const float* qrev, const double* q0, int nbins,
-                                int nq, int nq_pad, double* tables, hipStream_t stream);
+// qstep != 0: the recurrence's q values (tile start qrev plus steps of qstep).
+hipError_t launch_finish_tables(const double* raw, const float* qrev, int qt, float qstep,
+                                const double* q0, int nbins, int nq, int nq_pad, double* tables,
+                                hipStream_t stream);
 
 }  // namespace csx

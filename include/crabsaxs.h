@@ -138,18 +138,24 @@ csx_status csx_ctx_reset_timing(csx_ctx *ctx);
  * with the least modelled time for the call's q count and work items, the
  * work spread over the CUs plus twice the longest item, so small structures
  * take narrow tiles and more workgroups), a multiple of 8 up
- * to 64, 10, 20, 25, 50 or 63 (sin mode 2 has 8, 10, 16, 20, 24, 25, 32,
+ * to 64, 10, 20, 25, 50, 63, 100, 125 or 128 (sin mode 2 has 8, 10, 16, 20, 24, 25, 32,
  * 40, 48, 50, 56, 63 and 64;
  * sin modes 0 and 1 have 8, 16, 32 and 64; another tile falls back to the
  * automatic choice); sin_mode 0 (software
  * polynomial after exact reduction in revolutions), 1 (hardware V_SIN_F32
- * after the same reduction) or 2 (V_SIN_F32 on the unreduced argument; used
- * only when |q| r_max / pi <= 250 and otherwise replaced by 1); j_chunk
+ * after the same reduction), 2 (V_SIN_F32 on the unreduced argument; used
+ * only when |q| r_max / pi <= 250 and otherwise replaced by 1) or 3, the
+ * default (on a uniform q grid whose step advances the phase of the largest
+ * pair by at most 1/4 revolution, a sine recurrence along each tile's q
+ * values, restarted from V_SIN_F32/V_COS_F32 per tile, where its modelled
+ * time is below mode 2's; tiles 16, 25, 32, 50, 64, 100, 125 and 128;
+ * otherwise mode 2, with the same range fallback); j_chunk
  * atoms per work item: 0 (automatic: the largest of 512, 256, 128, 64 and
  * 32 giving at least one full 256 x j_chunk item per CU, so small structures
  * spread over the device), 32, 64, 128 or a multiple of 256.  The chunk orders
  * the FP64 sums: a structure's tables are bitwise reproducible for a given
- * chunk (and, with the automatic chunk, a given CU count). */
+ * chunk (and, with the automatic chunk, a given CU count), and under the
+ * recurrence for a given q tile as well. */
 csx_status csx_ctx_set_tuning(csx_ctx *ctx, int q_tile, int sin_mode,
                               int j_chunk);
 

@@ -102,18 +102,25 @@ def test_every_q_tile_gives_identical_sums(gpu_ctx):
 
 
 def test_automatic_shape_for_a_large_structure(gpu_ctx):
-    """20,000 atoms (1,671 work items) are bound by total work: the automatic
-    choice is the tile with the least padded work (50 for 200 q, four sweeps
-    without padding; 63 for 500 q), launched persistent, 4 workgroups per
-    CU, as there are more than two tasks per workgroup."""
+    """20,000 atoms (1,671 work items) are bound by total work.  On a uniform
+    grid they take the sine recurrence with a 100-wide tile for 200 and 500
+    q (no padding, per-pair start amortised over 100 steps); the direct
+    hardware sine (mode 2) takes the tile with the least padded work (50 for
+    200 q, four sweeps without padding; 63 for 500 q).  Both launch
+    persistent, 4 workgroups per CU, as there are more than two tasks per
+    workgroup."""
     s = csx.generate_globule(20000, 35.0, 1.0, seed=1)
     cus = gpu_ctx.cu_count()
-    gpu_ctx.set_tuning()
     gpu_ctx.set_persistence()
-    for nq, tile in ((200, 50), (500, 63)):
-        csx.debye_intensity(s, np.linspace(0.0, 1.0, nq))
-        assert gpu_ctx.last_launch() == {"q_tile": tile, "sin_mode": 2,
-                                         "persistent_workgroups": 4 * cus}, nq
+    try:
+        for mode, tiles in ((3, (100, 100)), (2, (50, 63))):
+            gpu_ctx.set_tuning(0, mode, 0)
+            for nq, tile in zip((200, 500), tiles):
+                csx.debye_intensity(s, np.linspace(0.0, 1.0, nq))
+                assert gpu_ctx.last_launch() == {"q_tile": tile, "sin_mode": mode,
+                                                 "persistent_workgroups": 4 * cus}, (mode, nq)
+    finally:
+        gpu_ctx.set_tuning()
 
 
 def test_ensemble_nnls_recovers_weights(gpu_ctx):
@@ -141,9 +148,10 @@ def test_batch_autotuner_applies_a_measured_shape(gpu_ctx):
                                                    "persist": (0, 2)})
         assert best["tried"] == 4 and best["sample"] == 3 and best["kernel_ms"] > 0
         tuned = csx.batch_debye_intensity(cs, q)
-        gpu_ctx.set_tuning(0, 2, 256)
+        gpu_ctx.set_tuning(best["q_tile"], best["sin_mode"], 256)
         gpu_ctx.set_persistence(0)
-        # with the same j chunk every shape sums the same pairs in the same order
+        # with the same j chunk, sweep and q tile both launch forms sum the
+        # same pairs in the same order
         assert np.array_equal(tuned, csx.batch_debye_intensity(cs, q))
     finally:
         gpu_ctx.set_tuning()

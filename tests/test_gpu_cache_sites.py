@@ -151,7 +151,7 @@ def test_cache_keys_on_the_q_grid_and_arithmetic(gpu_ctx):
             csx.debye_intensity(s, q2)
             gpu_ctx.set_tuning(0, 1, 0)  # another sine: not the cached tables
             b = csx.debye_intensity(s, q1)
-            gpu_ctx.set_tuning(0, 2, 0)
+            gpu_ctx.set_tuning(0, 3, 0)
             c = csx.debye_intensity(s, q1)  # cached again
         finally:
             gpu_ctx.set_timing(False)

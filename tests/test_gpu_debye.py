@@ -193,7 +193,7 @@ def test_automatic_j_chunk(gpu_ctx, n):
     assert info["j_chunk"] == expected_auto_chunk(n, gpu_ctx.cu_count()), info
     b = csx.debye_intensity(csx.Structure.from_arrays(s.xyz, s.element), q)
     assert np.array_equal(a, b)
-    gpu_ctx.set_tuning(0, 2, info["j_chunk"])
+    gpu_ctx.set_tuning(0, 3, info["j_chunk"])
     try:
         c = csx.debye_intensity(csx.Structure.from_arrays(s.xyz, s.element), q)
     finally:
@@ -306,3 +306,86 @@ def test_coordinate_features_reference_fixture_and_edges(gpu_ctx):
         csx.coordinate_features(csx.Structure.from_arrays(np.zeros((0, 3)), np.zeros(0, np.uint8)))
     with pytest.raises(csx.InvalidInput):
         csx.coordinate_features(two, bins=5000)
+
+
+# ------------------------------------------------ sine recurrence (mode 3) --
+
+@pytest.mark.parametrize("tile", [0, 16, 25, 32, 50, 64, 100, 125, 128])
+def test_recurrence_tiles_against_fp64(gpu_ctx, tile):
+    """Every recurrence tile (restarts at every tile start) against FP64 GPU
+    Debye on a uniform grid, including q = 0 and a last tile with padding."""
+    s = csx.generate_globule(3000, 20.0, 1.0, seed=21)
+    q = np.linspace(0.0, 1.0, 301)
+    want = csx.debye_intensity(s, q, precision=F64)
+    gpu_ctx.set_tuning(tile, 3, 0)
+    try:
+        got = csx.debye_intensity(s, q)
+        launch = gpu_ctx.last_launch()
+    finally:
+        gpu_ctx.set_tuning()
+    assert launch["sin_mode"] == 3 and launch["q_tile"] == (tile or launch["q_tile"]), launch
+    err = rel(got, want).max()
+    print("recurrence tile", tile, err, flush=True)
+    assert err < 1e-5
+
+
+def test_recurrence_against_cpu_oracle(gpu_ctx):
+    """The recurrence's solvent intensity against the FP64 CPU oracle on a
+    uniform grid that does not start at 0, in 25-wide tiles (every tile
+    starts mid-grid)."""
+    s = csx.generate_globule(1200, 16.0, 1.0, seed=3)
+    q = 0.003 + 0.0049 * np.arange(100)
+    p = csx.SolventParams(c1=1.02, c2=1.5)
+    gpu_ctx.set_tuning(25, 3, 0)
+    try:
+        got = csx.solvent_corrected_intensity(s, q, p)
+        assert gpu_ctx.last_launch()["sin_mode"] == 3
+    finally:
+        gpu_ctx.set_tuning()
+    want = O.combine_solvent_basis(O.debye_solvent_basis(oracle_atoms(s), q, c1=1.02, c2=1.5),
+                                   q, 1.02, 1.5)
+    err = rel(got, want).max()
+    print("recurrence vs oracle", err, flush=True)
+    assert err < 1e-5
+
+
+def test_recurrence_fallbacks(gpu_ctx):
+    """A grid that is not uniform, or whose step turns the largest pair's
+    phase by more than 1/4 revolution, takes the direct hardware sine, with
+    the bits of an explicit mode 2; a descending uniform grid takes the
+    recurrence."""
+    s = csx.generate_globule(2000, 17.0, 1.0, seed=8)
+    gpu_ctx.set_tuning()
+    q = np.linspace(0.0, 0.8, 120)
+    q[57] += 1e-6
+    a = csx.debye_intensity(s, q)
+    assert gpu_ctx.last_launch()["sin_mode"] == 2
+    gpu_ctx.set_tuning(0, 2, 0)
+    try:
+        b = csx.debye_intensity(csx.Structure.from_arrays(s.xyz, s.element), q)
+    finally:
+        gpu_ctx.set_tuning()
+    assert np.array_equal(a, b)
+    coarse = np.linspace(0.0, 3.0, 5)  # 0.75 A^-1 steps over a ~36 A diameter
+    got = csx.debye_intensity(s, coarse)
+    assert gpu_ctx.last_launch()["sin_mode"] == 2
+    assert rel(got, csx.debye_intensity(s, coarse, precision=F64)).max() < 1e-5
+    down = np.linspace(0.9, 0.01, 90)
+    got = csx.debye_intensity(s, down)
+    assert gpu_ctx.last_launch()["sin_mode"] == 3
+    assert rel(got, csx.debye_intensity(s, down, precision=F64)).max() < 1e-5
+
+
+def test_recurrence_mode_is_bitwise_reproducible(gpu_ctx):
+    """For a given j chunk and q tile the recurrence's tables are the same
+    on every call, for the grid and persistent launch forms alike."""
+    s = csx.generate_globule(6000, 24.0, 1.0, seed=13)
+    q = np.linspace(0.0, 0.6, 200)
+    with gpu_ctx.caching(0):
+        base = csx.debye_intensity(s, q)
+        try:
+            for wgs in (0, 1, 4):
+                gpu_ctx.set_persistence(wgs)
+                assert np.array_equal(csx.debye_intensity(s, q), base), wgs
+        finally:
+            gpu_ctx.set_persistence()
