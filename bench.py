@@ -606,16 +606,27 @@ def small_leg(csx, ctx, log):
             w = np.ones(nq)
             qa = np.linspace(0.01, 1.0, nq)
             csx.debye_grad_x(s, qa, w, ctx=ctx)
-            _, dta, kmsa, _ = timed(ctx, "debye_vjp_f32",
+            g, dta, kmsa, _ = timed(ctx, "debye_vjp_f32",
                                     lambda: csx.debye_grad_x(s, qa, w, ctx=ctx), 10)
+            # the VJP along a random direction against central differences
+            # (h = 1e-4 A) of the FP64 GPU intensity
+            d = np.random.default_rng(n + nq).normal(size=s.xyz.shape)
+            fd = (csx.debye_intensity(csx.Structure.from_arrays(s.xyz + 1e-4 * d, s.element), qa,
+                                      precision=csx.Precision.FP64_PLAIN, ctx=ctx)
+                  - csx.debye_intensity(csx.Structure.from_arrays(s.xyz - 1e-4 * d, s.element),
+                                        qa, precision=csx.Precision.FP64_PLAIN, ctx=ctx)) / 2e-4
+            lhs, rhs = float(np.sum(g * d)), float(w @ fd)
             row = {"atoms": n, "nq": nq, "debye_kernel_ms": kms, "debye_wall_ms": dt * 1e3,
                    "launch": launch, "max_rel_err_vs_fp64": rel_err(got, ref)[0],
-                   "vjp_kernel_ms": kmsa, "vjp_wall_ms": dta * 1e3}
+                   "vjp_kernel_ms": kmsa, "vjp_wall_ms": dta * 1e3,
+                   "vjp_directional_rel_diff": abs(lhs - rhs) / max(np.sum(np.abs(w * fd)), 1e-300)}
             out.append(row)
             log(f"small: {n} atoms x {nq} q: Debye kernel {kms * 1e3:.1f} us "
                 f"(wall {dt * 1e3:.3f} ms, {launch}), VJP kernel {kmsa * 1e3:.1f} us")
     return {"workload": "globules of 300-5,000 atoms, q = linspace(0, 1, nq) (VJP: "
-                        "linspace(0.01, 1, nq), unit weights), library default launch shape",
+                        "linspace(0.01, 1, nq), unit weights; its directional derivative "
+                        "against FP64 central differences, relative to sum |w dI/dh|), "
+                        "library default launch shape",
             "cases": out}
 
 

@@ -12,9 +12,9 @@ hipError_t launch_debye_jacobian(const float4* pts, const int* orig, const int* 
                                  KernelTimer* timer);
 hipError_t launch_debye_vjp(const float4* pts, const int4* blocks, int nb, const int2* items,
                             size_t nitems, const int* orig, int ntypes, const float* qrev,
-                            const float* AB, int nq_pad, int qt, int sin_mode, int qsplit,
-                            int tsplit, size_t items_per_launch, double* part, double* gacc,
-                            double* out, hipStream_t stream, KernelTimer* timer);
+                            float qstep, const float* AB, int nq_pad, int qt, int sin_mode,
+                            int qsplit, int tsplit, size_t items_per_launch, double* part,
+                            double* gacc, double* out, hipStream_t stream, KernelTimer* timer);
 int jacobian_q_tile();
 int vjp_q_tile(int tuned, size_t nq, int sin_mode, size_t nitems, int cus);
 int vjp_qsplit(size_t nqc, size_t nitems);
@@ -60,12 +60,15 @@ static csx_status vjp(Call& call, const csx_structure* s, PointSet* ps, const do
   for (int I = 0; I < nb; ++I)
     for (int J = I; J < nb; ++J) items.insert(items.end(), {I, J});
   const size_t nitems = items.size() / 2;
-  // The unreduced hardware sine takes |q r / 2 pi| <= 256 revolutions.
+  // The unreduced hardware sine takes |q r / 2 pi| <= 256 revolutions; on
+  // a uniform grid within that range the sines and cosines come from the
+  // recurrence (sine mode 3).
   double umax = 0.0;
   for (size_t k = 0; k < nq; ++k)
     umax = std::max(umax, std::fabs(static_cast<double>(static_cast<float>(q[k] / (2.0 * kPi)))) *
                               2.0 * ps->max_radius);
-  const int sin_mode = umax <= 250.0 ? 2 : 1;
+  const float qstep = umax <= 250.0 ? uniform_q_step(q, nq, ps->max_radius) : 0.f;
+  const int sin_mode = umax <= 250.0 ? (qstep != 0.f ? 3 : 2) : 1;
   int tuned, split;
   {
     std::lock_guard<std::mutex> lock(call.ctx()->mu);
@@ -76,11 +79,16 @@ static csx_status vjp(Call& call, const csx_structure* s, PointSet* ps, const do
   const size_t nq_pad = (nq + qt - 1) / qt * qt;
   const std::vector<double> A = type_amplitudes_for(s, p, implicit_h, q, nq, T);
   // AB[a][b][0][k] = qe_k B_ab(k), AB[a][b][1][k] = B_ab(k) = w_k 2 A_a A_b / qe_k, where
-  // qe_k = 2 pi (q_k / 2 pi)_f32 is the q the kernel's phase uses.
+  // qe_k is the q the kernel's phase uses: 2 pi (q_k / 2 pi)_f32, or under the
+  // recurrence 2 pi times the chunk's first such value plus the steps since.
   std::vector<float> qrev(nq_pad, 0.0f), AB(static_cast<size_t>(T) * T * 2 * nq_pad, 0.0f);
+  for (size_t k = 0; k < nq; ++k) qrev[k] = static_cast<float>(q[k] / (2.0 * kPi));
   for (size_t k = 0; k < nq; ++k) {
-    qrev[k] = static_cast<float>(q[k] / (2.0 * kPi));
-    const double qe = 2.0 * kPi * static_cast<double>(qrev[k]);
+    const size_t kt = k % static_cast<size_t>(qt);
+    const double qe = 2.0 * kPi *
+                      (sin_mode == 3 ? static_cast<double>(qrev[k - kt]) +
+                                           static_cast<double>(kt) * static_cast<double>(qstep)
+                                     : static_cast<double>(qrev[k]));
     if (qe == 0.0) continue;
     for (int a = 0; a < T; ++a)
       for (int b = 0; b < T; ++b) {
@@ -137,7 +145,8 @@ static csx_status vjp(Call& call, const csx_structure* s, PointSet* ps, const do
   CSX_HIP(hipMemcpyAsync(d_items.as<void>(), items.data(), items.size() * sizeof(int),
                          hipMemcpyHostToDevice, st), "hipMemcpyAsync");
   CSX_HIP(launch_debye_vjp(ps->pts32.as<float4>(), d_blocks.as<int4>(), nb, d_items.as<int2>(),
-                           nitems, ps->orig32.as<int>(), T, d_q.as<float>(), d_ab.as<float>(),
+                           nitems, ps->orig32.as<int>(), T, d_q.as<float>(), qstep,
+                           d_ab.as<float>(),
                            static_cast<int>(nq_pad), qt, sin_mode, qsplit, tsplit, ipl,
                            d_part.as<double>(),
                            d_acc.as<double>(), d_out.as<double>(), st, call.timer()),

@@ -42,12 +42,85 @@ __device__ inline void sincos_rev(float v, float& s, float& c) {
   c = __builtin_amdgcn_cosf(f);
 }
 
+// 2 sin(pi t) and cos(pi t) for |t| <= 1/4 (t in revolutions): Taylor series
+// to t^9 and t^10, truncation below 4e-9 of the values; the sine keeps its
+// relative accuracy for tiny t.
+__device__ inline float two_sin_pi_small(float t) {
+  const float t2 = t * t;
+  float p = 0.16429176926612854f;
+  p = __builtin_fmaf(p, t2, -1.1985290050506592f);
+  p = __builtin_fmaf(p, t2, 5.100327968597412f);
+  p = __builtin_fmaf(p, t2, -10.33542537689209f);
+  p = __builtin_fmaf(p, t2, 6.2831854820251465f);
+  return t * p;
+}
+__device__ inline float cos_pi_small(float t) {
+  const float t2 = t * t;
+  float p = -0.025806890800595284f;
+  p = __builtin_fmaf(p, t2, 0.23533062636852264f);
+  p = __builtin_fmaf(p, t2, -1.3352627754211426f);
+  p = __builtin_fmaf(p, t2, 4.058712005615234f);
+  p = __builtin_fmaf(p, t2, -4.934802055358887f);
+  return __builtin_fmaf(p, t2, 1.0f);
+}
+
+// The pair's sums over the chunk's QT q values, C = sum_k A_k cos 2 pi v_k
+// and S = sum_k B_k sin 2 pi v_k, v_k = q_k r.  SM 1 and 2 evaluate the sine
+// and cosine per q.  SM 3 (uniform q grid: q_k = q0 + k qstep) carries
+// s_k and c_k by Reinsch's recurrence with the step difference kept as its
+// own variable (d_{k+1} = d_k + lam x_k, x_{k+1} = x_k + d_{k+1}, lam =
+// -(2 sin(pi th))^2, th = qstep r), four FMAs and two adds per q instead of
+// a multiply, two quarter-rate transcendentals and two FMAs.  Its start
+// comes from the sine and cosine of the half step before q0,
+// (sh, ch) at 2 pi (q0 r - th / 2): s_0 = sh cos(pi th) + ch sin(pi th),
+// c_0 = ch cos(pi th) - sh sin(pi th), and the first differences
+// s_0 - s_{-1} = 2 sin(pi th) ch, c_0 - c_{-1} = -2 sin(pi th) sh.
+template <int QT, int SM>
+__device__ __attribute__((always_inline)) inline void pair_q_sums(float r, const float* qk,
+                                                                  float qstep, const float* ak,
+                                                                  const float* bk, float& C,
+                                                                  float& S) {
+  C = 0.f;
+  S = 0.f;
+  if constexpr (SM == 3) {
+    const float th = qstep * r;
+    const float h2 = two_sin_pi_small(th);
+    const float g = cos_pi_small(th);
+    const float hh = 0.5f * h2;
+    const float v = __builtin_fmaf(-0.5f, th, qk[0] * r);
+    const float sh = __builtin_amdgcn_sinf(v), ch = __builtin_amdgcn_cosf(v);
+    float sn = __builtin_fmaf(ch, hh, sh * g);
+    float cs = __builtin_fmaf(-sh, hh, ch * g);
+    float ds = h2 * ch, dc = -(h2 * sh);
+    const float lam = -(h2 * h2);
+#pragma unroll
+    for (int k = 0; k < QT; ++k) {
+      C = __builtin_fmaf(ak[k], cs, C);
+      S = __builtin_fmaf(bk[k], sn, S);
+      if (k + 1 < QT) {
+        ds = __builtin_fmaf(lam, sn, ds);
+        dc = __builtin_fmaf(lam, cs, dc);
+        sn += ds;
+        cs += dc;
+      }
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < QT; ++k) {
+      float sn, cs;
+      sincos_rev<SM>(qk[k] * r, sn, cs);
+      C = __builtin_fmaf(ak[k], cs, C);
+      S = __builtin_fmaf(bk[k], sn, S);
+    }
+  }
+}
+
 template <int QT, int SM>
 __global__ __launch_bounds__(kAdjBlock) void debye_vjp_f32(
     const float4* __restrict__ pts, const int4* __restrict__ blocks /* start, len, type, 0 */,
     const int2* __restrict__ items /* (I, J), I <= J */, int ntypes,
-    const float* __restrict__ qrev, const float* __restrict__ AB /* [T][T][2][nq_pad] */,
-    int nq_pad, int qsplit, int tsplit,
+    const float* __restrict__ qrev, float qstep,
+    const float* __restrict__ AB /* [T][T][2][nq_pad] */, int nq_pad, int qsplit, int tsplit,
     double* __restrict__ part /* [items][qsplit * tsplit][2][3][256] */) {
   __shared__ float4 tile[kAdjBlock];
   __shared__ float accj[4][3][kAdjBlock];
@@ -56,9 +129,8 @@ __global__ __launch_bounds__(kAdjBlock) void debye_vjp_f32(
   const int2 it = items[blockIdx.x];
   const int4 bi = blocks[it.x], bj = blocks[it.y];
   const bool diag = it.x == it.y;
-  const float4 zero = make_float4(0.f, 0.f, 0.f, 0.f);
-  const float4 pi = tid < bi.y ? pts[bi.x + tid] : zero;
-  tile[tid] = tid < bj.y ? pts[bj.x + tid] : zero;
+  const float4 pi = tid < bi.y ? pts[bi.x + tid] : make_float4(0.f, 0.f, 0.f, 0.f);
+  tile[tid] = tid < bj.y ? pts[bj.x + tid] : make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
   for (int w = 0; w < 4; ++w)
 #pragma unroll
@@ -82,10 +154,11 @@ __global__ __launch_bounds__(kAdjBlock) void debye_vjp_f32(
       qab[2][tid] = B[qc + tid];
     }
     __syncthreads();
-    float qk[QT], ak[QT], bk[QT];
+    constexpr int kQk = SM == 3 ? 1 : QT;  // the recurrence needs the chunk's first q only
+    float qk[kQk], ak[QT], bk[QT];
 #pragma unroll
     for (int k = 0; k < QT; ++k) {
-      qk[k] = qab[0][k];
+      if (k < kQk) qk[k] = qab[0][k];
       ak[k] = qab[1][k];
       bk[k] = qab[2][k];
     }
@@ -105,14 +178,8 @@ __global__ __launch_bounds__(kAdjBlock) void debye_vjp_f32(
       const float r0 = dd * ir;
       const float r = __builtin_fmaf(__builtin_fmaf(-r0, r0, dd), 0.5f * ir, r0);
       const float wgt = use ? pi.w * pj.w * ir * ir * ir : 0.f;
-      float C = 0.f, S = 0.f;
-#pragma unroll
-      for (int k = 0; k < QT; ++k) {
-        float sn, cs;
-        sincos_rev<SM>(qk[k] * r, sn, cs);
-        C = __builtin_fmaf(ak[k], cs, C);
-        S = __builtin_fmaf(bk[k], sn, S);
-      }
+      float C, S;
+      pair_q_sums<QT, SM>(r, qk, qstep, ak, bk, C, S);
       const float h = __builtin_fmaf(r, C, -S) * wgt;
       gi[0] = __builtin_fmaf(h, dx, gi[0]);
       gi[1] = __builtin_fmaf(h, dy, gi[1]);
@@ -286,12 +353,15 @@ int vjp_qsplit(size_t nqc, size_t nitems) {
 // Large structures reduce to the least padded work; small ones, with few
 // block pairs, to one sweep per workgroup, which narrow tiles shorten.
 // Measured on MI355X over globules of 300 to 20,000 atoms and 50 to 300 q:
-// the chosen tile is within 3 % of the fastest.  Sine mode 1 has the tiles
+// the chosen tile is within 3 % of the fastest.  Under the recurrence (sine
+// mode 3) a pair.q costs four FMAs and two adds and a pair about 75 more
+// (the recurrence's start adds two series and the sine and cosine of the
+// half step).  Sine mode 1 has the tiles
 // 16 and 32 only; a tuned tile instantiated for the sine mode is taken as it
 // is.
 int vjp_q_tile(int tuned, size_t nq, int sin_mode, size_t nitems, int cus) {
   static const int kTiles[] = {16, 32, 48, 60, 64};
-  const int ntiles = sin_mode == 2 ? 5 : 2;
+  const int ntiles = sin_mode >= 2 ? 5 : 2;
   for (int k = 0; k < ntiles; ++k)
     if (kTiles[k] == tuned) return tuned;
   int best = 16;
@@ -303,7 +373,8 @@ int vjp_q_tile(int tuned, size_t nq, int sin_mode, size_t nitems, int cus) {
     const size_t c = static_cast<size_t>(std::max(cus, 1));
     const double sweeps = double((nqc + qsplit - 1) / qsplit);
     const double rounds = double((nitems * qsplit + c - 1) / c);
-    const double cost = (7.0 * qt + 45.0) * (qt >= 48 ? 1.055 : 1.0) * sweeps * rounds;
+    const double cost = (sin_mode == 3 ? 6.0 * qt + 75.0 : (7.0 * qt + 45.0) * (qt >= 48 ? 1.055 : 1.0)) *
+                        sweeps * rounds;
     if (cost < best_cost) best = qt, best_cost = cost;
   }
   return best;
@@ -324,12 +395,13 @@ hipError_t launch_debye_jacobian(const float4* pts, const int* orig, const int* 
 
 hipError_t launch_debye_vjp(const float4* pts, const int4* blocks, int nb, const int2* items,
                             size_t nitems, const int* orig, int ntypes, const float* qrev,
-                            const float* AB, int nq_pad, int qt, int sin_mode, int qsplit,
-                            int tsplit, size_t items_per_launch, double* part, double* gacc,
-                            double* out, hipStream_t stream, KernelTimer* timer) {
+                            float qstep, const float* AB, int nq_pad, int qt, int sin_mode,
+                            int qsplit, int tsplit, size_t items_per_launch, double* part,
+                            double* gacc, double* out, hipStream_t stream, KernelTimer* timer) {
   if (nb <= 0) return hipSuccess;
-  const bool hw_only = qt == 48 || qt == 60 || qt == 64;  // sine mode 2 only
-  if ((qt != 16 && qt != 32 && !(hw_only && sin_mode == 2)) || nq_pad % qt != 0 || qsplit < 1 ||
+  const bool hw_only = qt == 48 || qt == 60 || qt == 64;  // sine modes 2 and 3 only
+  if ((sin_mode == 3) != (qstep != 0.f) || sin_mode < 1 || sin_mode > 3) return hipErrorInvalidValue;
+  if ((qt != 16 && qt != 32 && !(hw_only && sin_mode >= 2)) || nq_pad % qt != 0 || qsplit < 1 ||
       qsplit > nq_pad / qt || items_per_launch == 0 ||
       (tsplit != 1 && tsplit != 2 && tsplit != 4 && tsplit != 8))
     return hipErrorInvalidValue;
@@ -342,21 +414,26 @@ hipError_t launch_debye_vjp(const float4* pts, const int4* blocks, int nb, const
       TimerScope scope(timer, "debye_vjp_f32", stream);
 #define CSX_VJP(QT_, SM_)                                                               \
   debye_vjp_f32<QT_, SM_><<<grid, kAdjBlock, 0, stream>>>(pts, blocks, items + k0, ntypes, qrev, \
-                                                          AB, nq_pad, qsplit, tsplit, part)
-      if (qt == 64)
-        CSX_VJP(64, 2);
-      else if (qt == 60)
-        CSX_VJP(60, 2);
-      else if (qt == 48)
-        CSX_VJP(48, 2);
-      else if (qt == 32 && sin_mode == 2)
-        CSX_VJP(32, 2);
-      else if (qt == 32)
-        CSX_VJP(32, 1);
-      else if (sin_mode == 2)
-        CSX_VJP(16, 2);
-      else
-        CSX_VJP(16, 1);
+                                                          qstep, AB, nq_pad, qsplit, tsplit, part)
+#define CSX_VJP_MODES(QT_)  \
+  if (sin_mode == 3)        \
+    CSX_VJP(QT_, 3);        \
+  else if (sin_mode == 2)   \
+    CSX_VJP(QT_, 2);
+      if (qt == 64) {
+        CSX_VJP_MODES(64)
+      } else if (qt == 60) {
+        CSX_VJP_MODES(60)
+      } else if (qt == 48) {
+        CSX_VJP_MODES(48)
+      } else if (qt == 32) {
+        CSX_VJP_MODES(32)
+        else CSX_VJP(32, 1);
+      } else {
+        CSX_VJP_MODES(16)
+        else CSX_VJP(16, 1);
+      }
+#undef CSX_VJP_MODES
 #undef CSX_VJP
     }
     reduce_vjp<<<nb, kAdjBlock, 0, stream>>>(part, k0, k1, qsplit * tsplit, nb, gacc);

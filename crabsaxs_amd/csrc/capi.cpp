@@ -433,7 +433,7 @@ static int choose_q_tile(int tuned, int sin_mode, size_t nq, size_t nitems, doub
 // evaluates q_k = fp32(q_t / 2 pi) + (k - t) step per tile starting at t,
 // within about 1e-7 of q_k relative: as close as the FP32 rounding of a
 // direct sweep's q.
-static float uniform_q_step(const double* q, size_t nq, double rmax) {
+float uniform_q_step(const double* q, size_t nq, double rmax) {
   if (nq < 3) return 0.f;
   const double step = (q[nq - 1] - q[0]) / static_cast<double>(nq - 1);
   double qmax = 0.0;

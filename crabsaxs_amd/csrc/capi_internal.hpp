@@ -398,6 +398,11 @@ csx_status get_point_set(csx_structure* s, const csx_solvent_params* p,
 // Work items of a point set for a j chunk, built on first use.
 csx_status get_item_set(PointSet& ps, int j_chunk, std::shared_ptr<ItemSet>* out);
 int pair_j_chunk(int tuned, const PointSet& ps, int cus);
+// The step of a uniform q grid in revolutions as FP32, or 0 where the sine
+// recurrence cannot take the grid (not uniform, fewer than 3 points, or a
+// step turning the phase of the largest pair distance 2 r_max by more than
+// 1/4 revolution).
+float uniform_q_step(const double* q, size_t nq, double rmax);
 
 // The cached result of `key` on a point set.  On return the entry's mutex
 // is locked by *lock; entry->ready tells whether it holds the result.  When
