@@ -16,7 +16,7 @@ hipError_t launch_debye_vjp(const float4* pts, const int4* blocks, int nb, const
                             int qsplit, int tsplit, size_t items_per_launch, double* part,
                             double* gacc, double* out, hipStream_t stream, KernelTimer* timer);
 int jacobian_q_tile();
-int vjp_q_tile(int tuned, size_t nq, int sin_mode, size_t nitems, int cus);
+int vjp_q_tile(int tuned, size_t nq, int sin_mode, size_t nitems, int cus, int max_tile);
 int vjp_qsplit(size_t nqc, size_t nitems);
 
 namespace {
@@ -75,7 +75,8 @@ static csx_status vjp(Call& call, const csx_structure* s, PointSet* ps, const do
     tuned = call.ctx()->tuning.vjp_q_tile;
     split = call.ctx()->tuning.vjp_split;
   }
-  const int qt = vjp_q_tile(tuned, nq, sin_mode, nitems, call.ctx()->cu_count);
+  const int qt = vjp_q_tile(tuned, nq, sin_mode, nitems, call.ctx()->cu_count,
+                            sin_mode == 3 ? rec_max_tile(qstep, ps->max_radius) : 1 << 30);
   const size_t nq_pad = (nq + qt - 1) / qt * qt;
   const std::vector<double> A = type_amplitudes_for(s, p, implicit_h, q, nq, T);
   // AB[a][b][0][k] = qe_k B_ab(k), AB[a][b][1][k] = B_ab(k) = w_k 2 A_a A_b / qe_k, where

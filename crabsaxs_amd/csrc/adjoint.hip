@@ -51,8 +51,10 @@ __device__ inline float two_sin_pi_small(float t) {
   p = __builtin_fmaf(p, t2, -1.1985290050506592f);
   p = __builtin_fmaf(p, t2, 5.100327968597412f);
   p = __builtin_fmaf(p, t2, -10.33542537689209f);
-  p = __builtin_fmaf(p, t2, 6.2831854820251465f);
-  return t * p;
+  // 2 pi t with 2 pi split in two FP32 parts (the FP32 2 pi alone is 2.8e-8
+  // too large, a bias every pair's step would share)
+  return __builtin_fmaf(t, 6.2831854820251465f,
+                        __builtin_fmaf(t * t2, p, t * -1.7484555314695172e-07f));
 }
 __device__ inline float cos_pi_small(float t) {
   const float t2 = t * t;
@@ -356,18 +358,20 @@ int vjp_qsplit(size_t nqc, size_t nitems) {
 // the chosen tile is within 3 % of the fastest.  Under the recurrence (sine
 // mode 3) a pair.q costs four FMAs and two adds and a pair about 75 more
 // (the recurrence's start adds two series and the sine and cosine of the
-// half step).  Sine mode 1 has the tiles
+// half step), and its tiles are held to max_tile (the pair kernel's limit
+// on the phase a tile accumulates).  Sine mode 1 has the tiles
 // 16 and 32 only; a tuned tile instantiated for the sine mode is taken as it
 // is.
-int vjp_q_tile(int tuned, size_t nq, int sin_mode, size_t nitems, int cus) {
+int vjp_q_tile(int tuned, size_t nq, int sin_mode, size_t nitems, int cus, int max_tile) {
   static const int kTiles[] = {16, 32, 48, 60, 64};
   const int ntiles = sin_mode >= 2 ? 5 : 2;
   for (int k = 0; k < ntiles; ++k)
-    if (kTiles[k] == tuned) return tuned;
+    if (kTiles[k] == tuned && (k == 0 || tuned <= max_tile)) return tuned;
   int best = 16;
   double best_cost = 1e300;
   for (int k = 0; k < ntiles; ++k) {
     const int qt = kTiles[k];
+    if (k > 0 && qt > max_tile) continue;
     const size_t nqc = (nq + qt - 1) / qt;
     const size_t qsplit = static_cast<size_t>(vjp_qsplit(nqc, nitems));
     const size_t c = static_cast<size_t>(std::max(cus, 1));

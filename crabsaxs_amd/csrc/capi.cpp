@@ -408,13 +408,14 @@ static double tail_weight(int sin_mode) { return sin_mode == 3 ? 0.5 : 2.0; }
 // most).
 // The tuned tile wins where it is instantiated for the mode.
 static int choose_q_tile(int tuned, int sin_mode, size_t nq, size_t nitems, double sum_j,
-                         double max_j, int cus) {
+                         double max_j, int cus, int max_tile = 1 << 30) {
   const int* tiles = sin_mode == 3 ? kTilesRec : sin_mode == 2 ? kTilesHw : kTilesAll;
   const int ntiles = sin_mode == 3 ? 8 : sin_mode == 2 ? 13 : 4;
   int best = tiles[0];
   double best_cost = 1e300;
   const double items = static_cast<double>(std::max<size_t>(nitems, 1));
   for (int k = 0; k < ntiles; ++k) {
+    if (tiles[k] > max_tile && k > 0) continue;
     const double sweeps = static_cast<double>((nq + tiles[k] - 1) / tiles[k]);
     const double per_cu = std::ceil(items * sweeps / std::max(cus, 1));
     const double cost = sweep_cost(sin_mode, tiles[k]) *
@@ -469,14 +470,36 @@ static int pair_sin_mode(int tuned, const double* q, size_t nq, double rmax) {
   return tuned >= 2 && !(umax <= 250.0) ? 1 : tuned;
 }
 
+// The widest recurrence tile for a step (revolutions) and point radius: the
+// recurrence's phase error grows with the phase it accumulates within a
+// tile, (QT - 1) th for the largest pair (th = |step| 2 r_max): each step
+// carries the FP32 rounding of th and of the step constant, where a direct
+// sweep rounds each phase once.  Tiles are held to kRecTilePhase
+// revolutions of it.  Measured on MI355X against FP64 GPU Debye: 20,000
+// atoms x 500 q (2.8 revolutions at QT 125) 1.0e-6 max rel err (direct
+// sweep 5.8e-7); 108,237 points x 200 q (r_max 100 A, 0.08 revolutions per
+// step) 5.5e-6 at QT 100 (7.9 revolutions), 2.4e-6 at QT 50 (3.9), direct
+// sweep 2.8e-6.  The 16-wide tile is always allowed (uniform_q_step keeps
+// th <= 1/4, so 3.75 revolutions at most).
+constexpr double kRecTilePhase = 4.0;
+int rec_max_tile(float step, double rmax) {
+  const double th = std::fabs(static_cast<double>(step)) * 2.0 * rmax;
+  return th > 0.0 ? static_cast<int>(std::min(1e6, 1.0 + kRecTilePhase / th)) : 1 << 30;
+}
+
 // Sweep mode, q tile and recurrence step of a pair-table launch for one
 // point set.  Mode 3 (the default) takes the recurrence wherever the grid
-// allows it (uniform_q_step), and the direct hardware sine (mode 2)
-// otherwise or when the tuned tile is one that only the direct sweep has
-// (on MI355X the recurrence's best tile beats the direct sweep's from 1,200
-// atoms up by 1.2-1.9x; at 300 atoms the direct sweep is 2 us faster).  The
-// plan follows from the q grid, the tuning and the set's own work items, so
-// a set gets the same plan, and the same bits, alone and in a batch.
+// allows it (uniform_q_step) and the call has at least kRecMinPairQ pair.q,
+// and the direct hardware sine (mode 2) otherwise or when the tuned tile is
+// one that only the direct sweep has.  On MI355X the recurrence's best tile
+// beats the direct sweep's from 1,200 atoms x 100 q (7e7 pair.q) up by
+// 1.3-1.9x; a call of a few million pair.q is one short task per CU, whose
+// length the recurrence's per-pair start and dependent steps set (300 atoms
+// x 100 q: 6.8 us direct, 9.2 us recurrence).  The plan follows from the q
+// grid, the tuning and the set itself, so a set gets the same plan, and the
+// same bits, alone and in a batch.
+constexpr double kRecMinPairQ = 2e7;
+
 struct PairPlan {
   int sin_mode = 2;
   int qt = 64;
@@ -484,15 +507,17 @@ struct PairPlan {
 };
 
 static PairPlan pair_plan(const Tuning& tune, const double* q, size_t nq, double rmax,
-                          size_t nitems, double sum_j, double max_j, int cus) {
+                          double npts, size_t nitems, double sum_j, double max_j, int cus) {
   PairPlan plan;
   plan.sin_mode = pair_sin_mode(tune.sin_mode, q, nq, rmax);
   if (plan.sin_mode == 3) {
     const float step = uniform_q_step(q, nq, rmax);
     const bool direct_only = tune.q_tile != 0 && !tile_listed(kTilesRec, 8, tune.q_tile) &&
                              tile_listed(kTilesHw, 13, tune.q_tile);
-    if (step != 0.f && !direct_only) {
-      plan.qt = choose_q_tile(tune.q_tile, 3, nq, nitems, sum_j, max_j, cus);
+    const bool tiny = tune.q_tile == 0 && 0.5 * npts * npts * static_cast<double>(nq) < kRecMinPairQ;
+    if (step != 0.f && !direct_only && !tiny) {
+      plan.qt = choose_q_tile(tune.q_tile, 3, nq, nitems, sum_j, max_j, cus,
+                              rec_max_tile(step, rmax));
       plan.qstep = step;
       return plan;
     }
@@ -715,7 +740,9 @@ static csx_status pair_tables_batch(Call& call, const std::vector<PointSet*>& se
     size_t ni;
     double sum_j, max_j;
     item_stats({all_items[c]}, &ni, &sum_j, &max_j);
-    plans[c] = pair_plan(tune, q, nq, sets[c]->max_radius, ni, sum_j, max_j, cus);
+    plans[c] = pair_plan(tune, q, nq, sets[c]->max_radius,
+                         static_cast<double>(sets[c]->natoms + sets[c]->nsites), ni, sum_j, max_j,
+                         cus);
     keys[c] = table_key(plans[c].sin_mode, plans[c].qt, chunk[c], q, nq);
   }
 
@@ -1401,8 +1428,9 @@ csx_status csx_debye_pair_tables_noalloc(csx_structure* s, const double* q, size
     max_j = std::max(max_j, static_cast<double>(is->h_items[k]));
   }
   const int cus = ctx->cu_count;
-  const PairPlan plan = pair_plan(tune, q, nq, ps->max_radius, static_cast<size_t>(is->nitems),
-                                  sum_j, max_j, cus);
+  const PairPlan plan = pair_plan(tune, q, nq, ps->max_radius,
+                                  static_cast<double>(ps->natoms + ps->nsites),
+                                  static_cast<size_t>(is->nitems), sum_j, max_j, cus);
   const int sin_mode = plan.sin_mode, qt = plan.qt;
   const size_t nq_pad = (nq + qt - 1) / qt * qt;
   const size_t nitems = static_cast<size_t>(is->nitems), nbins = static_cast<size_t>(ps->nbins);

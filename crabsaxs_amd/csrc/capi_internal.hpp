@@ -403,6 +403,9 @@ int pair_j_chunk(int tuned, const PointSet& ps, int cus);
 // step turning the phase of the largest pair distance 2 r_max by more than
 // 1/4 revolution).
 float uniform_q_step(const double* q, size_t nq, double rmax);
+// The widest q tile the recurrence takes for that step and point radius (its
+// phase error grows with the phase a tile accumulates).
+int rec_max_tile(float step, double rmax);
 
 // The cached result of `key` on a point set.  On return the entry's mutex
 // is locked by *lock; entry->ready tells whether it holds the result.  When

@@ -189,15 +189,20 @@ __device__ inline float sin2pi(float u) {
 
 // 2 sin(pi t) for |t| <= 1/4 (t in revolutions): the Taylor series to t^9,
 // truncation below 3.4e-9 of the value, relative accuracy kept for tiny t
-// (the recurrence's step constants need it for short pairs).
+// (the recurrence's step constants need it for short pairs) and no bias:
+// the step sets the frequency of every pair's recurrence, and a relative
+// bias e in it evaluates each tile at q (1 + e) after k steps, which moves
+// I(q) by e dlnI/dlnq where it is steep.
 __device__ inline float two_sin_pi_small(float t) {
   const float t2 = t * t;
   float p = 0.16429176926612854f;
   p = __builtin_fmaf(p, t2, -1.1985290050506592f);
   p = __builtin_fmaf(p, t2, 5.100327968597412f);
   p = __builtin_fmaf(p, t2, -10.33542537689209f);
-  p = __builtin_fmaf(p, t2, 6.2831854820251465f);
-  return t * p;
+  // 2 pi t with 2 pi split in two FP32 parts (the FP32 2 pi alone is 2.8e-8
+  // too large, a bias every pair's step would share)
+  return __builtin_fmaf(t, 6.2831854820251465f,
+                        __builtin_fmaf(t * t2, p, t * -1.7484555314695172e-07f));
 }
 
 // The pairs of lane atom pi with tile[0 .. jn): acc[k] += occ_j / r *

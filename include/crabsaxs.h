@@ -147,9 +147,12 @@ csx_status csx_ctx_reset_timing(csx_ctx *ctx);
  * only when |q| r_max / pi <= 250 and otherwise replaced by 1) or 3, the
  * default (on a uniform q grid whose step advances the phase of the largest
  * pair by at most 1/4 revolution, a sine recurrence along each tile's q
- * values, restarted from V_SIN_F32/V_COS_F32 per tile, where its modelled
- * time is below mode 2's; tiles 16, 25, 32, 50, 64, 100, 125 and 128;
- * otherwise mode 2, with the same range fallback); j_chunk
+ * values, restarted from V_SIN_F32/V_COS_F32 per tile, for calls of at
+ * least 2e7 pair.q or with a tuned recurrence tile; tiles 16, 25, 32, 50,
+ * 64, 100, 125 and 128, held to those whose q span turns the largest pair's
+ * phase by at most 4 revolutions; otherwise mode 2, with the same range
+ * fallback);
+ * j_chunk
  * atoms per work item: 0 (automatic: the largest of 512, 256, 128, 64 and
  * 32 giving at least one full 256 x j_chunk item per CU, so small structures
  * spread over the device), 32, 64, 128 or a multiple of 256.  The chunk orders

@@ -103,8 +103,10 @@ def test_every_q_tile_gives_identical_sums(gpu_ctx):
 
 def test_automatic_shape_for_a_large_structure(gpu_ctx):
     """20,000 atoms (1,671 work items) are bound by total work.  On a uniform
-    grid they take the sine recurrence with a 100-wide tile for 200 and 500
-    q (no padding, per-pair start amortised over 100 steps); the direct
+    grid they take the sine recurrence: for 500 q a 100-wide tile (no
+    padding, the per-pair start amortised over 100 steps); for 200 q the
+    0.005 A^-1 step turns the largest pair's phase by 0.058 revolutions, so
+    tiles are held to 70 q (4 revolutions) and the choice is 50.  The direct
     hardware sine (mode 2) takes the tile with the least padded work (50 for
     200 q, four sweeps without padding; 63 for 500 q).  Both launch
     persistent, 4 workgroups per CU, as there are more than two tasks per
@@ -113,7 +115,7 @@ def test_automatic_shape_for_a_large_structure(gpu_ctx):
     cus = gpu_ctx.cu_count()
     gpu_ctx.set_persistence()
     try:
-        for mode, tiles in ((3, (100, 100)), (2, (50, 63))):
+        for mode, tiles in ((3, (50, 100)), (2, (50, 63))):
             gpu_ctx.set_tuning(0, mode, 0)
             for nq, tile in zip((200, 500), tiles):
                 csx.debye_intensity(s, np.linspace(0.0, 1.0, nq))

@@ -389,3 +389,21 @@ def test_recurrence_mode_is_bitwise_reproducible(gpu_ctx):
                 assert np.array_equal(csx.debye_intensity(s, q), base), wgs
         finally:
             gpu_ctx.set_persistence()
+
+
+def test_recurrence_tiles_are_held_to_the_phase_limit(gpu_ctx):
+    """A coarse uniform grid (0.02 A^-1 steps over a ~50 A diameter: about
+    0.16 revolutions of the largest pair's phase per step) keeps the
+    recurrence's tiles to 4 revolutions: a tuned 128-wide tile falls back to
+    one of at most 25 q, and the result stays within 1e-5 of FP64."""
+    s = csx.generate_globule(6000, 24.0, 1.0, seed=17)
+    q = np.linspace(0.0, 2.0, 100)
+    want = csx.debye_intensity(s, q, precision=F64)
+    gpu_ctx.set_tuning(128, 3, 0)
+    try:
+        got = csx.debye_intensity(s, q)
+        launch = gpu_ctx.last_launch()
+    finally:
+        gpu_ctx.set_tuning()
+    assert launch["sin_mode"] == 3 and launch["q_tile"] in (16, 25), launch
+    assert rel(got, want).max() < 1e-5
