@@ -407,3 +407,24 @@ def test_recurrence_tiles_are_held_to_the_phase_limit(gpu_ctx):
         gpu_ctx.set_tuning()
     assert launch["sin_mode"] == 3 and launch["q_tile"] in (16, 25), launch
     assert rel(got, want).max() < 1e-5
+
+
+def test_tiny_calls_keep_the_direct_sweep(gpu_ctx):
+    """Below 2e7 pair.q a call is one short task per CU, where the direct
+    sweep's shorter chain is faster: 300 atoms x 100 q (4.5e6 pair.q) take
+    the direct hardware sine by default, 1,200 atoms x 100 q (7.2e7) the
+    recurrence; a tuned recurrence tile overrides the rule.  All within
+    1e-5 of FP64."""
+    gpu_ctx.set_tuning()
+    for n, mode in ((300, 2), (1200, 3)):
+        s = csx.generate_globule(n, 35.0 * (n / 20000) ** (1 / 3), 1.0, seed=5)
+        q = np.linspace(0.0, 1.0, 100)
+        got = csx.debye_intensity(s, q)
+        assert gpu_ctx.last_launch()["sin_mode"] == mode, n
+        assert rel(got, csx.debye_intensity(s, q, precision=F64)).max() < 1e-5
+    gpu_ctx.set_tuning(16, 3, 0)
+    try:
+        csx.debye_intensity(csx.generate_globule(300, 9.0, 1.0, seed=5), np.linspace(0.0, 1.0, 100))
+        assert gpu_ctx.last_launch()["sin_mode"] == 3
+    finally:
+        gpu_ctx.set_tuning()
