@@ -150,7 +150,9 @@ class Context {
     detail::check(csx_ctx_kernel_time(ptr_, name.c_str(), &n, &ms));
     return {n, ms};
   }
-  void set_tuning(int q_tile = 0, int sin_mode = 2, int j_chunk = 0) {
+  // Pair-kernel launch shape (csx_ctx_set_tuning); sin_mode 3, the default,
+  // takes the sine recurrence on uniform q grids.
+  void set_tuning(int q_tile = 0, int sin_mode = 3, int j_chunk = 0) {
     detail::check(csx_ctx_set_tuning(ptr_, q_tile, sin_mode, j_chunk));
   }
   void set_vjp_tile(int q_tile = 0) { detail::check(csx_ctx_set_vjp_tile(ptr_, q_tile)); }
