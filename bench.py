@@ -498,7 +498,17 @@ def multipole_leg(csx, ctx, log):
         lq = np.clip(lq_full, min(8, L), L)
         flops = 2.0 * n * float(np.sum((lq + 1) ** 2))
         conv = lq_full <= L
+        # Harmonic tiles the kernel streams: each 16-q block reads, for every
+        # 32-point slab, the FP16 hi/lo fragments (2 KB) of the column tiles
+        # up to its largest order (16 columns per tile, ceil((2l+1)/16) per l)
+        lqp = np.concatenate([lq, np.zeros((-len(lq)) % 16)]).astype(int)
+        tiles_to = np.cumsum([-(-(2 * ll + 1) // 16) for ll in range(L + 1)])
+        slabs = -(-n // 32)
+        ybytes = 2048.0 * slabs * float(sum(tiles_to[lqp[b:b + 16].max()]
+                                            for b in range(0, len(lqp), 16)))
         row = {"wall_ms": dt * 1e3, "kernel_ms": kms,
+               "harmonic_tile_bytes": ybytes,
+               "harmonic_stream_tb_s": ybytes / (kms / 1e3) / 1e12,
                "equiv_pair_q_per_s": pairq / dt,
                "speedup_vs_fp64_debye": dt64 / dt,
                "mfma_tflops_algorithmic": flops / (kms / 1e3) / 1e12,
