@@ -283,7 +283,9 @@ class Call {
           st.second += ms;
         }
       }
-      timer_.reset();
+      // a call may launch and finish again (one launch per plan group of a
+      // batch): later launches are timed too
+      timer_.reset(new KernelTimer);
     }
     return CSX_OK;
   }

@@ -38,6 +38,24 @@ def test_persistent_launch_is_bitwise_identical(gpu_ctx):
         gpu_ctx.set_persistence()
 
 
+def test_every_plan_group_of_a_batch_is_timed(gpu_ctx):
+    """A 300-atom set (4.5e6 pair.q, the direct sweep) and a 4,000-atom set
+    (the recurrence) have different plans, so the batch launches the pair
+    kernel twice and waits after each; the kernel timer counts both."""
+    cs = [csx.generate_globule(300, 9.0, 1.0, seed=5), csx.generate_globule(4000, 20.0, 1.0, seed=6)]
+    q = np.linspace(0.0, 0.5, 100)
+    gpu_ctx.reset_timing()
+    gpu_ctx.set_timing(True)
+    try:
+        got = csx.batch_debye_intensity(cs, q)
+    finally:
+        gpu_ctx.set_timing(False)
+    n, ms = gpu_ctx.kernel_time("debye_pairs_f32")
+    assert n == 2 and ms > 0
+    for k, c in enumerate(cs):
+        assert np.array_equal(got[k], csx.debye_intensity(c, q))
+
+
 def test_autotuner_picks_a_measured_shape(gpu_ctx, tmp_path, monkeypatch):
     monkeypatch.setattr(autotune, "CACHE", tmp_path / "tuning.json")
     s = csx.generate_globule(4000, 20.0, 1.0, seed=3)
