@@ -666,6 +666,20 @@ def adjoint_leg(csx, ctx, log):
     fd = (shifted(1.0) - shifted(-1.0)) / (2 * h)
     ad = float(np.sum(g * d))
     log(f"adjoint: {kms:.1f} ms, directional {ad:.6e} vs FD {fd:.6e}")
+    # The full Jacobian dI(q)/dx (300 q x 20,000 atoms x 3, FP64: 144 MB) from
+    # its own kernel (every ordered pair, direct sine and cosine), and its
+    # contraction with the same weights, which is the VJP above.
+    csx.debye_jacobian(s, q, truth, ctx=ctx)
+    jac, dtj, kmsj, _ = timed(ctx, "debye_jacobian_f32",
+                              lambda: csx.debye_jacobian(s, q, truth, ctx=ctx), 2)
+    gj = np.tensordot(w, jac, axes=1)
+    jacobian = {"kernel_ms": kmsj, "wall_ms": dtj * 1e3, "bytes": int(jac.nbytes),
+                "pair_q_per_s": pairq / (kmsj / 1e3),
+                "note": "pair.q counts unordered pairs; the kernel evaluates both orders",
+                "contracted_vs_vjp_rel_l2": float(np.linalg.norm(gj - g) / np.linalg.norm(g))}
+    del jac
+    log(f"adjoint: Jacobian {kmsj:.1f} ms kernel, {dtj * 1e3:.1f} ms per call, w.J vs VJP "
+        f"{jacobian['contracted_vs_vjp_rel_l2']:.2e}")
     e = csx.ExperimentalCurve(q, y, sigma)
     fits = {}
     bounds = csx.FitOptions()
@@ -712,6 +726,7 @@ def adjoint_leg(csx, ctx, log):
             "vjp_pair_q_per_s": pairq / (kms / 1e3),
             "directional_derivative": {"adjoint": ad, "central_difference_fp64": fd,
                                        "h_angstrom": h, "rel_diff": abs(ad - fd) / abs(fd)},
+            "jacobian": jacobian,
             "fits": fits, "fit_explained": fit_explained}
 
 
