@@ -8,8 +8,8 @@ namespace csx {
 
 hipError_t launch_debye_jacobian(const float4* pts, const int* orig, const int* type, int npts,
                                  const int* seg_start, int ntypes, const float* qrev,
-                                 const float* Wp, int nq_pad, double* out, hipStream_t stream,
-                                 KernelTimer* timer);
+                                 const float* Wp, int nq_pad, int sin_mode, double* out,
+                                 hipStream_t stream, KernelTimer* timer);
 hipError_t launch_debye_vjp(const float4* pts, const int4* blocks, int nb, const int2* items,
                             size_t nitems, const int* orig, int ntypes, const float* qrev,
                             float qstep, const float* AB, int nq_pad, int qt, int sin_mode,
@@ -193,10 +193,14 @@ static csx_status jacobian_run(Call& call, const csx_structure* s, PointSet* ps,
                          hipMemcpyHostToDevice, st), "hipMemcpyAsync");
   CSX_HIP(hipMemcpyAsync(d_w.as<void>(), Wp.data(), Wp.size() * sizeof(float),
                          hipMemcpyHostToDevice, st), "hipMemcpyAsync");
+  // The unreduced hardware sine takes |q r / 2 pi| <= 256 revolutions.
+  double umax = 0.0;
+  for (size_t k = 0; k < nq; ++k)
+    umax = std::max(umax, std::fabs(static_cast<double>(qrev[k])) * 2.0 * ps->max_radius);
   CSX_HIP(launch_debye_jacobian(ps->pts32.as<float4>(), ps->orig32.as<int>(), ps->type32.as<int>(),
                                 static_cast<int>(n), ps->seg32.as<int>(), T, d_q.as<float>(),
-                                d_w.as<float>(), static_cast<int>(nq_pad), d_out.as<double>(), st,
-                                call.timer()),
+                                d_w.as<float>(), static_cast<int>(nq_pad), umax <= 250.0 ? 2 : 1,
+                                d_out.as<double>(), st, call.timer()),
           "debye_jacobian_f32");
   CSX_HIP(hipMemcpyAsync(out, d_out.as<void>(), nq * 3 * n * sizeof(double), hipMemcpyDeviceToHost,
                          st), "hipMemcpyAsync");

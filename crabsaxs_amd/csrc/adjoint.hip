@@ -19,7 +19,10 @@
 //
 // debye_jacobian_f32 (dI(q)/dx for every q): a workgroup owns 256
 // type-sorted atoms and QT q values and sweeps all other atoms (each pair is
-// visited from both ends; the output is q x n x 3 anyway).
+// visited from both ends; the output is q x n x 3 anyway).  Per pair.q one
+// V_SIN_F32 / V_COS_F32 pair (on q r / 2 pi itself where the host has
+// checked it is at most 250 revolutions, else after v - rint(v)), the weight
+// and three FMAs.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -257,7 +260,7 @@ __global__ void scatter_vjp(const double* __restrict__ gacc, const int4* __restr
   for (int c = 0; c < 3; ++c) out[size_t(o) * 3 + c] = gacc[(size_t(Bk) * 3 + c) * kAdjBlock + tid];
 }
 
-template <int QT>
+template <int QT, int SM>
 __global__ __launch_bounds__(kAdjBlock) void debye_jacobian_f32(
     const float4* __restrict__ pts, const int* __restrict__ orig, const int* __restrict__ type,
     int npts, const int* __restrict__ seg_start, int ntypes, const float* __restrict__ qrev,
@@ -301,16 +304,18 @@ __global__ __launch_bounds__(kAdjBlock) void debye_jacobian_f32(
         const float r = __builtin_sqrtf(use ? d2 : 1.0f);
         const float ir = __builtin_amdgcn_rcpf(r);
         const float ir3 = use ? pj.w * ir * ir * ir : 0.f;
+        // occ_j (x_i - x_j) / r^3 once per pair; per q the weight, the sine
+        // and cosine, and three FMAs
+        const float ex = ir3 * dx, ey = ir3 * dy, ez = ir3 * dz;
 #pragma unroll
         for (int k = 0; k < QT; ++k) {
           const float v = qk[k] * r;  // revolutions
           float sn, cs;
-          sincos_rev<1>(v, sn, cs);
-          const float term = __builtin_fmaf(kTwoPi * v, cs, -sn);
-          const float h = w[k] * term * ir3;
-          g[k][0] = __builtin_fmaf(h, dx, g[k][0]);
-          g[k][1] = __builtin_fmaf(h, dy, g[k][1]);
-          g[k][2] = __builtin_fmaf(h, dz, g[k][2]);
+          sincos_rev<SM>(v, sn, cs);
+          const float h = w[k] * __builtin_fmaf(kTwoPi * v, cs, -sn);
+          g[k][0] = __builtin_fmaf(h, ex, g[k][0]);
+          g[k][1] = __builtin_fmaf(h, ey, g[k][1]);
+          g[k][2] = __builtin_fmaf(h, ez, g[k][2]);
         }
       }
 #pragma unroll
@@ -386,14 +391,18 @@ int vjp_q_tile(int tuned, size_t nq, int sin_mode, size_t nitems, int cus, int m
 
 hipError_t launch_debye_jacobian(const float4* pts, const int* orig, const int* type, int npts,
                                  const int* seg_start, int ntypes, const float* qrev,
-                                 const float* Wp, int nq_pad, double* out, hipStream_t stream,
-                                 KernelTimer* timer) {
+                                 const float* Wp, int nq_pad, int sin_mode, double* out,
+                                 hipStream_t stream, KernelTimer* timer) {
   if (npts <= 0) return hipSuccess;
-  if (nq_pad % kJacQT != 0) return hipErrorInvalidValue;
+  if (nq_pad % kJacQT != 0 || (sin_mode != 1 && sin_mode != 2)) return hipErrorInvalidValue;
   dim3 grid((npts + kAdjBlock - 1) / kAdjBlock, nq_pad / kJacQT);
   TimerScope scope(timer, "debye_jacobian_f32", stream);
-  debye_jacobian_f32<kJacQT><<<grid, kAdjBlock, 0, stream>>>(pts, orig, type, npts, seg_start,
-                                                             ntypes, qrev, Wp, nq_pad, out);
+  if (sin_mode == 2)
+    debye_jacobian_f32<kJacQT, 2><<<grid, kAdjBlock, 0, stream>>>(pts, orig, type, npts, seg_start,
+                                                                  ntypes, qrev, Wp, nq_pad, out);
+  else
+    debye_jacobian_f32<kJacQT, 1><<<grid, kAdjBlock, 0, stream>>>(pts, orig, type, npts, seg_start,
+                                                                  ntypes, qrev, Wp, nq_pad, out);
   return hipGetLastError();
 }
 
