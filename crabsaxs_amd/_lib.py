@@ -174,6 +174,8 @@ SIGNATURES = {
     "csx_pcg32": (C.c_int, [C.c_uint64, C.c_uint64, C.c_size_t, C.c_int, C.POINTER(C.c_uint32)]),
     "csx_ctx_set_vjp_split": (C.c_int, [C.c_void_p, C.c_int]),
     "csx_ctx_set_multipole_group": (C.c_int, [C.c_void_p, C.c_int]),
+    "csx_ctx_set_multipole_radial": (C.c_int, [C.c_void_p, C.c_int]),
+    "csx_ctx_last_multipole": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int)]),
     "csx_ctx_last_vjp_launch": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int),
                                           C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "csx_debye_pair_tables_noalloc": (C.c_int, [C.c_void_p, c_double_p, C.c_size_t, C.c_void_p,

@@ -416,6 +416,18 @@ class Context:
         """Column tiles per multipole tile group (32, 64, 128); 0 picks per call."""
         _check(_lib.load().csx_ctx_set_multipole_group(self.ptr, tiles))
 
+    def set_multipole_radial(self, mode: int = 0) -> None:
+        """Radial interpolation of the multipole coefficients: 0 automatic,
+        1 every point, 2 always (8 Chebyshev nodes per radial bin)."""
+        _check(_lib.load().csx_ctx_set_multipole_radial(self.ptr, mode))
+
+    def last_multipole(self) -> dict:
+        """Radial bin width (0: per point) and points or nodes contracted by
+        the last multipole coefficient computation."""
+        h, n = C.c_double(), C.c_int()
+        _check(_lib.load().csx_ctx_last_multipole(self.ptr, C.byref(h), C.byref(n)))
+        return {"bin_width": h.value, "points": n.value}
+
     def set_vjp_split(self, split: int = 0) -> None:
         """Workgroups per sweep of the coordinate adjoint; 0 picks per call."""
         _check(_lib.load().csx_ctx_set_vjp_split(self.ptr, split))

@@ -1120,6 +1120,24 @@ csx_status csx_ctx_set_multipole_group(csx_ctx* ctx, int tiles) {
   return CSX_OK;
 }
 
+csx_status csx_ctx_set_multipole_radial(csx_ctx* ctx, int mode) {
+  if (!ctx) return fail(CSX_INVALID_INPUT, "context is null");
+  if (mode < 0 || mode > 2)
+    return fail(CSX_INVALID_INPUT,
+                "multipole radial mode must be 0 (automatic), 1 (per point) or 2 (nodes)");
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  ctx->tuning.mp_radial = mode;
+  return CSX_OK;
+}
+
+csx_status csx_ctx_last_multipole(csx_ctx* ctx, double* bin_width, int* points) {
+  if (!ctx) return fail(CSX_INVALID_INPUT, "context is null");
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  if (bin_width) *bin_width = ctx->last_mp_bin;
+  if (points) *points = ctx->last_mp_points;
+  return CSX_OK;
+}
+
 csx_status csx_ctx_set_vjp_split(csx_ctx* ctx, int split) {
   if (!ctx) return fail(CSX_INVALID_INPUT, "context is null");
   if (split != 0 && split != 1 && split != 2 && split != 4 && split != 8)

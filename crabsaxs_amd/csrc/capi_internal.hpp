@@ -5,6 +5,7 @@
 
 #include <atomic>
 #include <map>
+#include <tuple>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -43,6 +44,7 @@ struct Tuning {
   int vjp_q_tile = 0;  // q values per sweep of the coordinate adjoint (0: automatic)
   int vjp_split = 0;   // workgroups per sweep of the coordinate adjoint (0: automatic)
   int mp_group_tiles = 0;  // column tiles per multipole tile group (0: automatic)
+  int mp_radial = 0;  // multipole radial interpolation: 0 automatic, 1 per point, 2 always
   size_t cache_bytes = size_t(256) << 20;  // cached results kept per point set (0: off)
 };
 
@@ -61,6 +63,8 @@ struct csx_ctx {
   int last_items[2] = {0, 0};      // j chunk and work items of the last pair-table launch
   int scratch_status = 0;          // hipError_t of the no-allocation scratch set-up
   int last_vjp[4] = {0, 0, 0, 0};  // q tile, q split, step split, launches
+  double last_mp_bin = 0.0;        // radial bin width of the last multipole geometry used
+  int last_mp_points = 0;          // points (or nodes) it contracts
   // Grow-only device buffers kept per stream (see csx::Scratch), and
   // grow-only pinned host staging buffers per stream; freed with the context.
   std::map<hipStream_t, std::vector<std::pair<void*, size_t>>> scratch, pinned;
@@ -139,13 +143,21 @@ class DevArray {
 
 // Multipole geometry of one point set at one l_max: the points in segments
 // by type (atoms, then sites), each ordered by radius and padded to whole
-// slabs, and the FP32 real spherical harmonics in MFMA B-operand tiles.
+// slabs, and the FP32 real spherical harmonics in MFMA B-operand tiles; or,
+// with radial interpolation, per type the Chebyshev nodes of each radial
+// bin as virtual points and their harmonic moments in the same tiles.
 struct MultipoleGeometry {
   int L = -1, ntiles = 0, npad = 0, npts = 0, nchunk = 0;
   int aexp = 14;  // exponent of the j_l occ operand scale of the coefficient kernel
+  int yexp = 12;  // exponent of the harmonic (or moment) operand scale
   double rmax_atoms = 0.0, rmax_sites = 0.0;
-  // Per padded point: radius, occupancy (0 for padding); chunks: (first
-  // slab, end slab, type) of each one-type chunk of the coefficient kernel.
+  // Radial interpolation: bin width (0: the points themselves) and the
+  // number of virtual points (kNodes per nonempty bin of each type).
+  double bin_width = 0.0;
+  int nvirtual = 0;
+  // Per padded point (or virtual point): radius, occupancy (0 for padding);
+  // chunks: (first slab, end slab, type) of each one-type chunk of the
+  // coefficient kernel.
   DevArray radius, occ, chunks, Y, tile_l;
   std::vector<int> h_tile_l;      // l of each column tile
   std::vector<int> h_type_chunk;  // per type t: its chunks are [h_type_chunk[t], [t + 1])
@@ -181,7 +193,8 @@ struct CacheEntry {
 // sites as an extra type) laid out for both Debye kernels.
 struct PointSet {
   std::mutex geom_mu;
-  std::map<int, std::unique_ptr<MultipoleGeometry>> geoms;  // by l_max
+  // by (l_max, radial bin width (0: per point), radial mode)
+  std::map<std::tuple<int, double, int>, std::unique_ptr<MultipoleGeometry>> geoms;
   int ntypes = 0;         // atom types (+1 when sites are present)
   int natoms = 0, nsites = 0;
   int nbins = 0;          // ntypes (ntypes + 1) / 2

@@ -18,10 +18,25 @@ int multipole_max_types();
 hipError_t launch_multipole_geometry(const double* xyz, int npts, const double* centre, int l_max,
                                      const double* norms, const int* tile_first, int ntiles,
                                      int npad, float* radius, void* Y, hipStream_t stream);
+int multipole_nodes();
+int multipole_harmonic_exponent();
+hipError_t launch_multipole_point_terms(const double* xyz, const double* occ, const int* bin_of,
+                                        int p0, int p1, const double* centre, int l_max,
+                                        const double* norms, double h, const double* node_x,
+                                        const double* node_den, double* terms, double* weights,
+                                        hipStream_t stream);
+hipError_t launch_multipole_bin_moments(const double* terms, const double* weights,
+                                        const int* bins, int nbins, int p0, int nlm,
+                                        double* moments, hipStream_t stream);
+hipError_t launch_multipole_moment_max(const double* moments, size_t n, unsigned long long* out,
+                                       hipStream_t stream);
+hipError_t launch_multipole_moment_tiles(const double* moments, int nvirt, int l_max,
+                                         const int* tile_first, int ntiles, int npad, int yexp,
+                                         void* Y, hipStream_t stream);
 hipError_t launch_multipole_coeff(const float* radius, const float* occ, const int* chunks,
                                   int nchunk, int npad, const void* Y, int ntiles,
                                   const int* tile_l, int tpg, int max_l_range, int aexp,
-                                  const int* type_chunk, int ntypes, const float* qv,
+                                  int yexp, const int* type_chunk, int ntypes, const float* qv,
                                   const int* qlmax, int nqp, float* partial, double* coef,
                                   hipStream_t stream, KernelTimer* timer);
 hipError_t launch_multipole_contract(const double* coef, const double* amp, int ntypes, int nch,
@@ -62,10 +77,40 @@ int adaptive_l_max(double q, double max_radius, int L) {
 
 }  // namespace
 
-static csx_status get_geometry(Call& call, csx_structure* s, PointSet& ps, int L,
-                               MultipoleGeometry** out) {
+// Radial-interpolation choice of a point set for one call: 1 = per point,
+// 2 = nodes always, 0 = nodes where they are fewer than half the points.
+// The bin width is the largest power of two h with q_max h <= 1.04, where
+// 8 Chebyshev nodes per bin interpolate j_l(q r) to 1.0e-9 (0 = per point:
+// tuned off, or q_max beyond 2^10 h-steps).
+static double radial_bin_width(const double* q, size_t nq, int mode) {
+  if (mode == 1) return 0.0;
+  double qmax = 0.0;
+  for (size_t k = 0; k < nq; ++k) qmax = std::max(qmax, std::fabs(q[k]));
+  if (!(qmax > 0.0)) return 128.0;
+  const double h = std::min(128.0, std::exp2(std::floor(std::log2(1.04 / qmax))));
+  return h >= 1.0 / 1024.0 ? h : 0.0;
+}
+
+// Chebyshev nodes of the first kind on [-1, 1] and their Lagrange
+// denominators prod_{k != m} (x_m - x_k).
+static void chebyshev_nodes(int n, std::vector<double>& x, std::vector<double>& den) {
+  x.resize(n);
+  den.resize(n);
+  for (int m = 0; m < n; ++m) x[m] = std::cos((2.0 * m + 1.0) * kPi / (2.0 * n));
+  for (int m = 0; m < n; ++m) {
+    double d = 1.0;
+    for (int k = 0; k < n; ++k)
+      if (k != m) d *= x[m] - x[k];
+    den[m] = d;
+  }
+}
+
+// The multipole geometry of a point set for l_max L and bin width h (0: per
+// point) under radial mode `mode`, built on first use and cached on the set.
+static csx_status get_geometry(Call& call, csx_structure* s, PointSet& ps, int L, double h,
+                               int mode, MultipoleGeometry** out) {
   std::lock_guard<std::mutex> lock(ps.geom_mu);
-  auto& slot = ps.geoms[L];
+  auto& slot = ps.geoms[std::make_tuple(L, h, mode)];
   if (slot) {
     *out = slot.get();
     return CSX_OK;
@@ -125,44 +170,6 @@ static csx_status get_geometry(Call& call, csx_structure* s, PointSet& ps, int L
     }
     key.swap(tmp);
   }
-  // Padded layout and the chunk table (first slab, end slab, type).
-  std::vector<int> chunks;
-  std::vector<double> xyz;
-  std::vector<float> occ;
-  int slab = 0;
-  std::vector<int> type_chunk(T + 2, 0);
-  const int slab_points = multipole_slab();
-  float occ_max = 0.0f;
-  for (int t = 0; t <= T; ++t) {
-    const int count = seg[t + 1] - seg[t];
-    type_chunk[t] = static_cast<int>(chunks.size() / 4);
-    if (count == 0) continue;
-    const int slabs = (count + slab_points - 1) / slab_points;
-    for (int k = 0; k < slabs * slab_points; ++k) {
-      const int idx = static_cast<int>(key[seg[t] + std::min(k, count - 1)] & 0xffffffffu);
-      const double* p = point(idx);
-      xyz.insert(xyz.end(), p, p + 3);
-      const float o = k >= count ? 0.0f : idx < n ? static_cast<float>(s->occ[idx]) : 1.0f;
-      occ.push_back(o);
-      occ_max = std::max(occ_max, std::fabs(o));
-    }
-    const int per = multipole_chunk() / slab_points;
-    for (int c = 0; c < slabs; c += per)
-      chunks.insert(chunks.end(), {slab + c, slab + std::min(slabs, c + per), t, 0});
-    slab += slabs;
-  }
-  g->npad = slab * slab_points;
-  // j_l occ enters the FP16 contraction times 2^aexp: the largest occupancy
-  // maps below 2^14 (occupancy 1 -> 2^14, j_l <= 1).
-  g->aexp = 14;
-  if (occ_max > 1.0f) g->aexp -= static_cast<int>(std::ceil(std::log2(static_cast<double>(occ_max))));
-  if (!(occ_max < 1e30f)) return fail(CSX_INVALID_INPUT, "occupancy not finite");
-  g->nchunk = static_cast<int>(chunks.size() / 4);
-  type_chunk[T + 1] = g->nchunk;
-  // Types of the point set: the atom types, then the site type when the set
-  // has sites; the chunk table keeps the site chunks last.
-  if (ps.nsites == 0) type_chunk.pop_back();
-  g->h_type_chunk = type_chunk;
   for (int i = 0; i < npts; ++i) {
     const double* p = point(i);
     double r2 = 0.0;
@@ -173,38 +180,235 @@ static csx_status get_geometry(Call& call, csx_structure* s, PointSet& ps, int L
     double& rm = i < n ? g->rmax_atoms : g->rmax_sites;
     rm = std::max(rm, std::sqrt(r2));
   }
+  // Radial bins: per type, the runs of points whose sort-key radius falls in
+  // one [b h, (b + 1) h); each bin becomes kNodes virtual points.
+  struct Bin {
+    int t, b, p0, p1;
+  };
+  std::vector<Bin> bins;
+  std::vector<int> bin_of;
+  const int nodes = multipole_nodes();
+  if (h > 0.0) {
+    bin_of.resize(npts);
+    for (int t = 0; t <= T; ++t)
+      for (int k = seg[t]; k < seg[t + 1]; ++k) {
+        float r2f;
+        const uint32_t bits = static_cast<uint32_t>(key[k] >> 32);
+        std::memcpy(&r2f, &bits, sizeof r2f);
+        const int b = static_cast<int>(std::floor(std::sqrt(static_cast<double>(r2f)) / h));
+        if (bins.empty() || bins.back().t != t || bins.back().b != b) bins.push_back({t, b, k, k});
+        bins.back().p1 = k + 1;
+        bin_of[k] = b;
+      }
+    const size_t nvirt = bins.size() * static_cast<size_t>(nodes);
+    if (bins.size() > 65535 || (mode != 2 && 2 * nvirt > static_cast<size_t>(npts))) {
+      bins.clear();
+      h = 0.0;
+    }
+  }
+  g->bin_width = h;
   const std::vector<double> norms = harmonic_norms(L);
   const hipStream_t st = call.stream();
-  DevBuf d_xyz, d_centre, d_norms, d_first;
-  CSX_HIP(d_xyz.alloc(xyz.size() * sizeof(double), st), "hipMallocAsync");
-  CSX_HIP(d_centre.alloc(3 * sizeof(double), st), "hipMallocAsync");
-  CSX_HIP(d_norms.alloc(norms.size() * sizeof(double), st), "hipMallocAsync");
-  CSX_HIP(d_first.alloc(tile_first.size() * sizeof(int), st), "hipMallocAsync");
-  CSX_HIP(hipMemcpyAsync(d_xyz.as<void>(), xyz.data(), xyz.size() * sizeof(double),
-                         hipMemcpyHostToDevice, st), "hipMemcpyAsync");
-  CSX_HIP(hipMemcpyAsync(d_centre.as<void>(), s->centre, 3 * sizeof(double),
-                         hipMemcpyHostToDevice, st), "hipMemcpyAsync");
-  CSX_HIP(hipMemcpyAsync(d_norms.as<void>(), norms.data(), norms.size() * sizeof(double),
-                         hipMemcpyHostToDevice, st), "hipMemcpyAsync");
-  CSX_HIP(hipMemcpyAsync(d_first.as<void>(), tile_first.data(), tile_first.size() * sizeof(int),
-                         hipMemcpyHostToDevice, st), "hipMemcpyAsync");
-  CSX_HIP(g->radius.alloc(g->npad * sizeof(float)), "hipMalloc radii");
-  CSX_HIP(g->occ.upload(occ.data(), occ.size() * sizeof(float)), "upload occupancy");
-  CSX_HIP(g->chunks.upload(chunks.data(), chunks.size() * sizeof(int)), "upload chunks");
-  CSX_HIP(g->tile_l.upload(tile_l.data(), tile_l.size() * sizeof(int)), "upload tiles");
-  CSX_HIP(g->type_chunk.upload(type_chunk.data(), type_chunk.size() * sizeof(int)),
-          "upload type chunks");
-  const size_t ybytes = static_cast<size_t>(g->ntiles) * g->npad * 16 * sizeof(float);
-  CSX_HIP(g->Y.alloc(ybytes), "hipMalloc harmonics");
-  CSX_HIP(launch_multipole_geometry(d_xyz.as<double>(), g->npad, d_centre.as<double>(), L,
-                                    d_norms.as<double>(), d_first.as<int>(), g->ntiles, g->npad,
-                                    g->radius.as<float>(), g->Y.as<void>(), st),
-          "multipole_geometry");
-  d_xyz.reset();
-  d_centre.reset();
-  d_norms.reset();
-  d_first.reset();
-  CSX_HIP(hipStreamSynchronize(st), "hipStreamSynchronize");
+  if (h == 0.0) {
+    // Padded layout and the chunk table (first slab, end slab, type).
+    std::vector<int> chunks;
+    std::vector<double> xyz;
+    std::vector<float> occ;
+    int slab = 0;
+    std::vector<int> type_chunk(T + 2, 0);
+    const int slab_points = multipole_slab();
+    float occ_max = 0.0f;
+    for (int t = 0; t <= T; ++t) {
+      const int count = seg[t + 1] - seg[t];
+      type_chunk[t] = static_cast<int>(chunks.size() / 4);
+      if (count == 0) continue;
+      const int slabs = (count + slab_points - 1) / slab_points;
+      for (int k = 0; k < slabs * slab_points; ++k) {
+        const int idx = static_cast<int>(key[seg[t] + std::min(k, count - 1)] & 0xffffffffu);
+        const double* p = point(idx);
+        xyz.insert(xyz.end(), p, p + 3);
+        const float o = k >= count ? 0.0f : idx < n ? static_cast<float>(s->occ[idx]) : 1.0f;
+        occ.push_back(o);
+        occ_max = std::max(occ_max, std::fabs(o));
+      }
+      const int per = multipole_chunk() / slab_points;
+      for (int c = 0; c < slabs; c += per)
+        chunks.insert(chunks.end(), {slab + c, slab + std::min(slabs, c + per), t, 0});
+      slab += slabs;
+    }
+    g->npad = slab * slab_points;
+    // j_l occ enters the FP16 contraction times 2^aexp: the largest occupancy
+    // maps below 2^14 (occupancy 1 -> 2^14, j_l <= 1).
+    g->aexp = 14;
+    if (occ_max > 1.0f) g->aexp -= static_cast<int>(std::ceil(std::log2(static_cast<double>(occ_max))));
+    if (!(occ_max < 1e30f)) return fail(CSX_INVALID_INPUT, "occupancy not finite");
+    g->nchunk = static_cast<int>(chunks.size() / 4);
+    type_chunk[T + 1] = g->nchunk;
+    // Types of the point set: the atom types, then the site type when the set
+    // has sites; the chunk table keeps the site chunks last.
+    if (ps.nsites == 0) type_chunk.pop_back();
+    g->h_type_chunk = type_chunk;
+    DevBuf d_xyz, d_centre, d_norms, d_first;
+    CSX_HIP(d_xyz.alloc(xyz.size() * sizeof(double), st), "hipMallocAsync");
+    CSX_HIP(d_centre.alloc(3 * sizeof(double), st), "hipMallocAsync");
+    CSX_HIP(d_norms.alloc(norms.size() * sizeof(double), st), "hipMallocAsync");
+    CSX_HIP(d_first.alloc(tile_first.size() * sizeof(int), st), "hipMallocAsync");
+    CSX_HIP(hipMemcpyAsync(d_xyz.as<void>(), xyz.data(), xyz.size() * sizeof(double),
+                           hipMemcpyHostToDevice, st), "hipMemcpyAsync");
+    CSX_HIP(hipMemcpyAsync(d_centre.as<void>(), s->centre, 3 * sizeof(double),
+                           hipMemcpyHostToDevice, st), "hipMemcpyAsync");
+    CSX_HIP(hipMemcpyAsync(d_norms.as<void>(), norms.data(), norms.size() * sizeof(double),
+                           hipMemcpyHostToDevice, st), "hipMemcpyAsync");
+    CSX_HIP(hipMemcpyAsync(d_first.as<void>(), tile_first.data(), tile_first.size() * sizeof(int),
+                           hipMemcpyHostToDevice, st), "hipMemcpyAsync");
+    CSX_HIP(g->radius.alloc(g->npad * sizeof(float)), "hipMalloc radii");
+    CSX_HIP(g->occ.upload(occ.data(), occ.size() * sizeof(float)), "upload occupancy");
+    CSX_HIP(g->chunks.upload(chunks.data(), chunks.size() * sizeof(int)), "upload chunks");
+    CSX_HIP(g->tile_l.upload(tile_l.data(), tile_l.size() * sizeof(int)), "upload tiles");
+    CSX_HIP(g->type_chunk.upload(type_chunk.data(), type_chunk.size() * sizeof(int)),
+            "upload type chunks");
+    const size_t ybytes = static_cast<size_t>(g->ntiles) * g->npad * 16 * sizeof(float);
+    CSX_HIP(g->Y.alloc(ybytes), "hipMalloc harmonics");
+    CSX_HIP(launch_multipole_geometry(d_xyz.as<double>(), g->npad, d_centre.as<double>(), L,
+                                      d_norms.as<double>(), d_first.as<int>(), g->ntiles, g->npad,
+                                      g->radius.as<float>(), g->Y.as<void>(), st),
+            "multipole_geometry");
+    d_xyz.reset();
+    d_centre.reset();
+    d_norms.reset();
+    d_first.reset();
+    CSX_HIP(hipStreamSynchronize(st), "hipStreamSynchronize");
+    g->yexp = multipole_harmonic_exponent();
+  } else {
+    // Virtual points per type: kNodes per bin at the bin's Chebyshev nodes,
+    // occupancy 1 (the point occupancies are in the moments), padded to
+    // whole slabs with zero occupancy; chunks of one slab.
+    std::vector<double> nx, nden;
+    chebyshev_nodes(nodes, nx, nden);
+    const int slab_points = multipole_slab();
+    std::vector<int> chunks, hbins, type_chunk(T + 2, 0);
+    std::vector<float> vrad, vocc;
+    int vslab = 0;
+    size_t bi = 0;
+    for (int t = 0; t <= T; ++t) {
+      type_chunk[t] = static_cast<int>(chunks.size() / 4);
+      const size_t b0 = bi;
+      while (bi < bins.size() && bins[bi].t == t) ++bi;
+      const int count = static_cast<int>(bi - b0) * nodes;
+      if (count == 0) continue;
+      const int slabs = (count + slab_points - 1) / slab_points;
+      for (size_t b = b0; b < bi; ++b) {
+        hbins.insert(hbins.end(), {bins[b].p0, bins[b].p1, static_cast<int>(vrad.size()), 0});
+        for (int m = 0; m < nodes; ++m) {
+          vrad.push_back(static_cast<float>((bins[b].b + 0.5) * h + 0.5 * h * nx[m]));
+          vocc.push_back(1.0f);
+        }
+      }
+      for (int k = count; k < slabs * slab_points; ++k) {
+        vrad.push_back(vrad.back());
+        vocc.push_back(0.0f);
+      }
+      for (int c = 0; c < slabs; ++c) chunks.insert(chunks.end(), {vslab + c, vslab + c + 1, t, 0});
+      vslab += slabs;
+    }
+    g->npad = vslab * slab_points;
+    g->aexp = 14;  // occupancy 1: j_l maps below 2^14
+    g->nchunk = static_cast<int>(chunks.size() / 4);
+    type_chunk[T + 1] = g->nchunk;
+    if (ps.nsites == 0) type_chunk.pop_back();
+    g->h_type_chunk = type_chunk;
+    // Real points in the sorted order, FP64 coordinates and occupancies.
+    std::vector<double> xyz(3 * size_t(npts)), occ(npts);
+    for (int k = 0; k < npts; ++k) {
+      const int idx = static_cast<int>(key[k] & 0xffffffffu);
+      const double* p = point(idx);
+      std::copy(p, p + 3, xyz.begin() + 3 * size_t(k));
+      occ[k] = idx < n ? s->occ[idx] : 1.0;
+      if (!std::isfinite(occ[k])) return fail(CSX_INVALID_INPUT, "occupancy not finite");
+    }
+    const int nlm = (L + 1) * (L + 1);
+    // Point chunks of whole bins, each with at most ~256 MB of FP64 terms
+    // (or one bin).
+    const size_t cap = std::max<size_t>(1, (size_t(256) << 20) / (size_t(nlm) * sizeof(double)));
+    std::vector<std::pair<size_t, size_t>> pchunks;  // bin ranges
+    size_t maxpts = 0;
+    for (size_t b = 0; b < bins.size();) {
+      size_t e = b, pts = 0;
+      while (e < bins.size() && (e == b || pts + size_t(bins[e].p1 - bins[e].p0) <= cap))
+        pts += size_t(bins[e].p1 - bins[e].p0), ++e;
+      pchunks.push_back({b, e});
+      maxpts = std::max(maxpts, pts);
+      b = e;
+    }
+    DevBuf d_xyz, d_occ, d_bin, d_centre, d_norms, d_first, d_bins, d_terms, d_w, d_mom, d_max;
+    CSX_HIP(d_xyz.alloc(xyz.size() * sizeof(double), st), "hipMallocAsync");
+    CSX_HIP(d_occ.alloc(occ.size() * sizeof(double), st), "hipMallocAsync");
+    CSX_HIP(d_bin.alloc(bin_of.size() * sizeof(int), st), "hipMallocAsync");
+    CSX_HIP(d_centre.alloc(3 * sizeof(double), st), "hipMallocAsync");
+    CSX_HIP(d_norms.alloc(norms.size() * sizeof(double), st), "hipMallocAsync");
+    CSX_HIP(d_first.alloc(tile_first.size() * sizeof(int), st), "hipMallocAsync");
+    CSX_HIP(d_bins.alloc(hbins.size() * sizeof(int), st), "hipMallocAsync");
+    CSX_HIP(d_terms.alloc(maxpts * nlm * sizeof(double), st), "hipMallocAsync");
+    CSX_HIP(d_w.alloc(maxpts * nodes * sizeof(double), st), "hipMallocAsync");
+    CSX_HIP(d_mom.alloc(size_t(g->npad) * nlm * sizeof(double), st), "hipMallocAsync");
+    CSX_HIP(d_max.alloc(sizeof(unsigned long long), st), "hipMallocAsync");
+    CSX_HIP(hipMemsetAsync(d_mom.as<void>(), 0, size_t(g->npad) * nlm * sizeof(double), st),
+            "hipMemsetAsync");
+    CSX_HIP(hipMemcpyAsync(d_xyz.as<void>(), xyz.data(), xyz.size() * sizeof(double),
+                           hipMemcpyHostToDevice, st), "hipMemcpyAsync");
+    CSX_HIP(hipMemcpyAsync(d_occ.as<void>(), occ.data(), occ.size() * sizeof(double),
+                           hipMemcpyHostToDevice, st), "hipMemcpyAsync");
+    CSX_HIP(hipMemcpyAsync(d_bin.as<void>(), bin_of.data(), bin_of.size() * sizeof(int),
+                           hipMemcpyHostToDevice, st), "hipMemcpyAsync");
+    CSX_HIP(hipMemcpyAsync(d_centre.as<void>(), s->centre, 3 * sizeof(double),
+                           hipMemcpyHostToDevice, st), "hipMemcpyAsync");
+    CSX_HIP(hipMemcpyAsync(d_norms.as<void>(), norms.data(), norms.size() * sizeof(double),
+                           hipMemcpyHostToDevice, st), "hipMemcpyAsync");
+    CSX_HIP(hipMemcpyAsync(d_first.as<void>(), tile_first.data(), tile_first.size() * sizeof(int),
+                           hipMemcpyHostToDevice, st), "hipMemcpyAsync");
+    CSX_HIP(hipMemcpyAsync(d_bins.as<void>(), hbins.data(), hbins.size() * sizeof(int),
+                           hipMemcpyHostToDevice, st), "hipMemcpyAsync");
+    for (const auto& pc : pchunks) {
+      const int p0 = bins[pc.first].p0, p1 = bins[pc.second - 1].p1;
+      CSX_HIP(launch_multipole_point_terms(d_xyz.as<double>(), d_occ.as<double>(), d_bin.as<int>(),
+                                           p0, p1, d_centre.as<double>(), L, d_norms.as<double>(),
+                                           h, nx.data(), nden.data(), d_terms.as<double>(),
+                                           d_w.as<double>(), st),
+              "multipole_point_terms");
+      CSX_HIP(launch_multipole_bin_moments(d_terms.as<double>(), d_w.as<double>(),
+                                           d_bins.as<int>() + 4 * pc.first,
+                                           static_cast<int>(pc.second - pc.first), p0, nlm,
+                                           d_mom.as<double>(), st),
+              "multipole_bin_moments");
+    }
+    // The moments enter the FP16 contraction times 2^yexp: the largest maps
+    // below 2^14.
+    unsigned long long mbits = 0;
+    CSX_HIP(launch_multipole_moment_max(d_mom.as<double>(), size_t(g->npad) * nlm,
+                                        d_max.as<unsigned long long>(), st),
+            "multipole_moment_max");
+    CSX_HIP(hipMemcpyAsync(&mbits, d_max.as<void>(), sizeof mbits, hipMemcpyDeviceToHost, st),
+            "hipMemcpyAsync");
+    CSX_HIP(hipStreamSynchronize(st), "hipStreamSynchronize");
+    double mmax;
+    std::memcpy(&mmax, &mbits, sizeof mmax);
+    if (!std::isfinite(mmax)) return fail(CSX_INVALID_INPUT, "multipole moments not finite");
+    g->yexp = mmax > 0.0 ? std::max(-60, std::min(60, static_cast<int>(std::floor(std::log2(16384.0 / mmax)))))
+                         : multipole_harmonic_exponent();
+    const size_t ybytes = static_cast<size_t>(g->ntiles) * g->npad * 16 * sizeof(float);
+    CSX_HIP(g->Y.alloc(ybytes), "hipMalloc harmonics");
+    CSX_HIP(launch_multipole_moment_tiles(d_mom.as<double>(), g->npad, L, d_first.as<int>(),
+                                          g->ntiles, g->npad, g->yexp, g->Y.as<void>(), st),
+            "multipole_moment_tiles");
+    CSX_HIP(g->radius.upload(vrad.data(), vrad.size() * sizeof(float)), "upload node radii");
+    CSX_HIP(g->occ.upload(vocc.data(), vocc.size() * sizeof(float)), "upload node occupancy");
+    CSX_HIP(g->chunks.upload(chunks.data(), chunks.size() * sizeof(int)), "upload chunks");
+    CSX_HIP(g->tile_l.upload(tile_l.data(), tile_l.size() * sizeof(int)), "upload tiles");
+    CSX_HIP(g->type_chunk.upload(type_chunk.data(), type_chunk.size() * sizeof(int)),
+            "upload type chunks");
+    g->nvirtual = static_cast<int>(bins.size()) * nodes;
+    CSX_HIP(hipStreamSynchronize(st), "hipStreamSynchronize");
+  }
   *out = g.get();
   slot = std::move(g);
   return CSX_OK;
@@ -218,6 +422,7 @@ static csx_status type_coefficients(Call& call, PointSet& ps, const MultipoleGeo
                                     std::shared_ptr<CacheEntry>* out,
                                     std::unique_lock<std::mutex>* lock) {
   std::string key = "multipole:" + std::to_string(g.L) + ":";
+  key.append(reinterpret_cast<const char*>(&g.bin_width), sizeof g.bin_width);
   key.append(reinterpret_cast<const char*>(q), nq * sizeof(double));
   std::shared_ptr<CacheEntry> e = cache_acquire(call.ctx(), ps, key, lock);
   *out = e;
@@ -267,7 +472,8 @@ static csx_status type_coefficients(Call& call, PointSet& ps, const MultipoleGeo
     return fail(CSX_INVALID_INPUT, "multipole l_max too large for the device's LDS");
   CSX_HIP(launch_multipole_coeff(g.radius.as<float>(), g.occ.as<float>(), g.chunks.as<int>(),
                                  g.nchunk, g.npad, g.Y.as<void>(), g.ntiles, g.tile_l.as<int>(),
-                                 tpg, l_range, g.aexp, g.type_chunk.as<int>(), TT, d_q.as<float>(),
+                                 tpg, l_range, g.aexp, g.yexp, g.type_chunk.as<int>(), TT,
+                                 d_q.as<float>(),
                                  e->qlmax.as<int>(), static_cast<int>(nqp), d_partial.as<float>(),
                                  e->coef.as<double>(), st, call.timer()),
           "multipole_coeff");
@@ -297,8 +503,18 @@ static csx_status multipole_run(csx_structure* s, const double* q, size_t nq, in
   CSX_TRY(get_point_set(s, mode == 0 ? nullptr : p, &ps));
   if (ps->ntypes > multipole_max_types())
     return fail(CSX_INVALID_INPUT, "too many form-factor types for the multipole path");
+  int radial;
+  {
+    std::lock_guard<std::mutex> lock(call.ctx()->mu);
+    radial = call.ctx()->tuning.mp_radial;
+  }
   MultipoleGeometry* g = nullptr;
-  CSX_TRY(get_geometry(call, s, *ps, L, &g));
+  CSX_TRY(get_geometry(call, s, *ps, L, radial_bin_width(q, nq, radial), radial, &g));
+  {
+    std::lock_guard<std::mutex> lock(call.ctx()->mu);
+    call.ctx()->last_mp_bin = g->bin_width;
+    call.ctx()->last_mp_points = g->bin_width > 0.0 ? g->nvirtual : g->npts;
+  }
   const bool grid = mode != 0 && p->hydration_model != CSX_HYDRATION_ANALYTIC;
   const double rmax = grid ? std::max(g->rmax_atoms, g->rmax_sites) : g->rmax_atoms;
   std::shared_ptr<CacheEntry> coef;

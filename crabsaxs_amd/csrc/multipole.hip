@@ -48,6 +48,7 @@ constexpr int kQB = 16;      // q values per MFMA row block
 constexpr int kThreads = 256;
 constexpr int kChunk = 512;  // points per workgroup (16 slabs)
 constexpr int kFrag = 512;   // FP16 values in one operand fragment (64 lanes x 8)
+constexpr int kNodes = 8;    // Chebyshev nodes per radial bin
 
 // FP32 operands as FP16 pairs x = hi + lo (22 significant bits together).
 // The harmonics are stored times 2^12 (|Y_lm| <= sqrt((2l + 1) / 4 pi) stays
@@ -78,16 +79,14 @@ __device__ inline void split_pair(float v0, float v1, uint32_t& hi, uint32_t& lo
 }
 
 // ------------------------------------------------------------ geometry --
-__global__ void multipole_geometry(const double* __restrict__ xyz, int npts,
-                                   const double* __restrict__ centre3, int l_max,
-                                   const double* __restrict__ norms, const int* __restrict__ tile_first,
-                                   int npad, float* __restrict__ radius, _Float16* __restrict__ Y) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= npts) return;
-  const double px = xyz[3 * i] - centre3[0], py = xyz[3 * i + 1] - centre3[1],
-               pz = xyz[3 * i + 2] - centre3[2];
+// The real harmonics of one point (offset p from the centre) by the
+// associated-Legendre recurrence of spherical.rs:54-93, in FP64:
+// store(l, column, Y) with column = l + m for the cos part and l - m for the
+// sin part (m > 0), column l for m = 0.  Returns the radius.
+template <class Store>
+__device__ inline double point_harmonics(double px, double py, double pz, int l_max,
+                                         const double* __restrict__ norms, Store store) {
   const double r = sqrt(px * px + py * py + pz * pz);
-  radius[i] = float(r);
   double x = 1.0, phi = 0.0;
   if (!(r < 1e-12)) {
     x = fmin(fmax(pz / r, -1.0), 1.0);
@@ -97,17 +96,6 @@ __global__ void multipole_geometry(const double* __restrict__ xyz, int npts,
   const double sp = sin(phi), cp = cos(phi);
   double sm = 0.0, cm = 1.0, pmm = 1.0;
   const double sqrt2 = 1.4142135623730951;
-  // Y(l, column) with column = l + m (cos part) or l - m (sin part).
-  const int nslab = npad / kSlab;
-  auto store = [&](int l, int col, double v) {
-    const int t = tile_first[l] + col / 16;
-    const double y = v * double(1 << kYExp);
-    const _Float16 hi = static_cast<_Float16>(static_cast<float>(y));
-    const _Float16 lo = static_cast<_Float16>(static_cast<float>(y - double(hi)));
-    const size_t h = (size_t(t) * nslab + i / kSlab) * (2 * kFrag) + frag_half(i % kSlab, col & 15);
-    Y[h] = hi;
-    Y[h + kFrag] = lo;
-  };
   for (int m = 0; m <= l_max; ++m) {
     if (m > 0) {
       pmm *= -double(2 * m - 1) * st;
@@ -137,8 +125,135 @@ __global__ void multipole_geometry(const double* __restrict__ xyz, int npts,
       cur = next;
     }
   }
+  return r;
 }
 
+// Per point: its radius and its harmonics as FP16 (hi, lo) pairs times
+// 2^kYExp in the column tiles.
+__global__ void multipole_geometry(const double* __restrict__ xyz, int npts,
+                                   const double* __restrict__ centre3, int l_max,
+                                   const double* __restrict__ norms, const int* __restrict__ tile_first,
+                                   int npad, float* __restrict__ radius, _Float16* __restrict__ Y) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= npts) return;
+  const int nslab = npad / kSlab;
+  const double r = point_harmonics(
+      xyz[3 * i] - centre3[0], xyz[3 * i + 1] - centre3[1], xyz[3 * i + 2] - centre3[2], l_max,
+      norms, [&](int l, int col, double v) {
+        const int t = tile_first[l] + col / 16;
+        const double y = v * double(1 << kYExp);
+        const _Float16 hi = static_cast<_Float16>(static_cast<float>(y));
+        const _Float16 lo = static_cast<_Float16>(static_cast<float>(y - double(hi)));
+        const size_t h = (size_t(t) * nslab + i / kSlab) * (2 * kFrag) + frag_half(i % kSlab, col & 15);
+        Y[h] = hi;
+        Y[h + kFrag] = lo;
+      });
+  radius[i] = float(r);
+}
+
+// ------------------------------------------- radial interpolation nodes --
+// Radial bins of width h (from the centre) hold consecutive points of one
+// type.  Within a bin, j_l(q r) is replaced by its interpolant at the
+// kNodes Chebyshev nodes of the bin: sum_i occ_i j_l(q r_i) Y_lm,i =
+// sum_m j_l(q rho_m) M_m,lm with the moments M_m,lm = sum_i occ_i w_m(r_i)
+// Y_lm,i (w_m: the Lagrange basis of the nodes).  |d^n j_l(x) / dx^n| <= 1,
+// so the error is at most (q h / 2)^n / (2^(n-1) n!) per point (1.0e-9 at
+// n = 8, q h = 1.04).
+
+// Chebyshev nodes on [-1, 1] and the Lagrange denominators prod_{k != m}
+// (x_m - x_k), from the host (multipole_chebyshev_nodes).
+struct NodeTable {
+  double x[kNodes], den[kNodes];
+};
+
+// occ_i Y_lm,i (FP64, point-major rows of nlm) and the point's kNodes
+// Lagrange weights at u = (r - centre of its bin) / (h / 2), for the sorted
+// real points [p0, p1).
+__global__ void multipole_point_terms(const double* __restrict__ xyz, const double* __restrict__ occ,
+                                      const int* __restrict__ bin_of, int p0, int p1,
+                                      const double* __restrict__ centre3, int l_max,
+                                      const double* __restrict__ norms, double h, NodeTable nodes,
+                                      double* __restrict__ terms, double* __restrict__ weights) {
+  const int i = p0 + blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= p1) return;
+  const int nlm = (l_max + 1) * (l_max + 1);
+  const double o = occ[i];
+  double* row = terms + size_t(i - p0) * nlm;
+  const double r = point_harmonics(xyz[3 * i] - centre3[0], xyz[3 * i + 1] - centre3[1],
+                                   xyz[3 * i + 2] - centre3[2], l_max, norms,
+                                   [&](int l, int col, double v) { row[l * l + col] = o * v; });
+  const double u = (r - (double(bin_of[i]) + 0.5) * h) / (0.5 * h);
+  double pre[kNodes + 1], suf[kNodes + 1];
+  pre[0] = 1.0;
+  suf[kNodes] = 1.0;
+#pragma unroll
+  for (int k = 0; k < kNodes; ++k) pre[k + 1] = pre[k] * (u - nodes.x[k]);
+#pragma unroll
+  for (int k = kNodes - 1; k >= 0; --k) suf[k] = suf[k + 1] * (u - nodes.x[k]);
+#pragma unroll
+  for (int m = 0; m < kNodes; ++m)
+    weights[size_t(i - p0) * kNodes + m] = pre[m] * suf[m + 1] / nodes.den[m];
+}
+
+// moments[(v0 + m) * nlm + lm] = sum over the bin's points, in order, of
+// weight_m * term_lm.  One thread per (bin, lm); bins: (first point, end
+// point, first virtual point, 0), all inside [p0, ...).
+__global__ void multipole_bin_moments(const double* __restrict__ terms,
+                                      const double* __restrict__ weights,
+                                      const int4* __restrict__ bins, int p0, int nlm,
+                                      double* __restrict__ moments) {
+  const int lm = blockIdx.x * blockDim.x + threadIdx.x;
+  if (lm >= nlm) return;
+  const int4 b = bins[blockIdx.y];
+  double acc[kNodes];
+#pragma unroll
+  for (int m = 0; m < kNodes; ++m) acc[m] = 0.0;
+  for (int i = b.x; i < b.y; ++i) {
+    const double t = terms[size_t(i - p0) * nlm + lm];
+#pragma unroll
+    for (int m = 0; m < kNodes; ++m) acc[m] = fma(weights[size_t(i - p0) * kNodes + m], t, acc[m]);
+  }
+#pragma unroll
+  for (int m = 0; m < kNodes; ++m) moments[size_t(b.z + m) * nlm + lm] = acc[m];
+}
+
+// *out = max |v| over v[0, n) as the bit pattern of a non-negative double
+// (such patterns order like the values); *out starts at 0.
+__global__ void abs_max_f64(const double* __restrict__ v, size_t n,
+                            unsigned long long* __restrict__ out) {
+  double m = 0.0;
+  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += size_t(gridDim.x) * blockDim.x)
+    m = fmax(m, fabs(v[i]));
+  for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
+  if ((threadIdx.x & 63) == 0) atomicMax(out, __double_as_longlong(m));
+}
+
+__global__ void zero_u64(unsigned long long* p) { *p = 0ull; }
+
+// The moments of virtual point v as FP16 (hi, lo) pairs times 2^yexp in the
+// column tiles (the B operand layout of multipole_geometry).
+__global__ void multipole_moment_tiles(const double* __restrict__ moments, int nvirt, int l_max,
+                                       const int* __restrict__ tile_first, int npad, int yexp,
+                                       _Float16* __restrict__ Y) {
+  const int nlm = (l_max + 1) * (l_max + 1);
+  const int lm = blockIdx.x * blockDim.x + threadIdx.x;
+  if (lm >= nlm) return;
+  int l = int(sqrtf(float(lm)));
+  while (l * l > lm) --l;
+  while ((l + 1) * (l + 1) <= lm) ++l;
+  const int col = lm - l * l;
+  const int nslab = npad / kSlab;
+  const int t = tile_first[l] + col / 16;
+  for (int v = blockIdx.y; v < nvirt; v += gridDim.y) {
+    const double y = ldexp(moments[size_t(v) * nlm + lm], yexp);
+    const _Float16 hi = static_cast<_Float16>(static_cast<float>(y));
+    const _Float16 lo = static_cast<_Float16>(static_cast<float>(y - double(hi)));
+    const size_t hh = (size_t(t) * nslab + v / kSlab) * (2 * kFrag) + frag_half(v % kSlab, col & 15);
+    Y[hh] = hi;
+    Y[hh + kFrag] = lo;
+  }
+}
 
 __global__ void fill_zero(_Float16* __restrict__ p, size_t n) {
   const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -421,8 +536,8 @@ __device__ inline bool bessel_pair(float x0f, float x1f, int lq, int l_lo, int l
 }
 
 // partial[(chunk * nqp + q) * ncol + col]: the sum over the chunk's points
-// (all of one type) of occ j_l Y, times 2^-(kYExp + aexp) where ascale =
-// 2^aexp scales the j_l occ operands.
+// (all of one type) of occ j_l Y, times 2^-(yexp + aexp) where ascale =
+// 2^aexp scales the j_l occ operands and 2^yexp the harmonic operands.
 // At least WPE waves per SIMD: 2 (256 registers), 3 (168) or 4 (128).
 template <int MAXT, int WPE>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(WPE))) void multipole_coeff(
@@ -676,6 +791,59 @@ constexpr int kGroupTiles[] = {32, 48, 64, 128};
 
 }  // namespace
 
+int multipole_nodes() { return kNodes; }
+
+int multipole_harmonic_exponent() { return kYExp; }
+
+hipError_t launch_multipole_point_terms(const double* xyz, const double* occ, const int* bin_of,
+                                        int p0, int p1, const double* centre, int l_max,
+                                        const double* norms, double h, const double* node_x,
+                                        const double* node_den, double* terms, double* weights,
+                                        hipStream_t stream) {
+  if (p1 <= p0) return hipSuccess;
+  if (!(h > 0.0)) return hipErrorInvalidValue;
+  NodeTable nodes;
+  for (int k = 0; k < kNodes; ++k) {
+    nodes.x[k] = node_x[k];
+    nodes.den[k] = node_den[k];
+  }
+  multipole_point_terms<<<(p1 - p0 + 127) / 128, 128, 0, stream>>>(
+      xyz, occ, bin_of, p0, p1, centre, l_max, norms, h, nodes, terms, weights);
+  return hipGetLastError();
+}
+
+hipError_t launch_multipole_bin_moments(const double* terms, const double* weights,
+                                        const int* bins, int nbins, int p0, int nlm,
+                                        double* moments, hipStream_t stream) {
+  if (nbins <= 0) return hipSuccess;
+  if (nbins > 65535) return hipErrorInvalidValue;
+  multipole_bin_moments<<<dim3((nlm + 255) / 256, nbins), 256, 0, stream>>>(
+      terms, weights, reinterpret_cast<const int4*>(bins), p0, nlm, moments);
+  return hipGetLastError();
+}
+
+hipError_t launch_multipole_moment_max(const double* moments, size_t n, unsigned long long* out,
+                                       hipStream_t stream) {
+  zero_u64<<<1, 1, 0, stream>>>(out);
+  const size_t blocks = std::min<size_t>((n + 255) / 256, 2048);
+  if (blocks > 0) abs_max_f64<<<unsigned(blocks), 256, 0, stream>>>(moments, n, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_multipole_moment_tiles(const double* moments, int nvirt, int l_max,
+                                         const int* tile_first, int ntiles, int npad, int yexp,
+                                         void* Yraw, hipStream_t stream) {
+  _Float16* Y = static_cast<_Float16*>(Yraw);
+  if (npad % kSlab != 0 || nvirt > npad) return hipErrorInvalidValue;
+  const size_t ny = size_t(ntiles) * npad * 32;  // 16 columns x (hi, lo) per point
+  fill_zero<<<(ny + 255) / 256, 256, 0, stream>>>(Y, ny);
+  const int nlm = (l_max + 1) * (l_max + 1);
+  if (nvirt > 0)
+    multipole_moment_tiles<<<dim3((nlm + 255) / 256, std::min(nvirt, 4096)), 256, 0, stream>>>(
+        moments, nvirt, l_max, tile_first, npad, yexp, Y);
+  return hipGetLastError();
+}
+
 int multipole_chunk() { return kChunk; }
 
 int multipole_slab() { return kSlab; }
@@ -709,11 +877,12 @@ hipError_t launch_multipole_geometry(const double* xyz, int npts, const double* 
 hipError_t launch_multipole_coeff(const float* radius, const float* occ, const int* chunks,
                                   int nchunk, int npad, const void* Yraw, int ntiles,
                                   const int* tile_l, int tpg, int max_l_range, int aexp,
-                                  const int* type_chunk, int ntypes, const float* qv,
+                                  int yexp, const int* type_chunk, int ntypes, const float* qv,
                                   const int* qlmax, int nqp, float* partial, double* coef,
                                   hipStream_t stream, KernelTimer* timer) {
   if (nqp % kQB != 0 || npad % kSlab != 0 || ntiles <= 0 || nchunk <= 0 || ntypes <= 0 ||
-      (tpg != 32 && tpg != 48 && tpg != 64 && tpg != 128) || aexp < -60 || aexp > 60)
+      (tpg != 32 && tpg != 48 && tpg != 64 && tpg != 128) || aexp < -60 || aexp > 60 ||
+      yexp < -60 || yexp > 60)
     return hipErrorInvalidValue;
   const int ngroups = (ntiles + tpg - 1) / tpg;
   const long long total = (long long)(nqp / kQB) * nchunk * ngroups;
@@ -722,7 +891,7 @@ hipError_t launch_multipole_coeff(const float* radius, const float* occ, const i
   const int4* ck = reinterpret_cast<const int4*>(chunks);
   const int nslab = npad / kSlab;
   const _Float16* Y = static_cast<const _Float16*>(Yraw);
-  const float ascale = ldexpf(1.f, aexp), unscale = ldexpf(1.f, -(kYExp + aexp));
+  const float ascale = ldexpf(1.f, aexp), unscale = ldexpf(1.f, -(yexp + aexp));
   {
     TimerScope scope(timer, "multipole_coeff", stream);
     if (tpg == 128)

@@ -211,6 +211,22 @@ csx_status csx_ctx_set_vjp_tile(csx_ctx *ctx, int q_tile);
  * counterpart. */
 csx_status csx_ctx_set_multipole_group(csx_ctx *ctx, int tiles);
 
+/* Radial interpolation of the multipole coefficients: 0 (the default:
+ * automatic), 1 (every point enters the coefficient kernel), 2 (always
+ * interpolate).  Interpolation groups each type's points into radial bins
+ * of width h, the largest power of two with h * max|q| <= 1.04, and
+ * replaces j_l(q r) inside a bin by its interpolant at the bin's 8
+ * Chebyshev nodes (error at most 1.0e-9 of |j_l| <= 1 per point); the
+ * kernel then contracts the nodes with the bins' harmonic moments.  The
+ * automatic choice interpolates where the nodes number at most half the
+ * points.  No reference counterpart (spherical.rs:266-565 sums every point). */
+csx_status csx_ctx_set_multipole_radial(csx_ctx *ctx, int mode);
+
+/* The radial bin width (0: per point) and the number of points (or nodes)
+ * contracted by this context's last multipole coefficient computation.  No
+ * reference counterpart. */
+csx_status csx_ctx_last_multipole(csx_ctx *ctx, double *bin_width, int *points);
+
 /* Workgroups sharing each 256-step sweep of the coordinate adjoint: 0 (the
  * default: automatic, 2, 4 or 8 where the block pairs and q chunks leave
  * most CUs idle, else 1), or 1, 2, 4 or 8.  Each part writes its own
