@@ -25,7 +25,9 @@ the metric:
               3 A grid 3-6 A out as hydration sites, 200 q, l_max 15 / 25 /
               40 (and 64, where the expansion converges), against FP64 GPU
               Debye on the same atoms and sites, with the q range where each
-              order has converged; the six-term solvent basis at l_max 25
+              order has converged; the coefficient kernel on the radial
+              interpolation nodes (the default) and on every point; the
+              six-term solvent basis at l_max 25
   adjoint     20k structure, 300 q: dI/dx vector-Jacobian product throughput,
               checked against a central difference of FP64 GPU Debye; the
               Nelder-Mead and gradient fits of a noisy target (scale 2.3,
@@ -91,6 +93,9 @@ def parse():
                          "for counter collection")
     ap.add_argument("--mp-group", type=int, default=0,
                     help="column tiles per multipole tile group (32, 64, 128; 0: automatic)")
+    ap.add_argument("--mp-radial", type=int, default=0,
+                    help="multipole radial interpolation (0: automatic, 1: every point, "
+                         "2: nodes)")
     a = ap.parse_args()
     a.legs = set() if a.legs in ("", "none") else set(a.legs.split(","))
     if a.no_cpu:
@@ -115,7 +120,7 @@ def copy_calibration(nbytes: int) -> None:
     print(json.dumps({"copy_calibration_bytes": nbytes, "read": nbytes, "written": nbytes}))
 
 
-def multipole_profile(L: int, group: int = 0) -> None:
+def multipole_profile(L: int, group: int = 0, radial: int = 0) -> None:
     """The multipole workload's solvent intensity (atoms and hydration
     sites) at one l_max, 1 warm-up and 3 timed launches and nothing else, so
     that counters of a profiler run belong to that order only."""
@@ -124,6 +129,7 @@ def multipole_profile(L: int, group: int = 0) -> None:
     ctx = csx.default_context()
     ctx.set_cache_limit(0)  # every call runs the coefficient kernel
     ctx.set_multipole_group(group)
+    ctx.set_multipole_radial(radial)
     s, q, _ = workloads.prolate_100k()
     p = csx.SolventParams(c1=1.0, c2=1.0, hydration_model=csx.HydrationModel.ExplicitSites)
     s.handle(ctx)
@@ -136,7 +142,8 @@ def multipole_profile(L: int, group: int = 0) -> None:
     ctx.set_timing(False)
     n, ms = ctx.kernel_time("multipole_coeff")
     print(json.dumps({"multipole_profile_l_max": L, "group_tiles": group, "points": len(s),
-                      "q": int(q.size), "multipole_coeff_ms": ms / n}))
+                      "q": int(q.size), "multipole_coeff_ms": ms / n,
+                      "contracted": ctx.last_multipole()}))
 
 
 def traffic_per_launch(kernel: str):
@@ -490,30 +497,46 @@ def multipole_leg(csx, ctx, log):
            "l_max": {}}
     for L in (15, 25, 40, 64):  # the config's orders, then the largest supported
         mp = csx.MultipoleParams(l_max=L)
+        # every point in the coefficient kernel, for comparison
+        ctx.set_multipole_radial(1)
+        csx.multipole_solvent_intensity(s, q, mp, p, ctx=ctx)
+        got1, dt1, kms1, _ = timed(ctx, "multipole_coeff",
+                                   lambda: csx.multipole_solvent_intensity(s, q, mp, p, ctx=ctx), 3)
+        ctx.set_multipole_radial()
         csx.multipole_solvent_intensity(s, q, mp, p, ctx=ctx)
         got, dt, kms, _ = timed(ctx, "multipole_coeff",
                                 lambda: csx.multipole_solvent_intensity(s, q, mp, p, ctx=ctx), 3)
+        used = ctx.last_multipole()
+        npts = used["points"]
         err = np.abs(got - ref) / np.abs(ref)
-        # 2 N sum_q (L_q + 1)^2 with the per-q order clamp(ceil(q R) + 8, min(8, L), L)
+        # 2 P sum_q (L_q + 1)^2 with the per-q order clamp(ceil(q R) + 8, min(8, L), L)
+        # over the P points (or radial nodes) the kernel contracts; with P = N
+        # it is the reference formulation's count
         lq_full = np.ceil(q * rmax) + 8
         lq = np.clip(lq_full, min(8, L), L)
-        flops = 2.0 * n * float(np.sum((lq + 1) ** 2))
+        per_point = 2.0 * float(np.sum((lq + 1) ** 2))
         conv = lq_full <= L
         # Harmonic tiles the kernel streams: each 16-q block reads, for every
         # 32-point slab, the FP16 hi/lo fragments (2 KB) of the column tiles
         # up to its largest order (16 columns per tile, ceil((2l+1)/16) per l)
         lqp = np.concatenate([lq, np.zeros((-len(lq)) % 16)]).astype(int)
         tiles_to = np.cumsum([-(-(2 * ll + 1) // 16) for ll in range(L + 1)])
-        slabs = -(-n // 32)
-        ybytes = 2048.0 * slabs * float(sum(tiles_to[lqp[b:b + 16].max()]
-                                            for b in range(0, len(lqp), 16)))
+        ytiles = float(sum(tiles_to[lqp[b:b + 16].max()] for b in range(0, len(lqp), 16)))
+        ybytes = 2048.0 * -(-npts // 32) * ytiles
         row = {"wall_ms": dt * 1e3, "kernel_ms": kms,
+               "bin_width_angstrom": used["bin_width"], "contracted_points": npts,
                "harmonic_tile_bytes": ybytes,
                "harmonic_stream_tb_s": ybytes / (kms / 1e3) / 1e12,
                "equiv_pair_q_per_s": pairq / dt,
                "speedup_vs_fp64_debye": dt64 / dt,
-               "mfma_tflops_algorithmic": flops / (kms / 1e3) / 1e12,
-               "mfma_frac": flops / (kms / 1e3) / PEAK_FP32_MFMA,
+               "reference_formulation_tflops": n * per_point / (kms / 1e3) / 1e12,
+               "mfma_tflops_executed": npts * per_point / (kms / 1e3) / 1e12,
+               "mfma_frac_executed": npts * per_point / (kms / 1e3) / PEAK_FP32_MFMA,
+               "every_point": {"kernel_ms": kms1, "wall_ms": dt1 * 1e3,
+                               "mfma_frac": n * per_point / (kms1 / 1e3) / PEAK_FP32_MFMA,
+                               "max_rel_vs_nodes": rel_err(got1, got)[0],
+                               "max_rel_err_converged": float((np.abs(got1 - ref) / np.abs(ref))[conv].max())
+                               if conv.any() else None},
                "max_rel_err": float(err.max()), "mean_rel_err": float(err.mean()),
                "converged_q": ([round(float(q[conv].min()), 4), round(float(q[conv].max()), 4)]
                                if conv.any() else None),
@@ -524,7 +547,8 @@ def multipole_leg(csx, ctx, log):
         row["gate_1e-5_holds_where_converged"] = (row["max_rel_err_converged"] is not None
                                                   and row["max_rel_err_converged"] <= 1e-5)
         out["l_max"][str(L)] = row
-        log(f"multipole: L={L} {dt * 1e3:.2f} ms (kernel {kms:.3f} ms), converged for q <= "
+        log(f"multipole: L={L} {dt * 1e3:.2f} ms (kernel {kms:.3f} ms on {npts} points, every "
+            f"point {kms1:.3f} ms), converged for q <= "
             f"{row['converged_q']}, max rel err there {row['max_rel_err_converged']}, "
             f"overall {err.max():.2e}")
     # The six-term solvent basis (three amplitude channels) at L = 25,
@@ -736,7 +760,7 @@ def main():
         copy_calibration(args.copy_calibration)
         return
     if args.multipole_profile:
-        multipole_profile(args.multipole_profile, args.mp_group)
+        multipole_profile(args.multipole_profile, args.mp_group, args.mp_radial)
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not "

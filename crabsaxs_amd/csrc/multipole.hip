@@ -7,6 +7,13 @@
 //                      MFMA B-operand fragments.  Each l owns ceil((2l+1)/16)
 //                      column tiles of 16 (column c of l is m = c - l; padding
 //                      is 0).  A tile is stored in slabs of 32 points.
+//  multipole_point_terms, multipole_bin_moments, multipole_moment_tiles
+//                      the radial-interpolation geometry, once per (point
+//                      set, l_max, bin width h): occ_i Y_lm,i per point and
+//                      its Lagrange weights at the 8 Chebyshev nodes of its
+//                      radial bin; per (bin, lm) the node moments (FP64, in
+//                      point order); the moments as FP16 (hi, lo) tiles.  The
+//                      coefficient kernel then takes the nodes as its points.
 //  multipole_coeff     S_t[q][lm] = sum_{i of type t} occ_i j_l(q r_i) Y_lm,i
 //                      on v_mfma_f32_16x16x32_f16 (16 q rows x 16 columns,
 //                      k = 32 points), each FP32 product formed from FP16

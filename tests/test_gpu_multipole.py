@@ -117,8 +117,17 @@ def test_centre_point_and_distant_neighbours(gpu_ctx):
     assert rel(got, d).max() < 1e-5  # L = 40 has converged for q R <= 10
 
 
+@pytest.fixture(params=[1, 2], ids=["points", "nodes"])
+def radial(request, gpu_ctx):
+    """Every point in the coefficient kernel (1), or radial interpolation
+    nodes (2)."""
+    gpu_ctx.set_multipole_radial(request.param)
+    yield request.param
+    gpu_ctx.set_multipole_radial()
+
+
 @pytest.mark.parametrize("model", [csx.HydrationModel.Analytic, csx.HydrationModel.AdaptiveGrid])
-def test_solvent_paths_match_exact_oracle(gpu_ctx, model):
+def test_solvent_paths_match_exact_oracle(gpu_ctx, model, radial):
     s = csx.generate_globule(700, 14.0, 1.0, seed=22)
     q = np.linspace(0.0, 0.5, 41)
     p = csx.SolventParams(c1=1.02, c2=0.6, hydration_model=model,
@@ -133,7 +142,8 @@ def test_solvent_paths_match_exact_oracle(gpu_ctx, model):
                                  voxel_volume=vol, basis=True, exact=True)
     got_i = csx.multipole_solvent_intensity(s, q, mp, p)
     got_b = csx.multipole_solvent_basis(s, q, mp, p)
-    print(model.name, rel(got_i, want_i).max(), flush=True)
+    assert (gpu_ctx.last_multipole()["bin_width"] > 0) == (radial == 2)
+    print(model.name, radial, rel(got_i, want_i).max(), flush=True)
     assert rel(got_i, want_i).max() < 1e-5
     scale = np.abs(want_b).max(axis=1, keepdims=True)
     assert (np.abs(got_b - want_b) / scale).max() < 1e-5
@@ -183,3 +193,50 @@ def test_prolate_with_shell_sites_large(gpu_ctx):
     want = O.multipole_intensity(oatoms(full), q, 25, exact=True)
     print("prolate", len(full), rel(got, want).max(), flush=True)
     assert rel(got, want).max() < 1e-5
+
+
+def test_radial_nodes_match_every_point(gpu_ctx):
+    """Radial interpolation (8 Chebyshev nodes per bin of width h, the
+    largest power of two with h max|q| <= 1.04) against every point in the
+    coefficient kernel, on the 10,000-atom prolate with shell sites: both
+    forms agree to FP32 accuracy and meet the exact oracle; the bin width
+    follows the q range; the automatic choice takes the nodes here and the
+    points for a structure smaller than its nodes."""
+    s = csx.generate_globule(10000, 30.0 / 2 ** (1 / 3), 2.0, seed=23)
+    sites = csx.generate_shell_sites(s, 3.0, 3.0, 6.0)
+    full = csx.Structure.from_arrays(np.vstack([s.xyz, sites]),
+                                     np.concatenate([s.element, np.full(len(sites), 3, dtype=np.uint8)]))
+    n = len(full)
+    q = np.linspace(0.005, 0.5, 24)
+    mp = csx.MultipoleParams(l_max=25)
+    try:
+        got = {}
+        for mode in (1, 2):
+            gpu_ctx.set_multipole_radial(mode)
+            got[mode] = csx.multipole_intensity(full, q, mp)
+            info = gpu_ctx.last_multipole()
+            if mode == 1:
+                assert info == {"bin_width": 0.0, "points": n}
+            else:
+                assert info["bin_width"] == 2.0 and 0 < info["points"] <= n // 2, info
+        print("nodes vs points", rel(got[2], got[1]).max(), flush=True)
+        assert rel(got[2], got[1]).max() < 2e-6
+        want = O.multipole_intensity(oatoms(full), q, 25, exact=True)
+        assert rel(got[2], want).max() < 1e-5
+        # the bin width follows max|q|: 1.04 / 1.0 -> 1, 1.04 / 0.25 -> 4
+        for qmax, h in ((1.0, 1.0), (0.25, 4.0)):
+            csx.multipole_intensity(full, np.linspace(0.0, qmax, 8), mp)
+            assert gpu_ctx.last_multipole()["bin_width"] == h, qmax
+        gpu_ctx.set_multipole_radial(0)
+        csx.multipole_intensity(full, q, mp)
+        assert gpu_ctx.last_multipole()["bin_width"] == 2.0
+        small = csx.generate_globule(40, 20.0, 1.0, seed=3)
+        csx.multipole_intensity(small, q, mp)
+        assert gpu_ctx.last_multipole() == {"bin_width": 0.0, "points": 40}
+    finally:
+        gpu_ctx.set_multipole_radial()
+
+
+def test_radial_mode_is_validated(gpu_ctx):
+    with pytest.raises(csx.InvalidInput):
+        gpu_ctx.set_multipole_radial(3)
