@@ -283,6 +283,7 @@ std::shared_ptr<CacheEntry> cache_acquire(csx_ctx* ctx, PointSet& ps, const std:
     if (!e) {
       e = std::make_shared<CacheEntry>();
       e->key = key;
+      e->cached = true;
     }
     ps.cache.insert(ps.cache.begin(), e);
   } else {
@@ -299,7 +300,7 @@ void cache_commit(csx_ctx* ctx, PointSet& ps, const std::shared_ptr<CacheEntry>&
     std::lock_guard<std::mutex> g(ctx->mu);
     limit = ctx->tuning.cache_bytes;
   }
-  if (limit == 0) return;
+  if (limit == 0 || !e->cached) return;
   std::lock_guard<std::mutex> g(ps.cache_mu);
   // The entry goes first (another call may have dropped it meanwhile); then
   // the most recently used ready entries are kept within the limit.  Entries
@@ -350,6 +351,7 @@ void cache_store(csx_ctx* ctx, PointSet& ps, const std::string& key,
   }
   auto e = std::make_shared<CacheEntry>();
   e->key = key;
+  e->cached = true;
   e->tables = tables;
   e->bytes = tables.size() * sizeof(double) + key.size();
   e->ready = true;

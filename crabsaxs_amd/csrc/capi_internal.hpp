@@ -183,9 +183,14 @@ struct CacheEntry {
   std::string key;
   std::mutex mu;
   std::atomic<bool> ready{false};
+  bool cached = false;  // created in the point set's cache (else private to one call)
   size_t bytes = 0;
   std::vector<double> tables;
+  // Multipole coefficients and per-q orders: in coef / qlmax for a cached
+  // entry, in the computing call's scratch for a private one.
   DevArray coef, qlmax;
+  double* coef_ptr = nullptr;
+  int* qlmax_ptr = nullptr;
   int nqp = 0, ncol = 0;
 };
 

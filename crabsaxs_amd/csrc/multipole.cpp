@@ -445,8 +445,22 @@ static csx_status type_coefficients(Call& call, PointSet& ps, const MultipoleGeo
     qlmax[k] = adaptive_l_max(q[k], rmax, g.L);
   }
   const hipStream_t st = call.stream();
-  CSX_HIP(e->qlmax.alloc(nqp * sizeof(int)), "hipMalloc orders");
-  CSX_HIP(e->coef.alloc(size_t(TT) * nqp * ncol * sizeof(double)), "hipMalloc coefficients");
+  // A cached entry owns its arrays; a private one (no cache) lives in the
+  // call's scratch and ends with the call.
+  const size_t coef_bytes = size_t(TT) * nqp * ncol * sizeof(double);
+  if (e->cached) {
+    CSX_HIP(e->qlmax.alloc(nqp * sizeof(int)), "hipMalloc orders");
+    CSX_HIP(e->coef.alloc(coef_bytes), "hipMalloc coefficients");
+    e->qlmax_ptr = e->qlmax.as<int>();
+    e->coef_ptr = e->coef.as<double>();
+  } else {
+    void* pq = nullptr;
+    void* pc = nullptr;
+    CSX_HIP(call.take_scratch(nqp * sizeof(int), &pq), "hipMalloc orders");
+    CSX_HIP(call.take_scratch(coef_bytes, &pc), "hipMalloc coefficients");
+    e->qlmax_ptr = static_cast<int*>(pq);
+    e->coef_ptr = static_cast<double*>(pc);
+  }
   Scratch d_q(call), d_partial(call);
   CSX_HIP(d_q.alloc(nqp * sizeof(float), st), "hipMallocAsync");
   CSX_HIP(d_partial.alloc(static_cast<size_t>(g.nchunk) * nqp * ncol * sizeof(float), st),
@@ -458,7 +472,7 @@ static csx_status type_coefficients(Call& call, PointSet& ps, const MultipoleGeo
   std::memcpy(in + nqp * sizeof(float), qlmax.data(), nqp * sizeof(int));
   CSX_HIP(hipMemcpyAsync(d_q.as<void>(), in, nqp * sizeof(float), hipMemcpyHostToDevice, st),
           "hipMemcpyAsync");
-  CSX_HIP(hipMemcpyAsync(e->qlmax.as<void>(), in + nqp * sizeof(float), nqp * sizeof(int),
+  CSX_HIP(hipMemcpyAsync(e->qlmax_ptr, in + nqp * sizeof(float), nqp * sizeof(int),
                          hipMemcpyHostToDevice, st),
           "hipMemcpyAsync");
   int tuned;
@@ -474,12 +488,13 @@ static csx_status type_coefficients(Call& call, PointSet& ps, const MultipoleGeo
                                  g.nchunk, g.npad, g.Y.as<void>(), g.ntiles, g.tile_l.as<int>(),
                                  tpg, l_range, g.aexp, g.yexp, g.type_chunk.as<int>(), TT,
                                  d_q.as<float>(),
-                                 e->qlmax.as<int>(), static_cast<int>(nqp), d_partial.as<float>(),
-                                 e->coef.as<double>(), st, call.timer()),
+                                 e->qlmax_ptr, static_cast<int>(nqp), d_partial.as<float>(),
+                                 e->coef_ptr, st, call.timer()),
           "multipole_coeff");
-  // The entry is complete once the stream has run the kernels; every later
-  // use of it is stream-ordered after a synchronisation of this call.
-  CSX_HIP(hipStreamSynchronize(st), "hipStreamSynchronize");
+  // A cached entry is complete once the stream has run the kernels; every
+  // later use of it is stream-ordered after a synchronisation of this call.
+  // A private one is used only on this stream.
+  if (e->cached) CSX_HIP(hipStreamSynchronize(st), "hipStreamSynchronize");
   e->nqp = static_cast<int>(nqp);
   e->ncol = ncol;
   e->bytes = size_t(TT) * nqp * ncol * sizeof(double) + nqp * sizeof(int) + key.size();
@@ -567,8 +582,8 @@ static csx_status multipole_run(csx_structure* s, const double* q, size_t nq, in
   CSX_HIP(hipMemcpyAsync(d_amp.as<void>(), h_in, amp.size() * sizeof(double),
                          hipMemcpyHostToDevice, st),
           "hipMemcpyAsync");
-  CSX_HIP(launch_multipole_contract(coef->coef.as<double>(), d_amp.as<double>(), TT, nch,
-                                    g->tile_l.as<int>(), coef->qlmax.as<int>(), nqp, coef->ncol,
+  CSX_HIP(launch_multipole_contract(coef->coef_ptr, d_amp.as<double>(), TT, nch,
+                                    g->tile_l.as<int>(), coef->qlmax_ptr, nqp, coef->ncol,
                                     d_out.as<double>(), st),
           "multipole_contract");
   CSX_HIP(hipMemcpyAsync(h_res, d_out.as<void>(), size_t(nqp) * nterms * sizeof(double),
