@@ -327,9 +327,9 @@ static csx_status get_geometry(Call& call, csx_structure* s, PointSet& ps, int L
       if (!std::isfinite(occ[k])) return fail(CSX_INVALID_INPUT, "occupancy not finite");
     }
     const int nlm = (L + 1) * (L + 1);
-    // Point chunks of whole bins, each with at most ~256 MB of FP64 terms
+    // Point chunks of whole bins, each with at most ~512 MB of FP64 terms
     // (or one bin).
-    const size_t cap = std::max<size_t>(1, (size_t(256) << 20) / (size_t(nlm) * sizeof(double)));
+    const size_t cap = std::max<size_t>(1, (size_t(512) << 20) / (size_t(nlm) * sizeof(double)));
     std::vector<std::pair<size_t, size_t>> pchunks;  // bin ranges
     size_t maxpts = 0;
     for (size_t b = 0; b < bins.size();) {
@@ -348,7 +348,7 @@ static csx_status get_geometry(Call& call, csx_structure* s, PointSet& ps, int L
     CSX_HIP(d_norms.alloc(norms.size() * sizeof(double), st), "hipMallocAsync");
     CSX_HIP(d_first.alloc(tile_first.size() * sizeof(int), st), "hipMallocAsync");
     CSX_HIP(d_bins.alloc(hbins.size() * sizeof(int), st), "hipMallocAsync");
-    CSX_HIP(d_terms.alloc(maxpts * nlm * sizeof(double), st), "hipMallocAsync");
+    CSX_HIP(d_terms.alloc((maxpts + 15) / 16 * 16 * nlm * sizeof(double), st), "hipMallocAsync");
     CSX_HIP(d_w.alloc(maxpts * nodes * sizeof(double), st), "hipMallocAsync");
     CSX_HIP(d_mom.alloc(size_t(g->npad) * nlm * sizeof(double), st), "hipMallocAsync");
     CSX_HIP(d_max.alloc(sizeof(unsigned long long), st), "hipMallocAsync");

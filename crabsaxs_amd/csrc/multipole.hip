@@ -173,53 +173,121 @@ struct NodeTable {
   double x[kNodes], den[kNodes];
 };
 
-// occ_i Y_lm,i (FP64, point-major rows of nlm) and the point's kNodes
-// Lagrange weights at u = (r - centre of its bin) / (h / 2), for the sorted
-// real points [p0, p1).
+// The terms of the radial-interpolation geometry: occ_i Y_lm,i in FP64 in
+// tiles of 16 points, terms[((i - p0) / 16 * nlm + lm) * 16 + (i - p0) % 16],
+// so that a wave's stores of one (l, m) over 64 consecutive points and a
+// wave's loads of 64 consecutive lm for one tile are both contiguous.
+__device__ inline size_t term_index(int k, int lm, int nlm) {
+  return (size_t(k >> 4) * nlm + lm) * 16 + (k & 15);
+}
+
+// One thread per (point, m) of the sorted real points [p0, p1): the m chain
+// of the associated-Legendre recurrence (spherical.rs:54-93, as
+// point_harmonics), times occ_i; the m = 0 threads also write the point's
+// kNodes Lagrange weights at u = (r - centre of its bin) / (h / 2).
 __global__ void multipole_point_terms(const double* __restrict__ xyz, const double* __restrict__ occ,
                                       const int* __restrict__ bin_of, int p0, int p1,
                                       const double* __restrict__ centre3, int l_max,
                                       const double* __restrict__ norms, double h, NodeTable nodes,
                                       double* __restrict__ terms, double* __restrict__ weights) {
   const int i = p0 + blockIdx.x * blockDim.x + threadIdx.x;
+  const int m = blockIdx.y;
   if (i >= p1) return;
   const int nlm = (l_max + 1) * (l_max + 1);
+  const int k = i - p0;
+  const double px = xyz[3 * i] - centre3[0], py = xyz[3 * i + 1] - centre3[1],
+               pz = xyz[3 * i + 2] - centre3[2];
+  const double r = sqrt(px * px + py * py + pz * pz);
+  double x = 1.0, phi = 0.0;
+  if (!(r < 1e-12)) {
+    x = fmin(fmax(pz / r, -1.0), 1.0);
+    phi = atan2(py, px);
+  }
+  const double st = sqrt(fmax(1.0 - x * x, 0.0));
+  double pmm = 1.0;
+  for (int a = 1; a <= m; ++a) pmm *= -double(2 * a - 1) * st;
+  double sm = 0.0, cm = 1.0;
+  if (m > 0) sincos(double(m) * phi, &sm, &cm);
   const double o = occ[i];
-  double* row = terms + size_t(i - p0) * nlm;
-  const double r = point_harmonics(xyz[3 * i] - centre3[0], xyz[3 * i + 1] - centre3[1],
-                                   xyz[3 * i + 2] - centre3[2], l_max, norms,
-                                   [&](int l, int col, double v) { row[l * l + col] = o * v; });
+  const double sqrt2 = 1.4142135623730951;
+  auto emit = [&](int l, double p) {
+    const double nrm = norms[l * l + l + m];
+    if (m == 0) {
+      terms[term_index(k, l * l + l, nlm)] = o * nrm * p;
+    } else {
+      const double v = o * sqrt2 * nrm * p;
+      terms[term_index(k, l * l + l + m, nlm)] = v * cm;
+      terms[term_index(k, l * l + l - m, nlm)] = v * sm;
+    }
+  };
+  emit(m, pmm);
+  if (m < l_max) {
+    const double pm1 = x * double(2 * m + 1) * pmm;
+    emit(m + 1, pm1);
+    double prev = pmm, cur = pm1;
+    for (int l = m + 2; l <= l_max; ++l) {
+      const double next = (double(2 * l - 1) * x * cur - double(l + m - 1) * prev) / double(l - m);
+      emit(l, next);
+      prev = cur;
+      cur = next;
+    }
+  }
+  if (m != 0) return;
   const double u = (r - (double(bin_of[i]) + 0.5) * h) / (0.5 * h);
   double pre[kNodes + 1], suf[kNodes + 1];
   pre[0] = 1.0;
   suf[kNodes] = 1.0;
 #pragma unroll
-  for (int k = 0; k < kNodes; ++k) pre[k + 1] = pre[k] * (u - nodes.x[k]);
+  for (int a = 0; a < kNodes; ++a) pre[a + 1] = pre[a] * (u - nodes.x[a]);
 #pragma unroll
-  for (int k = kNodes - 1; k >= 0; --k) suf[k] = suf[k + 1] * (u - nodes.x[k]);
+  for (int a = kNodes - 1; a >= 0; --a) suf[a] = suf[a + 1] * (u - nodes.x[a]);
 #pragma unroll
-  for (int m = 0; m < kNodes; ++m)
-    weights[size_t(i - p0) * kNodes + m] = pre[m] * suf[m + 1] / nodes.den[m];
+  for (int a = 0; a < kNodes; ++a) weights[size_t(k) * kNodes + a] = pre[a] * suf[a + 1] / nodes.den[a];
 }
 
 // moments[(v0 + m) * nlm + lm] = sum over the bin's points, in order, of
-// weight_m * term_lm.  One thread per (bin, lm); bins: (first point, end
-// point, first virtual point, 0), all inside [p0, ...).
+// weight_m * term_lm.  One thread per (bin, lm): per 16-point tile of the
+// bin, one contiguous 128-byte row of terms (the wave's 64 lm together read
+// 8 KB); points of a tile outside the bin are skipped.  bins: (first point,
+// end point, first virtual point, 0), all inside [p0, ...).
 __global__ void multipole_bin_moments(const double* __restrict__ terms,
                                       const double* __restrict__ weights,
                                       const int4* __restrict__ bins, int p0, int nlm,
                                       double* __restrict__ moments) {
+  __shared__ double wt[16 * kNodes];  // the tile's weights
   const int lm = blockIdx.x * blockDim.x + threadIdx.x;
-  if (lm >= nlm) return;
+  const bool live = lm < nlm;
   const int4 b = bins[blockIdx.y];
   double acc[kNodes];
 #pragma unroll
   for (int m = 0; m < kNodes; ++m) acc[m] = 0.0;
-  for (int i = b.x; i < b.y; ++i) {
-    const double t = terms[size_t(i - p0) * nlm + lm];
+  const int k0 = b.x - p0, k1 = b.y - p0;
+  for (int t = k0 >> 4; t <= (k1 - 1) >> 4; ++t) {
+    __syncthreads();  // the previous tile's weights have been read
+    for (int e = threadIdx.x; e < 16 * kNodes; e += blockDim.x) {
+      const int kk = t * 16 + e / kNodes;
+      wt[e] = kk >= k0 && kk < k1 ? weights[size_t(kk) * kNodes + e % kNodes] : 0.0;
+    }
+    __syncthreads();
+    if (!live) continue;
+    const double2* row = reinterpret_cast<const double2*>(terms + (size_t(t) * nlm + lm) * 16);
+    double v[16];
 #pragma unroll
-    for (int m = 0; m < kNodes; ++m) acc[m] = fma(weights[size_t(i - p0) * kNodes + m], t, acc[m]);
+    for (int u = 0; u < 8; ++u) {
+      const double2 d = row[u];
+      v[2 * u] = d.x;
+      v[2 * u + 1] = d.y;
+    }
+    // the tile's points in order; the bounds are the same for every lane
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+      const int kk = t * 16 + u;
+      if (kk < k0 || kk >= k1) continue;
+#pragma unroll
+      for (int m = 0; m < kNodes; ++m) acc[m] = fma(wt[u * kNodes + m], v[u], acc[m]);
+    }
   }
+  if (!live) return;
 #pragma unroll
   for (int m = 0; m < kNodes; ++m) moments[size_t(b.z + m) * nlm + lm] = acc[m];
 }
@@ -808,13 +876,13 @@ hipError_t launch_multipole_point_terms(const double* xyz, const double* occ, co
                                         const double* node_den, double* terms, double* weights,
                                         hipStream_t stream) {
   if (p1 <= p0) return hipSuccess;
-  if (!(h > 0.0)) return hipErrorInvalidValue;
+  if (!(h > 0.0) || l_max < 0 || l_max > 65534) return hipErrorInvalidValue;
   NodeTable nodes;
   for (int k = 0; k < kNodes; ++k) {
     nodes.x[k] = node_x[k];
     nodes.den[k] = node_den[k];
   }
-  multipole_point_terms<<<(p1 - p0 + 127) / 128, 128, 0, stream>>>(
+  multipole_point_terms<<<dim3((p1 - p0 + 63) / 64, l_max + 1), 64, 0, stream>>>(
       xyz, occ, bin_of, p0, p1, centre, l_max, norms, h, nodes, terms, weights);
   return hipGetLastError();
 }
@@ -824,7 +892,7 @@ hipError_t launch_multipole_bin_moments(const double* terms, const double* weigh
                                         double* moments, hipStream_t stream) {
   if (nbins <= 0) return hipSuccess;
   if (nbins > 65535) return hipErrorInvalidValue;
-  multipole_bin_moments<<<dim3((nlm + 255) / 256, nbins), 256, 0, stream>>>(
+  multipole_bin_moments<<<dim3((nlm + 63) / 64, nbins), 64, 0, stream>>>(
       terms, weights, reinterpret_cast<const int4*>(bins), p0, nlm, moments);
   return hipGetLastError();
 }
@@ -832,7 +900,7 @@ hipError_t launch_multipole_bin_moments(const double* terms, const double* weigh
 hipError_t launch_multipole_moment_max(const double* moments, size_t n, unsigned long long* out,
                                        hipStream_t stream) {
   zero_u64<<<1, 1, 0, stream>>>(out);
-  const size_t blocks = std::min<size_t>((n + 255) / 256, 2048);
+  const size_t blocks = std::min<size_t>((n + 255) / 256, 256);
   if (blocks > 0) abs_max_f64<<<unsigned(blocks), 256, 0, stream>>>(moments, n, out);
   return hipGetLastError();
 }
