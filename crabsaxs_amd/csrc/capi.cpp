@@ -135,6 +135,7 @@ static csx_status build_point_set(csx_structure* s, const csx_solvent_params* p,
     const int t = s->type_of[i];
     for (int a = 0; a < 3; ++a) pts32[4 * k + a] = static_cast<float>(s->xyz[3 * i + a] - s->centre[a]);
     pts32[4 * k + 3] = static_cast<float>(s->occ[i]);
+    ps.occ_max = std::max(ps.occ_max, std::fabs(static_cast<double>(pts32[4 * k + 3])));
     ps.max_radius = std::max(ps.max_radius, std::sqrt(double(pts32[4 * k]) * pts32[4 * k] +
                                                       double(pts32[4 * k + 1]) * pts32[4 * k + 1] +
                                                       double(pts32[4 * k + 2]) * pts32[4 * k + 2]));
@@ -146,6 +147,7 @@ static csx_status build_point_set(csx_structure* s, const csx_solvent_params* p,
     for (int a = 0; a < 3; ++a)
       pts32[4 * (n + k) + a] = static_cast<float>(ps.site_xyz[3 * k + a] - s->centre[a]);
     pts32[4 * (n + k) + 3] = 1.0f;
+    ps.occ_max = std::max(ps.occ_max, 1.0);
     const float* p3 = &pts32[4 * (n + k)];
     ps.max_radius = std::max(ps.max_radius, std::sqrt(double(p3[0]) * p3[0] + double(p3[1]) * p3[1] +
                                                       double(p3[2]) * p3[2]));
@@ -506,11 +508,54 @@ struct PairPlan {
   int sin_mode = 2;
   int qt = 64;
   float qstep = 0.f;
+  double h = 0.0;  // radial interpolation (mode 4): bin width and bins per table
+  int nk = 0;
 };
 
+// Radial interpolation of the pair tables (sine mode 4): bins of width h,
+// the largest power of two with h max|q| <= 1.04 (q the effective FP32 q of
+// the table columns), where 8 Chebyshev nodes interpolate j_0(q r) to
+// 1.0e-9; pair distances up to 2 r_max need floor(2 r_max / h) + 2 bins,
+// whose LDS accumulators (64 B per bin) must fit a workgroup beside the
+// 4 KB point tile.  Returns false where they do not.
+static bool radial_bins(const double* q, size_t nq, double rmax, size_t lds, double* h, int* nk) {
+  double qmax = 0.0;
+  for (size_t k = 0; k < nq; ++k)
+    qmax = std::max(qmax, 2.0 * kPi * std::fabs(static_cast<double>(static_cast<float>(q[k] / (2.0 * kPi)))));
+  *h = qmax > 0.0 ? std::min(64.0, std::exp2(std::floor(std::log2(1.04 / qmax)))) : 64.0;
+  const double bins = std::floor(2.0 * rmax / *h) + 2.0;
+  if (!(*h >= 1.0 / 4096.0) || !(bins * radial_lds_bytes(1) + 4104.0 <= static_cast<double>(lds)))
+    return false;
+  *nk = static_cast<int>(bins);
+  return true;
+}
+
+// Fixed-point exponent of a point set's radial accumulators: one
+// contribution occ_i occ_j L_m (|L_m| <= 1.27) times 2^F stays below 2^30,
+// and a table bin's sum over all the set's pairs below 2^62.
+static int radial_scale_exponent(const PointSet& ps) {
+  const double cmax = std::max(ps.occ_max * ps.occ_max * 1.3, 1e-300);
+  const double n = static_cast<double>(ps.natoms + ps.nsites);
+  const double pairs = std::max(0.5 * n * (n - 1.0), 1.0);
+  const int f1 = static_cast<int>(std::floor(30.0 - std::log2(cmax)));
+  const int f2 = static_cast<int>(std::floor(62.0 - std::log2(pairs * cmax)));
+  return std::max(-60, std::min({f1, f2, 60}));
+}
+
 static PairPlan pair_plan(const Tuning& tune, const double* q, size_t nq, double rmax,
-                          double npts, size_t nitems, double sum_j, double max_j, int cus) {
+                          double npts, size_t nitems, double sum_j, double max_j, int cus,
+                          size_t lds) {
   PairPlan plan;
+  if (tune.sin_mode == 4) {
+    if (radial_bins(q, nq, rmax, lds, &plan.h, &plan.nk)) {
+      plan.sin_mode = 4;
+      plan.qt = 1;
+      return plan;
+    }
+    Tuning direct = tune;
+    direct.sin_mode = 2;
+    return pair_plan(direct, q, nq, rmax, npts, nitems, sum_j, max_j, cus, lds);
+  }
   plan.sin_mode = pair_sin_mode(tune.sin_mode, q, nq, rmax);
   if (plan.sin_mode == 3) {
     const float step = uniform_q_step(q, nq, rmax);
@@ -602,7 +647,11 @@ static csx_status pair_tables_launch(Call& call, const std::vector<PointSet*>& s
   Scratch d_q(call), d_partial(call), d_out(call), d_counter(call), d_pts(call), d_items(call),
       d_bins(call);
   CSX_HIP(d_q.alloc(nq_pad * sizeof(float), st), "hipMallocAsync");
-  CSX_HIP(d_partial.alloc(nitems * nq_pad * sizeof(double), st), "hipMallocAsync");
+  CSX_HIP(d_partial.alloc(sin_mode == 4 ? size_t(nbins) * plan.nk * radial_node_count() *
+                                              sizeof(unsigned long long)
+                                        : nitems * nq_pad * sizeof(double),
+                          st),
+          "hipMallocAsync");
   CSX_HIP(d_out.alloc(nbins * nq_pad * sizeof(double), st), "hipMallocAsync");
   CSX_HIP(d_counter.alloc(sizeof(int), st), "hipMallocAsync");
   // q and the tables go through the stream's pinned staging, so both copies
@@ -658,7 +707,7 @@ static csx_status pair_tables_launch(Call& call, const std::vector<PointSet*>& s
     items = d_items.as<int4>();
     bins = d_bins.as<int>();
   }
-  const int persist_wgs = pair_persist_wgs(tune.persist, nitems, nq_pad, qt, cus);
+  const int persist_wgs = sin_mode == 4 ? 0 : pair_persist_wgs(tune.persist, nitems, nq_pad, qt, cus);
   {
     std::lock_guard<std::mutex> lock(call.ctx()->mu);
     call.ctx()->last_launch[0] = qt;
@@ -667,11 +716,52 @@ static csx_status pair_tables_launch(Call& call, const std::vector<PointSet*>& s
     call.ctx()->last_items[0] = chunk[todo[0]];
     call.ctx()->last_items[1] = static_cast<int>(nitems);
   }
-  CSX_HIP(launch_debye_pairs(pts, items, static_cast<int>(nitems), bins, static_cast<int>(nbins),
-                             d_q.as<float>(), plan.qstep, static_cast<int>(nq_pad), qt, sin_mode,
-                             persist_wgs, d_counter.as<int>(), d_partial.as<double>(),
-                             d_out.as<double>(), st, call.timer()),
-          "debye_pairs_f32");
+  if (sin_mode == 4) {
+    // Per table bin, the fixed-point scale of its point set; the Chebyshev
+    // nodes on [-1, 1] and their inverse Lagrange denominators.
+    const int nn = radial_node_count();
+    const size_t sbytes = nbins * (sizeof(float) + sizeof(double)) + 2 * nn * sizeof(float);
+    void* h_s = nullptr;
+    CSX_HIP(call.take_pinned(3, sbytes, &h_s), "hipHostMalloc");
+    double* unscale = static_cast<double*>(h_s);
+    float* scale = reinterpret_cast<float*>(unscale + nbins);
+    float* nodes = scale + nbins;
+    size_t b = 0;
+    for (size_t t = 0; t < todo.size(); ++t) {
+      const PointSet& ps = *sets[todo[t]];
+      const int f = radial_scale_exponent(ps);
+      for (int k = 0; k < ps.nbins; ++k, ++b) {
+        scale[b] = std::ldexp(1.0f, f);
+        unscale[b] = std::ldexp(1.0, -f);
+      }
+    }
+    for (int m = 0; m < nn; ++m) nodes[m] = static_cast<float>(std::cos((2.0 * m + 1.0) * kPi / (2.0 * nn)));
+    for (int m = 0; m < nn; ++m) {
+      double d = 1.0;
+      for (int k = 0; k < nn; ++k)
+        if (k != m) d *= std::cos((2.0 * m + 1.0) * kPi / (2.0 * nn)) - std::cos((2.0 * k + 1.0) * kPi / (2.0 * nn));
+      nodes[nn + m] = static_cast<float>(1.0 / d);
+    }
+    Scratch d_scale(call);
+    CSX_HIP(d_scale.alloc(sbytes, st), "hipMallocAsync");
+    CSX_HIP(hipMemcpyAsync(d_scale.as<void>(), h_s, sbytes, hipMemcpyHostToDevice, st),
+            "hipMemcpyAsync");
+    const double* d_unscale = d_scale.as<double>();
+    const float* d_bscale = reinterpret_cast<const float*>(d_unscale + nbins);
+    CSX_HIP(launch_debye_pairs_radial(pts, items, static_cast<int>(nitems), bins,
+                                      static_cast<int>(nbins), d_bscale, d_unscale, plan.h,
+                                      plan.nk, nodes, nodes + nn, d_q.as<float>(),
+                                      static_cast<int>(nq_pad),
+                                      d_partial.as<unsigned long long>(), d_out.as<double>(),
+                                      call.ctx()->lds_per_workgroup, st, call.timer()),
+            "debye_pairs_radial");
+  } else {
+    CSX_HIP(launch_debye_pairs(pts, items, static_cast<int>(nitems), bins, static_cast<int>(nbins),
+                               d_q.as<float>(), plan.qstep, static_cast<int>(nq_pad), qt,
+                               sin_mode, persist_wgs, d_counter.as<int>(),
+                               d_partial.as<double>(), d_out.as<double>(), st, call.timer()),
+            "debye_pairs_f32");
+  }
   CSX_HIP(hipMemcpyAsync(h_out, d_out.as<void>(), nbins * nq_pad * sizeof(double),
                          hipMemcpyDeviceToHost, st),
           "hipMemcpyAsync");
@@ -744,7 +834,7 @@ static csx_status pair_tables_batch(Call& call, const std::vector<PointSet*>& se
     item_stats({all_items[c]}, &ni, &sum_j, &max_j);
     plans[c] = pair_plan(tune, q, nq, sets[c]->max_radius,
                          static_cast<double>(sets[c]->natoms + sets[c]->nsites), ni, sum_j, max_j,
-                         cus);
+                         cus, call.ctx()->lds_per_workgroup);
     keys[c] = table_key(plans[c].sin_mode, plans[c].qt, chunk[c], q, nq);
   }
 
@@ -799,7 +889,9 @@ static csx_status pair_tables_batch(Call& call, const std::vector<PointSet*>& se
     groups[g].push_back(c);
   }
   for (size_t g = 0; g < groups.size(); ++g) {
-    if (gplans[g].sin_mode != 3) {
+    if (gplans[g].sin_mode == 4) {
+      for (size_t c : groups[g]) gplans[g].nk = std::max(gplans[g].nk, plans[c].nk);
+    } else if (gplans[g].sin_mode != 3) {
       std::vector<std::shared_ptr<ItemSet>> gi;
       for (size_t c : groups[g]) gi.push_back(all_items[c]);
       size_t ni;
@@ -1075,8 +1167,8 @@ csx_status csx_ctx_set_tuning(csx_ctx* ctx, int q_tile, int sin_mode, int j_chun
     return fail(CSX_INVALID_INPUT,
                 "q_tile must be 0 (automatic), a multiple of 8 up to 64, 10, 20, 25, 50, 63, "
                 "100, 125 or 128");
-  if (sin_mode < 0 || sin_mode > 3)
-    return fail(CSX_INVALID_INPUT, "sin_mode must be 0, 1, 2 or 3");
+  if (sin_mode < 0 || sin_mode > 4)
+    return fail(CSX_INVALID_INPUT, "sin_mode must be 0, 1, 2, 3 or 4");
   if (!(j_chunk == 0 || j_chunk == 32 || j_chunk == 64 || j_chunk == 128 ||
         (j_chunk >= 256 && j_chunk % 256 == 0 && j_chunk <= (1 << 20))))
     return fail(CSX_INVALID_INPUT,
@@ -1417,6 +1509,9 @@ csx_status csx_debye_pair_tables_noalloc(csx_structure* s, const double* q, size
     tune = ctx->tuning;
     scratch_status = ctx->scratch_status;
   }
+  // the radial sweep's accumulators are not part of the fixed scratch: the
+  // direct sweep computes these tables
+  if (tune.sin_mode == 4) tune.sin_mode = 2;
   size_t cap = 0;
   unsigned char* base = static_cast<unsigned char*>(noalloc_scratch(ctx->device, &cap));
   if (!base)
@@ -1450,7 +1545,8 @@ csx_status csx_debye_pair_tables_noalloc(csx_structure* s, const double* q, size
   const int cus = ctx->cu_count;
   const PairPlan plan = pair_plan(tune, q, nq, ps->max_radius,
                                   static_cast<double>(ps->natoms + ps->nsites),
-                                  static_cast<size_t>(is->nitems), sum_j, max_j, cus);
+                                  static_cast<size_t>(is->nitems), sum_j, max_j, cus,
+                                  ctx->lds_per_workgroup);
   const int sin_mode = plan.sin_mode, qt = plan.qt;
   const size_t nq_pad = (nq + qt - 1) / qt * qt;
   const size_t nitems = static_cast<size_t>(is->nitems), nbins = static_cast<size_t>(ps->nbins);

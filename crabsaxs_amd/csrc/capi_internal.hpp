@@ -210,6 +210,7 @@ struct PointSet {
   // FP32 pair-table layout: type-sorted, centred, w = occupancy.
   DevArray pts32;             // float4[natoms + nsites]
   double max_radius = 0.0;    // of the FP32 points about the centroid
+  double occ_max = 0.0;       // largest |occupancy| of the FP32 points
   DevArray orig32;            // int[npts]: original index of each sorted point
   DevArray type32;            // int[npts]: type of each sorted point
   DevArray seg32;             // int[ntypes + 1]: type segments of the sorted order

@@ -597,6 +597,174 @@ static void launch_pairs_t(int nitems, int persist_wgs, int* counter, const floa
   }
 }
 
+namespace {
+
+// ------------------------------------------ radial interpolation sweep --
+// Pair tables from the pair-distance distribution: for a pair at distance r
+// in radial bin k = floor(r / h), j_0(q r) is replaced by its interpolant at
+// the bin's 8 Chebyshev nodes rho_km = (k + 1/2 + x_m / 2) h, whose error is
+// at most (q h / 2)^8 / (2^7 8!) (1.0e-9 at q h = 1.04; every derivative of
+// j_0 is bounded by 1).  So
+//   sum_pairs occ_i occ_j sin(q r) / r = sum_km A_km sin(q rho_km) / rho_km,
+//   A_km = sum over the bin's pairs of occ_i occ_j L_m(u), u = 2 r / h - 2k - 1,
+// with L_m the Lagrange basis of the nodes (|L_m| <= 1.27 on [-1, 1]).  Each
+// contribution is rounded to a fixed-point integer (times 2^F, F per point
+// set) and added to 64-bit integer accumulators, first per work item in LDS
+// and then per table bin in global memory: integer addition does not depend
+// on the order, so the tables are the same bits in every run, alone or in a
+// batch.
+constexpr int kRadNodes = 8;
+struct RadialNodes {
+  float x[kRadNodes];    // Chebyshev nodes on [-1, 1]
+  float inv[kRadNodes];  // 1 / prod_{k != m} (x_m - x_k)
+};
+
+__global__ __launch_bounds__(kBlock) void debye_pairs_radial(
+    const float4* __restrict__ pts, const int4* __restrict__ items,
+    const int* __restrict__ bin_first, int nbins, const float* __restrict__ bin_scale, float inv_h,
+    int nk, int copies, RadialNodes nodes, unsigned long long* __restrict__ acc) {
+  // [kRadNodes][copies][nk]: for one node, neighbouring bins are 8 bytes
+  // apart, so the lanes' adds (bins spread by their pair distances) meet few
+  // banks, and lanes l and l' with l = l' mod copies never add to the same
+  // word unless they are the same copy
+  extern __shared__ unsigned long long lacc[];
+  __shared__ float4 tile[kBlock];
+  __shared__ int sbin;
+  const int4 item = items[blockIdx.x];
+  if (threadIdx.x == 0) {  // the item's table bin: the last b with bin_first[b] <= item
+    int lo = 0, hi = nbins - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (bin_first[mid] <= int(blockIdx.x)) lo = mid; else hi = mid - 1;
+    }
+    sbin = lo;
+  }
+  for (int e = threadIdx.x; e < nk * kRadNodes * copies; e += kBlock) lacc[e] = 0ull;
+  __syncthreads();
+  unsigned long long* const mine = lacc + (threadIdx.x % copies) * nk;
+  const int node_stride = copies * nk;
+  const int bin = sbin;
+  const float fscale = bin_scale[bin];
+  const int i = item.x + threadIdx.x;
+  const bool vi = threadIdx.x < item.y;
+  const float4 pi = vi ? pts[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+  const float ci = pi.w * fscale;
+  const int jend = item.z + item.w;
+  const float two_inv_h = 2.f * inv_h;
+  for (int t0 = item.z; t0 < jend; t0 += kBlock) {
+    __syncthreads();
+    const int j = t0 + threadIdx.x;
+    tile[threadIdx.x] = j < jend ? pts[j] : make_float4(0.f, 0.f, 0.f, 0.f);
+    __syncthreads();
+    if (!vi) continue;
+    const int jn = min(kBlock, jend - t0);
+    for (int t = 0; t < jn; ++t) {
+      if (t0 + t <= i) continue;  // each unordered pair once (j > i)
+      const float4 pj = tile[t];
+      const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+      const float d2 = fmaxf(__builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz)), 1e-30f);
+      const float y = __builtin_amdgcn_rsqf(d2);
+      const float r0 = d2 * y;
+      const float r = __builtin_fmaf(__builtin_fmaf(-r0, r0, d2), 0.5f * y, r0);
+      const int k = min(int(r * inv_h), nk - 1);
+      const float u = __builtin_fmaf(r, two_inv_h, -float(2 * k + 1));
+      float pre[kRadNodes + 1], suf[kRadNodes + 1];
+      pre[0] = 1.f;
+      suf[kRadNodes] = 1.f;
+#pragma unroll
+      for (int m = 0; m < kRadNodes; ++m) pre[m + 1] = pre[m] * (u - nodes.x[m]);
+#pragma unroll
+      for (int m = kRadNodes - 1; m >= 0; --m) suf[m] = suf[m + 1] * (u - nodes.x[m]);
+      const float c = ci * pj.w;
+#pragma unroll
+      for (int m = 0; m < kRadNodes; ++m) {
+        const int v = __float2int_rn(c * (pre[m] * suf[m + 1] * nodes.inv[m]));
+        if (v != 0)
+          atomicAdd(mine + m * node_stride + k,
+                    static_cast<unsigned long long>(static_cast<long long>(v)));
+      }
+    }
+  }
+  __syncthreads();
+  unsigned long long* out = acc + size_t(bin) * nk * kRadNodes;  // [nk][kRadNodes]
+  for (int e = threadIdx.x; e < nk * kRadNodes; e += kBlock) {
+    const unsigned long long* c = lacc + (e % kRadNodes) * copies * nk + e / kRadNodes;
+    unsigned long long v = 0ull;
+    for (int r = 0; r < copies; ++r) v += c[r * nk];
+    if (v != 0ull) atomicAdd(out + e, v);
+  }
+}
+
+// out[b][q] = 2^-F_b sum_km A_km sin(2 pi q rho_km) / rho_km in FP64, q in
+// revolutions (the pair tables' raw form: the host divides by 2 pi q).  One
+// thread per (bin, q).
+__global__ void radial_node_sums(const unsigned long long* __restrict__ acc, int nbins, int nk,
+                                 double h, RadialNodes nodes, const double* __restrict__ bin_unscale,
+                                 const float* __restrict__ qrev, int nq_pad, double* __restrict__ out) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  const int b = blockIdx.y;
+  if (q >= nq_pad || b >= nbins) return;
+  const double w = 2.0 * 3.141592653589793 * double(qrev[q]);
+  const unsigned long long* a = acc + size_t(b) * nk * kRadNodes;
+  double s = 0.0;
+  for (int k = 0; k < nk; ++k)
+#pragma unroll
+    for (int m = 0; m < kRadNodes; ++m) {
+      const long long v = static_cast<long long>(a[k * kRadNodes + m]);
+      if (v == 0) continue;
+      const double rho = (double(k) + 0.5 + 0.5 * double(nodes.x[m])) * h;
+      s += double(v) * (w == 0.0 ? 0.0 : sin(w * rho) / rho);
+    }
+  out[size_t(b) * nq_pad + q] = s * bin_unscale[b];
+}
+
+__global__ void zero_u64_array(unsigned long long* p, size_t n) {
+  const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0ull;
+}
+
+}  // namespace
+
+int radial_node_count() { return kRadNodes; }
+
+size_t radial_lds_bytes(int nk) { return size_t(nk) * kRadNodes * sizeof(unsigned long long); }
+
+// Accumulator copies per workgroup that fit beside the point tile.
+static int radial_copies(int nk, size_t lds) {
+  for (int c : {8, 4, 2})
+    if (c * radial_lds_bytes(nk) + 4104 <= lds) return c;
+  return 1;
+}
+
+hipError_t launch_debye_pairs_radial(const float4* pts, const int4* items, int nitems,
+                                     const int* bin_first, int nbins, const float* bin_scale,
+                                     const double* bin_unscale, double h, int nk,
+                                     const float* node_x, const float* node_inv,
+                                     const float* qrev, int nq_pad, unsigned long long* acc,
+                                     double* out, size_t lds, hipStream_t stream,
+                                     KernelTimer* timer) {
+  if (!(h > 0.0) || nk <= 0 || nbins <= 0 || nbins > 65535 || radial_lds_bytes(nk) + 4104 > lds)
+    return hipErrorInvalidValue;
+  const int copies = radial_copies(nk, lds);
+  RadialNodes nodes;
+  for (int m = 0; m < kRadNodes; ++m) {
+    nodes.x[m] = node_x[m];
+    nodes.inv[m] = node_inv[m];
+  }
+  const size_t nacc = size_t(nbins) * nk * kRadNodes;
+  zero_u64_array<<<unsigned((nacc + 255) / 256), 256, 0, stream>>>(acc, nacc);
+  {
+    TimerScope scope(timer, "debye_pairs_radial", stream);
+    if (nitems > 0)
+      debye_pairs_radial<<<nitems, kBlock, copies * radial_lds_bytes(nk), stream>>>(
+          pts, items, bin_first, nbins, bin_scale, float(1.0 / h), nk, copies, nodes, acc);
+  }
+  radial_node_sums<<<dim3((nq_pad + 63) / 64, nbins), 64, 0, stream>>>(acc, nbins, nk, h, nodes,
+                                                                       bin_unscale, qrev, nq_pad,
+                                                                       out);
+  return hipGetLastError();
+}
+
 hipError_t launch_debye_pairs(const float4* pts, const int4* items, int nitems, const int* bin_first,
                               int nbins, const float* qrev, float qstep, int nq_pad, int qt,
                               int sin_mode, int persist_wgs, int* counter, double* partial,

@@ -114,6 +114,19 @@ void* noalloc_scratch(int device, size_t* bytes);
 hipError_t launch_write_floats(float* dst, const double* q, int n, int n_total, double divisor,
                                hipStream_t stream);
 // qstep != 0: the recurrence's q values (tile start qrev plus steps of qstep).
+// Pair tables by radial interpolation of the pair-distance distribution
+// (debye.hip): per table bin b the node accumulators (int64 [nbins][nk][8],
+// scaled by bin_scale[b] = 2^F_b) and out[b][q] = the raw table (sum of
+// occ_i occ_j sin(2 pi q r) / r) at the q in revolutions qrev.
+int radial_node_count();
+size_t radial_lds_bytes(int nk);
+hipError_t launch_debye_pairs_radial(const float4* pts, const int4* items, int nitems,
+                                     const int* bin_first, int nbins, const float* bin_scale,
+                                     const double* bin_unscale, double h, int nk,
+                                     const float* node_x, const float* node_inv,
+                                     const float* qrev, int nq_pad, unsigned long long* acc,
+                                     double* out, size_t lds, hipStream_t stream,
+                                     KernelTimer* timer);
 hipError_t launch_finish_tables(const double* raw, const float* qrev, int qt, float qstep,
                                 const double* q0, int nbins, int nq, int nq_pad, double* tables,
                                 hipStream_t stream);
