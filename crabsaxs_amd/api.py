@@ -362,7 +362,9 @@ class Context:
         """Pair-kernel launch shape; q_tile 0 picks the tile per call and
         j_chunk 0 the j chunk per structure; sin_mode 3 (the default) runs the
         sine recurrence on uniform q grids where it models faster, 2 always
-        evaluates the hardware sine."""
+        evaluates the hardware sine, 4 interpolates j_0(q r) radially over
+        the pair distances (8 Chebyshev nodes per distance bin, order-free
+        fixed-point sums)."""
         _check(_lib.load().csx_ctx_set_tuning(self.ptr, q_tile, sin_mode, j_chunk))
 
     def set_cache_limit(self, nbytes: int = DEFAULT_CACHE_LIMIT) -> None:

@@ -308,6 +308,67 @@ def test_coordinate_features_reference_fixture_and_edges(gpu_ctx):
         csx.coordinate_features(two, bins=5000)
 
 
+# ------------------------------------ radial interpolation sweep (mode 4) --
+
+def test_radial_sweep_against_fp64_and_oracle(gpu_ctx):
+    """Mode 4 (pair distances binned at h = the largest power of two with
+    h max|q| <= 1.04, j_0 interpolated at 8 Chebyshev nodes per bin, 64-bit
+    fixed-point sums) on a uniform and an irregular grid against FP64 GPU
+    Debye, and the solvent intensity against the FP64 CPU oracle."""
+    s = csx.generate_globule(3000, 20.0, 1.0, seed=21)
+    grids = (np.linspace(0.0, 1.0, 301), np.sort(np.random.default_rng(2).uniform(0.0, 1.2, 97)))
+    small = csx.generate_globule(1200, 16.0, 1.0, seed=3)
+    qq = 0.003 + 0.0049 * np.arange(100)
+    p = csx.SolventParams(c1=1.02, c2=1.5)
+    gpu_ctx.set_tuning(0, 4, 0)
+    try:
+        for q in grids:
+            got = csx.debye_intensity(s, q)
+            assert gpu_ctx.last_launch() == {"q_tile": 1, "sin_mode": 4, "persistent_workgroups": 0}
+            err = rel(got, csx.debye_intensity(s, q, precision=F64)).max()
+            print("radial vs fp64", q.size, err, flush=True)
+            assert err < 1e-5
+        got = csx.solvent_corrected_intensity(small, qq, p)
+    finally:
+        gpu_ctx.set_tuning()
+    want = O.combine_solvent_basis(O.debye_solvent_basis(oracle_atoms(small), qq, c1=1.02, c2=1.5),
+                                   qq, 1.02, 1.5)
+    assert rel(got, want).max() < 1e-5
+
+
+def test_radial_sweep_sums_do_not_depend_on_order(gpu_ctx):
+    """Integer sums: the tables are the same bits for every j chunk (which
+    splits and orders the pairs), alone or in a batch, call after call."""
+    cs = [csx.generate_globule(n, 10.0 + n / 400, 1.0, seed=60 + n) for n in (300, 1700, 2500)]
+    q = np.linspace(0.0, 0.8, 90)
+    try:
+        gpu_ctx.set_tuning(0, 4, 0)
+        base = [csx.debye_intensity(c, q) for c in cs]
+        batch = csx.batch_debye_intensity(cs, q)
+        for k in range(3):
+            assert np.array_equal(batch[k], base[k])
+        for chunk in (64, 256, 1024):
+            gpu_ctx.set_tuning(0, 4, chunk)
+            for c, b in zip(cs, base):
+                assert np.array_equal(csx.debye_intensity(c, q), b), chunk
+    finally:
+        gpu_ctx.set_tuning()
+
+
+def test_radial_sweep_falls_back_beyond_its_bins(gpu_ctx):
+    """A structure whose pair distances need more bins than a workgroup's
+    LDS holds (2,000 A apart at h = 1 A) takes a direct sweep instead."""
+    s = csx.Structure.from_arrays([[0.0, 0.0, 0.0], [1.5, 0.0, 0.0], [2000.0, 0.0, 0.0]], [1, 1, 3])
+    q = np.linspace(0.0, 1.0, 20)
+    gpu_ctx.set_tuning(0, 4, 0)
+    try:
+        got = csx.debye_intensity(s, q)
+        assert gpu_ctx.last_launch()["sin_mode"] in (1, 2)
+    finally:
+        gpu_ctx.set_tuning()
+    assert rel(got, csx.debye_intensity(s, q, precision=F64)).max() < 1e-5
+
+
 # ------------------------------------------------ sine recurrence (mode 3) --
 
 @pytest.mark.parametrize("tile", [0, 16, 25, 32, 50, 64, 100, 125, 128])
