@@ -346,13 +346,30 @@ def ensemble_leg(csx, ctx, m, log):
     order = np.argsort(sizes)
     worst = 0.0
     checked = []
+    wants = {}
     for k in (order[0], order[m // 2], order[-1]):
         got = csx.combine_solvent_basis(bases[k], q, 1.02, 1.0)
         want = csx.solvent_corrected_intensity(conformers[k], q, csx.SolventParams(c1=1.02, c2=1.0),
                                                precision=F64, ctx=ctx)
+        wants[k] = want
         e, _ = rel_err(got, want)
         worst = max(worst, e)
         checked.append(int(sizes[k]))
+    # The same batch with the pair tables from the radial interpolation of
+    # the pair distances (sine mode 4), reported beside the leg's value.
+    ctx.set_tuning(0, 4, 0)
+    ctx.set_persistence()
+    csx.batch_debye_solvent_basis(conformers, q, p, ctx=ctx)
+    rbases, dtr, kmsr, launches_r = timed(ctx, "debye_pairs_radial",
+                                          lambda: csx.batch_debye_solvent_basis(conformers, q, p,
+                                                                                ctx=ctx), reps)
+    rworst = max(rel_err(csx.combine_solvent_basis(rbases[k], q, 1.02, 1.0), w)[0]
+                 for k, w in wants.items())
+    radial = {"value": m / dtr, "unit": "conformer profiles/s", "batch_s": dtr,
+              "kernel_ms_per_batch": kmsr * launches_r, "launches_per_batch": launches_r,
+              "equivalent_pair_q_per_s": pairq / dtr, "max_rel_err_vs_fp64_gpu": rworst}
+    log(f"ensemble: radial interpolation {dtr:.3f} s per batch ({m / dtr:.0f} profiles/s)")
+    autotune.apply(ctx, tuned)  # the rest of the leg runs with the tuned shape
     def mixture(pick, w_true, seed, confs=conformers, bas=bases, noise=0.01):
         """Weights recovered from a 5-conformer mixture with Gaussian noise
         (sigma = noise x I) by both weight solvers over all m conformers."""
@@ -452,6 +469,7 @@ def ensemble_leg(csx, ctx, m, log):
         "fp32_frac": pairq / (kms * launches / 1e3) * VALU_OPS_PER_PAIRQ / PEAK_FP32_VECTOR,
         "launch": launch, "autotuned": tuned,
         "max_rel_err_vs_fp64_gpu": worst, "checked_atoms": checked,
+        "radial_interpolation": radial,
         "weights": weights,
         "profile_rank": {"relative_tolerance": rank, "conformers": m, "q": int(q.size),
                          "note": "singular values of the m unit-norm profiles (pool with the "
@@ -900,6 +918,30 @@ def main():
                               "max_rel_err": float(e.max()), "mean_rel_err": float(e.mean())}
         result["per_q_err_vs_fp64_gpu"] = per_q_report(q, e)
         log(f"fp64: {dt64:.3f} s, max rel err {e.max():.2e}")
+    # The same workload with the pair tables from the radial interpolation of
+    # the pair distances (sine mode 4: 8 Chebyshev nodes per distance bin,
+    # order-free fixed-point sums): it evaluates no per-pair sine, so it is
+    # reported beside the headline, as the equivalent rate at its accuracy.
+    ctx.set_tuning(0, 4, 0)
+    csx.solvent_corrected_intensity(s, q, params, precision=F32, ctx=ctx)
+    outr, dtr, kmsr, _ = timed(ctx, "debye_pairs_radial",
+                               lambda: csx.solvent_corrected_intensity(s, q, params, precision=F32,
+                                                                      ctx=ctx), 5)
+    again = csx.solvent_corrected_intensity(s, q, params, precision=F32, ctx=ctx)
+    result["radial_interpolation"] = {
+        "kernel": "debye_pairs_radial", "kernel_ms": kmsr, "ms_per_call": dtr * 1e3,
+        "equivalent_pair_q_per_s": pairq / dtr, "kernel_equivalent_pair_q_per_s": pairq / (kmsr / 1e3),
+        "speedup_vs_headline": ms_per_step / (dtr * 1e3), "launch": ctx.last_launch(),
+        "bitwise_repeat": bool(np.array_equal(outr, again)),
+        "note": "pair tables from 64-bit fixed-point sums of occ_i occ_j L_m(u) over the 8 "
+                "Chebyshev nodes of each pair-distance bin (width 1 A at q_max = 1), then "
+                "sin(q rho) / rho at the nodes in FP64; j_0 interpolated to 1e-9"}
+    if out64 is not None:
+        er = np.abs(outr - out64) / np.abs(out64)
+        result["radial_interpolation"].update(max_rel_err_vs_fp64_gpu=float(er.max()),
+                                              mean_rel_err_vs_fp64_gpu=float(er.mean()))
+    log(f"radial: {dtr * 1e3:.3f} ms per call, kernel {kmsr:.3f} ms")
+    apply_tune()
     if "cpu" in args.legs:
         baseline, parity = cpu_leg(s, q, params, out, out64)
         result["cpu_baseline"] = baseline

@@ -835,7 +835,37 @@ static csx_status pair_tables_batch(Call& call, const std::vector<PointSet*>& se
     plans[c] = pair_plan(tune, q, nq, sets[c]->max_radius,
                          static_cast<double>(sets[c]->natoms + sets[c]->nsites), ni, sum_j, max_j,
                          cus, call.ctx()->lds_per_workgroup);
-    keys[c] = table_key(plans[c].sin_mode, plans[c].qt, chunk[c], q, nq);
+    // the radial sweep's integer sums do not depend on the chunk
+    keys[c] = table_key(plans[c].sin_mode, plans[c].qt, plans[c].sin_mode == 4 ? 0 : chunk[c], q,
+                        nq);
+  }
+  // The radial sweep pays a fixed cost per work item (clearing and flushing
+  // its bin accumulators), so its items are as large as the launch allows:
+  // the largest j chunk up to 4096 that still gives 4 items per CU over the
+  // launch's pairs.  Its tables do not depend on the chunk.
+  {
+    double pairs = 0.0;
+    for (size_t c = 0; c < sets.size(); ++c)
+      if (plans[c].sin_mode == 4) {
+        const double n = static_cast<double>(sets[c]->natoms + sets[c]->nsites);
+        pairs += 0.5 * n * n;
+      }
+    if (pairs > 0.0) {
+      int rchunk = tune.j_chunk;
+      if (rchunk == 0) {
+        rchunk = 64;
+        for (int cand : {4096, 2048, 1024, 512, 256, 128})
+          if (pairs / (256.0 * cand) >= 4.0 * std::max(cus, 1)) {
+            rchunk = cand;
+            break;
+          }
+      }
+      for (size_t c = 0; c < sets.size(); ++c)
+        if (plans[c].sin_mode == 4 && chunk[c] != rchunk) {
+          chunk[c] = rchunk;
+          CSX_TRY(get_item_set(*sets[c], chunk[c], &all_items[c]));
+        }
+    }
   }
 
   P.assign(sets.size(), std::vector<double>());
