@@ -751,7 +751,7 @@ static csx_status pair_tables_launch(Call& call, const std::vector<PointSet*>& s
     CSX_HIP(launch_debye_pairs_radial(pts, items, static_cast<int>(nitems), bins,
                                       static_cast<int>(nbins), d_bscale, d_unscale, plan.h,
                                       plan.nk, nodes, nodes + nn, d_q.as<float>(),
-                                      static_cast<int>(nq_pad),
+                                      static_cast<int>(nq_pad), tune.q_tile,
                                       d_partial.as<unsigned long long>(), d_out.as<double>(),
                                       call.ctx()->lds_per_workgroup, st, call.timer()),
             "debye_pairs_radial");
@@ -839,35 +839,14 @@ static csx_status pair_tables_batch(Call& call, const std::vector<PointSet*>& se
     keys[c] = table_key(plans[c].sin_mode, plans[c].qt, plans[c].sin_mode == 4 ? 0 : chunk[c], q,
                         nq);
   }
-  // The radial sweep pays a fixed cost per work item (clearing and flushing
-  // its bin accumulators), so its items are as large as the launch allows:
-  // the largest j chunk up to 4096 that still gives 4 items per CU over the
-  // launch's pairs.  Its tables do not depend on the chunk.
-  {
-    double pairs = 0.0;
-    for (size_t c = 0; c < sets.size(); ++c)
-      if (plans[c].sin_mode == 4) {
-        const double n = static_cast<double>(sets[c]->natoms + sets[c]->nsites);
-        pairs += 0.5 * n * n;
-      }
-    if (pairs > 0.0) {
-      int rchunk = tune.j_chunk;
-      if (rchunk == 0) {
-        rchunk = 64;
-        for (int cand : {4096, 2048, 1024, 512, 256, 128})
-          if (pairs / (256.0 * cand) >= 4.0 * std::max(cus, 1)) {
-            rchunk = cand;
-            break;
-          }
-      }
-      for (size_t c = 0; c < sets.size(); ++c)
-        if (plans[c].sin_mode == 4 && chunk[c] != rchunk) {
-          chunk[c] = rchunk;
-          CSX_TRY(get_item_set(*sets[c], chunk[c], &all_items[c]));
-        }
+  // The radial sweep's work items: 256 x 256 by default (on 20,000 atoms
+  // x 500 q, 0.61 ms against 0.65 at the pair sweep's 512 and 0.80 at
+  // 1024).  Its tables do not depend on the chunk.
+  for (size_t c = 0; c < sets.size(); ++c)
+    if (plans[c].sin_mode == 4 && tune.j_chunk == 0 && chunk[c] > 256) {
+      chunk[c] = 256;
+      CSX_TRY(get_item_set(*sets[c], chunk[c], &all_items[c]));
     }
-  }
-
   P.assign(sets.size(), std::vector<double>());
   std::vector<size_t> todo;
   std::shared_ptr<CacheEntry> entry;  // the one set's entry, locked while computed
@@ -1199,6 +1178,10 @@ csx_status csx_ctx_set_tuning(csx_ctx* ctx, int q_tile, int sin_mode, int j_chun
                 "100, 125 or 128");
   if (sin_mode < 0 || sin_mode > 4)
     return fail(CSX_INVALID_INPUT, "sin_mode must be 0, 1, 2, 3 or 4");
+  if (sin_mode == 4 && !(q_tile == 0 || q_tile == 8 || q_tile == 16 || q_tile == 32))
+    return fail(CSX_INVALID_INPUT,
+                "with sin_mode 4, q_tile sets the LDS accumulator copies: 0 (automatic), 8, 16 "
+                "or 32");
   if (!(j_chunk == 0 || j_chunk == 32 || j_chunk == 64 || j_chunk == 128 ||
         (j_chunk >= 256 && j_chunk % 256 == 0 && j_chunk <= (1 << 20))))
     return fail(CSX_INVALID_INPUT,

@@ -22,6 +22,7 @@
 //                    FMA), restarted from hardware sines at every tile.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <map>
 #include <mutex>
@@ -623,10 +624,10 @@ __global__ __launch_bounds__(kBlock) void debye_pairs_radial(
     const float4* __restrict__ pts, const int4* __restrict__ items,
     const int* __restrict__ bin_first, int nbins, const float* __restrict__ bin_scale, float inv_h,
     int nk, int copies, RadialNodes nodes, unsigned long long* __restrict__ acc) {
-  // [kRadNodes][copies][nk]: for one node, neighbouring bins are 8 bytes
-  // apart, so the lanes' adds (bins spread by their pair distances) meet few
-  // banks, and lanes l and l' with l = l' mod copies never add to the same
-  // word unless they are the same copy
+  // [kRadNodes][nk][copies], lane l adding to copy l mod copies: the copies
+  // of one (node, bin) are neighbouring words, so lanes of different copies
+  // never meet in a bank whatever their bins, and lanes of one copy meet only
+  // where their bins agree modulo (banks / copies)
   extern __shared__ unsigned long long lacc[];
   __shared__ float4 tile[kBlock];
   __shared__ int sbin;
@@ -641,8 +642,8 @@ __global__ __launch_bounds__(kBlock) void debye_pairs_radial(
   }
   for (int e = threadIdx.x; e < nk * kRadNodes * copies; e += kBlock) lacc[e] = 0ull;
   __syncthreads();
-  unsigned long long* const mine = lacc + (threadIdx.x % copies) * nk;
-  const int node_stride = copies * nk;
+  unsigned long long* const mine = lacc + threadIdx.x % copies;
+  const int node_stride = nk * copies;
   const int bin = sbin;
   const float fscale = bin_scale[bin];
   const int i = item.x + threadIdx.x;
@@ -676,46 +677,91 @@ __global__ __launch_bounds__(kBlock) void debye_pairs_radial(
 #pragma unroll
       for (int m = kRadNodes - 1; m >= 0; --m) suf[m] = suf[m + 1] * (u - nodes.x[m]);
       const float c = ci * pj.w;
+      unsigned long long* const slot = mine + k * copies;
 #pragma unroll
       for (int m = 0; m < kRadNodes; ++m) {
         const int v = __float2int_rn(c * (pre[m] * suf[m + 1] * nodes.inv[m]));
-        if (v != 0)
-          atomicAdd(mine + m * node_stride + k,
-                    static_cast<unsigned long long>(static_cast<long long>(v)));
+        atomicAdd(slot + m * node_stride,
+                  static_cast<unsigned long long>(static_cast<long long>(v)));
       }
     }
   }
   __syncthreads();
   unsigned long long* out = acc + size_t(bin) * nk * kRadNodes;  // [nk][kRadNodes]
   for (int e = threadIdx.x; e < nk * kRadNodes; e += kBlock) {
-    const unsigned long long* c = lacc + (e % kRadNodes) * copies * nk + e / kRadNodes;
+    const unsigned long long* c = lacc + ((e % kRadNodes) * nk + e / kRadNodes) * copies;
     unsigned long long v = 0ull;
-    for (int r = 0; r < copies; ++r) v += c[r * nk];
+    for (int r = 0; r < copies; ++r) v += c[r];
     if (v != 0ull) atomicAdd(out + e, v);
   }
 }
 
-// out[b][q] = 2^-F_b sum_km A_km sin(2 pi q rho_km) / rho_km in FP64, q in
-// revolutions (the pair tables' raw form: the host divides by 2 pi q).  One
-// thread per (bin, q).
-__global__ void radial_node_sums(const unsigned long long* __restrict__ acc, int nbins, int nk,
-                                 double h, RadialNodes nodes, const double* __restrict__ bin_unscale,
-                                 const float* __restrict__ qrev, int nq_pad, double* __restrict__ out) {
-  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+// a_km = 2^-F_b A_km / rho_km in FP64, in place of the integer sums (one
+// thread per accumulator, [bin][k][m]).
+__global__ void radial_node_weights(unsigned long long* __restrict__ acc, size_t n, int nk,
+                                    double h, RadialNodes nodes,
+                                    const double* __restrict__ bin_unscale) {
+  const size_t e = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  const int m = int(e % kRadNodes);
+  const size_t kb = e / kRadNodes;
+  const int k = int(kb % size_t(nk));
+  const size_t b = kb / size_t(nk);
+  const double rho = (double(k) + 0.5 + 0.5 * double(nodes.x[m])) * h;
+  const double a = double(static_cast<long long>(acc[e])) * bin_unscale[b] / rho;
+  reinterpret_cast<double*>(acc)[e] = a;
+}
+
+// out[b][q] = sum_km a_km sin(2 pi q rho_km) in FP64, q in revolutions (the
+// pair tables' raw form: the host divides by 2 pi q).  A workgroup owns 64 q
+// of one bin.  The bins k come in ranges of kNodeRange; each range's sum
+// rotates the 8 nodes' phasors from one sincos at its first bin by the step
+// angle 2 pi q h (an FP64 rotation: after n steps the phasor is off by ~n
+// ulp).  Wave w adds ranges w, w + kNodeWaves, ... in order and the waves'
+// sums are added in wave order: the bits depend on the bins' accumulators
+// only, not on how many empty bins follow them (a batch gives its sets the
+// largest set's bin count).
+constexpr int kNodeWaves = 4;
+constexpr int kNodeRange = 16;
+__global__ __launch_bounds__(64 * kNodeWaves) void radial_node_sums(
+    const double* __restrict__ a, int nk, double h, RadialNodes nodes,
+    const float* __restrict__ qrev, int nq_pad, double* __restrict__ out) {
+  __shared__ double part[kNodeWaves][64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int q = blockIdx.x * 64 + lane;
   const int b = blockIdx.y;
-  if (q >= nq_pad || b >= nbins) return;
-  const double w = 2.0 * 3.141592653589793 * double(qrev[q]);
-  const unsigned long long* a = acc + size_t(b) * nk * kRadNodes;
+  const double w = q < nq_pad ? 2.0 * 3.141592653589793 * double(qrev[q]) : 0.0;
+  const double* ab = a + size_t(b) * nk * kRadNodes;
   double s = 0.0;
-  for (int k = 0; k < nk; ++k)
+  if (w != 0.0) {
+    double sd, cd;
+    sincos(w * h, &sd, &cd);
+    for (int k0 = wave * kNodeRange; k0 < nk; k0 += kNodeWaves * kNodeRange) {
+      const int k1 = min(nk, k0 + kNodeRange);
+      double sn[kRadNodes], cs[kRadNodes];
 #pragma unroll
-    for (int m = 0; m < kRadNodes; ++m) {
-      const long long v = static_cast<long long>(a[k * kRadNodes + m]);
-      if (v == 0) continue;
-      const double rho = (double(k) + 0.5 + 0.5 * double(nodes.x[m])) * h;
-      s += double(v) * (w == 0.0 ? 0.0 : sin(w * rho) / rho);
+      for (int m = 0; m < kRadNodes; ++m)
+        sincos(w * ((double(k0) + 0.5 + 0.5 * double(nodes.x[m])) * h), &sn[m], &cs[m]);
+      double r = 0.0;
+      for (int k = k0; k < k1; ++k) {
+#pragma unroll
+        for (int m = 0; m < kRadNodes; ++m) {
+          r = __builtin_fma(ab[k * kRadNodes + m], sn[m], r);
+          const double t = __builtin_fma(sn[m], cd, cs[m] * sd);
+          cs[m] = __builtin_fma(cs[m], cd, -(sn[m] * sd));
+          sn[m] = t;
+        }
+      }
+      s += r;
     }
-  out[size_t(b) * nq_pad + q] = s * bin_unscale[b];
+  }
+  part[wave][lane] = s;
+  __syncthreads();
+  if (wave == 0 && q < nq_pad) {
+    double v = part[0][lane];
+    for (int r = 1; r < kNodeWaves; ++r) v += part[r][lane];
+    out[size_t(b) * nq_pad + q] = v;
+  }
 }
 
 __global__ void zero_u64_array(unsigned long long* p, size_t n) {
@@ -729,23 +775,27 @@ int radial_node_count() { return kRadNodes; }
 
 size_t radial_lds_bytes(int nk) { return size_t(nk) * kRadNodes * sizeof(unsigned long long); }
 
-// Accumulator copies per workgroup that fit beside the point tile.
-static int radial_copies(int nk, size_t lds) {
-  for (int c : {8, 4, 2})
-    if (c * radial_lds_bytes(nk) + 4104 <= lds) return c;
-  return 1;
+// Accumulator copies per workgroup: the requested power of two (1-32), or by
+// default 8, fewer where they do not fit beside the point tile.  (20,000
+// atoms x 500 q: 0.61 ms at 8 copies, 1.22 ms at 16 or 32, whose LDS leaves
+// room for fewer workgroups per CU.)
+static int radial_copies(int nk, size_t lds, int want) {
+  int c = want > 0 ? std::min(want, 32) : 8;
+  while (c > 1 && c * radial_lds_bytes(nk) + 4104 > lds) c >>= 1;
+  return c;
 }
 
 hipError_t launch_debye_pairs_radial(const float4* pts, const int4* items, int nitems,
                                      const int* bin_first, int nbins, const float* bin_scale,
                                      const double* bin_unscale, double h, int nk,
                                      const float* node_x, const float* node_inv,
-                                     const float* qrev, int nq_pad, unsigned long long* acc,
-                                     double* out, size_t lds, hipStream_t stream,
-                                     KernelTimer* timer) {
-  if (!(h > 0.0) || nk <= 0 || nbins <= 0 || nbins > 65535 || radial_lds_bytes(nk) + 4104 > lds)
+                                     const float* qrev, int nq_pad, int copies_wanted,
+                                     unsigned long long* acc, double* out, size_t lds,
+                                     hipStream_t stream, KernelTimer* timer) {
+  if (!(h > 0.0) || nk <= 0 || nbins <= 0 || nbins > 65535 || radial_lds_bytes(nk) + 4104 > lds ||
+      copies_wanted < 0 || (copies_wanted & (copies_wanted - 1)) != 0)
     return hipErrorInvalidValue;
-  const int copies = radial_copies(nk, lds);
+  const int copies = radial_copies(nk, lds, copies_wanted);
   RadialNodes nodes;
   for (int m = 0; m < kRadNodes; ++m) {
     nodes.x[m] = node_x[m];
@@ -759,9 +809,10 @@ hipError_t launch_debye_pairs_radial(const float4* pts, const int4* items, int n
       debye_pairs_radial<<<nitems, kBlock, copies * radial_lds_bytes(nk), stream>>>(
           pts, items, bin_first, nbins, bin_scale, float(1.0 / h), nk, copies, nodes, acc);
   }
-  radial_node_sums<<<dim3((nq_pad + 63) / 64, nbins), 64, 0, stream>>>(acc, nbins, nk, h, nodes,
-                                                                       bin_unscale, qrev, nq_pad,
-                                                                       out);
+  radial_node_weights<<<unsigned((nacc + 255) / 256), 256, 0, stream>>>(acc, nacc, nk, h, nodes,
+                                                                        bin_unscale);
+  radial_node_sums<<<dim3((nq_pad + 63) / 64, nbins), 64 * kNodeWaves, 0, stream>>>(
+      reinterpret_cast<const double*>(acc), nk, h, nodes, qrev, nq_pad, out);
   return hipGetLastError();
 }
 

@@ -117,16 +117,17 @@ hipError_t launch_write_floats(float* dst, const double* q, int n, int n_total, 
 // Pair tables by radial interpolation of the pair-distance distribution
 // (debye.hip): per table bin b the node accumulators (int64 [nbins][nk][8],
 // scaled by bin_scale[b] = 2^F_b) and out[b][q] = the raw table (sum of
-// occ_i occ_j sin(2 pi q r) / r) at the q in revolutions qrev.
+// occ_i occ_j sin(2 pi q r) / r) at the q in revolutions qrev.  copies_wanted:
+// LDS accumulator copies per workgroup (0: automatic, else a power of two).
 int radial_node_count();
 size_t radial_lds_bytes(int nk);
 hipError_t launch_debye_pairs_radial(const float4* pts, const int4* items, int nitems,
                                      const int* bin_first, int nbins, const float* bin_scale,
                                      const double* bin_unscale, double h, int nk,
                                      const float* node_x, const float* node_inv,
-                                     const float* qrev, int nq_pad, unsigned long long* acc,
-                                     double* out, size_t lds, hipStream_t stream,
-                                     KernelTimer* timer);
+                                     const float* qrev, int nq_pad, int copies_wanted,
+                                     unsigned long long* acc, double* out, size_t lds,
+                                     hipStream_t stream, KernelTimer* timer);
 hipError_t launch_finish_tables(const double* raw, const float* qrev, int qt, float qstep,
                                 const double* q0, int nbins, int nq, int nq_pad, double* tables,
                                 hipStream_t stream);

@@ -156,8 +156,9 @@ csx_status csx_ctx_reset_timing(csx_ctx *ctx);
  * width h, the largest power of two with h max|q| <= 1.04, so that j_0(q r)
  * is interpolated to within 1.0e-9, into 64-bit fixed-point sums whose
  * bits do not depend on the order; the tables are the node sums of
- * sin(q rho) / rho in FP64; the q tile does not apply; where the distance
- * bins do not fit a workgroup's LDS, mode 2.  About 5x the throughput of
+ * sin(q rho) / rho in FP64; q_tile then sets the LDS accumulator copies
+ * per workgroup: 0 (automatic), 8, 16 or 32, fewer where they do not fit;
+ * where the distance bins do not fit a workgroup's LDS, mode 2.  About 5x the throughput of
  * mode 3 on 20,000 atoms x 500 q at the same accuracy; the no-allocation
  * entry point computes with mode 2 instead);
  * j_chunk

@@ -338,7 +338,8 @@ def test_radial_sweep_against_fp64_and_oracle(gpu_ctx):
 
 def test_radial_sweep_sums_do_not_depend_on_order(gpu_ctx):
     """Integer sums: the tables are the same bits for every j chunk (which
-    splits and orders the pairs), alone or in a batch, call after call."""
+    splits and orders the pairs) and every count of LDS accumulator copies,
+    alone or in a batch, call after call."""
     cs = [csx.generate_globule(n, 10.0 + n / 400, 1.0, seed=60 + n) for n in (300, 1700, 2500)]
     q = np.linspace(0.0, 0.8, 90)
     try:
@@ -347,12 +348,18 @@ def test_radial_sweep_sums_do_not_depend_on_order(gpu_ctx):
         batch = csx.batch_debye_intensity(cs, q)
         for k in range(3):
             assert np.array_equal(batch[k], base[k])
-        for chunk in (64, 256, 1024):
-            gpu_ctx.set_tuning(0, 4, chunk)
+        for copies, chunk in ((0, 64), (8, 256), (16, 1024), (32, 256)):
+            gpu_ctx.set_tuning(copies, 4, chunk)
             for c, b in zip(cs, base):
-                assert np.array_equal(csx.debye_intensity(c, q), b), chunk
+                assert np.array_equal(csx.debye_intensity(c, q), b), (copies, chunk)
     finally:
         gpu_ctx.set_tuning()
+
+
+def test_radial_sweep_copies_are_validated(gpu_ctx):
+    with pytest.raises(csx.InvalidInput):
+        gpu_ctx.set_tuning(24, 4, 0)
+    gpu_ctx.set_tuning()
 
 
 def test_radial_sweep_falls_back_beyond_its_bins(gpu_ctx):
