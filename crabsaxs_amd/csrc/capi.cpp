@@ -531,8 +531,9 @@ static bool radial_bins(const double* q, size_t nq, double rmax, size_t lds, dou
 }
 
 // Fixed-point exponent of a point set's radial accumulators: one
-// contribution occ_i occ_j L_m (|L_m| <= 1.27) times 2^F stays below 2^30,
-// and a table bin's sum over all the set's pairs below 2^62.
+// contribution occ_i occ_j T_p(u) (|T_p| <= 1 on the bin, with a margin of
+// 1.3 for u rounded past its ends) times 2^F stays below 2^30, and a table
+// bin's sum over all the set's pairs below 2^62.
 static int radial_scale_exponent(const PointSet& ps) {
   const double cmax = std::max(ps.occ_max * ps.occ_max * 1.3, 1e-300);
   const double n = static_cast<double>(ps.natoms + ps.nsites);
@@ -717,15 +718,12 @@ static csx_status pair_tables_launch(Call& call, const std::vector<PointSet*>& s
     call.ctx()->last_items[1] = static_cast<int>(nitems);
   }
   if (sin_mode == 4) {
-    // Per table bin, the fixed-point scale of its point set; the Chebyshev
-    // nodes on [-1, 1] and their inverse Lagrange denominators.
-    const int nn = radial_node_count();
-    const size_t sbytes = nbins * (sizeof(float) + sizeof(double)) + 2 * nn * sizeof(float);
+    // Per table bin, the fixed-point scale of its point set.
+    const size_t sbytes = nbins * (sizeof(float) + sizeof(double));
     void* h_s = nullptr;
     CSX_HIP(call.take_pinned(3, sbytes, &h_s), "hipHostMalloc");
     double* unscale = static_cast<double*>(h_s);
     float* scale = reinterpret_cast<float*>(unscale + nbins);
-    float* nodes = scale + nbins;
     size_t b = 0;
     for (size_t t = 0; t < todo.size(); ++t) {
       const PointSet& ps = *sets[todo[t]];
@@ -735,13 +733,6 @@ static csx_status pair_tables_launch(Call& call, const std::vector<PointSet*>& s
         unscale[b] = std::ldexp(1.0, -f);
       }
     }
-    for (int m = 0; m < nn; ++m) nodes[m] = static_cast<float>(std::cos((2.0 * m + 1.0) * kPi / (2.0 * nn)));
-    for (int m = 0; m < nn; ++m) {
-      double d = 1.0;
-      for (int k = 0; k < nn; ++k)
-        if (k != m) d *= std::cos((2.0 * m + 1.0) * kPi / (2.0 * nn)) - std::cos((2.0 * k + 1.0) * kPi / (2.0 * nn));
-      nodes[nn + m] = static_cast<float>(1.0 / d);
-    }
     Scratch d_scale(call);
     CSX_HIP(d_scale.alloc(sbytes, st), "hipMallocAsync");
     CSX_HIP(hipMemcpyAsync(d_scale.as<void>(), h_s, sbytes, hipMemcpyHostToDevice, st),
@@ -750,7 +741,7 @@ static csx_status pair_tables_launch(Call& call, const std::vector<PointSet*>& s
     const float* d_bscale = reinterpret_cast<const float*>(d_unscale + nbins);
     CSX_HIP(launch_debye_pairs_radial(pts, items, static_cast<int>(nitems), bins,
                                       static_cast<int>(nbins), d_bscale, d_unscale, plan.h,
-                                      plan.nk, nodes, nodes + nn, d_q.as<float>(),
+                                      plan.nk, d_q.as<float>(),
                                       static_cast<int>(nq_pad), tune.q_tile,
                                       d_partial.as<unsigned long long>(), d_out.as<double>(),
                                       call.ctx()->lds_per_workgroup, st, call.timer()),

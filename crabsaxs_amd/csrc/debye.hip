@@ -23,6 +23,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdint>
 #include <map>
 #include <mutex>
@@ -603,29 +604,33 @@ namespace {
 // ------------------------------------------ radial interpolation sweep --
 // Pair tables from the pair-distance distribution: for a pair at distance r
 // in radial bin k = floor(r / h), j_0(q r) is replaced by its interpolant at
-// the bin's 8 Chebyshev nodes rho_km = (k + 1/2 + x_m / 2) h, whose error is
-// at most (q h / 2)^8 / (2^7 8!) (1.0e-9 at q h = 1.04; every derivative of
-// j_0 is bounded by 1).  So
+// the bin's 8 Chebyshev nodes rho_km = (k + 1/2 + x_m / 2) h,
+// x_m = cos((2m + 1) pi / 16), whose error is at most (q h / 2)^8 / (2^7 8!)
+// (1.0e-9 at q h = 1.04; every derivative of j_0 is bounded by 1).  So
 //   sum_pairs occ_i occ_j sin(q r) / r = sum_km A_km sin(q rho_km) / rho_km,
 //   A_km = sum over the bin's pairs of occ_i occ_j L_m(u), u = 2 r / h - 2k - 1,
-// with L_m the Lagrange basis of the nodes (|L_m| <= 1.27 on [-1, 1]).  Each
-// contribution is rounded to a fixed-point integer (times 2^F, F per point
-// set) and added to 64-bit integer accumulators, first per work item in LDS
-// and then per table bin in global memory: integer addition does not depend
-// on the order, so the tables are the same bits in every run, alone or in a
-// batch.
+// with L_m the Lagrange basis of the nodes.  By the discrete orthogonality
+// of the Chebyshev polynomials at these nodes,
+//   L_m(u) = sum_p (2 - [p = 0]) / 8 T_p(x_m) T_p(u),
+// so a pair adds its Chebyshev moments c T_p(u), p = 0..7 (|T_p| <= 1, from
+// the three-term recurrence: 8 VALU operations for the 8), and the 8 x 8 map
+// to the nodes runs once per bin after the sums.  Each moment is rounded to
+// a fixed-point integer (times 2^F, F per point set) and added to 64-bit
+// integer accumulators, first per work item in LDS and then per table bin in
+// global memory: integer addition does not depend on the order, so the
+// tables are the same bits in every run, alone or in a batch.
 constexpr int kRadNodes = 8;
-struct RadialNodes {
-  float x[kRadNodes];    // Chebyshev nodes on [-1, 1]
-  float inv[kRadNodes];  // 1 / prod_{k != m} (x_m - x_k)
+struct RadialMap {
+  double x[kRadNodes];             // Chebyshev nodes on [-1, 1]
+  double m[kRadNodes][kRadNodes];  // node m's weight of moment p: (2 - [p = 0]) / 8 T_p(x_m)
 };
 
 __global__ __launch_bounds__(kBlock) void debye_pairs_radial(
     const float4* __restrict__ pts, const int4* __restrict__ items,
     const int* __restrict__ bin_first, int nbins, const float* __restrict__ bin_scale, float inv_h,
-    int nk, int copies, RadialNodes nodes, unsigned long long* __restrict__ acc) {
+    int nk, int copies, unsigned long long* __restrict__ acc) {
   // [kRadNodes][nk][copies], lane l adding to copy l mod copies: the copies
-  // of one (node, bin) are neighbouring words, so lanes of different copies
+  // of one (moment, bin) are neighbouring words, so lanes of different copies
   // never meet in a bank whatever their bins, and lanes of one copy meet only
   // where their bins agree modulo (banks / copies)
   extern __shared__ unsigned long long lacc[];
@@ -669,20 +674,21 @@ __global__ __launch_bounds__(kBlock) void debye_pairs_radial(
       const float r = __builtin_fmaf(__builtin_fmaf(-r0, r0, d2), 0.5f * y, r0);
       const int k = min(int(r * inv_h), nk - 1);
       const float u = __builtin_fmaf(r, two_inv_h, -float(2 * k + 1));
-      float pre[kRadNodes + 1], suf[kRadNodes + 1];
-      pre[0] = 1.f;
-      suf[kRadNodes] = 1.f;
-#pragma unroll
-      for (int m = 0; m < kRadNodes; ++m) pre[m + 1] = pre[m] * (u - nodes.x[m]);
-#pragma unroll
-      for (int m = kRadNodes - 1; m >= 0; --m) suf[m] = suf[m + 1] * (u - nodes.x[m]);
-      const float c = ci * pj.w;
+      // c T_p(u): T_0 = 1, T_1 = u, T_{p+1} = 2u T_p - T_{p-1}
+      const float u2 = u + u;
+      float s0 = ci * pj.w;
+      float s1 = s0 * u;
       unsigned long long* const slot = mine + k * copies;
+      atomicAdd(slot, static_cast<unsigned long long>(static_cast<long long>(__float2int_rn(s0))));
+      atomicAdd(slot + node_stride,
+                static_cast<unsigned long long>(static_cast<long long>(__float2int_rn(s1))));
 #pragma unroll
-      for (int m = 0; m < kRadNodes; ++m) {
-        const int v = __float2int_rn(c * (pre[m] * suf[m + 1] * nodes.inv[m]));
-        atomicAdd(slot + m * node_stride,
-                  static_cast<unsigned long long>(static_cast<long long>(v)));
+      for (int p = 2; p < kRadNodes; ++p) {
+        const float s2 = __builtin_fmaf(u2, s1, -s0);
+        atomicAdd(slot + p * node_stride,
+                  static_cast<unsigned long long>(static_cast<long long>(__float2int_rn(s2))));
+        s0 = s1;
+        s1 = s2;
       }
     }
   }
@@ -696,20 +702,27 @@ __global__ __launch_bounds__(kBlock) void debye_pairs_radial(
   }
 }
 
-// a_km = 2^-F_b A_km / rho_km in FP64, in place of the integer sums (one
-// thread per accumulator, [bin][k][m]).
-__global__ void radial_node_weights(unsigned long long* __restrict__ acc, size_t n, int nk,
-                                    double h, RadialNodes nodes,
+// a_km = 2^-F_b A_km / rho_km in FP64, A_km = sum_p map[m][p] (moment p), in
+// place of the integer moment sums (one thread per (bin, k): [bin][k][8]).
+__global__ void radial_node_weights(unsigned long long* __restrict__ acc, size_t nbk, int nk,
+                                    double h, RadialMap map,
                                     const double* __restrict__ bin_unscale) {
   const size_t e = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (e >= n) return;
-  const int m = int(e % kRadNodes);
-  const size_t kb = e / kRadNodes;
-  const int k = int(kb % size_t(nk));
-  const size_t b = kb / size_t(nk);
-  const double rho = (double(k) + 0.5 + 0.5 * double(nodes.x[m])) * h;
-  const double a = double(static_cast<long long>(acc[e])) * bin_unscale[b] / rho;
-  reinterpret_cast<double*>(acc)[e] = a;
+  if (e >= nbk) return;
+  const int k = int(e % size_t(nk));
+  const double unscale = bin_unscale[e / size_t(nk)];
+  double t[kRadNodes];
+#pragma unroll
+  for (int p = 0; p < kRadNodes; ++p)
+    t[p] = double(static_cast<long long>(acc[e * kRadNodes + p]));
+  double* out = reinterpret_cast<double*>(acc) + e * kRadNodes;
+#pragma unroll
+  for (int m = 0; m < kRadNodes; ++m) {
+    double a = 0.0;
+#pragma unroll
+    for (int p = 0; p < kRadNodes; ++p) a = __builtin_fma(map.m[m][p], t[p], a);
+    out[m] = a * unscale / ((double(k) + 0.5 + 0.5 * map.x[m]) * h);
+  }
 }
 
 // out[b][q] = sum_km a_km sin(2 pi q rho_km) in FP64, q in revolutions (the
@@ -724,7 +737,7 @@ __global__ void radial_node_weights(unsigned long long* __restrict__ acc, size_t
 constexpr int kNodeWaves = 4;
 constexpr int kNodeRange = 16;
 __global__ __launch_bounds__(64 * kNodeWaves) void radial_node_sums(
-    const double* __restrict__ a, int nk, double h, RadialNodes nodes,
+    const double* __restrict__ a, int nk, double h, RadialMap map,
     const float* __restrict__ qrev, int nq_pad, double* __restrict__ out) {
   __shared__ double part[kNodeWaves][64];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -741,7 +754,7 @@ __global__ __launch_bounds__(64 * kNodeWaves) void radial_node_sums(
       double sn[kRadNodes], cs[kRadNodes];
 #pragma unroll
       for (int m = 0; m < kRadNodes; ++m)
-        sincos(w * ((double(k0) + 0.5 + 0.5 * double(nodes.x[m])) * h), &sn[m], &cs[m]);
+        sincos(w * ((double(k0) + 0.5 + 0.5 * map.x[m]) * h), &sn[m], &cs[m]);
       double r = 0.0;
       for (int k = k0; k < k1; ++k) {
 #pragma unroll
@@ -788,7 +801,6 @@ static int radial_copies(int nk, size_t lds, int want) {
 hipError_t launch_debye_pairs_radial(const float4* pts, const int4* items, int nitems,
                                      const int* bin_first, int nbins, const float* bin_scale,
                                      const double* bin_unscale, double h, int nk,
-                                     const float* node_x, const float* node_inv,
                                      const float* qrev, int nq_pad, int copies_wanted,
                                      unsigned long long* acc, double* out, size_t lds,
                                      hipStream_t stream, KernelTimer* timer) {
@@ -796,23 +808,26 @@ hipError_t launch_debye_pairs_radial(const float4* pts, const int4* items, int n
       copies_wanted < 0 || (copies_wanted & (copies_wanted - 1)) != 0)
     return hipErrorInvalidValue;
   const int copies = radial_copies(nk, lds, copies_wanted);
-  RadialNodes nodes;
+  RadialMap map;
+  const double pi = 3.141592653589793;
   for (int m = 0; m < kRadNodes; ++m) {
-    nodes.x[m] = node_x[m];
-    nodes.inv[m] = node_inv[m];
+    map.x[m] = std::cos((2 * m + 1) * pi / (2 * kRadNodes));
+    for (int p = 0; p < kRadNodes; ++p)
+      map.m[m][p] = (p == 0 ? 1.0 : 2.0) / kRadNodes * std::cos(p * (2 * m + 1) * pi / (2 * kRadNodes));
   }
-  const size_t nacc = size_t(nbins) * nk * kRadNodes;
+  const size_t nbk = size_t(nbins) * nk;
+  const size_t nacc = nbk * kRadNodes;
   zero_u64_array<<<unsigned((nacc + 255) / 256), 256, 0, stream>>>(acc, nacc);
   {
     TimerScope scope(timer, "debye_pairs_radial", stream);
     if (nitems > 0)
       debye_pairs_radial<<<nitems, kBlock, copies * radial_lds_bytes(nk), stream>>>(
-          pts, items, bin_first, nbins, bin_scale, float(1.0 / h), nk, copies, nodes, acc);
+          pts, items, bin_first, nbins, bin_scale, float(1.0 / h), nk, copies, acc);
   }
-  radial_node_weights<<<unsigned((nacc + 255) / 256), 256, 0, stream>>>(acc, nacc, nk, h, nodes,
-                                                                        bin_unscale);
+  radial_node_weights<<<unsigned((nbk + 255) / 256), 256, 0, stream>>>(acc, nbk, nk, h, map,
+                                                                       bin_unscale);
   radial_node_sums<<<dim3((nq_pad + 63) / 64, nbins), 64 * kNodeWaves, 0, stream>>>(
-      reinterpret_cast<const double*>(acc), nk, h, nodes, qrev, nq_pad, out);
+      reinterpret_cast<const double*>(acc), nk, h, map, qrev, nq_pad, out);
   return hipGetLastError();
 }
 

@@ -115,7 +115,8 @@ hipError_t launch_write_floats(float* dst, const double* q, int n, int n_total, 
                                hipStream_t stream);
 // qstep != 0: the recurrence's q values (tile start qrev plus steps of qstep).
 // Pair tables by radial interpolation of the pair-distance distribution
-// (debye.hip): per table bin b the node accumulators (int64 [nbins][nk][8],
+// (debye.hip): per table bin b the Chebyshev moment accumulators (int64
+// [nbins][nk][8], then FP64 node weights in place,
 // scaled by bin_scale[b] = 2^F_b) and out[b][q] = the raw table (sum of
 // occ_i occ_j sin(2 pi q r) / r) at the q in revolutions qrev.  copies_wanted:
 // LDS accumulator copies per workgroup (0: automatic, else a power of two).
@@ -124,7 +125,6 @@ size_t radial_lds_bytes(int nk);
 hipError_t launch_debye_pairs_radial(const float4* pts, const int4* items, int nitems,
                                      const int* bin_first, int nbins, const float* bin_scale,
                                      const double* bin_unscale, double h, int nk,
-                                     const float* node_x, const float* node_inv,
                                      const float* qrev, int nq_pad, int copies_wanted,
                                      unsigned long long* acc, double* out, size_t lds,
                                      hipStream_t stream, KernelTimer* timer);

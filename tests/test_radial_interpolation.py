@@ -63,3 +63,26 @@ def test_moments_reproduce_the_point_sum():
             direct = np.sum(occ * spherical_jn(l, q * r) * y)
             via = np.sum(spherical_jn(l, q * nodes) * m)
             assert abs(via - direct) <= 1.1e-9 * np.sum(np.abs(occ * y)), (q, l)
+
+
+def test_chebyshev_moments_give_the_lagrange_weights():
+    """The radial Debye sweep accumulates c T_p(u) per pair and maps the
+    moments to the nodes by (2 - [p = 0]) / 8 T_p(x_m): at the Chebyshev
+    nodes this equals the Lagrange basis L_m(u) (discrete orthogonality), so
+    the node sums match the Lagrange form for any u in [-1, 1]."""
+    x, den = chebyshev()
+    n = len(x)
+    u = np.random.default_rng(3).uniform(-1, 1, 400)
+    t = np.cos(np.outer(np.arange(n), np.arccos(u)))  # T_p(u), [p][pair]
+    rec = np.empty_like(t)  # the kernel's recurrence
+    rec[0], rec[1] = 1.0, u
+    for p in range(2, n):
+        rec[p] = 2 * u * rec[p - 1] - rec[p - 2]
+    assert np.abs(rec - t).max() < 1e-13
+    mp = np.array([[(1.0 if p == 0 else 2.0) / n * np.cos(p * (2 * m + 1) * np.pi / (2 * n))
+                    for p in range(n)] for m in range(n)])
+    assert np.allclose(mp @ rec, lagrange(u, x, den).T, atol=1e-13)
+    # a wrong normalisation (2 / n for p = 0 too) misses it
+    bad = mp.copy()
+    bad[:, 0] *= 2
+    assert not np.allclose(bad @ rec, lagrange(u, x, den).T, atol=1e-6)
