@@ -15,6 +15,11 @@ hipError_t launch_debye_vjp(const float4* pts, const int4* blocks, int nb, const
                             float qstep, const float* AB, int nq_pad, int qt, int sin_mode,
                             int qsplit, int tsplit, size_t items_per_launch, double* part,
                             double* gacc, double* out, hipStream_t stream, KernelTimer* timer);
+hipError_t launch_debye_vjp_radial(const float4* pts, const int4* blocks, int nb, const int2* items,
+                                   size_t nitems, const int* orig, int ntypes, const double* qv,
+                                   const double* B, int nq, double h, int nk, int tsplit,
+                                   size_t items_per_launch, float* coef, double* part, double* gacc,
+                                   double* out, hipStream_t stream, KernelTimer* timer);
 int jacobian_q_tile();
 int vjp_q_tile(int tuned, size_t nq, int sin_mode, size_t nitems, int cus, int max_tile);
 int vjp_qsplit(size_t nqc, size_t nitems);
@@ -44,6 +49,84 @@ std::vector<double> type_amplitudes_for(const csx_structure* s, const csx_solven
 
 }  // namespace
 
+// Sine mode 4: F_ab(r) per radial bin from Chebyshev coefficients (the
+// kernel file describes it); B_ab(q) = 2 w_q A_a A_b / q at the exact q.
+static csx_status vjp_radial(Call& call, const csx_structure* s, PointSet* ps, const double* q,
+                             size_t nq, const double* weights, const csx_solvent_params* p,
+                             bool implicit_h, const std::vector<int>& blocks,
+                             const std::vector<int>& items, double h, int nk, double* out) {
+  const int T = ps->ntypes;
+  const size_t n = s->n;
+  const int nb = static_cast<int>(blocks.size() / 4);
+  const size_t nitems = items.size() / 2;
+  const std::vector<double> A = type_amplitudes_for(s, p, implicit_h, q, nq, T);
+  std::vector<double> B(static_cast<size_t>(T) * T * nq, 0.0);
+  for (size_t k = 0; k < nq; ++k) {
+    if (q[k] == 0.0) continue;
+    for (int a = 0; a < T; ++a)
+      for (int b = 0; b < T; ++b)
+        B[(static_cast<size_t>(a) * T + b) * nq + k] =
+            2.0 * weights[k] * A[a * nq + k] * A[b * nq + k] / q[k];
+  }
+  const size_t fill = 4 * static_cast<size_t>(std::max(call.ctx()->cu_count, 1));
+  int tsplit = 1;
+  while (tsplit < 8 && 2 * nitems * static_cast<size_t>(tsplit) <= fill) tsplit *= 2;
+  int split;
+  size_t budget;
+  {
+    std::lock_guard<std::mutex> lock(call.ctx()->mu);
+    split = call.ctx()->tuning.vjp_split;
+    budget = call.ctx()->tuning.workspace_limit;
+  }
+  if (split > 0) tsplit = split;
+  const size_t per_item = static_cast<size_t>(tsplit) * 2 * 3 * kBlock * sizeof(double);
+  const size_t ipl = std::max<size_t>(1, std::min(nitems, budget / per_item));
+  {
+    std::lock_guard<std::mutex> lock(call.ctx()->mu);
+    int* last = call.ctx()->last_vjp;
+    last[0] = 0;  // no q tile: the radial sweep
+    last[1] = 1;
+    last[2] = tsplit;
+    last[3] = static_cast<int>((nitems + ipl - 1) / ipl);
+  }
+  const hipStream_t st = call.stream();
+  Scratch d_q(call), d_b(call), d_coef(call), d_blocks(call), d_items(call), d_part(call),
+      d_acc(call), d_out(call);
+  CSX_HIP(d_q.alloc(nq * sizeof(double), st), "hipMallocAsync");
+  CSX_HIP(d_b.alloc(B.size() * sizeof(double), st), "hipMallocAsync");
+  CSX_HIP(d_coef.alloc(static_cast<size_t>(T) * T * nk * 8 * sizeof(float), st), "hipMallocAsync");
+  CSX_HIP(d_blocks.alloc(blocks.size() * sizeof(int), st), "hipMallocAsync");
+  CSX_HIP(d_items.alloc(items.size() * sizeof(int), st), "hipMallocAsync");
+  CSX_HIP(d_part.alloc(ipl * per_item, st), "hipMallocAsync");
+  CSX_HIP(d_acc.alloc(static_cast<size_t>(nb) * 3 * kBlock * sizeof(double), st), "hipMallocAsync");
+  CSX_HIP(d_out.alloc(n * 3 * sizeof(double), st), "hipMallocAsync");
+  CSX_HIP(hipMemcpyAsync(d_q.as<void>(), q, nq * sizeof(double), hipMemcpyHostToDevice, st),
+          "hipMemcpyAsync");
+  CSX_HIP(hipMemcpyAsync(d_b.as<void>(), B.data(), B.size() * sizeof(double),
+                         hipMemcpyHostToDevice, st), "hipMemcpyAsync");
+  CSX_HIP(hipMemcpyAsync(d_blocks.as<void>(), blocks.data(), blocks.size() * sizeof(int),
+                         hipMemcpyHostToDevice, st), "hipMemcpyAsync");
+  CSX_HIP(hipMemcpyAsync(d_items.as<void>(), items.data(), items.size() * sizeof(int),
+                         hipMemcpyHostToDevice, st), "hipMemcpyAsync");
+  CSX_HIP(launch_debye_vjp_radial(ps->pts32.as<float4>(), d_blocks.as<int4>(), nb,
+                                  d_items.as<int2>(), nitems, ps->orig32.as<int>(), T,
+                                  d_q.as<double>(), d_b.as<double>(), static_cast<int>(nq), h, nk,
+                                  tsplit, ipl, d_coef.as<float>(), d_part.as<double>(),
+                                  d_acc.as<double>(), d_out.as<double>(), st, call.timer()),
+          "debye_vjp_radial");
+  CSX_HIP(hipMemcpyAsync(out, d_out.as<void>(), n * 3 * sizeof(double), hipMemcpyDeviceToHost, st),
+          "hipMemcpyAsync");
+  d_q.reset();
+  d_b.reset();
+  d_coef.reset();
+  d_blocks.reset();
+  d_items.reset();
+  d_part.reset();
+  d_acc.reset();
+  d_out.reset();
+  return call.finish();
+}
+
 static csx_status vjp(Call& call, const csx_structure* s, PointSet* ps, const double* q, size_t nq,
                       const double* weights, const csx_solvent_params* p, bool implicit_h,
                       double* out) {
@@ -60,6 +143,15 @@ static csx_status vjp(Call& call, const csx_structure* s, PointSet* ps, const do
   for (int I = 0; I < nb; ++I)
     for (int J = I; J < nb; ++J) items.insert(items.end(), {I, J});
   const size_t nitems = items.size() / 2;
+  int tune_sm;
+  {
+    std::lock_guard<std::mutex> lock(call.ctx()->mu);
+    tune_sm = call.ctx()->tuning.sin_mode;
+  }
+  double rh = 0.0;
+  int rnk = 0;
+  if (tune_sm == 4 && radial_distance_bins(q, nq, ps->max_radius, kVjpRadialMaxBins, &rh, &rnk))
+    return vjp_radial(call, s, ps, q, nq, weights, p, implicit_h, blocks, items, rh, rnk, out);
   // The unreduced hardware sine takes |q r / 2 pi| <= 256 revolutions; on
   // a uniform grid within that range the sines and cosines come from the
   // recurrence (sine mode 3).

@@ -26,6 +26,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 
 #include "kernels.hpp"
 
@@ -212,6 +213,128 @@ __global__ __launch_bounds__(kAdjBlock) void debye_vjp_f32(
   for (int c = 0; c < 3; ++c) {
     out[c * kAdjBlock + tid] = gi64[c];
     out[(3 + c) * kAdjBlock + tid] = gj64[c];
+  }
+}
+
+// ---------------------------------------------- radial interpolation --
+// Sine mode 4: for one type pair the q sum of a pair,
+//   F_ab(r) = sum_q B_ab(q) (q r cos q r - sin q r),  B_ab = 2 w_q A_a A_b / q,
+// is a function of r alone, band-limited by q_max.  On each radial bin of
+// width h (h q_max <= 1.04) it is replaced by its interpolant at 8 Chebyshev
+// nodes, kept as the Chebyshev coefficients c_p of that bin, so a pair reads
+// 8 coefficients and runs Clenshaw's recurrence instead of a q loop.  The
+// error is at most (q_max h / 2)^8 / (2^7 8!) sum_q |B q^8| (q r + 9) per pair
+// (the 8th derivative of u cos u - sin u is bounded by u + 8).
+constexpr int kVjpNodes = 8;
+struct VjpNodes {
+  double x[kVjpNodes];             // Chebyshev nodes on [-1, 1]
+  double m[kVjpNodes][kVjpNodes];  // coefficient p's weight of node m: (2 - [p = 0]) / 8 T_p(x_m)
+};
+
+// coef[ab][k][p] (FP32) from FP64 node values.  One 64-lane workgroup per
+// (type pair, bin): lane m + 8 s sums node m over the q values q = s mod 8.
+__global__ __launch_bounds__(64) void vjp_radial_coefficients(
+    const double* __restrict__ qv, const double* __restrict__ B /* [T*T][nq] */, int nq, int nk,
+    double h, VjpNodes nodes, float* __restrict__ coef) {
+  __shared__ double part[8][kVjpNodes];
+  __shared__ double fnode[kVjpNodes];
+  const int m = threadIdx.x & 7, sl = threadIdx.x >> 3;
+  const int ab = blockIdx.x / nk, k = blockIdx.x % nk;
+  const double rho = (double(k) + 0.5 + 0.5 * nodes.x[m]) * h;
+  const double* Bq = B + size_t(ab) * nq;
+  double f = 0.0;
+  for (int i = sl; i < nq; i += 8) {
+    const double u = qv[i] * rho;
+    double sn, cs;
+    sincos(u, &sn, &cs);
+    f = __builtin_fma(Bq[i], __builtin_fma(u, cs, -sn), f);
+  }
+  part[sl][m] = f;
+  __syncthreads();
+  if (threadIdx.x < kVjpNodes) {
+    double v = 0.0;
+    for (int r = 0; r < 8; ++r) v += part[r][threadIdx.x];
+    fnode[threadIdx.x] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < kVjpNodes) {
+    double c = 0.0;
+    for (int n = 0; n < kVjpNodes; ++n) c = __builtin_fma(nodes.m[threadIdx.x][n], fnode[n], c);
+    coef[size_t(blockIdx.x) * kVjpNodes + threadIdx.x] = float(c);
+  }
+}
+
+// The pair sweep of debye_vjp_f32 with F(r) from the bin's coefficients
+// (the item's type pair's table in LDS: 2 float4 per bin).  No q chunks: the
+// FP32 sums of one sweep fold into the FP64 partials once.
+__global__ __launch_bounds__(kAdjBlock) void debye_vjp_radial(
+    const float4* __restrict__ pts, const int4* __restrict__ blocks /* start, len, type, 0 */,
+    const int2* __restrict__ items /* (I, J), I <= J */, int ntypes,
+    const float* __restrict__ coef /* [T][T][nk][8] */, int nk, float inv_h, int tsplit,
+    double* __restrict__ part /* [items][tsplit][2][3][256] */) {
+  extern __shared__ float4 ctab[];
+  __shared__ float4 tile[kAdjBlock];
+  __shared__ float accj[4][3][kAdjBlock];
+  const int tid = threadIdx.x, wave = tid >> 6;
+  const int2 it = items[blockIdx.x];
+  const int4 bi = blocks[it.x], bj = blocks[it.y];
+  const bool diag = it.x == it.y;
+  const float4 pi = tid < bi.y ? pts[bi.x + tid] : make_float4(0.f, 0.f, 0.f, 0.f);
+  tile[tid] = tid < bj.y ? pts[bj.x + tid] : make_float4(0.f, 0.f, 0.f, 0.f);
+  const float4* cg =
+      reinterpret_cast<const float4*>(coef + (size_t(bi.z) * ntypes + bj.z) * nk * kVjpNodes);
+  for (int e = tid; e < 2 * nk; e += kAdjBlock) ctab[e] = cg[e];
+#pragma unroll
+  for (int w = 0; w < 4; ++w)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) accj[w][c][tid] = 0.f;
+  const int ta = int(blockIdx.y) * (kAdjBlock / tsplit), tb = ta + kAdjBlock / tsplit;
+  const float two_inv_h = 2.f * inv_h;
+  __syncthreads();
+  float gi[3] = {0.f, 0.f, 0.f};
+  float4 pnext = tile[(ta + tid) & (kAdjBlock - 1)];
+  for (int t = ta; t < tb; ++t) {
+    const int jj = (t + tid) & (kAdjBlock - 1);
+    const float4 pj = pnext;
+    pnext = tile[(t + 1 + tid) & (kAdjBlock - 1)];
+    const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+    const float d2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
+    const bool use = d2 > 1e-20f && (!diag || jj > tid);
+    const float dd = use ? d2 : 1.0f;
+    const float ir = __builtin_amdgcn_rsqf(dd);
+    const float r0 = dd * ir;
+    const float r = __builtin_fmaf(__builtin_fmaf(-r0, r0, dd), 0.5f * ir, r0);
+    const float wgt = use ? pi.w * pj.w * ir * ir * ir : 0.f;
+    const int k = min(int(r * inv_h), nk - 1);
+    const float u = __builtin_fmaf(r, two_inv_h, -float(2 * k + 1));
+    const float4 lo = ctab[2 * k], hi = ctab[2 * k + 1];  // c_0..c_3, c_4..c_7
+    // Clenshaw: b_p = c_p + 2u b_{p+1} - b_{p+2}; F = c_0 + u b_1 - b_2
+    const float u2 = u + u;
+    const float b7 = hi.w;
+    const float b6 = __builtin_fmaf(u2, b7, hi.z);
+    const float b5 = __builtin_fmaf(u2, b6, hi.y - b7);
+    const float b4 = __builtin_fmaf(u2, b5, hi.x - b6);
+    const float b3 = __builtin_fmaf(u2, b4, lo.w - b5);
+    const float b2 = __builtin_fmaf(u2, b3, lo.z - b4);
+    const float b1 = __builtin_fmaf(u2, b2, lo.y - b3);
+    const float F = __builtin_fmaf(u, b1, lo.x - b2);
+    const float hh = F * wgt;
+    gi[0] = __builtin_fmaf(hh, dx, gi[0]);
+    gi[1] = __builtin_fmaf(hh, dy, gi[1]);
+    gi[2] = __builtin_fmaf(hh, dz, gi[2]);
+    // slot jj belongs to this lane at this step (see debye_vjp_f32)
+    accj[wave][0][jj] -= hh * dx;
+    accj[wave][1][jj] -= hh * dy;
+    accj[wave][2][jj] -= hh * dz;
+  }
+  __syncthreads();
+  double* out = part + (size_t(blockIdx.x) * gridDim.y + blockIdx.y) * 2 * 3 * kAdjBlock;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    out[c * kAdjBlock + tid] = double(gi[c]);
+    out[(3 + c) * kAdjBlock + tid] =
+        ((double(accj[0][c][tid]) + double(accj[1][c][tid])) + double(accj[2][c][tid])) +
+        double(accj[3][c][tid]);
   }
 }
 
@@ -450,6 +573,43 @@ hipError_t launch_debye_vjp(const float4* pts, const int4* blocks, int nb, const
 #undef CSX_VJP
     }
     reduce_vjp<<<nb, kAdjBlock, 0, stream>>>(part, k0, k1, qsplit * tsplit, nb, gacc);
+  }
+  scatter_vjp<<<nb, kAdjBlock, 0, stream>>>(gacc, blocks, orig, out);
+  return hipGetLastError();
+}
+
+// Sine mode 4 (see vjp_radial_coefficients): the coefficient tables into
+// coef (T * T * nk * 8 floats), then the sweeps and the same reduction.
+hipError_t launch_debye_vjp_radial(const float4* pts, const int4* blocks, int nb, const int2* items,
+                                   size_t nitems, const int* orig, int ntypes, const double* qv,
+                                   const double* B, int nq, double h, int nk, int tsplit,
+                                   size_t items_per_launch, float* coef, double* part, double* gacc,
+                                   double* out, hipStream_t stream, KernelTimer* timer) {
+  if (nb <= 0) return hipSuccess;
+  if (!(h > 0.0) || nk <= 0 || nk > kVjpRadialMaxBins || nq <= 0 || ntypes <= 0 ||
+      items_per_launch == 0 || (tsplit != 1 && tsplit != 2 && tsplit != 4 && tsplit != 8))
+    return hipErrorInvalidValue;
+  VjpNodes nodes;
+  const double pi = 3.141592653589793;
+  for (int m = 0; m < kVjpNodes; ++m) nodes.x[m] = std::cos((2 * m + 1) * pi / (2 * kVjpNodes));
+  for (int p = 0; p < kVjpNodes; ++p)
+    for (int m = 0; m < kVjpNodes; ++m)
+      nodes.m[p][m] = (p == 0 ? 1.0 : 2.0) / kVjpNodes *
+                      std::cos(p * (2 * m + 1) * pi / (2 * kVjpNodes));
+  vjp_radial_coefficients<<<unsigned(ntypes * ntypes * nk), 64, 0, stream>>>(qv, B, nq, nk, h,
+                                                                             nodes, coef);
+  const size_t nacc = size_t(nb) * 3 * kAdjBlock;
+  zero_f64<<<(nacc + 255) / 256, 256, 0, stream>>>(gacc, nacc);
+  const size_t lds = size_t(nk) * kVjpNodes * sizeof(float);
+  for (size_t k0 = 0; k0 < nitems; k0 += items_per_launch) {
+    const size_t k1 = std::min(nitems, k0 + items_per_launch);
+    dim3 grid(static_cast<unsigned>(k1 - k0), tsplit);
+    {
+      TimerScope scope(timer, "debye_vjp_radial", stream);
+      debye_vjp_radial<<<grid, kAdjBlock, lds, stream>>>(pts, blocks, items + k0, ntypes, coef, nk,
+                                                         float(1.0 / h), tsplit, part);
+    }
+    reduce_vjp<<<nb, kAdjBlock, 0, stream>>>(part, k0, k1, tsplit, nb, gacc);
   }
   scatter_vjp<<<nb, kAdjBlock, 0, stream>>>(gacc, blocks, orig, out);
   return hipGetLastError();

@@ -518,16 +518,28 @@ struct PairPlan {
 // 1.0e-9; pair distances up to 2 r_max need floor(2 r_max / h) + 2 bins,
 // whose LDS accumulators (64 B per bin) must fit a workgroup beside the
 // 4 KB point tile.  Returns false where they do not.
-static bool radial_bins(const double* q, size_t nq, double rmax, size_t lds, double* h, int* nk) {
+// Radial bins of width h, the largest power of two (at most 64 A) with
+// h max|q| <= 1.04 over the FP32 phase grid's q, covering distances up to
+// 2 rmax: false where that takes more than max_bins bins.
+bool radial_distance_bins(const double* q, size_t nq, double rmax, double max_bins, double* h,
+                          int* nk) {
   double qmax = 0.0;
   for (size_t k = 0; k < nq; ++k)
     qmax = std::max(qmax, 2.0 * kPi * std::fabs(static_cast<double>(static_cast<float>(q[k] / (2.0 * kPi)))));
   *h = qmax > 0.0 ? std::min(64.0, std::exp2(std::floor(std::log2(1.04 / qmax)))) : 64.0;
   const double bins = std::floor(2.0 * rmax / *h) + 2.0;
-  if (!(*h >= 1.0 / 4096.0) || !(bins * radial_lds_bytes(1) + 4104.0 <= static_cast<double>(lds)))
-    return false;
+  if (!(*h >= 1.0 / 4096.0) || !(bins <= max_bins)) return false;
   *nk = static_cast<int>(bins);
   return true;
+}
+
+// The radial pair sweep's bins: one accumulator copy of them beside the
+// 4 KB point tile in a workgroup's LDS.
+static bool radial_bins(const double* q, size_t nq, double rmax, size_t lds, double* h, int* nk) {
+  return radial_distance_bins(q, nq, rmax,
+                              (static_cast<double>(lds) - 4104.0) /
+                                  static_cast<double>(radial_lds_bytes(1)),
+                              h, nk);
 }
 
 // Fixed-point exponent of a point set's radial accumulators: one

@@ -424,6 +424,8 @@ int pair_j_chunk(int tuned, const PointSet& ps, int cus);
 // step turning the phase of the largest pair distance 2 r_max by more than
 // 1/4 revolution).
 float uniform_q_step(const double* q, size_t nq, double rmax);
+bool radial_distance_bins(const double* q, size_t nq, double rmax, double max_bins, double* h,
+                          int* nk);
 // The widest q tile the recurrence takes for that step and point radius (its
 // phase error grows with the phase a tile accumulates).
 int rec_max_tile(float step, double rmax);

@@ -120,6 +120,9 @@ hipError_t launch_write_floats(float* dst, const double* q, int n, int n_total, 
 // scaled by bin_scale[b] = 2^F_b) and out[b][q] = the raw table (sum of
 // occ_i occ_j sin(2 pi q r) / r) at the q in revolutions qrev.  copies_wanted:
 // LDS accumulator copies per workgroup (0: automatic, else a power of two).
+// Distance bins of the radial coordinate adjoint: its per-pair coefficient
+// table (32 B per bin) sits in LDS.
+constexpr int kVjpRadialMaxBins = 2048;
 int radial_node_count();
 size_t radial_lds_bytes(int nk);
 hipError_t launch_debye_pairs_radial(const float4* pts, const int4* items, int nitems,

@@ -157,3 +157,76 @@ def test_vjp_step_split_and_several_launches(gpu_ctx):
         gpu_ctx.set_workspace_limit()
     for split in (2, 4, 8):
         assert np.abs(results[split] - results[1]).max() < 1e-5 * np.abs(want).max()
+
+
+# -------------------------------------------- radial interpolation (mode 4) --
+
+@pytest.fixture
+def radial_mode(gpu_ctx):
+    gpu_ctx.set_tuning(0, 4, 0)
+    yield gpu_ctx
+    gpu_ctx.set_tuning()
+    gpu_ctx.set_vjp_split()
+
+
+def test_radial_vjp_matches_oracle(radial_mode):
+    """Sine mode 4: F_ab(r) from per-bin Chebyshev coefficients instead of
+    the q loop, on the shapes of the q-loop tests (uniform and non-uniform
+    grids, step splits 1-8) against the oracle gradient."""
+    s = csx.generate_globule(900, 13.0, 1.0, seed=56)
+    solvent = csx.SolventParams(c1=1.0, c2=1.2)
+    w = np.random.default_rng(6).normal(size=150)
+    for shift in (0.0, 1e-7):
+        q = np.linspace(0.005, 1.2, 150)
+        q[71] += shift
+        want = O.debye_grad_x(oatoms(s), q, w, solvent=(1.0, 1.2, 3.0, 0.334))
+        for split in (0, 1, 8):
+            radial_mode.set_vjp_split(split)
+            got = csx.debye_grad_x(s, q, w, solvent)
+            assert radial_mode.last_vjp_launch()["q_tile"] == 0
+            e = np.abs(got - want).max() / np.abs(want).max()
+            print("radial vjp shift", shift, "split", split, radial_mode.last_vjp_launch(),
+                  "max err", e, flush=True)
+            assert e < 1e-5
+    radial_mode.set_vjp_split()
+    # deterministic, and the pair contributions cancel in the sum
+    q = np.linspace(0.0, 0.6, 70)
+    w = np.random.default_rng(9).normal(size=q.size)
+    c = csx.generate_domain_chain(3000, 3, seed=8)
+    g1 = csx.debye_grad_x(c, q, w)
+    assert np.array_equal(g1, csx.debye_grad_x(c, q, w))
+    assert np.abs(g1.sum(axis=0)).max() <= 1e-6 * np.abs(g1).sum(axis=0).max()
+
+
+def test_radial_vjp_directional_derivative_full_size(radial_mode):
+    """20,000 atoms, 300 q in mode 4 against central differences of the
+    FP64 GPU intensity (the bound of the q-loop test)."""
+    s = csx.generate_globule(20000, 35.0, 1.0, seed=1)
+    q = np.linspace(0.01, 1.0, 300)
+    p = csx.SolventParams(c1=1.02, c2=2.0)
+    rng = np.random.default_rng(5)
+    w = rng.normal(size=q.size)
+    g = csx.debye_grad_x(s, q, w, p)
+    assert radial_mode.last_vjp_launch()["q_tile"] == 0
+    d = rng.normal(size=s.xyz.shape)
+    h = 1e-4
+    up = csx.Structure.from_arrays(s.xyz + h * d, s.element)
+    dn = csx.Structure.from_arrays(s.xyz - h * d, s.element)
+    fd = (csx.solvent_corrected_intensity(up, q, p, precision=F64)
+          - csx.solvent_corrected_intensity(dn, q, p, precision=F64)) / (2 * h)
+    lhs, rhs = float(np.sum(g * d)), float(w @ fd)
+    print("radial directional", lhs, rhs, abs(lhs - rhs) / np.sum(np.abs(w * fd)), flush=True)
+    assert abs(lhs - rhs) <= 2e-4 * np.sum(np.abs(w * fd)) + 1e-6 * abs(rhs)
+
+
+def test_radial_vjp_falls_back_beyond_its_bins(radial_mode):
+    """Points 20,000 A apart need more than 2,048 bins of 1 A: the q-loop
+    sweep runs instead."""
+    s = csx.Structure.from_arrays([[0.0, 0.0, 0.0], [1.5, 0.0, 0.0], [20000.0, 0.0, 0.0]],
+                                  [1, 1, 3])
+    q = np.linspace(0.01, 1.0, 20)
+    w = np.ones(q.size)
+    got = csx.debye_grad_x(s, q, w)
+    assert radial_mode.last_vjp_launch()["q_tile"] != 0
+    want = O.debye_grad_x(oatoms(s), q, w)
+    assert np.abs(got - want).max() <= 1e-4 * np.abs(want).max()
