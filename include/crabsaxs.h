@@ -160,7 +160,8 @@ csx_status csx_ctx_reset_timing(csx_ctx *ctx);
  * per workgroup: 0 (automatic), 8, 16 or 32, fewer where they do not fit;
  * where the distance bins do not fit a workgroup's LDS, mode 2.  About 5x the throughput of
  * mode 3 on 20,000 atoms x 500 q at the same accuracy; the no-allocation
- * entry point computes with mode 2 instead);
+ * entry point computes with mode 2 instead; the coordinate VJP follows it,
+ * see csx_debye_grad_x);
  * j_chunk
  * atoms per work item: 0 (automatic: the largest of 512, 256, 128, 64 and
  * 32 giving at least one full 256 x j_chunk item per CU, so small structures
@@ -523,7 +524,11 @@ csx_status csx_compute_curve(csx_structure *s, const double *q, size_t nq,
  * (p == NULL) or of solvent_corrected_intensity with Analytic hydration
  * (solvent.rs:490-500, c1 and c2 fixed) with respect to the atom
  * coordinates, before the >= 0 clamp: FP32 pair terms, FP64 accumulation.
- * Grid hydration models are rejected (their sites are not differentiable). */
+ * Grid hydration models are rejected (their sites are not differentiable).
+ * With csx_ctx_set_tuning's sin_mode 4 the VJP takes each pair's q sum
+ * sum_q B_ab(q) (qr cos qr - sin qr) from 8 Chebyshev coefficients of its
+ * distance bin (width h, h max|q| <= 1.04; up to 2,048 bins, else the q
+ * loop) instead of a q loop. */
 
 /* grad[3*i + c] = sum_q weights[q] dI(q)/dx_{i,c}  (vector-Jacobian product;
  * with weights = d chi^2 / d I(q) this is d chi^2 / dx). */
