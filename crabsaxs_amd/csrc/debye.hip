@@ -235,18 +235,6 @@ __device__ __attribute__((always_inline)) inline void tile_sweep(const float4* t
   }
 }
 
-// The same pairs over the uniform q grid q0 + k qstep (revolutions), k < QT:
-// s_k = sin(2 pi (q0 + k qstep) r) by Reinsch's recurrence
-//   d_{k+1} = d_k + lam s_k,  s_{k+1} = s_k + d_{k+1},
-//   lam = 2 cos(2 pi th) - 2 = -(2 sin(pi th))^2,  th = qstep r,
-// which keeps the step difference d_k = s_k - s_{k-1} as its own variable:
-// a rounding of s moves the sinusoid's amplitude by that rounding, and one
-// of d by d's relative rounding, so the error does not grow as 1/th for
-// short pairs the way the three-term form s_{k+1} = 2 cos(th) s_k - s_{k-1}
-// does.  Start: s_0 = sin(2 pi q0 r) and d_0 = s_0 - s_{-1} =
-// 2 sin(pi th) cos(2 pi (q0 r - th / 2)), from V_SIN_F32 / V_COS_F32 on the
-// unreduced arguments (the host keeps |q0 r| <= 250 revolutions) and the
-// step constant from its own series (the host keeps |th| <= 1/4).
 // The same pairs over the uniform q grid q0 + k qstep (revolutions), k < QT.
 // Each pair runs Reinsch's form of the sine recurrence on its weighted sine
 // s_k = w sin(2 pi (q0 + k qstep) r), w = occ_j / r:
