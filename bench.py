@@ -949,9 +949,10 @@ def main():
         "equivalent_pair_q_per_s": pairq / dtr, "kernel_equivalent_pair_q_per_s": pairq / (kmsr / 1e3),
         "speedup_vs_headline": ms_per_step / (dtr * 1e3), "launch": ctx.last_launch(),
         "bitwise_repeat": bool(np.array_equal(outr, again)),
-        "note": "pair tables from 64-bit fixed-point sums of occ_i occ_j L_m(u) over the 8 "
-                "Chebyshev nodes of each pair-distance bin (width 1 A at q_max = 1), then "
-                "sin(q rho) / rho at the nodes in FP64; j_0 interpolated to 1e-9"}
+        "note": "pair tables from 64-bit fixed-point sums of the Chebyshev moments "
+                "occ_i occ_j T_p(u), p < 8, per pair-distance bin (width 1 A at q_max = 1), "
+                "mapped to the bin's 8 Chebyshev nodes, then sin(q rho) / rho at the nodes "
+                "in FP64; j_0 interpolated to 1e-9"}
     if out64 is not None:
         er = np.abs(outr - out64) / np.abs(out64)
         result["radial_interpolation"].update(max_rel_err_vs_fp64_gpu=float(er.max()),

@@ -152,10 +152,11 @@ csx_status csx_ctx_reset_timing(csx_ctx *ctx);
  * 64, 100, 125 and 128, held to those whose q span turns the largest pair's
  * phase by at most 4 revolutions; otherwise mode 2, with the same range
  * fallback) or 4 (radial interpolation of the pair distances: each pair
- * adds occ_i occ_j L_m(u) for the 8 Chebyshev nodes of its distance bin of
+ * adds its 8 Chebyshev moments occ_i occ_j T_p(u) on its distance bin of
  * width h, the largest power of two with h max|q| <= 1.04, so that j_0(q r)
- * is interpolated to within 1.0e-9, into 64-bit fixed-point sums whose
- * bits do not depend on the order; the tables are the node sums of
+ * at the bin's 8 Chebyshev nodes interpolates it to within 1.0e-9, into
+ * 64-bit fixed-point sums whose bits do not depend on the order; the
+ * moments map to the nodes per bin, and the tables are the node sums of
  * sin(q rho) / rho in FP64; q_tile then sets the LDS accumulator copies
  * per workgroup: 0 (automatic), 8, 16 or 32, fewer where they do not fit;
  * where the distance bins do not fit a workgroup's LDS, mode 2.  About 5x the throughput of
